@@ -1,0 +1,135 @@
+/* ytf.h -- C-ABI of the MI355X (gfx950) layered-media Green's-function hot
+ * path.  Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Every entry point replaces a piece of the reference Julia code
+ * (HortenseBeaudoin/YellowThinFilms); the file:line it stands in for is
+ * given with each declaration.  The reference's own Julia API
+ * (mod_GreenOperator.jl:56-57, mod_CavityTransfer.jl:48-49) stays in the
+ * caller; a ccall shim binds the functions below.
+ *
+ * Conventions (the reference's):
+ *   - complex numbers are interleaved (re, im) doubles = Julia ComplexF64;
+ *   - arrays are column-major (first index fastest), as Julia's;
+ *   - layer indices are 1-based, as Julia's;
+ *   - time convention exp(-i w t), loss Im(eps) > 0, eps0 = mu0 = c = 1.
+ *
+ * Return codes: YTF_OK; YTF_EINVAL (contract violation, message in
+ * ytf_last_error; the reference raises ErrorException there); YTF_EHIP (HIP
+ * runtime failure).  Device pointers (suffix _dev) must be allocated on the
+ * context's device; work is enqueued on the context's stream and is
+ * asynchronous unless the function says otherwise.
+ */
+#ifndef YTF_H
+#define YTF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YTF_OK 0
+#define YTF_EINVAL 1
+#define YTF_EHIP 2
+
+/* Material kinds: which branch of _eigen_QV a layer takes
+ * (mod_ModeSolver.jl:203-265): isotropic/uniaxial eps with isotropic mu use
+ * the closed form, everything else the general Berreman eigen-solve. */
+#define YTF_EPS_ISOTROPIC 0
+#define YTF_EPS_UNIAXIAL 1
+#define YTF_EPS_BIAXIAL 2
+#define YTF_EPS_GENERAL 3
+#define YTF_MU_ISOTROPIC 0
+#define YTF_MU_GENERAL 1
+
+/* One layer: FlatOpticalLayer / FlatLayer (mod_ModeSolver.jl:71-80).
+ * eps, mu: 3x3 complex, column-major, interleaved; d: thickness (also the
+ * reference plane of the semi-infinite first and last layers). */
+typedef struct ytf_layer {
+  double eps[18];
+  double mu[18];
+  double d;
+  int32_t eps_kind;
+  int32_t mu_kind;
+} ytf_layer;
+
+/* Per-node status bits written to flags_dev. */
+#define YTF_NODE_VALID 1u    /* CavityTransferCache.valid  (mod_CavityTransfer.jl:167) */
+#define YTF_NODE_TVALID 2u   /* CavityTransferCache.Tvalid[o]                          */
+#define YTF_NODE_SINGULAR 4u /* an exactly singular 4x4 solve (SingularException)      */
+#define YTF_NODE_NOMODES 8u  /* mode solve failed (no QR convergence / >2 modes a side) */
+
+typedef struct ytf_ctx ytf_ctx;
+
+/* Context = one device + one stream (the reference's per-task thread pool). */
+int ytf_ctx_create(int device, ytf_ctx **ctx);
+int ytf_ctx_destroy(ytf_ctx *ctx);
+/* Message of the last failed call on ctx (ctx may be NULL for ctx_create). */
+const char *ytf_last_error(const ytf_ctx *ctx);
+/* Use an existing hipStream_t (NULL = the context's own stream). */
+int ytf_ctx_set_stream(ytf_ctx *ctx, void *hip_stream);
+/* Block until all work enqueued on the context's stream is done. */
+int ytf_ctx_synchronize(ytf_ctx *ctx);
+
+/* Device cache outputs of the node solve, structure-of-arrays with the node
+ * index fastest (X[e * n_nodes + node], complex).  Any pointer may be NULL.
+ *   Qsrc_dev  [4]  Qf1 Qf2 Qb1 Qb2 of the source layer       (Qf_src, Qb_src)
+ *   sf_dev    [6]  forward seed, 2x3 column-major (mode, moment x/y/z)
+ *   sb_dev    [6]  backward seed, Modal_amp convention sb = -sib_raw
+ *   T_dev     [16] T^o, 4x4 column-major
+ *   basis_dev [16] obs-layer basis Exf Eyf Ezf Exb Eyb Ezb KZf KZb (2 modes each)
+ *   flags_dev [1]  uint8 YTF_NODE_* bits
+ * (CavityTransferCache1D/2D fields, mod_CavityTransfer.jl:160-178, 273-294.) */
+typedef struct ytf_node_cache {
+  double *Qsrc_dev;
+  double *sf_dev;
+  double *sb_dev;
+  double *T_dev;
+  double *basis_dev;
+  uint8_t *flags_dev;
+} ytf_node_cache;
+
+/* Batched Green k-node solve: for every node (kx[k], ky[k]) of one stack,
+ * modes -> S-matrix cascade -> T^o -> point seed -> obs-layer basis, and
+ * optionally the spectral dyadic g~(k; zsrc[p], zobs[p]) of each depth pair.
+ *
+ * Replaces the per-node body of build_cavity_transfer_1d/_2d
+ * (mod_CavityTransfer.jl:224-251, 383-388: Modes_Stack, S_all_ingredients,
+ * S_all_Matrix, _cavity_transfer_columns, Source_projection_Berreman_xyz,
+ * _fill_obs_basis!) for one observation layer, fused with the per-node
+ * strength of _strengths_row! (mod_ClosedFormField.jl:616-659; amplitude_at
+ * :539-566 + direct_amplitude :577-602 + obs phases) divided by its
+ * quadrature weight.
+ *
+ *   layers: host array of n_layers (>= 2); omega: angular frequency;
+ *   kx_dev, ky_dev: n_nodes doubles each;
+ *   source_layer in 1..n_layers-1, obs_layer in 1..n_layers (1-based);
+ *   zsrc, zobs: host arrays of n_pairs local depths (zsrc as Dipole.z, zobs
+ *     as closed_form_G's z_obs);
+ *   ghat_dev: [n_pairs][9][n_nodes] complex, entry c + 3q = field component c
+ *     of unit moment q, or NULL;
+ *   cache: device outputs (or NULL). */
+int ytf_green_nodes(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                    const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache);
+
+/* Spectral convolution on the uniform DFT grid (the "G(kx,ky) F{J}" route of
+ * mod_UniformSource.jl:229-248):
+ *     E = scale * bfft2( g~ .* fft2(J) )        per field component c:
+ *     E_c = scale * bfft2( sum_q g~_{c,q} fft2(J_q) ).
+ * With scale = dx*dy * dkx*dky*nx*ny/(2pi)^2 / (nx*ny) this is
+ * _inverse_transform(g~ .* _forward_transform(J)) exactly (|ramp| = 1).
+ *   nx, ny: powers of two in 8..2048;
+ *   ghat_dev: [9][nx*ny] complex with node index y + ny*x (y fastest);
+ *   J_dev, E_dev: (nx, ny, 3) complex column-major (E may not alias J);
+ *   work_dev: scratch of 3*nx*ny complex. */
+int ytf_spectral_apply(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
+                       double *work_dev, double scale);
+
+/* Library version string. */
+const char *ytf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YTF_H */

@@ -1,0 +1,264 @@
+"""ctypes wrapper of the CPU oracle (ytf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+
+Stacks are passed as flat arrays: eps, mu (n, 3, 3) complex (converted to the
+column-major interleaved layout the C side reads), d (n,), ek (n,) eps kind
+(0 isotropic, 1 uniaxial, 2 biaxial, 3 general 3x3), mk (n,) mu kind (0 iso).
+Layer indices are 1-based, as in the reference.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_d = ct.POINTER(ct.c_double)
+_i = ct.POINTER(ct.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+        _LIB = ct.CDLL(path)
+    return _LIB
+
+
+def _p(a, typ=_d):
+    return a.ctypes.data_as(typ)
+
+
+def _flat33(m):
+    """(n,3,3) complex -> (n*18,) doubles, column-major 3x3, interleaved."""
+    m = np.asarray(m, dtype=np.complex128)
+    colmaj = np.transpose(m, (0, 2, 1)).reshape(m.shape[0], 9)
+    return np.ascontiguousarray(colmaj).view(np.float64).reshape(-1).copy()
+
+
+class Stack:
+    def __init__(self, eps, mu, d, ek, mk):
+        self.eps = np.asarray(eps, dtype=np.complex128).reshape(-1, 3, 3)
+        self.mu = np.asarray(mu, dtype=np.complex128).reshape(-1, 3, 3)
+        self.d = np.ascontiguousarray(d, dtype=np.float64)
+        self.ek = np.ascontiguousarray(ek, dtype=np.int32)
+        self.mk = np.ascontiguousarray(mk, dtype=np.int32)
+        self.n = len(self.d)
+        self._e = _flat33(self.eps)
+        self._m = _flat33(self.mu)
+
+    def args(self):
+        return (ct.c_int(self.n), _p(self._e), _p(self._m), _p(self.d), _p(self.ek, _i), _p(self.mk, _i))
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def eigen4(A):
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    buf = A.reshape(-1, order="F").copy()
+    Q = np.zeros(4, np.complex128)
+    V = np.zeros(16, np.complex128)
+    rc = lib().ora_eigen4(_p(buf.view(np.float64)), _p(Q.view(np.float64)), _p(V.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("oracle eigen4 did not converge")
+    return Q, V.reshape(4, 4, order="F")
+
+
+def berreman(eps, mu, omega, kx, ky):
+    e = _flat33(np.asarray(eps)[None])
+    m = _flat33(np.asarray(mu)[None])
+    B = np.zeros(16, np.complex128)
+    lib().ora_berreman(_p(e), _p(m), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), _p(B.view(np.float64)))
+    return B.reshape(4, 4, order="F")
+
+
+def modes_layer(eps, mu, ek, mk, omega, kx, ky):
+    e = _flat33(np.asarray(eps)[None])
+    m = _flat33(np.asarray(mu)[None])
+    Q = np.zeros(4, np.complex128)
+    V = np.zeros(16, np.complex128)
+    M = np.zeros(36, np.complex128)
+    rc = lib().ora_modes_layer(_p(e), _p(m), ct.c_int(ek), ct.c_int(mk), ct.c_double(omega), ct.c_double(kx),
+                               ct.c_double(ky), _p(Q.view(np.float64)), _p(V.view(np.float64)), _p(M.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("oracle modes_layer failed (%d)" % rc)
+    return Q, V.reshape(4, 4, order="F"), _unpack_modes(M)
+
+
+def _unpack_modes(M):
+    return dict(Vf=M[0:8].reshape(4, 2, order="F"), Vb=M[8:16].reshape(4, 2, order="F"),
+                Wf=M[16:24].reshape(4, 2, order="F"), Wb=M[24:32].reshape(4, 2, order="F"),
+                Qf=M[32:34].copy(), Qb=M[34:36].copy())
+
+
+def stack_smatrix(st, omega, kx, ky):
+    n = st.n
+    modes = np.zeros(36 * n, np.complex128)
+    Sint = np.zeros(16 * max(n - 1, 1), np.complex128)
+    Sprop = np.zeros(16 * n, np.complex128)
+    SL = np.zeros(16 * n, np.complex128)
+    SR = np.zeros(16 * n, np.complex128)
+    rc = lib().ora_stack_smatrix(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky),
+                                 _p(modes.view(np.float64)), _p(Sint.view(np.float64)), _p(Sprop.view(np.float64)),
+                                 _p(SL.view(np.float64)), _p(SR.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("oracle stack_smatrix failed")
+    M = [_unpack_modes(modes[36 * l:36 * (l + 1)]) for l in range(n)]
+    r = lambda a, l: a[16 * l:16 * (l + 1)].reshape(4, 4, order="F")
+    return (M, [r(Sint, l) for l in range(n - 1)], [r(Sprop, l) for l in range(n)], [r(SL, l) for l in range(n)],
+            [r(SR, l) for l in range(n)])
+
+
+class Cache:
+    """Oracle node cache: per node Q (4), sf, sb (2x3), and per obs layer T
+    (4x4) + basis (16: Exf Eyf Ezf Exb Eyb Ezb KZf KZb, 2 modes each)."""
+
+    def __init__(self, st, omega, src, obs, kx, ky, rec, valid, Tvalid):
+        self.st, self.omega, self.src, self.obs = st, omega, src, list(obs)
+        self.kx, self.ky = kx, ky
+        self.rec, self.valid, self.Tvalid = rec, valid, Tvalid
+        nobs = len(self.obs)
+        rl = 16 + 32 * nobs
+        R = rec.reshape(-1, rl)
+        self.Qf = R[:, 0:2]
+        self.Qb = R[:, 2:4]
+        self.sf = R[:, 4:10].reshape(-1, 3, 2).transpose(0, 2, 1)
+        self.sb = R[:, 10:16].reshape(-1, 3, 2).transpose(0, 2, 1)
+        self.T = {}
+        self.basis = {}
+        for i, o in enumerate(self.obs):
+            b = 16 + 32 * i
+            self.T[o] = R[:, b:b + 16].reshape(-1, 4, 4).transpose(0, 2, 1)
+            self.basis[o] = R[:, b + 16:b + 32].reshape(-1, 8, 2)
+        self.Tv = {o: Tvalid.reshape(-1, nobs)[:, i].astype(bool) for i, o in enumerate(self.obs)}
+
+
+def nodes_batch(st, omega, kx, ky, src, obs, nthreads=0):
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    nn = len(kx)
+    ob = np.ascontiguousarray(obs, np.int32)
+    nobs = len(ob)
+    rec = np.zeros(nn * (16 + 32 * nobs), np.complex128)
+    valid = np.zeros(nn, np.int32)
+    Tvalid = np.zeros(nn * nobs, np.int32)
+    lib().ora_nodes_batch(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), ct.c_int(src),
+                          _p(ob, _i), ct.c_int(nobs), _p(rec.view(np.float64)), _p(valid, _i), _p(Tvalid, _i),
+                          ct.c_int(nthreads))
+    return Cache(st, omega, src, obs, kx, ky, rec, valid.astype(bool), Tvalid)
+
+
+def ghat_from_cache(cache, obs_layer, zsrc, zobs, nthreads=0):
+    zsrc = np.ascontiguousarray(np.atleast_1d(zsrc), np.float64)
+    zobs = np.ascontiguousarray(np.atleast_1d(zobs), np.float64)
+    npairs = len(zsrc)
+    nn = len(cache.kx)
+    g = np.zeros(nn * npairs * 9, np.complex128)
+    ob = np.ascontiguousarray(cache.obs, np.int32)
+    valid = np.ascontiguousarray(cache.valid, np.int32)
+    rc = lib().ora_ghat_from_cache(ct.c_int(cache.st.n), _p(cache.st.d), ct.c_int(cache.src), _p(ob, _i),
+                                   ct.c_int(len(ob)), ct.c_int(obs_layer), _p(cache.rec.view(np.float64)),
+                                   _p(valid, _i), _p(cache.Tvalid, _i), ct.c_int64(nn), ct.c_int(npairs), _p(zsrc),
+                                   _p(zobs), _p(g.view(np.float64)), ct.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError("ghat_from_cache: obs layer not in cache")
+    # [node][pair][c + 3q] -> (nn, npairs, 3(c), 3(q))
+    return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
+
+
+def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0):
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    zsrc = np.ascontiguousarray(np.atleast_1d(zsrc), np.float64)
+    zobs = np.ascontiguousarray(np.atleast_1d(zobs), np.float64)
+    nn, npairs = len(kx), len(zsrc)
+    g = np.zeros(nn * npairs * 9, np.complex128)
+    lib().ora_ghat_batch(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), ct.c_int(src),
+                         ct.c_int(obs_layer), ct.c_int(npairs), _p(zsrc), _p(zobs), _p(g.view(np.float64)),
+                         ct.c_int(nthreads))
+    return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
+
+
+def count_node_ops(st, omega, kx, ky, src, obs_layer, zsrc, zobs):
+    out = np.zeros(10, np.uint64)
+    lib().ora_count_node_ops(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), ct.c_int(src),
+                             ct.c_int(obs_layer), ct.c_double(zsrc), ct.c_double(zobs),
+                             out.ctypes.data_as(ct.POINTER(ct.c_uint64)))
+    keys = ["add", "mul", "div", "sqrt", "trans"]
+    return {"solve": dict(zip(keys, map(int, out[:5]))), "ghat": dict(zip(keys, map(int, out[5:])))}
+
+
+def modal_amp(st, omega, kx, ky, layer, z, sif, sib):
+    sif = _c(sif); sib = _c(sib)
+    Ef = np.zeros(2 * st.n, np.complex128)
+    Eb = np.zeros(2 * st.n, np.complex128)
+    rc = lib().ora_modal_amp(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), ct.c_int(layer),
+                             ct.c_double(z), _p(sif.view(np.float64)), _p(sib.view(np.float64)),
+                             _p(Ef.view(np.float64)), _p(Eb.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("modal_amp failed")
+    return Ef.reshape(st.n, 2), Eb.reshape(st.n, 2)
+
+
+def modal_amp_profile(st, omega, kx, ky, layer, z, sif, sib, zs):
+    sif = _c(sif); sib = _c(sib)
+    zs = np.ascontiguousarray(zs, np.float64)
+    Ef = np.zeros(2 * len(zs), np.complex128)
+    Eb = np.zeros(2 * len(zs), np.complex128)
+    rc = lib().ora_modal_amp_profile(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky),
+                                     ct.c_int(layer), ct.c_double(z), _p(sif.view(np.float64)),
+                                     _p(sib.view(np.float64)), _p(zs), ct.c_int(len(zs)), _p(Ef.view(np.float64)),
+                                     _p(Eb.view(np.float64)))
+    if rc != 0:
+        raise RuntimeError("modal_amp_profile failed")
+    return Ef.reshape(-1, 2), Eb.reshape(-1, 2)
+
+
+def seed_from_source(st, omega, kx, ky, layer, z, sif, sib):
+    sif = _c(sif); sib = _c(sib)
+    u = np.zeros(4, np.complex128)
+    lib().ora_seed_from_source(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), ct.c_int(layer),
+                               ct.c_double(z), _p(sif.view(np.float64)), _p(sib.view(np.float64)),
+                               _p(u.view(np.float64)))
+    return u
+
+
+def closed_form_G_1d(cache, obs_layer, wt, zsrc, zobs, xs, ys, with_direct=True, nthreads=0):
+    wt = np.ascontiguousarray(wt, np.float64)
+    zobs = np.ascontiguousarray(np.atleast_1d(zobs), np.float64)
+    xs = np.ascontiguousarray(np.atleast_1d(xs), np.float64)
+    ys = np.ascontiguousarray(np.atleast_1d(ys), np.float64)
+    nx, ny, nz = len(xs), len(ys), len(zobs)
+    G = np.zeros(nx * ny * nz * 9, np.complex128)
+    ob = np.ascontiguousarray(cache.obs, np.int32)
+    valid = np.ascontiguousarray(cache.valid, np.int32)
+    rc = lib().ora_closed_form_G_1d(ct.c_int(cache.st.n), _p(cache.st.d), ct.c_double(cache.omega),
+                                    ct.c_int(cache.src), _p(ob, _i), ct.c_int(len(ob)), ct.c_int(obs_layer),
+                                    _p(cache.kx), _p(wt), _p(cache.rec.view(np.float64)), _p(valid, _i),
+                                    _p(cache.Tvalid, _i), ct.c_int64(len(cache.kx)), ct.c_double(zsrc), _p(zobs),
+                                    ct.c_int(nz), _p(xs), ct.c_int(nx), _p(ys), ct.c_int(ny),
+                                    ct.c_int(1 if with_direct else 0), _p(G.view(np.float64)), ct.c_int(nthreads))
+    if rc != 0:
+        raise RuntimeError("closed_form_G_1d failed")
+    return G.reshape((nx, ny, nz, 3, 3), order="F")
+
+
+def spectral_apply(g, J, grid):
+    """The transform pair of mod_UniformSource.jl:229-248 (numpy.fft stands in
+    for FFTW): J^ = dx dy ramp fft(J), S_c = sum_q g_cq J^_q,
+    E = (dkx dky Nx Ny / (2pi)^2) ifft(S conj(ramp)),
+    ramp = cis(kx L + ky Ly) (_centering_ramp :145-152).
+    g: (Nx, Ny, 3, 3) complex (c, q); J: (Nx, Ny, 3).  Returns E (Nx, Ny, 3)."""
+    kx, ky = np.asarray(grid["kx"]), np.asarray(grid["ky"])
+    ramp = np.exp(1j * (kx[:, None] * grid["L"] + ky[None, :] * grid["Ly"]))
+    Jh = np.stack([grid["dx"] * grid["dy"] * ramp * np.fft.fft2(J[:, :, q]) for q in range(3)], axis=-1)
+    S = np.einsum("xycq,xyq->xyc", g, Jh)
+    pref = grid["dkx"] * grid["dky"] * grid["Nx"] * grid["Ny"] / (2 * np.pi) ** 2
+    return np.stack([pref * np.fft.ifft2(S[:, :, c] * np.conj(ramp)) for c in range(3)], axis=-1)

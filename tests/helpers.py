@@ -1,0 +1,175 @@
+"""Shared test helpers: the reference's test stacks and quadrature rules,
+restated (Gz_ClosedForm_TestHelpers.jl:6-69, Gz_ClosedForm_*_Validation.jl),
+and conversion of product stacks to the oracle's flat form."""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as po  # noqa: E402
+from yellowthinfilms_amd.materials import (IsotropicMaterial, UniaxialMaterial, BiaxialMaterial,  # noqa: E402
+                                           GyrotropicMaterial, FlatOpticalLayer, FlatLayer, stack_arrays)
+
+NM = 1.0 / 600.0  # lambda = 600 nm = 1
+
+
+def relerr(a, b):
+    """relerr -- Gz_ClosedForm_TestHelpers.jl:6"""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def ostack(stack):
+    """oracle Stack of a product stack (list of layers)."""
+    return po.Stack(*stack_arrays(stack))
+
+
+def lay(m, d):
+    return FlatOpticalLayer(m, d)
+
+
+# Foundation stacks -- Gz_ClosedForm_Foundation_Validation.jl:87-101
+m_vac = IsotropicMaterial(1.0 + 0.001j)
+m_gl = IsotropicMaterial(2.25 + 0.001j)
+m_hi = IsotropicMaterial(6.0 + 0.3j)
+m_uni = UniaxialMaterial(2.0 + 0.01j, 3.5 + 0.01j)
+m_bi = BiaxialMaterial(2.0 + 0.01j, 3.0 + 0.01j, 4.0 + 0.01j)
+m_gyr = GyrotropicMaterial(np.array([[4.0 + 0.01j, 0.3j, 0], [-0.3j, 4.0 + 0.01j, 0], [0, 0, 3.0 + 0.01j]]))
+
+FOUNDATION_STACKS = {
+    "iso 3-layer vac/glass/vac": [lay(m_vac, 2.0), lay(m_gl, 2.0), lay(m_vac, 2.0)],
+    "iso 5-layer lossy": [lay(m_vac, 1.0), lay(m_gl, 0.7), lay(m_hi, 0.3), lay(m_gl, 1.1), lay(m_vac, 1.0)],
+    "uniaxial 4-layer": [lay(m_vac, 1.0), lay(m_uni, 0.8), lay(m_gl, 0.5), lay(m_vac, 1.0)],
+    "biaxial 4-layer": [lay(m_vac, 1.0), lay(m_bi, 0.8), lay(m_gl, 0.5), lay(m_vac, 1.0)],
+    "gyrotropic 4-layer": [lay(m_gyr, 1.0), lay(m_gl, 0.6), lay(m_gyr, 0.9), lay(m_vac, 1.0)],
+}
+FOUNDATION_KTS = [(0.3, 0.0), (1.2, 0.4), (2.9, -1.1), (8.0, 3.0)]
+
+
+def c1_stack():
+    """Air / 50 nm uniaxial film / glass at lambda = 600 nm (the CPU-runnable
+    plain configuration)."""
+    return [lay(IsotropicMaterial(1.0), 1.0), lay(UniaxialMaterial(2.5 + 0.01j, 3.0 + 0.01j), 50 * NM),
+            lay(IsotropicMaterial(2.25), 1.0)]
+
+
+def c2_stack():
+    """5-layer stack with a 40 nm gyrotropic film (g = 0.3, Im eps = 0.02): the
+    batched plain GPU operator configuration."""
+    gy = np.array([[3 + 0.02j, 0.3j, 0], [-0.3j, 3 + 0.02j, 0], [0, 0, 2.8 + 0.02j]])
+    return [lay(IsotropicMaterial(1 + 0.02j), 1.0), lay(IsotropicMaterial(2.25 + 0.02j), 60 * NM),
+            lay(GyrotropicMaterial(gy), 40 * NM), lay(IsotropicMaterial(4.0 + 0.02j), 80 * NM),
+            lay(IsotropicMaterial(2.25 + 0.02j), 1.0)]
+
+
+def gl(a, b, n):
+    x, w = np.polynomial.legendre.leggauss(n)
+    return 0.5 * (b - a) * x + 0.5 * (a + b), 0.5 * (b - a) * w
+
+
+def panels(breaks, n):
+    br = np.unique(np.asarray(breaks, dtype=float))
+    xs, ws = [], []
+    for a, b in zip(br[:-1], br[1:]):
+        x, w = gl(a, b, n)
+        xs.append(x)
+        ws.append(w)
+    return np.concatenate(xs), np.concatenate(ws)
+
+
+def graded(x, lo, hi, levels=40, scale=1.0):
+    """x and x +- scale*2^-j (j = 1..levels) inside (lo, hi)."""
+    pts = [x] + [x + sg * scale * 0.5 ** j for j in range(1, levels + 1) for sg in (-1, 1)]
+    return [t for t in pts if lo < t < hi]
+
+
+def kr_rule(singular, kmax, width=0.5, order=16, levels=40):
+    """Composite GL rule on [0, kmax]: panels of `width`, geometrically graded
+    toward each value in `singular` (light lines)."""
+    br = [0.0, kmax] + list(np.arange(0.0, kmax, width))
+    for s in singular:
+        br += graded(s, 0.0, kmax, levels=levels, scale=min(s, 1.0))
+    return panels(sorted(set(br)), order)
+
+
+def kr_grid(stack, omega, kmax=20.0, width=0.5, order=8):
+    """kρ_grid -- Gz_ClosedForm_TestHelpers.jl:10-24: GL panels of `width`
+    broken at the layers' light lines."""
+    brk = {0.0, kmax}
+    for l in stack:
+        for i in range(2):
+            brk.add(float(np.sqrt(complex(l.eps.eps[i, i])).real * omega))
+    brk = sorted(b for b in brk if 0 <= b <= kmax)
+    x, w = np.polynomial.legendre.leggauss(order)
+    ks, ws = [], []
+    for a, b in zip(brk[:-1], brk[1:]):
+        npan = max(1, int(np.ceil((b - a) / width)))
+        for j in range(npan):
+            lo = a + j * (b - a) / npan
+            hi = lo + (b - a) / npan
+            ks.append((hi - lo) / 2 * x + (hi + lo) / 2)
+            ws.append((hi - lo) / 2 * w)
+    return np.concatenate(ks), np.concatenate(ws)
+
+
+def uniform_phi(Np):
+    """uniform_φ / build_kphi_grid -- Gz_ClosedForm_TestHelpers.jl:26, mod_KxGrid.jl:390-394"""
+    return 2 * np.pi * np.arange(Np) / Np, np.full(Np, 2 * np.pi / Np)
+
+
+def polar_nodes(kr, wr, phis, wphis):
+    """flattened (k fastest) 2D nodes and weights kρ wρ wφ / (4π²)
+    (mod_ClosedFormField.jl:723-727, _node_kxky :526-528)."""
+    K, P = np.meshgrid(kr, phis, indexing="ij")
+    W = np.outer(kr * wr, wphis) / (4 * np.pi ** 2)
+    kx = (K * np.cos(P)).ravel(order="F")
+    ky = (K * np.sin(P)).ravel(order="F")
+    return kx, ky, W.ravel(order="F")
+
+
+def lateral_sum(g, W, kx, ky, xs, ys):
+    """G(x, y) = sum_n W_n g_n exp(i (kx_n x + ky_n y))  (the exact separable
+    lateral sum, mod_ClosedFormField.jl:743-753).  g: (n, 3, 3)."""
+    xs = np.atleast_1d(xs)
+    ys = np.atleast_1d(ys)
+    Px = np.exp(1j * np.outer(kx, xs))
+    Py = np.exp(1j * np.outer(ky, ys))
+    S = g * W[:, None, None]
+    return np.einsum("nc,nd,nij->cdij", Px, Py, S)
+
+
+def E_free(r, p, k, eps):
+    """Free-space dipole field, e^{-iwt}, eps0 = mu0 = c = 1
+    (Gz_ClosedForm_Point_Validation.jl:164-167)."""
+    r = np.asarray(r, dtype=float)
+    p = np.asarray(p, dtype=complex)
+    R = np.linalg.norm(r)
+    n = r / R
+    return (np.exp(1j * k * R) / (4 * np.pi * eps)) * (
+        k ** 2 * np.cross(np.cross(n, p), n) / R + (3 * n * np.dot(n, p) - p) * (1 / R ** 3 - 1j * k / R ** 2))
+
+
+def gl_rule(pans, order=8, singular=()):
+    """gl_rule -- mod_KxGrid.jl:361-384: composite GL, with kρ = L ± u² on a
+    panel that ends on a branch point L in `singular` (smooth in u)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    S = set(float(s) for s in singular)
+    nodes, wts = [], []
+    for a, b in pans:
+        a, b = float(a), float(b)
+        if a in S or b in S:
+            L, sg = (a, 1.0) if a in S else (b, -1.0)
+            U = np.sqrt(b - a)
+            u = 0.5 * U * (1 + x)
+            nodes.append(L + sg * u ** 2)
+            wts.append(0.5 * U * w * 2 * u)
+        else:
+            h, c = 0.5 * (b - a), 0.5 * (a + b)
+            nodes.append(c + h * x)
+            wts.append(h * w)
+    return np.concatenate(nodes), np.concatenate(wts)

@@ -1,0 +1,102 @@
+"""ctypes binding of libytf.so (the C-ABI of include/ytf.h).
+
+The library is built in-tree by yellowthinfilms_amd/build.py.  There is no
+fallback: if libytf.so is missing or cannot be loaded, every entry point
+raises.
+"""
+import ctypes as ct
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libytf.so")
+
+YTF_OK, YTF_EINVAL, YTF_EHIP = 0, 1, 2
+NODE_VALID, NODE_TVALID, NODE_SINGULAR, NODE_NOMODES = 1, 2, 4, 8
+
+
+class YtfLayer(ct.Structure):
+    _fields_ = [("eps", ct.c_double * 18), ("mu", ct.c_double * 18), ("d", ct.c_double),
+                ("eps_kind", ct.c_int32), ("mu_kind", ct.c_int32)]
+
+
+class YtfNodeCache(ct.Structure):
+    _fields_ = [("Qsrc_dev", ct.c_void_p), ("sf_dev", ct.c_void_p), ("sb_dev", ct.c_void_p),
+                ("T_dev", ct.c_void_p), ("basis_dev", ct.c_void_p), ("flags_dev", ct.c_void_p)]
+
+
+class YtfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+_SIGS = {
+    "ytf_version": (ct.c_char_p, []),
+    "ytf_ctx_create": (ct.c_int, [ct.c_int, ct.POINTER(ct.c_void_p)]),
+    "ytf_ctx_destroy": (ct.c_int, [ct.c_void_p]),
+    "ytf_last_error": (ct.c_char_p, [ct.c_void_p]),
+    "ytf_ctx_set_stream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
+    "ytf_green_nodes": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_void_p,
+                                   ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
+                                   ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
+                                   ct.POINTER(YtfNodeCache)]),
+    "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                      ct.c_void_p, ct.c_double]),
+}
+
+EXPORTS = tuple(_SIGS)
+
+
+def lib():
+    """The loaded libytf.so; raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"{LIB_PATH} is not built: run `python -m yellowthinfilms_amd.build` "
+                                   "(there is no CPU fallback)")
+            L = ct.CDLL(LIB_PATH)
+            for name, (res, args) in _SIGS.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+        return _lib
+
+
+def check(rc, ctx=None):
+    if rc != YTF_OK:
+        msg = lib().ytf_last_error(ctx)
+        raise YtfError(rc, msg.decode() if msg else f"ytf error {rc}")
+
+
+class Context:
+    """One ytf_ctx per device; every call is put on torch's current stream so
+    that torch-owned buffers and the kernels are ordered on one stream."""
+
+    _per_device = {}
+
+    def __init__(self, device):
+        h = ct.c_void_p()
+        L = lib()
+        rc = L.ytf_ctx_create(int(device), ct.byref(h))
+        if rc != YTF_OK:
+            raise YtfError(rc, L.ytf_last_error(None).decode())
+        self.handle = h
+        self.device = int(device)
+
+    @classmethod
+    def get(cls, device=0):
+        if device not in cls._per_device:
+            cls._per_device[device] = Context(device)
+        return cls._per_device[device]
+
+    def bind_stream(self):
+        import torch
+        s = torch.cuda.current_stream(self.device)
+        check(lib().ytf_ctx_set_stream(self.handle, ct.c_void_p(s.cuda_stream)), self.handle)

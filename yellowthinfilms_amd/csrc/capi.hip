@@ -1,0 +1,231 @@
+// capi.hip -- implementation of the C-ABI declared in include/ytf.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ytf.h"
+#include "ytf_kernels.h"
+
+using ytf::cd;
+using ytf::LayerDev;
+
+static_assert(sizeof(LayerDev) == sizeof(ytf_layer), "ytf_layer and LayerDev must share one layout");
+
+struct ytf_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;
+  // small per-call inputs (layers, z pairs), grown on demand; `small_host`
+  // mirrors what the device copy holds so unchanged inputs are not re-sent
+  void *small = nullptr;
+  size_t small_bytes = 0;
+  std::vector<char> small_host;
+  // FFT twiddle tables, one per length
+  std::map<int, cd *> tw;
+};
+
+static thread_local std::string g_err;
+
+static int fail(ytf_ctx *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static int fail(ytf_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  g_err = buf;
+  return code;
+}
+#define HIPCHK(c, expr)                                                                           \
+  do {                                                                                            \
+    hipError_t _e = (expr);                                                                       \
+    if (_e != hipSuccess) return fail((c), YTF_EHIP, "%s: %s", #expr, hipGetErrorString(_e));     \
+  } while (0)
+
+static int ensure_small(ytf_ctx *c, size_t bytes) {
+  if (bytes <= c->small_bytes) return YTF_OK;
+  if (c->small) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(c->small));
+    c->small = nullptr;
+    c->small_bytes = 0;
+  }
+  size_t nb = bytes < 65536 ? 65536 : bytes;
+  HIPCHK(c, hipMalloc(&c->small, nb));
+  c->small_bytes = nb;
+  return YTF_OK;
+}
+
+static int twiddles(ytf_ctx *c, int n, cd **out) {
+  auto it = c->tw.find(n);
+  if (it != c->tw.end()) { *out = it->second; return YTF_OK; }
+  std::vector<cd> h(n);
+  for (int j = 0; j < n; ++j) {
+    // exp(-2 pi i j / n) with the argument reduced exactly to the first octant
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+    h[j].r = (double)cosl(a);
+    h[j].i = (double)sinl(a);
+  }
+  cd *d = nullptr;
+  HIPCHK(c, hipMalloc(&d, sizeof(cd) * n));
+  HIPCHK(c, hipMemcpy(d, h.data(), sizeof(cd) * n, hipMemcpyHostToDevice));
+  c->tw[n] = d;
+  *out = d;
+  return YTF_OK;
+}
+
+extern "C" {
+
+const char *ytf_version(void) { return "yellowthinfilms_amd 0.1 (gfx950)"; }
+
+int ytf_ctx_create(int device, ytf_ctx **out) {
+  if (!out) return fail(nullptr, YTF_EINVAL, "ytf_ctx_create: out is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(nullptr, YTF_EHIP, "ytf_ctx_create: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(nullptr, YTF_EINVAL, "ytf_ctx_create: device %d out of range 0..%d", device, n - 1);
+  ytf_ctx *c = new ytf_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, YTF_EHIP, "ytf_ctx_create: stream creation failed");
+  }
+  c->stream = c->own;
+  *out = c;
+  return YTF_OK;
+}
+
+int ytf_ctx_destroy(ytf_ctx *c) {
+  if (!c) return YTF_OK;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  for (auto &kv : c->tw) hipFree(kv.second);
+  if (c->small) hipFree(c->small);
+  if (c->own) hipStreamDestroy(c->own);
+  delete c;
+  return YTF_OK;
+}
+
+const char *ytf_last_error(const ytf_ctx *c) { return c ? c->err.c_str() : g_err.c_str(); }
+
+int ytf_ctx_set_stream(ytf_ctx *c, void *s) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_ctx_set_stream: ctx is NULL");
+  c->stream = s ? (hipStream_t)s : c->own;
+  return YTF_OK;
+}
+
+int ytf_ctx_synchronize(ytf_ctx *c) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_ctx_synchronize: ctx is NULL");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return YTF_OK;
+}
+
+int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                    const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_green_nodes: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_green_nodes: need at least 2 layers (got %d)", n_layers);
+  if (source_layer < 1 || source_layer > n_layers - 1)
+    return fail(c, YTF_EINVAL,
+                "ytf_green_nodes: source_layer=%d must be in 1:%d (the last, semi-infinite layer cannot host a source)",
+                source_layer, n_layers - 1);
+  if (obs_layer < 1 || obs_layer > n_layers)
+    return fail(c, YTF_EINVAL, "ytf_green_nodes: obs_layer=%d out of range 1:%d", obs_layer, n_layers);
+  if (n_nodes < 0) return fail(c, YTF_EINVAL, "ytf_green_nodes: n_nodes < 0");
+  if (n_nodes > 0 && (!kx_dev || !ky_dev)) return fail(c, YTF_EINVAL, "ytf_green_nodes: kx/ky are NULL");
+  if (n_pairs < 0 || (n_pairs > 0 && (!zsrc || !zobs))) return fail(c, YTF_EINVAL, "ytf_green_nodes: bad depth pairs");
+  if (ghat_dev && n_pairs == 0) return fail(c, YTF_EINVAL, "ytf_green_nodes: ghat requested with no depth pairs");
+  if (!(omega > 0) || !std::isfinite(omega)) return fail(c, YTF_EINVAL, "ytf_green_nodes: omega must be > 0");
+  for (int l = 0; l < n_layers; ++l) {
+    if (layers[l].eps_kind < 0 || layers[l].eps_kind > 3 || layers[l].mu_kind < 0 || layers[l].mu_kind > 1)
+      return fail(c, YTF_EINVAL, "ytf_green_nodes: layer %d has an unknown material kind", l + 1);
+  }
+  const double ds = layers[source_layer - 1].d;
+  for (int p = 0; p < n_pairs; ++p) {
+    const double zp = source_layer == 1 ? ds - zsrc[p] : zsrc[p];
+    const double tol = 1.4901161193847656e-08;
+    if (!(zp >= -tol && zp <= ds + tol))
+      return fail(c, YTF_EINVAL, "amplitude_at: zsrc_local=%g outside [0,%g]", zsrc[p], ds);
+  }
+  if (n_nodes == 0) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t lb = sizeof(ytf_layer) * n_layers, zb = sizeof(double) * (size_t)n_pairs;
+  std::vector<char> hb(lb + 2 * zb);
+  std::memcpy(hb.data(), layers, lb);
+  if (n_pairs) {
+    std::memcpy(hb.data() + lb, zsrc, zb);
+    std::memcpy(hb.data() + lb + zb, zobs, zb);
+  }
+  if (hb != c->small_host) {
+    // the previous kernels may still read the buffer: wait, copy, wait (the
+    // host bytes are pageable and local to this call)
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = ensure_small(c, hb.size() + 64);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->small, hb.data(), hb.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->small_host = hb;
+  }
+  char *sm = (char *)c->small;
+  ytf::NodeParams P;
+  P.L = (const LayerDev *)sm;
+  P.nl = n_layers;
+  P.omega = omega;
+  P.kx = kx_dev;
+  P.ky = ky_dev;
+  P.nn = n_nodes;
+  P.src = source_layer - 1;
+  P.obs = obs_layer - 1;
+  P.npairs = n_pairs;
+  P.zsrc = (const double *)(sm + lb);
+  P.zobs = (const double *)(sm + lb + zb);
+  P.ghat = (cd *)ghat_dev;
+  P.Qsrc = cache ? (cd *)cache->Qsrc_dev : nullptr;
+  P.sf = cache ? (cd *)cache->sf_dev : nullptr;
+  P.sb = cache ? (cd *)cache->sb_dev : nullptr;
+  P.T = cache ? (cd *)cache->T_dev : nullptr;
+  P.basis = cache ? (cd *)cache->basis_dev : nullptr;
+  P.flags = cache ? (unsigned char *)cache->flags_dev : nullptr;
+  if ((P.sf == nullptr) != (P.sb == nullptr)) return fail(c, YTF_EINVAL, "ytf_green_nodes: sf and sb go together");
+  int64_t blocks = (n_nodes + 63) / 64;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  hipError_t e = ytf::launch_node_kernel(P, (int)blocks, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_green_nodes: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
+
+int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
+                       double *work_dev, double scale) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_spectral_apply: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!pow2_in(nx, 8, 2048) || !pow2_in(ny, 8, 2048))
+    return fail(c, YTF_EINVAL, "ytf_spectral_apply: nx=%d, ny=%d must be powers of two in 8..2048", nx, ny);
+  if (!ghat_dev || !J_dev || !E_dev || !work_dev) return fail(c, YTF_EINVAL, "ytf_spectral_apply: NULL buffer");
+  if ((const void *)E_dev == (const void *)J_dev) return fail(c, YTF_EINVAL, "ytf_spectral_apply: E aliases J");
+  HIPCHK(c, hipSetDevice(c->device));
+  cd *twx = nullptr, *twy = nullptr;
+  int rc = twiddles(c, nx, &twx);
+  if (rc) return rc;
+  rc = twiddles(c, ny, &twy);
+  if (rc) return rc;
+  hipError_t e = ytf::launch_spectral_apply(nx, ny, (const cd *)ghat_dev, (const cd *)J_dev, (cd *)E_dev,
+                                            (cd *)work_dev, twx, twy, scale, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_spectral_apply: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,265 @@
+// fft_kernels.hip -- LDS-staged FP64 FFTs and the fused spectral convolution
+// of the uniform-grid Green operator on gfx950.
+//
+// Reference behaviour: the spectral transform pair of mod_UniformSource.jl
+// (_forward_transform :229-230, _inverse_transform :246-248): for a source
+// density J on the nx x ny grid,
+//     E = (dkx dky nx ny / (2 pi)^2) * ifft( g~ .* conj(ramp) .* (dx dy ramp .* fft(J)) ),
+// and since |ramp| = 1 this is E = scale * bfft2(g~ .* fft2(J)) with the
+// whole scalar factor (including ifft's 1/(nx ny)) in `scale`.
+//
+// Three passes over HBM, each LDS-staged (radix-8 Stockham FFTs, one 1D
+// transform per N/8 threads):
+//   1. fft along x of J (nx, ny, 3) column-major, written transposed into the
+//      work buffer laid out [c][x][y] (y fastest);
+//   2. per pair of x columns, all three components: fft along y, the 3x3
+//      multiply by g~ (stored y-fastest, node index y + ny*x), inverse fft
+//      along y -- fused, in place in the work buffer;
+//   3. inverse fft along x, transposed back into E (nx, ny, 3).
+// Power-of-two sizes 8..2048.
+#include <hip/hip_runtime.h>
+
+#include "ytf_device.hpp"
+#include "ytf_kernels.h"
+
+namespace ytf {
+
+template <bool INV>
+__device__ __forceinline__ cd mul_mi(cd a) {  // a * (-i) forward, a * (+i) inverse
+  return INV ? cd{-a.i, a.r} : cd{a.i, -a.r};
+}
+template <bool INV>
+__device__ __forceinline__ void dft2(cd &a0, cd &a1) {
+  const cd t = a0;
+  a0 = t + a1;
+  a1 = t - a1;
+}
+template <bool INV>
+__device__ __forceinline__ void dft4(cd &a0, cd &a1, cd &a2, cd &a3) {
+  const cd t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_mi<INV>(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+template <bool INV>
+__device__ __forceinline__ void dft8(cd *a) {
+  cd e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  cd o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const double h = 0.70710678118654752440;
+  // w8^k o_k, w8 = exp(-+ 2 pi i / 8)
+  const cd w1 = INV ? cd{h, h} : cd{h, -h};
+  const cd w3 = INV ? cd{-h, h} : cd{-h, -h};
+  o1 = o1 * w1;
+  o2 = mul_mi<INV>(o2);
+  o3 = o3 * w3;
+  a[0] = e0 + o0; a[4] = e0 - o0;
+  a[1] = e1 + o1; a[5] = e1 - o1;
+  a[2] = e2 + o2; a[6] = e2 - o2;
+  a[3] = e3 + o3; a[7] = e3 - o3;
+}
+
+// One Stockham radix-R stage on a row in LDS: current length n, stride s.
+// Thread t in [0, N/8) performs 8/R butterflies.  tw[j] = exp(-2 pi i j/N).
+template <int N, int R, bool INV>
+__device__ __forceinline__ void stage(cd *row, int t, int n, int s, const cd *tw) {
+  constexpr int NB = 8 / R;
+  const int m = n / R;
+  cd a[8];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * (N / 8);
+    const int q = b % s, p = b / s;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[u * R + r] = row[q + s * (p + r * m)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * (N / 8);
+    const int q = b % s, p = b / s;
+    cd *x = a + u * R;
+    if (R == 8) dft8<INV>(x);
+    if (R == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
+    if (R == 2) dft2<INV>(x[0], x[1]);
+    const int step = N / n;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      cd w = tw[(p * k) * step];
+      if (INV) w.i = -w.i;
+      row[q + s * (R * p + k)] = k == 0 ? x[0] : x[k] * w;
+    }
+  }
+  __syncthreads();
+}
+
+// Full length-N transform of `row` (in LDS) by N/8 cooperating threads; all
+// threads of the block must call it (it synchronises the block).
+template <int N, bool INV>
+__device__ __forceinline__ void fft_lds(cd *row, int t, const cd *tw) {
+  int n = N, s = 1;
+#pragma unroll
+  for (int st = 0; st < 12; ++st) {
+    if (n >= 8) {
+      stage<N, 8, INV>(row, t, n, s, tw);
+      n /= 8;
+      s *= 8;
+    }
+  }
+  if (n == 4) stage<N, 4, INV>(row, t, n, s, tw);
+  if (n == 2) stage<N, 2, INV>(row, t, n, s, tw);
+}
+
+// Rows per block of the x passes.
+template <int N> struct XTile { static constexpr int YT = (8192 / N) < 8 ? (8192 / N) : 8; };
+
+// Pass 1: work[c][x][y] = fft_x(J)[x, y, c]
+template <int NX>
+__global__ __launch_bounds__(1024) void k_fft_x_fwd(const cd *__restrict__ J, cd *__restrict__ work, int ny,
+                                                    const cd *__restrict__ tw) {
+  constexpr int YT = XTile<NX>::YT;
+  extern __shared__ cd lds[];
+  const int c = blockIdx.y;
+  const int y0 = blockIdx.x * YT;
+  const int tid = threadIdx.x;
+  constexpr int TPR = NX / 8;
+  // coalesced load of YT rows
+  for (int e = tid; e < YT * NX; e += YT * TPR) {
+    const int r = e / NX, x = e % NX;
+    lds[r * NX + x] = J[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)];
+  }
+  __syncthreads();
+  fft_lds<NX, false>(lds + (tid / TPR) * NX, tid % TPR, tw);
+  // transposed store: consecutive threads -> consecutive y
+  for (int e = tid; e < YT * NX; e += YT * TPR) {
+    const int r = e % YT, x = e / YT;
+    work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)] = lds[r * NX + x];
+  }
+}
+
+// Pass 3: E[x, y, c] = bfft_x(work)[c][x][y]
+template <int NX>
+__global__ __launch_bounds__(1024) void k_fft_x_inv(const cd *__restrict__ work, cd *__restrict__ E, int ny,
+                                                    const cd *__restrict__ tw) {
+  constexpr int YT = XTile<NX>::YT;
+  extern __shared__ cd lds[];
+  const int c = blockIdx.y;
+  const int y0 = blockIdx.x * YT;
+  const int tid = threadIdx.x;
+  constexpr int TPR = NX / 8;
+  for (int e = tid; e < YT * NX; e += YT * TPR) {
+    const int r = e % YT, x = e / YT;
+    lds[r * NX + x] = work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)];
+  }
+  __syncthreads();
+  fft_lds<NX, true>(lds + (tid / TPR) * NX, tid % TPR, tw);
+  for (int e = tid; e < YT * NX; e += YT * TPR) {
+    const int r = e / NX, x = e % NX;
+    E[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)] = lds[r * NX + x];
+  }
+}
+
+template <int N> struct YTile { static constexpr int XT = (N <= 1024) ? 2 : 1; };
+
+// Pass 2: for XT columns x, all c: fft_y, 3x3 multiply by g~ (times scale),
+// bfft_y; in place on work[c][x][y].  g~ is [9][nn] with node y + ny*x.
+template <int NY>
+__global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const cd *__restrict__ g, int nx,
+                                                    const cd *__restrict__ tw, double scale) {
+  constexpr int XT = YTile<NY>::XT;
+  constexpr int TPR = NY / 8;
+  extern __shared__ cd lds[];  // [3*XT][NY]
+  const int x0 = blockIdx.x * XT;
+  const int tid = threadIdx.x;
+  const int nthr = 3 * XT * TPR;
+  const int64_t nn = (int64_t)nx * NY;
+  for (int e = tid; e < 3 * XT * NY; e += nthr) {
+    const int col = e / NY, y = e % NY;
+    const int c = col / XT, xi = col % XT;
+    lds[col * NY + y] = work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)];
+  }
+  __syncthreads();
+  fft_lds<NY, false>(lds + (tid / TPR) * NY, tid % TPR, tw);
+  for (int e = tid; e < XT * NY; e += nthr) {
+    const int xi = e / NY, y = e % NY;
+    const int64_t node = y + (int64_t)NY * (x0 + xi);
+    const cd j0 = lds[(0 * XT + xi) * NY + y], j1 = lds[(1 * XT + xi) * NY + y], j2 = lds[(2 * XT + xi) * NY + y];
+    cd out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cd acc = g[(c + 0) * nn + node] * j0;
+      acc = fmac(acc, g[(c + 3) * nn + node], j1);
+      acc = fmac(acc, g[(c + 6) * nn + node], j2);
+      out[c] = rmul(scale, acc);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * NY + y] = out[c];
+  }
+  __syncthreads();
+  fft_lds<NY, true>(lds + (tid / TPR) * NY, tid % TPR, tw);
+  for (int e = tid; e < 3 * XT * NY; e += nthr) {
+    const int col = e / NY, y = e % NY;
+    const int c = col / XT, xi = col % XT;
+    work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)] = lds[col * NY + y];
+  }
+}
+
+template <int N>
+static hipError_t launch_x(bool inv, const cd *in, cd *out, int ny, const cd *tw, hipStream_t st) {
+  constexpr int YT = XTile<N>::YT;
+  const dim3 grid(ny / YT, 3), block(YT * (N / 8));
+  const size_t sh = (size_t)YT * N * sizeof(cd);
+  if (inv) hipLaunchKernelGGL(k_fft_x_inv<N>, grid, block, sh, st, in, out, ny, tw);
+  else hipLaunchKernelGGL(k_fft_x_fwd<N>, grid, block, sh, st, in, out, ny, tw);
+  return hipGetLastError();
+}
+template <int N>
+static hipError_t launch_y(cd *work, const cd *g, int nx, const cd *tw, double scale, hipStream_t st) {
+  constexpr int XT = YTile<N>::XT;
+  const dim3 grid(nx / XT), block(3 * XT * (N / 8));
+  const size_t sh = (size_t)3 * XT * N * sizeof(cd);
+  hipLaunchKernelGGL(k_fft_y_mul<N>, grid, block, sh, st, work, g, nx, tw, scale);
+  return hipGetLastError();
+}
+
+static hipError_t dispatch_x(int n, bool inv, const cd *in, cd *out, int ny, const cd *tw, hipStream_t st) {
+  switch (n) {
+    case 8: return launch_x<8>(inv, in, out, ny, tw, st);
+    case 16: return launch_x<16>(inv, in, out, ny, tw, st);
+    case 32: return launch_x<32>(inv, in, out, ny, tw, st);
+    case 64: return launch_x<64>(inv, in, out, ny, tw, st);
+    case 128: return launch_x<128>(inv, in, out, ny, tw, st);
+    case 256: return launch_x<256>(inv, in, out, ny, tw, st);
+    case 512: return launch_x<512>(inv, in, out, ny, tw, st);
+    case 1024: return launch_x<1024>(inv, in, out, ny, tw, st);
+    case 2048: return launch_x<2048>(inv, in, out, ny, tw, st);
+  }
+  return hipErrorInvalidValue;
+}
+static hipError_t dispatch_y(int n, cd *work, const cd *g, int nx, const cd *tw, double scale, hipStream_t st) {
+  switch (n) {
+    case 8: return launch_y<8>(work, g, nx, tw, scale, st);
+    case 16: return launch_y<16>(work, g, nx, tw, scale, st);
+    case 32: return launch_y<32>(work, g, nx, tw, scale, st);
+    case 64: return launch_y<64>(work, g, nx, tw, scale, st);
+    case 128: return launch_y<128>(work, g, nx, tw, scale, st);
+    case 256: return launch_y<256>(work, g, nx, tw, scale, st);
+    case 512: return launch_y<512>(work, g, nx, tw, scale, st);
+    case 1024: return launch_y<1024>(work, g, nx, tw, scale, st);
+    case 2048: return launch_y<2048>(work, g, nx, tw, scale, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_spectral_apply(int nx, int ny, const cd *ghat_xy, const cd *J, cd *E, cd *work, const cd *tw_x,
+                                 const cd *tw_y, double scale, hipStream_t stream) {
+  hipError_t e = dispatch_x(nx, false, J, work, ny, tw_x, stream);
+  if (e != hipSuccess) return e;
+  e = dispatch_y(ny, work, ghat_xy, nx, tw_y, scale, stream);
+  if (e != hipSuccess) return e;
+  return dispatch_x(nx, true, work, E, ny, tw_x, stream);
+}
+
+}  // namespace ytf

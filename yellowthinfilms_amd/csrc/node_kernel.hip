@@ -1,0 +1,1074 @@
+// node_kernel.hip -- the batched Green k-node solve on gfx950, FP64.
+//
+// One lane solves one in-plane wavevector node (kx, ky) of a planar stack:
+//   Berreman 4x4 modes of every layer (closed form for isotropic/uniaxial
+//   layers, general complex eigen-decomposition otherwise), forward/backward
+//   classification and continuity matching, interface S-matrices, the
+//   Redheffer cascade, the cavity resummation of the source layer, the walk to
+//   the observation layer (T^o), the point-source seed projection and the
+//   observation layer's mode-field basis; optionally the spectral dyadic
+//   g~(k; z, z') for a list of (source depth, observation depth) pairs.
+//
+// Reference behaviour followed (HortenseBeaudoin/YellowThinFilms):
+//   modes            mod_ModeSolver.jl:122-193, 203-265, 274-349, 358-392
+//   S-matrices       mod_ScatteringMatrix.jl:27-154
+//   seed projection  mod_SourceProj.jl:34-60
+//   cavity, walk, T  mod_FieldSolver.jl:24-53, 72-152, 191-341
+//   node + basis     mod_CavityTransfer.jl:87-130, 224-251, 339-359
+//   spectral dyadic  mod_ClosedFormField.jl:59-69, 539-602, 616-659
+//
+// GPU-specific restructuring (same mathematics):
+//   * Single forward pass over the layers.  SRight[k] (the reference's
+//     backward suffix scan) is accumulated left to right as
+//     eff_k * eff_{k+1} * ... with eff_k = Sprop[k] * Sint[k] ("*" the Redheffer
+//     star product, which is associative), so no per-layer S-matrix storage
+//     is needed; only the suffixes a node actually uses are kept.
+//   * The source and observation layers' modes are re-solved after the pass
+//     (with the recorded matching swaps), so their 4x4 mode matrices are not
+//     held in registers through the cascade.
+//   * The Redheffer product with a diagonal propagation matrix is specialised
+//     (its 2x2 inverses are exact identities).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ytf_device.hpp"
+#include "ytf_kernels.h"
+
+namespace ytf {
+
+// ---------------------------------------------------------------------------
+// Berreman matrix -- mod_ModeSolver.jl:122-193
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void berreman(const LayerDev &L, double w, double kx, double ky, cd *B) {
+  const cd e11 = L.eps[0], e21 = L.eps[1], e31 = L.eps[2], e12 = L.eps[3], e22 = L.eps[4], e32 = L.eps[5];
+  const cd e13 = L.eps[6], e23 = L.eps[7], e33 = L.eps[8];
+  const cd m11 = L.mu[0], m21 = L.mu[1], m31 = L.mu[2], m12 = L.mu[3], m22 = L.mu[4], m32 = L.mu[5];
+  const cd m13 = L.mu[6], m23 = L.mu[7], m33 = L.mu[8];
+  const cd k1 = mk(kx), k2 = mk(ky), om = mk(w), one = mk(1.0);
+  const cd denE = one / e33, denM = one / m33, iw = one / om;
+  const cd Se11 = e11 - e13 * denE * e31, Se12 = e12 - e13 * denE * e32;
+  const cd Se21 = e21 - e23 * denE * e31, Se22 = e22 - e23 * denE * e32;
+  const cd Sm11 = m11 - m13 * denM * m31, Sm12 = m12 - m13 * denM * m32;
+  const cd Sm21 = m21 - m23 * denM * m31, Sm22 = m22 - m23 * denM * m32;
+  const cd rE1 = denE * e31, rE2 = denE * e32, sE1 = e13 * denE, sE2 = e23 * denE;
+  const cd rM1 = denM * m31, rM2 = denM * m32, sM1 = m13 * denM, sM2 = m23 * denM;
+  const cd t21 = -conj(k2), t22 = conj(k1);
+  const cd t31 = iw * k1, t32 = iw * k2;
+  const cd RsM1 = sM2, RsM2 = -sM1, RsE1 = sE2, RsE2 = -sE1;
+  B[0] = RsM1 * t21 - k1 * rE1;        // B11_11
+  B[4] = RsM1 * t22 - k1 * rE2;        // B11_12
+  B[1] = RsM2 * t21 - k2 * rE1;        // B11_21
+  B[5] = RsM2 * t22 - k2 * rE2;        // B11_22
+  B[8] = om * Sm21 - t31 * denE * t21;   // B12_11
+  B[12] = om * Sm22 - t31 * denE * t22;  // B12_12
+  B[9] = -om * Sm11 - t32 * denE * t21;  // B12_21
+  B[13] = -om * Sm12 - t32 * denE * t22; // B12_22
+  B[2] = t31 * denM * t21 - om * Se21;   // B21_11
+  B[6] = t31 * denM * t22 - om * Se22;   // B21_12
+  B[3] = t32 * denM * t21 + om * Se11;   // B21_21
+  B[7] = t32 * denM * t22 + om * Se12;   // B21_22
+  B[10] = RsE1 * t21 - k1 * rM1;       // B22_11
+  B[14] = RsE1 * t22 - k1 * rM2;       // B22_12
+  B[11] = RsE2 * t21 - k2 * rM1;       // B22_21
+  B[15] = RsE2 * t22 - k2 * rM2;       // B22_22
+}
+
+// ---------------------------------------------------------------------------
+// General complex 4x4 eigen-decomposition (stands in for LAPACK geevx):
+// diagonal balancing, Householder Hessenberg, shifted QR to Schur form with
+// accumulated Z, triangular back-substitution, back-transform, zgeev-style
+// normalisation (unit 2-norm, largest component real positive), then the
+// (Re, Im) ascending order of Julia's eigen.  Returns false on no convergence.
+// ---------------------------------------------------------------------------
+template <int IHI>
+__device__ __forceinline__ bool qr_deflate(cd *A, cd *Z) {
+  const double ulp = 2.220446049250313e-16;
+#define AA(i, j) A[(i) + 4 * (j)]
+#define ZZ(i, j) Z[(i) + 4 * (j)]
+  for (int it = 0; it < 60; ++it) {
+    double hk = cabs1(AA(IHI, IHI - 1));
+    double tst = cabs1(AA(IHI - 1, IHI - 1)) + cabs1(AA(IHI, IHI));
+    if (tst == 0.0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tst += cabs1(AA(j, IHI));
+    }
+    if (hk <= ulp * tst) {
+      AA(IHI, IHI - 1) = mk(0.0);
+      return true;
+    }
+    cd mu;
+    if (it % 11 == 10) {
+      mu = AA(IHI, IHI) + mk(0.75 * cabs(AA(IHI, IHI - 1)));
+    } else {
+      const cd a = AA(IHI - 1, IHI - 1), b = AA(IHI - 1, IHI), c = AA(IHI, IHI - 1), d = AA(IHI, IHI);
+      const cd tr2 = rmul(0.5, a + d), dd = rmul(0.5, a - d);
+      const cd disc = csqrt(dd * dd + b * c);
+      const cd l1 = tr2 + disc, l2 = tr2 - disc;
+      mu = sel(abs2(l1 - d) <= abs2(l2 - d), l1, l2);
+    }
+#pragma unroll
+    for (int k = 0; k <= IHI; ++k) AA(k, k) = AA(k, k) - mu;
+    double cs[IHI];
+    cd sn[IHI];
+#pragma unroll
+    for (int k = 0; k < IHI; ++k) {
+      const cd a = AA(k, k), b = AA(k + 1, k);
+      const double na = cabs(a), nb = cabs(b);
+      const double nrm = hypot(na, nb);
+      double c = 1.0;
+      cd s = mk(0.0);
+      if (nrm != 0.0) {
+        if (na == 0.0) {
+          c = 0.0;
+          s = mk(1.0);
+        } else {
+          c = na / nrm;
+          s = rmul(1.0 / (na * nrm), a * conj(b));
+        }
+      }
+      cs[k] = c;
+      sn[k] = s;
+#pragma unroll
+      for (int j = k; j < 4; ++j) {
+        const cd x = AA(k, j), y = AA(k + 1, j);
+        AA(k, j) = rmul(c, x) + s * y;
+        AA(k + 1, j) = rmul(c, y) - conj(s) * x;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < IHI; ++k) {
+      const double c = cs[k];
+      const cd s = sn[k];
+#pragma unroll
+      for (int i = 0; i <= (k + 1 < IHI ? k + 1 : IHI); ++i) {
+        const cd x = AA(i, k), y = AA(i, k + 1);
+        AA(i, k) = rmul(c, x) + y * conj(s);
+        AA(i, k + 1) = rmul(c, y) - x * s;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const cd x = ZZ(i, k), y = ZZ(i, k + 1);
+        ZZ(i, k) = rmul(c, x) + y * conj(s);
+        ZZ(i, k + 1) = rmul(c, y) - x * s;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k <= IHI; ++k) AA(k, k) = AA(k, k) + mu;
+  }
+  return false;
+#undef AA
+#undef ZZ
+}
+
+__device__ __forceinline__ void cswap_col(bool c, cd *Q, cd *V, int a, int b) {
+  const cd qa = Q[a], qb = Q[b];
+  Q[a] = sel(c, qb, qa);
+  Q[b] = sel(c, qa, qb);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const cd va = V[i + 4 * a], vb = V[i + 4 * b];
+    V[i + 4 * a] = sel(c, vb, va);
+    V[i + 4 * b] = sel(c, va, vb);
+  }
+}
+__device__ __forceinline__ bool lessRI(cd a, cd b) { return a.r < b.r || (a.r == b.r && a.i < b.i); }
+
+__device__ __forceinline__ bool eigen4(cd *A, cd *Q, cd *V) {
+  const double ulp = 2.220446049250313e-16;
+#define AA(i, j) A[(i) + 4 * (j)]
+  // balance (EISPACK balanc, radix 2, no permutation)
+  double D[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    bool conv = true;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double c = 0.0, r = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (j == i) continue;
+        c += cabs1(AA(j, i));
+        r += cabs1(AA(i, j));
+      }
+      if (c == 0.0 || r == 0.0) continue;
+      double g = r * 0.5, f = 1.0;
+      const double s = c + r;
+      while (c < g) { f *= 2.0; c *= 4.0; }
+      g = r * 2.0;
+      while (c >= g) { f *= 0.5; c *= 0.25; }
+      if ((c + r) / f < 0.95 * s) {
+        conv = false;
+        D[i] *= f;
+        const double fi = 1.0 / f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) AA(i, j) = rmul(fi, AA(i, j));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) AA(j, i) = rmul(f, AA(j, i));
+      }
+    }
+    if (conv) break;
+  }
+  // Householder reduction to Hessenberg form, accumulating Z
+  cd Z[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Z[i] = mk((i % 5 == 0) ? 1.0 : 0.0);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    double xn2 = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) xn2 += abs2(AA(i, k));
+    if (xn2 != 0.0) {
+      const double xn = sqrt(xn2);
+      const cd x0 = AA(k + 1, k);
+      const double ax0 = cabs(x0);
+      const cd ph = ax0 == 0.0 ? mk(1.0) : rmul(1.0 / ax0, x0);
+      const cd alpha = rmul(-xn, ph);
+      cd v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = (i < k + 1) ? mk(0.0) : AA(i, k);
+      v[k + 1] = x0 - alpha;
+      double vn = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < 4; ++i) vn += abs2(v[i]);
+      if (vn != 0.0) {
+        const double tau = 2.0 / vn;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          cd s = mk(0.0);
+#pragma unroll
+          for (int i = k + 1; i < 4; ++i) s = fmac_cj(s, v[i], AA(i, j));
+          s = rmul(tau, s);
+#pragma unroll
+          for (int i = k + 1; i < 4; ++i) AA(i, j) = AA(i, j) - v[i] * s;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          cd s = mk(0.0), sz = mk(0.0);
+#pragma unroll
+          for (int j = k + 1; j < 4; ++j) {
+            s = fmac(s, AA(i, j), v[j]);
+            sz = fmac(sz, Z[i + 4 * j], v[j]);
+          }
+          s = rmul(tau, s);
+          sz = rmul(tau, sz);
+#pragma unroll
+          for (int j = k + 1; j < 4; ++j) {
+            AA(i, j) = AA(i, j) - s * conj(v[j]);
+            Z[i + 4 * j] = Z[i + 4 * j] - sz * conj(v[j]);
+          }
+        }
+#pragma unroll
+        for (int i = k + 2; i < 4; ++i) AA(i, k) = mk(0.0);
+      }
+    }
+  }
+  bool ok = qr_deflate<3>(A, Z);
+  ok = ok && qr_deflate<2>(A, Z);
+  ok = ok && qr_deflate<1>(A, Z);
+  // eigenvectors of the upper-triangular Schur factor, back-transformed
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const cd lam = AA(k, k);
+    cd x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = mk(i == k ? 1.0 : 0.0);
+    double smin = ulp * cabs(lam);
+    smin = smin > 1e-290 ? smin : 1e-290;
+#pragma unroll
+    for (int j = k - 1; j >= 0; --j) {
+      cd s = mk(0.0);
+#pragma unroll
+      for (int l = j + 1; l <= k; ++l) s = fmac(s, AA(j, l), x[l]);
+      cd den = AA(j, j) - lam;
+      if (cabs(den) < smin) den = mk(smin);
+      x[j] = -(s / den);
+    }
+    double s2 = 0.0;
+    cd col[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cd s = mk(0.0);
+#pragma unroll
+      for (int j = 0; j <= k; ++j) s = fmac(s, Z[i + 4 * j], x[j]);
+      col[i] = rmul(D[i], s);
+      s2 += abs2(col[i]);
+    }
+    const double inv = 1.0 / sqrt(s2);
+    double best = -1.0;
+    cd p = mk(1.0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      col[i] = rmul(inv, col[i]);
+      const double a2 = abs2(col[i]);
+      if (a2 > best) {
+        best = a2;
+        p = col[i];
+      }
+    }
+    const double ap = cabs(p);
+    const cd rot = mk(p.r / ap, -(p.i / ap));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const cd v = col[i] * rot;
+      // the largest component is exactly real
+      const bool is_max = (abs2(col[i]) == best);
+      V[i + 4 * k] = cd{v.r, is_max ? 0.0 : v.i};
+    }
+    Q[k] = lam;
+  }
+#undef AA
+  // sort eigenpairs by (Re, Im): 4-element sorting network
+  cswap_col(lessRI(Q[1], Q[0]), Q, V, 0, 1);
+  cswap_col(lessRI(Q[3], Q[2]), Q, V, 2, 3);
+  cswap_col(lessRI(Q[2], Q[0]), Q, V, 0, 2);
+  cswap_col(lessRI(Q[3], Q[1]), Q, V, 1, 3);
+  cswap_col(lessRI(Q[2], Q[1]), Q, V, 1, 2);
+  return ok;
+}
+
+// Closed-form TE/TM modes, isotropic/uniaxial eps, isotropic mu
+// -- mod_ModeSolver.jl:243-265
+__device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double kx, double ky, cd *Q, cd *V) {
+  const cd et = L.eps[0], ez = L.eps[8], mr = L.mu[0];
+  const cd wc = mk(w);
+  const double kr = hypot(kx, ky);
+  const double phi = kr == 0.0 ? 0.0 : atan2(ky, kx);
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  const cd kr2 = mk(kr * kr);
+  const cd kzo = csqrt(wc * wc * et * mr - kr2);
+  const cd kze = csqrt(et * (wc * wc * mr - kr2 / ez));
+  const cd A = mk(1.0) / (wc * mr);
+  const cd Bn = mk(1.0) / (wc * et);
+  const cd c = mk(cp), s = mk(sp);
+  V[0] = -s; V[1] = c; V[2] = -(A * kzo * c); V[3] = -(A * kzo * s);
+  V[4] = -s; V[5] = c; V[6] = A * kzo * c; V[7] = A * kzo * s;
+  V[8] = Bn * kze * c; V[9] = Bn * kze * s; V[10] = -s; V[11] = c;
+  V[12] = -(Bn * kze * c); V[13] = -(Bn * kze * s); V[14] = -s; V[15] = c;
+  Q[0] = kzo; Q[1] = -kzo; Q[2] = kze; Q[3] = -kze;
+}
+
+// _modes_layer -- mod_ModeSolver.jl:274-326.  Output columns/entries are in
+// slot order (f1, f2, b1, b2).  Returns false if the eigen solve failed or
+// more than two modes fall on one side.
+__device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double kx, double ky, cd *Vm, cd *Qm) {
+  const double tol = 1.4901161193847656e-08;
+  cd Qs[4], Vs[16];
+  bool ok = true;
+  if ((L.ek == 0 || L.ek == 1) && L.mk == 0) {
+    eigen_closed(L, w, kx, ky, Qs, Vs);
+  } else {
+    cd B[16];
+    berreman(L, w, kx, ky, B);
+    ok = eigen4(B, Qs, Vs);
+  }
+  // class per raw mode: 0 forward, 1 backward, 2 degenerate
+  int cls[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    // Pz = 1/4 Re(V . R4s V) = 1/2 Re(conj(V1) V4 - conj(V2) V3)
+    const double pz = 0.5 * ((Vs[4 * j].r * Vs[4 * j + 3].r + Vs[4 * j].i * Vs[4 * j + 3].i) -
+                             (Vs[4 * j + 1].r * Vs[4 * j + 2].r + Vs[4 * j + 1].i * Vs[4 * j + 2].i));
+    const double iq = Qs[j].i;
+    int c = 2;
+    if (fabs(pz) > tol) c = pz > 0 ? 0 : 1;
+    else if (fabs(iq) > tol) c = iq > tol ? 0 : 1;
+    cls[j] = c;
+  }
+  int nf = 0, nb = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    nf += cls[j] == 0;
+    nb += cls[j] == 1;
+  }
+  ok = ok && nf <= 2 && nb <= 2;
+  int dst[4];
+  int cf = 0, cb = 0, cdg = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int dj;
+    if (cls[j] == 0) dj = cf++;
+    else if (cls[j] == 1) dj = 2 + cb++;
+    else {
+      const int i = cdg++;
+      dj = (nf + i < 2) ? nf + i : 2 + nb + (i - (2 - nf));
+    }
+    dst[j] = dj;
+  }
+#pragma unroll
+  for (int slot = 0; slot < 4; ++slot) {
+    cd q = mk(0.0);
+    cd v0 = mk(0.0), v1 = mk(0.0), v2 = mk(0.0), v3 = mk(0.0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool t = dst[j] == slot;
+      q = sel(t, Qs[j], q);
+      v0 = sel(t, Vs[4 * j], v0);
+      v1 = sel(t, Vs[4 * j + 1], v1);
+      v2 = sel(t, Vs[4 * j + 2], v2);
+      v3 = sel(t, Vs[4 * j + 3], v3);
+    }
+    Qm[slot] = q;
+    Vm[4 * slot] = v0; Vm[4 * slot + 1] = v1; Vm[4 * slot + 2] = v2; Vm[4 * slot + 3] = v3;
+  }
+  return ok;
+}
+
+// _match_modes (eigpenalty 0) of one side (cols c0, c0+1): C = Wa' Vb with
+// W = R4s V.  Returns the swap decision and applies it.
+__device__ __forceinline__ bool match_side(const cd *Vprev, cd *V, cd *Q, int c0) {
+  cd C[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const cd *a = Vprev + 4 * (c0 + i);
+      const cd *b = V + 4 * (c0 + j);
+      // Wa = (a4, -a3, -a2, a1)
+      cd s = conj(a[3]) * b[0];
+      s = s - conj(a[2]) * b[1];
+      s = s - conj(a[1]) * b[2];
+      s = fmac_cj(s, a[0], b[3]);
+      C[i + 2 * j] = s;
+    }
+  const double sid = abs2(C[0]) + abs2(C[3]);
+  const double ssw = abs2(C[2]) + abs2(C[1]);
+  const bool sw = ssw > sid;
+  cswap_col(sw, Q, V, c0, c0 + 1);
+  return sw;
+}
+__device__ __forceinline__ void apply_swaps(bool swf, bool swb, cd *V, cd *Q) {
+  cswap_col(swf, Q, V, 0, 1);
+  cswap_col(swb, Q, V, 2, 3);
+}
+
+// 4x4 LU with partial pivoting, X = A \ B (B 4xN).  Returns false on an
+// exactly zero pivot.
+template <int N>
+__device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double amax = cabs(A[k + 4 * k]);
+    int p = k;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      const double ai = cabs(A[i + 4 * k]);
+      if (ai > amax) { amax = ai; p = i; }
+    }
+    ok = ok && amax != 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      const bool t = p == i;
+#pragma unroll
+      for (int j = k; j < 4; ++j) {
+        const cd a = A[k + 4 * j], b = A[i + 4 * j];
+        A[k + 4 * j] = sel(t, b, a);
+        A[i + 4 * j] = sel(t, a, b);
+      }
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const cd a = B[k + 4 * j], b = B[i + 4 * j];
+        B[k + 4 * j] = sel(t, b, a);
+        B[i + 4 * j] = sel(t, a, b);
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      const cd l = A[i + 4 * k] / A[k + 4 * k];
+#pragma unroll
+      for (int j = k + 1; j < 4; ++j) A[i + 4 * j] = A[i + 4 * j] - l * A[k + 4 * j];
+#pragma unroll
+      for (int j = 0; j < N; ++j) B[i + 4 * j] = B[i + 4 * j] - l * B[k + 4 * j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+      cd s = B[i + 4 * j];
+#pragma unroll
+      for (int k = i + 1; k < 4; ++k) s = s - A[i + 4 * k] * B[k + 4 * j];
+      B[i + 4 * j] = s / A[i + 4 * i];
+    }
+  return ok;
+}
+
+// _S_Interface_Matrix -- mod_ScatteringMatrix.jl:27-47:
+// S = [-Vf_B  Vb_A] \ [-Vf_A  Vb_B]
+__device__ __forceinline__ bool s_interface(const cd *VA, const cd *VB, cd *S) {
+  cd L[16];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    L[i] = -VB[i];
+    L[8 + i] = VA[8 + i];
+    S[i] = -VA[i];
+    S[8 + i] = VB[8 + i];
+  }
+  return lu_solve4<4>(L, S);
+}
+
+// Redheffer star product -- mod_ScatteringMatrix.jl:65-85
+__device__ __forceinline__ void redheffer(const cd *S1, const cd *S2, cd *S) {
+  cd A11[4], A12[4], A21[4], A22[4], B11[4], B12[4], B21[4], B22[4];
+  blk(S1, 0, 0, A11); blk(S1, 0, 1, A12); blk(S1, 1, 0, A21); blk(S1, 1, 1, A22);
+  blk(S2, 0, 0, B11); blk(S2, 0, 1, B12); blk(S2, 1, 0, B21); blk(S2, 1, 1, B22);
+  cd M1[4], M2[4], t[4], u[4], X11[4], X12[4], X21[4], X22[4];
+  mm<2, 2, 2>(A12, B21, t);
+  M1[0] = mk(1.0) - t[0]; M1[1] = -t[1]; M1[2] = -t[2]; M1[3] = mk(1.0) - t[3];
+  mm<2, 2, 2>(B21, A12, t);
+  M2[0] = mk(1.0) - t[0]; M2[1] = -t[1]; M2[2] = -t[2]; M2[3] = mk(1.0) - t[3];
+  solve2<2>(M1, A11, t);
+  mm<2, 2, 2>(B11, t, X11);
+  solve2<2>(M1, A12, t);
+  mm<2, 2, 2>(t, B22, u);
+  mm<2, 2, 2>(B11, u, t);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) X12[i] = B12[i] + t[i];
+  solve2<2>(M2, B21, t);
+  mm<2, 2, 2>(t, A11, u);
+  mm<2, 2, 2>(A22, u, t);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) X21[i] = A21[i] + t[i];
+  solve2<2>(M2, B22, t);
+  mm<2, 2, 2>(A22, t, X22);
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      S[i + 4 * j] = X11[i + 2 * j];
+      S[i + 4 * (j + 2)] = X12[i + 2 * j];
+      S[i + 2 + 4 * j] = X21[i + 2 * j];
+      S[i + 2 + 4 * (j + 2)] = X22[i + 2 * j];
+    }
+}
+// S1 * diag(p)  (S2 a propagation matrix)
+__device__ __forceinline__ void redheffer_rdiag(const cd *S1, const cd *p, cd *S) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      S[i + 4 * j] = p[i] * S1[i + 4 * j];                                // B11 A11
+      S[i + 4 * (j + 2)] = p[i] * (S1[i + 4 * (j + 2)] * p[2 + j]);        // B11 (A12 B22)
+      S[i + 2 + 4 * j] = S1[i + 2 + 4 * j];                               // A21
+      S[i + 2 + 4 * (j + 2)] = S1[i + 2 + 4 * (j + 2)] * p[2 + j];         // A22 B22
+    }
+}
+// diag(p) * S2  (S1 a propagation matrix)
+__device__ __forceinline__ void redheffer_ldiag(const cd *p, const cd *S2, cd *S) {
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      S[i + 4 * j] = S2[i + 4 * j] * p[j];                                // B11 A11
+      S[i + 4 * (j + 2)] = S2[i + 4 * (j + 2)];                           // B12
+      S[i + 2 + 4 * j] = p[2 + i] * (S2[i + 2 + 4 * j] * p[j]);            // A22 (B21 A11)
+      S[i + 2 + 4 * (j + 2)] = p[2 + i] * S2[i + 2 + 4 * (j + 2)];         // A22 B22
+    }
+}
+
+// _cond2x2_eq > 1/tol  (mod_FieldSolver.jl:33-53), tol = 1e-8
+__device__ __forceinline__ bool ill_conditioned(const cd *A) {
+  const double n1 = sqrt(abs2(A[0]) + abs2(A[1]));
+  const double n2 = sqrt(abs2(A[2]) + abs2(A[3]));
+  if (!(n1 > 0 && n2 > 0 && isfinite(n1) && isfinite(n2))) return true;
+  const cd a0 = rmul(1.0 / n1, A[0]), a1 = rmul(1.0 / n1, A[1]);
+  const cd a2 = rmul(1.0 / n2, A[2]), a3 = rmul(1.0 / n2, A[3]);
+  // G = A' A
+  const cd g00 = conj(a0) * a0 + conj(a1) * a1, g11 = conj(a2) * a2 + conj(a3) * a3;
+  const cd g01 = conj(a0) * a2 + conj(a1) * a3, g10 = conj(a2) * a0 + conj(a3) * a1;
+  const double tr = (g00 + g11).r;
+  const double det = (g00 * g11 - g01 * g10).r;
+  const double dd = tr * tr - 4.0 * det;
+  const double disc = sqrt(dd > 0.0 ? dd : 0.0);
+  const double l1 = (tr + disc) / 2.0, l2 = (tr - disc) / 2.0;
+  if (l2 <= 0.0) return true;
+  return sqrt(l1 / l2) > 1e8;
+}
+
+// Source_projection_Berreman_xyz -- mod_SourceProj.jl:34-60.  sif, sib:
+// 2x3 column-major (moments x, y, z).
+__device__ __forceinline__ bool source_projection(const LayerDev &L, double w, double kx, double ky, const cd *V,
+                                                  cd *sif, cd *sib) {
+  const cd etz1 = L.eps[6], etz2 = L.eps[7], ezz = L.eps[8];
+  const cd mio = mk(0.0, -w);
+  const cd den = ezz * mk(w);
+  cd J[12];
+  // column q = 0, 1: tangential dipoles; q = 2: z dipole
+  const cd zr = rmul(0.0, mio);
+  J[0] = mk(kx) * zr / den; J[1] = mk(ky) * zr / den; J[2] = zr - etz2 * zr / ezz; J[3] = -mio + etz1 * zr / ezz;
+  J[4] = mk(kx) * zr / den; J[5] = mk(ky) * zr / den; J[6] = mio - etz2 * zr / ezz; J[7] = -zr + etz1 * zr / ezz;
+  J[8] = mk(kx) * mio / den; J[9] = mk(ky) * mio / den; J[10] = zr - etz2 * mio / ezz; J[11] = -zr + etz1 * mio / ezz;
+  // A = W' V, B = W' J with W = R4s V: W_col = (v4, -v3, -v2, v1)
+  cd A[16], B[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const cd *a = V + 4 * i;
+    const cd w0 = a[3], w1 = -a[2], w2 = -a[1], w3 = a[0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const cd *b = V + 4 * j;
+      cd s = conj(w0) * b[0];
+      s = fmac_cj(s, w1, b[1]);
+      s = fmac_cj(s, w2, b[2]);
+      s = fmac_cj(s, w3, b[3]);
+      A[i + 4 * j] = s;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const cd *b = J + 4 * j;
+      cd s = conj(w0) * b[0];
+      s = fmac_cj(s, w1, b[1]);
+      s = fmac_cj(s, w2, b[2]);
+      s = fmac_cj(s, w3, b[3]);
+      B[i + 4 * j] = s;
+    }
+  }
+  const bool ok = lu_solve4<3>(A, B);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    sif[2 * q] = B[4 * q]; sif[2 * q + 1] = B[4 * q + 1];
+    sib[2 * q] = B[4 * q + 2]; sib[2 * q + 1] = B[4 * q + 3];
+  }
+  return ok;
+}
+
+// mode-field basis of one layer (_mode_field_core, _fill_obs_basis!,
+// mod_CavityTransfer.jl:87-130): out[0..11] = Exf Eyf Ezf Exb Eyb Ezb (2 modes
+// each), out[12..15] = KZf, KZb.  A mode with a non-finite value stays zero.
+__device__ __forceinline__ void obs_basis(const LayerDev &L, double w, double kx, double ky, const cd *V, const cd *Q,
+                                          cd *out) {
+  const cd ezt1 = L.eps[2], ezt2 = L.eps[5], ezz = L.eps[8];
+  const cd iw = mk(1.0) / mk(w);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    cd vals[8];
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+      const cd *v = V + 4 * (2 * dir + m);
+      const cd ez = -((ezt1 * v[0] + ezt2 * v[1] + iw * (mk(kx) * v[3] - mk(ky) * v[2])) / ezz);
+      vals[3 * dir + 0] = v[0];
+      vals[3 * dir + 1] = v[1];
+      vals[3 * dir + 2] = ez;
+      vals[6 + dir] = Q[2 * dir + m];
+    }
+    bool fin = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) fin = fin && isfin(vals[i]);
+    const cd z = mk(0.0);
+    out[0 + m] = sel(fin, vals[0], z);
+    out[2 + m] = sel(fin, vals[1], z);
+    out[4 + m] = sel(fin, vals[2], z);
+    out[6 + m] = sel(fin, vals[3], z);
+    out[8 + m] = sel(fin, vals[4], z);
+    out[10 + m] = sel(fin, vals[5], z);
+    out[12 + m] = sel(fin, vals[6], z);
+    out[14 + m] = sel(fin, vals[7], z);
+  }
+}
+
+// clamped exp(i kz z) -- mod_ClosedFormField.jl:59-69
+__device__ __forceinline__ cd phase(cd kz, double z) {
+  const double a = -kz.i * z, b = kz.r * z;
+  if (!isfinite(a) || !isfinite(b) || a > 700.0 || a < -700.0) return mk(0.0);
+  return cexp(cd{a, b});
+}
+__device__ __forceinline__ cd fin0(cd z) { return isfin(z) ? z : mk(0.0); }
+
+// ---------------------------------------------------------------------------
+// The node kernel
+// ---------------------------------------------------------------------------
+enum { CASE_SAME = 0, CASE_RIGHT = 1, CASE_LEFT = 2 };
+
+template <int CASE>
+__device__ void solve_node(const NodeParams &P, int64_t k) {
+  const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
+  const int N = P.nl, s = P.src, o = P.obs;
+  bool ok = true, sing = false;
+  // running state of the forward pass
+  cd Vp[16], Qp[4];
+  cd acc[16];          // SLeft chain (j <= s), then suffix product for SRight[s]
+  cd accO[16], accO1[16];  // CASE_RIGHT: SRight[o], SRight[o+1]
+  cd RL[4], SLs22[4];  // SLeft[s] blocks (0,1) and (1,1)
+  cd SLo12[4], SLo22[4], SLo1_22[4];  // CASE_LEFT: SLeft[o], SLeft[o+1]
+  cd Sint21[4], Sint22[4];            // Sint[o] blocks (1,0), (1,1) when o != s
+  bool swS_f = false, swS_b = false, swO_f = false, swO_b = false;
+  bool sw_f_cum = false, sw_b_cum = false;  // net swap of the matched order vs raw
+#pragma unroll
+  for (int i = 0; i < 4; ++i) { RL[i] = mk(0.0); SLs22[i] = mk(i == 0 || i == 3 ? 1.0 : 0.0); }
+  for (int j = 0; j < N; ++j) {
+    const LayerDev &L = P.L[j];
+    cd V[16], Q[4];
+    ok = modes_layer(L, w, kx, ky, V, Q) && ok;
+    if (j == 0) {
+      sw_f_cum = false;
+      sw_b_cum = false;
+      // SLeft[0] = I, S0 = I * Sprop[0] = Sprop[0]
+      if (s > 0) {
+        cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = mk(0.0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[5 * i] = p[i];
+      }
+    } else {
+      const bool sf = match_side(Vp, V, Q, 0);
+      const bool sb = match_side(Vp, V, Q, 2);
+      sw_f_cum = sf;
+      sw_b_cum = sb;
+      cd Sint[16];
+      sing = !s_interface(Vp, V, Sint) || sing;
+      if (j <= s) {
+        // SLeft[j] = S0 * Sint[j-1]; S0 = SLeft[j] * Sprop[j]
+        cd SL[16];
+        redheffer(acc, Sint, SL);
+        if (j == s) {
+          blk(SL, 0, 1, RL);
+          blk(SL, 1, 1, SLs22);
+        }
+        if (CASE == CASE_LEFT) {
+          if (j == o) { blk(SL, 0, 1, SLo12); blk(SL, 1, 1, SLo22); }
+          if (j == o + 1) blk(SL, 1, 1, SLo1_22);
+        }
+        if (j < s) {
+          const cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
+          redheffer_rdiag(SL, p, acc);
+        }
+      }
+      if (j - 1 >= s) {
+        // eff_{j-1} = Sprop[j-1] * Sint[j-1]
+        const double dp = P.L[j - 1].d;
+        const cd p[4] = {expi(Qp[0], dp), expi(Qp[1], dp), expi(-Qp[2], dp), expi(-Qp[3], dp)};
+        cd eff[16];
+        redheffer_ldiag(p, Sint, eff);
+        if (j - 1 == s) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) acc[i] = eff[i];
+        } else {
+          cd t[16];
+          redheffer(acc, eff, t);
+#pragma unroll
+          for (int i = 0; i < 16; ++i) acc[i] = t[i];
+        }
+        if (CASE == CASE_RIGHT) {
+          if (j - 1 == o) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) accO[i] = eff[i];
+          } else if (j - 1 > o) {
+            cd t[16];
+            redheffer(accO, eff, t);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) accO[i] = t[i];
+          }
+          if (j - 1 == o + 1) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) accO1[i] = eff[i];
+          } else if (j - 1 > o + 1) {
+            cd t[16];
+            redheffer(accO1, eff, t);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) accO1[i] = t[i];
+          }
+        }
+      }
+      if (CASE != CASE_SAME && j == o + 1) {
+        blk(Sint, 1, 0, Sint21);
+        blk(Sint, 1, 1, Sint22);
+      }
+    }
+    if (j == s) { swS_f = sw_f_cum; swS_b = sw_b_cum; }
+    if (j == o) { swO_f = sw_f_cum; swO_b = sw_b_cum; }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Vp[i] = V[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Qp[i] = Q[i];
+  }
+  // SRight[s] (identity when s == N-1, not reachable for a source layer)
+  if (s == N - 1) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = mk((i % 5 == 0) ? 1.0 : 0.0);
+  }
+
+  // ---- source layer: modes (matched order), seeds ---------------------------
+  const LayerDev &Ls = P.L[s];
+  cd Vs[16], Qs[4];
+  // The forward pass applied, at layer s, only the swap relative to layer s-1's
+  // matched order; replaying the raw solve with the recorded decisions
+  // reproduces the matched order of layer s exactly.
+  modes_layer(Ls, w, kx, ky, Vs, Qs);
+  apply_swaps(swS_f, swS_b, Vs, Qs);
+  cd sif[6], sib[6];
+  sing = !source_projection(Ls, w, kx, ky, Vs, sif, sib) || sing;
+  bool seeds_fin = true;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) seeds_fin = seeds_fin && isfin(sif[i]) && isfin(sib[i]);
+
+  // ---- cavity of the source layer (_R_cavity, _cavity_resummation) ---------
+  const double ds = Ls.d;
+  cd RR[4], RRr11[4], RRr12[4];
+  {
+    const cd ph[4] = {expi(Qs[0], -ds), expi(Qs[1], -ds), expi(Qs[2], ds), expi(Qs[3], ds)};
+    cd RRr[16];
+    redheffer_ldiag(ph, acc, RRr);
+    blk(RRr, 1, 0, RR);
+    blk(RRr, 0, 0, RRr11);
+    blk(RRr, 0, 1, RRr12);
+  }
+  // F+ = (I - PF RL PB RR) \ (E+ + PF RL E-),  F- = E- + PB RR F+   (2x4 each)
+  cd Fp[8], Fm[8];
+  {
+    const cd PF[2] = {expi(Qs[0], ds), expi(Qs[1], ds)};
+    const cd PB[2] = {expi(Qs[2], -ds), expi(Qs[3], -ds)};
+    cd t[4], A[4];
+    // (PF*RL)*PB
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) t[i + 2 * j] = (PF[i] * RL[i + 2 * j]) * PB[j];
+    cd u[4];
+    mm<2, 2, 2>(t, RR, u);
+    A[0] = mk(1.0) - u[0]; A[1] = -u[1]; A[2] = -u[2]; A[3] = mk(1.0) - u[3];
+    cd rhs[8];
+    // E+ = [I 0], E- = [0 I];  RL*E- = [0 RL]
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const cd ep = mk((j == i) ? 1.0 : 0.0);
+        const cd rle = (j >= 2) ? RL[i + 2 * (j - 2)] : mk(0.0);
+        rhs[i + 2 * j] = ep + PF[i] * rle;
+      }
+    solve2<4>(A, rhs, Fp);
+    cd rf[8];
+    mm<2, 2, 4>(RR, Fp, rf);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const cd em = mk((j - 2 == i) ? 1.0 : 0.0);
+        Fm[i + 2 * j] = em + PB[i] * rf[i + 2 * j];
+      }
+  }
+
+  // ---- T^o (_cavity_transfer_columns / _walk_stack) -------------------------
+  cd T[16];
+  cd Vo[16], Qo[4];
+  if (CASE == CASE_SAME) {
+    // [RL F- ; RR F+]
+    cd a[8], b[8];
+    mm<2, 2, 4>(RL, Fm, a);
+    mm<2, 2, 4>(RR, Fp, b);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      T[4 * j] = a[2 * j]; T[4 * j + 1] = a[2 * j + 1];
+      T[4 * j + 2] = b[2 * j]; T[4 * j + 3] = b[2 * j + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Vo[i] = Vs[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Qo[i] = Qs[i];
+  } else {
+    const LayerDev &Lo = P.L[o];
+    modes_layer(Lo, w, kx, ky, Vo, Qo);
+    apply_swaps(swO_f, swO_b, Vo, Qo);
+    cd Ef[8], Ebnext[8];
+    if (CASE == CASE_RIGHT) {
+      cd cp[8];
+      mm<2, 2, 4>(RRr11, Fp, cp);  // c- = 0
+      if (o == N - 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) { Ef[i] = flushd(cp[i]); Ebnext[i] = mk(0.0); }
+      } else {
+        cd S11[4];
+        blk(accO, 0, 0, S11);
+        if (ill_conditioned(S11)) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) Ef[i] = mk(0.0);
+        } else {
+          cd bp[8];
+          solve2<4>(S11, cp, bp);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) Ef[i] = flushd(bp[i]);
+        }
+        if (o + 1 == N - 1) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) Ebnext[i] = mk(0.0);
+        } else {
+          cd S11b[4], S21b[4], bp[8], bm[8];
+          blk(accO1, 0, 0, S11b);
+          blk(accO1, 1, 0, S21b);
+          if (ill_conditioned(S11b)) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) bp[i] = mk(0.0);
+          } else {
+            solve2<4>(S11b, cp, bp);
+          }
+          mm<2, 2, 4>(S21b, bp, bm);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) Ebnext[i] = flushd(bm[i]);
+        }
+      }
+    } else {  // CASE_LEFT
+      cd am[8];
+      mm<2, 2, 4>(SLs22, Fm, am);  // a+ = 0
+      if (o == 0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Ef[i] = mk(0.0);
+      } else {
+        cd bm[8], bp[8];
+        if (ill_conditioned(SLo22)) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) bm[i] = mk(0.0);
+        } else {
+          solve2<4>(SLo22, am, bm);
+        }
+        mm<2, 2, 4>(SLo12, bm, bp);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Ef[i] = flushd(bp[i]);
+      }
+      if (o + 1 == s) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Ebnext[i] = Fm[i];
+      } else {
+        cd bm[8];
+        if (ill_conditioned(SLo1_22)) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) bm[i] = mk(0.0);
+        } else {
+          solve2<4>(SLo1_22, am, bm);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Ebnext[i] = flushd(bm[i]);
+      }
+    }
+    cd Eb[8];
+    if (CASE == CASE_RIGHT && o == N - 1) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Eb[i] = mk(0.0);
+    } else {
+      // re-reference: Eb = flush(S21(Sint_o) (Ef .* phf_o) + S22(Sint_o) Eb_next)
+      const double dob = Lo.d;
+      const cd ph[2] = {expi(Qo[0], dob), expi(Qo[1], dob)};
+      cd far[8], t1[8], t2[8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { far[2 * j] = ph[0] * Ef[2 * j]; far[2 * j + 1] = ph[1] * Ef[2 * j + 1]; }
+      mm<2, 2, 4>(Sint21, far, t1);
+      mm<2, 2, 4>(Sint22, Ebnext, t2);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Eb[i] = flushd(t1[i] + t2[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      T[4 * j] = Ef[2 * j]; T[4 * j + 1] = Ef[2 * j + 1];
+      T[4 * j + 2] = Eb[2 * j]; T[4 * j + 3] = Eb[2 * j + 1];
+    }
+  }
+  bool Tfin = true;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Tfin = Tfin && isfin(T[i]);
+
+  // ---- node validity (mod_CavityTransfer.jl:226-251) ------------------------
+  const bool valid = ok && !sing && seeds_fin;
+  const bool Tvalid = valid && Tfin;
+  cd basis[16];
+  obs_basis(P.L[o], w, kx, ky, Vo, Qo, basis);
+  const cd z0 = mk(0.0);
+  const int64_t nn = P.nn;
+  if (P.flags) P.flags[k] = (unsigned char)((valid ? 1 : 0) | (Tvalid ? 2 : 0) | (sing ? 4 : 0) | (ok ? 0 : 8));
+  if (P.Qsrc) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) P.Qsrc[i * nn + k] = sel(valid, Qs[i], z0);
+  }
+  if (P.sf) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      P.sf[i * nn + k] = sel(valid, sif[i], z0);
+      P.sb[i * nn + k] = sel(valid, -sib[i], z0);
+    }
+  }
+  if (P.T) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) P.T[i * nn + k] = sel(Tvalid, T[i], z0);
+  }
+  if (P.basis) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) P.basis[i * nn + k] = sel(valid, basis[i], z0);
+  }
+
+  // ---- spectral dyadic g~ for each (zsrc, zobs) pair -----------------------
+  if (P.ghat) {
+    const double d_src = Ls.d, d_obs = P.L[o].d;
+    for (int pr = 0; pr < P.npairs; ++pr) {
+      const double zsl = P.zsrc[pr], zob = P.zobs[pr];
+      const double zpos = s == 0 ? d_src - zsl : zsl;
+      cd Af[6], Ab[6], Afd[6], Abd[6];  // [m + 2q]
+#pragma unroll
+      for (int i = 0; i < 6; ++i) { Af[i] = z0; Ab[i] = z0; Afd[i] = z0; Abd[i] = z0; }
+      if (Tvalid) {
+        const cd Pf[2] = {expi(Qs[0], d_src - zpos), expi(Qs[1], d_src - zpos)};
+        const cd Pb[2] = {expi(-Qs[2], zpos), expi(-Qs[3], zpos)};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const cd u[4] = {sif[2 * q] * Pf[0], sif[2 * q + 1] * Pf[1], -(sib[2 * q] * Pb[0]), -(sib[2 * q + 1] * Pb[1])};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            cd v = T[i] * u[0];
+            v = fmac(v, T[i + 4], u[1]);
+            v = fmac(v, T[i + 8], u[2]);
+            v = fmac(v, T[i + 12], u[3]);
+            if (i < 2) Af[i + 2 * q] = v;
+            else Ab[i - 2 + 2 * q] = v;
+          }
+        }
+      }
+      if (CASE == CASE_SAME && valid) {
+        const double zeta = zob - zpos;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          if (zeta > 0) {
+            const cd e = phase(Qs[m], zeta);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Afd[m + 2 * q] = sif[m + 2 * q] * e;
+          } else if (zeta < 0) {
+            const cd e = phase(Qs[2 + m], zeta);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) Abd[m + 2 * q] = -(sib[m + 2 * q] * e);
+          }
+        }
+      }
+      cd g[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) g[i] = z0;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const cd pf = phase(basis[12 + m], zob), pb = phase(basis[14 + m], zob - d_obs);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const cd af = fin0(Af[m + 2 * q] * pf + Afd[m + 2 * q]);
+          const cd ab = fin0(Ab[m + 2 * q] * pb + Abd[m + 2 * q]);
+          g[0 + 3 * q] = g[0 + 3 * q] + (basis[0 + m] * af + basis[6 + m] * ab);
+          g[1 + 3 * q] = g[1 + 3 * q] + (basis[2 + m] * af + basis[8 + m] * ab);
+          g[2 + 3 * q] = g[2 + 3 * q] + (basis[4 + m] * af + basis[10 + m] * ab);
+        }
+      }
+      cd *gp = P.ghat + (int64_t)pr * 9 * nn;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) gp[i * nn + k] = sel(valid, g[i], z0);
+    }
+  }
+}
+
+template <int CASE>
+__global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.nn; k += stride) solve_node<CASE>(P, k);
+}
+
+hipError_t launch_node_kernel(const NodeParams &P, int blocks, hipStream_t stream) {
+  const int threads = 64;
+  if (P.obs == P.src) hipLaunchKernelGGL(node_kernel<CASE_SAME>, dim3(blocks), dim3(threads), 0, stream, P);
+  else if (P.obs > P.src) hipLaunchKernelGGL(node_kernel<CASE_RIGHT>, dim3(blocks), dim3(threads), 0, stream, P);
+  else hipLaunchKernelGGL(node_kernel<CASE_LEFT>, dim3(blocks), dim3(threads), 0, stream, P);
+  return hipGetLastError();
+}
+
+}  // namespace ytf

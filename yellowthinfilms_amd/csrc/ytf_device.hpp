@@ -1,0 +1,103 @@
+// ytf_device.hpp -- FP64 complex arithmetic and fixed-size matrix helpers for
+// the gfx950 node kernels.  Everything is written for full unrolling: all
+// matrix indices are compile-time constants after inlining, so the matrices
+// live in VGPRs; data-dependent choices (pivots, mode slots, swaps) are made
+// with selects, never with dynamic register-array indexing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ytf {
+
+struct cd {
+  double r, i;
+};
+
+__device__ __forceinline__ cd mk(double r, double i = 0.0) { return cd{r, i}; }
+__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cd operator-(cd a) { return cd{-a.r, -a.i}; }
+__device__ __forceinline__ cd operator*(cd a, cd b) {
+  return cd{__builtin_fma(a.r, b.r, -a.i * b.i), __builtin_fma(a.r, b.i, a.i * b.r)};
+}
+__device__ __forceinline__ cd rmul(double s, cd a) { return cd{s * a.r, s * a.i}; }
+__device__ __forceinline__ cd conj(cd a) { return cd{a.r, -a.i}; }
+__device__ __forceinline__ cd times_i(cd a) { return cd{-a.i, a.r}; }
+__device__ __forceinline__ double abs2(cd a) { return __builtin_fma(a.r, a.r, a.i * a.i); }
+__device__ __forceinline__ double cabs(cd a) { return hypot(a.r, a.i); }
+__device__ __forceinline__ double cabs1(cd a) { return fabs(a.r) + fabs(a.i); }
+__device__ __forceinline__ bool isfin(cd a) { return isfinite(a.r) && isfinite(a.i); }
+// a += b*c
+__device__ __forceinline__ cd fmac(cd acc, cd b, cd c) {
+  return cd{__builtin_fma(b.r, c.r, __builtin_fma(-b.i, c.i, acc.r)), __builtin_fma(b.r, c.i, __builtin_fma(b.i, c.r, acc.i))};
+}
+// conj(b)*c + acc
+__device__ __forceinline__ cd fmac_cj(cd acc, cd b, cd c) {
+  return cd{__builtin_fma(b.r, c.r, __builtin_fma(b.i, c.i, acc.r)), __builtin_fma(b.r, c.i, __builtin_fma(-b.i, c.r, acc.i))};
+}
+// Smith's division (robust against overflow of |b|^2)
+__device__ __forceinline__ cd operator/(cd a, cd b) {
+  if (fabs(b.r) >= fabs(b.i)) {
+    double t = b.i / b.r;
+    double den = __builtin_fma(b.i, t, b.r);
+    return cd{__builtin_fma(a.i, t, a.r) / den, __builtin_fma(-a.r, t, a.i) / den};
+  } else {
+    double t = b.r / b.i;
+    double den = __builtin_fma(b.r, t, b.i);
+    return cd{__builtin_fma(a.r, t, a.i) / den, __builtin_fma(a.i, t, -a.r) / den};
+  }
+}
+__device__ __forceinline__ cd cexp(cd z) {
+  double e = exp(z.r);
+  double s, c;
+  sincos(z.i, &s, &c);
+  return cd{e * c, e * s};
+}
+// exp(i*q*d) for real d
+__device__ __forceinline__ cd expi(cd q, double d) { return cexp(cd{-q.i * d, q.r * d}); }
+// principal square root with C99 signed-zero semantics
+__device__ __forceinline__ cd csqrt(cd z) {
+  if (z.r == 0.0 && z.i == 0.0) return cd{0.0, z.i};
+  double t = sqrt((fabs(z.r) + hypot(z.r, z.i)) * 0.5);
+  if (z.r >= 0.0) return cd{t, z.i / (2.0 * t)};
+  return cd{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
+}
+__device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cd{c ? a.r : b.r, c ? a.i : b.i}; }
+// |x| < 1e-250 -> 0 (denormal floor of the stack walk)
+__device__ __forceinline__ cd flushd(cd x) { return cabs(x) < 1e-250 ? cd{0.0, 0.0} : x; }
+
+// ---------------------------------------------------------------------------
+// Column-major fixed-size matrices: A[i + M*j]
+// ---------------------------------------------------------------------------
+template <int M, int K, int N>
+__device__ __forceinline__ void mm(const cd *A, const cd *B, cd *C) {
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+      cd s = A[i] * B[K * j];
+#pragma unroll
+      for (int k = 1; k < K; ++k) s = fmac(s, A[i + M * k], B[k + K * j]);
+      C[i + M * j] = s;
+    }
+}
+// 2x2 sub-block (bi,bj) of a 4x4
+__device__ __forceinline__ void blk(const cd *S, int bi, int bj, cd *B) {
+  B[0] = S[(2 * bi + 0) + 4 * (2 * bj + 0)];
+  B[1] = S[(2 * bi + 1) + 4 * (2 * bj + 0)];
+  B[2] = S[(2 * bi + 0) + 4 * (2 * bj + 1)];
+  B[3] = S[(2 * bi + 1) + 4 * (2 * bj + 1)];
+}
+__device__ __forceinline__ cd det2(const cd *A) { return A[0] * A[3] - A[2] * A[1]; }
+// X = A \ B for 2x2 A, B 2xN (Cramer per column)
+template <int N>
+__device__ __forceinline__ void solve2(const cd *A, const cd *B, cd *X) {
+  cd d = det2(A);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    cd b0 = B[2 * j], b1 = B[2 * j + 1];
+    X[2 * j] = (A[3] * b0 - A[2] * b1) / d;
+    X[2 * j + 1] = (A[0] * b1 - A[1] * b0) / d;
+  }
+}
+
+}  // namespace ytf

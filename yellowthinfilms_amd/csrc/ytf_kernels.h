@@ -1,0 +1,46 @@
+// ytf_kernels.h -- internal structures shared by the HIP kernels and the
+// C-ABI implementation (not part of the public interface).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ytf_device.hpp"
+
+namespace ytf {
+
+// One layer as the kernels read it: eps, mu column-major 3x3 complex.
+struct LayerDev {
+  cd eps[9];
+  cd mu[9];
+  double d;
+  int ek;  // 0 isotropic, 1 uniaxial, 2 biaxial, 3 general 3x3
+  int mk;  // 0 isotropic mu, 1 general
+};
+
+// Arguments of the node kernel.  All pointers are device pointers; outputs
+// are structure-of-arrays, node index fastest: X[e * nn + node].
+struct NodeParams {
+  const LayerDev *L;
+  int nl;
+  double omega;
+  const double *kx, *ky;
+  int64_t nn;
+  int src, obs;  // 0-based
+  int npairs;
+  const double *zsrc, *zobs;  // device, npairs each
+  cd *ghat;                   // [npairs][9][nn] (c + 3q) or null
+  cd *Qsrc;                   // [4][nn]  Qf1 Qf2 Qb1 Qb2
+  cd *sf, *sb;                // [6][nn]  2x3 column-major
+  cd *T;                      // [16][nn] 4x4 column-major
+  cd *basis;                  // [16][nn] Exf Eyf Ezf Exb Eyb Ezb KZf KZb
+  unsigned char *flags;       // [nn] bit0 valid, bit1 Tvalid, bit2 singular solve, bit3 mode solve failed
+};
+
+hipError_t launch_node_kernel(const NodeParams &P, int blocks, hipStream_t stream);
+
+// 2D spectral convolution E = scale * bfft2(g~ . fft2(J)) on a power-of-two
+// nx x ny grid; see fft_kernels.hip for layouts.
+hipError_t launch_spectral_apply(int nx, int ny, const cd *ghat_xy, const cd *J, cd *E, cd *work, const cd *tw_x,
+                                 const cd *tw_y, double scale, hipStream_t stream);
+
+}  // namespace ytf
