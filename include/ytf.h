@@ -66,7 +66,8 @@ int ytf_ctx_create(int device, ytf_ctx **ctx);
 int ytf_ctx_destroy(ytf_ctx *ctx);
 /* Message of the last failed call on ctx (ctx may be NULL for ctx_create). */
 const char *ytf_last_error(const ytf_ctx *ctx);
-/* Use an existing hipStream_t (NULL = the context's own stream). */
+/* Enqueue all later work on this hipStream_t; NULL selects the HIP null
+ * (legacy default) stream.  A new context uses a stream of its own. */
 int ytf_ctx_set_stream(ytf_ctx *ctx, void *hip_stream);
 /* Block until all work enqueued on the context's stream is done. */
 int ytf_ctx_synchronize(ytf_ctx *ctx);

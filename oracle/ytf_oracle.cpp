@@ -552,13 +552,16 @@ template <class R> bool eigen4(Mat<R, 4, 4> A, cx<R> *Q, Mat<R, 4, 4> &V) {
     R s2 = abs2(Vt(0, k));
     for (int i = 1; i < n; ++i) s2 = s2 + abs2(Vt(i, k));
     R inv = R(1.0) / r_sqrt(s2);
-    int im = 0;
-    double best = -1;
+    double best = 0;
     for (int i = 0; i < n; ++i) {
       Vt(i, k) = rmul(inv, Vt(i, k));
-      double a2 = val(abs2(Vt(i, k)));
-      if (a2 > best) { best = a2; im = i; }
+      best = std::max(best, val(abs2(Vt(i, k))));
     }
+    // the largest component (first of any near-tie, so rounding cannot
+    // change the choice) is made real and positive
+    int im = 0;
+    for (int i = 0; i < n; ++i)
+      if (val(abs2(Vt(i, k))) >= best * (1.0 - 1e-10)) { im = i; break; }
     cx<R> p = Vt(im, k);
     R ap = cabs(p);
     cx<R> rot = cx<R>(p.re / ap, -(p.im / ap));

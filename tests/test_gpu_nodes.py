@@ -83,7 +83,12 @@ def test_foundation_stacks_every_layer_pair(label):
 
 def test_c1_stack_radial_nodes_all_obs_layers():
     stack = c1_stack()
-    kr = np.concatenate([np.linspace(0, 6 * W0, 400), [W0 * 0.999, W0 * 1.001, 1.5 * W0]])
+    # near (not on) the light lines of air (k0) and glass (1.5 k0): exactly on a
+    # lossless light line kz is rounding noise and the reference's own mode
+    # sort is rounding-dependent there (its rules keep nodes off, mod_KxGrid.jl:355-359)
+    near = [W0 * (1 + s * e) for s in (-1, 1) for e in (1e-3, 1e-6)]
+    near += [1.5 * W0 * (1 + s * e) for s in (-1, 1) for e in (1e-3, 1e-6)]
+    kr = np.concatenate([np.linspace(0, 6 * W0, 400), near])
     for obs in (1, 2, 3):
         print(f"case: C1 stack obs={obs}", flush=True)
         do = stack[obs - 1].d
@@ -139,7 +144,15 @@ def test_spectral_apply_full_512_grid():
     idx = rng.choice(nx * ny, 1024, replace=False)
     xi, yi = idx // ny, idx % ny
     og = po.ghat_batch(ostack(stack), W0, grid["kx"][xi], grid["ky"][yi], 1, 1, [20 * NM], [40 * NM])[:, 0]
-    assert relerr(g[xi, yi], og) < 1e-10
+    err = relerr(g[xi, yi], og)
+    if err >= 1e-10:
+        d = np.array([np.linalg.norm(g[xi[k], yi[k]] - og[k]) / np.linalg.norm(og[k]) for k in range(len(xi))])
+        for k in np.argsort(-d)[:5]:
+            print("worst node", k, xi[k], yi[k], d[k], np.linalg.norm(og[k]), np.linalg.norm(g[xi[k], yi[k]]),
+                  flush=True)
+        g2 = op.build().ghat[0].cpu().numpy().reshape(3, 3, nx, ny).transpose(2, 3, 1, 0)
+        print("rebuild: rel err", relerr(g2[xi, yi], og), "vs first build", relerr(g2, g), flush=True)
+    assert err < 1e-10
     J = (rng.normal(size=(nx, ny, 3)) + 1j * rng.normal(size=(nx, ny, 3))) / math.sqrt(2)
     Jt = torch.as_tensor(np.ascontiguousarray(J.transpose(2, 1, 0)), device="cuda")
     E = op.apply(Jt).cpu().numpy().transpose(2, 1, 0)

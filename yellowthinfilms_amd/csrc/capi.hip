@@ -120,7 +120,9 @@ const char *ytf_last_error(const ytf_ctx *c) { return c ? c->err.c_str() : g_err
 
 int ytf_ctx_set_stream(ytf_ctx *c, void *s) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_ctx_set_stream: ctx is NULL");
-  c->stream = s ? (hipStream_t)s : c->own;
+  // NULL is the HIP null (legacy default) stream, the one torch's default
+  // current stream wraps; work is then ordered with everything on it
+  c->stream = (hipStream_t)s;
   return YTF_OK;
 }
 

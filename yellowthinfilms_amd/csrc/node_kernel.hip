@@ -293,25 +293,31 @@ __device__ __forceinline__ bool eigen4(cd *A, cd *Q, cd *V) {
       s2 += abs2(col[i]);
     }
     const double inv = 1.0 / sqrt(s2);
-    double best = -1.0;
-    cd p = mk(1.0);
+    // zgeev normalisation: unit 2-norm, the largest component made real and
+    // positive.  Near-ties (|v_i| equal to rounding, e.g. circular
+    // polarisation at normal incidence) take the first such component so the
+    // choice does not depend on rounding.
+    double best = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       col[i] = rmul(inv, col[i]);
-      const double a2 = abs2(col[i]);
-      if (a2 > best) {
-        best = a2;
-        p = col[i];
-      }
+      best = fmax(best, abs2(col[i]));
+    }
+    const double thr = best * (1.0 - 1e-10);
+    int imax = 3;
+    cd p = col[3];
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {
+      const bool t = abs2(col[i]) >= thr;
+      imax = t ? i : imax;
+      p = sel(t, col[i], p);
     }
     const double ap = cabs(p);
     const cd rot = mk(p.r / ap, -(p.i / ap));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const cd v = col[i] * rot;
-      // the largest component is exactly real
-      const bool is_max = (abs2(col[i]) == best);
-      V[i + 4 * k] = cd{v.r, is_max ? 0.0 : v.i};
+      V[i + 4 * k] = cd{v.r, i == imax ? 0.0 : v.i};
     }
     Q[k] = lam;
   }
