@@ -20,11 +20,18 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("YTF_ARCH", "gfx950")
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off (toolchain notes)
+    "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off: some of their forms have given wrong
+    # results on these GPUs without any error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
+
+
+# Optimisation level per source (default -O3).  node_general.hip is built at
+# -O1: its -O3 build gives wrong, run-to-run varying results on gfx950 (see
+# the header of that file); the -O1 build is exact against the oracle.
+OPT = {"node_general.hip": "-O1"}
 
 
 def sources():
@@ -33,7 +40,7 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, OPT.get(os.path.basename(src), "-O3"), *FLAGS, "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{p.stderr}")

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ytf.h"
+#include "ytf.h"
 #include "ytf_kernels.h"
 
 using ytf::cd;
@@ -201,9 +201,22 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   P.basis = cache ? (cd *)cache->basis_dev : nullptr;
   P.flags = cache ? (unsigned char *)cache->flags_dev : nullptr;
   if ((P.sf == nullptr) != (P.sb == nullptr)) return fail(c, YTF_EINVAL, "ytf_green_nodes: sf and sb go together");
+  // closed-form layers (isotropic/uniaxial eps, isotropic mu) and layers
+  // without xz/yz coupling never reach the general eigen-solver
+  P.needs_general = false;
+  for (int l = 0; l < n_layers; ++l) {
+    const ytf_layer &Ly = layers[l];
+    if ((Ly.eps_kind == 0 || Ly.eps_kind == 1) && Ly.mu_kind == 0) continue;
+    const int idx[4] = {2, 5, 6, 7};
+    for (int t = 0; t < 4; ++t)
+      if (Ly.eps[2 * idx[t]] != 0.0 || Ly.eps[2 * idx[t] + 1] != 0.0 || Ly.mu[2 * idx[t]] != 0.0 ||
+          Ly.mu[2 * idx[t] + 1] != 0.0)
+        P.needs_general = true;
+  }
   int64_t blocks = (n_nodes + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
-  hipError_t e = ytf::launch_node_kernel(P, (int)blocks, c->stream);
+  hipError_t e = (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, c->stream)
+                                            : ytf::launch_node_general(P, (int)blocks, c->stream);
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_green_nodes: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }

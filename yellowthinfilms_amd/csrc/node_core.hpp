@@ -1,4 +1,5 @@
-// node_kernel.hip -- the batched Green k-node solve on gfx950, FP64.
+// node_core.hpp -- the batched Green k-node solve on gfx950, FP64 (device
+// code shared by node_top.hip and node_general.hip).
 //
 // One lane solves one in-plane wavevector node (kx, ky) of a planar stack:
 //   Berreman 4x4 modes of every layer (closed form for isotropic/uniaxial
@@ -28,6 +29,7 @@
 //     held in registers through the cascade.
 //   * The Redheffer product with a diagonal propagation matrix is specialised
 //     (its 2x2 inverses are exact identities).
+#pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -331,6 +333,112 @@ __device__ __forceinline__ bool eigen4(cd *A, cd *Q, cd *V) {
   return ok;
 }
 
+// Eigen-decomposition of a block off-diagonal Berreman matrix
+// B = [0 P; R 0] (P = B12, R = B21): the form B takes whenever eps and mu have
+// no xz/yz coupling (isotropic, uniaxial, diagonal biaxial, gyrotropic about z,
+// any in-plane rotation of those).  Then B^2 = diag(P R, R P), so q^2 are the
+// eigenvalues lam of the 2x2 M = P R and, for q = +-sqrt(lam) with M a = lam a,
+// the eigenvector is v = [a; R a / q].  lam2 = det(P) det(R) / lam1 avoids the
+// cancellation of the quadratic formula.  Vectors get the same normalisation
+// and ordering as eigen4 (zgeev style, sorted by (Re, Im)).
+__device__ __forceinline__ void normalise_col(cd *col) {
+  double s2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) s2 += abs2(col[i]);
+  const double inv = 1.0 / sqrt(s2);
+  double best = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    col[i] = rmul(inv, col[i]);
+    best = fmax(best, abs2(col[i]));
+  }
+  const double thr = best * (1.0 - 1e-10);
+  int imax = 3;
+  cd p = col[3];
+#pragma unroll
+  for (int i = 2; i >= 0; --i) {
+    const bool t = abs2(col[i]) >= thr;
+    imax = t ? i : imax;
+    p = sel(t, col[i], p);
+  }
+  const double ap = cabs(p);
+  const cd rot = mk(p.r / ap, -(p.i / ap));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const cd v = col[i] * rot;
+    col[i] = cd{v.r, i == imax ? 0.0 : v.i};
+  }
+}
+
+__device__ __forceinline__ bool is_block_form(const LayerDev &L) {
+  const int idx[4] = {2, 5, 6, 7};  // (3,1) (3,2) (1,3) (2,3), column-major
+  bool z = true;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    z = z && L.eps[idx[t]].r == 0.0 && L.eps[idx[t]].i == 0.0;
+    z = z && L.mu[idx[t]].r == 0.0 && L.mu[idx[t]].i == 0.0;
+  }
+  return z;
+}
+
+__device__ __forceinline__ bool eigen_block(const cd *B, cd *Q, cd *V) {
+  // P = B[0:2, 2:4], R = B[2:4, 0:2] (column-major 4x4)
+  const cd P11 = B[8], P21 = B[9], P12 = B[12], P22 = B[13];
+  const cd R11 = B[2], R21 = B[3], R12 = B[6], R22 = B[7];
+  const cd m11 = P11 * R11 + P12 * R21, m12 = P11 * R12 + P12 * R22;
+  const cd m21 = P21 * R11 + P22 * R21, m22 = P21 * R12 + P22 * R22;
+  const cd half_tr = rmul(0.5, m11 + m22), dd = rmul(0.5, m11 - m22);
+  cd disc = csqrt(dd * dd + m12 * m21);
+  // larger-magnitude root first
+  const cd la = half_tr + disc, lb = half_tr - disc;
+  const cd l1 = sel(abs2(la) >= abs2(lb), la, lb);
+  const cd detM = (P11 * P22 - P12 * P21) * (R11 * R22 - R12 * R21);
+  const cd l2 = (abs2(l1) > 0.0) ? detM / l1 : mk(0.0);
+  const cd lam[2] = {l1, l2};
+  cd a[2][2];
+  const double scale = abs2(m11) + abs2(m12) + abs2(m21) + abs2(m22);
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    // (m12, lam - m11) and (lam - m22, m21) both solve (M - lam) a = 0; take the larger
+    const cd u1a = m12, u1b = lam[e] - m11, u2a = lam[e] - m22, u2b = m21;
+    const double n1 = abs2(u1a) + abs2(u1b), n2 = abs2(u2a) + abs2(u2b);
+    const bool use1 = n1 >= n2;
+    cd x = sel(use1, u1a, u2a), y = sel(use1, u1b, u2b);
+    // M = lam I (degenerate, non-defective): the coordinate vectors
+    const bool flat = fmax(n1, n2) <= 1e-28 * scale;
+    x = sel(flat, mk(e == 0 ? 1.0 : 0.0), x);
+    y = sel(flat, mk(e == 0 ? 0.0 : 1.0), y);
+    a[e][0] = x;
+    a[e][1] = y;
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const cd q = csqrt(lam[e]);
+    const cd Ra0 = R11 * a[e][0] + R12 * a[e][1], Ra1 = R21 * a[e][0] + R22 * a[e][1];
+    const cd iq = cinv(q);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const cd qs = sgn == 0 ? q : -q;
+      const cd iqs = sgn == 0 ? iq : -iq;
+      cd col[4] = {a[e][0], a[e][1], Ra0 * iqs, Ra1 * iqs};
+      normalise_col(col);
+      const int k = 2 * e + sgn;
+      Q[k] = qs;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) V[i + 4 * k] = col[i];
+    }
+  }
+  cswap_col(lessRI(Q[1], Q[0]), Q, V, 0, 1);
+  cswap_col(lessRI(Q[3], Q[2]), Q, V, 2, 3);
+  cswap_col(lessRI(Q[2], Q[0]), Q, V, 0, 2);
+  cswap_col(lessRI(Q[3], Q[1]), Q, V, 1, 3);
+  cswap_col(lessRI(Q[2], Q[1]), Q, V, 1, 2);
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ok = ok && isfin(Q[i]);
+  return ok;
+}
+
 // Closed-form TE/TM modes, isotropic/uniaxial eps, isotropic mu
 // -- mod_ModeSolver.jl:243-265
 __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double kx, double ky, cd *Q, cd *V) {
@@ -356,6 +464,7 @@ __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double
 // _modes_layer -- mod_ModeSolver.jl:274-326.  Output columns/entries are in
 // slot order (f1, f2, b1, b2).  Returns false if the eigen solve failed or
 // more than two modes fall on one side.
+template <bool GENERAL = true>
 __device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double kx, double ky, cd *Vm, cd *Qm) {
   const double tol = 1.4901161193847656e-08;
   cd Qs[4], Vs[16];
@@ -365,7 +474,8 @@ __device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double 
   } else {
     cd B[16];
     berreman(L, w, kx, ky, B);
-    ok = eigen4(B, Qs, Vs);
+    if (!GENERAL || is_block_form(L)) ok = eigen_block(B, Qs, Vs);
+    else ok = eigen4(B, Qs, Vs);
   }
   // class per raw mode: 0 forward, 1 backward, 2 degenerate
   int cls[4];
@@ -478,9 +588,11 @@ __device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
         B[i + 4 * j] = sel(t, a, b);
       }
     }
+    const cd rp = cinv(A[k + 4 * k]);
+    A[k + 4 * k] = rp;  // the back substitution reads the reciprocal pivot
 #pragma unroll
     for (int i = k + 1; i < 4; ++i) {
-      const cd l = A[i + 4 * k] / A[k + 4 * k];
+      const cd l = A[i + 4 * k] * rp;
 #pragma unroll
       for (int j = k + 1; j < 4; ++j) A[i + 4 * j] = A[i + 4 * j] - l * A[k + 4 * j];
 #pragma unroll
@@ -494,7 +606,7 @@ __device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
       cd s = B[i + 4 * j];
 #pragma unroll
       for (int k = i + 1; k < 4; ++k) s = s - A[i + 4 * k] * B[k + 4 * j];
-      B[i + 4 * j] = s / A[i + 4 * i];
+      B[i + 4 * j] = s * A[i + 4 * i];
     }
   return ok;
 }
@@ -683,9 +795,16 @@ __device__ __forceinline__ cd fin0(cd z) { return isfin(z) ? z : mk(0.0); }
 // ---------------------------------------------------------------------------
 // The node kernel
 // ---------------------------------------------------------------------------
-enum { CASE_SAME = 0, CASE_RIGHT = 1, CASE_LEFT = 2 };
+// CASE_TOP: source and observation both in the first layer (the source above
+// the stack).  Then RL = 0 and only the reflection block S21 of SRight[0] is
+// needed; it follows the bottom-up recursion
+//   R_k = eff21 + eff22 ((I - R_{k+1} eff12) \ R_{k+1}) eff11,  R_{N-1} = 0,
+// the S21 block of SRight[k] = eff_k * SRight[k+1] (mod_ScatteringMatrix.jl:65-85,
+// 130-135), with a 2x2 running state and no mode matching (layer 1's order is
+// never changed by _match_modes, and the deeper layers' order cancels in R).
+enum { CASE_SAME = 0, CASE_RIGHT = 1, CASE_LEFT = 2, CASE_TOP = 3 };
 
-template <int CASE>
+template <int CASE, bool GENERAL>
 __device__ void solve_node(const NodeParams &P, int64_t k) {
   const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
@@ -701,10 +820,49 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
   bool sw_f_cum = false, sw_b_cum = false;  // net swap of the matched order vs raw
 #pragma unroll
   for (int i = 0; i < 4; ++i) { RL[i] = mk(0.0); SLs22[i] = mk(i == 0 || i == 3 ? 1.0 : 0.0); }
-  for (int j = 0; j < N; ++j) {
+  if (CASE == CASE_TOP) {
+    cd R[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)};
+    cd Vn[16];
+    {
+      cd Qn[4];
+      ok = modes_layer<GENERAL>(P.L[N - 1], w, kx, ky, Vn, Qn) && ok;
+    }
+    for (int j = N - 2; j >= 0; --j) {
+      const LayerDev &L = P.L[j];
+      cd V[16], Q[4];
+      ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q) && ok;
+      cd Sint[16];
+      sing = !s_interface(V, Vn, Sint) || sing;
+      const cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
+      cd e11[4], e12[4], e21[4], e22[4];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          e11[r + 2 * c] = Sint[r + 4 * c] * p[c];
+          e12[r + 2 * c] = Sint[r + 4 * (c + 2)];
+          e21[r + 2 * c] = p[2 + r] * (Sint[r + 2 + 4 * c] * p[c]);
+          e22[r + 2 * c] = p[2 + r] * Sint[r + 2 + 4 * (c + 2)];
+        }
+      cd t[4], M[4], X[4], Y[4], Z[4];
+      mm<2, 2, 2>(R, e12, t);
+      M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+      solve2<2>(M, R, X);
+      mm<2, 2, 2>(X, e11, Y);
+      mm<2, 2, 2>(e22, Y, Z);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) R[i] = e21[i] + Z[i];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Vn[i] = V[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = mk(0.0);
+    acc[2] = R[0]; acc[3] = R[1]; acc[6] = R[2]; acc[7] = R[3];
+  }
+  for (int j = 0; CASE != CASE_TOP && j < N; ++j) {
     const LayerDev &L = P.L[j];
     cd V[16], Q[4];
-    ok = modes_layer(L, w, kx, ky, V, Q) && ok;
+    ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q) && ok;
     if (j == 0) {
       sw_f_cum = false;
       sw_b_cum = false;
@@ -800,7 +958,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
   // The forward pass applied, at layer s, only the swap relative to layer s-1's
   // matched order; replaying the raw solve with the recorded decisions
   // reproduces the matched order of layer s exactly.
-  modes_layer(Ls, w, kx, ky, Vs, Qs);
+  modes_layer<GENERAL>(Ls, w, kx, ky, Vs, Qs);
   apply_swaps(swS_f, swS_b, Vs, Qs);
   cd sif[6], sib[6];
   sing = !source_projection(Ls, w, kx, ky, Vs, sif, sib) || sing;
@@ -858,7 +1016,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
   // ---- T^o (_cavity_transfer_columns / _walk_stack) -------------------------
   cd T[16];
   cd Vo[16], Qo[4];
-  if (CASE == CASE_SAME) {
+  if (CASE == CASE_SAME || CASE == CASE_TOP) {
     // [RL F- ; RR F+]
     cd a[8], b[8];
     mm<2, 2, 4>(RL, Fm, a);
@@ -874,7 +1032,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
     for (int i = 0; i < 4; ++i) Qo[i] = Qs[i];
   } else {
     const LayerDev &Lo = P.L[o];
-    modes_layer(Lo, w, kx, ky, Vo, Qo);
+    modes_layer<GENERAL>(Lo, w, kx, ky, Vo, Qo);
     apply_swaps(swO_f, swO_b, Vo, Qo);
     cd Ef[8], Ebnext[8];
     if (CASE == CASE_RIGHT) {
@@ -1026,7 +1184,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
           }
         }
       }
-      if (CASE == CASE_SAME && valid) {
+      if ((CASE == CASE_SAME || CASE == CASE_TOP) && valid) {
         const double zeta = zob - zpos;
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
@@ -1063,18 +1221,11 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
   }
 }
 
-template <int CASE>
+template <int CASE, bool GENERAL>
 __global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.nn; k += stride) solve_node<CASE>(P, k);
-}
-
-hipError_t launch_node_kernel(const NodeParams &P, int blocks, hipStream_t stream) {
-  const int threads = 64;
-  if (P.obs == P.src) hipLaunchKernelGGL(node_kernel<CASE_SAME>, dim3(blocks), dim3(threads), 0, stream, P);
-  else if (P.obs > P.src) hipLaunchKernelGGL(node_kernel<CASE_RIGHT>, dim3(blocks), dim3(threads), 0, stream, P);
-  else hipLaunchKernelGGL(node_kernel<CASE_LEFT>, dim3(blocks), dim3(threads), 0, stream, P);
-  return hipGetLastError();
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.nn; k += stride)
+    solve_node<CASE, GENERAL>(P, k);
 }
 
 }  // namespace ytf

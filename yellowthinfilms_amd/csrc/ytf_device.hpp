@@ -34,18 +34,18 @@ __device__ __forceinline__ cd fmac(cd acc, cd b, cd c) {
 __device__ __forceinline__ cd fmac_cj(cd acc, cd b, cd c) {
   return cd{__builtin_fma(b.r, c.r, __builtin_fma(b.i, c.i, acc.r)), __builtin_fma(b.r, c.i, __builtin_fma(-b.i, c.r, acc.i))};
 }
-// Smith's division (robust against overflow of |b|^2)
-__device__ __forceinline__ cd operator/(cd a, cd b) {
-  if (fabs(b.r) >= fabs(b.i)) {
-    double t = b.i / b.r;
-    double den = __builtin_fma(b.i, t, b.r);
-    return cd{__builtin_fma(a.i, t, a.r) / den, __builtin_fma(-a.r, t, a.i) / den};
-  } else {
-    double t = b.r / b.i;
-    double den = __builtin_fma(b.r, t, b.i);
-    return cd{__builtin_fma(a.r, t, a.i) / den, __builtin_fma(a.i, t, -a.r) / den};
-  }
+// 1/b with one real reciprocal: b is first scaled by an exact power of two
+// so |b'|^2 neither overflows nor underflows (b = 0 gives Inf/NaN components
+// like IEEE division does).
+__device__ __forceinline__ cd cinv(cd b) {
+  const double m = fmax(fabs(b.r), fabs(b.i));
+  const int e = (m > 0.0 && isfinite(m)) ? ilogb(m) : 0;
+  const double br = ldexp(b.r, -e), bi = ldexp(b.i, -e);
+  const double rden = 1.0 / __builtin_fma(br, br, bi * bi);
+  const double sc = ldexp(rden, -e);
+  return cd{br * sc, -bi * sc};
 }
+__device__ __forceinline__ cd operator/(cd a, cd b) { return a * cinv(b); }
 __device__ __forceinline__ cd cexp(cd z) {
   double e = exp(z.r);
   double s, c;
@@ -91,12 +91,12 @@ __device__ __forceinline__ cd det2(const cd *A) { return A[0] * A[3] - A[2] * A[
 // X = A \ B for 2x2 A, B 2xN (Cramer per column)
 template <int N>
 __device__ __forceinline__ void solve2(const cd *A, const cd *B, cd *X) {
-  cd d = det2(A);
+  const cd rd = cinv(det2(A));
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     cd b0 = B[2 * j], b1 = B[2 * j + 1];
-    X[2 * j] = (A[3] * b0 - A[2] * b1) / d;
-    X[2 * j + 1] = (A[0] * b1 - A[1] * b0) / d;
+    X[2 * j] = (A[3] * b0 - A[2] * b1) * rd;
+    X[2 * j + 1] = (A[0] * b1 - A[1] * b0) * rd;
   }
 }
 

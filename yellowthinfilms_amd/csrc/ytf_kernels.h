@@ -34,9 +34,11 @@ struct NodeParams {
   cd *T;                      // [16][nn] 4x4 column-major
   cd *basis;                  // [16][nn] Exf Eyf Ezf Exb Eyb Ezb KZf KZb
   unsigned char *flags;       // [nn] bit0 valid, bit1 Tvalid, bit2 singular solve, bit3 mode solve failed
+  bool needs_general;         // some layer needs the general 4x4 eigen-solver
 };
 
-hipError_t launch_node_kernel(const NodeParams &P, int blocks, hipStream_t stream);
+hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream);
+hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream);
 
 // 2D spectral convolution E = scale * bfft2(g~ . fft2(J)) on a power-of-two
 // nx x ny grid; see fft_kernels.hip for layouts.
