@@ -47,6 +47,10 @@ BATCHED_GYRO_512 = dict(
 # separately).  Pinned by tests/test_workloads.py.
 BATCHED_GYRO_512_FLOPS_PER_NODE = 32350.0
 BATCHED_GYRO_512_SPECIAL_PER_NODE = 278.0
+# HBM bytes per launch of the node kernel on batched_gyro_512 (rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a 256 MiB device copy):
+# 6.03 MB read (kx, ky, stack) + 37.75 MB written (g~).
+BATCHED_GYRO_512_NODE_HBM_BYTES = 43.78e6
 
 
 def uniform_c128_source(nx, ny, seed):
