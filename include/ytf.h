@@ -127,6 +127,34 @@ int ytf_green_nodes(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double 
 int ytf_spectral_apply(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
                        double *work_dev, double scale);
 
+/* Precision of the radial synthesis. */
+#define YTF_SYNTH_FP64 0  /* complex FP64 throughout (the reference's arithmetic) */
+
+/* Radial Bessel synthesis: the 3x3 response of every depth pair at distinct
+ * radii, for a pixel on the local x axis.  Replaces the J0/J1/J2 contraction
+ * of _integrate_pixel_1d (mod_ClosedFormField.jl:344-390) over the Bessel row
+ * cache (_build_bessel_row_cache_vol :304-319) with the node weights of
+ * _weighted_mcache (:287-294):
+ *   R_p(rho) = sum_k (i wt_k / 2pi) kx_k sum_n J_n(kx_k rho) M_n[g~_p(kx_k, 0)]
+ * where g~_p is ytf_green_nodes' spectral dyadic of the 1D nodes (ky = 0).
+ *   kx_dev, wt_dev: n_nodes doubles (the CavityTransferCache1D's kx_nodes and
+ *     the quadrature weights);
+ *   ghat_dev: [n_pairs][9][n_nodes] complex (ytf_green_nodes output);
+ *   rho_dev: n_rho radii; R_dev: [n_pairs][9][n_rho] complex, entry c + 3q;
+ *   mode: YTF_SYNTH_*. */
+int ytf_radial_synthesis(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                         const double *ghat_dev, int n_rho, const double *rho_dev, double *R_dev, int mode);
+
+/* closed_form_G for a CavityTransferCache1D (mod_ClosedFormField.jl:435-487):
+ * the radial synthesis at the distinct radii of the pixel grid, rotated to
+ * every pixel's azimuth.  Depth pair p is the z row p.
+ *   xs, ys: host arrays of pixel offsets (nx, ny);
+ *   G_dev: (nx, ny, n_pairs, 3, 3) complex column-major (x, y, z, component,
+ *     moment). */
+int ytf_closed_form_G_1d(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                         const double *ghat_dev, const double *xs, int nx, const double *ys, int ny, double *G_dev,
+                         int mode);
+
 /* Library version string. */
 const char *ytf_version(void);
 

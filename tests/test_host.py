@@ -19,7 +19,7 @@ from yellowthinfilms_amd.materials import (IsotropicMaterial, UniaxialMaterial, 
 def _header_functions():
     src = open(os.path.join(ROOT, "include", "ytf.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(ytf_[a-z_0-9]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(ytf_[A-Za-z_0-9]+)\s*\(", src)))
 
 
 def test_library_loads_and_exports_every_header_symbol():

@@ -11,9 +11,11 @@ from .materials import (IsotropicMaterial, UniaxialMaterial, BiaxialMaterial, Gy
                         FlatOpticalLayer, FlatLayer, stack_arrays)
 from .green import (green_nodes, build_cavity_transfer, uniform_kxky_grid, fft_wavenumbers,
                     SpectralGreenOperator, NodeCache)
+from .synthesis import CavityTransfer1D, closed_form_G, radial_synthesis
+from . import quadrature
 
 __all__ = [
     "IsotropicMaterial", "UniaxialMaterial", "BiaxialMaterial", "GyrotropicMaterial", "FlatOpticalLayer",
     "FlatLayer", "stack_arrays", "green_nodes", "build_cavity_transfer", "uniform_kxky_grid", "fft_wavenumbers",
-    "SpectralGreenOperator", "NodeCache",
+    "SpectralGreenOperator", "NodeCache", "CavityTransfer1D", "closed_form_G", "radial_synthesis", "quadrature",
 ]

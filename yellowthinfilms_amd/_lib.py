@@ -47,7 +47,14 @@ _SIGS = {
                                    ct.POINTER(YtfNodeCache)]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
+    "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
+                                        ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
+    "ytf_closed_form_G_1d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
+                                        ct.POINTER(ct.c_double), ct.c_int, ct.POINTER(ct.c_double), ct.c_int,
+                                        ct.c_void_p, ct.c_int]),
 }
+
+SYNTH_MODES = {"fp64": 0}
 
 EXPORTS = tuple(_SIGS)
 

@@ -31,6 +31,12 @@ struct ytf_ctx {
   std::vector<char> small_host;
   // FFT twiddle tables, one per length
   std::map<int, cd *> tw;
+  // scratch of the synthesis kernels and its host-side inputs (radii, pixel
+  // coordinates, radius map), grown on demand, freed with the context
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  void *aux = nullptr;
+  size_t aux_bytes = 0;
 };
 
 static thread_local std::string g_err;
@@ -63,6 +69,21 @@ static int ensure_small(ytf_ctx *c, size_t bytes) {
   size_t nb = bytes < 65536 ? 65536 : bytes;
   HIPCHK(c, hipMalloc(&c->small, nb));
   c->small_bytes = nb;
+  return YTF_OK;
+}
+
+// Grow a context-owned device buffer to at least `bytes` (contents are not kept).
+static int ensure_buf(ytf_ctx *c, void **buf, size_t *cur, size_t bytes) {
+  if (bytes <= *cur) return YTF_OK;
+  if (*buf) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(*buf));
+    *buf = nullptr;
+    *cur = 0;
+  }
+  size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes;
+  HIPCHK(c, hipMalloc(buf, nb));
+  *cur = nb;
   return YTF_OK;
 }
 
@@ -111,6 +132,8 @@ int ytf_ctx_destroy(ytf_ctx *c) {
   hipStreamSynchronize(c->stream);
   for (auto &kv : c->tw) hipFree(kv.second);
   if (c->small) hipFree(c->small);
+  if (c->work) hipFree(c->work);
+  if (c->aux) hipFree(c->aux);
   if (c->own) hipStreamDestroy(c->own);
   delete c;
   return YTF_OK;
@@ -240,6 +263,99 @@ int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const
   hipError_t e = ytf::launch_spectral_apply(nx, ny, (const cd *)ghat_dev, (const cd *)J_dev, (cd *)E_dev,
                                             (cd *)work_dev, twx, twy, scale, c->stream);
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_spectral_apply: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Radial synthesis into R (device, [np][9][nr] complex); the caller holds the lock.
+static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const double *kx, const double *wt,
+                                   int np, const double *ghat, int nr, const double *rho, cd *R, int mode) {
+  if (mode != YTF_SYNTH_FP64) return fail(c, YTF_EINVAL, "%s: unknown precision mode %d", who, mode);
+  if (n < 0 || np < 1 || nr < 0) return fail(c, YTF_EINVAL, "%s: bad sizes (n_nodes=%lld, n_pairs=%d, n_rho=%d)", who,
+                                             (long long)n, np, nr);
+  if (nr == 0) return YTF_OK;
+  if (n > 0 && (!kx || !wt || !ghat)) return fail(c, YTF_EINVAL, "%s: NULL node buffer", who);
+  const int nks = n > 0 ? ytf::synth_fp64_splits(n, np, nr) : 1;
+  const size_t coef_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)(n > 0 ? n : 1));
+  const size_t part_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)nr * (size_t)nks);
+  int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + part_b);
+  if (rc) return rc;
+  cd *coef = (cd *)c->work;
+  cd *part = (cd *)((char *)c->work + coef_b);
+  hipError_t e = hipSuccess;
+  if (n > 0) {
+    e = ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream);
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "%s: coefficient launch failed: %s", who, hipGetErrorString(e));
+  }
+  // with no nodes the contraction kernel writes zeros
+  e = ytf::launch_synth_fp64(n, np, nr, nks, kx, rho, coef, part, R, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "%s: synthesis launch failed: %s", who, hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_radial_synthesis(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                         const double *ghat_dev, int n_rho, const double *rho_dev, double *R_dev, int mode) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_radial_synthesis: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_rho > 0 && (!rho_dev || !R_dev)) return fail(c, YTF_EINVAL, "ytf_radial_synthesis: NULL rho/R");
+  HIPCHK(c, hipSetDevice(c->device));
+  return radial_synthesis_locked(c, "ytf_radial_synthesis", n_nodes, kx_dev, wt_dev, n_pairs, ghat_dev, n_rho,
+                                 rho_dev, (cd *)R_dev, mode);
+}
+
+int ytf_closed_form_G_1d(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                         const double *ghat_dev, const double *xs, int nx, const double *ys, int ny, double *G_dev,
+                         int mode) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_closed_form_G_1d: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (nx < 0 || ny < 0 || (nx > 0 && !xs) || (ny > 0 && !ys))
+    return fail(c, YTF_EINVAL, "ytf_closed_form_G_1d: bad pixel axes (nx=%d, ny=%d)", nx, ny);
+  const int64_t npix = (int64_t)nx * ny;
+  if (npix == 0) return YTF_OK;
+  if (!G_dev) return fail(c, YTF_EINVAL, "ytf_closed_form_G_1d: G is NULL");
+  if (npix > (int64_t)1 << 31) return fail(c, YTF_EINVAL, "ytf_closed_form_G_1d: too many pixels");
+  HIPCHK(c, hipSetDevice(c->device));
+  // distinct radii in first-appearance order over the column-major pixel grid
+  std::vector<double> rho;
+  std::vector<int> rmap((size_t)npix);
+  std::map<double, int> seen;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double r = std::hypot(xs[i], ys[j]);
+      auto it = seen.find(r);
+      int idx;
+      if (it == seen.end()) {
+        idx = (int)rho.size();
+        seen.emplace(r, idx);
+        rho.push_back(r);
+      } else {
+        idx = it->second;
+      }
+      rmap[i + (size_t)nx * j] = idx;
+    }
+  const int nr = (int)rho.size();
+  const size_t rb = align256(sizeof(double) * nr), xb = align256(sizeof(double) * nx),
+               yb = align256(sizeof(double) * ny), mb = align256(sizeof(int) * (size_t)npix),
+               Rb = align256(sizeof(cd) * 9 * (size_t)n_pairs * nr);
+  // the previous call's kernels may still read aux: wait before refilling it
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, rb + xb + yb + mb + Rb);
+  if (rc) return rc;
+  char *a = (char *)c->aux;
+  double *rho_d = (double *)a, *xs_d = (double *)(a + rb), *ys_d = (double *)(a + rb + xb);
+  int *rmap_d = (int *)(a + rb + xb + yb);
+  cd *R = (cd *)(a + rb + xb + yb + mb);
+  HIPCHK(c, hipMemcpyAsync(rho_d, rho.data(), sizeof(double) * nr, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(xs_d, xs, sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ys_d, ys, sizeof(double) * ny, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rmap_d, rmap.data(), sizeof(int) * npix, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  rc = radial_synthesis_locked(c, "ytf_closed_form_G_1d", n_nodes, kx_dev, wt_dev, n_pairs, ghat_dev, nr, rho_d, R,
+                               mode);
+  if (rc) return rc;
+  hipError_t e = ytf::launch_synth_rotate(nx, ny, n_pairs, nr, xs_d, ys_d, rmap_d, R, (cd *)G_dev, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_closed_form_G_1d: rotation launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
 

@@ -45,4 +45,14 @@ hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stre
 hipError_t launch_spectral_apply(int nx, int ny, const cd *ghat_xy, const cd *J, cd *E, cd *work, const cd *tw_x,
                                  const cd *tw_y, double scale, hipStream_t stream);
 
+// Radial Bessel synthesis (synth.hip): node coefficients from g~, the FP64
+// contraction over node chunks + fixed-order reduction, the pixel rotation.
+hipError_t launch_synth_coef(int64_t n, int np, const double *kx, const double *wt, const cd *g, cd *coef,
+                             hipStream_t s);
+int synth_fp64_splits(int64_t n, int np, int nr);
+hipError_t launch_synth_fp64(int64_t n, int np, int nr, int nks, const double *kx, const double *rho,
+                             const cd *coef, cd *part, cd *R, hipStream_t s);
+hipError_t launch_synth_rotate(int nx, int ny, int np, int nr, const double *xs, const double *ys, const int *rmap,
+                               const cd *R, cd *G, hipStream_t s);
+
 }  // namespace ytf

@@ -8,6 +8,12 @@ batched_gyro_512: a 5-layer stack with a 40 nm gyrotropic film
   seed, basis, spectral dyadic) per grid node, then the FFT convolution of a
   512 x 512 x 3 complex N(0,1) source.  Source: layer 1 (air), 20 nm above the
   film; observation: layer 1, 40 nm above the film.
+
+point_uniaxial_1d: air / 50 nm uniaxial film eps = diag(2.5, 2.5, 3.0) + 0.01i /
+  glass 2.25; a dipole 10 nm above the film; 4096 Gauss-Legendre k-rho nodes on
+  [0, 6 k0] in panels broken at the light lines (u^2 substitution at the
+  lossless ones) plus a 512-node cosh tail; the J0/J1/J2 synthesis at 256
+  radii in (0, 2] and 8 observation heights in the air, complex FP64.
 """
 import math
 
@@ -26,6 +32,46 @@ def gyro_512_stack():
             FlatOpticalLayer(GyrotropicMaterial(gy), 40 * NM),
             FlatOpticalLayer(IsotropicMaterial(4.0 + 0.02j), 80 * NM),
             FlatOpticalLayer(IsotropicMaterial(2.25 + 0.02j), 1.0)]
+
+
+def point_uniaxial_stack():
+    return [FlatOpticalLayer(IsotropicMaterial(1.0), 1.0),
+            FlatOpticalLayer(UniaxialMaterial(2.5 + 0.01j, 3.0 + 0.01j), 50 * NM),
+            FlatOpticalLayer(IsotropicMaterial(2.25), 1.0)]
+
+
+POINT_UNIAXIAL_1D = dict(
+    name="point_uniaxial_1d",
+    description=("air / 50 nm uniaxial film (eps diag(2.5,2.5,3.0)+0.01i) / glass 2.25, lambda=600 nm, dipole 10 nm "
+                 "above the film; 4096 GL k-rho nodes on [0, 6k0] broken at the light lines + 512-node cosh tail; "
+                 "J0/J1/J2 synthesis at 256 radii x 8 heights; complex FP64"),
+    source_layer=1, zsrc=10 * NM,
+    obs_layer=1, heights_nm=(20, 30, 40, 60, 80, 120, 160, 240),
+    n_band=4096, order=16, kmax_k0=6.0, n_tail=512, n_rho=256, rho_max=2.0,
+)
+
+
+def point_uniaxial_rule(wl=POINT_UNIAXIAL_1D):
+    """(kx, wt) of point_uniaxial_1d: the band [0, 6 k0] and the cosh tail
+    (mod_KxGrid.jl:126-145, decay length = the smallest source-observation
+    separation)."""
+    from . import quadrature as Q
+    k0 = OMEGA
+    kmax = wl["kmax_k0"] * k0
+    lossless = [k0, 1.5 * k0]                       # air, glass
+    breaks = [0.0, kmax, k0, math.sqrt(2.5) * k0, 1.5 * k0]
+    kb, wb = Q.panel_rule(breaks, wl["n_band"], order=wl["order"], singular=lossless)
+    Leff = (min(wl["heights_nm"]) - wl["zsrc"] / NM) * NM
+    umax = Q.evanescent_umax(kmax, Leff)
+    kt, wt = Q.evanescent_band_nodes(kmax, umax, wl["n_tail"])
+    return np.concatenate([kb, kt]), np.concatenate([wb, wt])
+
+
+def point_uniaxial_geometry(wl=POINT_UNIAXIAL_1D):
+    """(zobs local depths in layer 1, radii) of point_uniaxial_1d."""
+    zobs = np.array([1.0 - h * NM for h in wl["heights_nm"]])
+    rho = wl["rho_max"] * np.arange(1, wl["n_rho"] + 1) / wl["n_rho"]
+    return zobs, rho
 
 
 BATCHED_GYRO_512 = dict(
