@@ -1,19 +1,25 @@
-"""Benchmark of the batched plain GPU Green operator on one MI355X.
+"""Benchmarks of the GPU Green-operator path on one MI355X.
 
-Workload (yellowthinfilms_amd.workloads.BATCHED_GYRO_512): a 5-layer stack
-with a 40 nm gyrotropic film on the uniform 512 x 512 (kx, ky) grid.  One
-step = one Green k-node per grid node (262,144 node solves -> spectral
-dyadic g~, the ytf_green_nodes kernel) + one FFT convolution of a
-512 x 512 x 3 complex N(0,1) source (ytf_spectral_apply).  Inputs are
-resident in HBM before the timed region.
+    python bench.py [--workload W] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+Workloads (yellowthinfilms_amd.workloads):
+  batched_gyro_512 (default, the headline line): a 5-layer stack with a 40 nm
+    gyrotropic film on the uniform 512 x 512 (kx, ky) grid.  One step = one
+    Green k-node per grid node (262,144 node solves -> spectral dyadic g~,
+    ytf_green_nodes) + one FFT convolution of a 512 x 512 x 3 complex N(0,1)
+    source (ytf_spectral_apply).  value = Green k-nodes/s of the whole step.
+  point_uniaxial_1d: air / uniaxial film / glass, 4608 k-rho nodes, the 1D
+    closed-form G at 256 radii x 8 heights (node solve + FP64 Bessel
+    synthesis).  value = Green k-nodes/s of the whole step.
+  bessel_gemm: 16384 k-rho nodes x 8192 radii x 64 depth pairs; node solve
+    (FP64) + the Bessel contraction on the matrix cores (split fp16 products).
+    value = contraction TFLOP/s (algorithmic) of the whole step.
 
-Prints one JSON line: value = Green k-nodes/s of the whole step; the apply
-rate, a roofline record of the dominant kernel (the node solve, FP64) measured
-with HIP events on the stream the kernels run on, and the CPU baseline (the
-oracle timed on this host's cores over the same grid, which also gives the
-relative error of the GPU's g~ against it on every node).
+Inputs are resident in HBM before the timed region.  Each line carries a
+roofline record of the dominant kernel, timed live with HIP events on the
+stream it is launched on (the library's kernel timing), and the CPU baseline
+(the FP64 oracle on this host's cores), whose leg also gives the relative
+error of the GPU result against the oracle.
 """
 import argparse
 import json
@@ -27,28 +33,51 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-PEAK_FP64 = 78.6e12   # MI355X FP64 (vector and matrix), FLOP/s
-PEAK_HBM = 8.0e12     # bytes/s
+PEAK_FP64 = 78.6e12    # MI355X FP64 (vector and matrix), FLOP/s
+PEAK_FP16 = 2.5e15     # MI355X dense fp16/bf16 matrix, FLOP/s
+PEAK_HBM = 8.0e12      # bytes/s
+METRIC = "Green k-nodes/s + operator applies/s, surrogate vs plain GPU, <=1e-6 rel err"
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-repeats", type=int, default=3)
-    ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="after the timed region, one device-to-device copy of this many MiB (a kernel of known "
-                         "traffic, to scale the FETCH_SIZE/WRITE_SIZE counters of a profiling run)")
-    a = ap.parse_args()
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (the multi-GPU sharding of the node solve "
-              "is design-only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def _cores():
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
 
+
+def _timed_steps(a, step, torch):
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _calib_copy(a, torch, dev):
+    # one device-to-device copy of known size: the scale of the FETCH_SIZE /
+    # WRITE_SIZE counters in a profiling run
+    if a.calib_copy_mb > 0:
+        src = torch.empty(a.calib_copy_mb * 2 ** 20 // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+
+def _base(a, wl, value, unit, wall, dtype):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": wall * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic (seeded inputs; stack, rule and grid as configured)",
+            "config": {"workload": wl["description"], "name": wl["name"]}}
+
+
+# ---------------------------------------------------------------------------
+def bench_gyro(a):
     import torch
     from yellowthinfilms_amd import uniform_kxky_grid, SpectralGreenOperator, workloads as W
+    from yellowthinfilms_amd._lib import Context
 
     wl = W.BATCHED_GYRO_512
     grid = uniform_kxky_grid(wl["L"], wl["dx"])
@@ -59,79 +88,46 @@ def main():
     op = SpectralGreenOperator(stack, W.OMEGA, grid, wl["source_layer"], wl["zsrc"], wl["obs_layer"], wl["zobs"])
     J = torch.as_tensor(W.uniform_c128_source(nx, ny, wl["seed"]), device=dev)
     E = torch.empty_like(J)
-    stream = torch.cuda.current_stream(dev)  # the ytf context is bound to this stream
+    ctx = Context.get(0)
+
+    def step():
+        op.build()
+        op.apply(J, out=E)
 
     for _ in range(a.warmup):
-        op.build()
-        op.apply(J, out=E)
-    torch.cuda.synchronize()
-
-    ev = lambda: torch.cuda.Event(enable_timing=True)
-    t_node, t_apply = [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        e0, e1, e2 = ev(), ev(), ev()
-        e0.record(stream)
-        op.build()
-        e1.record(stream)
-        op.apply(J, out=E)
-        e2.record(stream)
-        t_node.append((e0, e1))
-        t_apply.append((e1, e2))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    node_ms = float(np.mean([s.elapsed_time(e) for s, e in t_node]))
-    apply_ms = float(np.mean([s.elapsed_time(e) for s, e in t_apply]))
-    ms_per_step = wall * 1e3 / a.steps
-
-    if a.calib_copy_mb > 0:
-        src = torch.empty(a.calib_copy_mb * 2 ** 20 // 8, dtype=torch.float64, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    node_ms, nlaunch = ctx.kernel_time("node_solve")
+    apply_ms, _ = ctx.kernel_time("spectral_apply")
+    ctx.set_timing(False)
+    node_ms /= nlaunch
+    apply_ms /= a.steps
+    _calib_copy(a, torch, dev)
 
     flops = W.BATCHED_GYRO_512_FLOPS_PER_NODE * nn
     achieved = flops / (node_ms * 1e-3)
     cpu, rel_err = None, None
     if not a.no_cpu_baseline:
-        g = op.ghat[0].cpu().numpy()  # [9][node], node = y + ny*x
-        cpu, rel_err = cpu_baseline(stack, grid, wl, a.cpu_repeats, g)
-
+        cpu, rel_err = cpu_gyro(stack, grid, wl, a.cpu_repeats, op.ghat[0].cpu().numpy())
     apply_bytes = 16.0 * (3 * nn + 9 * nn + 3 * nn)
-    out = {
-        "metric": "Green k-nodes/s + operator applies/s, surrogate vs plain GPU, <=1e-6 rel err",
-        "value": nn * a.steps / wall,
-        "unit": "Green k-nodes/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded N(0,1) complex source; stack and grid as configured)",
-        "config": {"workload": wl["description"], "nodes_per_step": nn, "grid": [nx, ny],
-                   "path": "plain GPU (ytf_green_nodes + ytf_spectral_apply)"},
-        "node_solve_ms": node_ms,
-        "apply_ms": apply_ms,
-        "applies_per_s": 1e3 / apply_ms,
+    out = _base(a, wl, nn * a.steps / wall, "Green k-nodes/s", wall, "f64")
+    out["config"].update({"nodes_per_step": nn, "grid": [nx, ny], "path": "plain GPU (ytf_green_nodes + ytf_spectral_apply)"})
+    out.update({
+        "node_solve_ms": node_ms, "apply_ms": apply_ms, "applies_per_s": 1e3 / apply_ms,
         "node_solve_knodes_per_s": nn / (node_ms * 1e-3),
         "apply_hbm_gbps_algorithmic": apply_bytes / (apply_ms * 1e-3) / 1e9,
         "rel_err_vs_fp64_oracle": rel_err,
-        "roofline": {"kernel": "ytf node_kernel (Green k-node solve, FP64 vector ALU)", "bound": "mfma",
+        "roofline": {"kernel": "node_solve (ytf node_kernel, Green k-node solve, FP64)", "bound": "mfma",
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "traffic": W.BATCHED_GYRO_512_NODE_HBM_BYTES,
                      "flops_per_node": W.BATCHED_GYRO_512_FLOPS_PER_NODE, "nodes_per_launch": nn},
         "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    })
+    return out
 
 
-def cpu_baseline(stack, grid, wl, repeats, gpu_ghat):
+def cpu_gyro(stack, grid, wl, repeats, gpu_ghat):
     """The FP64 oracle (C++/OpenMP restatement of the reference node loop and
     strength) over the same 262,144 grid nodes on this host's cores: median of
     `repeats` full passes after one warm-up pass on 4096 nodes.  Also returns
@@ -143,7 +139,7 @@ def cpu_baseline(stack, grid, wl, repeats, gpu_ghat):
                  "sample": f"oracle unavailable: {exc}"}, None)
     from yellowthinfilms_amd.materials import stack_arrays
     from yellowthinfilms_amd import workloads as W
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = _cores()
     nx, ny = grid["Nx"], grid["Ny"]
     KX = np.repeat(grid["kx"], ny)
     KY = np.tile(grid["ky"], nx)
@@ -162,6 +158,189 @@ def cpu_baseline(stack, grid, wl, repeats, gpu_ghat):
     return ({"value": nx * ny / dt, "unit": "Green k-nodes/s", "cores": cores, "kind": "port",
              "sample": f"all {nx * ny} nodes of the same grid (node solve + spectral dyadic), median of {repeats}"},
             rel_err)
+
+
+# ---------------------------------------------------------------------------
+def bench_point(a):
+    import torch
+    from yellowthinfilms_amd import CavityTransfer1D, closed_form_G, workloads as W
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.POINT_UNIAXIAL_1D
+    kx, wt = W.point_uniaxial_rule()
+    zo, rho = W.point_uniaxial_geometry()
+    stack = W.point_uniaxial_stack()
+    cache = CavityTransfer1D(stack, W.OMEGA, kx, wt, wl["source_layer"])
+    res = {}
+
+    def step():
+        res["G"] = closed_form_G(cache, wl["obs_layer"], wl["zsrc"], zo, rho, [0.0], mode="fp64")
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    t = {k: ctx.kernel_time(k) for k in ("node_solve", "synth_coef", "synth_fp64", "synth_reduce", "synth_rotate")}
+    ctx.set_timing(False)
+    n, nr, nz = len(kx), len(rho), len(zo)
+    syn_ms = t["synth_fp64"][0] / t["synth_fp64"][1]
+    flops = 36.0 * n * nr * nz  # 9 complex coefficients x real J per (node, radius, depth pair)
+    cpu, rel_err = None, None
+    if not a.no_cpu_baseline:
+        G = res["G"].cpu().numpy().transpose(4, 3, 2, 1, 0)
+        cpu, rel_err = cpu_point(stack, kx, wt, wl, zo, rho, G, a.cpu_repeats)
+    out = _base(a, wl, n * a.steps / wall, "Green k-nodes/s", wall, "f64")
+    out["config"].update({"nodes": n, "radii": nr, "heights": nz, "path": "ytf_green_nodes + ytf_closed_form_G_1d"})
+    out.update({
+        "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
+        "rel_err_vs_fp64_oracle": rel_err,
+        "roofline": {"kernel": "synth_fp64 (radial Bessel contraction, FP64)", "bound": "mfma",
+                     "achieved": flops / (syn_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / (syn_ms * 1e-3) / PEAK_FP64, "traffic": None,
+                     "flops_per_launch": flops, "bessel_evals_per_launch": 2.0 * n * nr},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def cpu_point(stack, kx, wt, wl, zo, rho, G, repeats):
+    """The FP64 oracle over the whole workload (node solve + 1D synthesis),
+    median of `repeats`; also the relative error of the GPU's G against it."""
+    try:
+        from oracle import pyoracle as po
+    except Exception as exc:
+        return ({"value": None, "unit": "Green k-nodes/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    from yellowthinfilms_amd.materials import stack_arrays
+    from yellowthinfilms_amd import workloads as W
+    cores = _cores()
+    st = po.Stack(*stack_arrays(stack))
+    ts, Go = [], None
+    for _ in range(max(1, repeats)):
+        t = time.perf_counter()
+        c = po.nodes_batch(st, W.OMEGA, kx, np.zeros_like(kx), wl["source_layer"], [wl["obs_layer"]], nthreads=cores)
+        Go = po.closed_form_G_1d(c, wl["obs_layer"], wt, wl["zsrc"], zo, rho, [0.0], nthreads=cores)
+        ts.append(time.perf_counter() - t)
+    dt = float(np.median(ts))
+    rel = float(np.linalg.norm(G - Go) / np.linalg.norm(Go))
+    return ({"value": len(kx) / dt, "unit": "Green k-nodes/s", "cores": cores, "kind": "port",
+             "sample": "the whole workload (node solve + 1D synthesis at all radii and heights)"}, rel)
+
+
+# ---------------------------------------------------------------------------
+def bench_gemm(a):
+    import torch
+    from yellowthinfilms_amd import green_nodes, workloads as W
+    from yellowthinfilms_amd.synthesis import CavityTransfer1D, radial_synthesis
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.BESSEL_GEMM
+    kx, wt = W.bessel_gemm_rule()
+    zs, zo, rho = W.bessel_gemm_geometry()
+    stack = W.point_uniaxial_stack()
+    cache = CavityTransfer1D(stack, W.OMEGA, kx, wt, wl["source_layer"])
+    n, nr, npairs = len(kx), len(rho), len(zs)
+    res = {}
+    g = torch.empty((npairs, 9, n), dtype=torch.complex128, device="cuda")
+
+    def step():
+        green_nodes(stack, W.OMEGA, cache.kx, cache.ky, 1, wl["obs_layer"], zs, zo, cache=False, ghat_out=g)
+        res["R"] = radial_synthesis(cache, g, rho, mode="f16x3")
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    names = ("node_solve", "synth_coef", "bessel_table", "split_columns", "bessel_gemm", "mfma_combine")
+    t = {k: ctx.kernel_time(k) for k in names}
+    ctx.set_timing(False)
+    gemm_ms = t["bessel_gemm"][0] / t["bessel_gemm"][1]
+    alg = 4.0 * nr * n * 9 * npairs          # real J x complex coefficient, 9 complex columns per pair
+    issued = 3 * 2.0 * nr * n * 18 * npairs  # three fp16 products per term on the real GEMM
+    # accuracy against the FP64 contraction of the same g~ (GPU path pinned to the oracle)
+    R64 = radial_synthesis(cache, g, rho, mode="fp64")
+    rel_fp64 = float(torch.linalg.norm(res["R"] - R64) / torch.linalg.norm(R64))
+    cpu, rel_err = None, None
+    if not a.no_cpu_baseline:
+        cpu, rel_err = cpu_gemm(stack, kx, wt, wl, zs, zo, rho, res["R"].cpu().numpy())
+    out = _base(a, wl, alg * a.steps / wall / 1e12, "TFLOP/s (algorithmic contraction, whole step)", wall,
+                "f16x3 (split fp16, fp32 accumulate) + f64")
+    out["config"].update({"nodes": n, "radii": nr, "depth_pairs": npairs,
+                          "path": "ytf_green_nodes (FP64) + ytf_radial_synthesis(YTF_SYNTH_F16X3)"})
+    out.update({
+        "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
+        "rel_err_vs_gpu_fp64": rel_fp64,
+        "rel_err_vs_fp64_oracle": rel_err,
+        "roofline": {"kernel": "bessel_gemm (split-fp16 MFMA GEMM)", "bound": "mfma",
+                     "achieved": alg / (gemm_ms * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
+                     "frac": alg / (gemm_ms * 1e-3) / PEAK_FP16, "traffic": None,
+                     "flops_per_launch": alg, "mfma_flops_issued_per_launch": issued,
+                     "mfma_issued_tflops": issued / (gemm_ms * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def cpu_gemm(stack, kx, wt, wl, zs, zo, rho, R):
+    """The oracle's 1D synthesis (FP64, OpenMP over radii) on a bounded sample:
+    every 32nd radius and the 8 observation heights of one source height."""
+    try:
+        from oracle import pyoracle as po
+    except Exception as exc:
+        return ({"value": None, "unit": "synthesis terms/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    from yellowthinfilms_amd.materials import stack_arrays
+    from yellowthinfilms_amd import workloads as W
+    cores = _cores()
+    st = po.Stack(*stack_arrays(stack))
+    nobs = len(wl["obs_heights_nm"])
+    sel = np.arange(0, len(rho), 32)
+    src_i = 3  # source height index
+    pairs = np.arange(src_i * nobs, (src_i + 1) * nobs)
+    c = po.nodes_batch(st, W.OMEGA, kx, np.zeros_like(kx), wl["source_layer"], [wl["obs_layer"]], nthreads=cores)
+    t = time.perf_counter()
+    Go = po.closed_form_G_1d(c, wl["obs_layer"], wt, zs[pairs[0]], zo[pairs], rho[sel], [0.0], nthreads=cores)
+    dt = time.perf_counter() - t
+    Go = Go[:, 0]  # (nr_s, nz, 3, 3)
+    Rg = R[pairs][:, :, sel].reshape(nobs, 3, 3, len(sel)).transpose(3, 0, 2, 1)  # (nr_s, nz, c, q)
+    rel = float(np.linalg.norm(Rg - Go) / np.linalg.norm(Go))
+    terms = len(sel) * nobs * len(kx)
+    return ({"value": terms / dt, "unit": "synthesis terms (radius x node x depth pair)/s", "cores": cores,
+             "kind": "port", "sample": f"{len(sel)} radii x {nobs} depth pairs x {len(kx)} nodes (one source height)"},
+            rel)
+
+
+# ---------------------------------------------------------------------------
+def dict_ns(a, **kw):
+    b = argparse.Namespace(**vars(a))
+    for k, v in kw.items():
+        setattr(b, k, v)
+    return b
+
+
+WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="batched_gyro_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="after the timed region, one device-to-device copy of this many MiB (a kernel of known "
+                         "traffic, to scale the FETCH_SIZE/WRITE_SIZE counters of a profiling run)")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the multi-GPU sharding of the node solve "
+              "is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    out = WORKLOADS[a.workload](a)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -71,6 +71,16 @@ const char *ytf_last_error(const ytf_ctx *ctx);
 int ytf_ctx_set_stream(ytf_ctx *ctx, void *hip_stream);
 /* Block until all work enqueued on the context's stream is done. */
 int ytf_ctx_synchronize(ytf_ctx *ctx);
+/* Kernel timing (instrumentation; the reference times whole calls with
+ * @elapsed, Bench_GreenOperator.jl:188-197).  While on, each kernel launch
+ * of the context is bracketed by HIP events on the context's stream.
+ * set_timing clears the totals.  kernel_time synchronizes the stream and
+ * returns the summed time (ms) and launch count of one kernel name:
+ * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
+ * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
+ * "mfma_combine". */
+int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
+int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
 /* Device cache outputs of the node solve, structure-of-arrays with the node
  * index fastest (X[e * n_nodes + node], complex).  Any pointer may be NULL.
@@ -129,6 +139,11 @@ int ytf_spectral_apply(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, con
 
 /* Precision of the radial synthesis. */
 #define YTF_SYNTH_FP64 0  /* complex FP64 throughout (the reference's arithmetic) */
+/* Bessel values (FP64-evaluated) and node coefficients (power-of-two column
+ * scaled) each split into two fp16 numbers; three matrix-core products per
+ * term (hi*hi + hi*lo + lo*hi), FP32 accumulation, FP64 combination.
+ * Relative error of order 1e-6 of the FP64 result (reported by the tests). */
+#define YTF_SYNTH_F16X3 1
 
 /* Radial Bessel synthesis: the 3x3 response of every depth pair at distinct
  * radii, for a pixel on the local x axis.  Replaces the J0/J1/J2 contraction

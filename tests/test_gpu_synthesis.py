@@ -109,3 +109,122 @@ def test_no_nodes_and_bad_mode():
     cache2 = CavityTransfer1D(stack, W0, [1.0, 2.0], [0.5, 0.5], 1)
     with pytest.raises(YtfError):
         closed_form_G(cache2, 1, 2.0, [0.5], [0.1], [0.0])  # zsrc outside layer 1
+
+
+# ---------------------------------------------------------------------------
+# Matrix-core path (split fp16 products, FP32 accumulation).  Error bar: the
+# rounding bound of _f16x3_error_bound (each product ~2^-22 relative); on sums
+# without strong cancellation that is <= 2e-6 relative to the FP64 result.
+# ---------------------------------------------------------------------------
+F16X3_TOL = 2e-6
+
+
+@pytest.mark.parametrize("n,nr,npairs", [(1, 1, 1), (63, 5, 1), (130, 129, 3), (1000, 300, 8)])
+def test_f16x3_ragged_sizes_against_fp64(n, nr, npairs):
+    """Sizes that are not multiples of the 128 x 128 x 64 tiles (padding,
+    the MFMA lane layout) against the FP64 contraction of the same g~ (random
+    well-separated nodes: little cancellation, so a plain relative bar)."""
+    print(f"case: f16x3 ragged n={n} nr={nr} pairs={npairs}", flush=True)
+    from yellowthinfilms_amd import CavityTransfer1D, radial_synthesis, workloads as W
+    stack = W.point_uniaxial_stack()
+    rng = np.random.default_rng(n + nr)
+    kx = np.sort(rng.uniform(0.01, 40.0, n))
+    wt = rng.uniform(0.001, 0.05, n)
+    cache = CavityTransfer1D(stack, W0, kx, wt, 1)
+    zo = 1.0 - NM * rng.uniform(15, 200, npairs)
+    g = cache.ghat(1, 10 * NM, zo)
+    rho = np.sort(rng.uniform(0.0, 2.0, nr))
+    R64 = radial_synthesis(cache, g, rho, mode="fp64").cpu().numpy()
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    assert relerr(R16, R64) < F16X3_TOL, relerr(R16, R64)
+
+
+def _f16x3_error_bound(kx, wt, g, rho):
+    """|| B ||_2 for the per-entry rounding bound of the split-fp16 products:
+    a product of two hi+lo splits carries <= ~2^-22 relative error where both
+    factors are normal fp16 numbers; below that the splits keep an absolute
+    2^-25 (Bessel values) and 2^-25 * 2^e (coefficient column scaled by 2^-e,
+    2^e <= 2 colmax).  Per Bessel term of an entry:
+        B = 2^-20 sum_k |J||C| + 2^-23 colmax sum_k |J| + 2^-24 sum_k |C|."""
+    from scipy.special import jv
+    g = np.asarray(g)  # (np, 9, n)
+    w = kx * wt / (2 * np.pi)
+    gxx, gyx, gzx, gxy, gyy, gzy, gxz, gyz, gzz = (g[:, i] for i in range(9))
+    cols = [[(gxx + gyy) / 2, (gxy - gyx) / 2, gzz], [gxz, gyz, gzx, gzy], [(gxx - gyy) / 2, (gxy + gyx) / 2]]
+    X = np.outer(rho, kx)
+    B = []
+    for o in range(3):
+        J = np.abs(jv(o, X))
+        out = []
+        for c in cols[o]:
+            C = np.abs(c * w[None, :])  # (np, n)
+            cm = C.max(axis=1)
+            out.append(2.0 ** -20 * (J @ C.T) + 2.0 ** -23 * J.sum(axis=1)[:, None] * cm[None, :]
+                       + 2.0 ** -24 * C.sum(axis=1)[None, :])
+        B.append(out)
+    (A, Bc, E), (XZ, YZ, ZX, ZY), (Cc, D) = B
+    entries = [A + Cc, D + Bc, ZX, Bc + D, A + Cc, ZY, XZ, YZ, E]
+    return np.linalg.norm(np.stack(entries))
+
+
+def test_f16x3_point_uniaxial_workload():
+    """Split-fp16 error within its rounding bound on the full
+    point_uniaxial_1d rule (whose direct term cancels strongly), and
+    closed_form_G with the same mode."""
+    print("case: f16x3 point_uniaxial_1d", flush=True)
+    from yellowthinfilms_amd import CavityTransfer1D, closed_form_G, radial_synthesis, workloads as W
+    wl = W.POINT_UNIAXIAL_1D
+    kx, wt = W.point_uniaxial_rule()
+    zo, rho = W.point_uniaxial_geometry()
+    cache = CavityTransfer1D(W.point_uniaxial_stack(), W0, kx, wt, 1)
+    g = cache.ghat(1, wl["zsrc"], zo)
+    R64 = radial_synthesis(cache, g, rho, mode="fp64").cpu().numpy()
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    bound = _f16x3_error_bound(kx, wt, g.cpu().numpy(), rho)
+    err = np.linalg.norm(R16 - R64)
+    print("f16x3 point_uniaxial_1d: rel err", relerr(R16, R64), "abs err", err, "bound", bound, flush=True)
+    assert err <= bound
+    xs = np.linspace(-1.0, 1.0, 12)
+    G64 = _julia(closed_form_G(cache, 1, wl["zsrc"], zo, xs, xs, mode="fp64"))
+    G16 = _julia(closed_form_G(cache, 1, wl["zsrc"], zo, xs, xs, mode="f16x3"))
+    assert relerr(G16, G64) < 2e-5, relerr(G16, G64)
+
+
+def test_f16x3_gemm_workload_scaled_down():
+    """The mixed-precision GEMM workload's stack, pairs and rule at 1/4 of its
+    node count and 1/16 of its radii, against the FP64 path."""
+    print("case: f16x3 bessel_gemm scaled", flush=True)
+    from yellowthinfilms_amd import workloads as W, green_nodes
+    from yellowthinfilms_amd.synthesis import CavityTransfer1D, radial_synthesis
+    wl = dict(W.BESSEL_GEMM, n_band=3584, n_tail=512, n_rho=512)
+    kx, wt = W.bessel_gemm_rule(wl)
+    zs, zo, rho = W.bessel_gemm_geometry(wl)
+    cache = CavityTransfer1D(W.point_uniaxial_stack(), W0, kx, wt, 1)
+    _, g = green_nodes(cache.stack, W0, cache.kx, cache.ky, 1, 1, zs, zo, cache=False)
+    R64 = radial_synthesis(cache, g, rho, mode="fp64").cpu().numpy()
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    bound = _f16x3_error_bound(kx, wt, g.cpu().numpy(), rho)
+    err = np.linalg.norm(R16 - R64)
+    print("f16x3 bessel_gemm (scaled): rel err", relerr(R16, R64), "bound/|R|", bound / np.linalg.norm(R64), flush=True)
+    assert err <= bound
+
+
+def test_kernel_timing_counts_launches():
+    print("case: kernel timing", flush=True)
+    from yellowthinfilms_amd import CavityTransfer1D, closed_form_G, workloads as W
+    from yellowthinfilms_amd._lib import Context
+    kx, wt = W.point_uniaxial_rule(dict(W.POINT_UNIAXIAL_1D, n_band=256, n_tail=32))
+    cache = CavityTransfer1D(W.point_uniaxial_stack(), W0, kx, wt, 1)
+    ctx = Context.get(0)
+    ctx.set_timing(True)
+    try:
+        for _ in range(2):
+            closed_form_G(cache, 1, 10 * NM, [0.9], [0.1, 0.2], [0.0], mode="f16x3")
+        for name in ("node_solve", "synth_coef", "bessel_table", "split_columns", "bessel_gemm", "mfma_combine",
+                     "synth_rotate"):
+            ms, n = ctx.kernel_time(name)
+            assert n == 2 and ms > 0.0, (name, ms, n)
+        assert ctx.kernel_time("synth_fp64") == (0.0, 0)
+    finally:
+        ctx.set_timing(False)
+    assert ctx.kernel_time("node_solve") == (0.0, 0)

@@ -41,6 +41,8 @@ _SIGS = {
     "ytf_last_error": (ct.c_char_p, [ct.c_void_p]),
     "ytf_ctx_set_stream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "ytf_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
+    "ytf_ctx_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int]),
+    "ytf_ctx_kernel_time": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
     "ytf_green_nodes": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_void_p,
                                    ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
@@ -54,7 +56,7 @@ _SIGS = {
                                         ct.c_void_p, ct.c_int]),
 }
 
-SYNTH_MODES = {"fp64": 0}
+SYNTH_MODES = {"fp64": 0, "f16x3": 1}
 
 EXPORTS = tuple(_SIGS)
 
@@ -107,3 +109,13 @@ class Context:
         import torch
         s = torch.cuda.current_stream(self.device)
         check(lib().ytf_ctx_set_stream(self.handle, ct.c_void_p(s.cuda_stream)), self.handle)
+
+    def set_timing(self, on=True):
+        """Bracket every kernel launch of this context with HIP events (clears totals)."""
+        check(lib().ytf_ctx_set_timing(self.handle, 1 if on else 0), self.handle)
+
+    def kernel_time(self, name):
+        """(total ms, launches) of one kernel name since set_timing; synchronizes."""
+        ms, n = ct.c_double(), ct.c_int64()
+        check(lib().ytf_ctx_kernel_time(self.handle, name.encode(), ct.byref(ms), ct.byref(n)), self.handle)
+        return ms.value, n.value

@@ -37,6 +37,15 @@ struct ytf_ctx {
   size_t work_bytes = 0;
   void *aux = nullptr;
   size_t aux_bytes = 0;
+  // kernel timing: events bracketing each launch while enabled
+  bool timing = false;
+  struct Mark {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> totals;
 };
 
 static thread_local std::string g_err;
@@ -87,6 +96,48 @@ static int ensure_buf(ytf_ctx *c, void **buf, size_t *cur, size_t bytes) {
   return YTF_OK;
 }
 
+static hipEvent_t take_event(ytf_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch through `f`; while timing is on, bracket it with events on the
+// context's stream under `name`.
+template <class F>
+static hipError_t timed(ytf_ctx *c, const char *name, F &&f) {
+  if (!c->timing) return f();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return f();
+  hipEventRecord(a, c->stream);
+  hipError_t e = f();
+  hipEventRecord(b, c->stream);
+  c->marks.push_back({name, a, b});
+  return e;
+}
+
+// Fold finished marks into the per-name totals (synchronizes the stream).
+static int drain_marks(ytf_ctx *c) {
+  if (c->marks.empty()) return YTF_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &m : c->marks) {
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, m.a, m.b));
+    auto &t = c->totals[m.name];
+    t.first += ms;
+    t.second += 1;
+    c->pool.push_back(m.a);
+    c->pool.push_back(m.b);
+  }
+  c->marks.clear();
+  return YTF_OK;
+}
+
 static int twiddles(ytf_ctx *c, int n, cd **out) {
   auto it = c->tw.find(n);
   if (it != c->tw.end()) { *out = it->second; return YTF_OK; }
@@ -131,6 +182,11 @@ int ytf_ctx_destroy(ytf_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   for (auto &kv : c->tw) hipFree(kv.second);
+  for (auto &m : c->marks) {
+    hipEventDestroy(m.a);
+    hipEventDestroy(m.b);
+  }
+  for (auto e : c->pool) hipEventDestroy(e);
   if (c->small) hipFree(c->small);
   if (c->work) hipFree(c->work);
   if (c->aux) hipFree(c->aux);
@@ -238,8 +294,10 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   }
   int64_t blocks = (n_nodes + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
-  hipError_t e = (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, c->stream)
-                                            : ytf::launch_node_general(P, (int)blocks, c->stream);
+  hipError_t e = timed(c, "node_solve", [&] {
+    return (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, c->stream)
+                                      : ytf::launch_node_general(P, (int)blocks, c->stream);
+  });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_green_nodes: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
@@ -260,9 +318,32 @@ int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const
   if (rc) return rc;
   rc = twiddles(c, ny, &twy);
   if (rc) return rc;
-  hipError_t e = ytf::launch_spectral_apply(nx, ny, (const cd *)ghat_dev, (const cd *)J_dev, (cd *)E_dev,
-                                            (cd *)work_dev, twx, twy, scale, c->stream);
+  hipError_t e = timed(c, "spectral_apply", [&] {
+    return ytf::launch_spectral_apply(nx, ny, (const cd *)ghat_dev, (const cd *)J_dev, (cd *)E_dev, (cd *)work_dev,
+                                      twx, twy, scale, c->stream);
+  });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_spectral_apply: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_ctx_set_timing(ytf_ctx *c, int on) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_ctx_set_timing: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = drain_marks(c);
+  if (rc) return rc;
+  c->totals.clear();
+  c->timing = on != 0;
+  return YTF_OK;
+}
+
+int ytf_ctx_kernel_time(ytf_ctx *c, const char *name, double *total_ms, int64_t *launches) {
+  if (!c || !name || !total_ms || !launches) return fail(c, YTF_EINVAL, "ytf_ctx_kernel_time: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = drain_marks(c);
+  if (rc) return rc;
+  auto it = c->totals.find(name);
+  *total_ms = it == c->totals.end() ? 0.0 : it->second.first;
+  *launches = it == c->totals.end() ? 0 : it->second.second;
   return YTF_OK;
 }
 
@@ -271,26 +352,46 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 // Radial synthesis into R (device, [np][9][nr] complex); the caller holds the lock.
 static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const double *kx, const double *wt,
                                    int np, const double *ghat, int nr, const double *rho, cd *R, int mode) {
-  if (mode != YTF_SYNTH_FP64) return fail(c, YTF_EINVAL, "%s: unknown precision mode %d", who, mode);
+  if (mode != YTF_SYNTH_FP64 && mode != YTF_SYNTH_F16X3)
+    return fail(c, YTF_EINVAL, "%s: unknown precision mode %d", who, mode);
   if (n < 0 || np < 1 || nr < 0) return fail(c, YTF_EINVAL, "%s: bad sizes (n_nodes=%lld, n_pairs=%d, n_rho=%d)", who,
                                              (long long)n, np, nr);
   if (nr == 0) return YTF_OK;
   if (n > 0 && (!kx || !wt || !ghat)) return fail(c, YTF_EINVAL, "%s: NULL node buffer", who);
-  const int nks = n > 0 ? ytf::synth_fp64_splits(n, np, nr) : 1;
   const size_t coef_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)(n > 0 ? n : 1));
+  hipError_t e = hipSuccess;
+  if (mode == YTF_SYNTH_F16X3 && n > 0) {
+    const ytf::MfmaPlan P = ytf::mfma_plan(n, np, nr);
+    const size_t jb = align256(P.jtab_bytes), cb = align256(P.ctab_bytes), eb = align256(P.cexp_bytes),
+                 ob = align256(P.out_bytes);
+    int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + jb + cb + eb + ob);
+    if (rc) return rc;
+    char *w = (char *)c->work;
+    cd *coef = (cd *)w;
+    void *jtab = w + coef_b, *ctab = w + coef_b + jb;
+    int *cexp = (int *)(w + coef_b + jb + cb);
+    float *out = (float *)(w + coef_b + jb + cb + eb);
+    if ((e = timed(c, "synth_coef", [&] { return ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream); })) ||
+        (e = timed(c, "bessel_table", [&] { return ytf::launch_bessel_table(n, np, nr, kx, rho, jtab, c->stream); })) ||
+        (e = timed(c, "split_columns", [&] { return ytf::launch_split_columns(n, np, nr, coef, ctab, cexp, c->stream); })) ||
+        (e = timed(c, "bessel_gemm", [&] { return ytf::launch_bessel_gemm(n, np, nr, jtab, ctab, out, c->stream); })) ||
+        (e = timed(c, "mfma_combine", [&] { return ytf::launch_mfma_combine(n, np, nr, out, cexp, R, c->stream); })))
+      return fail(c, YTF_EHIP, "%s: matrix-core launch failed: %s", who, hipGetErrorString(e));
+    return YTF_OK;
+  }
+  const int nks = n > 0 ? ytf::synth_fp64_splits(n, np, nr) : 1;
   const size_t part_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)nr * (size_t)nks);
   int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + part_b);
   if (rc) return rc;
   cd *coef = (cd *)c->work;
   cd *part = (cd *)((char *)c->work + coef_b);
-  hipError_t e = hipSuccess;
-  if (n > 0) {
-    e = ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream);
-    if (e != hipSuccess) return fail(c, YTF_EHIP, "%s: coefficient launch failed: %s", who, hipGetErrorString(e));
-  }
+  if (n > 0 &&
+      (e = timed(c, "synth_coef", [&] { return ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream); })))
+    return fail(c, YTF_EHIP, "%s: coefficient launch failed: %s", who, hipGetErrorString(e));
   // with no nodes the contraction kernel writes zeros
-  e = ytf::launch_synth_fp64(n, np, nr, nks, kx, rho, coef, part, R, c->stream);
-  if (e != hipSuccess) return fail(c, YTF_EHIP, "%s: synthesis launch failed: %s", who, hipGetErrorString(e));
+  if ((e = timed(c, "synth_fp64", [&] { return ytf::launch_synth_fp64(n, np, nr, nks, kx, rho, coef, part, c->stream); })) ||
+      (e = timed(c, "synth_reduce", [&] { return ytf::launch_synth_reduce(np, nr, nks, part, R, c->stream); })))
+    return fail(c, YTF_EHIP, "%s: synthesis launch failed: %s", who, hipGetErrorString(e));
   return YTF_OK;
 }
 
@@ -354,7 +455,9 @@ int ytf_closed_form_G_1d(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, cons
   rc = radial_synthesis_locked(c, "ytf_closed_form_G_1d", n_nodes, kx_dev, wt_dev, n_pairs, ghat_dev, nr, rho_d, R,
                                mode);
   if (rc) return rc;
-  hipError_t e = ytf::launch_synth_rotate(nx, ny, n_pairs, nr, xs_d, ys_d, rmap_d, R, (cd *)G_dev, c->stream);
+  hipError_t e = timed(c, "synth_rotate", [&] {
+    return ytf::launch_synth_rotate(nx, ny, n_pairs, nr, xs_d, ys_d, rmap_d, R, (cd *)G_dev, c->stream);
+  });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_closed_form_G_1d: rotation launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }

@@ -199,15 +199,17 @@ int synth_fp64_splits(int64_t n, int np, int nr) {
 }
 
 hipError_t launch_synth_fp64(int64_t n, int np, int nr, int nks, const double *kx, const double *rho,
-                             const cd *coef, cd *part, cd *R, hipStream_t s) {
+                             const cd *coef, cd *part, hipStream_t s) {
   const int PC = 4;
   const int64_t chunk = (n + nks - 1) / nks;
   dim3 grid((unsigned)((nr + 255) / 256), (unsigned)((np + PC - 1) / PC), (unsigned)nks);
   hipLaunchKernelGGL(k_synth_fp64<PC>, grid, dim3(256), 0, s, n, np, nr, chunk, kx, rho, coef, part);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_reduce(int np, int nr, int nks, const cd *part, cd *R, hipStream_t s) {
   dim3 g2((unsigned)((nr + 255) / 256), np);
-  hipLaunchKernelGGL(k_synth_reduce, g2, dim3(256), 0, s, np, nr, nks, (const cd *)part, R);
+  hipLaunchKernelGGL(k_synth_reduce, g2, dim3(256), 0, s, np, nr, nks, part, R);
   return hipGetLastError();
 }
 

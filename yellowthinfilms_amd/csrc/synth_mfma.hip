@@ -1,0 +1,367 @@
+// synth_mfma.hip -- the radial Bessel contraction as a real x complex GEMM on
+// the gfx950 matrix cores, with split-precision compensated products.
+//
+// Same sum as synth.hip (mod_ClosedFormField.jl:344-390 over the row cache
+// :304-319), per Bessel order n:
+//     R_n[rho, col] = sum_k J_n(kx_k rho) C_n[k, col],
+// with C_n the node coefficient columns of that order (real and imaginary
+// parts as separate real columns, all depth pairs side by side):
+//     n = 0: a, b, e (3 per pair);  n = 1: xz, yz, zx, zy (4);  n = 2: c, d (2).
+//
+// Precision: J in [-1, 1] is split J = Jh + Jl into two fp16 numbers; each
+// coefficient column is scaled by a power of two 2^-e (|C| 2^-e < 1) and split
+// the same way.  The product is Jh Ch + Jh Cl + Jl Ch (the dropped Jl Cl is
+// below 2^-22 relative), accumulated in FP32 by v_mfma_f32_32x32x16_f16; the
+// scale is undone in FP64 when the orders are combined.  The Bessel values
+// come from FP64 evaluations; mode solves and g~ stay FP64.
+//
+// Layouts (all k-fastest, k padded to a multiple of 64 with zeros):
+//   Jtab  [order][hi/lo][Mpad][Kpad] fp16     Mpad = radii padded to 128
+//   Ctab  [order][hi/lo][Npad_n][Kpad] fp16   column = (p * nc_n + j) * 2 + re/im
+//   Cexp  [order][Npad_n] int                 column scale exponents
+//   Out   [order][Mpad][Npad_n] fp32
+#include <hip/hip_runtime.h>
+
+#include "ytf_kernels.h"
+
+namespace ytf {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f16x __attribute__((ext_vector_type(16)));
+
+__device__ inline void bessel012m(double x, double &b0, double &b1, double &b2) {
+  const double ax = fabs(x);
+  b0 = __ocml_j0_f64(ax);
+  const double t = __ocml_j1_f64(ax);
+  b1 = x < 0.0 ? -t : t;
+  if (ax < 0.5) {
+    const double h2 = 0.25 * ax * ax;
+    double term = 0.5 * h2, s = term;
+#pragma unroll
+    for (int m = 1; m < 9; ++m) {
+      term *= -h2 / (double)(m * (m + 2));
+      s += term;
+    }
+    b2 = s;
+  } else {
+    b2 = 2.0 * t / ax - b0;
+  }
+}
+
+__device__ inline void split_h(double v, _Float16 &h, _Float16 &l) {
+  h = (_Float16)v;
+  l = (_Float16)(v - (double)h);
+}
+
+// Bessel table: one thread = one radius x 8 consecutive nodes.
+__global__ void k_bessel_table(int64_t n, int nr, int Mpad, int64_t Kpad, const double *__restrict__ kx,
+                               const double *__restrict__ rho, _Float16 *__restrict__ J) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t kc = Kpad / 8;
+  if (t >= (int64_t)Mpad * kc) return;
+  const int r = (int)(t / kc);
+  const int64_t k0 = (t % kc) * 8;
+  h8 v[3][2];
+  const double rh = r < nr ? rho[r] : 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int64_t k = k0 + j;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    if (r < nr && k < n) bessel012m(kx[k] * rh, b0, b1, b2);
+    _Float16 h, l;
+    split_h(b0, h, l); v[0][0][j] = h; v[0][1][j] = l;
+    split_h(b1, h, l); v[1][0][j] = h; v[1][1][j] = l;
+    split_h(b2, h, l); v[2][0][j] = h; v[2][1][j] = l;
+  }
+  const size_t plane = (size_t)Mpad * Kpad;
+#pragma unroll
+  for (int o = 0; o < 3; ++o)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) *(h8 *)(J + (size_t)(2 * o + s) * plane + (size_t)r * Kpad + k0) = v[o][s];
+}
+
+struct MfmaCols {
+  int nc[3];       // complex columns per pair: 3, 4, 2
+  int first[3];    // first SC_ index of the order: 0, 3, 7
+  int npad[3];     // padded real columns
+  size_t ctab[3];  // element offset of each order's hi plane in Ctab
+  size_t cexp[3];  // offset in Cexp
+  size_t out[3];   // offset in Out
+};
+
+// One block = one real coefficient column: its scale exponent, then the split.
+// coef: [p][9][n] complex FP64 (synth.hip layout).
+__global__ void k_split_columns(int64_t n, int np, int64_t Kpad, MfmaCols cols, const cd *__restrict__ coef,
+                                _Float16 *__restrict__ Ctab, int *__restrict__ Cexp) {
+  const int o = blockIdx.y;
+  const int col = blockIdx.x;
+  if (col >= cols.npad[o]) return;
+  const int ncols = 2 * cols.nc[o] * np;
+  const int ri = col & 1, jj = (col >> 1) % cols.nc[o], p = (col >> 1) / cols.nc[o];
+  const bool live = col < ncols;
+  const double *src = live ? (const double *)(coef + ((size_t)p * 9 + cols.first[o] + jj) * n) + ri : nullptr;
+  __shared__ double red[256];
+  double m = 0.0;
+  if (live)
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, fabs(src[2 * k]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const double mx = red[0];
+  // 2^e > max|v|, so the scaled values lie in (-1, 1)
+  const int e = (mx > 0.0 && isfinite(mx)) ? ilogb(mx) + 1 : 0;
+  if (threadIdx.x == 0) Cexp[cols.cexp[o] + col] = e;
+  _Float16 *hi = Ctab + cols.ctab[o] + (size_t)col * Kpad;
+  _Float16 *lo = hi + (size_t)cols.npad[o] * Kpad;
+  for (int64_t k = threadIdx.x; k < Kpad; k += blockDim.x) {
+    double v = (live && k < n) ? ldexp(src[2 * k], -e) : 0.0;
+    if (!isfinite(v)) v = 0.0;
+    _Float16 h, l;
+    split_h(v, h, l);
+    hi[k] = h;
+    lo[k] = l;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: block tile 128 (radii) x 128 (columns), K step 64; 4 waves in 2 x 2,
+// each 64 x 64 = 2 x 2 MFMA tiles of 32 x 32.  Global -> registers prefetch of
+// the next K step while the current one is multiplied out of LDS.
+// ---------------------------------------------------------------------------
+constexpr int BM = 128, BN = 128, BK = 64, LDK = BK + 8;  // LDS row stride (halfs), padded
+
+struct GemmArgs {
+  const _Float16 *J;  // Jtab
+  const _Float16 *C;  // Ctab
+  float *out;         // Out
+  int64_t Kpad;
+  int Mpad;
+  int mtiles;
+  int ntiles[3];
+  int nt_total;  // ntiles[0] + ntiles[1] + ntiles[2]
+  int nblocks;
+  MfmaCols cols;
+};
+
+__global__ __launch_bounds__(256, 2) void k_bessel_gemm(GemmArgs A) {
+  // XCD-grouped tile order: hardware places block b on XCD b % 8; give each
+  // XCD a contiguous run of tiles (radius tile major), so the blocks that share
+  // a radius tile's J slab run on the same L2.
+  const int b = blockIdx.x;
+  const int per = A.nblocks / 8, rem = A.nblocks % 8, x = b % 8;
+  const int t = x * per + (x < rem ? x : rem) + b / 8;
+  const int mt = t / A.nt_total;
+  int u = t % A.nt_total, o = 0;
+  while (u >= A.ntiles[o]) { u -= A.ntiles[o]; ++o; }
+  const int nt = u;
+  const int m0 = mt * BM, n0 = nt * BN;
+  const int64_t K = A.Kpad;
+  const size_t jplane = (size_t)A.Mpad * K;
+  const _Float16 *Jh = A.J + (size_t)(2 * o) * jplane + (size_t)m0 * K;
+  const _Float16 *Jl = Jh + jplane;
+  const int npad = A.cols.npad[o];
+  const _Float16 *Ch = A.C + A.cols.ctab[o] + (size_t)n0 * K;
+  const _Float16 *Cl = Ch + (size_t)npad * K;
+
+  __shared__ _Float16 sAh[BM * LDK], sAl[BM * LDK], sBh[BN * LDK], sBl[BN * LDK];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // global load slots: 1024 chunks of 8 halfs per array and K step, 4 per thread
+  h8 ra[2][4], rb[2][4];
+  auto gload = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = tid + 256 * i;
+      const int row = c >> 3, kc = (c & 7) * 8;
+      ra[0][i] = *(const h8 *)(Jh + (size_t)row * K + k0 + kc);
+      ra[1][i] = *(const h8 *)(Jl + (size_t)row * K + k0 + kc);
+      rb[0][i] = *(const h8 *)(Ch + (size_t)row * K + k0 + kc);
+      rb[1][i] = *(const h8 *)(Cl + (size_t)row * K + k0 + kc);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int c = tid + 256 * i;
+      const int row = c >> 3, kc = (c & 7) * 8;
+      *(h8 *)(sAh + row * LDK + kc) = ra[0][i];
+      *(h8 *)(sAl + row * LDK + kc) = ra[1][i];
+      *(h8 *)(sBh + row * LDK + kc) = rb[0][i];
+      *(h8 *)(sBl + row * LDK + kc) = rb[1][i];
+    }
+  };
+
+  f16x acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  const int ksteps = (int)(K / BK);
+  gload(0);
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  for (int ks = 0; ks < ksteps; ++ks) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (ks + 1 < ksteps) gload((int64_t)(ks + 1) * BK);
+#pragma unroll
+    for (int s = 0; s < BK / 16; ++s) {
+      h8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int ar = wm * 64 + i * 32 + lr;
+        ah[i] = *(const h8 *)(sAh + ar * LDK + s * 16 + lk);
+        al[i] = *(const h8 *)(sAl + ar * LDK + s * 16 + lk);
+        const int br = wn * 64 + i * 32 + lr;
+        bh[i] = *(const h8 *)(sBh + br * LDK + s * 16 + lk);
+        bl[i] = *(const h8 *)(sBl + br * LDK + s * 16 + lk);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+  // D layout of the 32 x 32 tile: lane holds column lane % 32, rows
+  // 8 g + 4 (lane / 32) + e for element 4 g + e.
+  float *out = A.out + A.cols.out[o];
+  const int hi = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 64 + j * 32 + lr;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = m0 + wm * 64 + i * 32 + 8 * g + 4 * hi + e;
+          out[(size_t)row * npad + col] = acc[i][j][4 * g + e];
+        }
+    }
+}
+
+// Combine the three orders into R[p][c + 3q][r] (FP64), undoing the column scales.
+__global__ void k_mfma_combine(int np, int nr, MfmaCols cols, const float *__restrict__ out,
+                               const int *__restrict__ Cexp, cd *__restrict__ R) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (r >= nr) return;
+  cd s[9];
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    const float *row = out + cols.out[o] + (size_t)r * cols.npad[o];
+    const int *ex = Cexp + cols.cexp[o];
+    for (int jj = 0; jj < cols.nc[o]; ++jj) {
+      const int c = (p * cols.nc[o] + jj) * 2;
+      s[cols.first[o] + jj] = cd{ldexp((double)row[c], ex[c]), ldexp((double)row[c + 1], ex[c + 1])};
+    }
+  }
+  // index names as in synth.hip: A B E | XZ YZ ZX ZY | C D
+  cd *o9 = R + (size_t)p * 9 * nr + r;
+  o9[0 * nr] = s[0] + s[7];
+  o9[1 * nr] = s[8] - s[1];
+  o9[2 * nr] = s[5];
+  o9[3 * nr] = s[1] + s[8];
+  o9[4 * nr] = s[0] - s[7];
+  o9[5 * nr] = s[6];
+  o9[6 * nr] = s[3];
+  o9[7 * nr] = s[4];
+  o9[8 * nr] = s[2];
+}
+
+static MfmaCols make_cols(int np, int64_t Kpad, int Mpad) {
+  MfmaCols c;
+  const int nc[3] = {3, 4, 2}, first[3] = {0, 3, 7};
+  size_t ct = 0, ce = 0, co = 0;
+  for (int o = 0; o < 3; ++o) {
+    c.nc[o] = nc[o];
+    c.first[o] = first[o];
+    c.npad[o] = (2 * nc[o] * np + BN - 1) / BN * BN;
+    c.ctab[o] = ct;
+    c.cexp[o] = ce;
+    c.out[o] = co;
+    ct += (size_t)2 * c.npad[o] * Kpad;
+    ce += (size_t)c.npad[o];
+    co += (size_t)Mpad * c.npad[o];
+  }
+  return c;
+}
+
+MfmaPlan mfma_plan(int64_t n, int np, int nr) {
+  MfmaPlan P;
+  P.Kpad = (n + BK - 1) / BK * BK;
+  if (P.Kpad == 0) P.Kpad = BK;
+  P.Mpad = (nr + BM - 1) / BM * BM;
+  MfmaCols c = make_cols(np, P.Kpad, P.Mpad);
+  size_t ncol = 0;
+  for (int o = 0; o < 3; ++o) ncol += c.npad[o];
+  P.jtab_bytes = sizeof(_Float16) * 6 * (size_t)P.Mpad * P.Kpad;
+  P.ctab_bytes = sizeof(_Float16) * 2 * ncol * P.Kpad;
+  P.cexp_bytes = sizeof(int) * ncol;
+  P.out_bytes = sizeof(float) * (size_t)P.Mpad * ncol;
+  return P;
+}
+
+// The four launches of the matrix-core contraction (the C-ABI times each).
+hipError_t launch_bessel_table(int64_t n, int np, int nr, const double *kx, const double *rho, void *jtab,
+                               hipStream_t s) {
+  const MfmaPlan P = mfma_plan(n, np, nr);
+  const int64_t nthr = (int64_t)P.Mpad * (P.Kpad / 8);
+  hipLaunchKernelGGL(k_bessel_table, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, n, nr, P.Mpad, P.Kpad,
+                     kx, rho, (_Float16 *)jtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_columns(int64_t n, int np, int nr, const cd *coef, void *ctab, int *cexp, hipStream_t s) {
+  const MfmaPlan P = mfma_plan(n, np, nr);
+  const MfmaCols cols = make_cols(np, P.Kpad, P.Mpad);
+  int maxpad = cols.npad[0];
+  for (int o = 1; o < 3; ++o) maxpad = cols.npad[o] > maxpad ? cols.npad[o] : maxpad;
+  hipLaunchKernelGGL(k_split_columns, dim3(maxpad, 3), dim3(256), 0, s, n, np, P.Kpad, cols, coef,
+                     (_Float16 *)ctab, cexp);
+  return hipGetLastError();
+}
+
+hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out,
+                              hipStream_t s) {
+  const MfmaPlan P = mfma_plan(n, np, nr);
+  const MfmaCols cols = make_cols(np, P.Kpad, P.Mpad);
+  GemmArgs A;
+  A.J = (const _Float16 *)jtab;
+  A.C = (const _Float16 *)ctab;
+  A.out = out;
+  A.Kpad = P.Kpad;
+  A.Mpad = P.Mpad;
+  A.mtiles = P.Mpad / BM;
+  A.nt_total = 0;
+  for (int o = 0; o < 3; ++o) {
+    A.ntiles[o] = cols.npad[o] / BN;
+    A.nt_total += A.ntiles[o];
+  }
+  A.nblocks = A.mtiles * A.nt_total;
+  A.cols = cols;
+  hipLaunchKernelGGL(k_bessel_gemm, dim3(A.nblocks), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *cexp, cd *R, hipStream_t s) {
+  const MfmaPlan P = mfma_plan(n, np, nr);
+  const MfmaCols cols = make_cols(np, P.Kpad, P.Mpad);
+  hipLaunchKernelGGL(k_mfma_combine, dim3((nr + 255) / 256, np), dim3(256), 0, s, np, nr, cols, out, cexp, R);
+  return hipGetLastError();
+}
+
+}  // namespace ytf

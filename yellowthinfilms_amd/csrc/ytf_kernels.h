@@ -51,8 +51,25 @@ hipError_t launch_synth_coef(int64_t n, int np, const double *kx, const double *
                              hipStream_t s);
 int synth_fp64_splits(int64_t n, int np, int nr);
 hipError_t launch_synth_fp64(int64_t n, int np, int nr, int nks, const double *kx, const double *rho,
-                             const cd *coef, cd *part, cd *R, hipStream_t s);
+                             const cd *coef, cd *part, hipStream_t s);
+hipError_t launch_synth_reduce(int np, int nr, int nks, const cd *part, cd *R, hipStream_t s);
 hipError_t launch_synth_rotate(int nx, int ny, int np, int nr, const double *xs, const double *ys, const int *rmap,
                                const cd *R, cd *G, hipStream_t s);
+
+// The same contraction on the matrix cores with split fp16 products
+// (synth_mfma.hip): workspace sizes, then Bessel table + column split + GEMM +
+// FP64 combine into R.
+struct MfmaPlan {
+  int64_t Kpad;
+  int Mpad;
+  size_t jtab_bytes, ctab_bytes, cexp_bytes, out_bytes;
+};
+MfmaPlan mfma_plan(int64_t n, int np, int nr);
+hipError_t launch_bessel_table(int64_t n, int np, int nr, const double *kx, const double *rho, void *jtab,
+                               hipStream_t s);
+hipError_t launch_split_columns(int64_t n, int np, int nr, const cd *coef, void *ctab, int *cexp, hipStream_t s);
+hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out,
+                              hipStream_t s);
+hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *cexp, cd *R, hipStream_t s);
 
 }  // namespace ytf

@@ -14,6 +14,11 @@ point_uniaxial_1d: air / 50 nm uniaxial film eps = diag(2.5, 2.5, 3.0) + 0.01i /
   [0, 6 k0] in panels broken at the light lines (u^2 substitution at the
   lossless ones) plus a 512-node cosh tail; the J0/J1/J2 synthesis at 256
   radii in (0, 2] and 8 observation heights in the air, complex FP64.
+
+bessel_gemm: the same stack; 16384 k-rho nodes (14336 in the band + a
+  2048-node tail), 8192 radii in (0, 2], 64 depth pairs (8 source heights x 8
+  observation heights in the air): the J0/J1/J2 contraction as a real x complex
+  GEMM on the matrix cores with split fp16 products (mode solves FP64).
 """
 import math
 
@@ -72,6 +77,35 @@ def point_uniaxial_geometry(wl=POINT_UNIAXIAL_1D):
     zobs = np.array([1.0 - h * NM for h in wl["heights_nm"]])
     rho = wl["rho_max"] * np.arange(1, wl["n_rho"] + 1) / wl["n_rho"]
     return zobs, rho
+
+
+BESSEL_GEMM = dict(
+    name="bessel_gemm",
+    description=("J0/J1/J2 contraction of the air / 50 nm uniaxial film / glass stack: 8192 radii x 16384 k-rho "
+                 "nodes x 64 depth pairs (8 source x 8 observation heights), real x complex GEMM on the matrix "
+                 "cores, split fp16 products with FP32 accumulation; mode solves and g~ in FP64"),
+    source_layer=1, obs_layer=1,
+    src_heights_nm=(5, 10, 15, 20, 30, 40, 60, 80),
+    obs_heights_nm=(20, 30, 40, 60, 80, 120, 160, 240),
+    n_band=14336, order=16, kmax_k0=6.0, n_tail=2048, n_rho=8192, rho_max=2.0,
+)
+
+
+def bessel_gemm_rule(wl=BESSEL_GEMM):
+    """(kx, wt): band [0, 6 k0] broken at the light lines, cosh tail with the
+    decay length of the closest source/observation heights."""
+    rule = dict(POINT_UNIAXIAL_1D, n_band=wl["n_band"], order=wl["order"], kmax_k0=wl["kmax_k0"],
+                n_tail=wl["n_tail"], heights_nm=(min(wl["obs_heights_nm"]),),
+                zsrc=max(h for h in wl["src_heights_nm"] if h < min(wl["obs_heights_nm"])) * NM)
+    return point_uniaxial_rule(rule)
+
+
+def bessel_gemm_geometry(wl=BESSEL_GEMM):
+    """(zsrc, zobs) of every depth pair (source height major) and the radii."""
+    zs = np.repeat([h * NM for h in wl["src_heights_nm"]], len(wl["obs_heights_nm"]))
+    zo = np.tile([1.0 - h * NM for h in wl["obs_heights_nm"]], len(wl["src_heights_nm"]))
+    rho = wl["rho_max"] * np.arange(1, wl["n_rho"] + 1) / wl["n_rho"]
+    return zs, zo, rho
 
 
 BATCHED_GYRO_512 = dict(
