@@ -14,6 +14,9 @@ Workloads (yellowthinfilms_amd.workloads):
   bessel_gemm: 16384 k-rho nodes x 8192 radii x 64 depth pairs; node solve
     (FP64) + the Bessel contraction on the matrix cores (split fp16 products).
     value = contraction TFLOP/s (algorithmic) of the whole step.
+  uneven_256: 256 stacks of 2..24 hyperbolic layers with 1k..16k nodes each,
+    modes + total S-matrix per node in one persistent (graph) launch whose
+    shape the autotuner picks first.  value = stack-nodes/s.
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -313,6 +316,80 @@ def cpu_gemm(stack, kx, wt, wl, zs, zo, rho, R):
 
 
 # ---------------------------------------------------------------------------
+def bench_uneven(a):
+    import torch
+    from yellowthinfilms_amd import workloads as W
+    from yellowthinfilms_amd.batch import StackBatch
+
+    wl = W.UNEVEN_256
+    stacks, counts, kx, ky = W.uneven_batch()
+    dev = torch.device("cuda", 0)
+    kxd = torch.as_tensor(kx, device=dev)
+    kyd = torch.as_tensor(ky, device=dev)
+    b = StackBatch(stacks, counts, W.OMEGA)
+    report = b.tune(kxd, kyd, tune_db=a.tune_db)  # before the timed region
+    out = b.outputs()
+
+    def step():
+        b.solve(kxd, kyd, out=out)
+
+    for _ in range(a.warmup):
+        step()
+    b.ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    ms, nl = b.ctx.kernel_time("batch_solve")
+    b.ctx.set_timing(False)
+    ms /= nl
+    n = len(kx)
+    flops = W.UNEVEN_256_FLOPS_PER_LAUNCH
+    cpu, rel_err = None, None
+    if not a.no_cpu_baseline:
+        cpu, rel_err = cpu_uneven(stacks, counts, kx, ky, out)
+    valid = float((out["flags"] & 1).sum().item()) / n
+    res = _base(a, wl, n * a.steps / wall, "stack-nodes/s (modes + total S-matrix)", wall, "f64")
+    res["config"].update({"stacks": len(stacks), "nodes": n, "launch": b.config, "tuner": report[:400]})
+    res.update({
+        "batch_solve_ms": ms, "valid_fraction": valid,
+        "rel_err_vs_fp64_oracle": rel_err,
+        "roofline": {"kernel": "batch_solve (persistent uneven-batch kernel, one graph launch, FP64)",
+                     "bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / PEAK_FP64, "traffic": None,
+                     "flops_per_launch": flops, "nodes_per_launch": n},
+        "cpu_baseline": cpu,
+    })
+    return res
+
+
+def cpu_uneven(stacks, counts, kx, ky, out):
+    """The oracle (modes + S_all per node, OpenMP) on every 64th node of every
+    stack; the GPU's S on the same nodes is compared with it."""
+    try:
+        from oracle import pyoracle as po
+    except Exception as exc:
+        return ({"value": None, "unit": "stack-nodes/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    from yellowthinfilms_amd.materials import stack_arrays
+    from yellowthinfilms_amd import workloads as W
+    cores = _cores()
+    S = out["S"].cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(counts)])
+    dt, m, num, den = 0.0, 0, 0.0, 0.0
+    for s, st in enumerate(stacks):
+        idx = np.arange(off[s], off[s + 1], 64)
+        o = po.Stack(*stack_arrays(st))
+        t = time.perf_counter()
+        r = po.stack_S_batch(o, W.OMEGA, kx[idx], ky[idx], nthreads=cores)
+        dt += time.perf_counter() - t
+        m += len(idx)
+        g = S[:, idx].T.reshape(-1, 4, 4).transpose(0, 2, 1)
+        v = r["ok"]
+        num += float(np.sum(np.abs(g[v] - r["S"][v]) ** 2))
+        den += float(np.sum(np.abs(r["S"][v]) ** 2))
+    return ({"value": m / dt, "unit": "stack-nodes/s", "cores": cores, "kind": "port",
+             "sample": f"every 64th node of every stack ({m} nodes)"}, float(np.sqrt(num / den)))
+
+
+# ---------------------------------------------------------------------------
 def dict_ns(a, **kw):
     b = argparse.Namespace(**vars(a))
     for k, v in kw.items():
@@ -320,7 +397,8 @@ def dict_ns(a, **kw):
     return b
 
 
-WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm}
+WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
+             "uneven_256": bench_uneven}
 
 
 def main():
@@ -331,6 +409,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--tune-db", default=None, help="autotuner database file (uneven_256)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to scale the FETCH_SIZE/WRITE_SIZE counters of a profiling run)")

@@ -78,7 +78,7 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * returns the summed time (ms) and launch count of one kernel name:
  * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
  * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
- * "mfma_combine". */
+ * "mfma_combine", "batch_solve". */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
 int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
@@ -169,6 +169,38 @@ int ytf_radial_synthesis(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, co
 int ytf_closed_form_G_1d(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
                          const double *ghat_dev, const double *xs, int nx, const double *ys, int ny, double *G_dev,
                          int mode);
+
+/* Uneven batches: many independent stacks, each with its own node list,
+ * solved in one persistent launch (captured as a HIP graph).  Per node: the
+ * Berreman modes of every layer with continuity matching (Modes_Stack,
+ * mod_ModeSolver.jl:358-413) and the stack's total S-matrix
+ * SLeft[N] = SRight[1] of S_all_Matrix (mod_ScatteringMatrix.jl:112-154), with
+ * the first and last layers' modes.  Replaces the per-stack, per-node loop of
+ * Modes_Stack + S_all_Matrix over a batch of stacks.
+ *   layers: all stacks' layers concatenated (host); layer_counts[s] >= 2;
+ *   node_counts[s]: nodes of stack s (its nodes are contiguous in kx/ky, stack
+ *     after stack). */
+typedef struct ytf_batch ytf_batch;
+int ytf_batch_create(ytf_ctx *ctx, const ytf_layer *layers, const int32_t *layer_counts, const int64_t *node_counts,
+                     int n_stacks, double omega, ytf_batch **out);
+int ytf_batch_destroy(ytf_ctx *ctx, ytf_batch *batch);
+/* Outputs (device, complex, node index fastest, any but S may be NULL):
+ *   S_dev [16][n]: total S, 4x4 column-major (zero where flags lack VALID);
+ *   V1_dev, VN_dev [16][n]: [Vf Vb] of the first / last layer, 4x4 column-major;
+ *   Q1_dev, QN_dev [4][n]: Qf1 Qf2 Qb1 Qb2; flags_dev [n]: YTF_NODE_* bits. */
+int ytf_batch_solve(ytf_ctx *ctx, ytf_batch *batch, const double *kx_dev, const double *ky_dev, double *S_dev,
+                    double *V1_dev, double *Q1_dev, double *VN_dev, double *QN_dev, uint8_t *flags_dev);
+/* Launch shape: nodes per work item (multiple of 64), waves per block
+ * (1, 2, 4), persistent grid size. */
+int ytf_batch_set_config(ytf_ctx *ctx, ytf_batch *batch, int nodes_per_item, int waves_per_block, int blocks);
+int ytf_batch_get_config(ytf_ctx *ctx, const ytf_batch *batch, int *nodes_per_item, int *waves_per_block,
+                         int *blocks);
+/* Autotune the launch shape on this batch's inputs (writes S and flags).
+ * tune_db: optional text file; a stored choice for the same batch signature
+ * (device, stack/node/work counts) is reused, a new one appended.  report:
+ * optional buffer for a one-line summary. */
+int ytf_batch_tune(ytf_ctx *ctx, ytf_batch *batch, const double *kx_dev, const double *ky_dev, double *S_dev,
+                   uint8_t *flags_dev, const char *tune_db, char *report, int report_len);
 
 /* Library version string. */
 const char *ytf_version(void);

@@ -186,6 +186,25 @@ def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0):
     return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
 
 
+def stack_S_batch(st, omega, kx, ky, nthreads=0):
+    """Total S-matrix (SLeft[N]) and first/last layer modes per node:
+    dict S (n,4,4), V1, VN (n,4,4) = [Vf Vb], Q1, QN (n,4), ok (n,) bool."""
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    nn = len(kx)
+    S = np.zeros(nn * 16, np.complex128)
+    V1 = np.zeros(nn * 16, np.complex128)
+    VN = np.zeros(nn * 16, np.complex128)
+    Q1 = np.zeros(nn * 4, np.complex128)
+    QN = np.zeros(nn * 4, np.complex128)
+    ok = np.zeros(nn, np.int32)
+    lib().ora_stack_S_batch(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), _p(S.view(np.float64)),
+                            _p(V1.view(np.float64)), _p(Q1.view(np.float64)), _p(VN.view(np.float64)),
+                            _p(QN.view(np.float64)), _p(ok, _i), ct.c_int(nthreads))
+    m = lambda a: a.reshape(nn, 4, 4).transpose(0, 2, 1)
+    return dict(S=m(S), V1=m(V1), VN=m(VN), Q1=Q1.reshape(nn, 4), QN=QN.reshape(nn, 4), ok=ok.astype(bool))
+
+
 def count_node_ops(st, omega, kx, ky, src, obs_layer, zsrc, zobs):
     out = np.zeros(10, np.uint64)
     lib().ora_count_node_ops(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), ct.c_int(src),
@@ -193,6 +212,15 @@ def count_node_ops(st, omega, kx, ky, src, obs_layer, zsrc, zobs):
                              out.ctypes.data_as(ct.POINTER(ct.c_uint64)))
     keys = ["add", "mul", "div", "sqrt", "trans"]
     return {"solve": dict(zip(keys, map(int, out[:5]))), "ghat": dict(zip(keys, map(int, out[5:])))}
+
+
+def count_stack_ops(st, omega, kx, ky):
+    """Real add/mul/div, sqrt and transcendental counts of Modes_Stack + the
+    total S (SLeft[N]) for one node, in the reference's formulation."""
+    out = np.zeros(5, np.uint64)
+    lib().ora_count_stack_ops(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky),
+                              out.ctypes.data_as(ct.POINTER(ct.c_uint64)))
+    return dict(zip(["add", "mul", "div", "sqrt", "trans"], map(int, out)))
 
 
 def modal_amp(st, omega, kx, ky, layer, z, sif, sib):

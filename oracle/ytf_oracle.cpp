@@ -1337,6 +1337,49 @@ int ora_ghat_batch(int n, const double *eps, const double *mu, const double *d, 
   return 0;
 }
 
+// Batched stack S-matrix: per node, Modes_Stack (mod_ModeSolver.jl:404-413)
+// and the total S of S_all_Matrix (mod_ScatteringMatrix.jl:112-154), i.e.
+// SLeft[N] (= SRight[1]), with the first and last layers' modes.  Outputs per
+// node (complex, may be NULL): S 16 (4x4 column-major), V1 and VN 16 each
+// ([Vf Vb] 4x4 column-major), Q1 and QN 4 each (Qf1 Qf2 Qb1 Qb2); ok = 0 where
+// the mode solve failed (outputs zero).
+int ora_stack_S_batch(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
+                      double omega, const double *kx, const double *ky, int64_t nn, double *S, double *V1, double *Q1,
+                      double *VN, double *QN, int *ok, int nthreads) {
+  auto st = make_stack<double>(n, eps, mu, d, ek, mk);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t k = 0; k < nn; ++k) {
+    std::vector<Modes<double>> M;
+    bool good = modes_stack(st, omega, kx[k], ky[k], M);
+    Smat<double> Sm;
+    if (good) S_all(st, M, Sm);
+    ok[k] = good ? 1 : 0;
+    for (int i = 0; i < 16; ++i) {
+      cx<double> z = good ? Sm.SLeft[n - 1].a[i] : C<double>(0.0);
+      if (S) put(S + 2 * (16 * k + i), z);
+    }
+    const int ends[2] = {0, n - 1};
+    double *Vo[2] = {V1, VN}, *Qo[2] = {Q1, QN};
+    for (int e = 0; e < 2; ++e) {
+      const Modes<double> *m = good ? &M[ends[e]] : nullptr;
+      if (Vo[e])
+        for (int i = 0; i < 8; ++i) {
+          put(Vo[e] + 2 * (16 * k + i), m ? m->Vf.a[i] : C<double>(0.0));
+          put(Vo[e] + 2 * (16 * k + 8 + i), m ? m->Vb.a[i] : C<double>(0.0));
+        }
+      if (Qo[e])
+        for (int i = 0; i < 2; ++i) {
+          put(Qo[e] + 2 * (4 * k + i), m ? m->Qf[i] : C<double>(0.0));
+          put(Qo[e] + 2 * (4 * k + 2 + i), m ? m->Qb[i] : C<double>(0.0));
+        }
+    }
+  }
+  return 0;
+}
+
 // Op count of one node solve + one spectral dyadic (reference formulation).
 // out: add, mul, div, sqrt, trans.
 int ora_count_node_ops(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
@@ -1353,6 +1396,36 @@ int ora_count_node_ops(int n, const double *eps, const double *mu, const double 
   out[0] = solve.add; out[1] = solve.mul; out[2] = solve.div; out[3] = solve.sqrt; out[4] = solve.trans;
   out[5] = g_ops.add - solve.add; out[6] = g_ops.mul - solve.mul; out[7] = g_ops.div - solve.div;
   out[8] = g_ops.sqrt - solve.sqrt; out[9] = g_ops.trans - solve.trans;
+  return 0;
+}
+
+// Op count of Modes_Stack + the total S (SLeft[N]) of S_all_Matrix for one
+// node, in the reference's formulation (general eigen-solve per non-closed-form
+// layer, full 4x4 star products).  out: add, mul, div, sqrt, trans.
+int ora_count_stack_ops(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
+                        double omega, double kx, double ky, uint64_t *out) {
+  auto st = make_stack<CR>(n, eps, mu, d, ek, mk);
+  g_ops = Ops();
+  std::vector<Modes<CR>> M;
+  modes_stack(st, CR(omega), CR(kx), CR(ky), M);
+  // S_all_ingredients + the SLeft prefix scan of S_all_Matrix up to SLeft[N]
+  // (the suffix scan is not needed for the total S)
+  Mat<CR, 4, 4> S0 = eye<CR, 4>(), St;
+  {
+    cx<CR> dg[4];
+    S_prop(M[0], st[0].d, dg);
+    S0 = redheffer(S0, diag4(dg));
+  }
+  for (int j = 1; j < n; ++j) {
+    Mat<CR, 4, 4> Si = S_interface(M[j - 1].Vf, M[j - 1].Vb, M[j].Vf, M[j].Vb);
+    St = redheffer(S0, Si);
+    if (j < n - 1) {
+      cx<CR> dg[4];
+      S_prop(M[j], st[j].d, dg);
+      S0 = redheffer(St, diag4(dg));
+    }
+  }
+  out[0] = g_ops.add; out[1] = g_ops.mul; out[2] = g_ops.div; out[3] = g_ops.sqrt; out[4] = g_ops.trans;
   return 0;
 }
 

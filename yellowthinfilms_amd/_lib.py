@@ -54,6 +54,16 @@ _SIGS = {
     "ytf_closed_form_G_1d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
                                         ct.POINTER(ct.c_double), ct.c_int, ct.POINTER(ct.c_double), ct.c_int,
                                         ct.c_void_p, ct.c_int]),
+    "ytf_batch_create": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.POINTER(ct.c_int32),
+                                    ct.POINTER(ct.c_int64), ct.c_int, ct.c_double, ct.POINTER(ct.c_void_p)]),
+    "ytf_batch_destroy": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_batch_solve": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                   ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "ytf_batch_set_config": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int]),
+    "ytf_batch_get_config": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                        ct.POINTER(ct.c_int)]),
+    "ytf_batch_tune": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                  ct.c_char_p, ct.c_char_p, ct.c_int]),
 }
 
 SYNTH_MODES = {"fp64": 0, "f16x3": 1}

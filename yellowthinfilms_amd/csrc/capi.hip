@@ -4,7 +4,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <map>
 #include <mutex>
 #include <string>
@@ -347,7 +350,358 @@ int ytf_ctx_kernel_time(ytf_ctx *c, const char *name, double *total_ms, int64_t 
   return YTF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Uneven batches of stacks (batch.hip)
+// ---------------------------------------------------------------------------
+struct ytf_batch {
+  int device = 0;
+  int n_stacks = 0;
+  int64_t ntot = 0;
+  double omega = 0.0;
+  bool general = false;
+  std::vector<int> l0, nl;
+  std::vector<int64_t> n0, nn;
+  // device tables
+  void *dev = nullptr;  // layers | l0 | nl | n0 | nn | items_stack | items_first | counter
+  LayerDev *L = nullptr;
+  int *d_l0 = nullptr, *d_nl = nullptr, *d_is = nullptr, *d_if = nullptr;
+  int64_t *d_n0 = nullptr, *d_nn = nullptr;
+  unsigned int *counter = nullptr;
+  int n_items = 0;
+  // launch configuration
+  int nodes_per_item = 128, waves_per_block = 1, blocks = 0;
+  int items_for = 0;  // nodes_per_item the item table was built for
+  // captured graph of (counter reset + persistent kernel) for one argument set
+  hipGraphExec_t exec = nullptr;
+  std::vector<const void *> exec_key;
+  std::string signature;
+};
+
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Work items: runs of nodes_per_item nodes of one stack, heaviest stacks first.
+static int batch_items(ytf_ctx *c, ytf_batch *b) {
+  if (b->items_for == b->nodes_per_item) return YTF_OK;
+  std::vector<int> order(b->n_stacks);
+  for (int s = 0; s < b->n_stacks; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int q) { return b->nl[a] > b->nl[q]; });
+  std::vector<int> is, ifirst;
+  for (int s : order)
+    for (int64_t f = 0; f < b->nn[s]; f += b->nodes_per_item) {
+      is.push_back(s);
+      ifirst.push_back((int)f);
+    }
+  b->n_items = (int)is.size();
+  // the item tables were sized for 64-node items
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (b->n_items) {
+    HIPCHK(c, hipMemcpy(b->d_is, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_if, ifirst.data(), sizeof(int) * ifirst.size(), hipMemcpyHostToDevice));
+  }
+  b->items_for = b->nodes_per_item;
+  return YTF_OK;
+}
+
+static ytf::BatchLaunch batch_launch(ytf_batch *b, const double *kx, const double *ky, double *S, double *V1,
+                                     double *Q1, double *VN, double *QN, uint8_t *flags) {
+  ytf::BatchLaunch B;
+  B.L = b->L;
+  B.stack_l0 = b->d_l0;
+  B.stack_nl = b->d_nl;
+  B.stack_n0 = b->d_n0;
+  B.stack_nn = b->d_nn;
+  B.item_stack = b->d_is;
+  B.item_first = b->d_if;
+  B.n_items = b->n_items;
+  B.nodes_per_item = b->nodes_per_item;
+  B.omega = b->omega;
+  B.kx = kx;
+  B.ky = ky;
+  B.S = (cd *)S;
+  B.V1 = (cd *)V1;
+  B.Q1 = (cd *)Q1;
+  B.VN = (cd *)VN;
+  B.QN = (cd *)QN;
+  B.flags = flags;
+  B.ntot = b->ntot;
+  B.counter = b->counter;
+  B.general = b->general;
+  B.waves_per_block = b->waves_per_block;
+  B.blocks = b->blocks;
+  return B;
+}
+
+static int device_cus(ytf_ctx *c, int *cus) {
+  HIPCHK(c, hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  return YTF_OK;
+}
+
+static int batch_check_cfg(ytf_ctx *c, int npi, int wpb, int blocks) {
+  if (npi < 64 || npi > 4096 || (npi % 64) != 0)
+    return fail(c, YTF_EINVAL, "ytf_batch: nodes_per_item=%d must be a multiple of 64 in 64..4096", npi);
+  if (wpb != 1 && wpb != 2 && wpb != 4) return fail(c, YTF_EINVAL, "ytf_batch: waves_per_block=%d not in {1,2,4}", wpb);
+  if (blocks < 1 || blocks > 65536) return fail(c, YTF_EINVAL, "ytf_batch: blocks=%d out of range 1..65536", blocks);
+  return YTF_OK;
+}
+
+int ytf_batch_create(ytf_ctx *c, const ytf_layer *layers, const int32_t *layer_counts, const int64_t *node_counts,
+                     int n_stacks, double omega, ytf_batch **out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_batch_create: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!out || !layers || !layer_counts || !node_counts || n_stacks < 1)
+    return fail(c, YTF_EINVAL, "ytf_batch_create: bad arguments");
+  if (!(omega > 0) || !std::isfinite(omega)) return fail(c, YTF_EINVAL, "ytf_batch_create: omega must be > 0");
+  ytf_batch *b = new ytf_batch();
+  b->device = c->device;
+  b->n_stacks = n_stacks;
+  b->omega = omega;
+  int nlt = 0;
+  int64_t work = 0;
+  for (int s = 0; s < n_stacks; ++s) {
+    if (layer_counts[s] < 2 || node_counts[s] < 0 || node_counts[s] > ((int64_t)1 << 31) - 4096) {
+      delete b;
+      return fail(c, YTF_EINVAL, "ytf_batch_create: stack %d has %d layers and %lld nodes (need >= 2 layers)", s,
+                  layer_counts[s], (long long)node_counts[s]);
+    }
+    b->l0.push_back(nlt);
+    b->nl.push_back(layer_counts[s]);
+    b->n0.push_back(b->ntot);
+    b->nn.push_back(node_counts[s]);
+    nlt += layer_counts[s];
+    b->ntot += node_counts[s];
+    work += node_counts[s] * layer_counts[s];
+  }
+  for (int l = 0; l < nlt; ++l) {
+    const ytf_layer &Ly = layers[l];
+    if (Ly.eps_kind < 0 || Ly.eps_kind > 3 || Ly.mu_kind < 0 || Ly.mu_kind > 1) {
+      delete b;
+      return fail(c, YTF_EINVAL, "ytf_batch_create: layer %d has an unknown material kind", l);
+    }
+    if ((Ly.eps_kind == 0 || Ly.eps_kind == 1) && Ly.mu_kind == 0) continue;
+    const int idx[4] = {2, 5, 6, 7};
+    for (int t = 0; t < 4; ++t)
+      if (Ly.eps[2 * idx[t]] != 0.0 || Ly.eps[2 * idx[t] + 1] != 0.0 || Ly.mu[2 * idx[t]] != 0.0 ||
+          Ly.mu[2 * idx[t] + 1] != 0.0)
+        b->general = true;
+  }
+  int64_t max_items = 0;
+  for (int s = 0; s < n_stacks; ++s) max_items += (b->nn[s] + 63) / 64;
+  const size_t lb = align256(sizeof(ytf_layer) * nlt), ib = align256(sizeof(int) * n_stacks),
+               jb = align256(sizeof(int64_t) * n_stacks), tb = align256(sizeof(int) * (size_t)(max_items + 1));
+  if (hipSetDevice(c->device) != hipSuccess || hipMalloc(&b->dev, lb + 2 * ib + 2 * jb + 2 * tb + 256) != hipSuccess) {
+    delete b;
+    return fail(c, YTF_EHIP, "ytf_batch_create: device allocation failed");
+  }
+  char *d = (char *)b->dev;
+  b->L = (LayerDev *)d;
+  b->d_l0 = (int *)(d + lb);
+  b->d_nl = (int *)(d + lb + ib);
+  b->d_n0 = (int64_t *)(d + lb + 2 * ib);
+  b->d_nn = (int64_t *)(d + lb + 2 * ib + jb);
+  b->d_is = (int *)(d + lb + 2 * ib + 2 * jb);
+  b->d_if = (int *)(d + lb + 2 * ib + 2 * jb + tb);
+  b->counter = (unsigned int *)(d + lb + 2 * ib + 2 * jb + 2 * tb);
+  hipError_t e = hipMemcpy(b->L, layers, sizeof(ytf_layer) * nlt, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_l0, b->l0.data(), sizeof(int) * n_stacks, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_nl, b->nl.data(), sizeof(int) * n_stacks, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_n0, b->n0.data(), sizeof(int64_t) * n_stacks, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_nn, b->nn.data(), sizeof(int64_t) * n_stacks, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(b->dev);
+    delete b;
+    return fail(c, YTF_EHIP, "ytf_batch_create: upload failed: %s", hipGetErrorString(e));
+  }
+  int cus = 0;
+  if (device_cus(c, &cus)) {
+    hipFree(b->dev);
+    delete b;
+    return YTF_EHIP;
+  }
+  b->blocks = 4 * cus;  // one wave per SIMD: the kernel's register budget allows no more
+  std::ostringstream sig;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) sig << prop.gcnArchName << "/" << cus << "cu";
+  sig << "/s" << n_stacks << "/n" << b->ntot << "/w" << work << "/g" << (int)b->general;
+  b->signature = sig.str();
+  for (auto &ch : b->signature)
+    if (ch == ' ') ch = '_';
+  *out = b;
+  return YTF_OK;
+}
+
+int ytf_batch_destroy(ytf_ctx *c, ytf_batch *b) {
+  if (!b) return YTF_OK;
+  if (c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+  }
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->dev) hipFree(b->dev);
+  delete b;
+  return YTF_OK;
+}
+
+int ytf_batch_set_config(ytf_ctx *c, ytf_batch *b, int nodes_per_item, int waves_per_block, int blocks) {
+  if (!c || !b) return fail(c, YTF_EINVAL, "ytf_batch_set_config: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = batch_check_cfg(c, nodes_per_item, waves_per_block, blocks);
+  if (rc) return rc;
+  b->nodes_per_item = nodes_per_item;
+  b->waves_per_block = waves_per_block;
+  b->blocks = blocks;
+  return YTF_OK;
+}
+
+int ytf_batch_get_config(ytf_ctx *c, const ytf_batch *b, int *nodes_per_item, int *waves_per_block, int *blocks) {
+  if (!c || !b || !nodes_per_item || !waves_per_block || !blocks)
+    return fail(c, YTF_EINVAL, "ytf_batch_get_config: NULL argument");
+  *nodes_per_item = b->nodes_per_item;
+  *waves_per_block = b->waves_per_block;
+  *blocks = b->blocks;
+  return YTF_OK;
+}
+
+static int batch_solve_locked(ytf_ctx *c, ytf_batch *b, const double *kx, const double *ky, double *S, double *V1,
+                              double *Q1, double *VN, double *QN, uint8_t *flags, bool use_graph) {
+  if (b->ntot == 0) return YTF_OK;
+  if (!kx || !ky || !S) return fail(c, YTF_EINVAL, "ytf_batch_solve: kx, ky and S are required");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = batch_items(c, b);
+  if (rc) return rc;
+  const ytf::BatchLaunch B = batch_launch(b, kx, ky, S, V1, Q1, VN, QN, flags);
+  if (!use_graph) {
+    hipError_t e = timed(c, "batch_solve", [&] { return ytf::launch_batch(B, c->stream); });
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_batch_solve: launch failed: %s", hipGetErrorString(e));
+    return YTF_OK;
+  }
+  // one graph per argument set: (counter reset, persistent kernel)
+  std::vector<const void *> key = {kx, ky, S, V1, Q1, VN, QN, flags, (const void *)(intptr_t)b->nodes_per_item,
+                                   (const void *)(intptr_t)b->waves_per_block, (const void *)(intptr_t)b->blocks};
+  if (!b->exec || key != b->exec_key) {
+    if (b->exec) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipGraphExecDestroy(b->exec));
+      b->exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->own, hipStreamCaptureModeThreadLocal));
+    hipError_t e = ytf::launch_batch(B, c->own);
+    hipError_t e2 = hipStreamEndCapture(c->own, &graph);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return fail(c, YTF_EHIP, "ytf_batch_solve: graph capture failed: %s",
+                  hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    hipError_t e3 = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e3 != hipSuccess) {
+      b->exec = nullptr;
+      return fail(c, YTF_EHIP, "ytf_batch_solve: graph instantiation failed: %s", hipGetErrorString(e3));
+    }
+    b->exec_key = key;
+  }
+  hipError_t e = timed(c, "batch_solve", [&] { return hipGraphLaunch(b->exec, c->stream); });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_batch_solve: graph launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_batch_solve(ytf_ctx *c, ytf_batch *b, const double *kx_dev, const double *ky_dev, double *S_dev,
+                    double *V1_dev, double *Q1_dev, double *VN_dev, double *QN_dev, uint8_t *flags_dev) {
+  if (!c || !b) return fail(c, YTF_EINVAL, "ytf_batch_solve: NULL ctx or batch");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (b->device != c->device) return fail(c, YTF_EINVAL, "ytf_batch_solve: batch belongs to device %d", b->device);
+  return batch_solve_locked(c, b, kx_dev, ky_dev, S_dev, V1_dev, Q1_dev, VN_dev, QN_dev, flags_dev, true);
+}
+
+// Autotuner: time every candidate (nodes per item x waves per block x grid
+// size) on this batch's own inputs, keep the fastest.  With a tune-DB file, a
+// stored choice for the same batch signature is reused without timing, and a
+// new choice is appended.
+int ytf_batch_tune(ytf_ctx *c, ytf_batch *b, const double *kx_dev, const double *ky_dev, double *S_dev,
+                   uint8_t *flags_dev, const char *tune_db, char *report, int report_len) {
+  if (!c || !b) return fail(c, YTF_EINVAL, "ytf_batch_tune: NULL ctx or batch");
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto say = [&](const std::string &m) {
+    if (report && report_len > 0) {
+      std::strncpy(report, m.c_str(), report_len - 1);
+      report[report_len - 1] = 0;
+    }
+  };
+  if (tune_db && *tune_db) {
+    std::ifstream f(tune_db);
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream ls(line);
+      std::string sig;
+      int npi, wpb, blk;
+      double ms;
+      if (ls >> sig >> npi >> wpb >> blk >> ms && sig == b->signature && !batch_check_cfg(c, npi, wpb, blk)) {
+        b->nodes_per_item = npi;
+        b->waves_per_block = wpb;
+        b->blocks = blk;
+        say("db " + b->signature + " " + std::to_string(npi) + " " + std::to_string(wpb) + " " + std::to_string(blk));
+        return YTF_OK;
+      }
+    }
+  }
+  if (b->ntot == 0) return YTF_OK;
+  int cus = 0;
+  int rc = device_cus(c, &cus);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  double best = 1e300;
+  int bn = b->nodes_per_item, bw = b->waves_per_block, bb = b->blocks;
+  const int npis[4] = {64, 128, 256, 512};
+  const int wpbs[3] = {1, 2, 4};
+  std::ostringstream log;
+  for (int npi : npis)
+    for (int wpb : wpbs)
+      for (int over = 1; over <= 2; ++over) {
+        b->nodes_per_item = npi;
+        b->waves_per_block = wpb;
+        b->blocks = over * 4 * cus / wpb;
+        double ts[3];
+        for (int r = 0; r < 4; ++r) {
+          hipEventRecord(e0, c->stream);
+          rc = batch_solve_locked(c, b, kx_dev, ky_dev, S_dev, nullptr, nullptr, nullptr, nullptr, flags_dev, false);
+          if (rc) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            return rc;
+          }
+          hipEventRecord(e1, c->stream);
+          hipEventSynchronize(e1);
+          float ms = 0;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (r) ts[r - 1] = ms;
+        }
+        std::sort(ts, ts + 3);
+        log << npi << "/" << wpb << "/" << b->blocks << ":" << ts[1] << " ";
+        if (ts[1] < best) {
+          best = ts[1];
+          bn = npi;
+          bw = wpb;
+          bb = b->blocks;
+        }
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  b->nodes_per_item = bn;
+  b->waves_per_block = bw;
+  b->blocks = bb;
+  if (tune_db && *tune_db) {
+    std::ofstream f(tune_db, std::ios::app);
+    if (f) f << b->signature << " " << bn << " " << bw << " " << bb << " " << best << "\n";
+  }
+  std::ostringstream m;
+  m << "tuned " << b->signature << " best " << bn << " " << bw << " " << bb << " " << best << " ms; " << log.str();
+  say(m.str());
+  return YTF_OK;
+}
 
 // Radial synthesis into R (device, [np][9][nr] complex); the caller holds the lock.
 static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const double *kx, const double *wt,

@@ -72,4 +72,23 @@ hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const
                               hipStream_t s);
 hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *cexp, cd *R, hipStream_t s);
 
+// Uneven batch of stacks (batch.hip): total S-matrix + end-layer modes per
+// node, one persistent launch (counter reset + kernel).
+struct BatchLaunch {
+  const LayerDev *L;
+  const int *stack_l0, *stack_nl;
+  const int64_t *stack_n0, *stack_nn;
+  const int *item_stack, *item_first;
+  int n_items, nodes_per_item;
+  double omega;
+  const double *kx, *ky;
+  cd *S, *V1, *Q1, *VN, *QN;
+  unsigned char *flags;
+  int64_t ntot;
+  unsigned int *counter;
+  bool general;
+  int waves_per_block, blocks;
+};
+hipError_t launch_batch(const BatchLaunch &B, hipStream_t s);
+
 }  // namespace ytf

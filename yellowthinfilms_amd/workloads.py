@@ -19,12 +19,19 @@ bessel_gemm: the same stack; 16384 k-rho nodes (14336 in the band + a
   2048-node tail), 8192 radii in (0, 2], 64 depth pairs (8 source heights x 8
   observation heights in the air): the J0/J1/J2 contraction as a real x complex
   GEMM on the matrix cores with split fp16 products (mode solves FP64).
+
+uneven_256: 256 independent stacks, each with U{2..24} layers of thickness
+  U[5, 200] nm and random hyperbolic diagonal permittivities (each diagonal
+  entry +-U[1, 6] + i U[0.01, 0.5], at least one negative entry per stack),
+  and U{1024..16384} nodes kt = k_rho (cos phi, sin phi), k_rho ~ U[0, 8 k0],
+  phi ~ U[0, 2 pi): modes + total S-matrix per node, FP64, one persistent launch.
 """
 import math
 
 import numpy as np
 
-from .materials import IsotropicMaterial, UniaxialMaterial, GyrotropicMaterial, FlatOpticalLayer
+from .materials import (IsotropicMaterial, UniaxialMaterial, BiaxialMaterial, GyrotropicMaterial,
+                        FlatOpticalLayer)
 
 NM = 1.0 / 600.0
 OMEGA = 2 * math.pi
@@ -106,6 +113,43 @@ def bessel_gemm_geometry(wl=BESSEL_GEMM):
     zo = np.tile([1.0 - h * NM for h in wl["obs_heights_nm"]], len(wl["src_heights_nm"]))
     rho = wl["rho_max"] * np.arange(1, wl["n_rho"] + 1) / wl["n_rho"]
     return zs, zo, rho
+
+
+UNEVEN_256 = dict(
+    name="uneven_256",
+    description=("256 independent stacks, U{2..24} layers of U[5,200] nm with random hyperbolic diagonal eps "
+                 "(+-U[1,6] + i U[0.01,0.5]), U{1024..16384} nodes each (k_rho ~ U[0,8k0], phi ~ U[0,2pi)): "
+                 "Berreman modes + S-matrix cascade to the total S per node, FP64, one persistent launch"),
+    n_stacks=256, layers=(2, 24), d_nm=(5.0, 200.0), eps_abs=(1.0, 6.0), eps_im=(0.01, 0.5),
+    nodes=(1024, 16384), kmax_k0=8.0, seed=1,
+)
+
+
+def uneven_batch(wl=UNEVEN_256):
+    """(stacks, node_counts, kx, ky) of the uneven batch, seeded."""
+    rng = np.random.default_rng(wl["seed"])
+    stacks, counts = [], []
+    for _ in range(wl["n_stacks"]):
+        nl = int(rng.integers(wl["layers"][0], wl["layers"][1] + 1))
+        sg = rng.choice([-1.0, 1.0], size=(nl, 3))
+        if not (sg < 0).any():
+            sg[rng.integers(nl), rng.integers(3)] = -1.0
+        re = sg * rng.uniform(*wl["eps_abs"], size=(nl, 3))
+        im = rng.uniform(*wl["eps_im"], size=(nl, 3))
+        d = rng.uniform(*wl["d_nm"], size=nl) * NM
+        stacks.append([FlatOpticalLayer(BiaxialMaterial(*(re[l] + 1j * im[l])), d[l]) for l in range(nl)])
+        counts.append(int(rng.integers(wl["nodes"][0], wl["nodes"][1] + 1)))
+    n = sum(counts)
+    kr = rng.uniform(0.0, wl["kmax_k0"] * OMEGA, n)
+    ph = rng.uniform(0.0, 2 * math.pi, n)
+    return stacks, counts, kr * np.cos(ph), kr * np.sin(ph)
+
+
+# Algorithmic FP64 operations of one uneven_256 launch: per stack, the mean
+# op count (real adds + multiplies + divides) of Modes_Stack + the total S in
+# the reference's formulation over 32 seeded sample nodes (rng seed 0),
+# times the stack's node count.  Pinned by tests/test_workloads.py.
+UNEVEN_256_FLOPS_PER_LAUNCH = 2.8317e11
 
 
 BATCHED_GYRO_512 = dict(
