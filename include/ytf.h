@@ -78,7 +78,7 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * returns the summed time (ms) and launch count of one kernel name:
  * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
  * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
- * "mfma_combine", "batch_solve". */
+ * "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam". */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
 int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
@@ -91,6 +91,13 @@ int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_
  *   basis_dev [16] obs-layer basis Exf Eyf Ezf Exb Eyb Ezb KZf KZb (2 modes each)
  *   flags_dev [1]  uint8 YTF_NODE_* bits
  * (CavityTransferCache1D/2D fields, mod_CavityTransfer.jl:160-178, 273-294.) */
+/* Term classes of the spectral dyadic (_term_amplitudes). */
+#define YTF_TERMS_ALL 0       /* direct (same layer) + scattered            */
+#define YTF_TERMS_DIRECT 1    /* the same-layer direct term only            */
+#define YTF_TERMS_SCATTERED 2 /* everything but the direct term             */
+#define YTF_TERMS_TOEPLITZ 3  /* direct + f->f and b->b reflected terms     */
+#define YTF_TERMS_HANKEL 4    /* the cross-branch reflected terms b->f, f->b */
+
 typedef struct ytf_node_cache {
   double *Qsrc_dev;
   double *sf_dev;
@@ -119,10 +126,14 @@ typedef struct ytf_node_cache {
  *     as closed_form_G's z_obs);
  *   ghat_dev: [n_pairs][9][n_nodes] complex, entry c + 3q = field component c
  *     of unit moment q, or NULL;
- *   cache: device outputs (or NULL). */
+ *   cache: device outputs (or NULL);
+ *   terms: YTF_TERMS_* class of g~ (closed_form_G's `terms`,
+ *     mod_ClosedFormField.jl:143-160); DIRECT, TOEPLITZ and HANKEL need
+ *     obs_layer == source_layer. */
 int ytf_green_nodes(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
                     const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
-                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache);
+                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                    int terms);
 
 /* Spectral convolution on the uniform DFT grid (the "G(kx,ky) F{J}" route of
  * mod_UniformSource.jl:229-248):
@@ -201,6 +212,39 @@ int ytf_batch_get_config(ytf_ctx *ctx, const ytf_batch *batch, int *nodes_per_it
  * optional buffer for a one-line summary. */
 int ytf_batch_tune(ytf_ctx *ctx, ytf_batch *batch, const double *kx_dev, const double *ky_dev, double *S_dev,
                    uint8_t *flags_dev, const char *tune_db, char *report, int report_len);
+
+/* Learned surrogate of the spectral dyadic (no counterpart in the reference,
+ * which evaluates every node with the exact solver; that solve remains the
+ * fallback and the judge).  A fully connected network n_in -> width x depth
+ * (SiLU) -> n_out, trained and evaluated on the matrix cores in BF16 with FP32
+ * accumulation (FP32 master weights, Adam).  Inputs and targets are raw FP32
+ * rows (x[n][n_in], y[n][n_out], device); the network sees
+ * (x - in_shift) * in_scale and predicts (y - out_shift) / out_scale.
+ * Parameters in flat order: for each layer W [n_out_l][n_in_l] row-major, then
+ * b [n_out_l]. */
+typedef struct ytf_mlp ytf_mlp;
+/* flags: YTF_MLP_KZ_FEATURES -- the first layer also sees Re and Im of
+ * sqrt(1 - x0^2 - x1^2) (kz / omega of the air when x0, x1 are kx / omega,
+ * ky / omega), so its weights are [width][n_in + 2]. */
+#define YTF_MLP_KZ_FEATURES 1
+int ytf_mlp_create(ytf_ctx *ctx, int n_in, int width, int depth, int n_out, int flags, uint64_t seed, ytf_mlp **out);
+int ytf_mlp_destroy(ytf_ctx *ctx, ytf_mlp *mlp);
+int64_t ytf_mlp_num_params(const ytf_mlp *mlp);
+int ytf_mlp_get_params(ytf_ctx *ctx, ytf_mlp *mlp, float *params_host);
+/* Also resets the optimizer state. */
+int ytf_mlp_set_params(ytf_ctx *ctx, ytf_mlp *mlp, const float *params_host);
+int ytf_mlp_set_normalization(ytf_ctx *ctx, ytf_mlp *mlp, const float *in_shift, const float *in_scale,
+                              const float *out_shift, const float *out_scale);
+/* Inference: y_dev[n][n_out] = network(x_dev[n][n_in]). */
+int ytf_mlp_forward(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, float *y_dev);
+/* Mean-squared loss of the normalised outputs over the batch and its gradient
+ * (flat order, host), without an update (synchronizes). */
+int ytf_mlp_loss_grad(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, const float *y_dev, float *loss_host,
+                      float *grad_host);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8) on the batch; the batch loss
+ * (before the step) is copied to loss_dev when not NULL. */
+int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, const float *y_dev, float lr,
+                       float *loss_dev);
 
 /* Library version string. */
 const char *ytf_version(void);

@@ -173,7 +173,10 @@ def ghat_from_cache(cache, obs_layer, zsrc, zobs, nthreads=0):
     return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
 
 
-def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0):
+TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}
+
+
+def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0, terms="all"):
     kx = np.ascontiguousarray(kx, np.float64)
     ky = np.ascontiguousarray(ky, np.float64)
     zsrc = np.ascontiguousarray(np.atleast_1d(zsrc), np.float64)
@@ -182,7 +185,7 @@ def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0):
     g = np.zeros(nn * npairs * 9, np.complex128)
     lib().ora_ghat_batch(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), ct.c_int(src),
                          ct.c_int(obs_layer), ct.c_int(npairs), _p(zsrc), _p(zobs), _p(g.view(np.float64)),
-                         ct.c_int(nthreads))
+                         ct.c_int(nthreads), ct.c_int(TERMS[terms]))
     return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
 
 

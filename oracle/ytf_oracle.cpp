@@ -1060,8 +1060,10 @@ template <class R> inline cx<R> finite_or_zero(cx<R> z) { return isfin(z) ? z : 
 //   g[c,q] = sum_m E_c,f fz(Af_r Phi_f + Afd) + E_c,b fz(Ab_r Phi_b + Abd),
 // with amplitude_at (:539-566), direct_amplitude (:577-602, point source),
 // _obs_phases (:326-335); zpos convention :72-73.  Output g[c + 3q].
+// terms (_term_amplitudes, :143-160): 0 :all, 1 :direct, 2 :scattered,
+// 3 :toeplitz (direct + f->f, b->b reflected), 4 :hankel (b->f, f->b).
 template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_layer, int src_layer0, double d_src,
-                                  double d_obs, double zsrc_local, double zobs, cx<R> *g) {
+                                  double d_obs, double zsrc_local, double zobs, cx<R> *g, int terms = 0) {
   for (int i = 0; i < 9; ++i) g[i] = C<R>(0.0);
   double zpos = src_layer0 == 0 ? d_src - zsrc_local : zsrc_local;
   cx<R> Af[2][3], Ab[2][3], Afd[2][3], Abd[2][3];
@@ -1074,18 +1076,33 @@ template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_l
       Pb[m] = cexp(-times_i(nd.Qb[m]) * Cr(R(zpos)));
     }
     const Mat<R, 4, 4> &T = nd.T[obs_idx];
+    // seed columns feeding the forward (rows 0-1) and backward (rows 2-3)
+    // amplitudes: seeds=:f keeps columns 0-1, seeds=:b columns 2-3
+    bool useF[4] = {true, true, true, true}, useB[4] = {true, true, true, true};
+    if (terms == 1)
+      for (int k = 0; k < 4; ++k) useF[k] = useB[k] = false;
+    if (terms == 3)
+      for (int k = 0; k < 4; ++k) { useF[k] = k < 2; useB[k] = k >= 2; }
+    if (terms == 4)
+      for (int k = 0; k < 4; ++k) { useF[k] = k >= 2; useB[k] = k < 2; }
     for (int q = 0; q < 3; ++q) {
       cx<R> u[4] = {nd.sf(0, q) * Pf[0], nd.sf(1, q) * Pf[1], nd.sb(0, q) * Pb[0], nd.sb(1, q) * Pb[1]};
       cx<R> v[4];
       for (int i = 0; i < 4; ++i) {
-        cx<R> s = T(i, 0) * u[0];
-        for (int k = 1; k < 4; ++k) s = s + T(i, k) * u[k];
+        const bool *use = i < 2 ? useF : useB;
+        cx<R> s = C<R>(0.0);
+        bool first = true;
+        for (int k = 0; k < 4; ++k)
+          if (use[k]) {
+            s = first ? T(i, k) * u[k] : s + T(i, k) * u[k];
+            first = false;
+          }
         v[i] = s;
       }
       Af[0][q] = v[0]; Af[1][q] = v[1]; Ab[0][q] = v[2]; Ab[1][q] = v[3];
     }
   }
-  if (same_layer && nd.valid) {
+  if (same_layer && nd.valid && (terms == 0 || terms == 1 || terms == 3)) {
     double zeta = zobs - zpos;
     bool fwd = zeta > 0, bwd = zeta < 0;
     for (int m = 0; m < 2; ++m) {
@@ -1318,7 +1335,7 @@ int ora_ghat_from_cache(int n, const double *d, int src, const int *obs, int nob
 // Fused node solve + spectral dyadic, as timed for the CPU baseline.
 int ora_ghat_batch(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
                    double omega, const double *kx, const double *ky, int64_t nn, int src, int obs_layer, int npairs,
-                   const double *zsrc, const double *zobs, double *g, int nthreads) {
+                   const double *zsrc, const double *zobs, double *g, int nthreads, int terms) {
   auto st = make_stack<double>(n, eps, mu, d, ek, mk);
   std::vector<int> ob(1, obs_layer - 1);
 #ifdef _OPENMP
@@ -1330,7 +1347,7 @@ int ora_ghat_batch(int n, const double *eps, const double *mu, const double *d, 
     node_solve(st, omega, kx[k], ky[k], src - 1, ob, nd);
     for (int p = 0; p < npairs; ++p) {
       cx<double> gg[9];
-      node_ghat(nd, 0, obs_layer == src, src - 1, d[src - 1], d[obs_layer - 1], zsrc[p], zobs[p], gg);
+      node_ghat(nd, 0, obs_layer == src, src - 1, d[src - 1], d[obs_layer - 1], zsrc[p], zobs[p], gg, terms);
       for (int i = 0; i < 9; ++i) put(g + 2 * (9 * (npairs * k + p) + i), gg[i]);
     }
   }

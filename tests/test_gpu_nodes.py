@@ -159,3 +159,27 @@ def test_spectral_apply_full_512_grid():
     assert relerr(E, po.spectral_apply(g, J, grid)) < 1e-12
     E2 = op.apply(2 * Jt + 1j * Jt).cpu().numpy().transpose(2, 1, 0)
     assert relerr(E2, (2 + 1j) * E) < 1e-13
+
+
+@pytest.mark.parametrize("label", ["iso 5-layer lossy", "gyrotropic 4-layer"])
+def test_term_classes_against_oracle(label):
+    """terms = all / direct / scattered / toeplitz / hankel (closed_form_G's
+    term classes) in the source layer, and the same-layer requirement."""
+    from yellowthinfilms_amd import green_nodes
+    from yellowthinfilms_amd._lib import YtfError
+    stack = FOUNDATION_STACKS[label]
+    rng = np.random.default_rng(21)
+    kx, ky = rng.uniform(-9, 9, 48), rng.uniform(-9, 9, 48)
+    for src in (1, 2):
+        d = stack[src - 1].d
+        zs, zo = [0.3 * d, 0.6 * d], [0.7 * d, 0.2 * d]
+        for t in ("all", "direct", "scattered", "toeplitz", "hankel"):
+            print(f"case: {label} src={src} terms={t}", flush=True)
+            _, g = green_nodes(stack, W0, kx, ky, src, src, zs, zo, cache=False, terms=t)
+            gg = g.cpu().numpy().reshape(2, 3, 3, -1).transpose(3, 0, 2, 1)
+            og = po.ghat_batch(ostack(stack), W0, kx, ky, src, src, zs, zo, terms=t)
+            assert relerr(gg, og) < 1e-10, (t, relerr(gg, og))
+    with pytest.raises(YtfError):
+        green_nodes(stack, W0, kx, ky, 1, 2, [0.1], [0.1], cache=False, terms="direct")
+    with pytest.raises(ValueError):
+        green_nodes(stack, W0, kx, ky, 1, 1, [0.1], [0.1], cache=False, terms="reflected")

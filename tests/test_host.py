@@ -77,7 +77,7 @@ def test_argument_validation_without_gpu():
     L = _lib.lib()
     st = layer_structs(c2_stack())
     z = (ct.c_double * 1)(0.0)
-    rc = L.ytf_green_nodes(None, st, 5, 2 * math.pi, None, None, 0, 1, 1, 0, z, z, None, None)
+    rc = L.ytf_green_nodes(None, st, 5, 2 * math.pi, None, None, 0, 1, 1, 0, z, z, None, None, 0)
     assert rc == _lib.YTF_EINVAL
     assert b"ctx is NULL" in L.ytf_last_error(None)
     rc = L.ytf_spectral_apply(None, 512, 512, None, None, None, None, 1.0)

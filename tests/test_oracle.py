@@ -422,3 +422,38 @@ def test_op_count_is_stable_and_plausible():
     f = lambda c: c["solve"]["add"] + c["solve"]["mul"] + c["solve"]["div"]
     assert 1e4 < f(c1) < 1e5
     assert abs(f(c1) - f(c2)) / f(c1) < 0.1
+
+
+# ---------------------------------------------------------------------------
+# Term classes of the spectral dyadic (_term_amplitudes,
+# mod_ClosedFormField.jl:143-160): direct + scattered = all and
+# toeplitz + hankel = all on any medium; no scattering in a homogeneous stack.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("label", ["iso 5-layer lossy", "gyrotropic 4-layer", "biaxial 4-layer"])
+def test_term_classes_add_up(label):
+    stack = FOUNDATION_STACKS[label]
+    st = ostack(stack)
+    rng = np.random.default_rng(11)
+    kx, ky = rng.uniform(-9, 9, 40), rng.uniform(-9, 9, 40)
+    src = 2
+    d = stack[src - 1].d
+    zs, zo = [0.3 * d, 0.6 * d], [0.7 * d, 0.2 * d]
+    g = {t: po.ghat_batch(st, W0, kx, ky, src, src, zs, zo, terms=t)
+         for t in ("all", "direct", "scattered", "toeplitz", "hankel")}
+    assert relerr(g["direct"] + g["scattered"], g["all"]) < 1e-13
+    assert relerr(g["toeplitz"] + g["hankel"], g["all"]) < 1e-13
+    assert relerr(g["scattered"], g["all"]) > 1e-3  # both classes are present
+    # in another layer there is no direct term
+    g2 = po.ghat_batch(st, W0, kx, ky, src, 1, zs, [0.5, 0.6], terms="scattered")
+    g3 = po.ghat_batch(st, W0, kx, ky, src, 1, zs, [0.5, 0.6], terms="all")
+    assert np.array_equal(g2, g3)
+
+
+def test_homogeneous_stack_has_no_scattered_term():
+    ef = 2.0 + 0.05j
+    st = ostack([lay(IsotropicMaterial(ef), 1.0) for _ in range(3)])
+    rng = np.random.default_rng(12)
+    kx, ky = rng.uniform(-9, 9, 30), rng.uniform(-9, 9, 30)
+    g = po.ghat_batch(st, W0, kx, ky, 2, 2, [0.3], [0.7], terms="all")
+    gs = po.ghat_batch(st, W0, kx, ky, 2, 2, [0.3], [0.7], terms="scattered")
+    assert np.linalg.norm(gs) < 1e-14 * np.linalg.norm(g)

@@ -9,7 +9,9 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libytf.so")
+# YTF_LIBRARY: another build of the same library (same C-ABI), e.g. one made
+# with different compiler options
+LIB_PATH = os.environ.get("YTF_LIBRARY") or os.path.join(HERE, "libytf.so")
 
 YTF_OK, YTF_EINVAL, YTF_EHIP = 0, 1, 2
 NODE_VALID, NODE_TVALID, NODE_SINGULAR, NODE_NOMODES = 1, 2, 4, 8
@@ -46,7 +48,7 @@ _SIGS = {
     "ytf_green_nodes": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_void_p,
                                    ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
-                                   ct.POINTER(YtfNodeCache)]),
+                                   ct.POINTER(YtfNodeCache), ct.c_int]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
@@ -64,8 +66,22 @@ _SIGS = {
                                         ct.POINTER(ct.c_int)]),
     "ytf_batch_tune": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                   ct.c_char_p, ct.c_char_p, ct.c_int]),
+    "ytf_mlp_create": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_uint64,
+                                  ct.POINTER(ct.c_void_p)]),
+    "ytf_mlp_destroy": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_mlp_num_params": (ct.c_int64, [ct.c_void_p]),
+    "ytf_mlp_get_params": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_float)]),
+    "ytf_mlp_set_params": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_float)]),
+    "ytf_mlp_set_normalization": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_float), ct.POINTER(ct.c_float),
+                                             ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
+    "ytf_mlp_forward": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p]),
+    "ytf_mlp_loss_grad": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+                                     ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
+    "ytf_mlp_train_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_float,
+                                      ct.c_void_p]),
 }
 
+TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}
 SYNTH_MODES = {"fp64": 0, "f16x3": 1}
 
 EXPORTS = tuple(_SIGS)

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <random>
 #include <sstream>
 #include <map>
 #include <mutex>
@@ -217,7 +218,8 @@ int ytf_ctx_synchronize(ytf_ctx *c) {
 
 int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
                     const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
-                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache) {
+                    const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                    int terms) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_green_nodes: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
   if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_green_nodes: need at least 2 layers (got %d)", n_layers);
@@ -228,6 +230,14 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   if (obs_layer < 1 || obs_layer > n_layers)
     return fail(c, YTF_EINVAL, "ytf_green_nodes: obs_layer=%d out of range 1:%d", obs_layer, n_layers);
   if (n_nodes < 0) return fail(c, YTF_EINVAL, "ytf_green_nodes: n_nodes < 0");
+  if (terms < YTF_TERMS_ALL || terms > YTF_TERMS_HANKEL)
+    return fail(c, YTF_EINVAL, "closed_form_G: terms must be :all, :direct, :scattered, :toeplitz or :hankel (got %d)",
+                terms);
+  if ((terms == YTF_TERMS_DIRECT || terms == YTF_TERMS_TOEPLITZ || terms == YTF_TERMS_HANKEL) &&
+      obs_layer != source_layer)
+    return fail(c, YTF_EINVAL, "closed_form_G: terms=:%s needs obs_layer == source_layer (%d), got %d",
+                terms == YTF_TERMS_DIRECT ? "direct" : terms == YTF_TERMS_TOEPLITZ ? "toeplitz" : "hankel",
+                source_layer, obs_layer);
   if (n_nodes > 0 && (!kx_dev || !ky_dev)) return fail(c, YTF_EINVAL, "ytf_green_nodes: kx/ky are NULL");
   if (n_pairs < 0 || (n_pairs > 0 && (!zsrc || !zobs))) return fail(c, YTF_EINVAL, "ytf_green_nodes: bad depth pairs");
   if (ghat_dev && n_pairs == 0) return fail(c, YTF_EINVAL, "ytf_green_nodes: ghat requested with no depth pairs");
@@ -273,6 +283,7 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   P.src = source_layer - 1;
   P.obs = obs_layer - 1;
   P.npairs = n_pairs;
+  P.terms = terms;
   P.zsrc = (const double *)(sm + lb);
   P.zobs = (const double *)(sm + lb + zb);
   P.ghat = (cd *)ghat_dev;
@@ -704,6 +715,381 @@ int ytf_batch_tune(ytf_ctx *c, ytf_batch *b, const double *kx_dev, const double 
 }
 
 // Radial synthesis into R (device, [np][9][nr] complex); the caller holds the lock.
+
+// ---------------------------------------------------------------------------
+// Surrogate network (mlp.hip)
+// ---------------------------------------------------------------------------
+struct ytf_mlp {
+  int device = 0;
+  int nin = 0, width = 0, depth = 0, nout = 0, nl = 0;
+  int kzf = 0;  // two light-line features follow the inputs
+  std::vector<int> K, N, Kp, Np;       // real and padded (to 64) layer shapes
+  std::vector<size_t> offW, offb;      // offsets in the padded parameter array
+  std::vector<size_t> foffW, foffb;    // offsets in the flat (unpadded) layout
+  size_t np = 0, nflat = 0;
+  float *P = nullptr, *Mo = nullptr, *Vo = nullptr, *G = nullptr;  // FP32 master, Adam moments, gradient
+  __bf16 *P16 = nullptr;                                            // bf16 copy of P
+  float *norm = nullptr;  // in_shift[nin] in_scale[nin] out_shift[nout] out_scale[nout]
+  float *loss = nullptr;
+  long step = 0;
+  // activation workspace
+  void *act = nullptr;
+  size_t act_bytes = 0;
+};
+
+static size_t ru64(size_t x) { return (x + 63) / 64 * 64; }
+
+static int mlp_free(ytf_mlp *m) {
+  void *ps[] = {m->P, m->Mo, m->Vo, m->G, m->P16, m->norm, m->loss, m->act};
+  for (void *q : ps)
+    if (q) hipFree(q);
+  delete m;
+  return YTF_OK;
+}
+
+int ytf_mlp_create(ytf_ctx *c, int n_in, int width, int depth, int n_out, int flags, uint64_t seed, ytf_mlp **out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_mlp_create: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int kzf = (flags & YTF_MLP_KZ_FEATURES) ? 1 : 0;
+  if (!out || n_in < 1 || n_in + 2 * kzf > 64 || (kzf && n_in < 2) || width < 64 || width > 4096 || width % 64 ||
+      depth < 1 || depth > 32 || n_out < 1 || n_out > 4096 || (flags & ~YTF_MLP_KZ_FEATURES))
+    return fail(c, YTF_EINVAL,
+                "ytf_mlp_create: bad shape (n_in=%d, width=%d (multiple of 64), depth=%d, n_out=%d, flags=%d)", n_in,
+                width, depth, n_out, flags);
+  ytf_mlp *m = new ytf_mlp();
+  m->device = c->device;
+  m->kzf = kzf;
+  m->nin = n_in;
+  m->width = width;
+  m->depth = depth;
+  m->nout = n_out;
+  m->nl = depth + 1;
+  for (int l = 0; l < m->nl; ++l) {
+    const int k = l == 0 ? n_in + 2 * kzf : width, n = l == m->nl - 1 ? n_out : width;
+    m->K.push_back(k);
+    m->N.push_back(n);
+    m->Kp.push_back((int)ru64(k));
+    m->Np.push_back((int)ru64(n));
+    m->offW.push_back(m->np);
+    m->np += (size_t)m->Np[l] * m->Kp[l];
+    m->offb.push_back(m->np);
+    m->np += m->Np[l];
+    m->foffW.push_back(m->nflat);
+    m->nflat += (size_t)n * k;
+    m->foffb.push_back(m->nflat);
+    m->nflat += n;
+  }
+  hipError_t e = hipSetDevice(c->device);
+  const size_t fb = sizeof(float) * m->np;
+  if (e == hipSuccess) e = hipMalloc(&m->P, fb);
+  if (e == hipSuccess) e = hipMalloc(&m->Mo, fb);
+  if (e == hipSuccess) e = hipMalloc(&m->Vo, fb);
+  if (e == hipSuccess) e = hipMalloc(&m->G, fb);
+  if (e == hipSuccess) e = hipMalloc(&m->P16, sizeof(__bf16) * m->np);
+  if (e == hipSuccess) e = hipMalloc(&m->norm, sizeof(float) * 2 * (n_in + n_out));
+  if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(float));
+  if (e != hipSuccess) {
+    mlp_free(m);
+    return fail(c, YTF_EHIP, "ytf_mlp_create: allocation failed: %s", hipGetErrorString(e));
+  }
+  // Xavier-uniform weights, zero biases, zero padding; identity normalisation
+  std::vector<float> h(m->np, 0.0f);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l < m->nl; ++l) {
+    const float a = std::sqrt(6.0f / (float)(m->K[l] + m->N[l]));
+    std::uniform_real_distribution<float> U(-a, a);
+    for (int i = 0; i < m->N[l]; ++i)
+      for (int j = 0; j < m->K[l]; ++j) h[m->offW[l] + (size_t)i * m->Kp[l] + j] = U(rng);
+  }
+  std::vector<float> nrm(2 * (n_in + n_out), 0.0f);
+  for (int i = 0; i < n_in; ++i) nrm[n_in + i] = 1.0f;
+  for (int i = 0; i < n_out; ++i) nrm[2 * n_in + n_out + i] = 1.0f;
+  std::vector<float> zero(m->np, 0.0f);
+  e = hipMemcpy(m->P, h.data(), fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->Mo, zero.data(), fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->Vo, zero.data(), fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->G, zero.data(), fb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(m->norm, nrm.data(), sizeof(float) * nrm.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = ytf::launch_mlp_to_bf16((int64_t)m->np, m->P, m->P16, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    mlp_free(m);
+    return fail(c, YTF_EHIP, "ytf_mlp_create: initialisation failed: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return YTF_OK;
+}
+
+int ytf_mlp_destroy(ytf_ctx *c, ytf_mlp *m) {
+  if (!m) return YTF_OK;
+  if (c) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+  }
+  return mlp_free(m);
+}
+
+int64_t ytf_mlp_num_params(const ytf_mlp *m) { return m ? (int64_t)m->nflat : 0; }
+
+// flat (unpadded) <-> padded parameter layouts, on the host
+int ytf_mlp_get_params(ytf_ctx *c, ytf_mlp *m, float *host) {
+  if (!c || !m || !host) return fail(c, YTF_EINVAL, "ytf_mlp_get_params: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<float> h(m->np);
+  HIPCHK(c, hipMemcpy(h.data(), m->P, sizeof(float) * m->np, hipMemcpyDeviceToHost));
+  for (int l = 0; l < m->nl; ++l) {
+    for (int i = 0; i < m->N[l]; ++i)
+      for (int j = 0; j < m->K[l]; ++j) host[m->foffW[l] + (size_t)i * m->K[l] + j] = h[m->offW[l] + (size_t)i * m->Kp[l] + j];
+    for (int i = 0; i < m->N[l]; ++i) host[m->foffb[l] + i] = h[m->offb[l] + i];
+  }
+  return YTF_OK;
+}
+
+int ytf_mlp_set_params(ytf_ctx *c, ytf_mlp *m, const float *host) {
+  if (!c || !m || !host) return fail(c, YTF_EINVAL, "ytf_mlp_set_params: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<float> h(m->np, 0.0f), zero(m->np, 0.0f);
+  for (int l = 0; l < m->nl; ++l) {
+    for (int i = 0; i < m->N[l]; ++i)
+      for (int j = 0; j < m->K[l]; ++j) h[m->offW[l] + (size_t)i * m->Kp[l] + j] = host[m->foffW[l] + (size_t)i * m->K[l] + j];
+    for (int i = 0; i < m->N[l]; ++i) h[m->offb[l] + i] = host[m->foffb[l] + i];
+  }
+  HIPCHK(c, hipMemcpy(m->P, h.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(m->Mo, zero.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(m->Vo, zero.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
+  m->step = 0;
+  hipError_t e = ytf::launch_mlp_to_bf16((int64_t)m->np, m->P, m->P16, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_set_params: %s", hipGetErrorString(e));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return YTF_OK;
+}
+
+int ytf_mlp_set_normalization(ytf_ctx *c, ytf_mlp *m, const float *in_shift, const float *in_scale,
+                              const float *out_shift, const float *out_scale) {
+  if (!c || !m || !in_shift || !in_scale || !out_shift || !out_scale)
+    return fail(c, YTF_EINVAL, "ytf_mlp_set_normalization: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  std::vector<float> h;
+  h.insert(h.end(), in_shift, in_shift + m->nin);
+  h.insert(h.end(), in_scale, in_scale + m->nin);
+  h.insert(h.end(), out_shift, out_shift + m->nout);
+  h.insert(h.end(), out_scale, out_scale + m->nout);
+  for (int i = 0; i < m->nout; ++i)
+    if (!(out_scale[i] != 0.0f) || !std::isfinite(out_scale[i]))
+      return fail(c, YTF_EINVAL, "ytf_mlp_set_normalization: out_scale[%d] must be finite and nonzero", i);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(m->norm, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  return YTF_OK;
+}
+
+// Activation workspace for `rows` rows (training keeps every layer's Z and H).
+struct MlpAct {
+  float *part;  // split-K partial sums of the weight gradients
+  __bf16 *X0;
+  std::vector<float *> Z;
+  std::vector<__bf16 *> H;
+  float *Y;
+  __bf16 *D[2];
+};
+
+static int mlp_workspace(ytf_ctx *c, ytf_mlp *m, int64_t rows, bool train, MlpAct &A) {
+  std::vector<size_t> sz;
+  int maxw = 0;
+  for (int l = 0; l < m->nl; ++l) maxw = std::max(maxw, std::max(m->Np[l], m->Kp[l]));
+  const size_t part_b = train ? align256(sizeof(float) * 64 * (size_t)maxw * maxw) : 0;
+  sz.push_back(align256(sizeof(__bf16) * rows * m->Kp[0]));
+  const int nh = m->nl - 1;
+  for (int l = 0; l < nh; ++l) sz.push_back(train ? align256(sizeof(float) * rows * m->Np[l]) : 0);
+  for (int l = 0; l < nh; ++l) sz.push_back(train || l < 2 ? align256(sizeof(__bf16) * rows * m->Np[l]) : 0);
+  sz.push_back(align256(sizeof(float) * rows * m->Np[m->nl - 1]));
+  for (int i = 0; i < 2; ++i) sz.push_back(train ? align256(sizeof(__bf16) * rows * maxw) : 0);
+  size_t tot = part_b;
+  for (size_t b : sz) tot += b;
+  int rc = ensure_buf(c, &m->act, &m->act_bytes, tot);
+  if (rc) return rc;
+  char *p = (char *)m->act + part_b;
+  A.part = train ? (float *)m->act : nullptr;
+  size_t i = 0;
+  A.X0 = (__bf16 *)p;
+  p += sz[i++];
+  A.Z.assign(nh, nullptr);
+  A.H.assign(nh, nullptr);
+  for (int l = 0; l < nh; ++l) {
+    A.Z[l] = sz[i] ? (float *)p : nullptr;
+    p += sz[i++];
+  }
+  for (int l = 0; l < nh; ++l) {
+    A.H[l] = (__bf16 *)p;
+    p += sz[i++];
+  }
+  if (!train) {
+    // inference ping-pongs the two hidden buffers that were sized
+    char *h0 = (char *)m->act + part_b + sz[0];
+    for (int l = 0; l < nh; ++l) A.H[l] = (__bf16 *)(h0 + (l % 2) * sz[1 + nh]);
+  }
+  A.Y = (float *)p;
+  p += sz[i++];
+  A.D[0] = (__bf16 *)p;
+  p += sz[i++];
+  A.D[1] = (__bf16 *)p;
+  return YTF_OK;
+}
+
+static hipError_t mlp_forward_rows(ytf_ctx *c, ytf_mlp *m, const MlpAct &A, int64_t n, int64_t rows, const float *x,
+                                   bool save_z) {
+  hipError_t e =
+      ytf::launch_mlp_inputs(n, rows, m->nin, m->Kp[0], m->kzf, x, m->norm, m->norm + m->nin, A.X0, c->stream);
+  for (int l = 0; l < m->nl && e == hipSuccess; ++l) {
+    ytf::MlpGemm g{};
+    g.A = l == 0 ? A.X0 : A.H[l - 1];
+    g.lda = m->Kp[l];
+    g.B = m->P16 + m->offW[l];
+    g.ldb = m->Kp[l];
+    g.M = (int)rows;
+    g.N = m->Np[l];
+    g.K = m->Kp[l];
+    g.ldc = m->Np[l];
+    g.bias = m->P + m->offb[l];
+    if (l < m->nl - 1) {
+      g.C32 = save_z ? A.Z[l] : nullptr;
+      g.C16 = A.H[l];
+      e = timed(c, "mlp_gemm", [&] { return ytf::launch_mlp_gemm(g, ytf::MLP_EPI_BIAS_SILU, false, false, c->stream); });
+    } else {
+      g.C32 = A.Y;
+      e = timed(c, "mlp_gemm", [&] { return ytf::launch_mlp_gemm(g, ytf::MLP_EPI_BIAS_F32, false, false, c->stream); });
+    }
+  }
+  return e;
+}
+
+// forward + loss + backward into m->G (no update)
+static int mlp_loss_grad_locked(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x, const float *y, const char *who) {
+  if (n < 1 || n > ((int64_t)1 << 24) || !x || !y) return fail(c, YTF_EINVAL, "%s: bad batch (n=%lld)", who, (long long)n);
+  const int64_t rows = (int64_t)ru64(n);
+  MlpAct A;
+  int rc = mlp_workspace(c, m, rows, true, A);
+  if (rc) return rc;
+  hipError_t e = mlp_forward_rows(c, m, A, n, rows, x, true);
+  const int L = m->nl - 1;
+  const float *osh = m->norm + 2 * m->nin, *osc = osh + m->nout;
+  if (e == hipSuccess) e = ytf::launch_mlp_zero(1, m->loss, c->stream);
+  if (e == hipSuccess)
+    e = ytf::launch_mlp_mse(n, rows, m->nout, m->Np[L], A.Y, y, osh, osc, A.D[L % 2], m->loss, c->stream);
+  for (int l = L; l >= 0 && e == hipSuccess; --l) {
+    __bf16 *dZ = A.D[l % 2];
+    const __bf16 *Hin = l == 0 ? A.X0 : A.H[l - 1];
+    // dW[n][k] = sum_b dZ[b][n] Hin[b][k]
+    ytf::MlpGemm g{};
+    g.A = dZ;
+    g.lda = m->Np[l];
+    g.B = Hin;
+    g.ldb = m->Kp[l];
+    g.M = m->Np[l];
+    g.N = m->Kp[l];
+    g.K = (int)rows;
+    g.ldc = m->Kp[l];
+    // split the batch reduction so that the small weight tiles fill the chip:
+    // partial sums per slice in A.part, then a fixed-order sum into G
+    int splits = 1;
+    while (splits < 64 && (rows % (64 * splits * 2)) == 0 &&
+           (int64_t)(g.M / 64) * (g.N / 64) * splits < 512)
+      splits *= 2;
+    g.ksplit = splits;
+    g.split_stride = (int64_t)g.M * g.N;
+    g.C32 = splits > 1 ? A.part : m->G + m->offW[l];
+    e = timed(c, "mlp_gemm", [&] { return ytf::launch_mlp_gemm(g, ytf::MLP_EPI_F32, true, true, c->stream); });
+    if (e == hipSuccess && splits > 1)
+      e = ytf::launch_mlp_reduce(g.split_stride, splits, g.split_stride, A.part, m->G + m->offW[l], c->stream);
+    if (e == hipSuccess) e = ytf::launch_mlp_colsum(rows, m->Np[l], dZ, m->G + m->offb[l], c->stream);
+    if (e == hipSuccess && l > 0) {
+      // dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1})
+      ytf::MlpGemm h{};
+      h.A = dZ;
+      h.lda = m->Np[l];
+      h.B = m->P16 + m->offW[l];
+      h.ldb = m->Kp[l];
+      h.M = (int)rows;
+      h.N = m->Kp[l];
+      h.K = m->Np[l];
+      h.ldc = m->Kp[l];
+      h.Zin = A.Z[l - 1];
+      h.C16 = A.D[(l - 1) % 2];
+      e = timed(c, "mlp_gemm", [&] { return ytf::launch_mlp_gemm(h, ytf::MLP_EPI_DSILU, false, true, c->stream); });
+    }
+  }
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "%s: launch failed: %s", who, hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_mlp_loss_grad(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, const float *y_dev, float *loss_host,
+                      float *grad_host) {
+  if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_loss_grad: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = mlp_loss_grad_locked(c, m, n, x_dev, y_dev, "ytf_mlp_loss_grad");
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (loss_host) HIPCHK(c, hipMemcpy(loss_host, m->loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (grad_host) {
+    std::vector<float> h(m->np);
+    HIPCHK(c, hipMemcpy(h.data(), m->G, sizeof(float) * m->np, hipMemcpyDeviceToHost));
+    for (int l = 0; l < m->nl; ++l) {
+      for (int i = 0; i < m->N[l]; ++i)
+        for (int j = 0; j < m->K[l]; ++j)
+          grad_host[m->foffW[l] + (size_t)i * m->K[l] + j] = h[m->offW[l] + (size_t)i * m->Kp[l] + j];
+      for (int i = 0; i < m->N[l]; ++i) grad_host[m->foffb[l] + i] = h[m->offb[l] + i];
+    }
+  }
+  return YTF_OK;
+}
+
+int ytf_mlp_train_step(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, const float *y_dev, float lr,
+                       float *loss_dev) {
+  if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_train_step: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!(lr > 0.0f) || !std::isfinite(lr)) return fail(c, YTF_EINVAL, "ytf_mlp_train_step: lr must be > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = mlp_loss_grad_locked(c, m, n, x_dev, y_dev, "ytf_mlp_train_step");
+  if (rc) return rc;
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
+  hipError_t e = timed(c, "mlp_adam", [&] {
+    return ytf::launch_mlp_adam((int64_t)m->np, m->P, m->Mo, m->Vo, m->G, m->P16, lr, b1, b2, 1e-8f, c1, c2, c->stream);
+  });
+  if (e == hipSuccess && loss_dev) e = hipMemcpyAsync(loss_dev, m->loss, sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_train_step: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_mlp_forward(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, float *y_dev) {
+  if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_forward: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n < 0 || (n > 0 && (!x_dev || !y_dev))) return fail(c, YTF_EINVAL, "ytf_mlp_forward: bad arguments");
+  if (n == 0) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t chunk = 1 << 18;
+  const int64_t rows_cap = std::min<int64_t>((int64_t)ru64(n), chunk);
+  MlpAct A;
+  int rc = mlp_workspace(c, m, rows_cap, false, A);
+  if (rc) return rc;
+  const int L = m->nl - 1;
+  const float *osh = m->norm + 2 * m->nin, *osc = osh + m->nout;
+  for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+    const int64_t nc = std::min(chunk, n - r0), rows = (int64_t)ru64(nc);
+    hipError_t e = mlp_forward_rows(c, m, A, nc, rows, x_dev + r0 * m->nin, false);
+    if (e == hipSuccess) e = ytf::launch_mlp_outputs(nc, m->nout, m->Np[L], A.Y, osh, osc, y_dev + r0 * m->nout, c->stream);
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_forward: launch failed: %s", hipGetErrorString(e));
+  }
+  return YTF_OK;
+}
+
 static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const double *kx, const double *wt,
                                    int np, const double *ghat, int nr, const double *rho, cd *R, int mode) {
   if (mode != YTF_SYNTH_FP64 && mode != YTF_SYNTH_F16X3)

@@ -1167,24 +1167,35 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
       cd Af[6], Ab[6], Afd[6], Abd[6];  // [m + 2q]
 #pragma unroll
       for (int i = 0; i < 6; ++i) { Af[i] = z0; Ab[i] = z0; Afd[i] = z0; Abd[i] = z0; }
-      if (Tvalid) {
+      // term classes (_term_amplitudes, mod_ClosedFormField.jl:143-160): the
+      // seeds feeding the forward / backward amplitudes, and the direct term
+      const int terms = P.terms;
+      if (Tvalid && terms != TERMS_DIRECT) {
         const cd Pf[2] = {expi(Qs[0], d_src - zpos), expi(Qs[1], d_src - zpos)};
         const cd Pb[2] = {expi(-Qs[2], zpos), expi(-Qs[3], zpos)};
+        // seeds=:f keeps the forward seed columns (0, 1), seeds=:b the backward
+        // (2, 3): weights of the two seed halves in the forward (rows 0-1)
+        // and backward (rows 2-3) amplitudes.  The masks are arithmetic 0/1
+        // weights: the same masks written as selects between partial sums made
+        // hipcc (ROCm 7.2, gfx950) emit wrong T^o values for this kernel at -O1
+        // and -O3, while a host build of that source was exact.
+        const double wFf = terms == TERMS_HANKEL ? 0.0 : 1.0, wFb = terms == TERMS_TOEPLITZ ? 0.0 : 1.0;
+        const double wBf = terms == TERMS_TOEPLITZ ? 0.0 : 1.0, wBb = terms == TERMS_HANKEL ? 0.0 : 1.0;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const cd u[4] = {sif[2 * q] * Pf[0], sif[2 * q + 1] * Pf[1], -(sib[2 * q] * Pb[0]), -(sib[2 * q + 1] * Pb[1])};
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            cd v = T[i] * u[0];
-            v = fmac(v, T[i + 4], u[1]);
-            v = fmac(v, T[i + 8], u[2]);
-            v = fmac(v, T[i + 12], u[3]);
+            const cd vf = fmac(T[i] * u[0], T[i + 4], u[1]);
+            const cd vb = fmac(T[i + 8] * u[2], T[i + 12], u[3]);
+            const cd v = rmul(i < 2 ? wFf : wBf, vf) + rmul(i < 2 ? wFb : wBb, vb);
             if (i < 2) Af[i + 2 * q] = v;
             else Ab[i - 2 + 2 * q] = v;
           }
         }
       }
-      if ((CASE == CASE_SAME || CASE == CASE_TOP) && valid) {
+      if ((CASE == CASE_SAME || CASE == CASE_TOP) && valid &&
+          (terms == TERMS_ALL || terms == TERMS_DIRECT || terms == TERMS_TOEPLITZ)) {
         const double zeta = zob - zpos;
 #pragma unroll
         for (int m = 0; m < 2; ++m) {

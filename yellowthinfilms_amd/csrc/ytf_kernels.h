@@ -35,7 +35,11 @@ struct NodeParams {
   cd *basis;                  // [16][nn] Exf Eyf Ezf Exb Eyb Ezb KZf KZb
   unsigned char *flags;       // [nn] bit0 valid, bit1 Tvalid, bit2 singular solve, bit3 mode solve failed
   bool needs_general;         // some layer needs the general 4x4 eigen-solver
+  int terms;                  // TERMS_*: which terms of g~
 };
+
+// term classes of the spectral dyadic (mod_ClosedFormField.jl:143-160)
+enum { TERMS_ALL = 0, TERMS_DIRECT = 1, TERMS_SCATTERED = 2, TERMS_TOEPLITZ = 3, TERMS_HANKEL = 4 };
 
 hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream);
 hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream);
@@ -90,5 +94,36 @@ struct BatchLaunch {
   int waves_per_block, blocks;
 };
 hipError_t launch_batch(const BatchLaunch &B, hipStream_t s);
+
+// Surrogate network (mlp.hip): one BF16 matrix-core GEMM with fused epilogues,
+// plus the small elementwise kernels of training and inference.
+enum { MLP_EPI_F32 = 0, MLP_EPI_BIAS_SILU = 1, MLP_EPI_BIAS_F32 = 2, MLP_EPI_DSILU = 3 };
+struct MlpGemm {
+  const __bf16 *A;
+  int64_t lda;
+  const __bf16 *B;
+  int64_t ldb;
+  int M, N, K;  // multiples of 64, 64, 32
+  float *C32;
+  int64_t ldc;
+  __bf16 *C16;
+  const float *bias;
+  const float *Zin;
+  int ksplit;            // > 1: MLP_EPI_F32 only, partial sums at C32 + z * split_stride
+  int64_t split_stride;
+};
+hipError_t launch_mlp_gemm(const MlpGemm &p, int epi, bool at, bool bt, hipStream_t s);
+hipError_t launch_mlp_inputs(int64_t n, int64_t rows, int nin, int ldx, int kzf, const float *x, const float *shift,
+                             const float *scale, void *X, hipStream_t s);
+hipError_t launch_mlp_mse(int64_t n, int64_t rows, int nout, int ldy, const float *Y, const float *y,
+                          const float *oshift, const float *oscale, void *dZ, float *loss, hipStream_t s);
+hipError_t launch_mlp_colsum(int64_t rows, int ld, const void *dZ, float *db, hipStream_t s);
+hipError_t launch_mlp_outputs(int64_t n, int nout, int ldy, const float *Y, const float *oshift, const float *oscale,
+                              float *y, hipStream_t s);
+hipError_t launch_mlp_adam(int64_t n, float *w, float *m, float *v, const float *g, void *w16, float lr, float b1,
+                           float b2, float eps, float c1, float c2, hipStream_t s);
+hipError_t launch_mlp_to_bf16(int64_t n, const float *w, void *w16, hipStream_t s);
+hipError_t launch_mlp_zero(int64_t n, float *x, hipStream_t s);
+hipError_t launch_mlp_reduce(int64_t n, int splits, int64_t stride, const float *part, float *dst, hipStream_t s);
 
 }  // namespace ytf

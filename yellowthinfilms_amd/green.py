@@ -83,12 +83,14 @@ class NodeCache:
 
 
 def green_nodes(stack, omega, kx, ky, source_layer, obs_layer, zsrc=(), zobs=(), cache=True, ghat=True,
-                device=0, ghat_out=None):
+                device=0, ghat_out=None, terms="all"):
     """Batched Green k-node solve on the GPU (ytf_green_nodes).
 
     kx, ky: node wavevectors (torch float64 on the device, or array-likes);
     source_layer, obs_layer: 1-based; zsrc/zobs: depth pairs for the spectral
-    dyadic.  Returns (NodeCache or None, ghat (npairs, 9, n) complex128 or None).
+    dyadic; terms: "all", "direct", "scattered", "toeplitz" or "hankel"
+    (closed_form_G's term classes).  Returns (NodeCache or None, ghat
+    (npairs, 9, n) complex128 or None).
     """
     torch = _torch()
     dev = torch.device("cuda", device)
@@ -114,9 +116,16 @@ def green_nodes(stack, omega, kx, ky, source_layer, obs_layer, zsrc=(), zobs=(),
         ctx.handle, layers, len(stack), float(omega), ct.c_void_p(kx.data_ptr()), ct.c_void_p(ky.data_ptr()), n,
         int(source_layer), int(obs_layer), npairs, zs.ctypes.data_as(ct.POINTER(ct.c_double)),
         zo.ctypes.data_as(ct.POINTER(ct.c_double)), ct.c_void_p(g.data_ptr()) if g is not None else None,
-        ct.byref(cs) if cs is not None else None)
+        ct.byref(cs) if cs is not None else None, _terms(terms))
     _lib.check(rc, ctx.handle)
     return nc, g
+
+
+def _terms(terms):
+    try:
+        return _lib.TERMS[terms]
+    except KeyError:
+        raise ValueError(f"closed_form_G: terms must be one of {sorted(_lib.TERMS)} (got {terms!r})") from None
 
 
 def build_cavity_transfer(stack, omega, source_layer, kx, ky, obs_layers=None, device=0):
