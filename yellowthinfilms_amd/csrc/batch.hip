@@ -38,42 +38,6 @@ struct BatchArgs {
   unsigned int *counter;
 };
 
-__device__ __forceinline__ void prop4(const cd *Q, double d, cd *p) {
-  p[0] = expi(Q[0], d);
-  p[1] = expi(Q[1], d);
-  p[2] = expi(-Q[2], d);
-  p[3] = expi(-Q[3], d);
-}
-
-// The previous layer's mode matrix lives in LDS, one 4x4 per lane stored
-// [entry][lane] (conflict-free), so that only the current layer's modes, the
-// interface solve and the accumulated S occupy registers.
-struct LaneMat {
-  cd *p;
-  int lane;
-  __device__ __forceinline__ cd get(int i) const { return p[i * 64 + lane]; }
-  __device__ __forceinline__ void set(int i, cd v) const { p[i * 64 + lane] = v; }
-};
-
-// _match_modes (eigpenalty 0) of one side against the previous layer in LDS.
-__device__ __forceinline__ void match_side_lds(const LaneMat &Vp, cd *V, cd *Q, int c0) {
-  cd C[4];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int a = 4 * (c0 + i);
-      const cd *b = V + 4 * (c0 + j);
-      cd t = conj(Vp.get(a + 3)) * b[0];
-      t = t - conj(Vp.get(a + 2)) * b[1];
-      t = t - conj(Vp.get(a + 1)) * b[2];
-      t = fmac_cj(t, Vp.get(a), b[3]);
-      C[i + 2 * j] = t;
-    }
-  const bool sw = abs2(C[2]) + abs2(C[1]) > abs2(C[0]) + abs2(C[3]);
-  cswap_col(sw, Q, V, c0, c0 + 1);
-}
-
 template <bool GENERAL>
 __device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_t g, const LaneMat &Vp) {
   const double kx = A.kx[g], ky = A.ky[g], w = A.omega;

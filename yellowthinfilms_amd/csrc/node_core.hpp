@@ -804,23 +804,247 @@ __device__ __forceinline__ cd fin0(cd z) { return isfin(z) ? z : mk(0.0); }
 // never changed by _match_modes, and the deeper layers' order cancels in R).
 enum { CASE_SAME = 0, CASE_RIGHT = 1, CASE_LEFT = 2, CASE_TOP = 3 };
 
+// Per-lane LDS area of the general cases, [slot][64 lanes] (conflict-free):
+// the previous layer's mode matrix during the passes, and the 2x2 cascade
+// blocks that the cavity and the walk read afterwards.  Keeping them out of
+// registers lets a lane carry a whole layer solve without spilling.
+struct LaneMat {
+  cd *p;
+  int lane;
+  __device__ __forceinline__ cd get(int i) const { return p[i * 64 + lane]; }
+  __device__ __forceinline__ void set(int i, cd v) const { p[i * 64 + lane] = v; }
+  __device__ __forceinline__ void put4(int i, const cd *b) const {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) set(i + k, b[k]);
+  }
+  __device__ __forceinline__ void get4(int i, cd *b) const {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = get(i + k);
+  }
+};
+// slots: V 0-15; RL = SLeft[s]_12; SLs22 = SLeft[s]_22; Rs = SRight[s]_21;
+// Ts = SRight[s]_11; X0..X2: CASE_RIGHT SRight[o]_11, SRight[o+1]_11,
+// SRight[o+1]_21 -- CASE_LEFT SLeft[o]_12, SLeft[o]_22, SLeft[o+1]_22;
+// Sint[o] blocks 21 and 22.
+enum { SL_V = 0, SL_RL = 16, SL_SLS22 = 20, SL_RS = 24, SL_TS = 28, SL_X0 = 32, SL_X1 = 36, SL_X2 = 40,
+       SL_SI21 = 44, SL_SI22 = 48, SL_END = 52 };
+template <int CASE>
+struct LdsSlots {
+  static constexpr int n = CASE == CASE_TOP ? 1 : (CASE == CASE_SAME ? 32 : SL_END);
+};
+
+// _match_modes (eigpenalty 0) of one side against the previous layer in LDS.
+__device__ __forceinline__ bool match_side_lds(const LaneMat &Vp, cd *V, cd *Q, int c0) {
+  cd C[4];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int a = 4 * (c0 + i);
+      const cd *b = V + 4 * (c0 + j);
+      cd t = conj(Vp.get(a + 3)) * b[0];
+      t = t - conj(Vp.get(a + 2)) * b[1];
+      t = t - conj(Vp.get(a + 1)) * b[2];
+      t = fmac_cj(t, Vp.get(a), b[3]);
+      C[i + 2 * j] = t;
+    }
+  const bool sw = abs2(C[2]) + abs2(C[1]) > abs2(C[0]) + abs2(C[3]);
+  cswap_col(sw, Q, V, c0, c0 + 1);
+  return sw;
+}
+
+__device__ __forceinline__ void prop4(const cd *Q, double d, cd *p) {
+  p[0] = expi(Q[0], d);
+  p[1] = expi(Q[1], d);
+  p[2] = expi(-Q[2], d);
+  p[3] = expi(-Q[3], d);
+}
+
+// The cascade of the general cases in two lean passes (same algebra as
+// S_all_Matrix, mod_ScatteringMatrix.jl:112-154, restricted to the blocks the
+// cavity and the walk use):
+//   forward, j = 0..N-1: modes with continuity matching (the swap decisions
+//     are recorded per layer); the SLeft prefix needs only the blocks 12 and 22
+//     of A = SLeft[j] * Sprop[j], since (A * B)_12 = B12 + B11 (I - A12 B21)^-1
+//     A12 B22 and (A * B)_22 = A22 (I - B21 A12)^-1 B22;
+//   backward, k = N-2..s: SRight[k] = eff_k * SRight[k+1] with
+//     eff_k = Sprop[k] * Sint[k]; its blocks 21 (R) and 11 (T) follow the 2x2
+//     recursions R_k = e21 + e22 ((I - R e12) \ R) e11 and
+//     T_k = T_{k+1} ((I - e12 R) \ e11) from R = 0, T = I at k = N-1, with
+//     the modes re-solved in the recorded matched order.
+// Results go to the lane's LDS slots; returns the recorded swap bits
+// (bit 2j: layer j's forward pair swapped, bit 2j+1: its backward pair).
 template <int CASE, bool GENERAL>
-__device__ void solve_node(const NodeParams &P, int64_t k) {
+__device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, const LaneMat &st, bool &ok,
+                                    bool &sing) {
+  const double w = P.omega;
+  const int N = P.nl, s = P.src, o = P.obs;
+  uint64_t bits = 0;
+  {
+    cd L12[4], L22[4];
+    for (int j = 0; j < N; ++j) {
+      const LayerDev &L = P.L[j];
+      cd Q[4], Lm[16], Si[16];
+      const bool need_int = j > 0 && (j <= s || (CASE != CASE_SAME && j == o + 1));
+      {
+        cd V[16];
+        ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q) && ok;
+        if (j > 0) {
+          const bool sf = match_side_lds(st, V, Q, 0);
+          const bool sb = match_side_lds(st, V, Q, 2);
+          bits |= ((uint64_t)sf << (2 * j)) | ((uint64_t)sb << (2 * j + 1));
+          if (need_int) {
+            // _S_Interface_Matrix: [-Vf_B  Vb_A] \ [-Vf_A  Vb_B], A = layer j-1
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+              Lm[i] = -V[i];
+              Lm[8 + i] = st.get(SL_V + 8 + i);
+              Si[i] = -st.get(SL_V + i);
+              Si[8 + i] = V[8 + i];
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) st.set(SL_V + i, V[i]);
+      }
+      if (j == 0) {
+        // SLeft[0] = I;  A = I * Sprop[0]
+        if (s == 0) {
+          const cd z[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)}, id[4] = {mk(1.0), mk(0.0), mk(0.0), mk(1.0)};
+          st.put4(SL_RL, z);
+          st.put4(SL_SLS22, id);
+        }
+        cd p[4];
+        prop4(Q, L.d, p);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L12[i] = mk(0.0);
+        L22[0] = p[2]; L22[1] = mk(0.0); L22[2] = mk(0.0); L22[3] = p[3];
+      } else if (need_int) {
+        sing = !lu_solve4<4>(Lm, Si) || sing;  // Si = Sint[j-1]
+        if (j <= s) {
+          cd B11[4], B12[4], B21[4], B22[4];
+          blk(Si, 0, 0, B11); blk(Si, 0, 1, B12); blk(Si, 1, 0, B21); blk(Si, 1, 1, B22);
+          cd t[4], M[4], X[4], Y[4], S12[4], S22[4];
+          mm<2, 2, 2>(L12, B21, t);
+          M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+          solve2<2>(M, L12, X);
+          mm<2, 2, 2>(X, B22, Y);
+          mm<2, 2, 2>(B11, Y, t);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) S12[i] = B12[i] + t[i];
+          mm<2, 2, 2>(B21, L12, t);
+          M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+          solve2<2>(M, B22, X);
+          mm<2, 2, 2>(L22, X, S22);
+          if (j == s) {
+            st.put4(SL_RL, S12);
+            st.put4(SL_SLS22, S22);
+          }
+          if (CASE == CASE_LEFT) {
+            if (j == o) { st.put4(SL_X0, S12); st.put4(SL_X1, S22); }
+            if (j == o + 1) st.put4(SL_X2, S22);
+          }
+          if (j < s) {
+            cd p[4];
+            prop4(Q, L.d, p);
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+              for (int r = 0; r < 2; ++r) {
+                L12[r + 2 * c] = p[r] * (S12[r + 2 * c] * p[2 + c]);
+                L22[r + 2 * c] = S22[r + 2 * c] * p[2 + c];
+              }
+          }
+        }
+        if (CASE != CASE_SAME && j == o + 1) {
+          cd b[4];
+          blk(Si, 1, 0, b);
+          st.put4(SL_SI21, b);
+          blk(Si, 1, 1, b);
+          st.put4(SL_SI22, b);
+        }
+      }
+    }
+  }
+  {
+    cd R[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)}, T[4] = {mk(1.0), mk(0.0), mk(0.0), mk(1.0)};
+    if (CASE == CASE_RIGHT) {
+      if (o == N - 1) st.put4(SL_X0, T);
+      if (o + 1 == N - 1) { st.put4(SL_X1, T); st.put4(SL_X2, R); }
+    }
+    {
+      cd V[16], Q[4];
+      modes_layer<GENERAL>(P.L[N - 1], w, kx, ky, V, Q);
+      apply_swaps((bits >> (2 * (N - 1))) & 1, (bits >> (2 * (N - 1) + 1)) & 1, V, Q);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) st.set(SL_V + i, V[i]);
+    }
+    for (int kk = N - 2; kk >= s; --kk) {
+      const LayerDev &L = P.L[kk];
+      cd Q[4], Lm[16], Si[16];
+      {
+        cd V[16];
+        modes_layer<GENERAL>(L, w, kx, ky, V, Q);
+        apply_swaps((bits >> (2 * kk)) & 1, (bits >> (2 * kk + 1)) & 1, V, Q);
+        // Sint[kk]: A = layer kk, B = layer kk+1 (in LDS)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          Lm[i] = -st.get(SL_V + i);
+          Lm[8 + i] = V[8 + i];
+          Si[i] = -V[i];
+          Si[8 + i] = st.get(SL_V + 8 + i);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) st.set(SL_V + i, V[i]);
+      }
+      sing = !lu_solve4<4>(Lm, Si) || sing;
+      cd p[4];
+      prop4(Q, L.d, p);
+      cd e11[4], e12[4], e21[4], e22[4];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          e11[r + 2 * c] = Si[r + 4 * c] * p[c];
+          e12[r + 2 * c] = Si[r + 4 * (c + 2)];
+          e21[r + 2 * c] = p[2 + r] * (Si[r + 2 + 4 * c] * p[c]);
+          e22[r + 2 * c] = p[2 + r] * Si[r + 2 + 4 * (c + 2)];
+        }
+      cd t[4], M[4], X[4], Y[4], Z[4];
+      mm<2, 2, 2>(R, e12, t);
+      M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+      solve2<2>(M, R, X);
+      mm<2, 2, 2>(X, e11, Y);
+      mm<2, 2, 2>(e22, Y, Z);
+      mm<2, 2, 2>(e12, R, t);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) R[i] = e21[i] + Z[i];
+      M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+      solve2<2>(M, e11, X);
+      mm<2, 2, 2>(T, X, Y);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) T[i] = Y[i];
+      if (kk == s) { st.put4(SL_RS, R); st.put4(SL_TS, T); }
+      if (CASE == CASE_RIGHT) {
+        if (kk == o) st.put4(SL_X0, T);
+        if (kk == o + 1) { st.put4(SL_X1, T); st.put4(SL_X2, R); }
+      }
+    }
+  }
+  return bits;
+}
+
+template <int CASE, bool GENERAL>
+__device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
   bool ok = true, sing = false;
-  // running state of the forward pass
-  cd Vp[16], Qp[4];
-  cd acc[16];          // SLeft chain (j <= s), then suffix product for SRight[s]
-  cd accO[16], accO1[16];  // CASE_RIGHT: SRight[o], SRight[o+1]
   cd RL[4], SLs22[4];  // SLeft[s] blocks (0,1) and (1,1)
-  cd SLo12[4], SLo22[4], SLo1_22[4];  // CASE_LEFT: SLeft[o], SLeft[o+1]
-  cd Sint21[4], Sint22[4];            // Sint[o] blocks (1,0), (1,1) when o != s
+  cd Rs[4], Ts[4];     // SRight[s] blocks (1,0) and (0,0)
   bool swS_f = false, swS_b = false, swO_f = false, swO_b = false;
-  bool sw_f_cum = false, sw_b_cum = false;  // net swap of the matched order vs raw
-#pragma unroll
-  for (int i = 0; i < 4; ++i) { RL[i] = mk(0.0); SLs22[i] = mk(i == 0 || i == 3 ? 1.0 : 0.0); }
   if (CASE == CASE_TOP) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { RL[i] = mk(0.0); SLs22[i] = mk(i == 0 || i == 3 ? 1.0 : 0.0); Ts[i] = mk(0.0); }
     cd R[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)};
     cd Vn[16];
     {
@@ -856,100 +1080,21 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
       for (int i = 0; i < 16; ++i) Vn[i] = V[i];
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = mk(0.0);
-    acc[2] = R[0]; acc[3] = R[1]; acc[6] = R[2]; acc[7] = R[3];
-  }
-  for (int j = 0; CASE != CASE_TOP && j < N; ++j) {
-    const LayerDev &L = P.L[j];
-    cd V[16], Q[4];
-    ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q) && ok;
-    if (j == 0) {
-      sw_f_cum = false;
-      sw_b_cum = false;
-      // SLeft[0] = I, S0 = I * Sprop[0] = Sprop[0]
-      if (s > 0) {
-        cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
+    for (int i = 0; i < 4; ++i) Rs[i] = R[i];
+  } else {
+    const uint64_t bits = cascade_general<CASE, GENERAL>(P, kx, ky, st, ok, sing);
+    swS_f = (bits >> (2 * s)) & 1;
+    swS_b = (bits >> (2 * s + 1)) & 1;
+    swO_f = (bits >> (2 * o)) & 1;
+    swO_b = (bits >> (2 * o + 1)) & 1;
+    st.get4(SL_RL, RL);
+    st.get4(SL_RS, Rs);
+    st.get4(SL_TS, Ts);
+    if (CASE == CASE_LEFT) st.get4(SL_SLS22, SLs22);
+    else {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) acc[i] = mk(0.0);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[5 * i] = p[i];
-      }
-    } else {
-      const bool sf = match_side(Vp, V, Q, 0);
-      const bool sb = match_side(Vp, V, Q, 2);
-      sw_f_cum = sf;
-      sw_b_cum = sb;
-      cd Sint[16];
-      sing = !s_interface(Vp, V, Sint) || sing;
-      if (j <= s) {
-        // SLeft[j] = S0 * Sint[j-1]; S0 = SLeft[j] * Sprop[j]
-        cd SL[16];
-        redheffer(acc, Sint, SL);
-        if (j == s) {
-          blk(SL, 0, 1, RL);
-          blk(SL, 1, 1, SLs22);
-        }
-        if (CASE == CASE_LEFT) {
-          if (j == o) { blk(SL, 0, 1, SLo12); blk(SL, 1, 1, SLo22); }
-          if (j == o + 1) blk(SL, 1, 1, SLo1_22);
-        }
-        if (j < s) {
-          const cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
-          redheffer_rdiag(SL, p, acc);
-        }
-      }
-      if (j - 1 >= s) {
-        // eff_{j-1} = Sprop[j-1] * Sint[j-1]
-        const double dp = P.L[j - 1].d;
-        const cd p[4] = {expi(Qp[0], dp), expi(Qp[1], dp), expi(-Qp[2], dp), expi(-Qp[3], dp)};
-        cd eff[16];
-        redheffer_ldiag(p, Sint, eff);
-        if (j - 1 == s) {
-#pragma unroll
-          for (int i = 0; i < 16; ++i) acc[i] = eff[i];
-        } else {
-          cd t[16];
-          redheffer(acc, eff, t);
-#pragma unroll
-          for (int i = 0; i < 16; ++i) acc[i] = t[i];
-        }
-        if (CASE == CASE_RIGHT) {
-          if (j - 1 == o) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) accO[i] = eff[i];
-          } else if (j - 1 > o) {
-            cd t[16];
-            redheffer(accO, eff, t);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) accO[i] = t[i];
-          }
-          if (j - 1 == o + 1) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) accO1[i] = eff[i];
-          } else if (j - 1 > o + 1) {
-            cd t[16];
-            redheffer(accO1, eff, t);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) accO1[i] = t[i];
-          }
-        }
-      }
-      if (CASE != CASE_SAME && j == o + 1) {
-        blk(Sint, 1, 0, Sint21);
-        blk(Sint, 1, 1, Sint22);
-      }
+      for (int i = 0; i < 4; ++i) SLs22[i] = mk(0.0);
     }
-    if (j == s) { swS_f = sw_f_cum; swS_b = sw_b_cum; }
-    if (j == o) { swO_f = sw_f_cum; swO_b = sw_b_cum; }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) Vp[i] = V[i];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) Qp[i] = Q[i];
-  }
-  // SRight[s] (identity when s == N-1, not reachable for a source layer)
-  if (s == N - 1) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = mk((i % 5 == 0) ? 1.0 : 0.0);
   }
 
   // ---- source layer: modes (matched order), seeds ---------------------------
@@ -968,14 +1113,17 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
 
   // ---- cavity of the source layer (_R_cavity, _cavity_resummation) ---------
   const double ds = Ls.d;
-  cd RR[4], RRr11[4], RRr12[4];
+  // RRr = diag(ph) * SRight[s] (_R_cavity): its blocks 21 (RR) and 11
+  cd RR[4], RRr11[4];
   {
     const cd ph[4] = {expi(Qs[0], -ds), expi(Qs[1], -ds), expi(Qs[2], ds), expi(Qs[3], ds)};
-    cd RRr[16];
-    redheffer_ldiag(ph, acc, RRr);
-    blk(RRr, 1, 0, RR);
-    blk(RRr, 0, 0, RRr11);
-    blk(RRr, 0, 1, RRr12);
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        RR[r + 2 * c] = ph[2 + r] * (Rs[r + 2 * c] * ph[c]);
+        RRr11[r + 2 * c] = Ts[r + 2 * c] * ph[c];
+      }
   }
   // F+ = (I - PF RL PB RR) \ (E+ + PF RL E-),  F- = E- + PB RR F+   (2x4 each)
   cd Fp[8], Fm[8];
@@ -1043,7 +1191,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
         for (int i = 0; i < 8; ++i) { Ef[i] = flushd(cp[i]); Ebnext[i] = mk(0.0); }
       } else {
         cd S11[4];
-        blk(accO, 0, 0, S11);
+        st.get4(SL_X0, S11);
         if (ill_conditioned(S11)) {
 #pragma unroll
           for (int i = 0; i < 8; ++i) Ef[i] = mk(0.0);
@@ -1058,8 +1206,8 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
           for (int i = 0; i < 8; ++i) Ebnext[i] = mk(0.0);
         } else {
           cd S11b[4], S21b[4], bp[8], bm[8];
-          blk(accO1, 0, 0, S11b);
-          blk(accO1, 1, 0, S21b);
+          st.get4(SL_X1, S11b);
+          st.get4(SL_X2, S21b);
           if (ill_conditioned(S11b)) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) bp[i] = mk(0.0);
@@ -1078,7 +1226,9 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) Ef[i] = mk(0.0);
       } else {
-        cd bm[8], bp[8];
+        cd bm[8], bp[8], SLo12[4], SLo22[4];
+        st.get4(SL_X0, SLo12);
+        st.get4(SL_X1, SLo22);
         if (ill_conditioned(SLo22)) {
 #pragma unroll
           for (int i = 0; i < 8; ++i) bm[i] = mk(0.0);
@@ -1093,7 +1243,8 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) Ebnext[i] = Fm[i];
       } else {
-        cd bm[8];
+        cd bm[8], SLo1_22[4];
+        st.get4(SL_X2, SLo1_22);
         if (ill_conditioned(SLo1_22)) {
 #pragma unroll
           for (int i = 0; i < 8; ++i) bm[i] = mk(0.0);
@@ -1115,6 +1266,9 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
       cd far[8], t1[8], t2[8];
 #pragma unroll
       for (int j = 0; j < 4; ++j) { far[2 * j] = ph[0] * Ef[2 * j]; far[2 * j + 1] = ph[1] * Ef[2 * j + 1]; }
+      cd Sint21[4], Sint22[4];
+      st.get4(SL_SI21, Sint21);
+      st.get4(SL_SI22, Sint22);
       mm<2, 2, 4>(Sint21, far, t1);
       mm<2, 2, 4>(Sint22, Ebnext, t2);
 #pragma unroll
@@ -1234,9 +1388,11 @@ __device__ void solve_node(const NodeParams &P, int64_t k) {
 
 template <int CASE, bool GENERAL>
 __global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
+  __shared__ cd lds[LdsSlots<CASE>::n * 64];
+  const LaneMat st{lds, (int)threadIdx.x};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.nn; k += stride)
-    solve_node<CASE, GENERAL>(P, k);
+    solve_node<CASE, GENERAL>(P, k, st);
 }
 
 }  // namespace ytf
