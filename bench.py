@@ -17,6 +17,12 @@ Workloads (yellowthinfilms_amd.workloads):
   uneven_256: 256 stacks of 2..24 hyperbolic layers with 1k..16k nodes each,
     modes + total S-matrix per node in one persistent (graph) launch whose
     shape the autotuner picks first.  value = stack-nodes/s.
+  surrogate_1024: the learned surrogate of the scattered spectral dyadic,
+    trained first (2^22 solver samples, untimed), then on a 1024 x 1024 grid:
+    network + flags + exact FP64 fallback, against the plain GPU solve of the
+    same nodes.  value = Green k-nodes/s of the surrogate path.
+  nufft_1e6: type-1 NUFFT of 10^6 seeded points onto 1024^2 modes, eps 1e-9.
+    value = applies/s.
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -390,6 +396,158 @@ def cpu_uneven(stacks, counts, kx, ky, out):
 
 
 # ---------------------------------------------------------------------------
+def bench_surrogate(a):
+    import torch
+    from yellowthinfilms_amd import uniform_kxky_grid, green_nodes, workloads as W
+    from yellowthinfilms_amd.surrogate import GyroFamily, SurrogateGreen
+    from yellowthinfilms_amd._lib import Context, lib, check
+
+    wl = W.SURROGATE_1024
+    fam = GyroFamily(seed=wl["family_seed"])
+    sg = SurrogateGreen(fam, width=wl["width"], depth=wl["depth"], seed=wl["seed"])
+    t0 = time.perf_counter()
+    x, t = sg.generate(wl["n_train"], seed=wl["data_seed"])
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = sg.fit(x, t, epochs=wl["epochs"], batch=wl["batch"], lr=wl["lr"])
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del x, t
+    # evaluation grid: uniform 1024^2, |kx|, |ky| <= 3 omega
+    omega = 2 * math.pi * wl["f"]
+    grid = uniform_kxky_grid(wl["L"] / wl["f"], wl["dx"] / wl["f"])
+    nx, ny = grid["Nx"], grid["Ny"]
+    dev = torch.device("cuda", 0)
+    KX = torch.as_tensor(np.repeat(grid["kx"], ny), device=dev)
+    KY = torch.as_tensor(np.tile(grid["ky"], nx), device=dev)
+    hs, ho, loss = wl["h_src_nm"], wl["h_obs_nm"], wl["loss"]
+    st = fam.stack(loss)
+    nn = nx * ny
+    g_exact = torch.empty((1, 9, nn), dtype=torch.complex128, device=dev)
+    ctx = Context.get(0)
+    res = {}
+
+    def plain():
+        green_nodes(st, omega, KX, KY, 1, 1, [hs * W.NM], [1.0 - ho * W.NM], cache=False, ghat_out=g_exact,
+                    terms="scattered")
+
+    def surrogate():
+        res["g"], res["mask"] = sg.scattered_ghat(KX, KY, hs, ho, omega, loss, fallback=True)
+
+    for _ in range(a.warmup):
+        plain()
+        surrogate()
+    wall_plain = _timed_steps(dict_ns(a, warmup=0), plain, torch)
+    ctx.set_timing(True)
+    wall_sur = _timed_steps(dict_ns(a, warmup=0), surrogate, torch)
+    gemm_ms, gemm_n = ctx.kernel_time("mlp_gemm")
+    ctx.set_timing(False)
+    # errors of the spectral dyadic and of the field of a seeded source
+    ge = g_exact[0].T.contiguous()            # (nn, 9)
+    gs = res["g"]
+    rel_g = float(torch.linalg.norm(gs - ge) / torch.linalg.norm(ge))
+    J = torch.as_tensor(W.uniform_c128_source(nx, ny, wl["seed"]), device=dev)
+    work = torch.empty((3, nx, ny), dtype=torch.complex128, device=dev)
+    scale = grid["dx"] * grid["dy"] * grid["dkx"] * grid["dky"] / (2 * math.pi) ** 2
+
+    def field(g9n):
+        E = torch.empty_like(J)
+        gg = g9n.T.contiguous()  # [9][nn]
+        ctx.bind_stream()
+        import ctypes as ct
+        check(lib().ytf_spectral_apply(ctx.handle, nx, ny, ct.c_void_p(gg.data_ptr()), ct.c_void_p(J.data_ptr()),
+                                       ct.c_void_p(E.data_ptr()), ct.c_void_p(work.data_ptr()), scale), ctx.handle)
+        return E
+
+    E_ex, E_su = field(ge), field(gs)
+    rel_E = float(torch.linalg.norm(E_su - E_ex) / torch.linalg.norm(E_ex))
+    flagged = float(res["mask"].float().mean())
+    mlp_flops = sg.mlp.flops_per_sample * nn  # one forward per step
+    gemm_per_step = gemm_ms / a.steps
+    sur_rate, plain_rate = nn * a.steps / wall_sur, nn * a.steps / wall_plain
+    out = _base(a, wl, sur_rate, "Green k-nodes/s (surrogate + exact fallback)", wall_sur,
+                "bf16 (MLP, fp32 accumulate) + f64 (fallback)")
+    out["config"].update({"nodes": nn, "grid": [nx, ny], "mlp": f"6->{wl['width']}x{wl['depth']}->18 SiLU",
+                          "train_samples": wl["n_train"], "epochs": wl["epochs"]})
+    out.update({
+        "plain_gpu_knodes_per_s": plain_rate, "speedup_vs_plain_gpu": sur_rate / plain_rate,
+        "rel_err_ghat_vs_fp64": rel_g, "rel_err_field_vs_fp64": rel_E, "flagged_fraction": flagged,
+        "train_loss_per_epoch": [round(h, 6) for h in hist], "data_s": t_data, "train_s": t_train,
+        "roofline": {"kernel": "mlp_gemm (BF16 matrix-core layer products, all layers of one forward)",
+                     "bound": "mfma", "achieved": mlp_flops / (gemm_per_step * 1e-3) / 1e12,
+                     "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
+                     "frac": mlp_flops / (gemm_per_step * 1e-3) / PEAK_FP16, "traffic": None,
+                     "flops_per_sample": sg.mlp.flops_per_sample, "samples_per_launch_set": nn,
+                     "gemm_ms_per_forward": gemm_per_step},
+        "cpu_baseline": None,
+    })
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_nufft(a):
+    import torch
+    from yellowthinfilms_amd import nufft2d1, workloads as W
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.NUFFT_1E6
+    x, y, c = W.nufft_points(wl)
+    dev = torch.device("cuda", 0)
+    xd, yd, cd_ = (torch.as_tensor(v, device=dev) for v in (x, y, c))
+    N = wl["N"]
+    f = torch.empty((1, N, N), dtype=torch.complex128, device=dev)
+
+    def step():
+        nufft2d1(xd, yd, cd_, N, N, eps=wl["eps"], out=f)
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    t = {k: ctx.kernel_time(k) for k in ("nufft_sort", "nufft_spread", "nufft_fft")}
+    ctx.set_timing(False)
+    M = len(x)
+    alg_bytes = M * (16 + 16) + N * N * 16  # points + strengths in, modes out
+    ms = sum(v[0] for v in t.values()) / a.steps
+    cpu, rel_err = None, None
+    if not a.no_cpu_baseline:
+        cpu, rel_err = cpu_nufft(x, y, c, f[0].cpu().numpy(), N)
+    out = _base(a, wl, a.steps / wall, "NUFFT applies/s (10^6 points -> 1024^2 modes)", wall, "f64")
+    out["config"].update({"points": M, "modes": [N, N], "eps": wl["eps"]})
+    out.update({
+        "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
+        "rel_err_vs_fp64_oracle": rel_err,
+        "roofline": {"kernel": "nufft (sort + LDS-tiled spreading + FFT passes, one apply)", "bound": "hbm",
+                     "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                     "frac": alg_bytes / (ms * 1e-3) / PEAK_HBM, "traffic": None, "algorithmic_bytes": alg_bytes},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def cpu_nufft(x, y, c, f, N):
+    """The oracle's direct sum (OpenMP) at 256 seeded modes; the GPU's value
+    at those modes is compared with it."""
+    try:
+        from oracle import pyoracle as po
+    except Exception as exc:
+        return ({"value": None, "unit": "point-mode terms/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    cores = _cores()
+    rng = np.random.default_rng(5)
+    modes = np.stack([rng.integers(-N // 2, N // 2, 256), rng.integers(-N // 2, N // 2, 256)], axis=1)
+    t = time.perf_counter()
+    ref = po.nudft2d1(x, y, c, modes=modes, nthreads=cores)
+    dt = time.perf_counter() - t
+    got = f[modes[:, 1] + N // 2, modes[:, 0] + N // 2]
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    return ({"value": len(x) * len(modes) / dt, "unit": "point-mode terms/s (direct sum)", "cores": cores,
+             "kind": "port", "sample": f"{len(modes)} seeded modes x {len(x)} points"}, rel)
+
+
+# ---------------------------------------------------------------------------
 def dict_ns(a, **kw):
     b = argparse.Namespace(**vars(a))
     for k, v in kw.items():
@@ -398,7 +556,7 @@ def dict_ns(a, **kw):
 
 
 WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
-             "uneven_256": bench_uneven}
+             "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft}
 
 
 def main():

@@ -78,7 +78,8 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * returns the summed time (ms) and launch count of one kernel name:
  * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
  * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
- * "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam". */
+ * "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam", "nufft_sort",
+ * "nufft_spread", "nufft_fft". */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
 int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
@@ -245,6 +246,17 @@ int ytf_mlp_loss_grad(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev,
  * (before the step) is copied to loss_dev when not NULL. */
 int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, const float *y_dev, float lr,
                        float *loss_dev);
+
+/* 2D type-1 nonuniform FFT (FINUFFT's type 1 with iflag +1, the transform of
+ * closed_form_G's 2D lateral sum with lateral=:nufft,
+ * mod_ClosedFormField.jl:758-780):
+ *   f[t][k2][k1] = sum_j c[t][j] exp(+i (k1 x_j + k2 y_j)),
+ *   k1 = -floor(N1/2) .. , k2 = -floor(N2/2) ..   (k1 fastest: Julia's (N1, N2, ntrans)),
+ * to relative accuracy about eps (1e-15 <= eps < 1).
+ *   x_dev, y_dev: M doubles (any real; used mod 2 pi);
+ *   c_dev: [ntrans][M] complex; f_dev: [ntrans][N2][N1] complex; N1, N2 <= 1024. */
+int ytf_nufft2d1(ytf_ctx *ctx, int64_t n_points, const double *x_dev, const double *y_dev, int ntrans,
+                 const double *c_dev, int N1, int N2, double eps, double *f_dev);
 
 /* Library version string. */
 const char *ytf_version(void);

@@ -208,6 +208,25 @@ def stack_S_batch(st, omega, kx, ky, nthreads=0):
     return dict(S=m(S), V1=m(V1), VN=m(VN), Q1=Q1.reshape(nn, 4), QN=QN.reshape(nn, 4), ok=ok.astype(bool))
 
 
+def nudft2d1(x, y, c, modes=None, N1=None, N2=None, nthreads=0):
+    """sum_j c_j exp(+i (k1 x_j + k2 y_j)) at the listed modes ((n, 2) ints),
+    or on the whole (N1, N2) mode grid k = -floor(N/2).. (returned (N1, N2))."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    c = np.ascontiguousarray(c, np.complex128)
+    grid = modes is None
+    if grid:
+        K1, K2 = np.meshgrid(np.arange(N1) - N1 // 2, np.arange(N2) - N2 // 2, indexing="ij")
+        modes = np.stack([K1.ravel(), K2.ravel()], axis=1)
+    modes = np.asarray(modes)
+    k1 = np.ascontiguousarray(modes[:, 0], np.int32)
+    k2 = np.ascontiguousarray(modes[:, 1], np.int32)
+    f = np.zeros(len(k1), np.complex128)
+    lib().ora_nudft2d1_modes(ct.c_int64(len(x)), _p(x), _p(y), _p(c.view(np.float64)), ct.c_int64(len(k1)),
+                             _p(k1, _i), _p(k2, _i), _p(f.view(np.float64)), ct.c_int(nthreads))
+    return f.reshape(N1, N2) if grid else f
+
+
 def count_node_ops(st, omega, kx, ky, src, obs_layer, zsrc, zobs):
     out = np.zeros(10, np.uint64)
     lib().ora_count_node_ops(*st.args(), ct.c_double(omega), ct.c_double(kx), ct.c_double(ky), ct.c_int(src),

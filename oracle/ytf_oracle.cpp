@@ -1397,6 +1397,30 @@ int ora_stack_S_batch(int n, const double *eps, const double *mu, const double *
   return 0;
 }
 
+// Type-1 nonuniform DFT in 2D, summed directly (the definition FINUFFT's
+// type 1 with iflag +1 approximates; the 2D lateral sum of closed_form_G,
+// mod_ClosedFormField.jl:743-780): for each listed mode (k1, k2),
+//   f = sum_j c_j exp(+i (k1 x_j + k2 y_j)).
+int ora_nudft2d1_modes(int64_t M, const double *x, const double *y, const double *c, int64_t nmodes, const int *k1,
+                       const int *k2, double *f, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (int64_t m = 0; m < nmodes; ++m) {
+    double sr = 0.0, si = 0.0;
+    for (int64_t j = 0; j < M; ++j) {
+      const double ph = k1[m] * x[j] + k2[m] * y[j];
+      const double cr = std::cos(ph), ci = std::sin(ph);
+      sr += c[2 * j] * cr - c[2 * j + 1] * ci;
+      si += c[2 * j] * ci + c[2 * j + 1] * cr;
+    }
+    f[2 * m] = sr;
+    f[2 * m + 1] = si;
+  }
+  return 0;
+}
+
 // Op count of one node solve + one spectral dyadic (reference formulation).
 // out: add, mul, div, sqrt, trans.
 int ora_count_node_ops(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,

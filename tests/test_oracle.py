@@ -457,3 +457,36 @@ def test_homogeneous_stack_has_no_scattered_term():
     g = po.ghat_batch(st, W0, kx, ky, 2, 2, [0.3], [0.7], terms="all")
     gs = po.ghat_batch(st, W0, kx, ky, 2, 2, [0.3], [0.7], terms="scattered")
     assert np.linalg.norm(gs) < 1e-14 * np.linalg.norm(g)
+
+
+# ---------------------------------------------------------------------------
+# Nonuniform DFT (the transform FINUFFT's type 1 approximates, the 2D lateral
+# sum of mod_ClosedFormField.jl:743-780): pinned by the uniform-grid case,
+# where it is a plain DFT (numpy.fft), and by a brute-force sum.
+# ---------------------------------------------------------------------------
+def test_nudft_on_uniform_points_is_the_dft():
+    N1, N2 = 12, 10
+    m1, m2 = np.meshgrid(np.arange(N1), np.arange(N2), indexing="ij")
+    x = 2 * np.pi * m1.ravel() / N1 - np.pi
+    y = 2 * np.pi * m2.ravel() / N2 - np.pi
+    rng = np.random.default_rng(4)
+    c = rng.normal(size=x.size) + 1j * rng.normal(size=x.size)
+    f = po.nudft2d1(x, y, c, N1=N1, N2=N2)
+    # sum_m c_m e^{i k (2 pi m / N - pi)} = (-1)^k N1 N2 ifft2(c)[k mod N]
+    F = np.fft.ifft2(c.reshape(N1, N2)) * N1 * N2
+    k1 = np.arange(N1) - N1 // 2
+    k2 = np.arange(N2) - N2 // 2
+    ref = F[np.ix_(k1 % N1, k2 % N2)] * ((-1.0) ** k1)[:, None] * ((-1.0) ** k2)[None, :]
+    assert relerr(f, ref) < 1e-13
+
+
+def test_nudft_against_brute_force_and_linearity():
+    rng = np.random.default_rng(5)
+    x, y = rng.uniform(-np.pi, np.pi, 40), rng.uniform(-np.pi, np.pi, 40)
+    c = rng.normal(size=40) + 1j * rng.normal(size=40)
+    modes = np.array([[0, 0], [3, -2], [-5, 7], [11, 1]])
+    f = po.nudft2d1(x, y, c, modes=modes)
+    for m, (a, b) in enumerate(modes):
+        assert abs(f[m] - sum(c[j] * np.exp(1j * (a * x[j] + b * y[j])) for j in range(40))) < 1e-12
+    f2 = po.nudft2d1(x, y, (2 - 1j) * c, modes=modes)
+    assert relerr(f2, (2 - 1j) * f) < 1e-14

@@ -12,10 +12,11 @@ from .materials import (IsotropicMaterial, UniaxialMaterial, BiaxialMaterial, Gy
 from .green import (green_nodes, build_cavity_transfer, uniform_kxky_grid, fft_wavenumbers,
                     SpectralGreenOperator, NodeCache)
 from .synthesis import CavityTransfer1D, closed_form_G, radial_synthesis
+from .nufft import nufft2d1
 from . import quadrature
 
 __all__ = [
     "IsotropicMaterial", "UniaxialMaterial", "BiaxialMaterial", "GyrotropicMaterial", "FlatOpticalLayer",
     "FlatLayer", "stack_arrays", "green_nodes", "build_cavity_transfer", "uniform_kxky_grid", "fft_wavenumbers",
-    "SpectralGreenOperator", "NodeCache", "CavityTransfer1D", "closed_form_G", "radial_synthesis", "quadrature",
+    "SpectralGreenOperator", "NodeCache", "CavityTransfer1D", "closed_form_G", "radial_synthesis", "quadrature", "nufft2d1",
 ]

@@ -79,6 +79,8 @@ _SIGS = {
                                      ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
     "ytf_mlp_train_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_float,
                                       ct.c_void_p]),
+    "ytf_nufft2d1": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
+                                ct.c_int, ct.c_double, ct.c_void_p]),
 }
 
 TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}

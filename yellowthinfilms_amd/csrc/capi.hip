@@ -10,6 +10,7 @@
 #include <random>
 #include <sstream>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -35,6 +36,8 @@ struct ytf_ctx {
   std::vector<char> small_host;
   // FFT twiddle tables, one per length
   std::map<int, cd *> tw;
+  // NUFFT deconvolution factors per (modes, fine grid, kernel width)
+  std::map<std::tuple<int, int, int>, double *> nu_deconv;
   // scratch of the synthesis kernels and its host-side inputs (radii, pixel
   // coordinates, radius map), grown on demand, freed with the context
   void *work = nullptr;
@@ -186,6 +189,7 @@ int ytf_ctx_destroy(ytf_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   for (auto &kv : c->tw) hipFree(kv.second);
+  for (auto &kv : c->nu_deconv) hipFree(kv.second);
   for (auto &m : c->marks) {
     hipEventDestroy(m.a);
     hipEventDestroy(m.b);
@@ -1006,7 +1010,7 @@ static int mlp_loss_grad_locked(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *
     e = timed(c, "mlp_gemm", [&] { return ytf::launch_mlp_gemm(g, ytf::MLP_EPI_F32, true, true, c->stream); });
     if (e == hipSuccess && splits > 1)
       e = ytf::launch_mlp_reduce(g.split_stride, splits, g.split_stride, A.part, m->G + m->offW[l], c->stream);
-    if (e == hipSuccess) e = ytf::launch_mlp_colsum(rows, m->Np[l], dZ, m->G + m->offb[l], c->stream);
+    if (e == hipSuccess) e = ytf::launch_mlp_colsum(rows, m->Np[l], dZ, A.part, m->G + m->offb[l], c->stream);
     if (e == hipSuccess && l > 0) {
       // dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1})
       ytf::MlpGemm h{};
@@ -1087,6 +1091,117 @@ int ytf_mlp_forward(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, float
     if (e == hipSuccess) e = ytf::launch_mlp_outputs(nc, m->nout, m->Np[L], A.Y, osh, osc, y_dev + r0 * m->nout, c->stream);
     if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_forward: launch failed: %s", hipGetErrorString(e));
   }
+  return YTF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// 2D type-1 nonuniform FFT (nufft.hip)
+// ---------------------------------------------------------------------------
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton on P_n)
+static void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double dz = p1 / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
+// (-1)^k h / psi(k), k = -N/2 .. N - N/2 - 1, psi(k) = a int phi(z) cos(k a z) dz
+static std::vector<double> nu_deconv(int N, int nf, int w, double beta) {
+  const double h = 2.0 * 3.14159265358979323846 / nf, a = 0.5 * w * h;
+  std::vector<double> zq, wq;
+  gauss_legendre(4 * w + 80, zq, wq);
+  std::vector<double> phi(zq.size());
+  for (size_t q = 0; q < zq.size(); ++q) phi[q] = std::exp(beta * (std::sqrt(1.0 - zq[q] * zq[q]) - 1.0));
+  std::vector<double> d(N);
+  for (int kk = 0; kk < N; ++kk) {
+    const int k = kk - N / 2;
+    double s = 0.0;
+    for (size_t q = 0; q < zq.size(); ++q) s += wq[q] * phi[q] * std::cos(k * a * zq[q]);
+    const double psi = a * s;
+    d[kk] = ((k % 2) ? -1.0 : 1.0) * h / psi;
+  }
+  return d;
+}
+
+int ytf_nufft2d1(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev, int ntrans, const double *c_dev,
+                 int N1, int N2, double eps, double *f_dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_nufft2d1: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (M < 0 || M > ((int64_t)1 << 31) - 1 || ntrans < 1 || N1 < 2 || N2 < 2 || N1 > 1024 || N2 > 1024)
+    return fail(c, YTF_EINVAL, "ytf_nufft2d1: bad sizes (M=%lld, ntrans=%d, N1=%d, N2=%d; modes 2..1024)",
+                (long long)M, ntrans, N1, N2);
+  if (!(eps >= 1e-15 && eps < 1.0)) return fail(c, YTF_EINVAL, "ytf_nufft2d1: eps=%g must be in [1e-15, 1)", eps);
+  if (!f_dev || (M > 0 && (!x_dev || !y_dev || !c_dev))) return fail(c, YTF_EINVAL, "ytf_nufft2d1: NULL buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int w = ytf::nu_width(eps);
+  const double beta = 2.30 * w;
+  int nf = 64;
+  while (nf < 2 * N1 || nf < 2 * N2 || nf < 4 * w) nf *= 2;
+  const int nt = nf / 32, ntiles = nt * nt;
+  const int64_t Mc = M > 0 ? M : 1;
+  const size_t uv_b = align256(sizeof(double) * 2 * Mc), i_b = align256(sizeof(int) * Mc),
+               t_b = align256(sizeof(int) * (ntiles + 1)), g_b = align256(sizeof(cd) * (size_t)nf * nf),
+               T_b = align256(sizeof(cd) * (size_t)N1 * nf);
+  int rc = ensure_buf(c, &c->work, &c->work_bytes, uv_b + 2 * i_b + 3 * t_b + g_b + T_b);
+  if (rc) return rc;
+  char *p = (char *)c->work;
+  double *uv = (double *)p; p += uv_b;
+  int *tile = (int *)p; p += i_b;
+  int *order = (int *)p; p += i_b;
+  int *count = (int *)p; p += t_b;
+  int *start = (int *)p; p += t_b;
+  int *cursor = (int *)p; p += t_b;
+  cd *grid = (cd *)p; p += g_b;
+  cd *T1 = (cd *)p;
+  double *dd1 = nullptr, *dd2 = nullptr;
+  for (int dim = 0; dim < 2; ++dim) {
+    const int N = dim ? N2 : N1;
+    auto key = std::make_tuple(N, nf, w);
+    auto it = c->nu_deconv.find(key);
+    if (it == c->nu_deconv.end()) {
+      const std::vector<double> d = nu_deconv(N, nf, w, beta);
+      double *dp = nullptr;
+      HIPCHK(c, hipMalloc(&dp, sizeof(double) * N));
+      HIPCHK(c, hipMemcpy(dp, d.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+      it = c->nu_deconv.emplace(key, dp).first;
+    }
+    (dim ? dd2 : dd1) = it->second;
+  }
+  cd *tw = nullptr;
+  rc = twiddles(c, nf, &tw);
+  if (rc) return rc;
+  hipError_t e = timed(c, "nufft_sort", [&] {
+    return ytf::launch_nufft_sort(M, x_dev, y_dev, nf, uv, tile, count, start, cursor, order, c->stream);
+  });
+  for (int t = 0; t < ntrans && e == hipSuccess; ++t) {
+    e = timed(c, "nufft_spread", [&] {
+      return ytf::launch_nufft_spread(nf, w, beta, start, order, uv, (const cd *)c_dev + (size_t)t * Mc, grid,
+                                      c->stream);
+    });
+    if (e == hipSuccess)
+      e = timed(c, "nufft_fft", [&] {
+        return ytf::launch_nufft_fft(nf, grid, T1, (cd *)f_dev + (size_t)t * N1 * N2, N1, N2, dd1, dd2, tw,
+                                     c->stream);
+      });
+  }
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_nufft2d1: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
 

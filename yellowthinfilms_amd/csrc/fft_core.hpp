@@ -1,0 +1,102 @@
+// fft_core.hpp -- radix-8 Stockham FP64 FFT of one row held in LDS (device
+// code shared by the spectral convolution and the nonuniform FFT).
+// tw[j] = exp(-2 pi i j / N); INV selects exp(+...) (unnormalised).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ytf_device.hpp"
+
+namespace ytf {
+
+template <bool INV>
+__device__ __forceinline__ cd mul_mi(cd a) {  // a * (-i) forward, a * (+i) inverse
+  return INV ? cd{-a.i, a.r} : cd{a.i, -a.r};
+}
+template <bool INV>
+__device__ __forceinline__ void dft2(cd &a0, cd &a1) {
+  const cd t = a0;
+  a0 = t + a1;
+  a1 = t - a1;
+}
+template <bool INV>
+__device__ __forceinline__ void dft4(cd &a0, cd &a1, cd &a2, cd &a3) {
+  const cd t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_mi<INV>(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+template <bool INV>
+__device__ __forceinline__ void dft8(cd *a) {
+  cd e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  cd o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<INV>(e0, e1, e2, e3);
+  dft4<INV>(o0, o1, o2, o3);
+  const double h = 0.70710678118654752440;
+  // w8^k o_k, w8 = exp(-+ 2 pi i / 8)
+  const cd w1 = INV ? cd{h, h} : cd{h, -h};
+  const cd w3 = INV ? cd{-h, h} : cd{-h, -h};
+  o1 = o1 * w1;
+  o2 = mul_mi<INV>(o2);
+  o3 = o3 * w3;
+  a[0] = e0 + o0; a[4] = e0 - o0;
+  a[1] = e1 + o1; a[5] = e1 - o1;
+  a[2] = e2 + o2; a[6] = e2 - o2;
+  a[3] = e3 + o3; a[7] = e3 - o3;
+}
+
+// One Stockham radix-R stage on a row in LDS: current length n, stride s.
+// Thread t in [0, N/8) performs 8/R butterflies.  tw[j] = exp(-2 pi i j/N).
+template <int N, int R, bool INV>
+__device__ __forceinline__ void stage(cd *row, int t, int n, int s, const cd *tw) {
+  constexpr int NB = 8 / R;
+  const int m = n / R;
+  cd a[8];
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * (N / 8);
+    const int q = b % s, p = b / s;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[u * R + r] = row[q + s * (p + r * m)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = t + u * (N / 8);
+    const int q = b % s, p = b / s;
+    cd *x = a + u * R;
+    if (R == 8) dft8<INV>(x);
+    if (R == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
+    if (R == 2) dft2<INV>(x[0], x[1]);
+    const int step = N / n;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      cd w = tw[(p * k) * step];
+      if (INV) w.i = -w.i;
+      row[q + s * (R * p + k)] = k == 0 ? x[0] : x[k] * w;
+    }
+  }
+  __syncthreads();
+}
+
+// Full length-N transform of `row` (in LDS) by N/8 cooperating threads; all
+// threads of the block must call it (it synchronises the block).
+template <int N, bool INV>
+__device__ __forceinline__ void fft_lds(cd *row, int t, const cd *tw) {
+  int n = N, s = 1;
+#pragma unroll
+  for (int st = 0; st < 12; ++st) {
+    if (n >= 8) {
+      stage<N, 8, INV>(row, t, n, s, tw);
+      n /= 8;
+      s *= 8;
+    }
+  }
+  if (n == 4) stage<N, 4, INV>(row, t, n, s, tw);
+  if (n == 2) stage<N, 2, INV>(row, t, n, s, tw);
+}
+
+// Rows per block of the row passes (YT rows x N/8 threads each, <= 1024 threads).
+template <int N> struct XTile { static constexpr int YT = (8192 / N) < 8 ? (8192 / N) : 8; };
+
+}  // namespace ytf

@@ -175,13 +175,22 @@ __global__ void k_mlp_mse(int64_t n, int64_t rows, int nout, int ldy, const floa
   }
 }
 
-// db[c] = sum over rows of dZ[r][c]
-__global__ void k_mlp_colsum(int64_t rows, int ld, const __bf16 *__restrict__ dZ, float *__restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ld) return;
+// Column sums of dZ [rows][ld] in two deterministic stages: block (x, z)
+// sums 64 columns over row slice z (4 row groups per block, coalesced 64-column
+// rows), part[z][c]; then k_mlp_reduce adds the slices in order.
+__global__ __launch_bounds__(256) void k_mlp_colsum(int64_t rows, int ld, int slices, const __bf16 *__restrict__ dZ,
+                                                    float *__restrict__ part) {
+  __shared__ float red[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  const int64_t per = (rows + slices - 1) / slices;
+  const int64_t r0 = blockIdx.y * per, r1 = r0 + per < rows ? r0 + per : rows;
   float s = 0.0f;
-  for (int64_t r = 0; r < rows; ++r) s += (float)dZ[r * ld + c];
-  db[c] = s;
+  if (c < ld)
+    for (int64_t r = r0 + g; r < r1; r += 4) s += (float)dZ[r * ld + c];
+  red[g][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (g == 0 && c < ld) part[(int64_t)blockIdx.y * ld + c] = red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                                              red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
 // Y[rows][ldy] (normalised) -> y[n][nout] raw
@@ -235,8 +244,16 @@ hipError_t launch_mlp_mse(int64_t n, int64_t rows, int nout, int ldy, const floa
                      oscale, (__bf16 *)dZ, loss);
   return hipGetLastError();
 }
-hipError_t launch_mlp_colsum(int64_t rows, int ld, const void *dZ, float *db, hipStream_t s) {
-  hipLaunchKernelGGL(k_mlp_colsum, dim3((ld + 63) / 64), dim3(64), 0, s, rows, ld, (const __bf16 *)dZ, db);
+hipError_t launch_mlp_colsum(int64_t rows, int ld, const void *dZ, float *part, float *db, hipStream_t s) {
+  int slices = (int)((rows + 255) / 256);
+  if (slices > 64) slices = 64;
+  if (slices < 1) slices = 1;
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((ld + 63) / 64, slices), dim3(256), 0, s, rows, ld, slices,
+                     (const __bf16 *)dZ, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((ld + 255) / 256), dim3(256), 0, s, (int64_t)ld, slices, (int64_t)ld,
+                     (const float *)part, db);
   return hipGetLastError();
 }
 hipError_t launch_mlp_outputs(int64_t n, int nout, int ldy, const float *Y, const float *oshift, const float *oscale,

@@ -1,0 +1,286 @@
+// nufft.hip -- 2D type-1 nonuniform FFT, FP64, on gfx950:
+//     f[t][k2][k1] = sum_j c[t][j] exp(+i (k1 x_j + k2 y_j)),
+//     k1 in [-N1/2, N1/2), k2 in [-N2/2, N2/2) (N even; -floor(N/2).. for odd N),
+// the transform of the 2D lateral sum of closed_form_G (FINUFFT type 1,
+// iflag +1, mod_ClosedFormField.jl:758-780), to a tolerance eps.
+//
+// Method (spread, FFT, deconvolve): with x' = x + pi folded into [0, 2 pi),
+// h = 2 pi / nf (nf = fine grid size, a power of two >= 2N) and the
+// "exponential of semicircle" kernel phi(z) = exp(beta (sqrt(1 - z^2) - 1))
+// of half-width a = w h / 2 (w grid points, beta = 2.30 w),
+//     b_m = sum_j c_j phi((m h - x'_j) / a)        (periodically, per dimension),
+//     f_k = (-1)^k h F_k / psi(k),  F_k = sum_m b_m exp(+2 pi i k m / nf),
+//     psi(k) = a int_{-1}^{1} phi(z) cos(k a z) dz,
+// separably in 2D.  The spreading is binned by 32 x 32 fine-grid tiles: one
+// block per tile accumulates its points (with a halo of the kernel half-width)
+// in LDS with FP64 LDS atomics, then adds the tile to the fine grid with FP64
+// global atomics.  Two LDS-staged FFT passes (rows, then the kept columns)
+// compute only the N1 x N2 wanted modes and apply the deconvolution.
+#include <hip/hip_runtime.h>
+
+#include "fft_core.hpp"
+#include "ytf_kernels.h"
+
+namespace ytf {
+
+constexpr int NU_TS = 32;  // spreading tile (fine-grid points per side)
+
+__device__ __forceinline__ double fold2pi(double x) {
+  // x + pi reduced to [0, 2 pi)
+  const double tp = 6.283185307179586476925287;
+  double u = x + 3.141592653589793238462643;
+  u -= tp * floor(u / tp);
+  if (u >= tp) u -= tp;
+  if (u < 0.0) u = 0.0;
+  return u;
+}
+
+__device__ __forceinline__ double es_kernel(double z, double beta) {
+  const double s = 1.0 - z * z;
+  return s > 0.0 ? exp(beta * (sqrt(s) - 1.0)) : 0.0;
+}
+
+// fine-grid coordinates (u, v) in [0, nf) and the tile of each point
+__global__ void k_nu_bin(int64_t M, const double *__restrict__ x, const double *__restrict__ y, int nf, int nt,
+                         double *__restrict__ uv, int *__restrict__ tile, int *__restrict__ count) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const double sc = nf / 6.283185307179586476925287;
+  double u = fold2pi(x[j]) * sc, v = fold2pi(y[j]) * sc;
+  if (u >= nf) u -= nf;
+  if (v >= nf) v -= nf;
+  int tx = (int)(u / NU_TS), ty = (int)(v / NU_TS);
+  tx = tx < nt ? tx : nt - 1;
+  ty = ty < nt ? ty : nt - 1;
+  uv[2 * j] = u;
+  uv[2 * j + 1] = v;
+  const int t = tx + nt * ty;
+  tile[j] = t;
+  atomicAdd(count + t, 1);
+}
+
+// exclusive scan of the tile counts (one block); cursor = start
+__global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__restrict__ start, int *__restrict__ cursor) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (ntiles + 1023) / 1024;
+  int s = 0;
+  for (int i = 0; i < per; ++i) {
+    const int t = tid * per + i;
+    if (t < ntiles) s += count[t];
+  }
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = tid ? part[tid - 1] : 0;
+  for (int i = 0; i < per; ++i) {
+    const int t = tid * per + i;
+    if (t < ntiles) {
+      start[t] = run;
+      cursor[t] = run;
+      run += count[t];
+    }
+  }
+  if (tid == 1023) start[ntiles] = part[1023];
+}
+
+__global__ void k_nu_scatter(int64_t M, const int *__restrict__ tile, int *__restrict__ cursor, int *__restrict__ order) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const int pos = atomicAdd(cursor + tile[j], 1);
+  order[pos] = (int)j;
+}
+
+__global__ void k_nu_zero_int(int n, int *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ void k_nu_zero(int64_t n, cd *__restrict__ g) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) g[i] = cd{0.0, 0.0};
+}
+
+// One block per tile: spread the tile's points into LDS (tile + halo), then
+// add the LDS tile to the periodic fine grid.
+template <int W>
+__global__ __launch_bounds__(256) void k_nu_spread(int nf, int nt, double beta, const int *__restrict__ start,
+                                                   const int *__restrict__ order, const double *__restrict__ uv,
+                                                   const cd *__restrict__ c, cd *__restrict__ grid) {
+  constexpr int H = (W + 1) / 2;
+  constexpr int S = NU_TS + 2 * H;
+  __shared__ double tre[S * S], tim[S * S];
+  const int t = blockIdx.x;
+  const int tx = t % nt, ty = t / nt;
+  for (int i = threadIdx.x; i < S * S; i += blockDim.x) { tre[i] = 0.0; tim[i] = 0.0; }
+  __syncthreads();
+  const int p0 = start[t], p1 = start[t + 1];
+  const double inv_half = 2.0 / W;
+  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    const int j = order[p];
+    const double u = uv[2 * j], v = uv[2 * j + 1];
+    const cd cj = c[j];
+    const double mu = ceil(u - 0.5 * W), mv = ceil(v - 0.5 * W);
+    double ku[W], kv[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      ku[i] = es_kernel((mu + i - u) * inv_half, beta);
+      kv[i] = es_kernel((mv + i - v) * inv_half, beta);
+    }
+    const int ou = (int)mu - tx * NU_TS + H, ov = (int)mv - ty * NU_TS + H;
+#pragma unroll
+    for (int b = 0; b < W; ++b) {
+      const double wr = kv[b] * cj.r, wi = kv[b] * cj.i;
+      double *rr = tre + (ov + b) * S + ou, *ri = tim + (ov + b) * S + ou;
+#pragma unroll
+      for (int a = 0; a < W; ++a) {
+        atomicAdd(rr + a, ku[a] * wr);
+        atomicAdd(ri + a, ku[a] * wi);
+      }
+    }
+  }
+  __syncthreads();
+  if (p1 == p0) return;
+  for (int i = threadIdx.x; i < S * S; i += blockDim.x) {
+    const int a = i % S, b = i / S;
+    int gu = tx * NU_TS - H + a, gv = ty * NU_TS - H + b;
+    gu += gu < 0 ? nf : 0;
+    gu -= gu >= nf ? nf : 0;
+    gv += gv < 0 ? nf : 0;
+    gv -= gv >= nf ? nf : 0;
+    double *gp = (double *)(grid + (int64_t)gv * nf + gu);
+    if (tre[i] != 0.0) unsafeAtomicAdd(gp, tre[i]);
+    if (tim[i] != 0.0) unsafeAtomicAdd(gp + 1, tim[i]);
+  }
+}
+
+// Pass 1: exp(+) FFT of each fine-grid row v over u; keep modes k1, stored
+// transposed: T1[(k1 + N1/2)][v].
+template <int NF>
+__global__ __launch_bounds__(1024) void k_nu_rows(const cd *__restrict__ grid, cd *__restrict__ T1, int N1,
+                                                  const cd *__restrict__ tw) {
+  constexpr int YT = XTile<NF>::YT;
+  constexpr int TPR = NF / 8;
+  extern __shared__ cd lds[];
+  const int v0 = blockIdx.x * YT, tid = threadIdx.x;
+  for (int e = tid; e < YT * NF; e += YT * TPR) lds[e] = grid[(int64_t)v0 * NF + e];
+  __syncthreads();
+  fft_lds<NF, true>(lds + (tid / TPR) * NF, tid % TPR, tw);
+  const int h1 = N1 / 2;
+  for (int e = tid; e < YT * N1; e += YT * TPR) {
+    const int r = e % YT, kk = e / YT;      // kk = k1 + N1/2
+    const int k1 = kk - h1;
+    const int idx = k1 < 0 ? k1 + NF : k1;
+    T1[(int64_t)kk * NF + v0 + r] = lds[r * NF + idx];
+  }
+}
+
+// Pass 2: exp(+) FFT over v of each kept k1 row; keep modes k2, deconvolve,
+// f[(k2 + N2/2) * N1 + (k1 + N1/2)].
+template <int NF>
+__global__ __launch_bounds__(1024) void k_nu_cols(const cd *__restrict__ T1, cd *__restrict__ f, int N1, int N2,
+                                                  const double *__restrict__ d1, const double *__restrict__ d2,
+                                                  const cd *__restrict__ tw) {
+  constexpr int YT = XTile<NF>::YT;
+  constexpr int TPR = NF / 8;
+  extern __shared__ cd lds[];
+  const int r0 = blockIdx.x * YT, tid = threadIdx.x;
+  const int nrow = N1 - r0 < YT ? N1 - r0 : YT;
+  for (int e = tid; e < YT * NF; e += YT * TPR) {
+    const int r = e / NF;
+    lds[e] = r < nrow ? T1[(int64_t)r0 * NF + e] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_lds<NF, true>(lds + (tid / TPR) * NF, tid % TPR, tw);
+  const int h2 = N2 / 2;
+  for (int e = tid; e < YT * N2; e += YT * TPR) {
+    const int r = e % YT, kk2 = e / YT;
+    if (r >= nrow) continue;
+    const int k2 = kk2 - h2;
+    const int idx = k2 < 0 ? k2 + NF : k2;
+    const double s = d1[r0 + r] * d2[kk2];
+    f[(int64_t)kk2 * N1 + r0 + r] = rmul(s, lds[r * NF + idx]);
+  }
+}
+
+int nu_width(double eps) {
+  int w = (int)ceil(log10(1.0 / eps)) + 1;
+  if (w < 2) w = 2;
+  if (w > 16) w = 16;
+  return w + (w & 1);  // even widths only
+}
+
+template <int NF>
+static hipError_t run_ffts(const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
+                           const cd *tw, hipStream_t s) {
+  constexpr int YT = XTile<NF>::YT;
+  const size_t sh = sizeof(cd) * YT * NF;
+  hipLaunchKernelGGL(k_nu_rows<NF>, dim3(NF / YT), dim3(YT * NF / 8), sh, s, grid, T1, N1, tw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nu_cols<NF>, dim3((N1 + YT - 1) / YT), dim3(YT * NF / 8), sh, s, (const cd *)T1, f, N1, N2,
+                     d1, d2, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
+                             int *start, int *cursor, int *order, hipStream_t s) {
+  const int nt = nf / NU_TS;
+  const int ntiles = nt * nt;
+  hipLaunchKernelGGL(k_nu_zero_int, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || M == 0) {
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor);
+      e = hipGetLastError();
+    }
+    return e;
+  }
+  hipLaunchKernelGGL(k_nu_bin, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, x, y, nf, nt, uv, tile, count);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nu_scatter, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, (const int *)tile, cursor,
+                     order);
+  return hipGetLastError();
+}
+
+hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, const int *order, const double *uv,
+                               const cd *c, cd *grid, hipStream_t s) {
+  const int nt = nf / NU_TS;
+  const int64_t ng = (int64_t)nf * nf;
+  hipLaunchKernelGGL(k_nu_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, ng, grid);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+#define YTF_SPREAD(WW)                                                                                             \
+  case WW:                                                                                                         \
+    hipLaunchKernelGGL(k_nu_spread<WW>, dim3(nt * nt), dim3(256), 0, s, nf, nt, beta, start, order, uv, c, grid); \
+    break;
+  switch (w) {
+    YTF_SPREAD(2) YTF_SPREAD(4) YTF_SPREAD(6) YTF_SPREAD(8) YTF_SPREAD(10) YTF_SPREAD(12) YTF_SPREAD(14)
+    YTF_SPREAD(16)
+    default: return hipErrorInvalidValue;
+  }
+#undef YTF_SPREAD
+  return hipGetLastError();
+}
+
+hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
+                            const cd *tw, hipStream_t s) {
+  switch (nf) {
+    case 64: return run_ffts<64>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 128: return run_ffts<128>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 256: return run_ffts<256>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 512: return run_ffts<512>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 1024: return run_ffts<1024>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 2048: return run_ffts<2048>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace ytf

@@ -117,7 +117,7 @@ hipError_t launch_mlp_inputs(int64_t n, int64_t rows, int nin, int ldx, int kzf,
                              const float *scale, void *X, hipStream_t s);
 hipError_t launch_mlp_mse(int64_t n, int64_t rows, int nout, int ldy, const float *Y, const float *y,
                           const float *oshift, const float *oscale, void *dZ, float *loss, hipStream_t s);
-hipError_t launch_mlp_colsum(int64_t rows, int ld, const void *dZ, float *db, hipStream_t s);
+hipError_t launch_mlp_colsum(int64_t rows, int ld, const void *dZ, float *part, float *db, hipStream_t s);
 hipError_t launch_mlp_outputs(int64_t n, int nout, int ldy, const float *Y, const float *oshift, const float *oscale,
                               float *y, hipStream_t s);
 hipError_t launch_mlp_adam(int64_t n, float *w, float *m, float *v, const float *g, void *w16, float lr, float b1,
@@ -125,5 +125,14 @@ hipError_t launch_mlp_adam(int64_t n, float *w, float *m, float *v, const float 
 hipError_t launch_mlp_to_bf16(int64_t n, const float *w, void *w16, hipStream_t s);
 hipError_t launch_mlp_zero(int64_t n, float *x, hipStream_t s);
 hipError_t launch_mlp_reduce(int64_t n, int splits, int64_t stride, const float *part, float *dst, hipStream_t s);
+
+// 2D type-1 nonuniform FFT (nufft.hip).
+int nu_width(double eps);
+hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
+                             int *start, int *cursor, int *order, hipStream_t s);
+hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, const int *order, const double *uv,
+                               const cd *c, cd *grid, hipStream_t s);
+hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
+                            const cd *tw, hipStream_t s);
 
 }  // namespace ytf

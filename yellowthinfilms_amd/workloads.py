@@ -145,6 +145,35 @@ def uneven_batch(wl=UNEVEN_256):
     return stacks, counts, kr * np.cos(ph), kr * np.sin(ph)
 
 
+SURROGATE_1024 = dict(
+    name="surrogate_1024",
+    description=("learned surrogate of the scattered spectral dyadic of a seeded 4-layer gyrotropic family "
+                 "(g ~ U[0,0.5] drawn once): MLP 6->256x4->18 SiLU in BF16 with FP32 accumulation, trained on 2^22 "
+                 "exact FP64 solver samples (kx, ky, z, z', omega, loss); evaluated on a 1024x1024 uniform grid "
+                 "(|k| <= 3 omega) at z = 40 nm, z' = 50 nm, omega = 2pi, loss = 0.1, with the exact solve on "
+                 "flagged nodes (light line, origin, large response), against the plain GPU solve"),
+    family_seed=1, seed=7, data_seed=2, width=256, depth=4, n_train=1 << 22, epochs=8, batch=8192, lr=2e-3,
+    f=1.0, L=1024 / 12.0, dx=1.0 / 6.0, h_src_nm=40.0, h_obs_nm=50.0, loss=0.1,
+)
+
+
+NUFFT_1E6 = dict(
+    name="nufft_1e6",
+    description=("2D type-1 nonuniform FFT (iflag +1) of 10^6 seeded points (x, y) ~ U[-pi, pi)^2 with complex "
+                 "N(0,1) strengths onto 1024x1024 modes, eps 1e-9, FP64"),
+    M=1_000_000, N=1024, eps=1e-9, seed=1,
+)
+
+
+def nufft_points(wl=NUFFT_1E6):
+    rng = np.random.default_rng(wl["seed"])
+    M = wl["M"]
+    x = rng.uniform(-math.pi, math.pi, M)
+    y = rng.uniform(-math.pi, math.pi, M)
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)) / math.sqrt(2.0)
+    return x, y, c
+
+
 # Algorithmic FP64 operations of one uneven_256 launch: per stack, the mean
 # op count (real adds + multiplies + divides) of Modes_Stack + the total S in
 # the reference's formulation over 32 seeded sample nodes (rng seed 0),
