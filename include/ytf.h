@@ -79,7 +79,8 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
  * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
  * "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam", "nufft_sort",
- * "nufft_spread", "nufft_fft". */
+ * "nufft_spread", "nufft_fft", "lateral_direct", "op_fft_fwd", "op_mix",
+ * "op_fft_inv". */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
 int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
@@ -254,9 +255,116 @@ int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev
  *   k1 = -floor(N1/2) .. , k2 = -floor(N2/2) ..   (k1 fastest: Julia's (N1, N2, ntrans)),
  * to relative accuracy about eps (1e-15 <= eps < 1).
  *   x_dev, y_dev: M doubles (any real; used mod 2 pi);
- *   c_dev: [ntrans][M] complex; f_dev: [ntrans][N2][N1] complex; N1, N2 <= 1024. */
+ *   c_dev: [ntrans][M] complex; f_dev: [ntrans][N2][N1] complex; N1, N2 <= 2048. */
 int ytf_nufft2d1(ytf_ctx *ctx, int64_t n_points, const double *x_dev, const double *y_dev, int ntrans,
                  const double *c_dev, int N1, int N2, double eps, double *f_dev);
+
+/* Lateral sum of the 2D path. */
+#define YTF_LATERAL_DIRECT 0 /* exact separable sum (lateral=:direct)         */
+#define YTF_LATERAL_NUFFT 1  /* type-1 NUFFT, accuracy nufft_eps (:nufft)     */
+
+/* closed_form_G for a CavityTransferCache2D and a point source
+ * (mod_ClosedFormField.jl:711-782):
+ *   G(x_i, y_j, p, c, q) = sum_n W_n g~_p(n)[c + 3q] exp(i (kx_n x_i + ky_n y_j))
+ * with W_n the node weights (k_rho w_rho w_phi / 4 pi^2, times
+ * box_lateral_factor for a box footprint, :723-727); non-finite weighted
+ * strengths are zeroed (_finite_or_zero).
+ *   kx_dev, ky_dev: n_nodes doubles; w_dev: n_nodes complex;
+ *   ghat_dev: [n_pairs][9][n_nodes] complex (ytf_green_nodes), pair p = z row p;
+ *   xs, ys: host arrays of lateral offsets; YTF_LATERAL_NUFFT needs uniform
+ *     xs, ys with nx, ny <= 2048, and 1e-15 <= nufft_eps < 1;
+ *   G_dev: (nx, ny, n_pairs, 3, 3) complex column-major. */
+int ytf_closed_form_G_2d(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *ky_dev,
+                         const double *w_dev, int n_pairs, const double *ghat_dev, const double *xs, int nx,
+                         const double *ys, int ny, int lateral, double nufft_eps, double *G_dev);
+
+/* The pixel grid of uniform_kxky_grid (mod_UniformSource.jl:89-116) as the
+ * operator uses it: Nx x Ny pixels of pitch dx, dy.  The operator samples G
+ * on the 2Nx x 2Ny padded offsets (padded_offsets,
+ * mod_ClosedFormAssembly.jl:75-79); Nx, Ny must be powers of two in 4..1024. */
+typedef struct ytf_grid {
+  int32_t Nx, Ny;
+  double dx, dy;
+} ytf_grid;
+
+#define YTF_STRUCTURE_AUTO 0       /* structure=:auto        */
+#define YTF_STRUCTURE_STRUCTURED 1 /* structure=:structured  */
+#define YTF_STRUCTURE_DENSE 2      /* structure=:dense       */
+
+/* Keyword arguments of build_layer_operator (mod_GreenOperator.jl:430-433)
+ * and of the closed_form_G calls it makes. */
+typedef struct ytf_operator_options {
+  int32_t structure;   /* YTF_STRUCTURE_*                              (:auto)  */
+  int32_t check;       /* numerical Toeplitz+Hankel check under :auto  (true)   */
+  double check_tol;    /* residual above which the check fails         (1e-8)   */
+  int32_t skeleton;    /* skeleton planes for blocks without z structure (true) */
+  double skeleton_tol; /* interpolation error per node term            (1e-6)   */
+  int32_t lateral;     /* YTF_LATERAL_*                                (:direct)*/
+  double nufft_eps;    /*                                              (1e-9)   */
+  /* 1: z grids must be uniform (build_layer_operator); 0: any depths, dense
+   * blocks only (closed_form_field / precompute_ghats!, mod_ClosedFormAssembly.jl). */
+  int32_t uniform_z;
+} ytf_operator_options;
+
+/* Block kinds (ZBlock.kind, mod_GreenOperator.jl:85-96). */
+#define YTF_BLOCK_STRUCTURED 0
+#define YTF_BLOCK_TOEPLITZ_DENSE 1
+#define YTF_BLOCK_DENSE 2
+#define YTF_BLOCK_SKELETON 3
+#define YTF_BLOCK_TOEPLITZ_SKELETON 4
+
+/* Bits of build_layer_operator's `warnings` output. */
+#define YTF_WARN_STRUCTURE_CHECK 1u /* a Toeplitz+Hankel check failed: dense form used (the @warn of :462) */
+
+/* Precomputed Green operator of one source layer onto a set of observation
+ * layers on uniform z grids: LayerGreenOperator + build_layer_operator
+ * (mod_GreenOperator.jl:105-113, 430-480).  The nodes are those of a
+ * CavityTransferCache2D (every (k_rho, phi) node, its weight W_n as for
+ * ytf_closed_form_G_2d); the node solve runs inside the build.  A grouped
+ * build (build_layer_operator_grouped, :497-531) is one build on the
+ * concatenated nodes of all groups, since every block is linear in the
+ * quadrature sum.
+ *   z_src: n_src local source depths (Dipole.z), in the caller's order;
+ *   obs_layers: n_obs_layers layer indices (1-based, any order, no repeats);
+ *     n_obs[l] depths of layer obs_layers[l], concatenated in z_obs;
+ *   warnings: YTF_WARN_* bits (may be NULL).
+ * Blocks are stored in ascending observation layer order; a layer with no
+ * depths gets no block. */
+typedef struct ytf_operator ytf_operator;
+int ytf_operator_build(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                       int64_t n_nodes, const double *kx_dev, const double *ky_dev, const double *w_dev,
+                       const ytf_grid *grid, const double *z_src, int n_src, int n_obs_layers,
+                       const int32_t *obs_layers, const int32_t *n_obs, const double *z_obs,
+                       const ytf_operator_options *opt, uint32_t *warnings, ytf_operator **out);
+int ytf_operator_destroy(ytf_ctx *ctx, ytf_operator *op);
+/* apply_operator (mod_GreenOperator.jl:674-712): FFTs, per-node z coupling
+ * and inverse FFTs, all on the device.
+ *   P_dev: (Nx, Ny, n_src, 3) complex column-major dipole moments per pixel
+ *     and source plane (caller's order of z_src);
+ *   E_dev[b]: (Nx, Ny, n_obs(b), 3) complex for block b. */
+int ytf_operator_apply(ytf_ctx *ctx, ytf_operator *op, const double *P_dev, double *const *E_dev);
+
+typedef struct ytf_block_info {
+  int32_t obs_layer;   /* 1-based                                          */
+  int32_t kind;        /* YTF_BLOCK_*                                      */
+  int32_t n_obs;       /* observation planes                               */
+  int32_t n_src_skel;  /* source skeleton planes (n_src when kept whole)   */
+  int32_t n_obs_skel;  /* observation skeleton planes                      */
+  int32_t n_toeplitz;  /* stored Toeplitz planes (0 if none)               */
+  int32_t n_hankel;    /* stored Hankel planes (0 if none)                 */
+  int32_t n_dense;     /* stored dense plane pairs (0 if none)             */
+  int64_t bytes;       /* device bytes of the block                        */
+  double build_ms;     /* wall time of the block's build                   */
+  double check_residual; /* Toeplitz+Hankel check residual, or -1 if not run */
+} ytf_block_info;
+/* operator_summary (mod_GreenOperator.jl:735-747). */
+int ytf_operator_num_blocks(const ytf_operator *op);
+int ytf_operator_block_info(ytf_ctx *ctx, const ytf_operator *op, int block, ytf_block_info *info);
+/* One stored Ĝ plane of a block in the reference layout (Px, Py, 3, 3),
+ * Px = 2Nx, Py = 2Ny: part 0 Toeplitz plane m + n_src - 1 (m = j - i),
+ * part 1 Hankel plane i + j, part 2 dense pair i + n_src_skel * j (source
+ * planes in increasing cavity-frame depth). */
+int ytf_operator_ghat(ytf_ctx *ctx, const ytf_operator *op, int block, int part, int slot, double *ghat_dev);
 
 /* Library version string. */
 const char *ytf_version(void);

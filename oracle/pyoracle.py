@@ -312,3 +312,49 @@ def spectral_apply(g, J, grid):
     S = np.einsum("xycq,xyq->xyc", g, Jh)
     pref = grid["dkx"] * grid["dky"] * grid["Nx"] * grid["Ny"] / (2 * np.pi) ** 2
     return np.stack([pref * np.fft.ifft2(S[:, :, c] * np.conj(ramp)) for c in range(3)], axis=-1)
+
+
+def lateral_sum(g, W, kx, ky, xs, ys):
+    """The exact separable lateral sum of closed_form_G's 2D path
+    (mod_ClosedFormField.jl:743-753):
+        G(x_i, y_j) = sum_n W_n g_n exp(i (kx_n x_i + ky_n y_j)),
+    g: (n, 3, 3) spectral dyadics (c, q).  Returns (nx, ny, 3, 3)."""
+    xs = np.atleast_1d(np.asarray(xs, np.float64))
+    ys = np.atleast_1d(np.asarray(ys, np.float64))
+    Px = np.exp(1j * np.outer(kx, xs))
+    Py = np.exp(1j * np.outer(ky, ys))
+    S = np.asarray(g) * np.asarray(W)[:, None, None]
+    S = np.where(np.isfinite(S), S, 0.0)  # _finite_or_zero
+    return np.einsum("nc,nd,nij->cdij", Px, Py, S)
+
+
+def closed_form_G_2d(st, omega, kx, ky, W, src, obs_layer, zsrc, zobs, xs, ys, terms="all", nthreads=0):
+    """closed_form_G for a CavityTransferCache2D and a point source
+    (mod_ClosedFormField.jl:711-782) with the exact lateral sum: the oracle's
+    per-node spectral dyadic times the node weight, summed over the nodes.
+    Returns (nx, ny, nz, 3, 3) complex (x, y, z, c, q)."""
+    zobs = np.atleast_1d(np.asarray(zobs, np.float64))
+    g = ghat_batch(st, omega, kx, ky, src, obs_layer, np.full(len(zobs), float(zsrc)), zobs, nthreads=nthreads,
+                   terms=terms)
+    return np.stack([lateral_sum(g[:, p], W, kx, ky, xs, ys) for p in range(len(zobs))], axis=2)
+
+
+def linear_convolution(G, P):
+    """The lateral assembly as a literal sum over source pixels
+    (mod_ClosedFormAssembly.jl:10 and apply_operator, mod_GreenOperator.jl:
+    674-712, without the FFTs):
+        E[i, j, o, r] = sum_{i', j', k, c} G[i - i', j - j', k, o, r, c] P[i', j', k, c].
+    G: (2Nx-1, 2Ny-1, Ns, No, 3, 3) on the offsets -(N-1)..(N-1) (index
+    offset + N - 1); P: (Nx, Ny, Ns, 3) moments.  Returns E (Nx, Ny, No, 3)."""
+    Nx, Ny, Ns, _ = P.shape
+    No = G.shape[3]
+    E = np.zeros((Nx, Ny, No, 3), np.complex128)
+    for a in range(Nx):
+        for b in range(Ny):
+            p = P[a, b]
+            if not np.any(p):
+                continue
+            # window of offsets i - a for i = 0..Nx-1
+            Gw = G[Nx - 1 - a:2 * Nx - 1 - a, Ny - 1 - b:2 * Ny - 1 - b]
+            E += np.einsum("xykorc,kc->xyor", Gw, p)
+    return E

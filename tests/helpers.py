@@ -133,14 +133,8 @@ def polar_nodes(kr, wr, phis, wphis):
 
 
 def lateral_sum(g, W, kx, ky, xs, ys):
-    """G(x, y) = sum_n W_n g_n exp(i (kx_n x + ky_n y))  (the exact separable
-    lateral sum, mod_ClosedFormField.jl:743-753).  g: (n, 3, 3)."""
-    xs = np.atleast_1d(xs)
-    ys = np.atleast_1d(ys)
-    Px = np.exp(1j * np.outer(kx, xs))
-    Py = np.exp(1j * np.outer(ky, ys))
-    S = g * W[:, None, None]
-    return np.einsum("nc,nd,nij->cdij", Px, Py, S)
+    """The oracle's exact separable lateral sum (mod_ClosedFormField.jl:743-753)."""
+    return po.lateral_sum(g, W, kx, ky, xs, ys)
 
 
 def E_free(r, p, k, eps):

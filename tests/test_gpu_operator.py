@@ -1,0 +1,212 @@
+"""GPU parity of the 2D lateral sum (ytf_closed_form_G_2d) and of the padded,
+z-structured Green operator (ytf_operator_build / ytf_operator_apply) against
+the CPU oracle, through the C-ABI.
+
+The oracle route is literal: the oracle's per-node spectral dyadics, the
+exact separable lateral sum on every lateral offset, and the linear
+convolution over source pixels and planes (oracle/pyoracle.py
+linear_convolution).  Tolerances: FP64 with the exact lateral sum 1e-10
+relative over the whole output; with the NUFFT lateral sum, 1e-9 at
+nufft_eps = 1e-12."""
+import math
+import warnings
+
+import numpy as np
+import pytest
+
+from helpers import po, relerr, ostack, FOUNDATION_STACKS, kr_grid, uniform_phi
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+W0 = 2 * math.pi
+
+
+def _cache(stack, src, Np=12, kmax=12.0, order=6):
+    from yellowthinfilms_amd.green_operator import CavityTransfer2D
+    kr, wr = kr_grid(stack, W0, kmax=kmax, width=0.5, order=order)
+    phis, wph = uniform_phi(Np)
+    return CavityTransfer2D(stack, W0, src, kr, wr, phis, wph)
+
+
+def _grid(Nx, Ny, dx, dy):
+    return dict(Nx=Nx, Ny=Ny, dx=dx, dy=dy, midx=1 + Nx // 2, midy=1 + Ny // 2)
+
+
+def _oracle_apply(stack, cache, zs, zobs_by_layer, P, grid):
+    """E per obs layer, (Nx, Ny, No, 3), of moments P (Nx, Ny, Ns, 3)."""
+    st = ostack(stack)
+    Nx, Ny = grid["Nx"], grid["Ny"]
+    offx = (np.arange(2 * Nx - 1) - (Nx - 1)) * grid["dx"]
+    offy = (np.arange(2 * Ny - 1) - (Ny - 1)) * grid["dy"]
+    out = {}
+    for o, zo in zobs_by_layer.items():
+        G = np.stack([po.closed_form_G_2d(st, W0, cache.kx_host, cache.ky_host, cache.W_host, cache.source_layer, o,
+                                          z, zo, offx, offy) for z in zs], axis=2)
+        out[o] = po.linear_convolution(G, P)
+    return out
+
+
+def _moments(Nx, Ny, Ns, seed, n=4):
+    rng = np.random.default_rng(seed)
+    P = np.zeros((Nx, Ny, Ns, 3), np.complex128)
+    for _ in range(n):
+        i, j, k = rng.integers(Nx), rng.integers(Ny), rng.integers(Ns)
+        P[i, j, k] += rng.normal(size=3) + 1j * rng.normal(size=3)
+    return P
+
+
+def _run(op, P):
+    from yellowthinfilms_amd.green_operator import apply_operator
+    E = apply_operator(op, torch.as_tensor(P.transpose(3, 2, 1, 0).copy()))
+    torch.cuda.synchronize()
+    return {o: e.cpu().numpy().transpose(3, 2, 1, 0) for o, e in E.items()}
+
+
+@pytest.mark.parametrize("lateral,eps,tol", [("direct", 1e-9, 1e-10), ("nufft", 1e-12, 1e-9), ("nufft", 1e-9, 1e-6)])
+def test_closed_form_G_2d_against_oracle(lateral, eps, tol):
+    from yellowthinfilms_amd.green_operator import closed_form_G_2d
+    stack = FOUNDATION_STACKS["uniaxial 4-layer"]
+    cache = _cache(stack, 2)
+    st = ostack(stack)
+    xs = (np.arange(12) - 6) * 0.07
+    ys = (np.arange(10) - 5) * 0.09
+    for obs, zo in ((1, [0.3, 0.6]), (2, [0.1, 0.5, 0.7])):
+        print(f"case: closed_form_G_2d {lateral} eps={eps} obs={obs}", flush=True)
+        G = closed_form_G_2d(cache, obs, 0.35, zo, xs, ys, lateral=lateral, nufft_eps=eps)
+        torch.cuda.synchronize()
+        G = G.cpu().numpy().transpose(4, 3, 2, 1, 0)
+        ref = po.closed_form_G_2d(st, W0, cache.kx_host, cache.ky_host, cache.W_host, 2, obs, 0.35, zo, xs, ys)
+        e = relerr(G, ref)
+        print(f"  rel err {e:.3e}", flush=True)
+        assert e < tol, e
+
+
+@pytest.mark.parametrize("label,src,zs,zobs,kinds,lateral", [
+    ("iso 5-layer lossy", 2, [0.1, 0.3, 0.5], {1: [0.2, 0.6], 2: [0.15, 0.35, 0.55], 4: [0.4]},
+     {1: "dense", 2: "structured", 4: "dense"}, "direct"),
+    ("iso 5-layer lossy", 1, [0.1, 0.2, 0.3], {1: [0.05, 0.15, 0.25, 0.35], 2: [0.3, 0.5]},
+     {1: "structured", 2: "dense"}, "direct"),
+    ("biaxial 4-layer", 2, [0.2, 0.4, 0.6], {2: [0.1, 0.3, 0.5, 0.7], 3: [0.25]},
+     {2: "toeplitz_dense", 3: "dense"}, "direct"),
+    ("gyrotropic 4-layer", 2, [0.1, 0.3, 0.5], {2: [0.2, 0.4, 0.6], 4: [0.3, 0.5]},
+     {2: "structured", 4: "dense"}, "nufft"),
+])
+def test_operator_apply_against_oracle(label, src, zs, zobs, kinds, lateral):
+    from yellowthinfilms_amd.green_operator import build_layer_operator
+    stack = FOUNDATION_STACKS[label]
+    cache = _cache(stack, src)
+    grid = _grid(8, 8, 0.06, 0.05)
+    print(f"case: operator {label} src={src} {lateral}", flush=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        op = build_layer_operator(cache, grid, zs, zobs, skeleton=False, lateral=lateral, nufft_eps=1e-12)
+    assert {b["obs_layer"]: b["kind"] for b in op.blocks} == kinds
+    for b in op.blocks:
+        if b["kind"] == "structured":
+            assert b["check_residual"] is not None and b["check_residual"] < 1e-8
+    P = _moments(8, 8, len(zs), seed=src)
+    E = _run(op, P)
+    ref = _oracle_apply(stack, cache, zs, zobs, P, grid)
+    tol = 1e-10 if lateral == "direct" else 1e-9
+    for o in zobs:
+        e = relerr(E[o], ref[o])
+        print(f"  obs {o} ({kinds[o]}): rel err {e:.3e}", flush=True)
+        assert e < tol, (o, e)
+
+
+def test_structured_and_dense_forms_agree():
+    from yellowthinfilms_amd.green_operator import build_layer_operator
+    stack = FOUNDATION_STACKS["iso 5-layer lossy"]
+    cache = _cache(stack, 2)
+    grid = _grid(16, 8, 0.05, 0.05)
+    zs = [0.1, 0.2, 0.3, 0.4]
+    zobs = {2: [0.05, 0.15, 0.25, 0.35, 0.45, 0.55]}
+    print("case: structured vs dense", flush=True)
+    a = build_layer_operator(cache, grid, zs, zobs, structure="structured", skeleton=False)
+    b = build_layer_operator(cache, grid, zs, zobs, structure="dense", skeleton=False)
+    assert a.blocks[0]["kind"] == "structured" and b.blocks[0]["kind"] == "dense"
+    assert a.blocks[0]["n_toeplitz"] == a.blocks[0]["n_hankel"] == 9 and b.blocks[0]["n_dense"] == 24
+    P = _moments(16, 8, 4, seed=7, n=6)
+    Ea, Eb = _run(a, P)[2], _run(b, P)[2]
+    assert relerr(Ea, Eb) < 1e-11, relerr(Ea, Eb)
+
+
+def test_ghat_plane_is_the_fft_of_G_on_the_padded_offsets():
+    from yellowthinfilms_amd.green_operator import build_closed_form_green, padded_offsets
+    stack = FOUNDATION_STACKS["uniaxial 4-layer"]
+    cache = _cache(stack, 2)
+    grid = _grid(8, 4, 0.07, 0.05)
+    zo = [0.2, 0.5]
+    print("case: Ghat planes", flush=True)
+    g = build_closed_form_green(cache, 3, 0.4, grid, zo)
+    Gh = g.Ghat.cpu().numpy().transpose(4, 3, 2, 1, 0)  # (Px, Py, nz, 3, 3)
+    xs, ys = padded_offsets(grid)
+    G = po.closed_form_G_2d(ostack(stack), W0, cache.kx_host, cache.ky_host, cache.W_host, 2, 3, 0.4, zo, xs, ys)
+    Px, Py = len(xs), len(ys)
+    sign = (-1.0) ** np.add.outer(np.arange(Px), np.arange(Py))
+    ref = np.fft.fft2(G, axes=(0, 1)) * (sign * grid["dx"] * grid["dy"] / (Px * Py))[:, :, None, None, None]
+    assert relerr(Gh, ref) < 1e-10, relerr(Gh, ref)
+
+
+def test_closed_form_field_two_source_layers():
+    from yellowthinfilms_amd.green_operator import (CavityTransfer2D, Dipole, closed_form_field, apply_closed_form,
+                                                    point_current_sheet)
+    stack = FOUNDATION_STACKS["iso 5-layer lossy"]
+    caches = {2: _cache(stack, 2), 3: _cache(stack, 3)}
+    grid = _grid(8, 8, 0.05, 0.05)
+    dips = [Dipole((1.0, 0.5j, -0.2), W0, 0.1, 0.2, 2), Dipole("z", W0, -0.05, 0.5, 2, y=0.1),
+            Dipole((0.3, 0.0, 1.0), W0, 0.0, 0.1, 3, y=-0.15)]
+    zobs = {1: [0.3], 2: [0.35, 0.6]}
+    print("case: closed_form_field", flush=True)
+    E = closed_form_field(caches, grid, zobs, dips)
+    torch.cuda.synchronize()
+    E = {o: e.cpu().numpy().transpose(3, 2, 1, 0) for o, e in E.items()}
+    ref = {o: 0 for o in zobs}
+    for l in (2, 3):
+        ds = [d for d in dips if d.layer == l]
+        zs = sorted({d.z for d in ds})
+        P = np.zeros((8, 8, len(zs), 3), np.complex128)
+        for d in ds:
+            i = int(np.rint(d.x / grid["dx"])) + grid["midx"] - 1
+            j = int(np.rint(d.y / grid["dy"])) + grid["midy"] - 1
+            P[i, j, zs.index(d.z)] += np.asarray([1, 0, 0] if d.p == "x" else [0, 0, 1] if d.p == "z" else d.p)
+        r = _oracle_apply(stack, caches[l], zs, zobs, P, grid)
+        for o in zobs:
+            ref[o] = ref[o] + r[o]
+    for o in zobs:
+        assert relerr(E[o], ref[o]) < 1e-10, (o, relerr(E[o], ref[o]))
+    # the same two layer-2 dipoles as sheets (densities p / dx dy)
+    sheets = [point_current_sheet(grid, d.z, [(d.x, d.y, d.p)]) for d in dips[:2]]
+    E2 = apply_closed_form(caches[2], 2, grid, zobs[2], sheets)
+    torch.cuda.synchronize()
+    r2 = E2.cpu().numpy().transpose(3, 2, 1, 0)
+    only2 = closed_form_field({2: caches[2]}, grid, {2: zobs[2]}, dips[:2])[2].cpu().numpy().transpose(3, 2, 1, 0)
+    assert relerr(r2, only2) < 1e-13
+
+
+def test_operator_errors():
+    from yellowthinfilms_amd._lib import YtfError
+    from yellowthinfilms_amd.green_operator import build_layer_operator, apply_operator, dipole_moments, Dipole
+    stack = FOUNDATION_STACKS["iso 5-layer lossy"]
+    cache = _cache(stack, 2, Np=4, kmax=4.0)
+    grid = _grid(8, 8, 0.05, 0.05)
+    print("case: operator errors", flush=True)
+    with pytest.raises(YtfError, match="z grids must be uniform"):
+        build_layer_operator(cache, grid, [0.1, 0.2, 0.4], {2: [0.1]}, skeleton=False)
+    with pytest.raises(YtfError, match="must be increasing"):
+        build_layer_operator(cache, grid, [0.1, 0.2], {2: [0.3, 0.1]}, skeleton=False)
+    with pytest.raises(ValueError, match="structure must be"):
+        build_layer_operator(cache, grid, [0.1], {2: [0.1]}, structure="banded")
+    with pytest.raises(YtfError, match="powers of two"):
+        build_layer_operator(cache, _grid(6, 8, 0.05, 0.05), [0.1], {2: [0.1]}, skeleton=False)
+    op = build_layer_operator(cache, grid, [0.1, 0.2], {2: [0.3]}, skeleton=False)
+    with pytest.raises(ValueError, match="P must be"):
+        apply_operator(op, torch.zeros((3, 3, 8, 8), dtype=torch.complex128))
+    with pytest.raises(ValueError, match="not one of the operator's source planes"):
+        dipole_moments(op, [Dipole("x", W0, 0.0, 0.15, 2)])
+    with pytest.raises(ValueError, match="operator source layer"):
+        dipole_moments(op, [Dipole("x", W0, 0.0, 0.1, 3)])
+    P = dipole_moments(op, [Dipole("x", W0, 0.0, 0.1, 2), Dipole((0, 1, 0), W0, 0.05, 0.2, 2, y=-0.05)])
+    assert P[0, 0, 4, 4] == 1 and P[1, 1, 3, 5] == 1 and np.count_nonzero(P) == 2

@@ -490,3 +490,41 @@ def test_nudft_against_brute_force_and_linearity():
         assert abs(f[m] - sum(c[j] * np.exp(1j * (a * x[j] + b * y[j])) for j in range(40))) < 1e-12
     f2 = po.nudft2d1(x, y, (2 - 1j) * c, modes=modes)
     assert relerr(f2, (2 - 1j) * f) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# Lateral assembly: the linear convolution of the operator
+# (mod_ClosedFormAssembly.jl:10, mod_GreenOperator.jl:674-712)
+# ---------------------------------------------------------------------------
+def test_linear_convolution_reproduces_free_space_fields():
+    """Dipoles on pixels of a small grid in a homogeneous medium: the oracle's
+    2D closed_form_G on the offset grid, convolved with the moments, equals
+    the sum of the free-space dipole fields at every observation pixel and
+    plane.  G_xz and G_yz are odd in the lateral offset, so a reversed offset
+    or a swapped x/y axis fails this."""
+    wf, ef = 2 * np.pi, 1.0 + 1e-4j
+    stack = [lay(IsotropicMaterial(ef), 3.0) for _ in range(3)]
+    st = ostack(stack)
+    k = wf * np.sqrt(ef)
+    kr, wr = kr_rule([wf * np.sqrt(ef).real], kmax=36 / 0.7 + wf, width=0.5, order=16)
+    phis, wph = uniform_phi(48)
+    kx, ky, W = polar_nodes(kr, wr, phis, wph)
+    Nx, Ny, dx, dy = 4, 3, 0.2, 0.15
+    zs, zo = [1.0, 1.1], [1.8, 1.95]
+    offx = (np.arange(2 * Nx - 1) - (Nx - 1)) * dx
+    offy = (np.arange(2 * Ny - 1) - (Ny - 1)) * dy
+    G = np.stack([po.closed_form_G_2d(st, wf, kx, ky, W, 2, 2, z, zo, offx, offy) for z in zs], axis=2)
+    rng = np.random.default_rng(3)
+    P = np.zeros((Nx, Ny, 2, 3), np.complex128)
+    dips = [(0, 2, 0), (3, 1, 1), (2, 0, 0)]  # (i, j, source plane)
+    for (i, j, kk) in dips:
+        P[i, j, kk] = rng.normal(size=3) + 1j * rng.normal(size=3)
+    E = po.linear_convolution(G, P)
+    Eref = np.zeros_like(E)
+    for i in range(Nx):
+        for j in range(Ny):
+            for o, z in enumerate(zo):
+                for (a, b, kk) in dips:
+                    r = [(i - a) * dx, (j - b) * dy, z - zs[kk]]
+                    Eref[i, j, o] += E_free(r, P[a, b, kk], k, ef)
+    assert relerr(E, Eref) < 1e-9, relerr(E, Eref)

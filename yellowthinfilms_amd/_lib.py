@@ -27,6 +27,23 @@ class YtfNodeCache(ct.Structure):
                 ("T_dev", ct.c_void_p), ("basis_dev", ct.c_void_p), ("flags_dev", ct.c_void_p)]
 
 
+class YtfGrid(ct.Structure):
+    _fields_ = [("Nx", ct.c_int32), ("Ny", ct.c_int32), ("dx", ct.c_double), ("dy", ct.c_double)]
+
+
+class YtfOperatorOptions(ct.Structure):
+    _fields_ = [("structure", ct.c_int32), ("check", ct.c_int32), ("check_tol", ct.c_double),
+                ("skeleton", ct.c_int32), ("skeleton_tol", ct.c_double), ("lateral", ct.c_int32),
+                ("nufft_eps", ct.c_double), ("uniform_z", ct.c_int32)]
+
+
+class YtfBlockInfo(ct.Structure):
+    _fields_ = [("obs_layer", ct.c_int32), ("kind", ct.c_int32), ("n_obs", ct.c_int32), ("n_src_skel", ct.c_int32),
+                ("n_obs_skel", ct.c_int32), ("n_toeplitz", ct.c_int32), ("n_hankel", ct.c_int32),
+                ("n_dense", ct.c_int32), ("bytes", ct.c_int64), ("build_ms", ct.c_double),
+                ("check_residual", ct.c_double)]
+
+
 class YtfError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(msg)
@@ -81,10 +98,28 @@ _SIGS = {
                                       ct.c_void_p]),
     "ytf_nufft2d1": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
                                 ct.c_int, ct.c_double, ct.c_void_p]),
+    "ytf_closed_form_G_2d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                        ct.c_void_p, ct.POINTER(ct.c_double), ct.c_int, ct.POINTER(ct.c_double),
+                                        ct.c_int, ct.c_int, ct.c_double, ct.c_void_p]),
+    "ytf_operator_build": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int, ct.c_int64,
+                                      ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(YtfGrid),
+                                      ct.POINTER(ct.c_double), ct.c_int, ct.c_int, ct.POINTER(ct.c_int32),
+                                      ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double),
+                                      ct.POINTER(YtfOperatorOptions), ct.POINTER(ct.c_uint32),
+                                      ct.POINTER(ct.c_void_p)]),
+    "ytf_operator_destroy": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_operator_apply": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]),
+    "ytf_operator_num_blocks": (ct.c_int, [ct.c_void_p]),
+    "ytf_operator_block_info": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.POINTER(YtfBlockInfo)]),
+    "ytf_operator_ghat": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]),
 }
 
 TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}
 SYNTH_MODES = {"fp64": 0, "f16x3": 1}
+LATERAL = {"direct": 0, "nufft": 1}
+STRUCTURE = {"auto": 0, "structured": 1, "dense": 2}
+BLOCK_KINDS = {0: "structured", 1: "toeplitz_dense", 2: "dense", 3: "skeleton", 4: "toeplitz_skeleton"}
+WARN_STRUCTURE_CHECK = 1
 
 EXPORTS = tuple(_SIGS)
 

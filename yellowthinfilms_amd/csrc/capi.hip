@@ -15,153 +15,9 @@
 #include <string>
 #include <vector>
 
-#include "ytf.h"
-#include "ytf_kernels.h"
-
-using ytf::cd;
-using ytf::LayerDev;
+#include "ctx.hpp"
 
 static_assert(sizeof(LayerDev) == sizeof(ytf_layer), "ytf_layer and LayerDev must share one layout");
-
-struct ytf_ctx {
-  int device = 0;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
-  std::string err;
-  std::mutex mu;
-  // small per-call inputs (layers, z pairs), grown on demand; `small_host`
-  // mirrors what the device copy holds so unchanged inputs are not re-sent
-  void *small = nullptr;
-  size_t small_bytes = 0;
-  std::vector<char> small_host;
-  // FFT twiddle tables, one per length
-  std::map<int, cd *> tw;
-  // NUFFT deconvolution factors per (modes, fine grid, kernel width)
-  std::map<std::tuple<int, int, int>, double *> nu_deconv;
-  // scratch of the synthesis kernels and its host-side inputs (radii, pixel
-  // coordinates, radius map), grown on demand, freed with the context
-  void *work = nullptr;
-  size_t work_bytes = 0;
-  void *aux = nullptr;
-  size_t aux_bytes = 0;
-  // kernel timing: events bracketing each launch while enabled
-  bool timing = false;
-  struct Mark {
-    std::string name;
-    hipEvent_t a, b;
-  };
-  std::vector<Mark> marks;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, long>> totals;
-};
-
-static thread_local std::string g_err;
-
-static int fail(ytf_ctx *c, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
-static int fail(ytf_ctx *c, int code, const char *fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  if (c) c->err = buf;
-  g_err = buf;
-  return code;
-}
-#define HIPCHK(c, expr)                                                                           \
-  do {                                                                                            \
-    hipError_t _e = (expr);                                                                       \
-    if (_e != hipSuccess) return fail((c), YTF_EHIP, "%s: %s", #expr, hipGetErrorString(_e));     \
-  } while (0)
-
-static int ensure_small(ytf_ctx *c, size_t bytes) {
-  if (bytes <= c->small_bytes) return YTF_OK;
-  if (c->small) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(c->small));
-    c->small = nullptr;
-    c->small_bytes = 0;
-  }
-  size_t nb = bytes < 65536 ? 65536 : bytes;
-  HIPCHK(c, hipMalloc(&c->small, nb));
-  c->small_bytes = nb;
-  return YTF_OK;
-}
-
-// Grow a context-owned device buffer to at least `bytes` (contents are not kept).
-static int ensure_buf(ytf_ctx *c, void **buf, size_t *cur, size_t bytes) {
-  if (bytes <= *cur) return YTF_OK;
-  if (*buf) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(*buf));
-    *buf = nullptr;
-    *cur = 0;
-  }
-  size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes;
-  HIPCHK(c, hipMalloc(buf, nb));
-  *cur = nb;
-  return YTF_OK;
-}
-
-static hipEvent_t take_event(ytf_ctx *c) {
-  if (!c->pool.empty()) {
-    hipEvent_t e = c->pool.back();
-    c->pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-// Launch through `f`; while timing is on, bracket it with events on the
-// context's stream under `name`.
-template <class F>
-static hipError_t timed(ytf_ctx *c, const char *name, F &&f) {
-  if (!c->timing) return f();
-  hipEvent_t a = take_event(c), b = take_event(c);
-  if (!a || !b) return f();
-  hipEventRecord(a, c->stream);
-  hipError_t e = f();
-  hipEventRecord(b, c->stream);
-  c->marks.push_back({name, a, b});
-  return e;
-}
-
-// Fold finished marks into the per-name totals (synchronizes the stream).
-static int drain_marks(ytf_ctx *c) {
-  if (c->marks.empty()) return YTF_OK;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (auto &m : c->marks) {
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, m.a, m.b));
-    auto &t = c->totals[m.name];
-    t.first += ms;
-    t.second += 1;
-    c->pool.push_back(m.a);
-    c->pool.push_back(m.b);
-  }
-  c->marks.clear();
-  return YTF_OK;
-}
-
-static int twiddles(ytf_ctx *c, int n, cd **out) {
-  auto it = c->tw.find(n);
-  if (it != c->tw.end()) { *out = it->second; return YTF_OK; }
-  std::vector<cd> h(n);
-  for (int j = 0; j < n; ++j) {
-    // exp(-2 pi i j / n) with the argument reduced exactly to the first octant
-    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
-    h[j].r = (double)cosl(a);
-    h[j].i = (double)sinl(a);
-  }
-  cd *d = nullptr;
-  HIPCHK(c, hipMalloc(&d, sizeof(cd) * n));
-  HIPCHK(c, hipMemcpy(d, h.data(), sizeof(cd) * n, hipMemcpyHostToDevice));
-  c->tw[n] = d;
-  *out = d;
-  return YTF_OK;
-}
 
 extern "C" {
 
@@ -226,6 +82,16 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
                     int terms) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_green_nodes: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
+  return green_nodes_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n_nodes, source_layer, obs_layer, n_pairs,
+                            zsrc, zobs, ghat_dev, cache, terms);
+}
+
+}  // extern "C"
+
+int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                       const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                       const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                       int terms) {
   if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_green_nodes: need at least 2 layers (got %d)", n_layers);
   if (source_layer < 1 || source_layer > n_layers - 1)
     return fail(c, YTF_EINVAL,
@@ -320,6 +186,8 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   return YTF_OK;
 }
 
+extern "C" {
+
 static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
 
 int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
@@ -392,7 +260,6 @@ struct ytf_batch {
   std::string signature;
 };
 
-static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Work items: runs of nodes_per_item nodes of one stack, heaviest stacks first.
 static int batch_items(ytf_ctx *c, ytf_batch *b) {
@@ -1144,8 +1011,15 @@ int ytf_nufft2d1(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev
                  int N1, int N2, double eps, double *f_dev) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_nufft2d1: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
-  if (M < 0 || M > ((int64_t)1 << 31) - 1 || ntrans < 1 || N1 < 2 || N2 < 2 || N1 > 1024 || N2 > 1024)
-    return fail(c, YTF_EINVAL, "ytf_nufft2d1: bad sizes (M=%lld, ntrans=%d, N1=%d, N2=%d; modes 2..1024)",
+  return nufft2d1_locked(c, M, x_dev, y_dev, ntrans, c_dev, N1, N2, eps, f_dev);
+}
+
+}  // extern "C"
+
+int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev, int ntrans, const double *c_dev,
+                    int N1, int N2, double eps, double *f_dev) {
+  if (M < 0 || M > ((int64_t)1 << 31) - 1 || ntrans < 1 || N1 < 2 || N2 < 2 || N1 > 2048 || N2 > 2048)
+    return fail(c, YTF_EINVAL, "ytf_nufft2d1: bad sizes (M=%lld, ntrans=%d, N1=%d, N2=%d; modes 2..2048)",
                 (long long)M, ntrans, N1, N2);
   if (!(eps >= 1e-15 && eps < 1.0)) return fail(c, YTF_EINVAL, "ytf_nufft2d1: eps=%g must be in [1e-15, 1)", eps);
   if (!f_dev || (M > 0 && (!x_dev || !y_dev || !c_dev))) return fail(c, YTF_EINVAL, "ytf_nufft2d1: NULL buffer");
@@ -1204,6 +1078,8 @@ int ytf_nufft2d1(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_nufft2d1: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
+
+extern "C" {
 
 static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const double *kx, const double *wt,
                                    int np, const double *ghat, int nr, const double *rho, cd *R, int mode) {

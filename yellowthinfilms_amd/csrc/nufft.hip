@@ -279,6 +279,7 @@ hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N
     case 512: return run_ffts<512>(grid, T1, f, N1, N2, d1, d2, tw, s);
     case 1024: return run_ffts<1024>(grid, T1, f, N1, N2, d1, d2, tw, s);
     case 2048: return run_ffts<2048>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 4096: return run_ffts<4096>(grid, T1, f, N1, N2, d1, d2, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -1,0 +1,386 @@
+// operator.hip -- the 2D lateral sum of closed_form_G and the padded,
+// z-structured Green operator on gfx950, FP64.
+//
+// Lateral sum (closed_form_G for a CavityTransferCache2D,
+// mod_ClosedFormField.jl:711-782): per depth pair p and entry e = c + 3q,
+//     G(x_i, y_j) = sum_n W_n g~_n exp(i (kx_n x_i + ky_n y_j)),
+// either exactly (k_lat_direct, the separable sum of :743-753) or through the
+// type-1 NUFFT (k_lat_points + k_lat_strengths feed nufft.hip, :755-780).
+//
+// Operator (mod_GreenOperator.jl, mod_ClosedFormAssembly.jl): G is sampled on
+// the 2Nx x 2Ny padded offsets, FFT'd once (sign (-1)^(x+y) for the centring
+// shift, 1/(Px Py) folded in) and stored per plane as [entry][node] with the
+// spectral node index y + Py*x (y fastest).  The apply is
+//   1. k_opx_fwd: zero-padded FFT along x of each source-plane component,
+//      written transposed ([plane][x][y]);
+//   2. k_opy<.., false>: FFT along y in place (rows >= Ny are zeros);
+//   3. k_op_mix / k_op_zcomb: per spectral node, the z coupling of each block
+//      (Toeplitz, Hankel, dense plane pairs, skeleton fold and expansion);
+//   4. k_opy<.., true>: inverse FFT along y, keeping rows < Ny;
+//   5. k_opx_inv: inverse FFT along x, cropped to the Nx x Ny pixels.
+// Power-of-two padded sizes 8..2048 (FFTs in LDS, fft_core.hpp).
+#include <hip/hip_runtime.h>
+
+#include "fft_core.hpp"
+#include "ytf_kernels.h"
+
+namespace ytf {
+
+// Round-to-nearest reduction by 2 pi (Julia's rem2pi(x, RoundNearest)) with
+// a two-part 2 pi: x - n*C1 is exact for |n| < 2^20, the C2 term restores the
+// low bits of 2 pi.
+__device__ __forceinline__ double rem2pi_near(double x) {
+  const double C1 = 6.283185307179586231995926937088;  // double(2 pi)
+  const double C2 = 2.4492935982947063e-16;            // 2 pi - C1
+  const double n = rint(x * 0.15915494309189533577);
+  double r = __builtin_fma(-n, C1, x);
+  r = __builtin_fma(-n, C2, r);
+  return r;
+}
+
+__device__ __forceinline__ cd finite_or_zero(cd z) { return isfin(z) ? z : cd{0.0, 0.0}; }
+
+// NUFFT points and the node weights times the centring ramp:
+//   xf = rem2pi(kx px), yf = rem2pi(ky py), wr = W exp(i (kx x0 + ky y0))
+// (_nufft_nodes_2d, mod_ClosedFormField.jl:668-688, with a_x = 0 and
+// x0, y0 the offsets of the centre pixel).
+__global__ void k_lat_points(int64_t n, const double *__restrict__ kx, const double *__restrict__ ky,
+                             const cd *__restrict__ W, double px, double py, double x0, double y0,
+                             double *__restrict__ xf, double *__restrict__ yf, cd *__restrict__ wr) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double a = kx[k], b = ky[k];
+  xf[k] = rem2pi_near(a * px);
+  yf[k] = rem2pi_near(b * py);
+  double s, c;
+  sincos(a * x0 + b * y0, &s, &c);
+  wr[k] = W[k] * cd{c, s};
+}
+
+// Strengths of the lateral sum: out[p*tp + e*te][k] = finite_or_zero(w[k] g~[p][e][k]).
+// In place (out == g) only with tp = 9, te = 1.
+__global__ void k_lat_strengths(int64_t n, int np, const cd *g, const cd *__restrict__ w, cd *out, int64_t tp,
+                                int64_t te) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const cd wk = w[k];
+  for (int p = 0; p < np; ++p)
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const cd v = g[((int64_t)p * 9 + e) * n + k];
+      out[(p * tp + e * te) * n + k] = finite_or_zero(wk * v);
+    }
+}
+
+// Exact lateral sum.  One 16 x 16 pixel tile and one depth pair per block;
+// per chunk of 64 nodes the phases exp(i kx x_i), exp(i ky y_j) of the tile
+// and the 9 weighted strengths are staged in LDS, then every thread adds the
+// chunk's terms to its pixel's 9 entries.
+//   G[i + nx*(j + ny*(p*sp + e*se))]
+constexpr int LAT_T = 16, LAT_C = 64;
+__global__ __launch_bounds__(256) void k_lat_direct(int64_t n, const double *__restrict__ kx,
+                                                    const double *__restrict__ ky, const cd *__restrict__ W,
+                                                    const cd *__restrict__ g, const double *__restrict__ xs, int nx,
+                                                    const double *__restrict__ ys, int ny, cd *__restrict__ G,
+                                                    int64_t sp, int64_t se) {
+  __shared__ cd ax[LAT_C][LAT_T], ay[LAT_C][LAT_T], sv[9][LAT_C];
+  const int ti = threadIdx.x % LAT_T, tj = threadIdx.x / LAT_T;
+  const int i0 = blockIdx.x * LAT_T, j0 = blockIdx.y * LAT_T, p = blockIdx.z;
+  const int tid = threadIdx.x;
+  cd acc[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) acc[e] = cd{0.0, 0.0};
+  const cd *gp = g + (int64_t)p * 9 * n;
+  for (int64_t k0 = 0; k0 < n; k0 += LAT_C) {
+    const int m = (int)((n - k0) < LAT_C ? (n - k0) : LAT_C);
+    for (int e = tid; e < 2 * LAT_C * LAT_T; e += 256) {
+      const int h = e / (LAT_C * LAT_T), r = e % (LAT_C * LAT_T);
+      const int kk = r / LAT_T, t = r % LAT_T;
+      cd v = cd{0.0, 0.0};
+      if (kk < m) {
+        const int idx = (h ? j0 : i0) + t;
+        const bool in = h ? idx < ny : idx < nx;
+        if (in) {
+          const double ph = h ? ky[k0 + kk] * ys[idx] : kx[k0 + kk] * xs[idx];
+          sincos(ph, &v.i, &v.r);
+        }
+      }
+      if (h) ay[kk][t] = v;
+      else ax[kk][t] = v;
+    }
+    for (int e = tid; e < 9 * LAT_C; e += 256) {
+      const int ee = e / LAT_C, kk = e % LAT_C;
+      sv[ee][kk] = kk < m ? finite_or_zero(W[k0 + kk] * gp[ee * n + k0 + kk]) : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int kk = 0; kk < m; ++kk) {
+      const cd ph = ax[kk][ti] * ay[kk][tj];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) acc[e] = fmac(acc[e], sv[e][kk], ph);
+    }
+    __syncthreads();
+  }
+  const int i = i0 + ti, j = j0 + tj;
+  if (i < nx && j < ny) {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) G[i + (int64_t)nx * (j + (int64_t)ny * (p * sp + e * se))] = acc[e];
+  }
+}
+
+// --------------------------------------------------------------------------
+// Padded 2D FFT passes
+// --------------------------------------------------------------------------
+
+// Forward FFT along x of plane t = blockIdx.y, input rows zero-padded from
+// nx_in to PX and rows >= ny_rows skipped; output transposed: out[t][x][y]
+// (row stride py).  Input plane t starts at in + (t / pdiv) * pa + (t % pdiv) * pb.
+template <int PX>
+__global__ __launch_bounds__(1024) void k_opx_fwd(const cd *__restrict__ in, int pdiv, int64_t pa, int64_t pb,
+                                                  int in_row, int nx_in, int ny_rows, cd *__restrict__ out, int py,
+                                                  int64_t out_plane, const cd *__restrict__ tw) {
+  constexpr int YT = XTile<PX>::YT;
+  constexpr int TPR = PX / 8;
+  extern __shared__ cd lds[];
+  const int t = blockIdx.y, tid = threadIdx.x;
+  const cd *src = in + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
+  cd *dst = out + (int64_t)t * out_plane;
+  const int y0 = blockIdx.x * YT;
+  for (int e = tid; e < YT * PX; e += YT * TPR) {
+    const int r = e / PX, x = e % PX, y = y0 + r;
+    lds[e] = (x < nx_in && y < ny_rows) ? src[x + (int64_t)in_row * y] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_lds<PX, false>(lds + (tid / TPR) * PX, tid % TPR, tw);
+  for (int e = tid; e < YT * PX; e += YT * TPR) {
+    const int r = e % YT, x = e / YT, y = y0 + r;
+    if (y < ny_rows) dst[y + (int64_t)py * x] = lds[r * PX + x];
+  }
+}
+
+// Inverse FFT along x of plane t (in[t][x][y], rows < ny_rows), cropped to
+// x < nx_out and stored to out + (t / pdiv) * pa + (t % pdiv) * pb, row stride out_row.
+template <int PX>
+__global__ __launch_bounds__(1024) void k_opx_inv(const cd *__restrict__ in, int py, int64_t in_plane,
+                                                  cd *__restrict__ out, int pdiv, int64_t pa, int64_t pb, int out_row,
+                                                  int nx_out, int ny_rows, const cd *__restrict__ tw) {
+  constexpr int YT = XTile<PX>::YT;
+  constexpr int TPR = PX / 8;
+  extern __shared__ cd lds[];
+  const int t = blockIdx.y, tid = threadIdx.x;
+  const cd *src = in + (int64_t)t * in_plane;
+  cd *dst = out + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
+  const int y0 = blockIdx.x * YT;
+  for (int e = tid; e < YT * PX; e += YT * TPR) {
+    const int r = e % YT, x = e / YT, y = y0 + r;
+    lds[r * PX + x] = y < ny_rows ? src[y + (int64_t)py * x] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_lds<PX, true>(lds + (tid / TPR) * PX, tid % TPR, tw);
+  for (int e = tid; e < YT * PX; e += YT * TPR) {
+    const int r = e / PX, x = e % PX, y = y0 + r;
+    if (x < nx_out && y < ny_rows) dst[x + (int64_t)out_row * y] = lds[e];
+  }
+}
+
+// Columns per block of the y pass: PY/8 threads per column, <= 256 threads.
+template <int PY> struct OpY {
+  static constexpr int TPC = PY / 8;
+  static constexpr int XT = (256 / TPC) < 1 ? 1 : ((256 / TPC) > 64 ? 64 : (256 / TPC));
+};
+
+// FFT along y, in place on work[t][x][y] (y stride 1, x stride PY): input
+// rows >= ny_in are zeros, rows >= ny_out are not stored; the result is
+// multiplied by scale, and by (-1)^(x+y) when `sign`.
+template <int PY, bool INV>
+__global__ __launch_bounds__(256) void k_opy(cd *__restrict__ work, int px, int64_t plane, int ny_in, int ny_out,
+                                             double scale, int sign, const cd *__restrict__ tw) {
+  constexpr int TPC = OpY<PY>::TPC, XT = OpY<PY>::XT;
+  extern __shared__ cd lds[];
+  const int t = blockIdx.y, tid = threadIdx.x;
+  cd *base = work + (int64_t)t * plane;
+  const int x0 = blockIdx.x * XT;
+  for (int e = tid; e < XT * PY; e += XT * TPC) {
+    const int col = e / PY, y = e % PY, x = x0 + col;
+    lds[e] = (x < px && y < ny_in) ? base[y + (int64_t)PY * x] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_lds<PY, INV>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  for (int e = tid; e < XT * PY; e += XT * TPC) {
+    const int col = e / PY, y = e % PY, x = x0 + col;
+    if (x < px && y < ny_out) {
+      const double s = (sign && ((x + y) & 1)) ? -scale : scale;
+      base[y + (int64_t)PY * x] = rmul(s, lds[e]);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// z coupling per spectral node
+// --------------------------------------------------------------------------
+
+// E[j][r] (+)= sum_i sum_c K[slot(i, j)][r + 3c] J[i][c] on every node, with
+// slot = j - i + ns - 1 (Toeplitz), i + j (Hankel) or i + ns*j (dense pairs).
+// J: [ns][3][nn], E: [no][3][nn], K: [slot][9][nn].
+__global__ __launch_bounds__(256) void k_op_mix(int64_t nn, const cd *__restrict__ J, int ns, cd *__restrict__ E,
+                                                int no, const cd *__restrict__ K, int kind, int accumulate) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  for (int j = 0; j < no; ++j) {
+    cd a0, a1, a2;
+    if (accumulate) {
+      a0 = E[(3 * (int64_t)j + 0) * nn + k];
+      a1 = E[(3 * (int64_t)j + 1) * nn + k];
+      a2 = E[(3 * (int64_t)j + 2) * nn + k];
+    } else {
+      a0 = a1 = a2 = cd{0.0, 0.0};
+    }
+    for (int i = 0; i < ns; ++i) {
+      const int slot = kind == 0 ? j - i + ns - 1 : (kind == 1 ? i + j : i + ns * j);
+      const cd *Ks = K + (int64_t)slot * 9 * nn + k;
+      const cd j0 = J[(3 * (int64_t)i + 0) * nn + k], j1 = J[(3 * (int64_t)i + 1) * nn + k],
+               j2 = J[(3 * (int64_t)i + 2) * nn + k];
+      a0 = fmac(fmac(fmac(a0, Ks[0], j0), Ks[3 * nn], j1), Ks[6 * nn], j2);
+      a1 = fmac(fmac(fmac(a1, Ks[1 * nn], j0), Ks[4 * nn], j1), Ks[7 * nn], j2);
+      a2 = fmac(fmac(fmac(a2, Ks[2 * nn], j0), Ks[5 * nn], j1), Ks[8 * nn], j2);
+    }
+    E[(3 * (int64_t)j + 0) * nn + k] = a0;
+    E[(3 * (int64_t)j + 1) * nn + k] = a1;
+    E[(3 * (int64_t)j + 2) * nn + k] = a2;
+  }
+}
+
+// out[o][c] (+)= sum_i M[i + nin*o] in[i][c] on every node (the skeleton fold
+// J Ps and expansion Po Es; M is a small complex matrix, read uniformly).
+__global__ __launch_bounds__(256) void k_op_zcomb(int64_t nn, const cd *__restrict__ in, int nin,
+                                                  const cd *__restrict__ M, cd *__restrict__ out, int nout,
+                                                  int accumulate) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nn) return;
+  for (int o = 0; o < nout; ++o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cd a = accumulate ? out[(3 * (int64_t)o + c) * nn + k] : cd{0.0, 0.0};
+      for (int i = 0; i < nin; ++i) a = fmac(a, M[i + (int64_t)nin * o], in[(3 * (int64_t)i + c) * nn + k]);
+      out[(3 * (int64_t)o + c) * nn + k] = a;
+    }
+  }
+}
+
+// dst[e][node] += src[e][node] over n complex values (summing operators).
+__global__ void k_op_add(int64_t n, const cd *__restrict__ src, cd *__restrict__ dst) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = dst[k] + src[k];
+}
+
+// A stored plane [9][x][y] (node y + py*x) to the reference layout
+// (Px, Py, 3, 3) = out[x + px*(y + py*e)].
+__global__ void k_op_export(int px, int py, const cd *__restrict__ slot, cd *__restrict__ out) {
+  const int64_t nn = (int64_t)px * py;
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= 9 * nn) return;
+  const int e = (int)(k / nn);
+  const int64_t r = k % nn;
+  const int x = (int)(r % px), y = (int)(r / px);
+  out[k] = slot[e * nn + y + (int64_t)py * x];
+}
+
+// --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_lat_points(int64_t n, const double *kx, const double *ky, const cd *W, double px, double py,
+                             double x0, double y0, double *xf, double *yf, cd *wr, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lat_points, dim3(nblk(n, 256)), dim3(256), 0, s, n, kx, ky, W, px, py, x0, y0, xf, yf, wr);
+  return hipGetLastError();
+}
+hipError_t launch_lat_strengths(int64_t n, int np, const cd *g, const cd *w, cd *out, int64_t tp, int64_t te,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_lat_strengths, dim3(nblk(n, 256)), dim3(256), 0, s, n, np, g, w, out, tp, te);
+  return hipGetLastError();
+}
+hipError_t launch_lat_direct(int64_t n, const double *kx, const double *ky, const cd *W, int np, const cd *g,
+                             const double *xs, int nx, const double *ys, int ny, cd *G, int64_t sp, int64_t se,
+                             hipStream_t s) {
+  const dim3 grid((nx + LAT_T - 1) / LAT_T, (ny + LAT_T - 1) / LAT_T, np);
+  hipLaunchKernelGGL(k_lat_direct, grid, dim3(256), 0, s, n, kx, ky, W, g, xs, nx, ys, ny, G, sp, se);
+  return hipGetLastError();
+}
+
+template <int PX>
+static hipError_t opx(bool inv, const cd *in, int pdiv, int64_t pa, int64_t pb, int row, int nx, int ny_rows,
+                      cd *out, int py, int64_t plane, int planes, const cd *tw, hipStream_t s) {
+  constexpr int YT = XTile<PX>::YT;
+  const dim3 grid((ny_rows + YT - 1) / YT, planes), block(YT * (PX / 8));
+  const size_t sh = sizeof(cd) * YT * PX;
+  if (inv)
+    hipLaunchKernelGGL(k_opx_inv<PX>, grid, block, sh, s, in, py, plane, out, pdiv, pa, pb, row, nx, ny_rows, tw);
+  else
+    hipLaunchKernelGGL(k_opx_fwd<PX>, grid, block, sh, s, in, pdiv, pa, pb, row, nx, ny_rows, out, py, plane, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_opx(bool inv, int PX, const cd *in, int pdiv, int64_t pa, int64_t pb, int row, int nx, int ny_rows,
+                      cd *out, int py, int64_t plane, int planes, const cd *tw, hipStream_t s) {
+  if (planes <= 0 || ny_rows <= 0) return hipSuccess;
+  switch (PX) {
+#define YTF_OPX(N) \
+  case N: return opx<N>(inv, in, pdiv, pa, pb, row, nx, ny_rows, out, py, plane, planes, tw, s);
+    YTF_OPX(8) YTF_OPX(16) YTF_OPX(32) YTF_OPX(64) YTF_OPX(128) YTF_OPX(256) YTF_OPX(512) YTF_OPX(1024)
+    YTF_OPX(2048)
+#undef YTF_OPX
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int PY>
+static hipError_t opy(bool inv, cd *work, int px, int64_t plane, int planes, int ny_in, int ny_out, double scale,
+                      int sign, const cd *tw, hipStream_t s) {
+  constexpr int XT = OpY<PY>::XT, TPC = OpY<PY>::TPC;
+  const dim3 grid((px + XT - 1) / XT, planes), block(XT * TPC);
+  const size_t sh = sizeof(cd) * XT * PY;
+  if (inv) hipLaunchKernelGGL((k_opy<PY, true>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
+  else hipLaunchKernelGGL((k_opy<PY, false>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_opy(bool inv, int PY, cd *work, int px, int64_t plane, int planes, int ny_in, int ny_out,
+                      double scale, int sign, const cd *tw, hipStream_t s) {
+  if (planes <= 0) return hipSuccess;
+  switch (PY) {
+#define YTF_OPY(N) \
+  case N: return opy<N>(inv, work, px, plane, planes, ny_in, ny_out, scale, sign, tw, s);
+    YTF_OPY(8) YTF_OPY(16) YTF_OPY(32) YTF_OPY(64) YTF_OPY(128) YTF_OPY(256) YTF_OPY(512) YTF_OPY(1024)
+    YTF_OPY(2048)
+#undef YTF_OPY
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, int kind, int accumulate,
+                         hipStream_t s) {
+  if (nn == 0 || no == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_op_mix, dim3(nblk(nn, 256)), dim3(256), 0, s, nn, J, ns, E, no, K, kind, accumulate);
+  return hipGetLastError();
+}
+hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
+                           hipStream_t s) {
+  if (nn == 0 || nout == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_op_zcomb, dim3(nblk(nn, 256)), dim3(256), 0, s, nn, in, nin, M, out, nout, accumulate);
+  return hipGetLastError();
+}
+hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_op_add, dim3(nblk(n, 256)), dim3(256), 0, s, n, src, dst);
+  return hipGetLastError();
+}
+hipError_t launch_op_export(int px, int py, const cd *slot, cd *out, hipStream_t s) {
+  const int64_t n = 9 * (int64_t)px * py;
+  hipLaunchKernelGGL(k_op_export, dim3(nblk(n, 256)), dim3(256), 0, s, px, py, slot, out);
+  return hipGetLastError();
+}
+
+}  // namespace ytf

@@ -135,4 +135,24 @@ hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, con
 hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
                             const cd *tw, hipStream_t s);
 
+// 2D lateral sum and the padded Green operator (operator.hip).
+hipError_t launch_lat_points(int64_t n, const double *kx, const double *ky, const cd *W, double px, double py,
+                             double x0, double y0, double *xf, double *yf, cd *wr, hipStream_t s);
+hipError_t launch_lat_strengths(int64_t n, int np, const cd *g, const cd *w, cd *out, int64_t tp, int64_t te,
+                                hipStream_t s);
+hipError_t launch_lat_direct(int64_t n, const double *kx, const double *ky, const cd *W, int np, const cd *g,
+                             const double *xs, int nx, const double *ys, int ny, cd *G, int64_t sp, int64_t se,
+                             hipStream_t s);
+hipError_t launch_opx(bool inv, int PX, const cd *in, int pdiv, int64_t pa, int64_t pb, int row, int nx, int ny_rows,
+                      cd *out, int py, int64_t plane, int planes, const cd *tw, hipStream_t s);
+hipError_t launch_opy(bool inv, int PY, cd *work, int px, int64_t plane, int planes, int ny_in, int ny_out,
+                      double scale, int sign, const cd *tw, hipStream_t s);
+enum { OP_MIX_TOEPLITZ = 0, OP_MIX_HANKEL = 1, OP_MIX_DENSE = 2 };
+hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, int kind, int accumulate,
+                         hipStream_t s);
+hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
+                           hipStream_t s);
+hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
+hipError_t launch_op_export(int px, int py, const cd *slot, cd *out, hipStream_t s);
+
 }  // namespace ytf
