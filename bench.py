@@ -23,6 +23,11 @@ Workloads (yellowthinfilms_amd.workloads):
     same nodes.  value = Green k-nodes/s of the surrogate path.
   nufft_1e6: type-1 NUFFT of 10^6 seeded points onto 1024^2 modes, eps 1e-9.
     value = applies/s.
+  operator_512: the padded (linear-convolution) Green operator of the
+    batched_gyro_512 stack and grid, one source plane onto one observation
+    plane, built once (untimed) from 262,144 polar nodes with the NUFFT
+    lateral sum; one step = one apply_operator of a 512 x 512 x 3 complex
+    N(0,1) moment sheet.  value = operator applies/s.
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -547,6 +552,106 @@ def cpu_nufft(x, y, c, f, N):
              "kind": "port", "sample": f"{len(modes)} seeded modes x {len(x)} points"}, rel)
 
 
+def bench_operator(a):
+    import torch
+    from yellowthinfilms_amd import workloads as W, uniform_kxky_grid
+    from yellowthinfilms_amd.green_operator import CavityTransfer2D, build_layer_operator, apply_operator
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.OPERATOR_512
+    stack = W.gyro_512_stack()
+    grid = uniform_kxky_grid(wl["L"], wl["dx"])
+    Nx, Ny = grid["Nx"], grid["Ny"]
+    dev = torch.device("cuda", 0)
+    cache = CavityTransfer2D(stack, W.OMEGA, wl["source_layer"], *W.operator_nodes(wl))
+    t0 = time.perf_counter()
+    op = build_layer_operator(cache, grid, [wl["zsrc"]], {wl["obs_layer"]: [wl["zobs"]]}, lateral="nufft",
+                              nufft_eps=wl["eps"])
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    P = torch.as_tensor(W.uniform_c128_source(Nx, Ny, wl["seed"])[:, None], device=dev)  # (3, 1, Ny, Nx)
+
+    def step():
+        apply_operator(op, P)
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    t = {k: ctx.kernel_time(k) for k in ("op_fft_fwd", "op_mix", "op_fft_inv")}
+    ctx.set_timing(False)
+    Px, Py = 2 * Nx, 2 * Ny
+    alg_bytes = 16 * (3 * Nx * Ny + 9 * Px * Py + 3 * Nx * Ny)  # read J and Ĝ, write E
+    ms = sum(v[0] for v in t.values()) / a.steps
+    cpu, rel_err, rel_apply = None, None, None
+    if not a.no_cpu_baseline:
+        cpu, rel_err, rel_apply = cpu_operator(stack, cache, op, P, grid, wl)
+    out = _base(a, wl, a.steps / wall, "operator applies/s (512x512x3 sheet, padded 1024^2)", wall, "f64")
+    out["config"].update({"grid": [Nx, Ny], "padded": [Px, Py], "nodes": cache.n_nodes,
+                          "block": op.blocks[0]["kind"], "lateral": "nufft", "nufft_eps": wl["eps"]})
+    out.update({
+        "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
+        "build_s": build_s,
+        "rel_err_vs_fp64_oracle": rel_err,
+        "rel_err_apply_vs_oracle_fft_route": rel_apply,
+        "roofline": {"kernel": "operator apply (x FFT + fused y FFT / 3x3 product / inverse y FFT + inverse x FFT)",
+                     "bound": "hbm", "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": alg_bytes / (ms * 1e-3) / PEAK_HBM, "traffic": None,
+                     "algorithmic_bytes": alg_bytes},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def cpu_operator(stack, cache, op, P, grid, wl):
+    """Two checks and the CPU time of one apply.  (1) Four seeded dipoles:
+    the GPU field at 64 seeded pixels against the oracle's exact lateral sum
+    of its own spectral dyadics (mod_ClosedFormField.jl:743-753) at those
+    offsets.  (2) The oracle's FFT route of the apply (numpy FFT,
+    single-threaded) on the GPU-built Ĝ and the benchmark sheet: timed, and
+    compared with the GPU apply."""
+    import torch
+    from yellowthinfilms_amd.green_operator import apply_operator
+    try:
+        from oracle import pyoracle as po
+        from yellowthinfilms_amd.materials import stack_arrays
+    except Exception as exc:
+        return ({"value": None, "unit": "operator applies/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None, None)
+    Nx, Ny = grid["Nx"], grid["Ny"]
+    rng = np.random.default_rng(3)
+    dip = [(int(rng.integers(Nx)), int(rng.integers(Ny)), rng.normal(size=3) + 1j * rng.normal(size=3))
+           for _ in range(4)]
+    Pv = np.zeros((3, 1, Ny, Nx), np.complex128)
+    for i, j, p in dip:
+        Pv[:, 0, j, i] += p
+    Ev = apply_operator(op, torch.as_tensor(Pv, device=P.device))[wl["obs_layer"]][:, 0].cpu().numpy()  # (3, Ny, Nx)
+    st = po.Stack(*stack_arrays(stack))
+    g = po.ghat_batch(st, cache.omega, cache.kx_host, cache.ky_host, wl["source_layer"], wl["obs_layer"],
+                      [wl["zsrc"]], [wl["zobs"]], nthreads=_cores())[:, 0]
+    pix = [(int(rng.integers(Nx)), int(rng.integers(Ny))) for _ in range(64)]
+    pts = np.array([((x - i) * grid["dx"], (y - j) * grid["dy"]) for (x, y) in pix for (i, j, _) in dip])
+    Gp = po.lateral_sum_points(g, cache.W_host, cache.kx_host, cache.ky_host, pts).reshape(len(pix), len(dip), 3, 3)
+    ref = np.array([sum(Gp[m, d] @ dip[d][2] for d in range(len(dip))) for m in range(len(pix))])
+    got = np.array([Ev[:, y, x] for (x, y) in pix])
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    # the apply on the CPU
+    Gh = op.ghat_plane(0, "toeplitz", 0).cpu().numpy().transpose(3, 2, 1, 0)[:, :, None]  # (Px, Py, 1, 3, 3)
+    J = P[:, 0].cpu().numpy().transpose(2, 1, 0)  # (Nx, Ny, 3) moments
+    Eg = apply_operator(op, P)[wl["obs_layer"]][:, 0].cpu().numpy().transpose(2, 1, 0)
+    reps, t = 0, time.perf_counter()
+    while True:
+        Ec = po.apply_ghat(Gh, J)[:, :, 0]
+        reps += 1
+        if time.perf_counter() - t > 10.0 or reps >= 5:
+            break
+    dt = (time.perf_counter() - t) / reps
+    rel_apply = float(np.linalg.norm(Eg - Ec) / np.linalg.norm(Ec))
+    return ({"value": 1.0 / dt, "unit": "operator applies/s", "cores": 1, "kind": "port",
+             "sample": f"{reps} applies of the oracle's FFT route (numpy FFT) on the GPU-built Ĝ"}, rel, rel_apply)
+
+
 # ---------------------------------------------------------------------------
 def dict_ns(a, **kw):
     b = argparse.Namespace(**vars(a))
@@ -556,7 +661,8 @@ def dict_ns(a, **kw):
 
 
 WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
-             "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft}
+             "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft,
+             "operator_512": bench_operator}
 
 
 def main():

@@ -358,3 +358,41 @@ def linear_convolution(G, P):
             Gw = G[Nx - 1 - a:2 * Nx - 1 - a, Ny - 1 - b:2 * Ny - 1 - b]
             E += np.einsum("xykorc,kc->xyor", Gw, p)
     return E
+
+
+def ghat_from_G(G, dx, dy):
+    """build_closed_form_green's transform (mod_ClosedFormAssembly.jl:110-124;
+    numpy.fft stands in for FFTW): G on the padded offsets (Px, Py, ...) ->
+    fft over the two lateral axes, times (-1)^(i+j) dx dy / (Px Py)."""
+    Px, Py = G.shape[:2]
+    sign = (-1.0) ** np.add.outer(np.arange(Px), np.arange(Py))
+    sc = (sign * dx * dy / (Px * Py)).reshape((Px, Py) + (1,) * (G.ndim - 2))
+    return np.fft.fft2(G, axes=(0, 1)) * sc
+
+
+def apply_ghat(Ghat, J):
+    """apply_ghat (mod_ClosedFormAssembly.jl:155-196): J (Nx, Ny, 3) density,
+    zero-padded to (Px, Py) and fft'd; E^ = Ghat . J^ per padded node;
+    bfft and crop.  Ghat (Px, Py, Nz, 3, 3).  Returns E (Nx, Ny, Nz, 3)."""
+    Px, Py = Ghat.shape[:2]
+    Nx, Ny = J.shape[:2]
+    Jp = np.zeros((Px, Py, 3), np.complex128)
+    Jp[:Nx, :Ny] = J
+    Jh = np.fft.fft2(Jp, axes=(0, 1))
+    Eh = np.einsum("xyzrc,xyc->xyzr", Ghat, Jh)
+    E = np.fft.ifft2(Eh, axes=(0, 1)) * (Px * Py)
+    return E[:Nx, :Ny]
+
+
+def lateral_sum_points(g, W, kx, ky, pts, chunk=8192):
+    """The exact lateral sum of mod_ClosedFormField.jl:743-753 at a list of
+    offsets pts (m, 2): G_m = sum_n W_n g_n exp(i (kx_n x_m + ky_n y_m)),
+    summed over node chunks.  g: (n, 3, 3).  Returns (m, 3, 3)."""
+    pts = np.asarray(pts, np.float64)
+    S = np.asarray(g) * np.asarray(W)[:, None, None]
+    S = np.where(np.isfinite(S), S, 0.0).reshape(len(kx), 9)
+    out = np.zeros((len(pts), 9), np.complex128)
+    for a in range(0, len(kx), chunk):
+        ph = np.exp(1j * (np.outer(pts[:, 0], kx[a:a + chunk]) + np.outer(pts[:, 1], ky[a:a + chunk])))
+        out += ph @ S[a:a + chunk]
+    return out.reshape(len(pts), 3, 3)

@@ -210,3 +210,30 @@ def test_operator_errors():
         dipole_moments(op, [Dipole("x", W0, 0.0, 0.1, 3)])
     P = dipole_moments(op, [Dipole("x", W0, 0.0, 0.1, 2), Dipole((0, 1, 0), W0, 0.05, 0.2, 2, y=-0.05)])
     assert P[0, 0, 4, 4] == 1 and P[1, 1, 3, 5] == 1 and np.count_nonzero(P) == 2
+
+
+@pytest.mark.parametrize("label,src,zs,zobs,kinds", [
+    # one plane each: every block is one stored plane (the fused apply)
+    ("iso 5-layer lossy", 2, [0.3], {2: [0.45], 4: [0.2]}, {2: "structured", 4: "dense"}),
+    ("biaxial 4-layer", 2, [0.4], {2: [0.2]}, {2: "toeplitz_dense"}),
+    # one source plane or one observation plane: Hankel planes merged
+    ("iso 5-layer lossy", 2, [0.3], {2: [0.1, 0.3, 0.5]}, {2: "structured"}),
+    ("iso 5-layer lossy", 1, [0.1, 0.3, 0.5], {1: [0.2], 3: [0.1]}, {1: "structured", 3: "dense"}),
+])
+def test_single_plane_and_merged_blocks(label, src, zs, zobs, kinds):
+    from yellowthinfilms_amd.green_operator import build_layer_operator
+    stack = FOUNDATION_STACKS[label]
+    cache = _cache(stack, src)
+    grid = _grid(16, 8, 0.05, 0.06)
+    print(f"case: single/merged {label} src={src} Ns={len(zs)} No={[len(z) for z in zobs.values()]}", flush=True)
+    op = build_layer_operator(cache, grid, zs, zobs, skeleton=False)
+    assert {b["obs_layer"]: b["kind"] for b in op.blocks} == kinds
+    for b in op.blocks:
+        assert b["n_hankel"] == 0  # merged into the Toeplitz planes
+    P = _moments(16, 8, len(zs), seed=11, n=5)
+    E = _run(op, P)
+    ref = _oracle_apply(stack, cache, zs, zobs, P, grid)
+    for o in zobs:
+        e = relerr(E[o], ref[o])
+        print(f"  obs {o}: rel err {e:.3e}", flush=True)
+        assert e < 1e-10, (o, e)

@@ -528,3 +528,20 @@ def test_linear_convolution_reproduces_free_space_fields():
                     r = [(i - a) * dx, (j - b) * dy, z - zs[kk]]
                     Eref[i, j, o] += E_free(r, P[a, b, kk], k, ef)
     assert relerr(E, Eref) < 1e-9, relerr(E, Eref)
+
+
+def test_padded_fft_route_is_the_linear_convolution():
+    """The oracle's FFT route of the padded assembly (Ĝ from G on the 2Nx x 2Ny
+    offsets, zero-padded source, bfft, crop) equals the literal sum over
+    source pixels for an arbitrary G: the centring sign, the padding and the
+    crop are pinned independently of the physics."""
+    rng = np.random.default_rng(5)
+    Nx, Ny, Nz, dx, dy = 6, 4, 2, 0.3, 0.2
+    Px, Py = 2 * Nx, 2 * Ny
+    G = rng.normal(size=(Px, Py, Nz, 3, 3)) + 1j * rng.normal(size=(Px, Py, Nz, 3, 3))
+    J = rng.normal(size=(Nx, Ny, 3)) + 1j * rng.normal(size=(Nx, Ny, 3))
+    E = po.apply_ghat(po.ghat_from_G(G, dx, dy), J)
+    # offsets -(N-1)..(N-1) sit at padded index offset + N (xs[N] = 0)
+    Goff = G[1:, 1:][:, :, None]  # (2Nx-1, 2Ny-1, Ns=1, Nz, 3, 3)
+    Eref = po.linear_convolution(Goff, (J * dx * dy)[:, :, None, :])
+    assert relerr(E, Eref) < 1e-13, relerr(E, Eref)

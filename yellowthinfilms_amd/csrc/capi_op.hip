@@ -285,6 +285,43 @@ int fill_hankel(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double
   return fill_planes(c, B, b.obs, reqs, YTF_TERMS_HANKEL);
 }
 
+// With one source or one observation plane every Hankel plane coincides with
+// one Toeplitz plane (i + j = s and j - i = m pick the same pair), and with
+// one of each the dense scattered plane does too: store their sums, so the
+// apply reads one plane per pair.
+int merge_planes(ytf_ctx *c, OpBlock &b, int Ns, int64_t nn) {
+  const int No = (int)b.zo.size();
+  const size_t pb = sizeof(cd) * 9 * (size_t)nn;
+  if (b.H && b.T && (Ns == 1 || No == 1)) {
+    for (int s = 0; s < b.nH; ++s) {
+      const int i = Ns == 1 ? 0 : s, j = Ns == 1 ? s : 0;
+      hipError_t e = ytf::launch_op_add(9 * nn, b.H + (int64_t)s * 9 * nn, b.T + (int64_t)(j - i + Ns - 1) * 9 * nn,
+                                        c->stream);
+      if (e != hipSuccess) return fail(c, YTF_EHIP, "operator build: merge: %s", hipGetErrorString(e));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.H));
+    b.H = nullptr;
+    b.bytes -= (int64_t)(pb * b.nH);
+    b.nH = 0;
+  }
+  if (b.D && b.T && Ns == 1 && No == 1 && b.nT == 1) {
+    hipError_t e = ytf::launch_op_add(9 * nn, b.D, b.T, c->stream);
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "operator build: merge: %s", hipGetErrorString(e));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.D));
+    b.D = nullptr;
+    b.bytes -= (int64_t)pb;
+    b.nDs = b.nDo = 0;
+  }
+  return YTF_OK;
+}
+
+// The block is one stored plane between one source and one observation plane.
+bool single_plane(const OpBlock &b, int Ns) {
+  return Ns == 1 && b.zo.size() == 1 && !b.H && ((b.T && !b.D && b.nT == 1) || (b.D && !b.T));
+}
+
 }  // namespace
 
 extern "C" {
@@ -509,6 +546,7 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
       b.kind = YTF_BLOCK_DENSE;
       rc = fill_dense(c, B, b, zs_int, YTF_TERMS_ALL);
     }
+    if (!rc) rc = merge_planes(c, b, Ns, (int64_t)op->Px * op->Py);
     if (rc) return bail(rc);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     b.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -597,6 +635,27 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
   const cd *P = (const cd *)P_dev;
   const cd *pbase = op->flip ? P + (int64_t)(Ns - 1) * nxy : P;
   const int64_t pa = op->flip ? -nxy : nxy;
+  bool fused = !op->blocks.empty();
+  for (auto &b : op->blocks) fused = fused && single_plane(b, Ns);
+  if (fused) {
+    // one plane per block: FFT along x, then per block the fused y FFT,
+    // 3x3 product and inverse y FFT, and the inverse x FFT
+    hipError_t e = timed(c, "op_fft_fwd", [&] {
+      return ytf::launch_opx(false, Px, pbase, 3, pa, (int64_t)Ns * nxy, Nx, Nx, Ny, J, Py, nn, 3, twx, c->stream);
+    });
+    for (size_t bi = 0; bi < op->blocks.size() && e == hipSuccess; ++bi) {
+      const OpBlock &b = op->blocks[bi];
+      e = timed(c, "op_mix", [&] {
+        return ytf::launch_opy_mix1(Py, J, Ew, Px, b.T ? b.T : b.D, Ny, Ny, twy, c->stream);
+      });
+      if (e == hipSuccess)
+        e = timed(c, "op_fft_inv", [&] {
+          return ytf::launch_opx(true, Px, Ew, 3, nxy, nxy, Nx, Nx, Ny, (cd *)E_dev[bi], Py, nn, 3, twx, c->stream);
+        });
+    }
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_operator_apply: launch failed: %s", hipGetErrorString(e));
+    return YTF_OK;
+  }
   hipError_t e = timed(c, "op_fft_fwd", [&] {
     hipError_t r = ytf::launch_opx(false, Px, pbase, 3, pa, (int64_t)Ns * nxy, Nx, Nx, Ny, J, Py, nn, 3 * Ns, twx,
                                    c->stream);

@@ -131,6 +131,12 @@ __global__ __launch_bounds__(256) void k_lat_direct(int64_t n, const double *__r
 // Padded 2D FFT passes
 // --------------------------------------------------------------------------
 
+// Rows per block of the x passes: at most 512 threads and 8 rows (the rows
+// give the length of the contiguous segments of the transposed accesses).
+template <int PX> struct OpX {
+  static constexpr int YT = (4096 / PX) < 4 ? (4096 / PX < 1 ? 1 : 4096 / PX) : ((4096 / PX) > 8 ? 8 : 4096 / PX);
+};
+
 // Forward FFT along x of plane t = blockIdx.y, input rows zero-padded from
 // nx_in to PX and rows >= ny_rows skipped; output transposed: out[t][x][y]
 // (row stride py).  Input plane t starts at in + (t / pdiv) * pa + (t % pdiv) * pb.
@@ -138,7 +144,7 @@ template <int PX>
 __global__ __launch_bounds__(1024) void k_opx_fwd(const cd *__restrict__ in, int pdiv, int64_t pa, int64_t pb,
                                                   int in_row, int nx_in, int ny_rows, cd *__restrict__ out, int py,
                                                   int64_t out_plane, const cd *__restrict__ tw) {
-  constexpr int YT = XTile<PX>::YT;
+  constexpr int YT = OpX<PX>::YT;
   constexpr int TPR = PX / 8;
   extern __shared__ cd lds[];
   const int t = blockIdx.y, tid = threadIdx.x;
@@ -163,7 +169,7 @@ template <int PX>
 __global__ __launch_bounds__(1024) void k_opx_inv(const cd *__restrict__ in, int py, int64_t in_plane,
                                                   cd *__restrict__ out, int pdiv, int64_t pa, int64_t pb, int out_row,
                                                   int nx_out, int ny_rows, const cd *__restrict__ tw) {
-  constexpr int YT = XTile<PX>::YT;
+  constexpr int YT = OpX<PX>::YT;
   constexpr int TPR = PX / 8;
   extern __shared__ cd lds[];
   const int t = blockIdx.y, tid = threadIdx.x;
@@ -266,6 +272,54 @@ __global__ __launch_bounds__(256) void k_op_zcomb(int64_t nn, const cd *__restri
   }
 }
 
+// Single source and observation plane, fused: for XT columns x, FFT along y
+// of the three source components in[c][x][y] (rows >= ny_in zero), the 3x3
+// product with the stored plane K[e][x][y], inverse FFT along y, rows < ny_out
+// stored to out[r][x][y].
+template <int PY> struct OpYM {
+  // columns per block: one for long columns (three blocks fit a CU's LDS and
+  // interleave their load, FFT and store phases), more for short ones
+  static constexpr int XT = PY >= 512 ? 1 : (PY >= 128 ? 2 : 8);
+};
+template <int PY>
+__global__ __launch_bounds__(1024) void k_opy_mix1(const cd *__restrict__ in, cd *__restrict__ out, int64_t nn,
+                                                   const cd *__restrict__ K, int ny_in, int ny_out,
+                                                   const cd *__restrict__ tw) {
+  constexpr int XT = OpYM<PY>::XT;
+  constexpr int TPC = PY / 8;
+  extern __shared__ cd lds[];  // [3 * XT][PY]
+  const int x0 = blockIdx.x * XT, tid = threadIdx.x;
+  const int nthr = 3 * XT * TPC;
+  for (int e = tid; e < 3 * XT * PY; e += nthr) {
+    const int col = e / PY, y = e % PY;
+    const int c = col / XT, xi = col % XT;
+    lds[e] = y < ny_in ? in[c * nn + y + (int64_t)PY * (x0 + xi)] : cd{0.0, 0.0};
+  }
+  __syncthreads();
+  fft_lds<PY, false>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  for (int e = tid; e < XT * PY; e += nthr) {
+    const int xi = e / PY, y = e % PY;
+    const int64_t node = y + (int64_t)PY * (x0 + xi);
+    const cd j0 = lds[(0 * XT + xi) * PY + y], j1 = lds[(1 * XT + xi) * PY + y], j2 = lds[(2 * XT + xi) * PY + y];
+    cd o3[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      cd a = K[(r + 0) * nn + node] * j0;
+      a = fmac(a, K[(r + 3) * nn + node], j1);
+      o3[r] = fmac(a, K[(r + 6) * nn + node], j2);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) lds[(r * XT + xi) * PY + y] = o3[r];
+  }
+  __syncthreads();
+  fft_lds<PY, true>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  for (int e = tid; e < 3 * XT * PY; e += nthr) {
+    const int col = e / PY, y = e % PY;
+    const int c = col / XT, xi = col % XT;
+    if (y < ny_out) out[c * nn + y + (int64_t)PY * (x0 + xi)] = lds[e];
+  }
+}
+
 // dst[e][node] += src[e][node] over n complex values (summing operators).
 __global__ void k_op_add(int64_t n, const cd *__restrict__ src, cd *__restrict__ dst) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -313,7 +367,7 @@ hipError_t launch_lat_direct(int64_t n, const double *kx, const double *ky, cons
 template <int PX>
 static hipError_t opx(bool inv, const cd *in, int pdiv, int64_t pa, int64_t pb, int row, int nx, int ny_rows,
                       cd *out, int py, int64_t plane, int planes, const cd *tw, hipStream_t s) {
-  constexpr int YT = XTile<PX>::YT;
+  constexpr int YT = OpX<PX>::YT;
   const dim3 grid((ny_rows + YT - 1) / YT, planes), block(YT * (PX / 8));
   const size_t sh = sizeof(cd) * YT * PX;
   if (inv)
@@ -356,6 +410,31 @@ hipError_t launch_opy(bool inv, int PY, cd *work, int px, int64_t plane, int pla
     YTF_OPY(8) YTF_OPY(16) YTF_OPY(32) YTF_OPY(64) YTF_OPY(128) YTF_OPY(256) YTF_OPY(512) YTF_OPY(1024)
     YTF_OPY(2048)
 #undef YTF_OPY
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int PY>
+static hipError_t opy_mix1(const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
+                           hipStream_t s) {
+  constexpr int XT = OpYM<PY>::XT;
+  if (px % XT) return hipErrorInvalidValue;
+  const int64_t nn = (int64_t)px * PY;
+  const dim3 grid(px / XT), block(3 * XT * (PY / 8));
+  const size_t sh = sizeof(cd) * 3 * XT * PY;
+  hipLaunchKernelGGL(k_opy_mix1<PY>, grid, block, sh, s, in, out, nn, K, ny_in, ny_out, tw);
+  return hipGetLastError();
+}
+
+hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
+                           hipStream_t s) {
+  if (px < 2) return hipErrorInvalidValue;
+  switch (PY) {
+#define YTF_OPYM(N) \
+  case N: return opy_mix1<N>(in, out, px, K, ny_in, ny_out, tw, s);
+    YTF_OPYM(8) YTF_OPYM(16) YTF_OPYM(32) YTF_OPYM(64) YTF_OPYM(128) YTF_OPYM(256) YTF_OPYM(512) YTF_OPYM(1024)
+    YTF_OPYM(2048)
+#undef YTF_OPYM
   }
   return hipErrorInvalidValue;
 }

@@ -153,6 +153,8 @@ hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const c
 hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
                            hipStream_t s);
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
+hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
+                           hipStream_t s);
 hipError_t launch_op_export(int px, int py, const cd *slot, cd *out, hipStream_t s);
 
 }  // namespace ytf

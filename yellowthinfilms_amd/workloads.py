@@ -181,6 +181,31 @@ def nufft_points(wl=NUFFT_1E6):
 UNEVEN_256_FLOPS_PER_LAUNCH = 2.8317e11
 
 
+OPERATOR_512 = dict(
+    name="operator_512",
+    description=("padded linear-convolution Green operator (apply_operator) of the batched_gyro_512 stack on its "
+                 "512x512 grid (L=16, dx=1/16; G on the 1024x1024 padded offsets): one source plane (layer 1, "
+                 "20 nm above the film) onto one observation plane (layer 1, 40 nm above the film), built from "
+                 "262144 polar nodes (1024 Gauss-Legendre k_rho on [0, 8 k0] broken at the light lines x 256 "
+                 "uniform phi) with the NUFFT lateral sum (eps 1e-9); one apply = the FFT convolution of a "
+                 "512x512x3 N(0,1) complex moment sheet; complex FP64"),
+    L=16.0, dx=1.0 / 16.0,
+    source_layer=1, zsrc=20 * NM, obs_layer=1, zobs=1.0 - 40 * NM,
+    n_krho=1024, n_phi=256, kmax_k0=8.0, eps=1e-9, seed=1,
+)
+
+
+def operator_nodes(wl=OPERATOR_512):
+    """(k_rho, w_rho, phi, w_phi) of operator_512."""
+    from . import quadrature as Q
+    stack = gyro_512_stack()
+    kmax = wl["kmax_k0"] * OMEGA
+    breaks = [0.0, kmax] + [k for k in Q.light_lines(stack, OMEGA) if 0 < k < kmax]
+    kr, wr = Q.panel_rule(breaks, wl["n_krho"], order=16)
+    phi = 2 * math.pi * np.arange(wl["n_phi"]) / wl["n_phi"]
+    return kr, wr, phi, np.full(wl["n_phi"], 2 * math.pi / wl["n_phi"])
+
+
 BATCHED_GYRO_512 = dict(
     name="batched_gyro_512",
     description=("5-layer stack with a 40 nm gyrotropic film (g=0.3, Im eps=0.02), uniform 512x512 kx,ky grid "
