@@ -360,6 +360,14 @@ typedef struct ytf_block_info {
 /* operator_summary (mod_GreenOperator.jl:735-747). */
 int ytf_operator_num_blocks(const ytf_operator *op);
 int ytf_operator_block_info(ytf_ctx *ctx, const ytf_operator *op, int block, ytf_block_info *info);
+/* The z skeleton of one side of a block (ZSkeleton, mod_GreenOperator.jl:
+ * 202-208): side 0 the source planes (in increasing cavity-frame depth),
+ * side 1 the observation planes.  n_planes receives k, the skeleton planes
+ * (N when the side is kept whole); idx (k, 0-based, increasing), P (N x k
+ * complex column-major, host; the identity when whole) and err (the largest
+ * interpolation error over every node term) may be NULL. */
+int ytf_operator_skeleton(ytf_ctx *ctx, const ytf_operator *op, int block, int side, int32_t *n_planes,
+                          int32_t *idx, double *P, double *err);
 /* One stored Ĝ plane of a block in the reference layout (Px, Py, 3, 3),
  * Px = 2Nx, Py = 2Ny: part 0 Toeplitz plane m + n_src - 1 (m = j - i),
  * part 1 Hankel plane i + j, part 2 dense pair i + n_src_skel * j (source

@@ -396,3 +396,72 @@ def lateral_sum_points(g, W, kx, ky, pts, chunk=8192):
         ph = np.exp(1j * (np.outer(pts[:, 0], kx[a:a + chunk]) + np.outer(pts[:, 1], ky[a:a + chunk])))
         out += ph @ S[a:a + chunk]
     return out.reshape(len(pts), 3, 3)
+
+
+def _qlist(*arrays):
+    """Distinct finite non-zero values of node arrays (_qlist, mod_GreenOperator.jl:251-258)."""
+    v = np.concatenate([np.asarray(a, np.complex128).ravel() for a in arrays]) if arrays else np.zeros(0, complex)
+    v = v[np.isfinite(v) & (v != 0)]
+    return np.unique(v)
+
+
+def family_rows(fam, z):
+    """Rows exp(i Q (sigma z_j + c)) of the family fam = [(Q, sigma, c), ...]
+    (node x plane), underflowing to 0 below exp(-700) (_family_rows,
+    mod_GreenOperator.jl:222-230)."""
+    z = np.asarray(z, np.float64)
+    out = []
+    for Q, sig, c in fam:
+        x = 1j * np.asarray(Q, np.complex128)[:, None] * (sig * z[None, :] + c)
+        out.append(np.where(x.real < -700, 0.0, np.exp(np.where(x.real < -700, 0.0, x))))
+    return np.vstack(out) if out else np.zeros((0, len(z)), complex)
+
+
+def zskeleton(z, fam, tol, whole_frac=0.8):
+    """_zskeleton (mod_GreenOperator.jl:262-302) with numpy / scipy (LAPACK)
+    factorizations: R of the family's QR, column-pivoted QR of R (geqp3, as
+    qr(R, ColumnNorm())), the interpolative decomposition of the smallest
+    rank whose largest error over every row is <= tol (bisection), kept whole
+    at >= whole_frac of the planes.  Returns dict(idx, P, err, whole)."""
+    import scipy.linalg as sl
+    z = np.asarray(z, np.float64)
+    N = len(z)
+    whole = dict(idx=np.arange(N), P=np.eye(N, dtype=complex), err=0.0, whole=True)
+    if N <= 1:
+        return whole
+    A = family_rows(fam, z)
+    R = np.linalg.qr(A, mode="r") if A.shape[0] else np.zeros((0, N), complex)
+    if R.shape[0] < N:
+        R = np.vstack([R, np.zeros((N - R.shape[0], N), complex)])
+    Rt, p = sl.qr(R, mode="r", pivoting=True)
+    Rt = Rt[:N]
+
+    def interp(k):
+        P = np.zeros((N, k), complex)
+        P[p[:k], np.arange(k)] = 1
+        if k < N:
+            # back substitution; an exactly zero pivot (fewer independent rows
+            # than planes) takes zero weights where the reference's
+            # triangular solve would raise
+            T = np.zeros((k, N - k), complex)
+            for i in range(k - 1, -1, -1):
+                if Rt[i, i] != 0:
+                    T[i] = (Rt[i, k:] - Rt[i, i + 1:k] @ T[i + 1:k]) / Rt[i, i]
+            P[p[k:], :] = T.T
+        perm = np.argsort(p[:k])
+        return p[:k][perm], P[:, perm]
+
+    def maxerr(idx, P):
+        return float(np.max(np.abs(A - A[:, idx] @ P.T))) if A.size else 0.0
+
+    lo, hi = 0, N
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if maxerr(*interp(mid)) <= tol:
+            hi = mid
+        else:
+            lo = mid
+    if hi >= whole_frac * N:
+        return whole
+    idx, P = interp(hi)
+    return dict(idx=idx, P=P, err=maxerr(idx, P), whole=False)

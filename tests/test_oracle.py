@@ -545,3 +545,28 @@ def test_padded_fft_route_is_the_linear_convolution():
     Goff = G[1:, 1:][:, :, None]  # (2Nx-1, 2Ny-1, Ns=1, Nz, 3, 3)
     Eref = po.linear_convolution(Goff, (J * dx * dy)[:, :, None, :])
     assert relerr(E, Eref) < 1e-13, relerr(E, Eref)
+
+
+def test_zskeleton_of_a_family_of_known_rank():
+    """r distinct exponentials on N planes span an r-dimensional space: the
+    skeleton has exactly r planes, interpolates every row to rounding, and
+    P restricted to the skeleton planes is the identity; a random family
+    (full rank) is kept whole."""
+    z = np.linspace(0.0, 1.3, 20)
+    Q = np.array([1.0 + 0.01j, 2.5 + 0.2j, 4.0 + 0.05j])
+    sk = po.zskeleton(z, [(Q, 1.0, 0.0)], tol=1e-10)
+    assert not sk["whole"] and len(sk["idx"]) == 3 and sk["err"] < 1e-12
+    assert np.allclose(sk["P"][sk["idx"]], np.eye(3), atol=1e-13)
+    A = po.family_rows([(Q, 1.0, 0.0)], z)
+    assert np.max(np.abs(A - A[:, sk["idx"]] @ sk["P"].T)) < 1e-12
+    # two families with distinct (sigma, c) share the planes: rank 3 + 2
+    sk2 = po.zskeleton(z, [(Q, 1.0, 0.0), (np.array([3.0 + 0.1j, 0.5 + 0.0j]), -1.0, 0.4)], tol=1e-10)
+    assert len(sk2["idx"]) == 5
+    rng = np.random.default_rng(0)
+    sk3 = po.zskeleton(z, [(rng.normal(size=400) * 30 + 1j * rng.uniform(0, 0.1, 400), 1.0, 0.0)], tol=1e-10)
+    assert sk3["whole"]
+    # a looser tolerance gives a smaller skeleton within that tolerance
+    dense_Q = np.linspace(0.1, 6.0, 200) + 0.01j
+    a = po.zskeleton(z, [(dense_Q, 1.0, 0.0)], tol=1e-9)
+    b = po.zskeleton(z, [(dense_Q, 1.0, 0.0)], tol=1e-4)
+    assert len(b["idx"]) < len(a["idx"]) and b["err"] <= 1e-4

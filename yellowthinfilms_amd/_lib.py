@@ -112,6 +112,8 @@ _SIGS = {
     "ytf_operator_num_blocks": (ct.c_int, [ct.c_void_p]),
     "ytf_operator_block_info": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.POINTER(YtfBlockInfo)]),
     "ytf_operator_ghat": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]),
+    "ytf_operator_skeleton": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(ct.c_int32),
+                                         ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
 }
 
 TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}

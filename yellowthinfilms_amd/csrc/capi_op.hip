@@ -60,7 +60,9 @@ struct OpBlock {
   cd *T = nullptr, *H = nullptr, *D = nullptr;  // [slot][9][Px*Py]
   int nT = 0, nH = 0, nDs = 0, nDo = 0;          // D slot = i + nDs * j
   cd *PsM = nullptr, *PoM = nullptr;             // fold / expansion matrices (skeleton)
-  std::vector<int> sidx, oidx;
+  std::vector<int> sidx, oidx;                   // skeleton planes of each side (all if kept whole)
+  std::vector<std::complex<double>> Ps, Po;      // interpolation matrices (N x k), empty if whole
+  double serr = 0.0, oerr = 0.0;
   double build_ms = 0.0, resid = -1.0;
   int64_t bytes = 0;
 };
@@ -243,16 +245,260 @@ void hankel_pair(int s, int Ns, int No, int *i, int *j) {
   *i = s - (No - 1); *j = No - 1;
 }
 
-int fill_dense(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double> &zs, int terms) {
+// ---------------------------------------------------------------------------
+// z skeletons (_zskeleton, z_skeletons, mod_GreenOperator.jl:186-359)
+// ---------------------------------------------------------------------------
+struct Skel {
+  bool whole = true;
+  int N = 0;
+  std::vector<int> idx;  // skeleton planes, increasing
+  std::vector<zc> P;     // N x k column-major, P[idx, :] = I
+  double err = 0.0;
+};
+
+// Column-pivoted Householder QR of the N x N matrix R in place (pivot: the
+// largest remaining column norm, the first on ties, as LAPACK's geqp3 /
+// qr(R, ColumnNorm())); p receives the column order.
+void pivoted_qr(std::vector<zc> &R, int N, std::vector<int> &p) {
+  p.resize(N);
+  for (int j = 0; j < N; ++j) p[j] = j;
+  for (int k = 0; k < N; ++k) {
+    int jm = k;
+    double best = -1.0;
+    for (int j = k; j < N; ++j) {
+      double s = 0.0;
+      for (int i = k; i < N; ++i) s += std::norm(R[i + (size_t)N * j]);
+      if (s > best) { best = s; jm = j; }
+    }
+    if (jm != k) {
+      for (int i = 0; i < N; ++i) std::swap(R[i + (size_t)N * k], R[i + (size_t)N * jm]);
+      std::swap(p[k], p[jm]);
+    }
+    double nx = 0.0;
+    for (int i = k; i < N; ++i) nx += std::norm(R[i + (size_t)N * k]);
+    nx = std::sqrt(nx);
+    if (nx == 0.0) continue;
+    const zc alpha = R[k + (size_t)N * k];
+    const double aa = std::abs(alpha);
+    const zc ph = aa > 0.0 ? alpha / aa : zc(1.0, 0.0);
+    const zc beta = -ph * nx;
+    std::vector<zc> v(N - k);
+    v[0] = alpha - beta;
+    for (int i = k + 1; i < N; ++i) v[i - k] = R[i + (size_t)N * k];
+    const double vv = 2.0 * nx * (nx + aa);
+    for (int j = k + 1; j < N; ++j) {
+      zc w = 0.0;
+      for (int i = k; i < N; ++i) w += std::conj(v[i - k]) * R[i + (size_t)N * j];
+      const zc g = 2.0 * w / vv;
+      for (int i = k; i < N; ++i) R[i + (size_t)N * j] -= v[i - k] * g;
+    }
+    R[k + (size_t)N * k] = beta;
+    for (int i = k + 1; i < N; ++i) R[i + (size_t)N * k] = 0.0;
+  }
+}
+
+// The interpolative decomposition of rank k from the pivoted factor
+// (interp in _zskeleton_serial_blas): P[p[q], q] = 1, the other rows
+// Rt[1:k,1:k] \ Rt[1:k,k+1:N] transposed; skeleton planes sorted increasing.
+void interp(const std::vector<zc> &Rt, const std::vector<int> &p, int N, int k, std::vector<int> &idx,
+            std::vector<zc> &P) {
+  std::vector<zc> Pk((size_t)N * k, 0.0);
+  for (int q = 0; q < k; ++q) Pk[p[q] + (size_t)N * q] = 1.0;
+  for (int c = k; c < N; ++c) {
+    // T[:, c] = Rt[0:k, 0:k] \ Rt[0:k, c] (back substitution)
+    std::vector<zc> t(k);
+    for (int i = k - 1; i >= 0; --i) {
+      zc s = Rt[i + (size_t)N * c];
+      for (int m = i + 1; m < k; ++m) s -= Rt[i + (size_t)N * m] * t[m];
+      // an exactly zero pivot (fewer independent rows than planes): zero weight
+      const zc piv = Rt[i + (size_t)N * i];
+      t[i] = piv == zc(0.0, 0.0) ? zc(0.0, 0.0) : s / piv;
+    }
+    for (int q = 0; q < k; ++q) Pk[p[c] + (size_t)N * q] = t[q];
+  }
+  std::vector<int> perm(k);
+  for (int q = 0; q < k; ++q) perm[q] = q;
+  std::sort(perm.begin(), perm.end(), [&](int a, int b) { return p[a] < p[b]; });
+  idx.resize(k);
+  P.assign((size_t)N * k, 0.0);
+  for (int q = 0; q < k; ++q) {
+    idx[q] = p[perm[q]];
+    for (int i = 0; i < N; ++i) P[i + (size_t)N * q] = Pk[i + (size_t)N * perm[q]];
+  }
+}
+
+// The skeleton of one side: family rows from the node solve's mode
+// wavenumbers (Q: [4][n] device), tall-skinny QR on the device, pivoted QR
+// and rank bisection with the error checked on every row on the device.
+// Sides of more than 64 planes are kept whole.
+int compute_skeleton(ytf_ctx *c, int64_t n, const cd *Q, const int fam[4], const double sig[2], const double cc[2],
+                     const std::vector<double> &z, double tol, Skel *out) {
+  const int N = (int)z.size();
+  out->whole = true;
+  out->N = N;
+  out->idx.resize(N);
+  for (int j = 0; j < N; ++j) out->idx[j] = j;
+  out->P.assign((size_t)N * N, 0.0);
+  for (int j = 0; j < N; ++j) out->P[j + (size_t)N * j] = 1.0;
+  out->err = 0.0;
+  if (N <= 1 || N > 64 || n == 0) return YTF_OK;
+  const int64_t nrows = 4 * n;
+  const int64_t nch = (nrows + 63) / 64;
+  const int G = (int)std::min<int64_t>(256, nch);
+  DevBuf zb, rb, eb;
+  int rc = dev_alloc(c, zb, sizeof(double) * N);
+  if (!rc) rc = dev_alloc(c, rb, sizeof(cd) * ((size_t)G + 1) * N * N);
+  if (!rc) rc = dev_alloc(c, eb, sizeof(unsigned long long) + sizeof(int) * 64 + sizeof(cd) * 64 * 64 + 256);
+  if (!rc) rc = upload(c, zb.p, z.data(), sizeof(double) * N);
+  if (rc) return rc;
+  ytf::SkFamRows F;
+  F.Q = Q;
+  F.n = n;
+  for (int e = 0; e < 4; ++e) F.fam[e] = fam[e];
+  for (int f = 0; f < 2; ++f) {
+    F.sig[f] = sig[f];
+    F.c[f] = cc[f];
+  }
+  F.z = (const double *)zb.p;
+  cd *parts = (cd *)rb.p, *Rf = parts + (size_t)G * N * N;
+  hipError_t e = ytf::launch_sk_tsqr(F, nullptr, 0, N, nrows, parts, G, c->stream);
+  if (e == hipSuccess) e = ytf::launch_sk_tsqr(F, parts, 1, N, (int64_t)G * N, Rf, 1, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "z skeleton: QR: %s", hipGetErrorString(e));
+  std::vector<zc> R((size_t)N * N);
+  HIPCHK(c, hipMemcpyAsync(R.data(), Rf, sizeof(cd) * R.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> p;
+  pivoted_qr(R, N, p);
+  unsigned long long *errd = (unsigned long long *)eb.p;
+  int *idxd = (int *)((char *)eb.p + 256);
+  cd *Pd = (cd *)((char *)eb.p + 256 + sizeof(int) * 64);
+  auto maxerr = [&](const std::vector<int> &idx, const std::vector<zc> &P, double *err) -> int {
+    const int k = (int)idx.size();
+    std::vector<char> h(sizeof(int) * 64 + sizeof(cd) * (size_t)N * k);
+    std::memcpy(h.data(), idx.data(), sizeof(int) * k);
+    std::memcpy(h.data() + sizeof(int) * 64, P.data(), sizeof(cd) * (size_t)N * k);
+    int r = upload(c, idxd, h.data(), h.size());
+    if (r) return r;
+    hipError_t e2 = ytf::launch_sk_err(F, nrows, N, k, idxd, Pd, errd, c->stream);
+    if (e2 != hipSuccess) return fail(c, YTF_EHIP, "z skeleton: error check: %s", hipGetErrorString(e2));
+    unsigned long long bits = 0;
+    HIPCHK(c, hipMemcpyAsync(&bits, errd, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(err, &bits, sizeof(double));
+    return YTF_OK;
+  };
+  int lo = 0, hi = N;
+  std::vector<int> idx;
+  std::vector<zc> P;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    interp(R, p, N, mid, idx, P);
+    double err = 0.0;
+    rc = maxerr(idx, P, &err);
+    if (rc) return rc;
+    if (err <= tol) hi = mid;
+    else lo = mid;
+  }
+  if (hi >= 0.8 * N) return YTF_OK;  // whole_frac
+  interp(R, p, N, hi, idx, P);
+  double err = 0.0;
+  rc = maxerr(idx, P, &err);
+  if (rc) return rc;
+  out->whole = false;
+  out->idx = idx;
+  out->P = P;
+  out->err = err;
+  return YTF_OK;
+}
+
+// Dense plane pairs (_dense_planes, mod_GreenOperator.jl:568-574) on the
+// source planes zs and observation planes zo (all of a side's planes, or its
+// skeleton planes).
+int fill_dense(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double> &zs, int terms,
+               const std::vector<double> *zo_sub = nullptr) {
   const int64_t nn = (int64_t)B.Px * B.Py;
+  const std::vector<double> &zo = zo_sub ? *zo_sub : b.zo;
   b.nDs = (int)zs.size();
-  b.nDo = (int)b.zo.size();
+  b.nDo = (int)zo.size();
   int rc = alloc_planes(c, &b.D, b.nDs * b.nDo, nn, &b.bytes);
   if (rc) return rc;
   std::vector<Req> reqs;
   for (int j = 0; j < b.nDo; ++j)
-    for (int i = 0; i < b.nDs; ++i) reqs.push_back({zs[i], b.zo[j], b.D + (int64_t)(i + b.nDs * j) * 9 * nn});
+    for (int i = 0; i < b.nDs; ++i) reqs.push_back({zs[i], zo[j], b.D + (int64_t)(i + b.nDs * j) * 9 * nn});
   return fill_planes(c, B, b.obs, reqs, terms);
+}
+
+// Skeletons of the source side and of observation layer `obs` from one node
+// solve's mode wavenumbers (_cache_skeletons, _src_family, _obs_family,
+// mod_GreenOperator.jl:315-359).
+int side_skeletons(ytf_ctx *c, const Build &B, int obs, const std::vector<double> &zs_int,
+                   const std::vector<double> &zo, double tol, Skel *src_sk, bool need_src, Skel *obs_sk) {
+  const int64_t n = B.n;
+  DevBuf qb;
+  int rc = dev_alloc(c, qb, sizeof(cd) * 20 * (size_t)std::max<int64_t>(n, 1));
+  if (rc) return rc;
+  ytf_node_cache nc;
+  std::memset(&nc, 0, sizeof nc);
+  nc.Qsrc_dev = (double *)qb.p;
+  nc.basis_dev = (double *)((cd *)qb.p + 4 * n);
+  rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, n, B.src, obs, 0, nullptr, nullptr, nullptr, &nc,
+                          YTF_TERMS_ALL);
+  if (rc) return rc;
+  if (need_src) {
+    // source planes at cavity-frame depth zpos: Psi_f = e^{iQf (d - zpos)},
+    // Psi_b = e^{-iQb zpos}; a source in layer 1 feeds through Psi_f only
+    const double d = B.layers[B.src - 1].d;
+    std::vector<double> zpos(zs_int);
+    if (B.src == 1)
+      for (auto &v : zpos) v = d - v;
+    const int fam[4] = {0, 0, B.src != 1 ? 1 : -1, B.src != 1 ? 1 : -1};
+    const double sig[2] = {-1.0, -1.0}, cc[2] = {d, 0.0};
+    rc = compute_skeleton(c, n, (const cd *)qb.p, fam, sig, cc, zpos, tol, src_sk);
+    if (rc) return rc;
+  }
+  // observation planes: e^{iKZf z'}, e^{iKZb (z' - d)}; layer 1 carries the
+  // downward wave only, the last layer the upward one only
+  const double d = B.layers[obs - 1].d;
+  const int fam[4] = {obs != 1 ? 0 : -1, obs != 1 ? 0 : -1, obs != B.nl ? 1 : -1, obs != B.nl ? 1 : -1};
+  const double sig[2] = {1.0, 1.0}, cc[2] = {0.0, -d};
+  return compute_skeleton(c, n, (const cd *)qb.p + 16 * n, fam, sig, cc, zo, tol, obs_sk);
+}
+
+int upload_matrix(ytf_ctx *c, cd **dst, const std::vector<zc> &h, int64_t *bytes) {
+  HIPCHK(c, hipMalloc((void **)dst, sizeof(cd) * std::max<size_t>(h.size(), 1)));
+  *bytes += (int64_t)(sizeof(cd) * h.size());
+  return upload(c, *dst, h.data(), sizeof(cd) * h.size());
+}
+
+// The dense part of a block without z structure, on skeleton planes where a
+// side compresses (_planes_maybe_skeleton, mod_GreenOperator.jl:581-587).
+int fill_maybe_skeleton(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double> &zs_int, int terms,
+                        const Skel *ss, const Skel *os) {
+  if (!ss || !os || (ss->whole && os->whole)) return fill_dense(c, B, b, zs_int, terms);
+  std::vector<double> zs, zo;
+  for (int i : ss->idx) zs.push_back(zs_int[i]);
+  for (int j : os->idx) zo.push_back(b.zo[j]);
+  int rc = fill_dense(c, B, b, zs, terms, &zo);
+  if (rc) return rc;
+  b.sidx = ss->idx;
+  b.oidx = os->idx;
+  if (!ss->whole) b.Ps = ss->P;
+  if (!os->whole) b.Po = os->P;
+  b.serr = ss->err;
+  b.oerr = os->err;
+  if (!ss->whole) {
+    rc = upload_matrix(c, &b.PsM, ss->P, &b.bytes);  // (Ns x Rs): fold M[i + Ns q] = Ps[i, q]
+    if (rc) return rc;
+  }
+  if (!os->whole) {
+    const int No = (int)b.zo.size(), Ro = (int)os->idx.size();
+    std::vector<zc> t((size_t)Ro * No);
+    for (int j = 0; j < No; ++j)
+      for (int q = 0; q < Ro; ++q) t[q + (size_t)Ro * j] = os->P[j + (size_t)No * q];  // M[p + Ro j] = Po[j, p]
+    rc = upload_matrix(c, &b.PoM, t, &b.bytes);
+    if (rc) return rc;
+  }
+  return YTF_OK;
 }
 
 int fill_toeplitz(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double> &zs, int terms) {
@@ -305,7 +551,7 @@ int merge_planes(ytf_ctx *c, OpBlock &b, int Ns, int64_t nn) {
     b.bytes -= (int64_t)(pb * b.nH);
     b.nH = 0;
   }
-  if (b.D && b.T && Ns == 1 && No == 1 && b.nT == 1) {
+  if (b.D && b.T && Ns == 1 && No == 1 && b.nT == 1 && !b.PsM && !b.PoM) {
     hipError_t e = ytf::launch_op_add(9 * nn, b.D, b.T, c->stream);
     if (e != hipSuccess) return fail(c, YTF_EHIP, "operator build: merge: %s", hipGetErrorString(e));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -319,7 +565,7 @@ int merge_planes(ytf_ctx *c, OpBlock &b, int Ns, int64_t nn) {
 
 // The block is one stored plane between one source and one observation plane.
 bool single_plane(const OpBlock &b, int Ns) {
-  return Ns == 1 && b.zo.size() == 1 && !b.H && ((b.T && !b.D && b.nT == 1) || (b.D && !b.T));
+  return Ns == 1 && b.zo.size() == 1 && !b.H && !b.PsM && !b.PoM && ((b.T && !b.D && b.nT == 1) || (b.D && !b.T));
 }
 
 }  // namespace
@@ -503,10 +749,18 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
     }
   }
 
+  Skel src_sk;
+  bool have_src_sk = false;
   for (auto &ol : order) {
     const int o = ol.first, No = n_obs[ol.second];
     if (No == 0) continue;
     const auto t0 = std::chrono::steady_clock::now();
+    Skel obs_sk;
+    auto skel = [&](OpBlock &blk) -> int {
+      const int r = side_skeletons(c, B, o, zs_int, blk.zo, opt->skeleton_tol, &src_sk, !have_src_sk, &obs_sk);
+      have_src_sk = true;
+      return r;
+    };
     op->blocks.emplace_back();
     OpBlock &b = op->blocks.back();
     b.obs = o;
@@ -536,15 +790,21 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
         rc = fill_toeplitz(c, B, b, zs_int, YTF_TERMS_TOEPLITZ);
         if (!rc) rc = fill_hankel(c, B, b, zs_int);
       } else {
-        if (skeleton) return bail(fail(c, YTF_EINVAL, "ytf_operator_build: skeleton planes are not available"));
         b.kind = YTF_BLOCK_TOEPLITZ_DENSE;
-        rc = fill_toeplitz(c, B, b, zs_int, YTF_TERMS_DIRECT);
-        if (!rc) rc = fill_dense(c, B, b, zs_int, YTF_TERMS_SCATTERED);
+        if (skeleton) rc = skel(b);
+        if (!rc) rc = fill_toeplitz(c, B, b, zs_int, YTF_TERMS_DIRECT);
+        if (!rc)
+          rc = fill_maybe_skeleton(c, B, b, zs_int, YTF_TERMS_SCATTERED, skeleton ? &src_sk : nullptr,
+                                   skeleton ? &obs_sk : nullptr);
+        if (!rc && (b.PsM || b.PoM)) b.kind = YTF_BLOCK_TOEPLITZ_SKELETON;
       }
     } else {
-      if (skeleton) return bail(fail(c, YTF_EINVAL, "ytf_operator_build: skeleton planes are not available"));
       b.kind = YTF_BLOCK_DENSE;
-      rc = fill_dense(c, B, b, zs_int, YTF_TERMS_ALL);
+      if (skeleton) rc = skel(b);
+      if (!rc)
+        rc = fill_maybe_skeleton(c, B, b, zs_int, YTF_TERMS_ALL, skeleton ? &src_sk : nullptr,
+                                 skeleton ? &obs_sk : nullptr);
+      if (!rc && (b.PsM || b.PoM)) b.kind = YTF_BLOCK_SKELETON;
     }
     if (!rc) rc = merge_planes(c, b, Ns, (int64_t)op->Px * op->Py);
     if (rc) return bail(rc);
@@ -589,6 +849,30 @@ int ytf_operator_block_info(ytf_ctx *c, const ytf_operator *op, int block, ytf_b
   return YTF_OK;
 }
 
+int ytf_operator_skeleton(ytf_ctx *c, const ytf_operator *op, int block, int side, int32_t *n_planes,
+                          int32_t *idx, double *P, double *err) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_skeleton: ctx is NULL");
+  if (!op || block < 0 || block >= (int)op->blocks.size() || (side != 0 && side != 1) || !n_planes)
+    return fail(c, YTF_EINVAL, "ytf_operator_skeleton: bad block %d or side %d", block, side);
+  const OpBlock &b = op->blocks[block];
+  const int N = side == 0 ? (int)op->zs.size() : (int)b.zo.size();
+  const std::vector<int> &ix = side == 0 ? b.sidx : b.oidx;
+  const std::vector<zc> &Pm = side == 0 ? b.Ps : b.Po;
+  const int k = ix.empty() ? N : (int)ix.size();
+  *n_planes = k;
+  if (idx)
+    for (int q = 0; q < k; ++q) idx[q] = ix.empty() ? q : ix[q];
+  if (P) {
+    for (int e = 0; e < N * k; ++e) {
+      const zc v = Pm.empty() ? zc((e % N) == (e / N) ? 1.0 : 0.0, 0.0) : Pm[e];
+      P[2 * e] = v.real();
+      P[2 * e + 1] = v.imag();
+    }
+  }
+  if (err) *err = side == 0 ? b.serr : b.oerr;
+  return YTF_OK;
+}
+
 int ytf_operator_ghat(ytf_ctx *c, const ytf_operator *op, int block, int part, int slot, double *ghat_dev) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_ghat: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -614,9 +898,17 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
   HIPCHK(c, hipSetDevice(c->device));
   const int Ns = (int)op->zs.size(), Nx = op->Nx, Ny = op->Ny, Px = op->Px, Py = op->Py;
   const int64_t nn = (int64_t)Px * Py, nxy = (int64_t)Nx * Ny;
-  int no_max = 0;
-  for (auto &b : op->blocks) no_max = std::max(no_max, (int)b.zo.size());
-  const size_t need = sizeof(cd) * 3 * (size_t)nn * (size_t)(Ns + no_max);
+  int no_max = 0, rs = 0, ro_max = 0;
+  const OpBlock *fold = nullptr;  // every skeleton block shares the source skeleton
+  for (auto &b : op->blocks) {
+    no_max = std::max(no_max, (int)b.zo.size());
+    if (b.PsM && !fold) {
+      fold = &b;
+      rs = b.nDs;
+    }
+    if (b.PoM) ro_max = std::max(ro_max, b.nDo);
+  }
+  const size_t need = sizeof(cd) * 3 * (size_t)nn * (size_t)(Ns + no_max + rs + ro_max);
   if (need > op->work_bytes) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (op->work) hipFree(op->work);
@@ -625,7 +917,8 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
     HIPCHK(c, hipMalloc(&op->work, need));
     op->work_bytes = need;
   }
-  cd *J = (cd *)op->work, *Ew = J + 3 * (size_t)nn * Ns;
+  cd *J = (cd *)op->work, *Ew = J + 3 * (size_t)nn * Ns, *Js = Ew + 3 * (size_t)nn * no_max,
+     *Es = Js + 3 * (size_t)nn * rs;
   cd *twx = nullptr, *twy = nullptr;
   int rc = twiddles(c, Px, &twx);
   if (!rc) rc = twiddles(c, Py, &twy);
@@ -662,6 +955,8 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
     if (r == hipSuccess) r = ytf::launch_opy(false, Py, J, Px, nn, 3 * Ns, Ny, Py, 1.0, 0, twy, c->stream);
     return r;
   });
+  if (e == hipSuccess && fold)  // source planes folded onto the skeleton planes (_zfold)
+    e = timed(c, "op_mix", [&] { return ytf::launch_op_zcomb(nn, J, Ns, fold->PsM, Js, rs, 0, c->stream); });
   for (size_t bi = 0; bi < op->blocks.size() && e == hipSuccess; ++bi) {
     const OpBlock &b = op->blocks[bi];
     const int No = (int)b.zo.size();
@@ -677,7 +972,13 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
         acc = true;
       }
       if (r == hipSuccess && b.D) {
-        r = ytf::launch_op_mix(nn, J, Ns, Ew, No, b.D, ytf::OP_MIX_DENSE, acc, c->stream);
+        const cd *Jin = b.PsM ? Js : J;
+        if (b.PoM) {  // skeleton observation planes, interpolated to all planes (_zexpand_add!)
+          r = ytf::launch_op_mix(nn, Jin, b.nDs, Es, b.nDo, b.D, ytf::OP_MIX_DENSE, 0, c->stream);
+          if (r == hipSuccess) r = ytf::launch_op_zcomb(nn, Es, b.nDo, b.PoM, Ew, No, acc, c->stream);
+        } else {
+          r = ytf::launch_op_mix(nn, Jin, b.nDs, Ew, No, b.D, ytf::OP_MIX_DENSE, acc, c->stream);
+        }
         acc = true;
       }
       return r;

@@ -155,6 +155,20 @@ hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *o
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
 hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
                            hipStream_t s);
+
+// z skeletons (skeleton.hip): the exponential family of one side, its
+// tall-skinny QR factor and the interpolation error of a column ID.
+struct SkFamRows {
+  const cd *Q;  // [4][n] mode wavenumbers
+  int64_t n;
+  int fam[4];   // family of each entry, -1 skipped
+  double sig[2], c[2];
+  const double *z;  // N planes
+};
+hipError_t launch_sk_tsqr(const SkFamRows &F, const cd *Rin, int mode, int N, int64_t nrows, cd *Rout, int blocks,
+                          hipStream_t s);
+hipError_t launch_sk_err(const SkFamRows &F, int64_t nrows, int N, int k, const int *idx, const cd *P,
+                         unsigned long long *err, hipStream_t s);
 hipError_t launch_op_export(int px, int py, const cd *slot, cd *out, hipStream_t s);
 
 }  // namespace ytf

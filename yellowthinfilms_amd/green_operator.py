@@ -269,6 +269,24 @@ class LayerGreenOperator:
                 pass
             self.handle = None
 
+    def skeleton(self, block, side):
+        """(idx, P, err) of one side ("source" or "observation") of a block:
+        skeleton planes (0-based), the (N, k) interpolation matrix and its
+        largest error over the node terms (ZSkeleton, mod_GreenOperator.jl:202-208)."""
+        L = _lib.lib()
+        ctx = _lib.Context.get(self.device)
+        sd = {"source": 0, "observation": 1}[side]
+        k = ct.c_int32()
+        _lib.check(L.ytf_operator_skeleton(ctx.handle, self.handle, block, sd, ct.byref(k), None, None, None),
+                   ctx.handle)
+        N = len(self.z_src) if sd == 0 else self.blocks[block]["n_obs"]
+        idx = (ct.c_int32 * max(k.value, 1))()
+        P = np.zeros(2 * N * k.value, np.float64)
+        err = ct.c_double()
+        _lib.check(L.ytf_operator_skeleton(ctx.handle, self.handle, block, sd, ct.byref(k), idx,
+                                           P.ctypes.data_as(_DP), ct.byref(err)), ctx.handle)
+        return np.array(idx[:k.value]), P.view(np.complex128).reshape((N, k.value), order="F"), err.value
+
     def ghat_plane(self, block, part, slot):
         """One stored Ĝ plane (1/(Px Py) folded in) as a (3, 3, Py, Px) tensor
         (the reference's (Px, Py, 3, 3)); part "toeplitz", "hankel" or "dense"."""
