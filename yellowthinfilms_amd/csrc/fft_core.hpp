@@ -45,6 +45,13 @@ __device__ __forceinline__ void dft8(cd *a) {
   a[3] = e3 + o3; a[7] = e3 - o3;
 }
 
+// LDS layout of a row: element i at lpad(i) = i + i/8 (one pad slot per 8
+// elements, so the stride-8 writes of a radix-8 stage fall on distinct
+// banks), rows LdsRow<N>::S elements apart (N + N/8 + 1: consecutive rows
+// start on different banks, for the transposed loads and stores).
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 3); }
+template <int N> struct LdsRow { static constexpr int S = N + N / 8 + 1; };
+
 // One Stockham radix-R stage on a row in LDS: current length n, stride s.
 // Thread t in [0, N/8) performs 8/R butterflies.  tw[j] = exp(-2 pi i j/N).
 template <int N, int R, bool INV>
@@ -57,7 +64,7 @@ __device__ __forceinline__ void stage(cd *row, int t, int n, int s, const cd *tw
     const int b = t + u * (N / 8);
     const int q = b % s, p = b / s;
 #pragma unroll
-    for (int r = 0; r < R; ++r) a[u * R + r] = row[q + s * (p + r * m)];
+    for (int r = 0; r < R; ++r) a[u * R + r] = row[lpad(q + s * (p + r * m))];
   }
   __syncthreads();
 #pragma unroll
@@ -73,14 +80,15 @@ __device__ __forceinline__ void stage(cd *row, int t, int n, int s, const cd *tw
     for (int k = 0; k < R; ++k) {
       cd w = tw[(p * k) * step];
       if (INV) w.i = -w.i;
-      row[q + s * (R * p + k)] = k == 0 ? x[0] : x[k] * w;
+      row[lpad(q + s * (R * p + k))] = k == 0 ? x[0] : x[k] * w;
     }
   }
   __syncthreads();
 }
 
-// Full length-N transform of `row` (in LDS) by N/8 cooperating threads; all
-// threads of the block must call it (it synchronises the block).
+// Full length-N transform of `row` (in LDS, padded layout) by N/8
+// cooperating threads; all threads of the block must call it (it
+// synchronises the block).
 template <int N, bool INV>
 __device__ __forceinline__ void fft_lds(cd *row, int t, const cd *tw) {
   int n = N, s = 1;

@@ -30,6 +30,7 @@ template <int NX>
 __global__ __launch_bounds__(1024) void k_fft_x_fwd(const cd *__restrict__ J, cd *__restrict__ work, int ny,
                                                     const cd *__restrict__ tw) {
   constexpr int YT = XTile<NX>::YT;
+  constexpr int S = LdsRow<NX>::S;
   extern __shared__ cd lds[];
   const int c = blockIdx.y;
   const int y0 = blockIdx.x * YT;
@@ -38,14 +39,14 @@ __global__ __launch_bounds__(1024) void k_fft_x_fwd(const cd *__restrict__ J, cd
   // coalesced load of YT rows
   for (int e = tid; e < YT * NX; e += YT * TPR) {
     const int r = e / NX, x = e % NX;
-    lds[r * NX + x] = J[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)];
+    lds[r * S + lpad(x)] = J[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)];
   }
   __syncthreads();
-  fft_lds<NX, false>(lds + (tid / TPR) * NX, tid % TPR, tw);
+  fft_lds<NX, false>(lds + (tid / TPR) * S, tid % TPR, tw);
   // transposed store: consecutive threads -> consecutive y
   for (int e = tid; e < YT * NX; e += YT * TPR) {
     const int r = e % YT, x = e / YT;
-    work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)] = lds[r * NX + x];
+    work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)] = lds[r * S + lpad(x)];
   }
 }
 
@@ -54,6 +55,7 @@ template <int NX>
 __global__ __launch_bounds__(1024) void k_fft_x_inv(const cd *__restrict__ work, cd *__restrict__ E, int ny,
                                                     const cd *__restrict__ tw) {
   constexpr int YT = XTile<NX>::YT;
+  constexpr int S = LdsRow<NX>::S;
   extern __shared__ cd lds[];
   const int c = blockIdx.y;
   const int y0 = blockIdx.x * YT;
@@ -61,13 +63,13 @@ __global__ __launch_bounds__(1024) void k_fft_x_inv(const cd *__restrict__ work,
   constexpr int TPR = NX / 8;
   for (int e = tid; e < YT * NX; e += YT * TPR) {
     const int r = e % YT, x = e / YT;
-    lds[r * NX + x] = work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)];
+    lds[r * S + lpad(x)] = work[(y0 + r) + (int64_t)ny * (x + (int64_t)NX * c)];
   }
   __syncthreads();
-  fft_lds<NX, true>(lds + (tid / TPR) * NX, tid % TPR, tw);
+  fft_lds<NX, true>(lds + (tid / TPR) * S, tid % TPR, tw);
   for (int e = tid; e < YT * NX; e += YT * TPR) {
     const int r = e / NX, x = e % NX;
-    E[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)] = lds[r * NX + x];
+    E[x + (int64_t)NX * ((y0 + r) + (int64_t)ny * c)] = lds[r * S + lpad(x)];
   }
 }
 
@@ -80,7 +82,8 @@ __global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const
                                                     const cd *__restrict__ tw, double scale) {
   constexpr int XT = YTile<NY>::XT;
   constexpr int TPR = NY / 8;
-  extern __shared__ cd lds[];  // [3*XT][NY]
+  constexpr int S = LdsRow<NY>::S;
+  extern __shared__ cd lds[];  // [3*XT][S]
   const int x0 = blockIdx.x * XT;
   const int tid = threadIdx.x;
   const int nthr = 3 * XT * TPR;
@@ -88,14 +91,15 @@ __global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const
   for (int e = tid; e < 3 * XT * NY; e += nthr) {
     const int col = e / NY, y = e % NY;
     const int c = col / XT, xi = col % XT;
-    lds[col * NY + y] = work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)];
+    lds[col * S + lpad(y)] = work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)];
   }
   __syncthreads();
-  fft_lds<NY, false>(lds + (tid / TPR) * NY, tid % TPR, tw);
+  fft_lds<NY, false>(lds + (tid / TPR) * S, tid % TPR, tw);
   for (int e = tid; e < XT * NY; e += nthr) {
     const int xi = e / NY, y = e % NY;
     const int64_t node = y + (int64_t)NY * (x0 + xi);
-    const cd j0 = lds[(0 * XT + xi) * NY + y], j1 = lds[(1 * XT + xi) * NY + y], j2 = lds[(2 * XT + xi) * NY + y];
+    const int yp = lpad(y);
+    const cd j0 = lds[(0 * XT + xi) * S + yp], j1 = lds[(1 * XT + xi) * S + yp], j2 = lds[(2 * XT + xi) * S + yp];
     cd out[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -105,14 +109,14 @@ __global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const
       out[c] = rmul(scale, acc);
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * NY + y] = out[c];
+    for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * S + yp] = out[c];
   }
   __syncthreads();
-  fft_lds<NY, true>(lds + (tid / TPR) * NY, tid % TPR, tw);
+  fft_lds<NY, true>(lds + (tid / TPR) * S, tid % TPR, tw);
   for (int e = tid; e < 3 * XT * NY; e += nthr) {
     const int col = e / NY, y = e % NY;
     const int c = col / XT, xi = col % XT;
-    work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)] = lds[col * NY + y];
+    work[y + (int64_t)NY * ((x0 + xi) + (int64_t)nx * c)] = lds[col * S + lpad(y)];
   }
 }
 
@@ -120,7 +124,7 @@ template <int N>
 static hipError_t launch_x(bool inv, const cd *in, cd *out, int ny, const cd *tw, hipStream_t st) {
   constexpr int YT = XTile<N>::YT;
   const dim3 grid(ny / YT, 3), block(YT * (N / 8));
-  const size_t sh = (size_t)YT * N * sizeof(cd);
+  const size_t sh = (size_t)YT * LdsRow<N>::S * sizeof(cd);
   if (inv) hipLaunchKernelGGL(k_fft_x_inv<N>, grid, block, sh, st, in, out, ny, tw);
   else hipLaunchKernelGGL(k_fft_x_fwd<N>, grid, block, sh, st, in, out, ny, tw);
   return hipGetLastError();
@@ -129,7 +133,7 @@ template <int N>
 static hipError_t launch_y(cd *work, const cd *g, int nx, const cd *tw, double scale, hipStream_t st) {
   constexpr int XT = YTile<N>::XT;
   const dim3 grid(nx / XT), block(3 * XT * (N / 8));
-  const size_t sh = (size_t)3 * XT * N * sizeof(cd);
+  const size_t sh = (size_t)3 * XT * LdsRow<N>::S * sizeof(cd);
   hipLaunchKernelGGL(k_fft_y_mul<N>, grid, block, sh, st, work, g, nx, tw, scale);
   return hipGetLastError();
 }

@@ -168,15 +168,16 @@ __global__ __launch_bounds__(1024) void k_nu_rows(const cd *__restrict__ grid, c
   constexpr int TPR = NF / 8;
   extern __shared__ cd lds[];
   const int v0 = blockIdx.x * YT, tid = threadIdx.x;
-  for (int e = tid; e < YT * NF; e += YT * TPR) lds[e] = grid[(int64_t)v0 * NF + e];
+  constexpr int S = LdsRow<NF>::S;
+  for (int e = tid; e < YT * NF; e += YT * TPR) lds[(e / NF) * S + lpad(e % NF)] = grid[(int64_t)v0 * NF + e];
   __syncthreads();
-  fft_lds<NF, true>(lds + (tid / TPR) * NF, tid % TPR, tw);
+  fft_lds<NF, true>(lds + (tid / TPR) * S, tid % TPR, tw);
   const int h1 = N1 / 2;
   for (int e = tid; e < YT * N1; e += YT * TPR) {
     const int r = e % YT, kk = e / YT;      // kk = k1 + N1/2
     const int k1 = kk - h1;
     const int idx = k1 < 0 ? k1 + NF : k1;
-    T1[(int64_t)kk * NF + v0 + r] = lds[r * NF + idx];
+    T1[(int64_t)kk * NF + v0 + r] = lds[r * S + lpad(idx)];
   }
 }
 
@@ -191,12 +192,13 @@ __global__ __launch_bounds__(1024) void k_nu_cols(const cd *__restrict__ T1, cd 
   extern __shared__ cd lds[];
   const int r0 = blockIdx.x * YT, tid = threadIdx.x;
   const int nrow = N1 - r0 < YT ? N1 - r0 : YT;
+  constexpr int S = LdsRow<NF>::S;
   for (int e = tid; e < YT * NF; e += YT * TPR) {
     const int r = e / NF;
-    lds[e] = r < nrow ? T1[(int64_t)r0 * NF + e] : cd{0.0, 0.0};
+    lds[r * S + lpad(e % NF)] = r < nrow ? T1[(int64_t)r0 * NF + e] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<NF, true>(lds + (tid / TPR) * NF, tid % TPR, tw);
+  fft_lds<NF, true>(lds + (tid / TPR) * S, tid % TPR, tw);
   const int h2 = N2 / 2;
   for (int e = tid; e < YT * N2; e += YT * TPR) {
     const int r = e % YT, kk2 = e / YT;
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(1024) void k_nu_cols(const cd *__restrict__ T1, cd 
     const int k2 = kk2 - h2;
     const int idx = k2 < 0 ? k2 + NF : k2;
     const double s = d1[r0 + r] * d2[kk2];
-    f[(int64_t)kk2 * N1 + r0 + r] = rmul(s, lds[r * NF + idx]);
+    f[(int64_t)kk2 * N1 + r0 + r] = rmul(s, lds[r * S + lpad(idx)]);
   }
 }
 
@@ -219,7 +221,7 @@ template <int NF>
 static hipError_t run_ffts(const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
                            const cd *tw, hipStream_t s) {
   constexpr int YT = XTile<NF>::YT;
-  const size_t sh = sizeof(cd) * YT * NF;
+  const size_t sh = sizeof(cd) * YT * LdsRow<NF>::S;
   hipLaunchKernelGGL(k_nu_rows<NF>, dim3(NF / YT), dim3(YT * NF / 8), sh, s, grid, T1, N1, tw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -151,15 +151,16 @@ __global__ __launch_bounds__(1024) void k_opx_fwd(const cd *__restrict__ in, int
   const cd *src = in + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
   cd *dst = out + (int64_t)t * out_plane;
   const int y0 = blockIdx.x * YT;
+  constexpr int S = LdsRow<PX>::S;
   for (int e = tid; e < YT * PX; e += YT * TPR) {
     const int r = e / PX, x = e % PX, y = y0 + r;
-    lds[e] = (x < nx_in && y < ny_rows) ? src[x + (int64_t)in_row * y] : cd{0.0, 0.0};
+    lds[r * S + lpad(x)] = (x < nx_in && y < ny_rows) ? src[x + (int64_t)in_row * y] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PX, false>(lds + (tid / TPR) * PX, tid % TPR, tw);
+  fft_lds<PX, false>(lds + (tid / TPR) * S, tid % TPR, tw);
   for (int e = tid; e < YT * PX; e += YT * TPR) {
     const int r = e % YT, x = e / YT, y = y0 + r;
-    if (y < ny_rows) dst[y + (int64_t)py * x] = lds[r * PX + x];
+    if (y < ny_rows) dst[y + (int64_t)py * x] = lds[r * S + lpad(x)];
   }
 }
 
@@ -176,15 +177,16 @@ __global__ __launch_bounds__(1024) void k_opx_inv(const cd *__restrict__ in, int
   const cd *src = in + (int64_t)t * in_plane;
   cd *dst = out + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
   const int y0 = blockIdx.x * YT;
+  constexpr int S = LdsRow<PX>::S;
   for (int e = tid; e < YT * PX; e += YT * TPR) {
     const int r = e % YT, x = e / YT, y = y0 + r;
-    lds[r * PX + x] = y < ny_rows ? src[y + (int64_t)py * x] : cd{0.0, 0.0};
+    lds[r * S + lpad(x)] = y < ny_rows ? src[y + (int64_t)py * x] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PX, true>(lds + (tid / TPR) * PX, tid % TPR, tw);
+  fft_lds<PX, true>(lds + (tid / TPR) * S, tid % TPR, tw);
   for (int e = tid; e < YT * PX; e += YT * TPR) {
     const int r = e / PX, x = e % PX, y = y0 + r;
-    if (x < nx_out && y < ny_rows) dst[x + (int64_t)out_row * y] = lds[e];
+    if (x < nx_out && y < ny_rows) dst[x + (int64_t)out_row * y] = lds[r * S + lpad(x)];
   }
 }
 
@@ -205,17 +207,18 @@ __global__ __launch_bounds__(256) void k_opy(cd *__restrict__ work, int px, int6
   const int t = blockIdx.y, tid = threadIdx.x;
   cd *base = work + (int64_t)t * plane;
   const int x0 = blockIdx.x * XT;
+  constexpr int S = LdsRow<PY>::S;
   for (int e = tid; e < XT * PY; e += XT * TPC) {
     const int col = e / PY, y = e % PY, x = x0 + col;
-    lds[e] = (x < px && y < ny_in) ? base[y + (int64_t)PY * x] : cd{0.0, 0.0};
+    lds[col * S + lpad(y)] = (x < px && y < ny_in) ? base[y + (int64_t)PY * x] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PY, INV>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  fft_lds<PY, INV>(lds + (tid / TPC) * S, tid % TPC, tw);
   for (int e = tid; e < XT * PY; e += XT * TPC) {
     const int col = e / PY, y = e % PY, x = x0 + col;
     if (x < px && y < ny_out) {
       const double s = (sign && ((x + y) & 1)) ? -scale : scale;
-      base[y + (int64_t)PY * x] = rmul(s, lds[e]);
+      base[y + (int64_t)PY * x] = rmul(s, lds[col * S + lpad(y)]);
     }
   }
 }
@@ -287,20 +290,22 @@ __global__ __launch_bounds__(1024) void k_opy_mix1(const cd *__restrict__ in, cd
                                                    const cd *__restrict__ tw) {
   constexpr int XT = OpYM<PY>::XT;
   constexpr int TPC = PY / 8;
-  extern __shared__ cd lds[];  // [3 * XT][PY]
+  constexpr int S = LdsRow<PY>::S;
+  extern __shared__ cd lds[];  // [3 * XT][S]
   const int x0 = blockIdx.x * XT, tid = threadIdx.x;
   const int nthr = 3 * XT * TPC;
   for (int e = tid; e < 3 * XT * PY; e += nthr) {
     const int col = e / PY, y = e % PY;
     const int c = col / XT, xi = col % XT;
-    lds[e] = y < ny_in ? in[c * nn + y + (int64_t)PY * (x0 + xi)] : cd{0.0, 0.0};
+    lds[col * S + lpad(y)] = y < ny_in ? in[c * nn + y + (int64_t)PY * (x0 + xi)] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PY, false>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  fft_lds<PY, false>(lds + (tid / TPC) * S, tid % TPC, tw);
   for (int e = tid; e < XT * PY; e += nthr) {
     const int xi = e / PY, y = e % PY;
     const int64_t node = y + (int64_t)PY * (x0 + xi);
-    const cd j0 = lds[(0 * XT + xi) * PY + y], j1 = lds[(1 * XT + xi) * PY + y], j2 = lds[(2 * XT + xi) * PY + y];
+    const int yp = lpad(y);
+    const cd j0 = lds[(0 * XT + xi) * S + yp], j1 = lds[(1 * XT + xi) * S + yp], j2 = lds[(2 * XT + xi) * S + yp];
     cd o3[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -309,14 +314,14 @@ __global__ __launch_bounds__(1024) void k_opy_mix1(const cd *__restrict__ in, cd
       o3[r] = fmac(a, K[(r + 6) * nn + node], j2);
     }
 #pragma unroll
-    for (int r = 0; r < 3; ++r) lds[(r * XT + xi) * PY + y] = o3[r];
+    for (int r = 0; r < 3; ++r) lds[(r * XT + xi) * S + yp] = o3[r];
   }
   __syncthreads();
-  fft_lds<PY, true>(lds + (tid / TPC) * PY, tid % TPC, tw);
+  fft_lds<PY, true>(lds + (tid / TPC) * S, tid % TPC, tw);
   for (int e = tid; e < 3 * XT * PY; e += nthr) {
     const int col = e / PY, y = e % PY;
     const int c = col / XT, xi = col % XT;
-    if (y < ny_out) out[c * nn + y + (int64_t)PY * (x0 + xi)] = lds[e];
+    if (y < ny_out) out[c * nn + y + (int64_t)PY * (x0 + xi)] = lds[col * S + lpad(y)];
   }
 }
 
@@ -369,7 +374,7 @@ static hipError_t opx(bool inv, const cd *in, int pdiv, int64_t pa, int64_t pb, 
                       cd *out, int py, int64_t plane, int planes, const cd *tw, hipStream_t s) {
   constexpr int YT = OpX<PX>::YT;
   const dim3 grid((ny_rows + YT - 1) / YT, planes), block(YT * (PX / 8));
-  const size_t sh = sizeof(cd) * YT * PX;
+  const size_t sh = sizeof(cd) * YT * LdsRow<PX>::S;
   if (inv)
     hipLaunchKernelGGL(k_opx_inv<PX>, grid, block, sh, s, in, py, plane, out, pdiv, pa, pb, row, nx, ny_rows, tw);
   else
@@ -395,7 +400,7 @@ static hipError_t opy(bool inv, cd *work, int px, int64_t plane, int planes, int
                       int sign, const cd *tw, hipStream_t s) {
   constexpr int XT = OpY<PY>::XT, TPC = OpY<PY>::TPC;
   const dim3 grid((px + XT - 1) / XT, planes), block(XT * TPC);
-  const size_t sh = sizeof(cd) * XT * PY;
+  const size_t sh = sizeof(cd) * XT * LdsRow<PY>::S;
   if (inv) hipLaunchKernelGGL((k_opy<PY, true>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
   else hipLaunchKernelGGL((k_opy<PY, false>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
   return hipGetLastError();
@@ -421,7 +426,7 @@ static hipError_t opy_mix1(const cd *in, cd *out, int px, const cd *K, int ny_in
   if (px % XT) return hipErrorInvalidValue;
   const int64_t nn = (int64_t)px * PY;
   const dim3 grid(px / XT), block(3 * XT * (PY / 8));
-  const size_t sh = sizeof(cd) * 3 * XT * PY;
+  const size_t sh = sizeof(cd) * 3 * XT * LdsRow<PY>::S;
   hipLaunchKernelGGL(k_opy_mix1<PY>, grid, block, sh, s, in, out, nn, K, ny_in, ny_out, tw);
   return hipGetLastError();
 }
