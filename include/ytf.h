@@ -136,6 +136,17 @@ int ytf_green_nodes(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double 
                     const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
                     const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
                     int terms);
+/* The same for box sources of thickness dz_box centred at each zsrc
+ * (closed_form_G's Δz_box): the seed factor D in face-difference form
+ * (_boxphase, mod_ClosedFormField.jl:86-91), the exterior direct term
+ * (|z_obs - zpos| >= dz_box/2), inside the box's own slab the face ramps
+ * (interior_slab_at, :815-843) and, if ez_local, the local Ez
+ * -p_z / (eps_zz dz_box) (ez_local_density, :861-862).  dz_box = 0 is
+ * ytf_green_nodes.  The box must fit in its layer (_check_box_in_layer). */
+int ytf_green_nodes_box(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                        const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                        const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                        int terms, double dz_box, int ez_local);
 
 /* Spectral convolution on the uniform DFT grid (the "G(kx,ky) F{J}" route of
  * mod_UniformSource.jl:229-248):
@@ -304,6 +315,8 @@ typedef struct ytf_operator_options {
   /* 1: z grids must be uniform (build_layer_operator); 0: any depths, dense
    * blocks only (closed_form_field / precompute_ghats!, mod_ClosedFormAssembly.jl). */
   int32_t uniform_z;
+  double dz_box;       /* box sources of this thickness (Δz_box; 0: points)  */
+  int32_t ez_local;    /* local Ez inside a box's own slab          (true)   */
 } ytf_operator_options;
 
 /* Block kinds (ZBlock.kind, mod_GreenOperator.jl:85-96). */

@@ -176,16 +176,22 @@ def ghat_from_cache(cache, obs_layer, zsrc, zobs, nthreads=0):
 TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}
 
 
-def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0, terms="all"):
+def ghat_batch(st, omega, kx, ky, src, obs_layer, zsrc, zobs, nthreads=0, terms="all", dz_box=0.0, ez_local=True):
+    """Per-node spectral dyadic (n, npairs, c, q).  dz_box > 0: a box source of
+    that thickness centred at each zsrc (_boxphase, interior_slab_at,
+    ez_local_density, mod_ClosedFormField.jl:86-98, 815-862)."""
     kx = np.ascontiguousarray(kx, np.float64)
     ky = np.ascontiguousarray(ky, np.float64)
     zsrc = np.ascontiguousarray(np.atleast_1d(zsrc), np.float64)
     zobs = np.ascontiguousarray(np.atleast_1d(zobs), np.float64)
     nn, npairs = len(kx), len(zsrc)
     g = np.zeros(nn * npairs * 9, np.complex128)
-    lib().ora_ghat_batch(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), ct.c_int(src),
-                         ct.c_int(obs_layer), ct.c_int(npairs), _p(zsrc), _p(zobs), _p(g.view(np.float64)),
-                         ct.c_int(nthreads), ct.c_int(TERMS[terms]))
+    rc = lib().ora_ghat_batch_box(*st.args(), ct.c_double(omega), _p(kx), _p(ky), ct.c_int64(nn), ct.c_int(src),
+                                  ct.c_int(obs_layer), ct.c_int(npairs), _p(zsrc), _p(zobs), _p(g.view(np.float64)),
+                                  ct.c_int(nthreads), ct.c_int(TERMS[terms]), ct.c_double(dz_box),
+                                  ct.c_int(1 if ez_local else 0))
+    if rc != 0:
+        raise ValueError("ghat_batch: the box does not fit inside its layer")
     return g.reshape(nn, npairs, 3, 3).transpose(0, 1, 3, 2)
 
 

@@ -1055,6 +1055,35 @@ template <class R> inline cx<R> phase(cx<R> kz, double z) {
 }
 template <class R> inline cx<R> finite_or_zero(cx<R> z) { return isfin(z) ? z : C<R>(0.0); }
 
+// _boxphase (mod_ClosedFormField.jl:86-91): the box-averaged exponential in
+// face-difference form, [e^{iQ(l+dz/2)} - e^{iQ(l-dz/2)}] / (iQ dz), with the
+// 4th-order Taylor guard for |Q dz| < 1e-3; dz = 0 is the clamped phase.
+template <class R> inline cx<R> boxphase(cx<R> Q, double l, double dz) {
+  if (dz == 0.0) return phase(Q, l);
+  const cx<R> y = rmul(R(dz), Q);
+  if (val(cabs(y)) < 1e-3) {
+    const cx<R> y2 = y * y;
+    return phase(Q, l) * (C<R>(1.0) - rmul(R(1.0 / 24.0), y2) + rmul(R(1.0 / 1920.0), y2 * y2));
+  }
+  return (phase(Q, l + dz / 2) - phase(Q, l - dz / 2)) / times_i(y);
+}
+// Julia's expm1(::Complex): expm1(x) - 2 e^x sin^2(t/2) + i e^x sin t
+template <class R> inline cx<R> cexpm1(cx<R> z) {
+  const double x = val(z.re), t = val(z.im);
+  const double em1 = std::expm1(x);
+  if (t == 0.0) return C<R>(em1, t);
+  const double er = em1 + 1.0, sh = std::sin(0.5 * t);
+  return C<R>(em1 - 2.0 * er * sh * sh, er * std::sin(t));
+}
+// _ramp (:790-796): expm1(iQa) / (iQ dz), regular as Q -> 0
+template <class R> inline cx<R> ramp(cx<R> Q, double a, double dz) {
+  if (val(cabs(Q)) * a < 1e-12) return C<R>(a / dz);
+  const cx<R> y = times_i(rmul(R(a), Q));
+  const cx<R> den = times_i(rmul(R(dz), Q));
+  if (val(y.re) < -700) return C<R>(-1.0) / den;
+  return cexpm1(y) / den;
+}
+
 // Spectral dyadic of one node for one (zsrc, zobs) pair: the strength of
 // _strengths_row! (:616-659) divided by its node weight,
 //   g[c,q] = sum_m E_c,f fz(Af_r Phi_f + Afd) + E_c,b fz(Ab_r Phi_b + Abd),
@@ -1063,7 +1092,8 @@ template <class R> inline cx<R> finite_or_zero(cx<R> z) { return isfin(z) ? z : 
 // terms (_term_amplitudes, :143-160): 0 :all, 1 :direct, 2 :scattered,
 // 3 :toeplitz (direct + f->f, b->b reflected), 4 :hankel (b->f, f->b).
 template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_layer, int src_layer0, double d_src,
-                                  double d_obs, double zsrc_local, double zobs, cx<R> *g, int terms = 0) {
+                                  double d_obs, double zsrc_local, double zobs, cx<R> *g, int terms = 0,
+                                  double dz = 0.0, bool ez_local = true, cx<R> ezz_src = cx<R>()) {
   for (int i = 0; i < 9; ++i) g[i] = C<R>(0.0);
   double zpos = src_layer0 == 0 ? d_src - zsrc_local : zsrc_local;
   cx<R> Af[2][3], Ab[2][3], Afd[2][3], Abd[2][3];
@@ -1072,8 +1102,13 @@ template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_l
   if (nd.valid && nd.Tvalid[obs_idx]) {
     cx<R> Pf[2], Pb[2];
     for (int m = 0; m < 2; ++m) {
-      Pf[m] = cexp(times_i(nd.Qf[m]) * Cr(R(d_src - zpos)));
-      Pb[m] = cexp(-times_i(nd.Qb[m]) * Cr(R(zpos)));
+      if (dz > 0.0) {  // _seed_phases of a box (:106-111)
+        Pf[m] = boxphase(nd.Qf[m], d_src - zpos, dz);
+        Pb[m] = boxphase(nd.Qb[m], -zpos, dz);
+      } else {
+        Pf[m] = cexp(times_i(nd.Qf[m]) * Cr(R(d_src - zpos)));
+        Pb[m] = cexp(-times_i(nd.Qb[m]) * Cr(R(zpos)));
+      }
     }
     const Mat<R, 4, 4> &T = nd.T[obs_idx];
     // seed columns feeding the forward (rows 0-1) and backward (rows 2-3)
@@ -1102,16 +1137,32 @@ template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_l
       Af[0][q] = v[0]; Af[1][q] = v[1]; Ab[0][q] = v[2]; Ab[1][q] = v[3];
     }
   }
+  bool inside = false;
   if (same_layer && nd.valid && (terms == 0 || terms == 1 || terms == 3)) {
     double zeta = zobs - zpos;
-    bool fwd = zeta > 0, bwd = zeta < 0;
-    for (int m = 0; m < 2; ++m) {
-      if (fwd) {
-        cx<R> e = phase(nd.Qf[m], zeta);
-        for (int q = 0; q < 3; ++q) Afd[m][q] = nd.sf(m, q) * e;
-      } else if (bwd) {
-        cx<R> e = phase(nd.Qb[m], zeta);
-        for (int q = 0; q < 3; ++q) Abd[m][q] = nd.sb(m, q) * e;
+    if (dz > 0.0 && std::fabs(zeta) < dz / 2) {
+      // interior_slab_at (:815-843): face ramps, already at z_obs
+      inside = true;
+      const double a = std::min(std::max(zobs - (zpos - dz / 2), 0.0), dz);
+      const double b = std::min(std::max((zpos + dz / 2) - zobs, 0.0), dz);
+      for (int m = 0; m < 2; ++m) {
+        const cx<R> rf = ramp(nd.Qf[m], a, dz), rb = ramp(-nd.Qb[m], b, dz);
+        for (int q = 0; q < 3; ++q) {
+          Afd[m][q] = nd.sf(m, q) * rf;
+          Abd[m][q] = nd.sb(m, q) * rb;
+        }
+      }
+    } else {
+      // _direct_branch (:208-214): point -> strict sign of zeta, box -> |zeta| >= dz/2
+      bool fwd = dz > 0.0 ? zeta >= dz / 2 : zeta > 0, bwd = dz > 0.0 ? zeta <= -dz / 2 : zeta < 0;
+      for (int m = 0; m < 2; ++m) {
+        if (fwd) {
+          cx<R> e = boxphase(nd.Qf[m], zeta, dz);
+          for (int q = 0; q < 3; ++q) Afd[m][q] = nd.sf(m, q) * e;
+        } else if (bwd) {
+          cx<R> e = boxphase(nd.Qb[m], zeta, dz);
+          for (int q = 0; q < 3; ++q) Abd[m][q] = nd.sb(m, q) * e;
+        }
       }
     }
   }
@@ -1126,6 +1177,9 @@ template <class R> void node_ghat(const NodeOut<R> &nd, int obs_idx, bool same_l
       g[2 + 3 * q] = g[2 + 3 * q] + (B[4 + m] * af + B[10 + m] * ab);
     }
   }
+  // the local Ez of a box inside its own slab (ez_local_density, :861-862):
+  // -p_z / (eps_zz dz), on the zz entry of valid nodes
+  if (inside && ez_local) g[8] = g[8] - C<R>(1.0) / (rmul(R(dz), ezz_src));
 }
 
 // Modal_amp -- mod_FieldSolver.jl:356-363, with one ModalSeed (z, sif, sib)
@@ -1333,6 +1387,40 @@ int ora_ghat_from_cache(int n, const double *d, int src, const int *obs, int nob
 }
 
 // Fused node solve + spectral dyadic, as timed for the CPU baseline.
+// Box source of thickness dz centred at each zsrc (dz = 0: point source);
+// ez_local adds the local Ez inside the box's own slab.  The box must fit in
+// its layer (_check_box_in_layer, mod_ClosedFormField.jl:93-98): returns 1
+// otherwise.
+int ora_ghat_batch_box(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
+                       double omega, const double *kx, const double *ky, int64_t nn, int src, int obs_layer,
+                       int npairs, const double *zsrc, const double *zobs, double *g, int nthreads, int terms,
+                       double dz, int ez_local) {
+  auto st = make_stack<double>(n, eps, mu, d, ek, mk);
+  const double ds = d[src - 1], tol = std::sqrt(2.220446049250313e-16) * std::max(ds, 1.0);
+  if (dz < 0.0) return 1;
+  for (int p = 0; p < npairs && dz > 0.0; ++p) {
+    const double zpos = src == 1 ? ds - zsrc[p] : zsrc[p];
+    if (!(zpos - dz / 2 >= -tol && zpos + dz / 2 <= ds + tol)) return 1;
+  }
+  const cx<double> ezz(eps[18 * (src - 1) + 16], eps[18 * (src - 1) + 17]);
+  std::vector<int> ob(1, obs_layer - 1);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t k = 0; k < nn; ++k) {
+    NodeOut<double> nd;
+    node_solve(st, omega, kx[k], ky[k], src - 1, ob, nd);
+    for (int p = 0; p < npairs; ++p) {
+      cx<double> gg[9];
+      node_ghat(nd, 0, obs_layer == src, src - 1, d[src - 1], d[obs_layer - 1], zsrc[p], zobs[p], gg, terms, dz,
+                ez_local != 0, ezz);
+      for (int i = 0; i < 9; ++i) put(g + 2 * (9 * (npairs * k + p) + i), gg[i]);
+    }
+  }
+  return 0;
+}
+
 int ora_ghat_batch(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
                    double omega, const double *kx, const double *ky, int64_t nn, int src, int obs_layer, int npairs,
                    const double *zsrc, const double *zobs, double *g, int nthreads, int terms) {

@@ -183,3 +183,39 @@ def test_term_classes_against_oracle(label):
         green_nodes(stack, W0, kx, ky, 1, 2, [0.1], [0.1], cache=False, terms="direct")
     with pytest.raises(ValueError):
         green_nodes(stack, W0, kx, ky, 1, 1, [0.1], [0.1], cache=False, terms="reflected")
+
+
+@pytest.mark.parametrize("label,src,obs,zc,dz,zos", [
+    ("iso 5-layer lossy", 2, 2, 0.35, 0.2, [0.6, 0.1, 0.3, 0.44, 0.25]),  # exterior and interior planes
+    ("iso 5-layer lossy", 1, 1, 0.3, 0.2, [0.2, 0.35, 0.6, 0.05]),         # source above the stack (CASE_TOP)
+    ("gyrotropic 4-layer", 2, 2, 0.3, 0.2, [0.5, 0.25, 0.1]),
+    ("uniaxial 4-layer", 2, 3, 0.4, 0.3, [0.25]),                           # another layer
+    ("biaxial 4-layer", 2, 1, 0.4, 0.3, [0.3]),
+])
+def test_box_sources_against_oracle(label, src, obs, zc, dz, zos):
+    """Box sources (closed_form_G's Δz_box): box seeds, the exterior direct term
+    and, inside the slab, the face ramps and the local Ez, for every term class."""
+    from yellowthinfilms_amd import green_nodes
+    stack = FOUNDATION_STACKS[label]
+    rng = np.random.default_rng(4)
+    kr = np.concatenate([rng.uniform(0, 3 * W0, 96), rng.uniform(3 * W0, 40 * W0, 32)])
+    ph = rng.uniform(0, 2 * np.pi, 128)
+    kx, ky = kr * np.cos(ph), kr * np.sin(ph)
+    st = ostack(stack)
+    classes = ["all", "scattered"] + (["direct", "toeplitz", "hankel"] if obs == src else [])
+    for terms in classes:
+        for ez in (True, False):
+            print(f"case: box {label} src={src} obs={obs} dz={dz} terms={terms} ez_local={ez}", flush=True)
+            _, g = green_nodes(stack, W0, kx, ky, src, obs, [zc] * len(zos), zos, cache=False, terms=terms,
+                               dz_box=dz, ez_local=ez)
+            torch.cuda.synchronize()
+            gg = g.cpu().numpy().reshape(len(zos), 3, 3, -1).transpose(3, 0, 2, 1)
+            og = po.ghat_batch(st, W0, kx, ky, src, obs, [zc] * len(zos), zos, terms=terms, dz_box=dz, ez_local=ez)
+            assert relerr(gg, og) < 1e-10, (terms, ez, relerr(gg, og))
+
+
+def test_box_must_fit_its_layer_gpu():
+    from yellowthinfilms_amd import green_nodes
+    from yellowthinfilms_amd._lib import YtfError
+    with pytest.raises(YtfError, match="does not fit inside its layer"):
+        green_nodes(FOUNDATION_STACKS["iso 5-layer lossy"], W0, [1.0], [0.0], 2, 2, [0.05], [0.3], dz_box=0.2)

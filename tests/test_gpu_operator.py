@@ -318,3 +318,46 @@ def test_toeplitz_skeleton_block():
     for o in zobs:
         print(f"  obs {o}: skeleton vs dense {relerr(Es[o], Ed[o]):.2e}", flush=True)
         assert relerr(Es[o], Ed[o]) < 1e-5
+
+
+def test_box_sources_closed_form_and_operator():
+    """A box source: footprint factor on the node weights, z extent in the node
+    kernel; the 2D closed form and the operator apply against the oracle."""
+    from yellowthinfilms_amd.green_operator import closed_form_G_2d, build_layer_operator, box_lateral_factor
+    from yellowthinfilms_amd.synthesis import CavityTransfer1D, closed_form_G
+    stack = FOUNDATION_STACKS["iso 5-layer lossy"]
+    cache = _cache(stack, 2)
+    st = ostack(stack)
+    box = (0.06, 0.05)
+    dz = 0.2
+    W = cache.W_host * box_lateral_factor(cache.kx_host, cache.ky_host, *box)
+    xs = (np.arange(8) - 4) * 0.06
+    ys = (np.arange(6) - 3) * 0.05
+    zo = [0.3, 0.45, 0.6]  # inside, on a face, outside
+    print("case: box closed_form_G_2d", flush=True)
+    G = closed_form_G_2d(cache, 2, 0.35, zo, xs, ys, box_lateral=box, dz_box=dz)
+    torch.cuda.synchronize()
+    G = G.cpu().numpy().transpose(4, 3, 2, 1, 0)
+    g = po.ghat_batch(st, W0, cache.kx_host, cache.ky_host, 2, 2, [0.35] * 3, zo, dz_box=dz)
+    ref = np.stack([po.lateral_sum(g[:, p], W, cache.kx_host, cache.ky_host, xs, ys) for p in range(3)], axis=2)
+    assert relerr(G, ref) < 1e-10, relerr(G, ref)
+    # the operator with box sources: planes inside and outside the slabs
+    grid = _grid(8, 8, 0.06, 0.05)
+    zs = [0.2, 0.35, 0.5]
+    zobs = {2: [0.25, 0.4, 0.55], 1: [0.3]}
+    print("case: box operator", flush=True)
+    op = build_layer_operator(cache, grid, zs, zobs, skeleton=False, structure="dense", box_lateral=box, dz_box=0.1)
+    P = _moments(8, 8, 3, seed=9, n=5)
+    E = _run(op, P)
+    offx = (np.arange(15) - 7) * grid["dx"]
+    offy = (np.arange(15) - 7) * grid["dy"]
+    for o, z in zobs.items():
+        gb = [po.ghat_batch(st, W0, cache.kx_host, cache.ky_host, 2, o, [zz] * len(z), z, dz_box=0.1) for zz in zs]
+        Go = np.stack([np.stack([po.lateral_sum(gk[:, p], W, cache.kx_host, cache.ky_host, offx, offy)
+                                 for p in range(len(z))], axis=2) for gk in gb], axis=2)
+        ref = po.linear_convolution(Go, P)
+        assert relerr(E[o], ref) < 1e-10, (o, relerr(E[o], ref))
+    # the 1D path has no interior slab term
+    c1 = CavityTransfer1D(stack, W0, np.linspace(0.1, 10, 50), np.full(50, 0.2), 2)
+    with pytest.raises(ValueError, match="inside the box's own z-slab"):
+        closed_form_G(c1, 2, 0.35, [0.3], [0.1], [0.0], dz_box=0.2)

@@ -570,3 +570,81 @@ def test_zskeleton_of_a_family_of_known_rank():
     a = po.zskeleton(z, [(dense_Q, 1.0, 0.0)], tol=1e-9)
     b = po.zskeleton(z, [(dense_Q, 1.0, 0.0)], tol=1e-4)
     assert len(b["idx"]) < len(a["idx"]) and b["err"] <= 1e-4
+
+
+# ---------------------------------------------------------------------------
+# Box sources (mod_ClosedFormField.jl:86-98 _boxphase, 815-862 interior slab
+# and local Ez)
+# ---------------------------------------------------------------------------
+def _slab_average(st, kx, ky, src, obs, zc, dz, zo, split=None, n=24, terms="all"):
+    """(1/dz) times the integral of the point-source g~ over source depths in
+    [zc - dz/2, zc + dz/2] (Gauss-Legendre, split at `split`)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    edges = [zc - dz / 2, zc + dz / 2] if split is None else [zc - dz / 2, split, zc + dz / 2]
+    zs, ws = [], []
+    for a, b in zip(edges[:-1], edges[1:]):
+        zs += list((b - a) / 2 * x + (a + b) / 2)
+        ws += list((b - a) / 2 * w)
+    g = po.ghat_batch(st, W0, kx, ky, src, obs, zs, [zo] * len(zs), terms=terms)
+    return np.tensordot(g, np.array(ws), axes=([1], [0])) / dz
+
+
+@pytest.mark.parametrize("label,src,obs,zc,dz,zo", [
+    ("iso 5-layer lossy", 2, 2, 0.35, 0.2, 0.6),    # same layer, exterior (forward)
+    ("iso 5-layer lossy", 2, 2, 0.35, 0.2, 0.1),    # same layer, exterior (backward)
+    ("uniaxial 4-layer", 2, 3, 0.4, 0.3, 0.25),     # another layer
+    ("gyrotropic 4-layer", 2, 2, 0.3, 0.2, 0.5),
+    ("iso 5-layer lossy", 1, 1, 0.3, 0.2, 0.2),     # source layer 1 (reversed depth)
+])
+def test_box_equals_the_slab_average_of_point_sources_outside(label, src, obs, zc, dz, zo):
+    st = ostack(FOUNDATION_STACKS[label])
+    kx, ky = np.array([0.3, 2.2, 5.5, 9.0]), np.array([0.1, -0.7, 1.5, 4.0])
+    box = po.ghat_batch(st, W0, kx, ky, src, obs, [zc], [zo], dz_box=dz)[:, 0]
+    avg = _slab_average(st, kx, ky, src, obs, zc, dz, zo)
+    assert relerr(box, avg) < 1e-12, relerr(box, avg)
+
+
+@pytest.mark.parametrize("zo", [0.3, 0.36, 0.42])
+def test_box_interior_ramps_equal_the_split_slab_average(zo):
+    """Inside the slab the direct term is the face ramps: the average of the
+    point sources below (forward) and above (backward) the observation plane."""
+    st = ostack(FOUNDATION_STACKS["iso 5-layer lossy"])
+    kx, ky = np.array([0.3, 2.2, 5.5, 9.0]), np.array([0.1, -0.7, 1.5, 4.0])
+    zc, dz = 0.35, 0.2
+    box = po.ghat_batch(st, W0, kx, ky, 2, 2, [zc], [zo], dz_box=dz, ez_local=False)[:, 0]
+    avg = _slab_average(st, kx, ky, 2, 2, zc, dz, zo, split=zo)
+    assert relerr(box, avg) < 1e-12, relerr(box, avg)
+    # the local Ez is -p_z / (eps_zz dz) on the zz entry only
+    with_ez = po.ghat_batch(st, W0, kx, ky, 2, 2, [zc], [zo], dz_box=dz)[:, 0]
+    d = with_ez - box
+    ezz = FOUNDATION_STACKS["iso 5-layer lossy"][1].eps.eps[2, 2]
+    assert np.allclose(d[:, 2, 2], -1 / (ezz * dz), rtol=1e-14)
+    d[:, 2, 2] = 0
+    assert np.max(np.abs(d)) == 0
+
+
+def test_box_direct_term_is_continuous_at_the_faces_and_local_ez_cancels_the_flat_part():
+    st = ostack(FOUNDATION_STACKS["iso 5-layer lossy"])
+    kx, ky = np.array([0.3, 2.2, 5.5]), np.array([0.1, -0.7, 1.5])
+    zc, dz = 0.35, 0.2
+    for face in (zc + dz / 2, zc - dz / 2):
+        out = po.ghat_batch(st, W0, kx, ky, 2, 2, [zc], [face], dz_box=dz, terms="direct", ez_local=False)[:, 0]
+        eps = 1e-9 * np.sign(zc - face)
+        inn = po.ghat_batch(st, W0, kx, ky, 2, 2, [zc], [face + eps], dz_box=dz, terms="direct",
+                            ez_local=False)[:, 0]
+        assert relerr(inn, out) < 1e-7, relerr(inn, out)
+    # deep in the evanescent range the modal zz part inside the slab tends to
+    # +1/(eps_zz dz): only its sum with the local Ez decays (like 1/k_rho^2)
+    ezz = FOUNDATION_STACKS["iso 5-layer lossy"][1].eps.eps[2, 2]
+    k = np.array([200.0, 400.0, 800.0])
+    modal = po.ghat_batch(st, W0, k, 0 * k, 2, 2, [zc], [zc + 0.01], dz_box=dz, ez_local=False, terms="direct")[:, 0]
+    total = po.ghat_batch(st, W0, k, 0 * k, 2, 2, [zc], [zc + 0.01], dz_box=dz, terms="direct")[:, 0]
+    assert np.allclose(modal[:, 2, 2] * ezz * dz, 1.0, rtol=2e-2)
+    r = np.abs(total[:, 2, 2]) / np.abs(modal[:, 2, 2])
+    assert r[0] < 5e-3 and r[1] < r[0] / 3 and r[2] < r[1] / 3, r
+
+
+def test_box_must_fit_its_layer():
+    st = ostack(FOUNDATION_STACKS["iso 5-layer lossy"])
+    with pytest.raises(ValueError, match="fit"):
+        po.ghat_batch(st, W0, [1.0], [0.0], 2, 2, [0.05], [0.3], dz_box=0.2)

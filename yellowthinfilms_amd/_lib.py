@@ -34,7 +34,8 @@ class YtfGrid(ct.Structure):
 class YtfOperatorOptions(ct.Structure):
     _fields_ = [("structure", ct.c_int32), ("check", ct.c_int32), ("check_tol", ct.c_double),
                 ("skeleton", ct.c_int32), ("skeleton_tol", ct.c_double), ("lateral", ct.c_int32),
-                ("nufft_eps", ct.c_double), ("uniform_z", ct.c_int32)]
+                ("nufft_eps", ct.c_double), ("uniform_z", ct.c_int32), ("dz_box", ct.c_double),
+                ("ez_local", ct.c_int32)]
 
 
 class YtfBlockInfo(ct.Structure):
@@ -66,6 +67,10 @@ _SIGS = {
                                    ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
                                    ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
                                    ct.POINTER(YtfNodeCache), ct.c_int]),
+    "ytf_green_nodes_box": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_void_p,
+                                       ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
+                                       ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
+                                       ct.POINTER(YtfNodeCache), ct.c_int, ct.c_double, ct.c_int]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,

@@ -83,7 +83,17 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_green_nodes: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
   return green_nodes_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n_nodes, source_layer, obs_layer, n_pairs,
-                            zsrc, zobs, ghat_dev, cache, terms);
+                            zsrc, zobs, ghat_dev, cache, terms, 0.0, 1);
+}
+
+int ytf_green_nodes_box(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                        const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                        const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                        int terms, double dz_box, int ez_local) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_green_nodes_box: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return green_nodes_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n_nodes, source_layer, obs_layer, n_pairs,
+                            zsrc, zobs, ghat_dev, cache, terms, dz_box, ez_local);
 }
 
 }  // extern "C"
@@ -91,7 +101,7 @@ int ytf_green_nodes(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
 int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
                        const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
                        const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
-                       int terms) {
+                       int terms, double dz_box, int ez_local) {
   if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_green_nodes: need at least 2 layers (got %d)", n_layers);
   if (source_layer < 1 || source_layer > n_layers - 1)
     return fail(c, YTF_EINVAL,
@@ -117,11 +127,17 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
       return fail(c, YTF_EINVAL, "ytf_green_nodes: layer %d has an unknown material kind", l + 1);
   }
   const double ds = layers[source_layer - 1].d;
+  if (!(dz_box >= 0.0) || !std::isfinite(dz_box)) return fail(c, YTF_EINVAL, "Δz_box = %g must be ≥ 0", dz_box);
   for (int p = 0; p < n_pairs; ++p) {
     const double zp = source_layer == 1 ? ds - zsrc[p] : zsrc[p];
     const double tol = 1.4901161193847656e-08;
     if (!(zp >= -tol && zp <= ds + tol))
       return fail(c, YTF_EINVAL, "amplitude_at: zsrc_local=%g outside [0,%g]", zsrc[p], ds);
+    // _check_box_in_layer (mod_ClosedFormField.jl:93-98)
+    const double bt = tol * std::max(ds, 1.0);
+    if (dz_box > 0.0 && !(zp - dz_box / 2 >= -bt && zp + dz_box / 2 <= ds + bt))
+      return fail(c, YTF_EINVAL, "box [%g, %g] (cavity frame) does not fit inside its layer [0, %g]", zp - dz_box / 2,
+                  zp + dz_box / 2, ds);
   }
   if (n_nodes == 0) return YTF_OK;
   HIPCHK(c, hipSetDevice(c->device));
@@ -154,6 +170,8 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
   P.obs = obs_layer - 1;
   P.npairs = n_pairs;
   P.terms = terms;
+  P.dz = dz_box;
+  P.ez_local = ez_local ? 1 : 0;
   P.zsrc = (const double *)(sm + lb);
   P.zobs = (const double *)(sm + lb + zb);
   P.ghat = (cd *)ghat_dev;

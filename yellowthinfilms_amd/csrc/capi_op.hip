@@ -100,6 +100,8 @@ struct Build {
   double dx, dy;
   int lateral;
   double eps;
+  double dz;     // box thickness (0: point sources)
+  int ez_local;  // local Ez inside a box's slab
   // device: NUFFT points and weights, padded offsets
   double *xf = nullptr, *yf = nullptr;
   cd *wr = nullptr;
@@ -136,7 +138,7 @@ int fill_planes(ytf_ctx *c, const Build &B, int obs_layer, const std::vector<Req
       zo[p] = reqs[r0 + p].zo;
     }
     rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, B.n, B.src, obs_layer, np, zs.data(), zo.data(),
-                            (double *)g, nullptr, terms);
+                            (double *)g, nullptr, terms, B.dz, B.ez_local);
     if (rc) return rc;
     hipError_t e = hipSuccess;
     if (B.lateral == YTF_LATERAL_NUFFT) {
@@ -204,7 +206,7 @@ int check_structure(ytf_ctx *c, const Build &B, const std::vector<double> &zs, c
   rc = upload(c, xb.p, xy, sizeof xy);
   if (rc) return rc;
   rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, B.n, B.src, B.src, 4, zsv, zov, (double *)gb.p,
-                          nullptr, YTF_TERMS_SCATTERED);
+                          nullptr, YTF_TERMS_SCATTERED, B.dz, B.ez_local);
   if (rc) return rc;
   const double *xd = (const double *)xb.p;
   hipError_t e = ytf::launch_lat_direct(B.n, B.kx, B.ky, B.W, 4, (const cd *)gb.p, xd, 3, xd + 3, 2, (cd *)ob.p, 9,
@@ -442,7 +444,7 @@ int side_skeletons(ytf_ctx *c, const Build &B, int obs, const std::vector<double
   nc.Qsrc_dev = (double *)qb.p;
   nc.basis_dev = (double *)((cd *)qb.p + 4 * n);
   rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, n, B.src, obs, 0, nullptr, nullptr, nullptr, &nc,
-                          YTF_TERMS_ALL);
+                          YTF_TERMS_ALL, 0.0, 1);
   if (rc) return rc;
   if (need_src) {
     // source planes at cavity-frame depth zpos: Psi_f = e^{iQf (d - zpos)},
@@ -713,6 +715,8 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
   B.dy = grid->dy;
   B.lateral = opt->lateral;
   B.eps = opt->nufft_eps;
+  B.dz = opt->dz_box;
+  B.ez_local = opt->ez_local;
   auto bail = [&](int rc) {
     hipStreamSynchronize(c->stream);
     for (auto &b : op->blocks) free_block(b);

@@ -165,7 +165,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
                        const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
                        const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
-                       int terms);
+                       int terms, double dz_box, int ez_local);
 // Uses the context's `work` buffer (may reallocate it).
 int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev, int ntrans, const double *c_dev,
                     int N1, int N2, double eps, double *f_dev);

@@ -792,6 +792,35 @@ __device__ __forceinline__ cd phase(cd kz, double z) {
 }
 __device__ __forceinline__ cd fin0(cd z) { return isfin(z) ? z : mk(0.0); }
 
+// Box sources (mod_ClosedFormField.jl:86-91, 790-796).
+// _boxphase: [e^{iQ(l+dz/2)} - e^{iQ(l-dz/2)}] / (iQ dz), 4th-order Taylor
+// guard below |Q dz| = 1e-3.
+__device__ __noinline__ cd boxphase(cd Q, double l, double dz) {
+  const cd y = rmul(dz, Q);
+  if (cabs(y) < 1e-3) {
+    const cd y2 = y * y;
+    return phase(Q, l) * (mk(1.0) - rmul(1.0 / 24.0, y2) + rmul(1.0 / 1920.0, y2 * y2));
+  }
+  return (phase(Q, l + 0.5 * dz) - phase(Q, l - 0.5 * dz)) / times_i(y);
+}
+// _ramp: expm1(iQa) / (iQ dz), with expm1(x + it) = expm1(x) - 2 e^x sin^2(t/2)
+// + i e^x sin t (Julia's complex expm1)
+__device__ __noinline__ cd ramp(cd Q, double a, double dz) {
+  if (cabs(Q) * a < 1e-12) return mk(a / dz);
+  const cd y = times_i(rmul(a, Q));
+  const cd den = times_i(rmul(dz, Q));
+  if (y.r < -700.0) return mk(-1.0) / den;
+  const double em1 = expm1(y.r);
+  cd e;
+  if (y.i == 0.0) {
+    e = cd{em1, 0.0};
+  } else {
+    const double er = em1 + 1.0, sh = sin(0.5 * y.i);
+    e = cd{em1 - 2.0 * er * sh * sh, er * sin(y.i)};
+  }
+  return e / den;
+}
+
 // ---------------------------------------------------------------------------
 // The node kernel
 // ---------------------------------------------------------------------------
@@ -1324,9 +1353,17 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       // term classes (_term_amplitudes, mod_ClosedFormField.jl:143-160): the
       // seeds feeding the forward / backward amplitudes, and the direct term
       const int terms = P.terms;
+      const double dz = P.dz;
       if (Tvalid && terms != TERMS_DIRECT) {
-        const cd Pf[2] = {expi(Qs[0], d_src - zpos), expi(Qs[1], d_src - zpos)};
-        const cd Pb[2] = {expi(-Qs[2], zpos), expi(-Qs[3], zpos)};
+        cd Pf[2] = {expi(Qs[0], d_src - zpos), expi(Qs[1], d_src - zpos)};
+        cd Pb[2] = {expi(-Qs[2], zpos), expi(-Qs[3], zpos)};
+        if (dz > 0.0) {  // box seeds D.Psi (_seed_phases)
+#pragma unroll
+          for (int m = 0; m < 2; ++m) {
+            Pf[m] = boxphase(Qs[m], d_src - zpos, dz);
+            Pb[m] = boxphase(Qs[2 + m], -zpos, dz);
+          }
+        }
         // seeds=:f keeps the forward seed columns (0, 1), seeds=:b the backward
         // (2, 3): weights of the two seed halves in the forward (rows 0-1)
         // and backward (rows 2-3) amplitudes.  The masks are arithmetic 0/1
@@ -1348,19 +1385,39 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
           }
         }
       }
+      bool inside = false;
       if ((CASE == CASE_SAME || CASE == CASE_TOP) && valid &&
           (terms == TERMS_ALL || terms == TERMS_DIRECT || terms == TERMS_TOEPLITZ)) {
         const double zeta = zob - zpos;
+        if (dz > 0.0 && fabs(zeta) < 0.5 * dz) {
+          // inside the box's own slab: the face ramps (interior_slab_at)
+          inside = true;
+          const double a = fmin(fmax(zob - (zpos - 0.5 * dz), 0.0), dz);
+          const double b = fmin(fmax((zpos + 0.5 * dz) - zob, 0.0), dz);
 #pragma unroll
-        for (int m = 0; m < 2; ++m) {
-          if (zeta > 0) {
-            const cd e = phase(Qs[m], zeta);
+          for (int m = 0; m < 2; ++m) {
+            const cd rf = ramp(Qs[m], a, dz), rb = ramp(-Qs[2 + m], b, dz);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) Afd[m + 2 * q] = sif[m + 2 * q] * e;
-          } else if (zeta < 0) {
-            const cd e = phase(Qs[2 + m], zeta);
+            for (int q = 0; q < 3; ++q) {
+              Afd[m + 2 * q] = sif[m + 2 * q] * rf;
+              Abd[m + 2 * q] = -(sib[m + 2 * q] * rb);
+            }
+          }
+        } else {
+          // point: strict sign of zeta; box: |zeta| >= dz/2 (_direct_branch)
+          const bool fwd = dz > 0.0 ? zeta >= 0.5 * dz : zeta > 0;
+          const bool bwd = dz > 0.0 ? zeta <= -0.5 * dz : zeta < 0;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) Abd[m + 2 * q] = -(sib[m + 2 * q] * e);
+          for (int m = 0; m < 2; ++m) {
+            if (fwd) {
+              const cd e = dz > 0.0 ? boxphase(Qs[m], zeta, dz) : phase(Qs[m], zeta);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) Afd[m + 2 * q] = sif[m + 2 * q] * e;
+            } else if (bwd) {
+              const cd e = dz > 0.0 ? boxphase(Qs[2 + m], zeta, dz) : phase(Qs[2 + m], zeta);
+#pragma unroll
+              for (int q = 0; q < 3; ++q) Abd[m + 2 * q] = -(sib[m + 2 * q] * e);
+            }
           }
         }
       }
@@ -1379,6 +1436,8 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
           g[2 + 3 * q] = g[2 + 3 * q] + (basis[4 + m] * af + basis[10 + m] * ab);
         }
       }
+      // the local Ez inside the slab (ez_local_density): -p_z / (eps_zz dz)
+      if (inside && P.ez_local) g[8] = g[8] - mk(1.0) / rmul(dz, Ls.eps[8]);
       cd *gp = P.ghat + (int64_t)pr * 9 * nn;
 #pragma unroll
       for (int i = 0; i < 9; ++i) gp[i * nn + k] = sel(valid, g[i], z0);

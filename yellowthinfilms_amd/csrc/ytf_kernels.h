@@ -36,6 +36,8 @@ struct NodeParams {
   unsigned char *flags;       // [nn] bit0 valid, bit1 Tvalid, bit2 singular solve, bit3 mode solve failed
   bool needs_general;         // some layer needs the general 4x4 eigen-solver
   int terms;                  // TERMS_*: which terms of g~
+  double dz;                  // box thickness (0: point source)
+  int ez_local;               // add the local Ez inside the box's slab
 };
 
 // term classes of the spectral dyadic (mod_ClosedFormField.jl:143-160)

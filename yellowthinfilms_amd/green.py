@@ -83,14 +83,16 @@ class NodeCache:
 
 
 def green_nodes(stack, omega, kx, ky, source_layer, obs_layer, zsrc=(), zobs=(), cache=True, ghat=True,
-                device=0, ghat_out=None, terms="all"):
+                device=0, ghat_out=None, terms="all", dz_box=0.0, ez_local=True):
     """Batched Green k-node solve on the GPU (ytf_green_nodes).
 
     kx, ky: node wavevectors (torch float64 on the device, or array-likes);
     source_layer, obs_layer: 1-based; zsrc/zobs: depth pairs for the spectral
     dyadic; terms: "all", "direct", "scattered", "toeplitz" or "hankel"
-    (closed_form_G's term classes).  Returns (NodeCache or None, ghat
-    (npairs, 9, n) complex128 or None).
+    (closed_form_G's term classes); dz_box > 0: box sources of that thickness
+    centred at zsrc (ytf_green_nodes_box), with the local Ez inside the box's
+    slab if ez_local.  Returns (NodeCache or None, ghat (npairs, 9, n)
+    complex128 or None).
     """
     torch = _torch()
     dev = torch.device("cuda", device)
@@ -112,11 +114,11 @@ def green_nodes(stack, omega, kx, ky, source_layer, obs_layer, zsrc=(), zobs=(),
     ctx.bind_stream()
     layers = layer_structs(stack)
     cs = nc.struct() if nc is not None else None
-    rc = _lib.lib().ytf_green_nodes(
+    rc = _lib.lib().ytf_green_nodes_box(
         ctx.handle, layers, len(stack), float(omega), ct.c_void_p(kx.data_ptr()), ct.c_void_p(ky.data_ptr()), n,
         int(source_layer), int(obs_layer), npairs, zs.ctypes.data_as(ct.POINTER(ct.c_double)),
         zo.ctypes.data_as(ct.POINTER(ct.c_double)), ct.c_void_p(g.data_ptr()) if g is not None else None,
-        ct.byref(cs) if cs is not None else None, _terms(terms))
+        ct.byref(cs) if cs is not None else None, _terms(terms), float(dz_box), 1 if ez_local else 0)
     _lib.check(rc, ctx.handle)
     return nc, g
 

@@ -126,21 +126,16 @@ class CavityTransfer2D:
             raise ValueError(f"{who}: obs_layer={obs_layer} not in this cache's obs_layers {self.obs_layers}")
 
 
-def _no_box_z(dz_box):
-    if dz_box:
-        raise NotImplementedError("closed_form_G: box sources with a z extent (Δz_box > 0) are not supported on "
-                                  "this path; point sources and box_lateral footprints are")
-
-
-def closed_form_G_2d(cache, obs_layer, zsrc_local, z_obs, xs, ys, box_lateral=None, dz_box=0.0, lateral="direct",
-                     nufft_eps=1e-9, terms="all"):
+def closed_form_G_2d(cache, obs_layer, zsrc_local, z_obs, xs, ys, box_lateral=None, dz_box=0.0, ez_local=True,
+                     lateral="direct", nufft_eps=1e-9, terms="all"):
     """G(x, y, z, c, q) of a unit moment q at (0, 0, zsrc_local) in the source
     layer, at pixels (xs[i], ys[j]) and depths z_obs in obs_layer
     (closed_form_G for a CavityTransferCache2D, mod_ClosedFormField.jl:711-782).
-    Returns a (3, 3, nz, ny, nx) complex128 tensor (the column-major
-    (nx, ny, nz, 3, 3) array)."""
+    A box source is box_lateral = (dx, dy) (footprint factor on the node
+    weights) and dz_box (the axial factor, the interior slab and, if
+    ez_local, the local Ez).  Returns a (3, 3, nz, ny, nx) complex128 tensor
+    (the column-major (nx, ny, nz, 3, 3) array)."""
     torch = _torch()
-    _no_box_z(dz_box)
     cache.check_obs(obs_layer)
     lat = _lateral(lateral)
     xs = np.ascontiguousarray(np.atleast_1d(np.asarray(xs, np.float64)))
@@ -148,7 +143,7 @@ def closed_form_G_2d(cache, obs_layer, zsrc_local, z_obs, xs, ys, box_lateral=No
     z_obs = np.atleast_1d(np.asarray(z_obs, np.float64))
     _, g = green_nodes(cache.stack, cache.omega, cache.kx, cache.ky, cache.source_layer, obs_layer,
                        np.full(len(z_obs), float(zsrc_local)), z_obs, cache=False, ghat=True, device=cache.device,
-                       terms=terms)
+                       terms=terms, dz_box=dz_box, ez_local=ez_local)
     W = cache.weights(box_lateral)
     G = torch.empty((3, 3, len(z_obs), len(ys), len(xs)), dtype=torch.complex128, device=cache.kx.device)
     ctx = _lib.Context.get(cache.device)
@@ -306,8 +301,7 @@ def _grid_struct(grid):
 
 
 def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skeleton, skeleton_tol, lateral,
-           nufft_eps, uniform_z, box_lateral, dz_box):
-    _no_box_z(dz_box)
+           nufft_eps, uniform_z, box_lateral, dz_box, ez_local=True):
     if structure not in _lib.STRUCTURE:
         raise ValueError("build_layer_operator: structure must be :auto, :structured or :dense")
     for o in z_obs_by_layer:
@@ -320,7 +314,7 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
                                               for o in obs]) if obs else np.zeros(1))
     opt = _lib.YtfOperatorOptions(_lib.STRUCTURE[structure], int(bool(check)), float(check_tol),
                                   int(bool(skeleton)), float(skeleton_tol), _lateral(lateral), float(nufft_eps),
-                                  int(bool(uniform_z)))
+                                  int(bool(uniform_z)), float(dz_box), int(bool(ez_local)))
     W = cache.weights(box_lateral)
     ctx = _lib.Context.get(cache.device)
     ctx.bind_stream()
@@ -333,7 +327,8 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
                                        ct.byref(h))
     _lib.check(rc, ctx.handle)
     kwargs = dict(structure=structure, check=check, check_tol=check_tol, skeleton=skeleton,
-                  skeleton_tol=skeleton_tol, lateral=lateral, nufft_eps=nufft_eps, box_lateral=box_lateral)
+                  skeleton_tol=skeleton_tol, lateral=lateral, nufft_eps=nufft_eps, box_lateral=box_lateral,
+                  dz_box=dz_box, ez_local=ez_local)
     op = LayerGreenOperator(h, cache, grid, zs, kwargs, cache.device)
     if warn.value & _lib.WARN_STRUCTURE_CHECK:
         for b in op.blocks:
@@ -345,12 +340,12 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
 
 def build_layer_operator(cache, grid, z_src, z_obs_by_layer, structure="auto", check=True, check_tol=1e-8,
                          skeleton=True, skeleton_tol=1e-6, lateral="direct", nufft_eps=1e-9, box_lateral=None,
-                         dz_box=0.0):
+                         dz_box=0.0, ez_local=True):
     """build_layer_operator (mod_GreenOperator.jl:430-480): the Ĝ blocks of
     `cache`'s source layer onto every layer of z_obs_by_layer (uniform z grids),
     Toeplitz+Hankel where the media allow, dense (or skeleton) elsewhere."""
     return _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skeleton, skeleton_tol, lateral,
-                  nufft_eps, True, box_lateral, dz_box)
+                  nufft_eps, True, box_lateral, dz_box, ez_local)
 
 
 def build_layer_operator_grouped(stack, omega, source_layer, groups, grid, z_src, z_obs_by_layer, device=0,

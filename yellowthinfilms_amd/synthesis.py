@@ -57,12 +57,24 @@ class CavityTransfer1D:
     def n_nodes(self):
         return self.kx.shape[0]
 
-    def ghat(self, obs_layer, zsrc_local, z_obs):
-        """g~(kx_k, 0; zsrc, z_obs[i]) for every node: (nz, 9, n) complex128."""
+    def ghat(self, obs_layer, zsrc_local, z_obs, dz_box=0.0):
+        """g~(kx_k, 0; zsrc, z_obs[i]) for every node: (nz, 9, n) complex128.
+        dz_box > 0: a box source of that thickness.  The 1D path has no
+        interior-slab term: an observation plane inside the box's own slab
+        raises, as direct_amplitude's branch does (mod_ClosedFormField.jl:208-214)."""
         z_obs = np.atleast_1d(np.asarray(z_obs, np.float64))
+        if dz_box > 0 and obs_layer == self.source_layer:
+            d = self.stack[self.source_layer - 1].d
+            zpos = d - float(zsrc_local) if self.source_layer == 1 else float(zsrc_local)
+            for z in z_obs:
+                zeta = z - zpos
+                if abs(zeta) < dz_box / 2:
+                    raise ValueError(f"direct_amplitude: z_obs is inside the box's own z-slab (|ζ|={abs(zeta)} < "
+                                     f"Δz/2={dz_box / 2}) -- that needs the interior-slab term, not the exterior "
+                                     "seed.")
         _, g = green_nodes(self.stack, self.omega, self.kx, self.ky, self.source_layer, obs_layer,
                            np.full(len(z_obs), float(zsrc_local)), z_obs, cache=False, ghat=True,
-                           device=self.device)
+                           device=self.device, dz_box=dz_box)
         return g
 
 
@@ -81,7 +93,7 @@ def radial_synthesis(cache, ghat, rho, mode="fp64"):
     return R
 
 
-def closed_form_G(cache, obs_layer, zsrc_local, z_obs, xs, ys, mode="fp64"):
+def closed_form_G(cache, obs_layer, zsrc_local, z_obs, xs, ys, mode="fp64", dz_box=0.0):
     """G(x, y, z, c, q) of a unit moment q at (0, 0, zsrc_local) in the source
     layer, at pixels (xs[i], ys[j]) and depths z_obs in obs_layer.
 
@@ -91,7 +103,7 @@ def closed_form_G(cache, obs_layer, zsrc_local, z_obs, xs, ys, mode="fp64"):
     torch = _torch()
     xs = np.ascontiguousarray(np.atleast_1d(np.asarray(xs, np.float64)))
     ys = np.ascontiguousarray(np.atleast_1d(np.asarray(ys, np.float64)))
-    g = cache.ghat(obs_layer, zsrc_local, z_obs)
+    g = cache.ghat(obs_layer, zsrc_local, z_obs, dz_box=dz_box)
     nz = g.shape[0]
     G = torch.empty((3, 3, nz, len(ys), len(xs)), dtype=torch.complex128, device=cache.kx.device)
     ctx = _lib.Context.get(cache.device)
