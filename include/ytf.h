@@ -154,7 +154,8 @@ int ytf_green_nodes_box(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, dou
  *     E_c = scale * bfft2( sum_q g~_{c,q} fft2(J_q) ).
  * With scale = dx*dy * dkx*dky*nx*ny/(2pi)^2 / (nx*ny) this is
  * _inverse_transform(g~ .* _forward_transform(J)) exactly (|ramp| = 1).
- *   nx, ny: powers of two in 8..2048;
+ *   nx, ny: powers of two in 8..2048, or any length 1..1024 (mixed-radix
+ *     transforms; uniform_kxky_grid's Nx = ceil(2L/dx) is any integer);
  *   ghat_dev: [9][nx*ny] complex with node index y + ny*x (y fastest);
  *   J_dev, E_dev: (nx, ny, 3) complex column-major (E may not alias J);
  *   work_dev: scratch of 3*nx*ny complex. */
@@ -342,7 +343,8 @@ typedef struct ytf_operator_options {
  *     n_obs[l] depths of layer obs_layers[l], concatenated in z_obs;
  *   warnings: YTF_WARN_* bits (may be NULL).
  * Blocks are stored in ascending observation layer order; a layer with no
- * depths gets no block. */
+ * depths gets no block.  Grid sizes: Nx, Ny powers of two up to 1024, or
+ * any size up to 512 (the padded 2Nx x 2Ny transforms are mixed-radix). */
 typedef struct ytf_operator ytf_operator;
 int ytf_operator_build(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
                        int64_t n_nodes, const double *kx_dev, const double *ky_dev, const double *w_dev,

@@ -129,6 +129,38 @@ def test_spectral_apply_small_grid_against_oracle():
     assert relerr(E, Eref) < 1e-10
 
 
+@pytest.mark.parametrize("L,dx,Ly,dy", [
+    (0.5, 0.1, None, None),        # 10 x 10: uniform_kxky_grid(L=0.5, dx=0.1) of the reference's validations
+    (1.0, 0.1, None, None),        # 20 x 20
+    (0.3, 0.1, None, None),        # 6 x 6
+    (0.25, 0.1, 0.35, 0.1),        # 5 x 7 (odd: the wavenumbers start at -ceil(N/2) dk)
+    (0.55, 0.1, 0.65, 0.1),        # 11 x 13 (primes above 7)
+    (3.0, 1 / 16, 1.875, 1 / 16),  # 96 x 60
+    (0.5, 1 / 16, 0.3125, 1 / 16),  # 16 x 10 (radix-8 x pass, ragged row blocks)
+    (31.25, 1 / 16, 0.1875, 1 / 16),  # 1000 x 6
+    (1 / 32, 1 / 64, 1021 / 128, 1 / 64),  # 4 x 1021 (a prime length)
+])
+def test_spectral_apply_any_grid_size(L, dx, Ly, dy):
+    """g~ and the FFT convolution on grids that are not powers of two, against
+    the oracle's g~ and numpy's transform pair (library check of the
+    mixed-radix LDS transforms)."""
+    from yellowthinfilms_amd import uniform_kxky_grid, SpectralGreenOperator
+    grid = uniform_kxky_grid(L, dx, Ly=Ly, dy=dy)
+    nx, ny = grid["Nx"], grid["Ny"]
+    print(f"case: spectral apply {nx}x{ny}", flush=True)
+    stack = c2_stack()
+    op = SpectralGreenOperator(stack, W0, grid, 1, 20 * NM, 1, 40 * NM).build()
+    KX, KY = np.meshgrid(grid["kx"], grid["ky"], indexing="ij")
+    og = po.ghat_batch(ostack(stack), W0, KX.ravel(), KY.ravel(), 1, 1, [20 * NM], [40 * NM])[:, 0]
+    og = og.reshape(nx, ny, 3, 3)
+    rng = np.random.default_rng(nx * ny)
+    J = (rng.normal(size=(nx, ny, 3)) + 1j * rng.normal(size=(nx, ny, 3))) / math.sqrt(2)
+    Jt = torch.as_tensor(np.ascontiguousarray(J.transpose(2, 1, 0)), device="cuda")
+    E = op.apply(Jt).cpu().numpy().transpose(2, 1, 0)
+    Eref = po.spectral_apply(og, J, grid)
+    assert relerr(E, Eref) < 1e-10, relerr(E, Eref)
+
+
 def test_spectral_apply_full_512_grid():
     """Full 512^2 grid: the GPU apply equals numpy's transform pair on the GPU g~
     (size-independent check of the FFT passes), g~ parity on a node sample,

@@ -199,8 +199,8 @@ def test_operator_errors():
         build_layer_operator(cache, grid, [0.1, 0.2], {2: [0.3, 0.1]}, skeleton=False)
     with pytest.raises(ValueError, match="structure must be"):
         build_layer_operator(cache, grid, [0.1], {2: [0.1]}, structure="banded")
-    with pytest.raises(YtfError, match="powers of two"):
-        build_layer_operator(cache, _grid(6, 8, 0.05, 0.05), [0.1], {2: [0.1]}, skeleton=False)
+    with pytest.raises(YtfError, match="unsupported"):  # 2Nx = 1200: above the mixed-radix transforms' 1024
+        build_layer_operator(cache, _grid(600, 8, 0.05, 0.05), [0.1], {2: [0.1]}, skeleton=False)
     op = build_layer_operator(cache, grid, [0.1, 0.2], {2: [0.3]}, skeleton=False)
     with pytest.raises(ValueError, match="P must be"):
         apply_operator(op, torch.zeros((3, 3, 8, 8), dtype=torch.complex128))
@@ -237,6 +237,57 @@ def test_single_plane_and_merged_blocks(label, src, zs, zobs, kinds):
         e = relerr(E[o], ref[o])
         print(f"  obs {o}: rel err {e:.3e}", flush=True)
         assert e < 1e-10, (o, e)
+
+
+@pytest.mark.parametrize("Nx,Ny,dx,dy,zs,zobs,lateral", [
+    # uniform_kxky_grid(L=0.5, dx=0.1): the grid of the reference's operator and point validations
+    (10, 10, 0.1, 0.1, [0.1, 0.3, 0.5], {1: [0.2, 0.6], 2: [0.15, 0.35, 0.55]}, "direct"),
+    # odd sizes (2Nx = 10, 2Ny = 14), NUFFT lateral sum onto 10 x 14 modes
+    (5, 7, 0.06, 0.05, [0.1, 0.3, 0.5], {2: [0.2, 0.4], 4: [0.3]}, "nufft"),
+    # prime factors > 7 (2Nx = 22, 2Ny = 26)
+    (11, 13, 0.04, 0.05, [0.2, 0.4], {2: [0.1, 0.3, 0.5]}, "direct"),
+    # one plane each with a power-of-two y: the fused y pass after a mixed-radix x pass (2Nx = 24)
+    (12, 8, 0.05, 0.06, [0.3], {2: [0.45], 4: [0.2]}, "direct"),
+    # one plane each, y not a power of two: unfused single-plane path (2Nx = 16, 2Ny = 12)
+    (8, 6, 0.05, 0.06, [0.3], {2: [0.45]}, "direct"),
+])
+def test_operator_any_grid_size(Nx, Ny, dx, dy, zs, zobs, lateral):
+    """build_layer_operator / apply_operator on grids that are not powers of
+    two (uniform_kxky_grid's Nx = ceil(2L/dx) is any integer), against the
+    oracle's literal linear convolution."""
+    from yellowthinfilms_amd.green_operator import build_layer_operator
+    stack = FOUNDATION_STACKS["iso 5-layer lossy"]
+    cache = _cache(stack, 2)
+    grid = _grid(Nx, Ny, dx, dy)
+    print(f"case: operator grid {Nx}x{Ny} Ns={len(zs)} {lateral}", flush=True)
+    op = build_layer_operator(cache, grid, zs, zobs, skeleton=False, lateral=lateral, nufft_eps=1e-12)
+    P = _moments(Nx, Ny, len(zs), seed=Nx + Ny, n=5)
+    E = _run(op, P)
+    ref = _oracle_apply(stack, cache, zs, zobs, P, grid)
+    tol = 1e-10 if lateral == "direct" else 1e-9
+    for o in zobs:
+        e = relerr(E[o], ref[o])
+        print(f"  obs {o}: rel err {e:.3e}", flush=True)
+        assert e < tol, (o, e)
+
+
+def test_ghat_planes_on_a_mixed_radix_grid():
+    """Ghat = FFT of G on the padded offsets for 2Nx = 20, 2Ny = 14 (numpy's FFT as the library check)."""
+    from yellowthinfilms_amd.green_operator import build_closed_form_green, padded_offsets
+    stack = FOUNDATION_STACKS["uniaxial 4-layer"]
+    cache = _cache(stack, 2)
+    grid = _grid(10, 7, 0.07, 0.05)
+    zo = [0.2, 0.5]
+    print("case: Ghat planes 20x14", flush=True)
+    g = build_closed_form_green(cache, 3, 0.4, grid, zo)
+    Gh = g.Ghat.cpu().numpy().transpose(4, 3, 2, 1, 0)
+    xs, ys = padded_offsets(grid)
+    G = po.closed_form_G_2d(ostack(stack), W0, cache.kx_host, cache.ky_host, cache.W_host, 2, 3, 0.4, zo, xs, ys)
+    Px, Py = len(xs), len(ys)
+    assert (Px, Py) == (20, 14)
+    sign = (-1.0) ** np.add.outer(np.arange(Px), np.arange(Py))
+    ref = np.fft.fft2(G, axes=(0, 1)) * (sign * grid["dx"] * grid["dy"] / (Px * Py))[:, :, None, None, None]
+    assert relerr(Gh, ref) < 1e-10, relerr(Gh, ref)
 
 
 def _oracle_families(stack, cache, obs):

@@ -206,14 +206,13 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
 
 extern "C" {
 
-static bool pow2_in(int n, int lo, int hi) { return n >= lo && n <= hi && (n & (n - 1)) == 0; }
-
 int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
                        double *work_dev, double scale) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_spectral_apply: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
-  if (!pow2_in(nx, 8, 2048) || !pow2_in(ny, 8, 2048))
-    return fail(c, YTF_EINVAL, "ytf_spectral_apply: nx=%d, ny=%d must be powers of two in 8..2048", nx, ny);
+  if (!ytf::fft_length_ok(nx) || !ytf::fft_length_ok(ny))
+    return fail(c, YTF_EINVAL,
+                "ytf_spectral_apply: nx=%d, ny=%d unsupported (powers of two 8..2048, any other size 1..1024)", nx, ny);
   if (!ghat_dev || !J_dev || !E_dev || !work_dev) return fail(c, YTF_EINVAL, "ytf_spectral_apply: NULL buffer");
   if ((const void *)E_dev == (const void *)J_dev) return fail(c, YTF_EINVAL, "ytf_spectral_apply: E aliases J");
   HIPCHK(c, hipSetDevice(c->device));

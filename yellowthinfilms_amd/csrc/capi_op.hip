@@ -22,7 +22,6 @@ int upload(ytf_ctx *c, void *dst, const void *src, size_t bytes) {
   return YTF_OK;
 }
 
-bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // uniform step of a grid, or NaN if it is not uniform (_uniform_step,
 // mod_GreenOperator.jl:383-386)
@@ -638,9 +637,10 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
     return fail(c, YTF_EINVAL, "build_layer_operator: structure must be :auto, :structured or :dense");
   if (opt->lateral != YTF_LATERAL_DIRECT && opt->lateral != YTF_LATERAL_NUFFT)
     return fail(c, YTF_EINVAL, "closed_form_G: lateral must be :direct or :nufft");
-  if (!pow2(grid->Nx) || !pow2(grid->Ny) || grid->Nx < 4 || grid->Ny < 4 || grid->Nx > 1024 || grid->Ny > 1024)
-    return fail(c, YTF_EINVAL, "ytf_operator_build: grid Nx=%d, Ny=%d must be powers of two in 4..1024", grid->Nx,
-                grid->Ny);
+  if (grid->Nx < 1 || grid->Ny < 1 || !ytf::fft_length_ok(2 * grid->Nx) || !ytf::fft_length_ok(2 * grid->Ny))
+    return fail(c, YTF_EINVAL,
+                "ytf_operator_build: grid Nx=%d, Ny=%d unsupported (powers of two up to 1024, any other size up to 512)",
+                grid->Nx, grid->Ny);
   if (!(grid->dx > 0) || !(grid->dy > 0)) return fail(c, YTF_EINVAL, "ytf_operator_build: grid pitch must be > 0");
   if (n_src < 1 || !z_src) return fail(c, YTF_EINVAL, "build_layer_operator: need at least one source plane");
   if (n_obs_layers < 0 || (n_obs_layers > 0 && (!obs_layers || !n_obs)))
@@ -934,6 +934,7 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
   const int64_t pa = op->flip ? -nxy : nxy;
   bool fused = !op->blocks.empty();
   for (auto &b : op->blocks) fused = fused && single_plane(b, Ns);
+  fused = fused && ytf::opy_mix1_supported(Py, Px);
   if (fused) {
     // one plane per block: FFT along x, then per block the fused y FFT,
     // 3x3 product and inverse y FFT, and the inverse x FFT

@@ -18,7 +18,8 @@
 //      (Toeplitz, Hankel, dense plane pairs, skeleton fold and expansion);
 //   4. k_opy<.., true>: inverse FFT along y, keeping rows < Ny;
 //   5. k_opx_inv: inverse FFT along x, cropped to the Nx x Ny pixels.
-// Power-of-two padded sizes 8..2048 (FFTs in LDS, fft_core.hpp).
+// Padded sizes: powers of two 8..2048 (radix-8 FFTs in LDS) or any length
+// up to FFT_RT_MAX (mixed-radix, fft_lds_rt) (fft_core.hpp).
 #include <hip/hip_runtime.h>
 
 #include "fft_core.hpp"
@@ -134,31 +135,34 @@ __global__ __launch_bounds__(256) void k_lat_direct(int64_t n, const double *__r
 // Rows per block of the x passes: at most 512 threads and 8 rows (the rows
 // give the length of the contiguous segments of the transposed accesses).
 template <int PX> struct OpX {
-  static constexpr int YT = (4096 / PX) < 4 ? (4096 / PX < 1 ? 1 : 4096 / PX) : ((4096 / PX) > 8 ? 8 : 4096 / PX);
+  static constexpr int Q = PX ? 4096 / PX : 1;  // PX = 0: runtime length (shape from rt_shape)
+  static constexpr int YT = Q < 4 ? (Q < 1 ? 1 : Q) : (Q > 8 ? 8 : Q);
 };
 
 // Forward FFT along x of plane t = blockIdx.y, input rows zero-padded from
 // nx_in to PX and rows >= ny_rows skipped; output transposed: out[t][x][y]
 // (row stride py).  Input plane t starts at in + (t / pdiv) * pa + (t % pdiv) * pb.
+// PX = 0: runtime length rc.n (rc.tpr threads per row, rc.rows rows).
 template <int PX>
 __global__ __launch_bounds__(1024) void k_opx_fwd(const cd *__restrict__ in, int pdiv, int64_t pa, int64_t pb,
                                                   int in_row, int nx_in, int ny_rows, cd *__restrict__ out, int py,
-                                                  int64_t out_plane, const cd *__restrict__ tw) {
-  constexpr int YT = OpX<PX>::YT;
-  constexpr int TPR = PX / 8;
+                                                  int64_t out_plane, const cd *__restrict__ tw, RowCfg rc) {
+  const int px = PX ? PX : rc.n;
+  const int YT = PX ? OpX<PX>::YT : rc.rows;
+  const int TPR = PX ? PX / 8 : rc.tpr;
+  const int S = PX ? LdsRow<PX>::S : lds_stride(rc.n);
   extern __shared__ cd lds[];
   const int t = blockIdx.y, tid = threadIdx.x;
   const cd *src = in + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
   cd *dst = out + (int64_t)t * out_plane;
   const int y0 = blockIdx.x * YT;
-  constexpr int S = LdsRow<PX>::S;
-  for (int e = tid; e < YT * PX; e += YT * TPR) {
-    const int r = e / PX, x = e % PX, y = y0 + r;
+  for (int e = tid; e < YT * px; e += YT * TPR) {
+    const int r = e / px, x = e % px, y = y0 + r;
     lds[r * S + lpad(x)] = (x < nx_in && y < ny_rows) ? src[x + (int64_t)in_row * y] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PX, false>(lds + (tid / TPR) * S, tid % TPR, tw);
-  for (int e = tid; e < YT * PX; e += YT * TPR) {
+  row_fft<PX, false>(lds + (tid / TPR) * S, lds + (YT + tid / TPR) * S, tid % TPR, TPR, px, tw);
+  for (int e = tid; e < YT * px; e += YT * TPR) {
     const int r = e % YT, x = e / YT, y = y0 + r;
     if (y < ny_rows) dst[y + (int64_t)py * x] = lds[r * S + lpad(x)];
   }
@@ -169,30 +173,31 @@ __global__ __launch_bounds__(1024) void k_opx_fwd(const cd *__restrict__ in, int
 template <int PX>
 __global__ __launch_bounds__(1024) void k_opx_inv(const cd *__restrict__ in, int py, int64_t in_plane,
                                                   cd *__restrict__ out, int pdiv, int64_t pa, int64_t pb, int out_row,
-                                                  int nx_out, int ny_rows, const cd *__restrict__ tw) {
-  constexpr int YT = OpX<PX>::YT;
-  constexpr int TPR = PX / 8;
+                                                  int nx_out, int ny_rows, const cd *__restrict__ tw, RowCfg rc) {
+  const int px = PX ? PX : rc.n;
+  const int YT = PX ? OpX<PX>::YT : rc.rows;
+  const int TPR = PX ? PX / 8 : rc.tpr;
+  const int S = PX ? LdsRow<PX>::S : lds_stride(rc.n);
   extern __shared__ cd lds[];
   const int t = blockIdx.y, tid = threadIdx.x;
   const cd *src = in + (int64_t)t * in_plane;
   cd *dst = out + (int64_t)(t / pdiv) * pa + (int64_t)(t % pdiv) * pb;
   const int y0 = blockIdx.x * YT;
-  constexpr int S = LdsRow<PX>::S;
-  for (int e = tid; e < YT * PX; e += YT * TPR) {
+  for (int e = tid; e < YT * px; e += YT * TPR) {
     const int r = e % YT, x = e / YT, y = y0 + r;
     lds[r * S + lpad(x)] = y < ny_rows ? src[y + (int64_t)py * x] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PX, true>(lds + (tid / TPR) * S, tid % TPR, tw);
-  for (int e = tid; e < YT * PX; e += YT * TPR) {
-    const int r = e / PX, x = e % PX, y = y0 + r;
+  row_fft<PX, true>(lds + (tid / TPR) * S, lds + (YT + tid / TPR) * S, tid % TPR, TPR, px, tw);
+  for (int e = tid; e < YT * px; e += YT * TPR) {
+    const int r = e / px, x = e % px, y = y0 + r;
     if (x < nx_out && y < ny_rows) dst[x + (int64_t)out_row * y] = lds[r * S + lpad(x)];
   }
 }
 
 // Columns per block of the y pass: PY/8 threads per column, <= 256 threads.
 template <int PY> struct OpY {
-  static constexpr int TPC = PY / 8;
+  static constexpr int TPC = PY ? PY / 8 : 1;  // PY = 0: runtime length (shape from rt_shape)
   static constexpr int XT = (256 / TPC) < 1 ? 1 : ((256 / TPC) > 64 ? 64 : (256 / TPC));
 };
 
@@ -201,24 +206,26 @@ template <int PY> struct OpY {
 // multiplied by scale, and by (-1)^(x+y) when `sign`.
 template <int PY, bool INV>
 __global__ __launch_bounds__(256) void k_opy(cd *__restrict__ work, int px, int64_t plane, int ny_in, int ny_out,
-                                             double scale, int sign, const cd *__restrict__ tw) {
-  constexpr int TPC = OpY<PY>::TPC, XT = OpY<PY>::XT;
+                                             double scale, int sign, const cd *__restrict__ tw, RowCfg rc) {
+  const int py = PY ? PY : rc.n;
+  const int TPC = PY ? OpY<PY>::TPC : rc.tpr;
+  const int XT = PY ? OpY<PY>::XT : rc.rows;
+  const int S = PY ? LdsRow<PY>::S : lds_stride(rc.n);
   extern __shared__ cd lds[];
   const int t = blockIdx.y, tid = threadIdx.x;
   cd *base = work + (int64_t)t * plane;
   const int x0 = blockIdx.x * XT;
-  constexpr int S = LdsRow<PY>::S;
-  for (int e = tid; e < XT * PY; e += XT * TPC) {
-    const int col = e / PY, y = e % PY, x = x0 + col;
-    lds[col * S + lpad(y)] = (x < px && y < ny_in) ? base[y + (int64_t)PY * x] : cd{0.0, 0.0};
+  for (int e = tid; e < XT * py; e += XT * TPC) {
+    const int col = e / py, y = e % py, x = x0 + col;
+    lds[col * S + lpad(y)] = (x < px && y < ny_in) ? base[y + (int64_t)py * x] : cd{0.0, 0.0};
   }
   __syncthreads();
-  fft_lds<PY, INV>(lds + (tid / TPC) * S, tid % TPC, tw);
-  for (int e = tid; e < XT * PY; e += XT * TPC) {
-    const int col = e / PY, y = e % PY, x = x0 + col;
+  row_fft<PY, INV>(lds + (tid / TPC) * S, lds + (XT + tid / TPC) * S, tid % TPC, TPC, py, tw);
+  for (int e = tid; e < XT * py; e += XT * TPC) {
+    const int col = e / py, y = e % py, x = x0 + col;
     if (x < px && y < ny_out) {
       const double s = (sign && ((x + y) & 1)) ? -scale : scale;
-      base[y + (int64_t)PY * x] = rmul(s, lds[col * S + lpad(y)]);
+      base[y + (int64_t)py * x] = rmul(s, lds[col * S + lpad(y)]);
     }
   }
 }
@@ -375,10 +382,11 @@ static hipError_t opx(bool inv, const cd *in, int pdiv, int64_t pa, int64_t pb, 
   constexpr int YT = OpX<PX>::YT;
   const dim3 grid((ny_rows + YT - 1) / YT, planes), block(YT * (PX / 8));
   const size_t sh = sizeof(cd) * YT * LdsRow<PX>::S;
+  const RowCfg rc{PX, PX / 8, YT};
   if (inv)
-    hipLaunchKernelGGL(k_opx_inv<PX>, grid, block, sh, s, in, py, plane, out, pdiv, pa, pb, row, nx, ny_rows, tw);
+    hipLaunchKernelGGL(k_opx_inv<PX>, grid, block, sh, s, in, py, plane, out, pdiv, pa, pb, row, nx, ny_rows, tw, rc);
   else
-    hipLaunchKernelGGL(k_opx_fwd<PX>, grid, block, sh, s, in, pdiv, pa, pb, row, nx, ny_rows, out, py, plane, tw);
+    hipLaunchKernelGGL(k_opx_fwd<PX>, grid, block, sh, s, in, pdiv, pa, pb, row, nx, ny_rows, out, py, plane, tw, rc);
   return hipGetLastError();
 }
 
@@ -392,7 +400,15 @@ hipError_t launch_opx(bool inv, int PX, const cd *in, int pdiv, int64_t pa, int6
     YTF_OPX(2048)
 #undef YTF_OPX
   }
-  return hipErrorInvalidValue;
+  if (PX < 1 || PX > FFT_RT_MAX) return hipErrorInvalidValue;
+  const RowCfg rc = rt_shape(PX, 512, 8, 1);
+  const dim3 grid((ny_rows + rc.rows - 1) / rc.rows, planes), block(rc.rows * rc.tpr);
+  const size_t sh = rt_lds_bytes(rc, 1);
+  if (inv)
+    hipLaunchKernelGGL(k_opx_inv<0>, grid, block, sh, s, in, py, plane, out, pdiv, pa, pb, row, nx, ny_rows, tw, rc);
+  else
+    hipLaunchKernelGGL(k_opx_fwd<0>, grid, block, sh, s, in, pdiv, pa, pb, row, nx, ny_rows, out, py, plane, tw, rc);
+  return hipGetLastError();
 }
 
 template <int PY>
@@ -401,8 +417,11 @@ static hipError_t opy(bool inv, cd *work, int px, int64_t plane, int planes, int
   constexpr int XT = OpY<PY>::XT, TPC = OpY<PY>::TPC;
   const dim3 grid((px + XT - 1) / XT, planes), block(XT * TPC);
   const size_t sh = sizeof(cd) * XT * LdsRow<PY>::S;
-  if (inv) hipLaunchKernelGGL((k_opy<PY, true>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
-  else hipLaunchKernelGGL((k_opy<PY, false>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw);
+  const RowCfg rc{PY, TPC, XT};
+  if (inv)
+    hipLaunchKernelGGL((k_opy<PY, true>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw, rc);
+  else
+    hipLaunchKernelGGL((k_opy<PY, false>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw, rc);
   return hipGetLastError();
 }
 
@@ -416,7 +435,15 @@ hipError_t launch_opy(bool inv, int PY, cd *work, int px, int64_t plane, int pla
     YTF_OPY(2048)
 #undef YTF_OPY
   }
-  return hipErrorInvalidValue;
+  if (PY < 1 || PY > FFT_RT_MAX) return hipErrorInvalidValue;
+  const RowCfg rc = rt_shape(PY, 256, 64, 1);
+  const dim3 grid((px + rc.rows - 1) / rc.rows, planes), block(rc.rows * rc.tpr);
+  const size_t sh = rt_lds_bytes(rc, 1);
+  if (inv)
+    hipLaunchKernelGGL((k_opy<0, true>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw, rc);
+  else
+    hipLaunchKernelGGL((k_opy<0, false>), grid, block, sh, s, work, px, plane, ny_in, ny_out, scale, sign, tw, rc);
+  return hipGetLastError();
 }
 
 template <int PY>
@@ -429,6 +456,17 @@ static hipError_t opy_mix1(const cd *in, cd *out, int px, const cd *K, int ny_in
   const size_t sh = sizeof(cd) * 3 * XT * LdsRow<PY>::S;
   hipLaunchKernelGGL(k_opy_mix1<PY>, grid, block, sh, s, in, out, nn, K, ny_in, ny_out, tw);
   return hipGetLastError();
+}
+
+bool opy_mix1_supported(int PY, int px) {
+  switch (PY) {
+#define YTF_OPYM(N) \
+  case N: return px >= 2 && px % OpYM<N>::XT == 0;
+    YTF_OPYM(8) YTF_OPYM(16) YTF_OPYM(32) YTF_OPYM(64) YTF_OPYM(128) YTF_OPYM(256) YTF_OPYM(512) YTF_OPYM(1024)
+    YTF_OPYM(2048)
+#undef YTF_OPYM
+  }
+  return false;
 }
 
 hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,

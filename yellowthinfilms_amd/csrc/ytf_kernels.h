@@ -157,6 +157,11 @@ hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *o
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
 hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
                            hipStream_t s);
+// The fused single-plane y pass exists for power-of-two PY 8..2048 with px a
+// multiple of its columns per block.
+bool opy_mix1_supported(int PY, int px);
+// Lengths the LDS FFT passes take: powers of two 8..2048, or 1..1024.
+bool fft_length_ok(int n);
 
 // z skeletons (skeleton.hip): the exponential family of one side, its
 // tall-skinny QR factor and the interpolation error of a column ID.
