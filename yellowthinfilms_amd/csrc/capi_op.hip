@@ -968,21 +968,20 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
     e = timed(c, "op_mix", [&] {
       hipError_t r = hipSuccess;
       bool acc = false;
-      if (b.T) {
-        r = ytf::launch_op_mix(nn, J, Ns, Ew, No, b.T, ytf::OP_MIX_TOEPLITZ, acc, c->stream);
+      if (b.T) {  // Toeplitz planes, with the Hankel planes in the same pass
+        r = ytf::launch_op_mix(nn, J, Ns, Ew, No, b.T, b.H, ytf::OP_MIX_TOEPLITZ, acc, c->stream);
         acc = true;
-      }
-      if (r == hipSuccess && b.H) {
-        r = ytf::launch_op_mix(nn, J, Ns, Ew, No, b.H, ytf::OP_MIX_HANKEL, acc, c->stream);
+      } else if (b.H) {
+        r = ytf::launch_op_mix(nn, J, Ns, Ew, No, b.H, nullptr, ytf::OP_MIX_HANKEL, acc, c->stream);
         acc = true;
       }
       if (r == hipSuccess && b.D) {
         const cd *Jin = b.PsM ? Js : J;
         if (b.PoM) {  // skeleton observation planes, interpolated to all planes (_zexpand_add!)
-          r = ytf::launch_op_mix(nn, Jin, b.nDs, Es, b.nDo, b.D, ytf::OP_MIX_DENSE, 0, c->stream);
+          r = ytf::launch_op_mix(nn, Jin, b.nDs, Es, b.nDo, b.D, nullptr, ytf::OP_MIX_DENSE, 0, c->stream);
           if (r == hipSuccess) r = ytf::launch_op_zcomb(nn, Es, b.nDo, b.PoM, Ew, No, acc, c->stream);
         } else {
-          r = ytf::launch_op_mix(nn, Jin, b.nDs, Ew, No, b.D, ytf::OP_MIX_DENSE, acc, c->stream);
+          r = ytf::launch_op_mix(nn, Jin, b.nDs, Ew, No, b.D, nullptr, ytf::OP_MIX_DENSE, acc, c->stream);
         }
         acc = true;
       }

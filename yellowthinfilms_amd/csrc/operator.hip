@@ -235,51 +235,77 @@ __global__ __launch_bounds__(256) void k_opy(cd *__restrict__ work, int px, int6
 // --------------------------------------------------------------------------
 
 // E[j][r] (+)= sum_i sum_c K[slot(i, j)][r + 3c] J[i][c] on every node, with
-// slot = j - i + ns - 1 (Toeplitz), i + j (Hankel) or i + ns*j (dense pairs).
-// J: [ns][3][nn], E: [no][3][nn], K: [slot][9][nn].
+// slot = j - i + ns - 1 (Toeplitz), i + j (Hankel) or i + ns*j (dense pairs);
+// with a Toeplitz K, H (if not null) adds the Hankel planes H[i + j] in the
+// same pass.  J: [ns][3][nn], E: [no][3][nn], K, H: [slot][9][nn].
+// Grid: (nodes / 256, observation planes / JT): thread = one spectral node
+// and JT observation planes, so a 128^2 padded grid with 16 planes still
+// fills the chip; the JT planes share each J load.
+template <int JT>
 __global__ __launch_bounds__(256) void k_op_mix(int64_t nn, const cd *__restrict__ J, int ns, cd *__restrict__ E,
-                                                int no, const cd *__restrict__ K, int kind, int accumulate) {
+                                                int no, const cd *__restrict__ K, const cd *__restrict__ H, int kind,
+                                                int accumulate) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= nn) return;
-  for (int j = 0; j < no; ++j) {
-    cd a0, a1, a2;
-    if (accumulate) {
-      a0 = E[(3 * (int64_t)j + 0) * nn + k];
-      a1 = E[(3 * (int64_t)j + 1) * nn + k];
-      a2 = E[(3 * (int64_t)j + 2) * nn + k];
-    } else {
-      a0 = a1 = a2 = cd{0.0, 0.0};
+  const int j0 = blockIdx.y * JT;
+  cd a[JT][3];
+#pragma unroll
+  for (int jj = 0; jj < JT; ++jj) {
+    const int j = j0 + jj;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      a[jj][r] = (accumulate && j < no) ? E[(3 * (int64_t)j + r) * nn + k] : cd{0.0, 0.0};
+  }
+  for (int i = 0; i < ns; ++i) {
+    const cd j0v = J[(3 * (int64_t)i + 0) * nn + k], j1v = J[(3 * (int64_t)i + 1) * nn + k],
+             j2v = J[(3 * (int64_t)i + 2) * nn + k];
+#pragma unroll
+    for (int jj = 0; jj < JT; ++jj) {
+      const int j = j0 + jj;
+      if (j < no) {
+        const int slot = kind == 0 ? j - i + ns - 1 : (kind == 1 ? i + j : i + ns * j);
+        const cd *Ks = K + (int64_t)slot * 9 * nn + k;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          a[jj][r] = fmac(fmac(fmac(a[jj][r], Ks[r * nn], j0v), Ks[(r + 3) * nn], j1v), Ks[(r + 6) * nn], j2v);
+        if (H) {
+          const cd *Hs = H + (int64_t)(i + j) * 9 * nn + k;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            a[jj][r] = fmac(fmac(fmac(a[jj][r], Hs[r * nn], j0v), Hs[(r + 3) * nn], j1v), Hs[(r + 6) * nn], j2v);
+        }
+      }
     }
-    for (int i = 0; i < ns; ++i) {
-      const int slot = kind == 0 ? j - i + ns - 1 : (kind == 1 ? i + j : i + ns * j);
-      const cd *Ks = K + (int64_t)slot * 9 * nn + k;
-      const cd j0 = J[(3 * (int64_t)i + 0) * nn + k], j1 = J[(3 * (int64_t)i + 1) * nn + k],
-               j2 = J[(3 * (int64_t)i + 2) * nn + k];
-      a0 = fmac(fmac(fmac(a0, Ks[0], j0), Ks[3 * nn], j1), Ks[6 * nn], j2);
-      a1 = fmac(fmac(fmac(a1, Ks[1 * nn], j0), Ks[4 * nn], j1), Ks[7 * nn], j2);
-      a2 = fmac(fmac(fmac(a2, Ks[2 * nn], j0), Ks[5 * nn], j1), Ks[8 * nn], j2);
+  }
+#pragma unroll
+  for (int jj = 0; jj < JT; ++jj) {
+    const int j = j0 + jj;
+    if (j < no) {
+#pragma unroll
+      for (int r = 0; r < 3; ++r) E[(3 * (int64_t)j + r) * nn + k] = a[jj][r];
     }
-    E[(3 * (int64_t)j + 0) * nn + k] = a0;
-    E[(3 * (int64_t)j + 1) * nn + k] = a1;
-    E[(3 * (int64_t)j + 2) * nn + k] = a2;
   }
 }
 
 // out[o][c] (+)= sum_i M[i + nin*o] in[i][c] on every node (the skeleton fold
 // J Ps and expansion Po Es; M is a small complex matrix, read uniformly).
+// Grid: (nodes / 256, nout): thread = one node and one output plane.
 __global__ __launch_bounds__(256) void k_op_zcomb(int64_t nn, const cd *__restrict__ in, int nin,
                                                   const cd *__restrict__ M, cd *__restrict__ out, int nout,
                                                   int accumulate) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int o = blockIdx.y;
   if (k >= nn) return;
-  for (int o = 0; o < nout; ++o) {
+  cd a[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cd a = accumulate ? out[(3 * (int64_t)o + c) * nn + k] : cd{0.0, 0.0};
-      for (int i = 0; i < nin; ++i) a = fmac(a, M[i + (int64_t)nin * o], in[(3 * (int64_t)i + c) * nn + k]);
-      out[(3 * (int64_t)o + c) * nn + k] = a;
-    }
+  for (int c = 0; c < 3; ++c) a[c] = accumulate ? out[(3 * (int64_t)o + c) * nn + k] : cd{0.0, 0.0};
+  for (int i = 0; i < nin; ++i) {
+    const cd m = M[i + (int64_t)nin * o];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] = fmac(a[c], m, in[(3 * (int64_t)i + c) * nn + k]);
   }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[(3 * (int64_t)o + c) * nn + k] = a[c];
 }
 
 // Single source and observation plane, fused: for XT columns x, FFT along y
@@ -482,16 +508,19 @@ hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, i
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, int kind, int accumulate,
-                         hipStream_t s) {
+hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, const cd *H, int kind,
+                         int accumulate, hipStream_t s) {
   if (nn == 0 || no == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_op_mix, dim3(nblk(nn, 256)), dim3(256), 0, s, nn, J, ns, E, no, K, kind, accumulate);
+  if (H && kind != OP_MIX_TOEPLITZ) return hipErrorInvalidValue;
+  constexpr int JT = 2;
+  hipLaunchKernelGGL(k_op_mix<JT>, dim3(nblk(nn, 256), (no + JT - 1) / JT), dim3(256), 0, s, nn, J, ns, E, no, K, H,
+                     kind, accumulate);
   return hipGetLastError();
 }
 hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
                            hipStream_t s) {
   if (nn == 0 || nout == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_op_zcomb, dim3(nblk(nn, 256)), dim3(256), 0, s, nn, in, nin, M, out, nout, accumulate);
+  hipLaunchKernelGGL(k_op_zcomb, dim3(nblk(nn, 256), nout), dim3(256), 0, s, nn, in, nin, M, out, nout, accumulate);
   return hipGetLastError();
 }
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s) {

@@ -150,8 +150,9 @@ hipError_t launch_opx(bool inv, int PX, const cd *in, int pdiv, int64_t pa, int6
 hipError_t launch_opy(bool inv, int PY, cd *work, int px, int64_t plane, int planes, int ny_in, int ny_out,
                       double scale, int sign, const cd *tw, hipStream_t s);
 enum { OP_MIX_TOEPLITZ = 0, OP_MIX_HANKEL = 1, OP_MIX_DENSE = 2 };
-hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, int kind, int accumulate,
-                         hipStream_t s);
+// H: Hankel planes added in a Toeplitz pass (kind OP_MIX_TOEPLITZ only), or NULL.
+hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const cd *K, const cd *H, int kind,
+                         int accumulate, hipStream_t s);
 hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
                            hipStream_t s);
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
