@@ -1047,10 +1047,14 @@ int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_
   while (nf < 2 * N1 || nf < 2 * N2 || nf < 4 * w) nf *= 2;
   const int nt = nf / 32, ntiles = nt * nt;
   const int64_t Mc = M > 0 ? M : 1;
+  const int nsub = ytf::nu_max_subproblems(M, nf);
+  // transforms per batch: fine grids and row-pass buffers within ~512 MiB
+  const size_t per_t = sizeof(cd) * ((size_t)nf * nf + (size_t)N1 * nf);
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)ntrans, (int64_t)((512u << 20) / per_t), 65535}));
   const size_t uv_b = align256(sizeof(double) * 2 * Mc), i_b = align256(sizeof(int) * Mc),
-               t_b = align256(sizeof(int) * (ntiles + 1)), g_b = align256(sizeof(cd) * (size_t)nf * nf),
-               T_b = align256(sizeof(cd) * (size_t)N1 * nf);
-  int rc = ensure_buf(c, &c->work, &c->work_bytes, uv_b + 2 * i_b + 3 * t_b + g_b + T_b);
+               t_b = align256(sizeof(int) * (ntiles + 1)), s_b = align256(sizeof(int) * nsub),
+               g_b = align256(sizeof(cd) * (size_t)nf * nf * nb), T_b = align256(sizeof(cd) * (size_t)N1 * nf * nb);
+  int rc = ensure_buf(c, &c->work, &c->work_bytes, uv_b + 2 * i_b + 4 * t_b + 2 * s_b + g_b + T_b);
   if (rc) return rc;
   char *p = (char *)c->work;
   double *uv = (double *)p; p += uv_b;
@@ -1059,6 +1063,9 @@ int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_
   int *count = (int *)p; p += t_b;
   int *start = (int *)p; p += t_b;
   int *cursor = (int *)p; p += t_b;
+  int *substart = (int *)p; p += t_b;
+  int *sub_tile = (int *)p; p += s_b;
+  int *sub_p0 = (int *)p; p += s_b;
   cd *grid = (cd *)p; p += g_b;
   cd *T1 = (cd *)p;
   double *dd1 = nullptr, *dd2 = nullptr;
@@ -1079,16 +1086,18 @@ int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_
   rc = twiddles(c, nf, &tw);
   if (rc) return rc;
   hipError_t e = timed(c, "nufft_sort", [&] {
-    return ytf::launch_nufft_sort(M, x_dev, y_dev, nf, uv, tile, count, start, cursor, order, c->stream);
+    return ytf::launch_nufft_sort(M, x_dev, y_dev, nf, uv, tile, count, start, cursor, substart, sub_tile, sub_p0,
+                                  order, c->stream);
   });
-  for (int t = 0; t < ntrans && e == hipSuccess; ++t) {
+  for (int t0 = 0; t0 < ntrans && e == hipSuccess; t0 += nb) {
+    const int b = std::min(nb, ntrans - t0);
     e = timed(c, "nufft_spread", [&] {
-      return ytf::launch_nufft_spread(nf, w, beta, start, order, uv, (const cd *)c_dev + (size_t)t * Mc, grid,
-                                      c->stream);
+      return ytf::launch_nufft_spread(M, b, nf, w, beta, start, substart, sub_tile, sub_p0, order, uv,
+                                      (const cd *)c_dev + (size_t)t0 * Mc, Mc, grid, c->stream);
     });
     if (e == hipSuccess)
       e = timed(c, "nufft_fft", [&] {
-        return ytf::launch_nufft_fft(nf, grid, T1, (cd *)f_dev + (size_t)t * N1 * N2, N1, N2, dd1, dd2, tw,
+        return ytf::launch_nufft_fft(b, nf, grid, T1, (cd *)f_dev + (size_t)t0 * N1 * N2, N1, N2, dd1, dd2, tw,
                                      c->stream);
       });
   }

@@ -11,11 +11,16 @@
 //     b_m = sum_j c_j phi((m h - x'_j) / a)        (periodically, per dimension),
 //     f_k = (-1)^k h F_k / psi(k),  F_k = sum_m b_m exp(+2 pi i k m / nf),
 //     psi(k) = a int_{-1}^{1} phi(z) cos(k a z) dz,
-// separably in 2D.  The spreading is binned by 32 x 32 fine-grid tiles: one
-// block per tile accumulates its points (with a halo of the kernel half-width)
-// in LDS with FP64 LDS atomics, then adds the tile to the fine grid with FP64
-// global atomics.  Two LDS-staged FFT passes (rows, then the kept columns)
-// compute only the N1 x N2 wanted modes and apply the deconvolution.
+// separably in 2D.  The spreading is binned by 32 x 32 fine-grid tiles and
+// each tile's sorted points are cut into subproblems of at most NU_SUB
+// points, so clustered nodes (the folded k-rho nodes of a lateral sum sit
+// mostly near the origin) still spread in many blocks: one block per
+// (subproblem, transform) accumulates its points (with a halo of the kernel
+// half-width) in LDS with FP64 LDS atomics, then adds the tile to its
+// transform's fine grid with FP64 global atomics.  A batch of transforms
+// shares the sort and runs in one launch per pass.  Two LDS-staged FFT
+// passes (rows, then the kept columns) compute only the N1 x N2 wanted modes
+// and apply the deconvolution.
 #include <hip/hip_runtime.h>
 
 #include "fft_core.hpp"
@@ -23,7 +28,8 @@
 
 namespace ytf {
 
-constexpr int NU_TS = 32;  // spreading tile (fine-grid points per side)
+constexpr int NU_TS = 32;    // spreading tile (fine-grid points per side)
+constexpr int NU_SUB = 512;  // points per spreading subproblem
 
 __device__ __forceinline__ double fold2pi(double x) {
   // x + pi reduced to [0, 2 pi)
@@ -59,15 +65,16 @@ __global__ void k_nu_bin(int64_t M, const double *__restrict__ x, const double *
   atomicAdd(count + t, 1);
 }
 
-// exclusive scan of the tile counts (one block); cursor = start
-__global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__restrict__ start, int *__restrict__ cursor) {
-  __shared__ int part[1024];
+// Exclusive scan over ntiles values of one block of 1024 threads: out[t] =
+// sum of val(t') for t' < t, out[ntiles] = the total.
+template <typename F>
+__device__ void block_scan(int ntiles, F val, int *__restrict__ out, int *part) {
   const int tid = threadIdx.x;
   const int per = (ntiles + 1023) / 1024;
   int s = 0;
   for (int i = 0; i < per; ++i) {
     const int t = tid * per + i;
-    if (t < ntiles) s += count[t];
+    if (t < ntiles) s += val(t);
   }
   part[tid] = s;
   __syncthreads();
@@ -81,12 +88,33 @@ __global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__rest
   for (int i = 0; i < per; ++i) {
     const int t = tid * per + i;
     if (t < ntiles) {
-      start[t] = run;
-      cursor[t] = run;
-      run += count[t];
+      out[t] = run;
+      run += val(t);
     }
   }
-  if (tid == 1023) start[ntiles] = part[1023];
+  if (tid == 1023) out[ntiles] = part[1023];
+  __syncthreads();
+}
+
+// Tile starts (start, and the scatter cursor) and subproblem starts
+// (substart: tile t owns subproblems substart[t] .. substart[t + 1] - 1).
+__global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__restrict__ start, int *__restrict__ cursor,
+                          int *__restrict__ substart) {
+  __shared__ int part[1024];
+  block_scan(ntiles, [&](int t) { return count[t]; }, start, part);
+  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) cursor[t] = start[t];
+  block_scan(ntiles, [&](int t) { return (count[t] + NU_SUB - 1) / NU_SUB; }, substart, part);
+}
+
+// Subproblem s -> (tile, first point): one thread per tile.
+__global__ void k_nu_subs(int ntiles, const int *__restrict__ start, const int *__restrict__ substart,
+                          int *__restrict__ sub_tile, int *__restrict__ sub_p0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  for (int s = substart[t], p = start[t]; s < substart[t + 1]; ++s, p += NU_SUB) {
+    sub_tile[s] = t;
+    sub_p0[s] = p;
+  }
 }
 
 __global__ void k_nu_scatter(int64_t M, const int *__restrict__ tile, int *__restrict__ cursor, int *__restrict__ order) {
@@ -106,20 +134,28 @@ __global__ void k_nu_zero(int64_t n, cd *__restrict__ g) {
   if (i < n) g[i] = cd{0.0, 0.0};
 }
 
-// One block per tile: spread the tile's points into LDS (tile + halo), then
-// add the LDS tile to the periodic fine grid.
+// One block per (subproblem, transform): spread the subproblem's points
+// into LDS (its tile + halo), then add the LDS tile to the transform's
+// periodic fine grid.  blockIdx.x = subproblem (blocks past the total exit),
+// blockIdx.y = transform (strengths c + y * c_stride, grid + y * g_stride).
 template <int W>
 __global__ __launch_bounds__(256) void k_nu_spread(int nf, int nt, double beta, const int *__restrict__ start,
-                                                   const int *__restrict__ order, const double *__restrict__ uv,
-                                                   const cd *__restrict__ c, cd *__restrict__ grid) {
+                                                   const int *__restrict__ substart, const int *__restrict__ sub_tile,
+                                                   const int *__restrict__ sub_p0, const int *__restrict__ order,
+                                                   const double *__restrict__ uv, const cd *__restrict__ c,
+                                                   int64_t c_stride, cd *__restrict__ grid, int64_t g_stride) {
   constexpr int H = (W + 1) / 2;
   constexpr int S = NU_TS + 2 * H;
   __shared__ double tre[S * S], tim[S * S];
-  const int t = blockIdx.x;
+  const int sub = blockIdx.x;
+  if (sub >= substart[nt * nt]) return;
+  const int t = sub_tile[sub];
   const int tx = t % nt, ty = t / nt;
+  c += blockIdx.y * c_stride;
+  grid += blockIdx.y * g_stride;
   for (int i = threadIdx.x; i < S * S; i += blockDim.x) { tre[i] = 0.0; tim[i] = 0.0; }
   __syncthreads();
-  const int p0 = start[t], p1 = start[t + 1];
+  const int p0 = sub_p0[sub], p1 = min(p0 + NU_SUB, start[t + 1]);
   const double inv_half = 2.0 / W;
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int j = order[p];
@@ -165,6 +201,8 @@ template <int NF>
 __global__ __launch_bounds__(1024) void k_nu_rows(const cd *__restrict__ grid, cd *__restrict__ T1, int N1,
                                                   const cd *__restrict__ tw) {
   constexpr int YT = XTile<NF>::YT;
+  grid += (int64_t)blockIdx.y * NF * NF;  // transform blockIdx.y
+  T1 += (int64_t)blockIdx.y * N1 * NF;
   constexpr int TPR = NF / 8;
   extern __shared__ cd lds[];
   const int v0 = blockIdx.x * YT, tid = threadIdx.x;
@@ -188,6 +226,8 @@ __global__ __launch_bounds__(1024) void k_nu_cols(const cd *__restrict__ T1, cd 
                                                   const double *__restrict__ d1, const double *__restrict__ d2,
                                                   const cd *__restrict__ tw) {
   constexpr int YT = XTile<NF>::YT;
+  T1 += (int64_t)blockIdx.y * N1 * NF;  // transform blockIdx.y
+  f += (int64_t)blockIdx.y * N1 * N2;
   constexpr int TPR = NF / 8;
   extern __shared__ cd lds[];
   const int r0 = blockIdx.x * YT, tid = threadIdx.x;
@@ -218,50 +258,59 @@ int nu_width(double eps) {
 }
 
 template <int NF>
-static hipError_t run_ffts(const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
+static hipError_t run_ffts(int nb, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
                            const cd *tw, hipStream_t s) {
   constexpr int YT = XTile<NF>::YT;
   const size_t sh = sizeof(cd) * YT * LdsRow<NF>::S;
-  hipLaunchKernelGGL(k_nu_rows<NF>, dim3(NF / YT), dim3(YT * NF / 8), sh, s, grid, T1, N1, tw);
+  hipLaunchKernelGGL(k_nu_rows<NF>, dim3(NF / YT, nb), dim3(YT * NF / 8), sh, s, grid, T1, N1, tw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nu_cols<NF>, dim3((N1 + YT - 1) / YT), dim3(YT * NF / 8), sh, s, (const cd *)T1, f, N1, N2,
-                     d1, d2, tw);
+  hipLaunchKernelGGL(k_nu_cols<NF>, dim3((N1 + YT - 1) / YT, nb), dim3(YT * NF / 8), sh, s, (const cd *)T1, f, N1,
+                     N2, d1, d2, tw);
   return hipGetLastError();
 }
 
+int nu_max_subproblems(int64_t M, int nf) {
+  const int nt = nf / NU_TS;
+  return nt * nt + (int)((M + NU_SUB - 1) / NU_SUB);
+}
+
 hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
-                             int *start, int *cursor, int *order, hipStream_t s) {
+                             int *start, int *cursor, int *substart, int *sub_tile, int *sub_p0, int *order,
+                             hipStream_t s) {
   const int nt = nf / NU_TS;
   const int ntiles = nt * nt;
   hipLaunchKernelGGL(k_nu_zero_int, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, count);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess || M == 0) {
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor);
-      e = hipGetLastError();
-    }
-    return e;
+  if (e != hipSuccess) return e;
+  if (M > 0) {
+    hipLaunchKernelGGL(k_nu_bin, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, x, y, nf, nt, uv, tile,
+                       count);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_nu_bin, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, x, y, nf, nt, uv, tile, count);
+  hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor, substart);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_nu_subs, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, (const int *)start,
+                     (const int *)substart, sub_tile, sub_p0);
+  if ((e = hipGetLastError()) != hipSuccess || M == 0) return e;
   hipLaunchKernelGGL(k_nu_scatter, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, (const int *)tile, cursor,
                      order);
   return hipGetLastError();
 }
 
-hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, const int *order, const double *uv,
-                               const cd *c, cd *grid, hipStream_t s) {
+hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, const int *start, const int *substart,
+                               const int *sub_tile, const int *sub_p0, const int *order, const double *uv,
+                               const cd *c, int64_t c_stride, cd *grid, hipStream_t s) {
   const int nt = nf / NU_TS;
   const int64_t ng = (int64_t)nf * nf;
-  hipLaunchKernelGGL(k_nu_zero, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, s, ng, grid);
+  hipLaunchKernelGGL(k_nu_zero, dim3((unsigned)((nb * ng + 255) / 256)), dim3(256), 0, s, nb * ng, grid);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-#define YTF_SPREAD(WW)                                                                                             \
-  case WW:                                                                                                         \
-    hipLaunchKernelGGL(k_nu_spread<WW>, dim3(nt * nt), dim3(256), 0, s, nf, nt, beta, start, order, uv, c, grid); \
+  if (e != hipSuccess || M == 0) return e;
+  const dim3 g(nu_max_subproblems(M, nf), nb);
+#define YTF_SPREAD(WW)                                                                                         \
+  case WW:                                                                                                     \
+    hipLaunchKernelGGL(k_nu_spread<WW>, g, dim3(256), 0, s, nf, nt, beta, start, substart, sub_tile, sub_p0, \
+                       order, uv, c, c_stride, grid, ng);                                                      \
     break;
   switch (w) {
     YTF_SPREAD(2) YTF_SPREAD(4) YTF_SPREAD(6) YTF_SPREAD(8) YTF_SPREAD(10) YTF_SPREAD(12) YTF_SPREAD(14)
@@ -272,16 +321,16 @@ hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, con
   return hipGetLastError();
 }
 
-hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
-                            const cd *tw, hipStream_t s) {
+hipError_t launch_nufft_fft(int nb, int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1,
+                            const double *d2, const cd *tw, hipStream_t s) {
   switch (nf) {
-    case 64: return run_ffts<64>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 128: return run_ffts<128>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 256: return run_ffts<256>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 512: return run_ffts<512>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 1024: return run_ffts<1024>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 2048: return run_ffts<2048>(grid, T1, f, N1, N2, d1, d2, tw, s);
-    case 4096: return run_ffts<4096>(grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 64: return run_ffts<64>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 128: return run_ffts<128>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 256: return run_ffts<256>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 512: return run_ffts<512>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 1024: return run_ffts<1024>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 2048: return run_ffts<2048>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
+    case 4096: return run_ffts<4096>(nb, grid, T1, f, N1, N2, d1, d2, tw, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -130,12 +130,21 @@ hipError_t launch_mlp_reduce(int64_t n, int splits, int64_t stride, const float 
 
 // 2D type-1 nonuniform FFT (nufft.hip).
 int nu_width(double eps);
+// Upper bound on the spreading subproblems of M points on an nf x nf fine grid.
+int nu_max_subproblems(int64_t M, int nf);
+// Sort (once per point set): fine-grid coordinates, tile bins, the sorted
+// order and the subproblem table (substart: ntiles + 1, sub_tile / sub_p0:
+// nu_max_subproblems entries).
 hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
-                             int *start, int *cursor, int *order, hipStream_t s);
-hipError_t launch_nufft_spread(int nf, int w, double beta, const int *start, const int *order, const double *uv,
-                               const cd *c, cd *grid, hipStream_t s);
-hipError_t launch_nufft_fft(int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1, const double *d2,
-                            const cd *tw, hipStream_t s);
+                             int *start, int *cursor, int *substart, int *sub_tile, int *sub_p0, int *order,
+                             hipStream_t s);
+// nb transforms at once: strengths c + t * c_stride, fine grids grid + t * nf^2.
+hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, const int *start, const int *substart,
+                               const int *sub_tile, const int *sub_p0, const int *order, const double *uv,
+                               const cd *c, int64_t c_stride, cd *grid, hipStream_t s);
+// nb transforms at once: grids grid + t * nf^2, T1 + t * N1 nf, f + t * N1 N2.
+hipError_t launch_nufft_fft(int nb, int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1,
+                            const double *d2, const cd *tw, hipStream_t s);
 
 // 2D lateral sum and the padded Green operator (operator.hip).
 hipError_t launch_lat_points(int64_t n, const double *kx, const double *ky, const cd *W, double px, double py,
