@@ -259,6 +259,25 @@ int ytf_mlp_loss_grad(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev,
  * (before the step) is copied to loss_dev when not NULL. */
 int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, const float *y_dev, float lr,
                        float *loss_dev);
+/* Surrogate evaluation of the scattered spectral dyadic above the stack, with
+ * the exact solve as fallback (no reference counterpart: the reference runs
+ * the exact solve, build_cavity_transfer_2d + closed_form_G's
+ * terms=:scattered, on every node).  One kernel evaluates the network on
+ * inputs (kx/omega, ky/omega, xconst[0 .. n_in-3]) -- n_in >= 2, 18 outputs,
+ * hidden width 128 or 256 -- and forms, per node, in FP64,
+ *     g~_s = t omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0,
+ *     t_e = y_e + i y_{9+e},  kz0 = sqrt(omega^2 - k^2) (Im >= 0),
+ * into ghat_dev [9][n] complex (entry c + 3q, as ytf_green_nodes).  A node is
+ * flagged (flags_dev[i] = 1) when |k - omega| < branch_tol omega (the air
+ * branch point), k < origin_tol omega, or |y|_2 > t_big (near a pole).  With
+ * fallback, the flagged nodes are solved exactly (ytf_green_nodes with source
+ * and observation layer 1, depths zsrc / zobs, YTF_TERMS_SCATTERED) and their
+ * g~ replaced.  *n_flagged = number of flagged nodes.  Synchronizes the
+ * context's stream once (to size the fallback). */
+int ytf_mlp_surrogate_ghat(ytf_ctx *ctx, ytf_mlp *mlp, const ytf_layer *layers, int n_layers, double omega,
+                           const double *kx_dev, const double *ky_dev, int64_t n, const float *xconst, double zsum,
+                           double zsrc, double zobs, double branch_tol, double origin_tol, double t_big, int fallback,
+                           double *ghat_dev, unsigned char *flags_dev, int64_t *n_flagged);
 
 /* 2D type-1 nonuniform FFT (FINUFFT's type 1 with iflag +1, the transform of
  * closed_form_G's 2D lateral sum with lateral=:nufft,

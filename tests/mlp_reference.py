@@ -74,3 +74,14 @@ def loss_grad(p, shp, x, y, in_shift, in_scale, out_shift, out_scale, emulate_bf
             dZ = r(dH * dsilu(Z[l - 1])).astype(np.float64)
     flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return L, flat
+
+
+def forward(p, shp, x, in_shift, in_scale, out_shift, out_scale, emulate_bf16=True, kz_features=False):
+    """De-normalised network outputs y (n, n_out), the forward half of loss_grad."""
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, np.float64))
+    layers = unflatten(np.asarray(p, np.float64), shp)
+    H = r(features(x, in_shift, in_scale, kz_features)).astype(np.float64)
+    for l, (W, b) in enumerate(layers):
+        z = H @ r(W).astype(np.float64).T + b
+        H = r(silu(z)).astype(np.float64) if l < len(layers) - 1 else z
+    return H * np.float64(out_scale) + np.float64(out_shift)

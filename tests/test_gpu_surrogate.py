@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from mlp_reference import loss_grad, shapes
+from mlp_reference import loss_grad, shapes, forward
 
 torch = pytest.importorskip("torch")
 
@@ -53,6 +53,31 @@ def test_gradients_against_cpu_reference(width, depth, n, kzf):
     yp = m.forward(torch.as_tensor(x, device="cuda")).cpu().numpy()
     Lf = np.mean(((yp - osh) / osc - (y - osh) / osc) ** 2)
     assert abs(Lf - L) <= 1e-3 * L
+
+
+@pytest.mark.parametrize("width,depth,n,kzf", [(256, 4, 1000, True), (256, 4, 333, False), (128, 3, 513, True),
+                                               (64, 2, 200, False)])
+def test_forward_against_cpu_reference(width, depth, n, kzf):
+    """ytf_mlp_forward (one fused kernel for hidden widths 128 and 256, the
+    layer products otherwise) against the float64 reference with the bf16
+    rounding points reproduced, on row counts that are not multiples of the
+    kernel's 128-row blocks."""
+    print(f"case: mlp forward width={width} depth={depth} n={n} kz_features={kzf}", flush=True)
+    from yellowthinfilms_amd.surrogate import MLP
+    rng = np.random.default_rng(width * depth + n)
+    m = MLP(6, width, depth, 18, seed=11, kz_features=kzf)
+    p = m.get_params()
+    p += rng.normal(0, 0.02, p.shape).astype(np.float32)
+    m.set_params(p)
+    ish, isc = rng.normal(size=6).astype(np.float32), rng.uniform(0.5, 2, 6).astype(np.float32)
+    osh, osc = rng.normal(size=18).astype(np.float32), rng.uniform(0.5, 2, 18).astype(np.float32)
+    m.set_normalization(ish, isc, osh, osc)
+    x = rng.normal(size=(n, 6)).astype(np.float32)
+    y = m.forward(torch.as_tensor(x, device="cuda")).cpu().numpy()
+    yr = forward(p, shapes(6 + 2 * kzf, width, depth, 18), x, ish, isc, osh, osc, kz_features=kzf)
+    err = np.linalg.norm(y - yr) / np.linalg.norm(yr)
+    print(f"  rel err {err:.2e}", flush=True)
+    assert err < 2e-3, err
 
 
 def test_training_fits_a_smooth_function():
@@ -110,3 +135,47 @@ def test_surrogate_of_the_scattered_dyadic_with_exact_fallback():
     assert e1 < e0 / 10
     assert torch.equal(gf[maskf], exact[maskf])
     assert ef < 0.5
+
+
+def test_fused_surrogate_matches_the_composed_path():
+    """ytf_mlp_surrogate_ghat (network, exact factors and flags in one kernel,
+    then the exact solve of the flagged nodes) against the same steps taken
+    one by one: ytf_mlp_forward, ghat_from_target and flags on the host side
+    of torch, and the exact scattered solve."""
+    print("case: fused surrogate vs composed path", flush=True)
+    from yellowthinfilms_amd.surrogate import GyroFamily, SurrogateGreen, ghat_from_target, NM
+    fam = GyroFamily(seed=1)
+    sg = SurrogateGreen(fam, seed=7)
+    rng = np.random.default_rng(4)
+    p = sg.mlp.get_params()
+    p += rng.normal(0, 0.05, p.shape).astype(np.float32)
+    sg.mlp.set_params(p)
+    sh, sc = fam.input_normalization()
+    sg.mlp.set_normalization(sh, sc, rng.normal(0, 0.3, 18).astype(np.float32),
+                             rng.uniform(0.2, 1.0, 18).astype(np.float32))
+    sg.t_ref = 1.0
+    omega, loss, hs, ho = 2 * math.pi * 1.03, 0.2, 35.0, 60.0
+    k = np.linspace(-2.95, 2.95, 77) * omega
+    KX, KY = np.meshgrid(k, k, indexing="ij")
+    kx = torch.as_tensor(KX.ravel(), device="cuda")
+    ky = torch.as_tensor(KY.ravel(), device="cuda")
+    n = kx.shape[0]
+    x = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+    x[:, 0] = (kx / omega).float()
+    x[:, 1] = (ky / omega).float()
+    x[:, 2], x[:, 3], x[:, 4], x[:, 5] = hs, ho, omega / (2 * math.pi), loss
+    t = sg.mlp.forward(x)
+    g_ref = ghat_from_target(t, kx, ky, hs * NM, ho * NM, omega)
+    mask_ref = sg.flags(kx, ky, t, omega, pole_factor=2.0)
+    g, mask = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=False, pole_factor=2.0)
+    assert torch.equal(mask, mask_ref)
+    assert 0 < int(mask.sum()) < n // 4
+    ok = torch.isfinite(g_ref).all(dim=1)
+    err = float(torch.linalg.norm((g - g_ref)[ok]) / torch.linalg.norm(g_ref[ok]))
+    print(f"  flagged {int(mask.sum())} of {n}; rel err vs composed {err:.2e}", flush=True)
+    assert err < 1e-12, err
+    gf, maskf = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_factor=2.0)
+    assert sg.last_flagged == int(maskf.sum())
+    exact = sg.exact_scattered(kx[maskf], ky[maskf], hs, ho, omega, loss)
+    assert torch.equal(gf[maskf], exact)
+    assert torch.equal(gf[~maskf], g[~maskf])

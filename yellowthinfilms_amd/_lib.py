@@ -101,6 +101,10 @@ _SIGS = {
                                      ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
     "ytf_mlp_train_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_float,
                                       ct.c_void_p]),
+    "ytf_mlp_surrogate_ghat": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double,
+                                          ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_float), ct.c_double,
+                                          ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_int,
+                                          ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
     "ytf_nufft2d1": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
                                 ct.c_int, ct.c_double, ct.c_void_p]),
     "ytf_closed_form_G_2d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,

@@ -617,6 +617,8 @@ struct ytf_mlp {
   size_t np = 0, nflat = 0;
   float *P = nullptr, *Mo = nullptr, *Vo = nullptr, *G = nullptr;  // FP32 master, Adam moments, gradient
   __bf16 *P16 = nullptr;                                            // bf16 copy of P
+  __bf16 *P16pk = nullptr;  // weights of P16 in matrix-core fragment order (the fused forward)
+  uint64_t version = 1, packed_version = 0;  // P16 changes / the version P16pk was packed from
   float *norm = nullptr;  // in_shift[nin] in_scale[nin] out_shift[nout] out_scale[nout]
   float *loss = nullptr;
   long step = 0;
@@ -628,7 +630,7 @@ struct ytf_mlp {
 static size_t ru64(size_t x) { return (x + 63) / 64 * 64; }
 
 static int mlp_free(ytf_mlp *m) {
-  void *ps[] = {m->P, m->Mo, m->Vo, m->G, m->P16, m->norm, m->loss, m->act};
+  void *ps[] = {m->P, m->Mo, m->Vo, m->G, m->P16, m->P16pk, m->norm, m->loss, m->act};
   for (void *q : ps)
     if (q) hipFree(q);
   delete m;
@@ -674,6 +676,7 @@ int ytf_mlp_create(ytf_ctx *c, int n_in, int width, int depth, int n_out, int fl
   if (e == hipSuccess) e = hipMalloc(&m->Vo, fb);
   if (e == hipSuccess) e = hipMalloc(&m->G, fb);
   if (e == hipSuccess) e = hipMalloc(&m->P16, sizeof(__bf16) * m->np);
+  if (e == hipSuccess) e = hipMalloc(&m->P16pk, sizeof(__bf16) * m->np);
   if (e == hipSuccess) e = hipMalloc(&m->norm, sizeof(float) * 2 * (n_in + n_out));
   if (e == hipSuccess) e = hipMalloc(&m->loss, sizeof(float));
   if (e != hipSuccess) {
@@ -751,6 +754,7 @@ int ytf_mlp_set_params(ytf_ctx *c, ytf_mlp *m, const float *host) {
   HIPCHK(c, hipMemcpy(m->Mo, zero.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(m->Vo, zero.data(), sizeof(float) * m->np, hipMemcpyHostToDevice));
   m->step = 0;
+  m->version += 1;
   hipError_t e = ytf::launch_mlp_to_bf16((int64_t)m->np, m->P, m->P16, c->stream);
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_set_params: %s", hipGetErrorString(e));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -946,6 +950,7 @@ int ytf_mlp_train_step(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, co
   int rc = mlp_loss_grad_locked(c, m, n, x_dev, y_dev, "ytf_mlp_train_step");
   if (rc) return rc;
   m->step += 1;
+  m->version += 1;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
   hipError_t e = timed(c, "mlp_adam", [&] {
@@ -956,12 +961,54 @@ int ytf_mlp_train_step(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, co
   return YTF_OK;
 }
 
+static ytf::MlpFused mlp_fused_params(const ytf_mlp *m, int64_t n) {
+  ytf::MlpFused p{};
+  p.n = n;
+  p.nin = m->nin;
+  p.kzf = m->kzf;
+  p.nl = m->nl;
+  p.nout = m->nout;
+  p.kp0 = m->Kp[0];
+  p.W16 = m->P16pk;
+  p.P32 = m->P;
+  for (int l = 0; l < m->nl; ++l) {
+    p.offW[l] = (int64_t)m->offW[l];
+    p.offb[l] = (int64_t)m->offb[l];
+  }
+  p.norm = m->norm;
+  return p;
+}
+
+static bool mlp_fused_ok(const ytf_mlp *m) {
+  return ytf::mlp_fused_supported(m->width, m->nin + 2 * m->kzf, m->nout, m->nl);
+}
+
+// Repack the weights into fragment order when they changed since the last pack.
+static hipError_t mlp_pack(ytf_ctx *c, ytf_mlp *m) {
+  if (m->packed_version == m->version) return hipSuccess;
+  hipError_t e = hipSuccess;
+  for (int l = 0; l < m->nl && e == hipSuccess; ++l)
+    e = ytf::launch_mlp_pack(m->Np[l], m->Kp[l], m->P16 + m->offW[l], m->P16pk + m->offW[l], c->stream);
+  if (e == hipSuccess) m->packed_version = m->version;
+  return e;
+}
+
 int ytf_mlp_forward(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, float *y_dev) {
   if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_forward: NULL argument");
   std::lock_guard<std::mutex> lk(c->mu);
   if (n < 0 || (n > 0 && (!x_dev || !y_dev))) return fail(c, YTF_EINVAL, "ytf_mlp_forward: bad arguments");
   if (n == 0) return YTF_OK;
   HIPCHK(c, hipSetDevice(c->device));
+  if (mlp_fused_ok(m)) {  // the whole network in one kernel, activations in LDS
+    ytf::MlpFused p = mlp_fused_params(m, n);
+    p.x = x_dev;
+    p.y = y_dev;
+    hipError_t e = mlp_pack(c, m);
+    if (e == hipSuccess)
+      e = timed(c, "mlp_fused", [&] { return ytf::launch_mlp_fused(p, m->width, false, c->stream); });
+    if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_forward: launch failed: %s", hipGetErrorString(e));
+    return YTF_OK;
+  }
   const int64_t chunk = 1 << 18;
   const int64_t rows_cap = std::min<int64_t>((int64_t)ru64(n), chunk);
   MlpAct A;
@@ -975,6 +1022,67 @@ int ytf_mlp_forward(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, float
     if (e == hipSuccess) e = ytf::launch_mlp_outputs(nc, m->nout, m->Np[L], A.Y, osh, osc, y_dev + r0 * m->nout, c->stream);
     if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_forward: launch failed: %s", hipGetErrorString(e));
   }
+  return YTF_OK;
+}
+
+int ytf_mlp_surrogate_ghat(ytf_ctx *c, ytf_mlp *m, const ytf_layer *layers, int n_layers, double omega,
+                           const double *kx_dev, const double *ky_dev, int64_t n, const float *xconst, double zsum,
+                           double zsrc, double zobs, double branch_tol, double origin_tol, double t_big, int fallback,
+                           double *ghat_dev, unsigned char *flags_dev, int64_t *n_flagged) {
+  if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n < 0 || n > INT32_MAX || (n > 0 && (!kx_dev || !ky_dev || !ghat_dev || !flags_dev)) ||
+      (m->nin > 2 && !xconst) || !n_flagged)
+    return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: bad arguments");
+  if (m->nout != 18 || m->nin < 2 || !mlp_fused_ok(m))
+    return fail(c, YTF_EINVAL,
+                "ytf_mlp_surrogate_ghat: needs 18 outputs, >= 2 inputs and a hidden width of 128 or 256 (got %d, %d, %d)",
+                m->nout, m->nin, m->width);
+  if (fallback && (!layers || n_layers < 2)) return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: no stack for the fallback");
+  *n_flagged = 0;
+  if (n == 0) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  // flagged-row counter and list
+  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, align256(sizeof(int)) + align256(sizeof(int) * (size_t)n));
+  if (rc) return rc;
+  int *count = (int *)c->aux;
+  int *list = (int *)((char *)c->aux + align256(sizeof(int)));
+  ytf::MlpFused p = mlp_fused_params(m, n);
+  p.kx = kx_dev;
+  p.ky = ky_dev;
+  for (int i = 0; i + 2 < m->nin; ++i) p.xconst[i] = xconst[i];
+  p.omega = omega;
+  p.zsum = zsum;
+  p.branch_tol = branch_tol;
+  p.origin_tol = origin_tol;
+  p.tbig = t_big;
+  p.ghat = ghat_dev;
+  p.flags = flags_dev;
+  p.n_flagged = count;
+  p.flagged = list;
+  hipError_t e = ytf::launch_mlp_zero(1, (float *)count, c->stream);  // zero bits = integer 0
+  if (e == hipSuccess) e = mlp_pack(c, m);
+  if (e == hipSuccess)
+    e = timed(c, "mlp_fused", [&] { return ytf::launch_mlp_fused(p, m->width, true, c->stream); });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
+  int nf = 0;
+  HIPCHK(c, hipMemcpyAsync(&nf, count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *n_flagged = nf;
+  if (!fallback || nf == 0) return YTF_OK;
+  // the exact FP64 solve (scattered terms, source and observation in layer 1) on the flagged rows
+  const size_t kb = align256(sizeof(double) * nf), gb = align256(sizeof(cd) * 9 * (size_t)nf);
+  rc = ensure_buf(c, &c->work, &c->work_bytes, 2 * kb + gb);
+  if (rc) return rc;
+  double *kxc = (double *)c->work, *kyc = (double *)((char *)c->work + kb);
+  double *gc = (double *)((char *)c->work + 2 * kb);
+  e = ytf::launch_mlp_gather_nodes(nf, list, kx_dev, ky_dev, kxc, kyc, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
+  rc = green_nodes_locked(c, layers, n_layers, omega, kxc, kyc, nf, 1, 1, 1, &zsrc, &zobs, gc, nullptr,
+                          YTF_TERMS_SCATTERED, 0.0, 1);
+  if (rc) return rc;
+  e = ytf::launch_mlp_scatter_rows(nf, list, gc, ghat_dev, n, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
 

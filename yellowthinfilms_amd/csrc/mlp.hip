@@ -25,9 +25,14 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 
 constexpr int TM = 64, TN = 64, TK = 32, LK = TK + 8;
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative from the hardware exp2 and reciprocal (1 ulp each;
+// the results are rounded to bf16): sigma(z) = 1 / (1 + 2^(-z log2 e)).
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -279,6 +284,335 @@ hipError_t launch_mlp_reduce(int64_t n, int splits, int64_t stride, const float 
 hipError_t launch_mlp_zero(int64_t n, float *x, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_mlp_zero, dim3(blocks_for(n)), dim3(256), 0, s, n, x);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// Whole-network forward in one kernel.  A block takes FB = 128 rows; their
+// activations stay in LDS as bf16 [row][feature] (row stride WIDTH + 8).
+// Every layer is computed transposed, C^T = W H^T, on
+// v_mfma_f32_32x32x16_bf16 with the weights (A operand) read from global
+// memory in fragment order (launch_mlp_pack: one coalesced 1 KiB load per wave,
+// 32-feature tile and 16-wide k step, two steps ahead; L2-resident, since
+// every block reads the same weights) and the activations (B operand) from LDS: the accumulator of a lane then holds 4
+// consecutive features of one row, stored back as one 8-byte LDS write.
+// Hidden layer: wave w owns features [w W/4, (w + 1) W/4) (W/128 tiles of 32)
+// for all 4 row tiles; output layer: ceil(nout / 32) feature tiles spread
+// over the waves.  The sums run over k in 16-wide steps in increasing order,
+// as in k_gemm, and the epilogues (bias, SiLU, output scaling) are the same.
+// Two variants:
+//   generic:   x[n][nin] (normalised in-kernel, light-line features) -> y[n][nout];
+//   surrogate: inputs (kx/omega, ky/omega, constants); the 18 outputs t go
+//              through LDS to one thread per row, which forms
+//              g~_s = t omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0 in FP64
+//              ([9][n] complex, entry c + 3q), and flags the row (|k - omega| <
+//              branch_tol omega, k < origin_tol omega, or |t|_2 > tbig) into
+//              the flag array and the flagged-row list.
+constexpr int FB = 128;
+
+typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+
+// Epilogue of one 4-feature group of a hidden layer: h = bf16(SiLU(z + b)) for
+// features f .. f + 3 of LDS row `row` (one 8-byte store).
+template <int LA>
+__device__ __forceinline__ void epi4(__bf16 *act, int row, int f, const f16v &z, int g, const float *b) {
+  b4 h;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) h[e] = (__bf16)silu(z[4 * g + e] + b[f + e]);
+  *(b4 *)(act + row * LA + f) = h;
+}
+
+// One half-layer phase of the fused forward.  The wave's accumulators accM
+// (FT feature tiles x the 2 row tiles of half `hm`) take the products of the
+// layer whose packed weights start at Wp (K = WIDTH, WIDTH / 16 steps), while
+// the epilogue of the other half's finished accumulators accE (bias bE, rows
+// of half `he`) is spread over the steps, one 4-feature group per step, so
+// the SiLU arithmetic overlaps the matrix-core work.  do_epi = false: products
+// only.
+template <int WIDTH, int FT, int LA>
+__device__ __forceinline__ void half_phase(const b8 *Wp, const __bf16 *actr, f16v (&accM)[FT][2], int hm,
+                                           f16v (&accE)[FT][2], int he, const float *bE, bool do_epi, __bf16 *act,
+                                           int f0, int lr, int lk, int hi) {
+  constexpr int NKS = WIDTH / 16;
+  static_assert(NKS == FT * 2 * 4, "one epilogue group per k step");
+#pragma unroll
+  for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) accM[ft][j][e] = 0.0f;
+  b8 a0[FT], a1[FT];
+#pragma unroll
+  for (int ft = 0; ft < FT; ++ft) {
+    a0[ft] = Wp[ft * NKS * 64];
+    a1[ft] = Wp[(ft * NKS + 1) * 64];
+  }
+#pragma unroll
+  for (int k = 0; k < NKS; ++k) {
+    b8 a2[FT], bb[2];
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft) a2[ft] = k + 2 < NKS ? Wp[(ft * NKS + k + 2) * 64] : a1[ft];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) bb[j] = *(const b8 *)(actr + ((2 * hm + j) * 32 + lr) * LA + 16 * k + lk);
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        accM[ft][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[ft], bb[j], accM[ft][j], 0, 0, 0);
+    if (do_epi) {  // group k of the other half: tile (k / 8, (k / 4) % 2), rows 8 g + 4 hi .. of g = k % 4
+      const int ft = k / 8, j = (k / 4) % 2, g = k % 4;
+      epi4<LA>(act, (2 * he + j) * 32 + lr, f0 + ft * 32 + 8 * g + 4 * hi, accE[ft][j], g, bE);
+    }
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft) {
+      a0[ft] = a1[ft];
+      a1[ft] = a2[ft];
+    }
+  }
+}
+
+// The full epilogue of one half (no overlapping products).
+template <int FT, int LA>
+__device__ __forceinline__ void half_epilogue(__bf16 *act, f16v (&acc)[FT][2], int h, const float *b, int f0, int lr,
+                                              int hi) {
+#pragma unroll
+  for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) epi4<LA>(act, (2 * h + j) * 32 + lr, f0 + ft * 32 + 8 * g + 4 * hi, acc[ft][j], g, b);
+}
+
+template <int WIDTH, bool SUR>
+__global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
+  constexpr int LA = WIDTH + 8;
+  constexpr int FT = WIDTH / 128;
+  constexpr int TS = 20;  // floats per row of the surrogate's output staging
+  static_assert(FB * TS * 4 <= FB * LA * 2, "output staging fits the activations");
+  __shared__ __bf16 act[FB * LA];
+  __shared__ float bsh[(MLP_FUSED_MAX_LAYERS - 1) * WIDTH];  // hidden-layer biases (read without vmcnt waits)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 31, hi = lane >> 5, lk = hi * 8;
+  const int64_t r0 = (int64_t)blockIdx.x * FB;
+  const float *ish = p.norm, *isc = p.norm + p.nin;
+  const float *osh = p.norm + 2 * p.nin, *osc = osh + p.nout;
+  const int kin = p.nin + 2 * p.kzf;
+  const int k0e = (kin + 15) / 16 * 16;
+
+  for (int e = tid; e < (p.nl - 1) * WIDTH; e += 256) bsh[e] = p.P32[p.offb[e / WIDTH] + e % WIDTH];
+  // layer-0 inputs, zero padded to k0e columns
+  for (int e = tid; e < FB * k0e; e += 256) {
+    const int r = e / k0e, c = e % k0e;
+    const int64_t row = r0 + r;
+    float v = 0.0f;
+    if (row < p.n && c < kin) {
+      float x0, x1;
+      if (SUR) {
+        x0 = (float)(p.kx[row] / p.omega);
+        x1 = (float)(p.ky[row] / p.omega);
+      } else {
+        x0 = p.x[row * p.nin];
+        x1 = p.nin > 1 ? p.x[row * p.nin + 1] : 0.0f;
+      }
+      if (c < p.nin) {
+        const float xc = c == 0 ? x0 : (c == 1 ? x1 : (SUR ? p.xconst[c - 2] : p.x[row * p.nin + c]));
+        v = (xc - ish[c]) * isc[c];
+      } else {
+        const float u2 = x0 * x0 + x1 * x1;
+        v = c == p.nin ? sqrtf(fmaxf(1.0f - u2, 0.0f)) : sqrtf(fmaxf(u2 - 1.0f, 0.0f));
+      }
+    }
+    act[r * LA + c] = (__bf16)v;
+  }
+  __syncthreads();
+
+  // hidden layers, in two halves of 64 rows (A: row tiles 0-1, B: 2-3) so
+  // that the epilogue of one half overlaps the products of the other:
+  //   layer 0: products of A and B; epilogue A;
+  //   layer l >= 1: [products l of A | epilogue l-1 of B], [products l of B | epilogue l of A];
+  //   then the epilogue of B of the last hidden layer.
+  // Every bracket ends with a barrier: a half's rows are rewritten only after
+  // every wave has read them, and read only after every wave has written them.
+  const int f0 = wave * (WIDTH / 4);
+  f16v accA[FT][2], accB[FT][2];
+  {
+    const int ksl = p.kp0 / 16, nks = k0e / 16;
+    const b8 *Wp = (const b8 *)(p.W16 + p.offW[0]) + (int64_t)(f0 / 32) * ksl * 64 + lane;
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) accA[ft][j][e] = accB[ft][j][e] = 0.0f;
+    for (int k = 0; k < nks; ++k) {
+      b8 a[FT], bb[4];
+#pragma unroll
+      for (int ft = 0; ft < FT; ++ft) a[ft] = Wp[((int64_t)ft * ksl + k) * 64];
+#pragma unroll
+      for (int st = 0; st < 4; ++st) bb[st] = *(const b8 *)(act + (st * 32 + lr) * LA + 16 * k + lk);
+#pragma unroll
+      for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accA[ft][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ft], bb[j], accA[ft][j], 0, 0, 0);
+          accB[ft][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ft], bb[2 + j], accB[ft][j], 0, 0, 0);
+        }
+    }
+  }
+  __syncthreads();
+  half_epilogue<FT, LA>(act, accA, 0, bsh, f0, lr, hi);
+  __syncthreads();
+  for (int l = 1; l < p.nl - 1; ++l) {
+    const b8 *Wp = (const b8 *)(p.W16 + p.offW[l]) + (int64_t)(f0 / 32) * (WIDTH / 16) * 64 + lane;
+    half_phase<WIDTH, FT, LA>(Wp, act, accA, 0, accB, 1, bsh + (l - 1) * WIDTH, true, act, f0, lr, lk, hi);
+    __syncthreads();
+    half_phase<WIDTH, FT, LA>(Wp, act, accB, 1, accA, 0, bsh + l * WIDTH, true, act, f0, lr, lk, hi);
+    __syncthreads();
+  }
+  half_epilogue<FT, LA>(act, accB, 1, bsh + (p.nl - 2) * WIDTH, f0, lr, hi);
+  __syncthreads();
+
+  // output layer
+  {
+    const int l = p.nl - 1;
+    const __bf16 *W = p.W16 + p.offW[l];
+    const float *b = p.P32 + p.offb[l];
+    const int nt = (p.nout + 31) / 32;
+    const int ft = nt == 1 ? 0 : (wave & 1);
+    const int st0 = nt == 1 ? wave : 2 * (wave >> 1);
+    const int nst = nt == 1 ? 1 : 2;
+    f16v acc[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[j][e] = 0.0f;
+    const b8 *Wp = (const b8 *)W + (int64_t)ft * (WIDTH / 16) * 64 + lane;
+    b8 a0 = Wp[0], a1 = Wp[64];
+    for (int k = 0; k < WIDTH / 16; ++k) {
+      const b8 a2 = k + 2 < WIDTH / 16 ? Wp[(k + 2) * 64] : a1;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        if (j < nst) {
+          const b8 bb = *(const b8 *)(act + ((st0 + j) * 32 + lr) * LA + 16 * k + lk);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bb, acc[j], 0, 0, 0);
+        }
+      }
+      a0 = a1;
+      a1 = a2;
+    }
+    __syncthreads();
+    float *tb = (float *)act;  // surrogate: [FB][TS] staged outputs
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (j >= nst) continue;
+      const int s = (st0 + j) * 32 + lr;
+      const int64_t row = r0 + s;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int f = ft * 32 + 8 * g + 4 * hi + e;
+          if (f < p.nout) {
+            const float v = (acc[j][4 * g + e] + b[f]) * osc[f] + osh[f];
+            if (SUR) tb[s * TS + f] = v;
+            else if (row < p.n) p.y[row * p.nout + f] = v;
+          }
+        }
+    }
+    if (SUR) {
+      __syncthreads();
+      const int64_t row = r0 + tid;
+      if (tid < FB && row < p.n) {
+        const float *t = tb + tid * TS;
+        double tn2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) tn2 += (double)t[c] * (double)t[c];
+        const double kx = p.kx[row], ky = p.ky[row], w = p.omega;
+        const double k2 = kx * kx + ky * ky, kr = sqrt(k2);
+        const bool flag = fabs(kr - w) < p.branch_tol * w || kr < p.origin_tol * w || sqrt(tn2) > p.tbig;
+        const cd kz = csqrt(cd{w * w - k2, 0.0});
+        const cd P = cexp(cd{-kz.i * p.zsum, kz.r * p.zsum});
+        const cd fac = (cd{w * P.r, w * P.i} * cd{1.0 + k2 / (w * w), 0.0}) / kz;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+          const cd v = cd{(double)t[e], (double)t[9 + e]} * fac;
+          p.ghat[2 * (e * p.n + row)] = v.r;
+          p.ghat[2 * (e * p.n + row) + 1] = v.i;
+        }
+        p.flags[row] = flag ? 1 : 0;
+        if (flag) p.flagged[atomicAdd(p.n_flagged, 1)] = (int)row;
+      }
+    }
+  }
+}
+
+__global__ void k_mlp_pack(int N, int K, const __bf16 *__restrict__ W, __bf16 *__restrict__ Wpk) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // one fragment lane (8 values)
+  const int64_t nfrag = (int64_t)(N / 32) * (K / 16) * 64;
+  if (t >= nfrag) return;
+  const int lane = (int)(t % 64);
+  const int64_t ts = t / 64;
+  const int step = (int)(ts % (K / 16)), tile = (int)(ts / (K / 16));
+  const int row = 32 * tile + lane % 32, col = 16 * step + 8 * (lane / 32);
+  *(b8 *)(Wpk + 8 * t) = *(const b8 *)(W + (int64_t)row * K + col);
+}
+
+hipError_t launch_mlp_pack(int N, int K, const __bf16 *W, __bf16 *Wpk, hipStream_t s) {
+  if (N % 32 || K % 16 || N <= 0 || K <= 0) return hipErrorInvalidValue;
+  const int64_t nfrag = (int64_t)(N / 32) * (K / 16) * 64;
+  hipLaunchKernelGGL(k_mlp_pack, dim3((unsigned)((nfrag + 255) / 256)), dim3(256), 0, s, N, K, W, Wpk);
+  return hipGetLastError();
+}
+
+bool mlp_fused_supported(int width, int nin_total, int nout, int nl) {
+  return (width == 128 || width == 256) && nin_total <= 64 && nout >= 1 && nout <= 64 && nl >= 2 &&
+         nl <= MLP_FUSED_MAX_LAYERS;
+}
+
+hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStream_t s) {
+  if (!mlp_fused_supported(width, p.nin + 2 * p.kzf, p.nout, p.nl) || (surrogate && p.nout != 18))
+    return hipErrorInvalidValue;
+  if (p.n <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((p.n + FB - 1) / FB));
+  if (width == 256) {
+    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<256, true>), grid, dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((k_mlp_fused<256, false>), grid, dim3(256), 0, s, p);
+  } else {
+    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<128, true>), grid, dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((k_mlp_fused<128, false>), grid, dim3(256), 0, s, p);
+  }
+  return hipGetLastError();
+}
+
+__global__ void k_mlp_scatter_rows(int n, const int *__restrict__ idx, const double *__restrict__ src,
+                                   double *__restrict__ ghat, int64_t ld) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 9 * (int64_t)n) return;
+  const int e = (int)(t / n), i = (int)(t % n);
+  const int64_t o = 2 * (e * ld + idx[i]);
+  ghat[o] = src[2 * t];
+  ghat[o + 1] = src[2 * t + 1];
+}
+
+__global__ void k_mlp_gather_nodes(int n, const int *__restrict__ idx, const double *__restrict__ kx,
+                                   const double *__restrict__ ky, double *__restrict__ kxo, double *__restrict__ kyo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  kxo[i] = kx[idx[i]];
+  kyo[i] = ky[idx[i]];
+}
+
+hipError_t launch_mlp_scatter_rows(int n, const int *idx, const double *src, double *ghat, int64_t ld, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_scatter_rows, dim3((unsigned)((9 * (int64_t)n + 255) / 256)), dim3(256), 0, s, n, idx, src,
+                     ghat, ld);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_gather_nodes(int n, const int *idx, const double *kx, const double *ky, double *kxo, double *kyo,
+                                   hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_gather_nodes, dim3((n + 255) / 256), dim3(256), 0, s, n, idx, kx, ky, kxo, kyo);
   return hipGetLastError();
 }
 

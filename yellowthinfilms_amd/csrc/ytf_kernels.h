@@ -128,6 +128,40 @@ hipError_t launch_mlp_to_bf16(int64_t n, const float *w, void *w16, hipStream_t 
 hipError_t launch_mlp_zero(int64_t n, float *x, hipStream_t s);
 hipError_t launch_mlp_reduce(int64_t n, int splits, int64_t stride, const float *part, float *dst, hipStream_t s);
 
+// Whole-network forward in one kernel (hidden width 128 or 256, n_out <= 64,
+// at most 8 hidden layers): 128 rows per block, activations kept in LDS.
+constexpr int MLP_FUSED_MAX_LAYERS = 9;  // hidden biases staged in LDS
+struct MlpFused {
+  int64_t n;                      // rows
+  int nin, kzf, nl, nout;         // raw inputs, light-line features, layers, outputs
+  int kp0;                        // padded K of layer 0 (row stride of its weights)
+  const __bf16 *W16;              // bf16 weights in fragment order (launch_mlp_pack)
+  const float *P32;               // FP32 parameters (biases)
+  int64_t offW[MLP_FUSED_MAX_LAYERS], offb[MLP_FUSED_MAX_LAYERS];
+  const float *norm;              // in_shift[nin] in_scale[nin] out_shift[nout] out_scale[nout]
+  // generic forward: x[n][nin] -> y[n][nout]
+  const float *x;
+  float *y;
+  // surrogate evaluation: inputs (kx/omega, ky/omega, xconst[0 .. nin-3]);
+  // output t (18 reals) -> g~_s = t omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0
+  // as [9][n] complex, flags and the list of flagged rows
+  const double *kx, *ky;
+  float xconst[62];
+  double omega, zsum, branch_tol, origin_tol, tbig;
+  double *ghat;
+  unsigned char *flags;
+  int *n_flagged, *flagged;
+};
+bool mlp_fused_supported(int width, int nin_total, int nout, int nl);
+// W [N][K] (row-major bf16) -> fragment order [N/32][K/16][64 lanes][8]: lane L
+// of (tile, step) holds row 32 tile + L % 32, columns 16 step + 8 (L / 32) .. + 8.
+hipError_t launch_mlp_pack(int N, int K, const __bf16 *W, __bf16 *Wpk, hipStream_t s);
+hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStream_t s);
+// g~[e][idx[i]] = src[e][i] for the n flagged rows (e < 9), complex.
+hipError_t launch_mlp_scatter_rows(int n, const int *idx, const double *src, double *ghat, int64_t ld, hipStream_t s);
+hipError_t launch_mlp_gather_nodes(int n, const int *idx, const double *kx, const double *ky, double *kxo, double *kyo,
+                                   hipStream_t s);
+
 // 2D type-1 nonuniform FFT (nufft.hip).
 int nu_width(double eps);
 // Upper bound on the spreading subproblems of M points on an nf x nf fine grid.

@@ -314,25 +314,32 @@ class SurrogateGreen:
         big = torch.linalg.norm(t_pred.double(), dim=1) > pole_factor * self.t_ref
         return near_branch | near_origin | big
 
-    def scattered_ghat(self, kx, ky, hs_nm, ho_nm, omega, loss, fallback=True, **flag_kw):
-        """g~_s (n, 9) complex on the device; returns (g, flagged mask)."""
+    def scattered_ghat(self, kx, ky, hs_nm, ho_nm, omega, loss, fallback=True, branch_tol=None, origin_tol=None,
+                       pole_factor=3.0):
+        """g~_s (n, 9) complex on the device; returns (g, flagged mask).
+        One call of ytf_mlp_surrogate_ghat: the network, the exact factors
+        (ghat_from_target) and the flags (`flags`) in one kernel, then the
+        exact FP64 solve of the flagged nodes."""
         torch = _torch()
+        from .green import layer_structs
         dev = torch.device("cuda", self.device)
-        kx = torch.as_tensor(kx, dtype=torch.float64, device=dev)
-        ky = torch.as_tensor(ky, dtype=torch.float64, device=dev)
+        kx = torch.as_tensor(kx, dtype=torch.float64, device=dev).contiguous()
+        ky = torch.as_tensor(ky, dtype=torch.float64, device=dev).contiguous()
         n = kx.shape[0]
-        x = torch.empty((n, 6), dtype=torch.float32, device=dev)
-        x[:, 0] = (kx / omega).float()
-        x[:, 1] = (ky / omega).float()
-        x[:, 2] = hs_nm
-        x[:, 3] = ho_nm
-        x[:, 4] = omega / (2 * math.pi)
-        x[:, 5] = loss
-        t = self.mlp.forward(x)
-        g = ghat_from_target(t, kx, ky, hs_nm * NM, ho_nm * NM, omega)
-        mask = self.flags(kx, ky, t, omega, **flag_kw)
-        if fallback:
-            idx = torch.nonzero(mask).flatten()
-            if idx.numel():
-                g[idx] = self.exact_scattered(kx[idx], ky[idx], hs_nm, ho_nm, omega, loss)
-        return g, mask
+        branch_tol = self.BRANCH_TOL if branch_tol is None else branch_tol
+        origin_tol = self.ORIGIN_TOL if origin_tol is None else origin_tol
+        g = torch.empty((9, n), dtype=torch.complex128, device=dev)
+        flags = torch.empty(n, dtype=torch.uint8, device=dev)
+        st = self.family.stack(loss)
+        layers = layer_structs(st)
+        xc = (ct.c_float * 4)(float(hs_nm), float(ho_nm), float(omega / (2 * math.pi)), float(loss))
+        nflag = ct.c_int64()
+        ctx = self.mlp.ctx
+        ctx.bind_stream()
+        _lib.check(_lib.lib().ytf_mlp_surrogate_ghat(
+            ctx.handle, self.mlp.handle, layers, len(st), float(omega), _p(kx), _p(ky), n, xc,
+            float((hs_nm + ho_nm) * NM), float(hs_nm * NM), float(st[0].d - ho_nm * NM), float(branch_tol),
+            float(origin_tol), float(pole_factor * self.t_ref), 1 if fallback else 0, _p(g), _p(flags),
+            ct.byref(nflag)), ctx.handle)
+        self.last_flagged = int(nflag.value)
+        return g.T, flags.bool()
