@@ -34,6 +34,11 @@ Workloads (yellowthinfilms_amd.workloads):
     (padded 128^2), z skeletons on; one step = one apply_operator of a full
     64 x 64 x 16 x 3 moment volume.  value = operator applies/s;
     vs_baseline against the reference's 90 ms per apply.
+  pole_scan_hyperbolic: the real-axis pole search of a source in the
+    hyperbolic biaxial layer of the reference's operator benchmark, 9
+    directions, k_rho from 0.5 k0 to 3000 at a relative step of 1e-3, then
+    the refinement of every maximum.  value = cavity-response evaluations/s
+    of the whole search.
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -842,6 +847,122 @@ def cpu_operator_volume(stack, cache, op, P, grid, wl, zs, zobs, kw):
                        "(read back beforehand) and the benchmark volume"}, rel, rel_apply)
 
 
+def bench_pole_scan(a):
+    import torch
+    from yellowthinfilms_amd import workloads as W
+    from yellowthinfilms_amd.kspace import real_axis_poles, _Evaluator
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.POLE_SCAN_HYPERBOLIC
+    stack = W.pole_scan_stack(wl)
+    phis, k_lo, k_hi, step, exclude = W.pole_scan_args(wl)
+    res = {}
+    calls = {"n": 0}
+    orig = _Evaluator.__call__
+
+    def counting(self, k, phi):  # nodes evaluated per scan (scan + refinement)
+        calls["n"] += int(np.asarray(k).size)
+        return orig(self, k, phi)
+
+    def step_fn():
+        res["poles"] = real_axis_poles(stack, W.OMEGA, wl["source_layer"], phis=phis, k_lo=k_lo, k_hi=k_hi,
+                                       scan_step=step, exclude=exclude)
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step_fn()
+    _Evaluator.__call__ = counting
+    ctx.set_timing(True)
+    try:
+        wall = _timed_steps(dict_ns(a, warmup=0), step_fn, torch)
+    finally:
+        _Evaluator.__call__ = orig
+    kt, kn = ctx.kernel_time("cavity_response")
+    ctx.set_timing(False)
+    nodes_per_scan = calls["n"] / a.steps
+    n_scan = len(phis) * (int(math.floor(math.log(k_hi / k_lo) / step + 1e-10)) + 1)
+    # the scan launch is the largest cavity_response launch of a step
+    flops = _pole_scan_flops(stack, wl)
+    cpu, cmp = None, None
+    if not a.no_cpu_baseline:
+        cpu, cmp = cpu_pole_scan(stack, wl, res["poles"])
+    out = _base(a, wl, nodes_per_scan * a.steps / wall, "cavity-response k-nodes/s (whole pole search)", wall, "f64")
+    out["config"].update({"directions": len(phis), "scan_nodes": n_scan, "k_lo": k_lo, "k_hi": k_hi,
+                          "scan_step": step})
+    out.update({
+        "scans_per_s": a.steps / wall, "nodes_per_scan": nodes_per_scan, "poles_found": len(res["poles"]),
+        "response_launches_per_scan": kn / a.steps, "response_kernel_ms_per_scan": kt / a.steps,
+        "poles_vs_oracle": cmp,
+        "roofline": {"kernel": "cavity_response (Berreman modes + S cascade + cavity per node, FP64), all launches "
+                               "of a scan", "bound": "mfma",
+                     "achieved": flops * nodes_per_scan / (kt / a.steps * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": flops * nodes_per_scan / (kt / a.steps * 1e-3) / PEAK_FP64,
+                     "traffic": None, "flops_per_node": flops},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def _pole_scan_flops(stack, wl):
+    """Mean op count of one cavity response (the oracle's reference-order
+    arithmetic with an op-counting real type) over 16 seeded nodes of the scan range."""
+    try:
+        from oracle import pyoracle as po
+        from yellowthinfilms_amd.materials import stack_arrays
+        from yellowthinfilms_amd import workloads as W
+    except Exception:
+        return float("nan")
+    st = po.Stack(*stack_arrays(stack))
+    rng = np.random.default_rng(0)
+    tot = []
+    for k in np.exp(rng.uniform(math.log(wl["k_lo_k0"] * W.OMEGA), math.log(wl["k_hi"]), 16)):
+        ph = rng.uniform(0, math.pi / 2)
+        c = po.count_response_ops(st, W.OMEGA, wl["source_layer"], k * math.cos(ph), k * math.sin(ph))
+        tot.append(c["add"] + c["mul"] + c["div"])
+    return float(np.mean(tot))
+
+
+def cpu_pole_scan(stack, wl, poles):
+    """The oracle's step-by-step real_axis_poles (its cavity response on the
+    host cores, the reference's sequential refinement): timed once, and its
+    poles compared with the GPU's."""
+    try:
+        from oracle import pyoracle as po
+        from yellowthinfilms_amd.materials import stack_arrays
+        from yellowthinfilms_amd import workloads as W
+    except Exception as exc:
+        return ({"value": None, "unit": "cavity-response k-nodes/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    st = po.Stack(*stack_arrays(stack))
+    phis, k_lo, k_hi, step, exclude = W.pole_scan_args(wl)
+    n_calls = {"n": 0}
+    orig = po.cavity_response
+
+    def counting(*args, **kw):
+        n_calls["n"] += len(np.atleast_1d(args[3]))
+        return orig(*args, **kw)
+
+    po.cavity_response = counting
+    try:
+        t = time.perf_counter()
+        o = po.real_axis_poles(st, W.OMEGA, wl["source_layer"], phis=phis, k_lo=k_lo, k_hi=k_hi, scan_step=step,
+                               exclude=exclude)
+        dt = time.perf_counter() - t
+    finally:
+        po.cavity_response = orig
+    cmp = None
+    if len(o) == len(poles):
+        dq = np.array([abs(p["q"] - q) / G for p, (q, G, _, _) in zip(poles, o)])
+        dG = np.array([abs(p["Gamma"] - G) / G for p, (q, G, _, _) in zip(poles, o)])
+        cmp = {"poles": len(o), "max_dq_over_gamma": float(dq.max()), "median_dgamma_rel": float(np.median(dG)),
+               "n_gamma_off_1e-3": int(np.sum(dG > 1e-3)), "n_q_off_1e-4_gamma": int(np.sum(dq > 1e-4))}
+    else:
+        cmp = {"poles": len(o), "gpu_poles": len(poles)}
+    return ({"value": n_calls["n"] / dt, "unit": "cavity-response k-nodes/s", "cores": _cores(), "kind": "port",
+             "sample": f"one whole pole search ({n_calls['n']} evaluations, {len(o)} poles; scan on all cores, "
+                       "refinement one evaluation at a time as the reference takes it)"}, cmp)
+
+
 # ---------------------------------------------------------------------------
 def dict_ns(a, **kw):
     b = argparse.Namespace(**vars(a))
@@ -852,7 +973,8 @@ def dict_ns(a, **kw):
 
 WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
              "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft,
-             "operator_512": bench_operator, "operator_volume_64": bench_operator_volume}
+             "operator_512": bench_operator, "operator_volume_64": bench_operator_volume,
+             "pole_scan_hyperbolic": bench_pole_scan}
 
 
 def main():

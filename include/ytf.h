@@ -148,6 +148,19 @@ int ytf_green_nodes_box(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, dou
                         const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
                         int terms, double dz_box, int ez_local);
 
+/* Cavity response of a source layer at in-plane wavevectors (the scan
+ * function of real_axis_poles, mod_KSpaceFeatures.jl:109-127 / 151-214):
+ *   out[i] = max(sigma(RL), sigma(RR) [, sigma(inv(I - PhiF RL PhiB RR)) for s > 1])
+ * with RL = SLeft[s][1:2,3:4] (the stack above, seen from the layer's top
+ * edge), RR = (Sint[s] * SRight[s+1])[3:4,1:2] (the stack below, at its bottom
+ * edge), sigma the largest singular value of a 2x2, PhiF = e^{i Qf d_s},
+ * PhiB = e^{-i Qb d_s}.  It peaks at the real-axis poles (guided modes,
+ * plasmons, polaritons) of the field of a source in layer s.  0 where the
+ * mode solve fails or an interface solve is exactly singular.
+ *   source_layer: 1 .. n_layers - 1; kx_dev, ky_dev, out_dev: n_nodes doubles. */
+int ytf_cavity_response(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                        const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev);
+
 /* Spectral convolution on the uniform DFT grid (the "G(kx,ky) F{J}" route of
  * mod_UniformSource.jl:229-248):
  *     E = scale * bfft2( g~ .* fft2(J) )        per field component c:

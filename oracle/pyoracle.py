@@ -251,6 +251,15 @@ def count_stack_ops(st, omega, kx, ky):
     return dict(zip(["add", "mul", "div", "sqrt", "trans"], map(int, out)))
 
 
+def count_response_ops(st, omega, s, kx, ky):
+    """Real add/mul/div, sqrt and transcendental counts of one _cavity_response
+    evaluation in the reference's formulation."""
+    out = np.zeros(5, np.uint64)
+    lib().ora_count_response_ops(*st.args(), ct.c_double(omega), ct.c_int(s), ct.c_double(kx), ct.c_double(ky),
+                                 out.ctypes.data_as(ct.POINTER(ct.c_uint64)))
+    return dict(zip(["add", "mul", "div", "sqrt", "trans"], map(int, out)))
+
+
 def modal_amp(st, omega, kx, ky, layer, z, sif, sib):
     sif = _c(sif); sib = _c(sib)
     Ef = np.zeros(2 * st.n, np.complex128)
@@ -471,3 +480,91 @@ def zskeleton(z, fam, tol, whole_frac=0.8):
         return whole
     idx, P = interp(hi)
     return dict(idx=idx, P=P, err=maxerr(idx, P), whole=False)
+
+
+def cavity_response(st, omega, s, kx, ky, nthreads=0):
+    """_cavity_response (mod_KSpaceFeatures.jl:109-127) at nodes (kx, ky); s 1-based."""
+    kx = np.ascontiguousarray(np.atleast_1d(kx), np.float64)
+    ky = np.ascontiguousarray(np.atleast_1d(ky), np.float64)
+    out = np.zeros(len(kx))
+    rc = lib().ora_cavity_response(*st.args(), ct.c_double(omega), ct.c_int(s), _p(kx), _p(ky),
+                                   ct.c_int64(len(kx)), _p(out), ct.c_int(nthreads))
+    if rc:
+        raise ValueError("cavity_response: source layer out of range")
+    return out
+
+
+def real_axis_poles(st, omega, s, phis=(0.0,), k_lo=None, k_hi=None, scan_step=1e-3, prominence=1.05,
+                    exclude=(), floor_rel=1e-6, floor_abs=1e-10):
+    """real_axis_poles (mod_KSpaceFeatures.jl:151-214), step by step as the
+    reference takes it: the log-spaced scan, the local maxima with their
+    prominence over the valleys within a factor 4 in k, golden-section search
+    of each maximum, and the half-level crossings by bisection.  Returns a list
+    of (q, Gamma, phi, prom)."""
+    # Julia's 0.0:step:L includes the last multiple of step <= L
+    L = np.log(k_hi / k_lo)
+    nstep = int(np.floor(L / scan_step + 1e-10))
+    ks = k_lo * np.exp(np.arange(nstep + 1) * scan_step)
+    n = len(ks)
+    W = int(np.ceil(np.log(4.0) / scan_step))
+    out = []
+    for phi in phis:
+        c, sn = np.cos(phi), np.sin(phi)
+
+        def f(k):
+            return float(cavity_response(st, omega, s, [k * c], [k * sn])[0])
+
+        y = np.log(cavity_response(st, omega, s, ks * c, ks * sn) + 1e-300)
+        ymin = max(y.max() + np.log(floor_rel), np.log(floor_abs))
+        for i in range(1, n - 1):
+            if not (y[i] > y[i - 1] and y[i] >= y[i + 1] and y[i] >= ymin):
+                continue
+            jl, lmin = i, y[i]
+            while jl > max(0, i - W) and y[jl - 1] <= y[i]:
+                jl -= 1
+                lmin = min(lmin, y[jl])
+            jr, rmin = i, y[i]
+            while jr < min(n - 1, i + W) and y[jr + 1] <= y[i]:
+                jr += 1
+                rmin = min(rmin, y[jr])
+            base = max(lmin, rmin)
+            prom = y[i] - base
+            if prom < np.log(prominence):
+                continue
+            a, b = ks[i - 1], ks[i + 1]
+            g = (np.sqrt(5.0) - 1) / 2
+            cc, dd = b - g * (b - a), a + g * (b - a)
+            fc, fd = f(cc), f(dd)
+            for _ in range(60):
+                if (b - a) <= 1e-12 * b:
+                    break
+                if fc >= fd:
+                    b, dd, fd = dd, cc, fc
+                    cc = b - g * (b - a)
+                    fc = f(cc)
+                else:
+                    a, cc, fc = cc, dd, fd
+                    dd = a + g * (b - a)
+                    fd = f(dd)
+            q = (a + b) / 2
+            P = max(f(q), np.exp(y[i]))
+            if any(abs(q - Lx) <= 1e-6 * q for Lx in exclude):
+                continue
+            lvl = np.exp(base) + (P - np.exp(base)) / np.sqrt(2.0)
+
+            def cross(lo, hi):
+                for _ in range(50):
+                    if abs(hi - lo) <= 1e-12 * q:
+                        break
+                    mid = (lo + hi) / 2
+                    if f(mid) > lvl:
+                        lo = mid
+                    else:
+                        hi = mid
+                return (lo + hi) / 2
+
+            kl = ks[jl + int(np.nonzero(y[jl:i + 1] == lmin)[0][-1])]
+            kr = ks[i + int(np.nonzero(y[i:jr + 1] == rmin)[0][0])]
+            G = min(q - cross(q, kl), cross(q, kr) - q)
+            out.append((q, max(G, 1e-12 * q), float(phi), prom))
+    return out

@@ -1260,7 +1260,48 @@ template <class R> void read_rec(NodeOut<R> &nd, int nobs, const double *rec, in
   }
 }
 
+// _σmax -- mod_KSpaceFeatures.jl:91-95: the largest singular value of a 2x2
+// from G = A'A (trace t, determinant dt).
+template <class R> double sigma_max2(const Mat<R, 2, 2> &A) {
+  Mat<R, 2, 2> G = adjmul<R, 2, 2, 2>(A, A);
+  const double t = val((G(0, 0) + G(1, 1)).re);
+  const double dt = val((G(0, 0) * G(1, 1) - G(0, 1) * G(1, 0)).re);
+  return std::sqrt(std::max((t + std::sqrt(std::max(t * t - 4.0 * dt, 0.0))) / 2.0, 0.0));
+}
+
+// _cavity_response -- mod_KSpaceFeatures.jl:109-127 (source layer s 1-based):
+// RL = SLeft[s][1:2, 3:4], RR = (Sint[s] * SRight[s+1])[3:4, 1:2] ("*" the
+// Redheffer product), r = max(sigma(RL), sigma(RR)) and, for s > 1, also
+// sigma(inv(I - PhiF RL PhiB RR)) with PhiF = e^{i Qf d}, PhiB = e^{-i Qb d};
+// 0 where the mode solve fails or r is not finite.
+template <class R> double cavity_response(const std::vector<Layer<R>> &st, R omega, int s, R kx, R ky) {
+  std::vector<Modes<R>> M;
+  if (!modes_stack(st, omega, kx, ky, M)) return 0.0;
+  Smat<R> S;
+  S_all(st, M, S);
+  const Mat<R, 4, 4> X = redheffer(S.Sint[s - 1], S.SRight[s]);
+  const Mat<R, 2, 2> RR = blk(X, 1, 0), RL = blk(S.SLeft[s - 1], 0, 1);
+  double r = std::max(sigma_max2(RL), sigma_max2(RR));
+  if (s > 1) {
+    const double d = st[s - 1].d;
+    cx<R> pf[2], pb[2];
+    sprop2(d, M[s - 1].Qf, pf);
+    sprop2(-d, M[s - 1].Qb, pb);
+    const Mat<R, 2, 2> A = mul<R, 2, 2, 2>(dmul(pf, RL), dmul(pb, RR));
+    Mat<R, 2, 2> Mm = sub(eye<R, 2>(), A);
+    const cx<R> det = Mm(0, 0) * Mm(1, 1) - Mm(0, 1) * Mm(1, 0);
+    Mat<R, 2, 2> Mi;
+    Mi(0, 0) = Mm(1, 1) / det;
+    Mi(1, 0) = -Mm(1, 0) / det;
+    Mi(0, 1) = -Mm(0, 1) / det;
+    Mi(1, 1) = Mm(0, 0) / det;
+    r = std::max(r, sigma_max2(Mi));
+  }
+  return std::isfinite(r) ? r : 0.0;
+}
+
 }  // namespace ora
+
 
 // ============================================================================
 // C API (ctypes) -- test infrastructure
@@ -1485,6 +1526,21 @@ int ora_stack_S_batch(int n, const double *eps, const double *mu, const double *
   return 0;
 }
 
+// Batched _cavity_response at nodes (kx, ky) (kx = k_rho cos(phi), ky = k_rho
+// sin(phi) as Modes_Stack(stack, omega, k_rho, phi)); s 1-based.
+int ora_cavity_response(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
+                        double omega, int s, const double *kx, const double *ky, int64_t nn, double *out,
+                        int nthreads) {
+  if (s < 1 || s > n - 1) return 1;
+  auto st = make_stack<double>(n, eps, mu, d, ek, mk);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t k = 0; k < nn; ++k) out[k] = cavity_response<double>(st, omega, s, kx[k], ky[k]);
+  return 0;
+}
+
 // Type-1 nonuniform DFT in 2D, summed directly (the definition FINUFFT's
 // type 1 with iflag +1 approximates; the 2D lateral sum of closed_form_G,
 // mod_ClosedFormField.jl:743-780): for each listed mode (k1, k2),
@@ -1554,6 +1610,19 @@ int ora_count_stack_ops(int n, const double *eps, const double *mu, const double
       S0 = redheffer(St, diag4(dg));
     }
   }
+  out[0] = g_ops.add; out[1] = g_ops.mul; out[2] = g_ops.div; out[3] = g_ops.sqrt; out[4] = g_ops.trans;
+  return 0;
+}
+
+// Op count of one _cavity_response evaluation (reference formulation:
+// Modes_Stack, S_all_ingredients, both scans of S_all_Matrix, the 2x2 parts).
+// out: add, mul, div, sqrt, trans.
+int ora_count_response_ops(int n, const double *eps, const double *mu, const double *d, const int *ek, const int *mk,
+                           double omega, int s, double kx, double ky, uint64_t *out) {
+  if (s < 1 || s > n - 1) return 1;
+  auto st = make_stack<CR>(n, eps, mu, d, ek, mk);
+  g_ops = Ops();
+  cavity_response<CR>(st, CR(omega), s, CR(kx), CR(ky));
   out[0] = g_ops.add; out[1] = g_ops.mul; out[2] = g_ops.div; out[3] = g_ops.sqrt; out[4] = g_ops.trans;
   return 0;
 }

@@ -71,6 +71,8 @@ _SIGS = {
                                        ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
                                        ct.POINTER(ct.c_double), ct.POINTER(ct.c_double), ct.c_void_p,
                                        ct.POINTER(YtfNodeCache), ct.c_int, ct.c_double, ct.c_int]),
+    "ytf_cavity_response": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int,
+                                       ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,

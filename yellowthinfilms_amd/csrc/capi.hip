@@ -98,6 +98,21 @@ int ytf_green_nodes_box(ytf_ctx *c, const ytf_layer *layers, int n_layers, doubl
 
 }  // extern "C"
 
+// Closed-form layers (isotropic/uniaxial eps, isotropic mu) and layers
+// without xz/yz coupling never reach the general eigen-solver.
+static bool stack_needs_general(const ytf_layer *layers, int n_layers) {
+  for (int l = 0; l < n_layers; ++l) {
+    const ytf_layer &Ly = layers[l];
+    if ((Ly.eps_kind == 0 || Ly.eps_kind == 1) && Ly.mu_kind == 0) continue;
+    const int idx[4] = {2, 5, 6, 7};
+    for (int t = 0; t < 4; ++t)
+      if (Ly.eps[2 * idx[t]] != 0.0 || Ly.eps[2 * idx[t] + 1] != 0.0 || Ly.mu[2 * idx[t]] != 0.0 ||
+          Ly.mu[2 * idx[t] + 1] != 0.0)
+        return true;
+  }
+  return false;
+}
+
 int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
                        const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
                        const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
@@ -182,18 +197,7 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
   P.basis = cache ? (cd *)cache->basis_dev : nullptr;
   P.flags = cache ? (unsigned char *)cache->flags_dev : nullptr;
   if ((P.sf == nullptr) != (P.sb == nullptr)) return fail(c, YTF_EINVAL, "ytf_green_nodes: sf and sb go together");
-  // closed-form layers (isotropic/uniaxial eps, isotropic mu) and layers
-  // without xz/yz coupling never reach the general eigen-solver
-  P.needs_general = false;
-  for (int l = 0; l < n_layers; ++l) {
-    const ytf_layer &Ly = layers[l];
-    if ((Ly.eps_kind == 0 || Ly.eps_kind == 1) && Ly.mu_kind == 0) continue;
-    const int idx[4] = {2, 5, 6, 7};
-    for (int t = 0; t < 4; ++t)
-      if (Ly.eps[2 * idx[t]] != 0.0 || Ly.eps[2 * idx[t] + 1] != 0.0 || Ly.mu[2 * idx[t]] != 0.0 ||
-          Ly.mu[2 * idx[t] + 1] != 0.0)
-        P.needs_general = true;
-  }
+  P.needs_general = stack_needs_general(layers, n_layers);
   int64_t blocks = (n_nodes + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
   hipError_t e = timed(c, "node_solve", [&] {
@@ -205,6 +209,41 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
 }
 
 extern "C" {
+
+int ytf_cavity_response(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                        const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_cavity_response: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_cavity_response: need at least 2 layers");
+  if (source_layer < 1 || source_layer > n_layers - 1)
+    return fail(c, YTF_EINVAL, "ytf_cavity_response: source_layer=%d must be in 1:%d", source_layer, n_layers - 1);
+  if (n_nodes < 0 || (n_nodes > 0 && (!kx_dev || !ky_dev || !out_dev)))
+    return fail(c, YTF_EINVAL, "ytf_cavity_response: bad nodes");
+  if (!(omega > 0) || !std::isfinite(omega)) return fail(c, YTF_EINVAL, "ytf_cavity_response: omega must be > 0");
+  for (int l = 0; l < n_layers; ++l)
+    if (layers[l].eps_kind < 0 || layers[l].eps_kind > 3 || layers[l].mu_kind < 0 || layers[l].mu_kind > 1)
+      return fail(c, YTF_EINVAL, "ytf_cavity_response: layer %d has an unknown material kind", l + 1);
+  if (n_nodes == 0) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t lb = sizeof(ytf_layer) * n_layers;
+  std::vector<char> hb(lb);
+  std::memcpy(hb.data(), layers, lb);
+  if (hb != c->small_host) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = ensure_small(c, hb.size() + 64);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->small, hb.data(), hb.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->small_host = hb;
+  }
+  const bool general = stack_needs_general(layers, n_layers);
+  hipError_t e = timed(c, "cavity_response", [&] {
+    return ytf::launch_cavity_response((const LayerDev *)c->small, n_layers, source_layer, omega, kx_dev, ky_dev,
+                                       n_nodes, out_dev, general, c->stream);
+  });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_cavity_response: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
 
 int ytf_spectral_apply(ytf_ctx *c, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
                        double *work_dev, double scale) {

@@ -97,6 +97,11 @@ struct BatchLaunch {
 };
 hipError_t launch_batch(const BatchLaunch &B, hipStream_t s);
 
+// Cavity response of a source layer (response.hip): the scan function of the
+// real-axis pole search; s 1-based.
+hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega, const double *kx, const double *ky,
+                                  int64_t n, double *out, bool general, hipStream_t st);
+
 // Surrogate network (mlp.hip): one BF16 matrix-core GEMM with fused epilogues,
 // plus the small elementwise kernels of training and inference.
 enum { MLP_EPI_F32 = 0, MLP_EPI_BIAS_SILU = 1, MLP_EPI_BIAS_F32 = 2, MLP_EPI_DSILU = 3 };

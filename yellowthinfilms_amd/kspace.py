@@ -1,0 +1,215 @@
+"""Real-axis pole search of the k_rho integrand (host side of
+mod_KSpaceFeatures.jl): where the field of a source has guided-mode,
+plasmon or polariton poles, found before any quadrature is built.
+
+  cavity_response  -- _cavity_response (:109-127) at a batch of nodes, one
+                      GPU launch (ytf_cavity_response);
+  real_axis_poles  -- real_axis_poles (:151-214): the log-spaced scan, the
+                      prominent local maxima, golden-section location of
+                      each and its half-width from the half-level crossings.
+                      Every evaluation runs on the GPU: the scan of all
+                      directions in one launch, then each refinement step of
+                      all candidates at once (the reference refines one
+                      candidate after another);
+  clusters         -- _clusters (:228-241), the intervals a rule resolves;
+  material_class, choose_engine -- (:54-82).
+"""
+import ctypes as ct
+import math
+
+import numpy as np
+
+from . import _lib
+from .green import layer_structs
+from .materials import stack_arrays
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def cavity_response(stack, omega, source_layer, kx, ky, device=0):
+    """_cavity_response at nodes (kx, ky) (kx = k_rho cos(phi), ky = k_rho
+    sin(phi)); returns a float64 device tensor."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    kx = torch.as_tensor(kx, dtype=torch.float64, device=dev).contiguous().reshape(-1)
+    ky = torch.as_tensor(ky, dtype=torch.float64, device=dev).contiguous().reshape(-1)
+    if kx.shape != ky.shape:
+        raise ValueError("cavity_response: kx and ky must have the same length")
+    out = torch.empty_like(kx)
+    ctx = _lib.Context.get(device)
+    ctx.bind_stream()
+    _lib.check(_lib.lib().ytf_cavity_response(ctx.handle, layer_structs(stack), len(stack), float(omega),
+                                              int(source_layer), ct.c_void_p(kx.data_ptr()),
+                                              ct.c_void_p(ky.data_ptr()), kx.shape[0], ct.c_void_p(out.data_ptr())),
+               ctx.handle)
+    return out
+
+
+class _Evaluator:
+    """f(k, phi) for arrays of (k, phi), one launch per call (host arrays in and out)."""
+
+    def __init__(self, stack, omega, source_layer, device):
+        self.args = (stack, omega, source_layer)
+        self.device = device
+        self.calls = 0
+
+    def __call__(self, k, phi):
+        k = np.asarray(k, np.float64)
+        phi = np.asarray(phi, np.float64)
+        if k.size == 0:
+            return np.zeros(0)
+        self.calls += 1
+        r = cavity_response(*self.args, k * np.cos(phi), k * np.sin(phi), device=self.device)
+        return r.cpu().numpy()
+
+
+def real_axis_poles(stack, omega, source_layer, phis=(0.0,), k_lo=None, k_hi=None, scan_step=1e-3,
+                    prominence=1.05, exclude=(), floor_rel=1e-6, floor_abs=1e-10, device=0):
+    """real_axis_poles (mod_KSpaceFeatures.jl:151-214).  Returns a list of
+    dicts (q, Gamma, phi, prom):
+      1. the response on k = k_lo e^{j scan_step} up to k_hi, every phi;
+      2. local maxima standing `prominence` x above the higher of the two
+         valleys around them (searched over a factor 4 in k) and above
+         max(floor_rel x the largest response along that phi, floor_abs);
+      3. q by golden-section search between the maximum's neighbours (60
+         steps at most, to 1e-12 relative), Gamma as the nearer of the two
+         crossings of b + (peak - b)/sqrt(2) (b the valley level), found by
+         bisection towards each valley.
+    Peaks within 1e-6 q of a value in `exclude` (the light lines) are dropped."""
+    if k_lo is None or k_hi is None or not (0 < k_lo < k_hi):
+        raise ValueError("real_axis_poles: need 0 < k_lo < k_hi")
+    f = _Evaluator(stack, omega, source_layer, device)
+    L = math.log(k_hi / k_lo)
+    nstep = int(math.floor(L / scan_step + 1e-10))
+    ks = k_lo * np.exp(np.arange(nstep + 1) * scan_step)   # Julia's 0.0:step:L
+    n = len(ks)
+    W = int(math.ceil(math.log(4.0) / scan_step))
+    phis = [float(p) for p in phis]
+    K = np.repeat(ks[None, :], len(phis), axis=0)
+    P = np.repeat(np.asarray(phis)[:, None], n, axis=1)
+    Y = np.log(f(K.ravel(), P.ravel()).reshape(len(phis), n) + 1e-300)
+    # candidates: (phi index, i, base, prom, left valley k, right valley k)
+    cand = []
+    for m, phi in enumerate(phis):
+        y = Y[m]
+        ymin = max(y.max() + math.log(floor_rel), math.log(floor_abs))
+        inner = np.arange(1, n - 1)
+        is_max = (y[1:-1] > y[:-2]) & (y[1:-1] >= y[2:]) & (y[1:-1] >= ymin)
+        for i in inner[is_max]:
+            # valleys: walk out from i while the samples stay <= y[i], at most W steps
+            lo = max(0, i - W)
+            above = np.nonzero(y[lo:i] > y[i])[0]
+            jl = lo + above[-1] + 1 if above.size else lo
+            lmin = y[jl:i + 1].min()
+            hi = min(n - 1, i + W)
+            above = np.nonzero(y[i + 1:hi + 1] > y[i])[0]
+            jr = i + above[0] if above.size else hi
+            rmin = y[i:jr + 1].min()
+            base = max(lmin, rmin)
+            prom = y[i] - base
+            if prom < math.log(prominence):
+                continue
+            kl = ks[jl + int(np.nonzero(y[jl:i + 1] == lmin)[0][-1])]
+            kr = ks[i + int(np.nonzero(y[i:jr + 1] == rmin)[0][0])]
+            cand.append((m, i, base, prom, kl, kr))
+    if not cand:
+        return []
+    nc = len(cand)
+    ph = np.array([phis[c[0]] for c in cand])
+    # golden-section search of every candidate at once
+    a = np.array([ks[c[1] - 1] for c in cand])
+    b = np.array([ks[c[1] + 1] for c in cand])
+    g = (math.sqrt(5.0) - 1) / 2
+    c_ = b - g * (b - a)
+    d_ = a + g * (b - a)
+    fcd = f(np.concatenate([c_, d_]), np.concatenate([ph, ph]))
+    fc, fd = fcd[:nc], fcd[nc:]
+    for _ in range(60):
+        act = np.nonzero(~((b - a) <= 1e-12 * b))[0]
+        if act.size == 0:
+            break
+        left = act[fc[act] >= fd[act]]    # the maximum is left of d: b <- d
+        right = act[fc[act] < fd[act]]
+        b[left], d_[left], fd[left] = d_[left], c_[left], fc[left]
+        c_[left] = b[left] - g * (b[left] - a[left])
+        a[right], c_[right], fc[right] = c_[right], d_[right], fd[right]
+        d_[right] = a[right] + g * (b[right] - a[right])
+        new = f(np.concatenate([c_[left], d_[right]]), np.concatenate([ph[left], ph[right]]))
+        fc[left] = new[:left.size]
+        fd[right] = new[left.size:]
+    q = (a + b) / 2
+    Pk = np.maximum(f(q, ph), np.exp([Y[c[0], c[1]] for c in cand]))
+    keep = np.array([not any(abs(q[j] - Lx) <= 1e-6 * q[j] for Lx in exclude) for j in range(nc)], bool)
+    base = np.array([c[2] for c in cand])
+    lvl = np.exp(base) + (Pk - np.exp(base)) / math.sqrt(2.0)
+    # half-level crossings, towards each valley (2 per candidate), bisected together
+    lo = np.concatenate([q, q])
+    hi = np.concatenate([[c[4] for c in cand], [c[5] for c in cand]])
+    lv = np.concatenate([lvl, lvl])
+    qq = np.concatenate([q, q])
+    pp = np.concatenate([ph, ph])
+    for _ in range(50):
+        act = np.nonzero(~(np.abs(hi - lo) <= 1e-12 * qq))[0]
+        if act.size == 0:
+            break
+        mid = (lo[act] + hi[act]) / 2
+        above = f(mid, pp[act]) > lv[act]
+        lo[act[above]] = mid[above]
+        hi[act[~above]] = mid[~above]
+    xr = (lo + hi) / 2
+    G = np.minimum(q - xr[:nc], xr[nc:] - q)
+    out = []
+    for j, cnd in enumerate(cand):
+        if keep[j]:
+            out.append(dict(q=float(q[j]), Gamma=float(max(G[j], 1e-12 * q[j])), phi=float(ph[j]),
+                            prom=float(cnd[3])))
+    return out
+
+
+def clusters(poles, merge_rel):
+    """_clusters (mod_KSpaceFeatures.jl:228-241): one [q, q] interval per pole
+    plus, for two poles adjacent in q within merge_rel q (or 4 widths) of each
+    other, the segment between them at the smaller width."""
+    ps = sorted(poles, key=lambda p: p["q"])
+    out = []
+    for i, p in enumerate(ps):
+        e = (p["q"], p["q"], p["Gamma"])
+        if e not in out:
+            out.append(e)
+        if i > 0:
+            r = ps[i - 1]
+            if p["q"] - r["q"] <= max(merge_rel * p["q"], 4 * min(p["Gamma"], r["Gamma"])) and p["q"] > r["q"]:
+                e = (r["q"], p["q"], min(p["Gamma"], r["Gamma"]))
+                if e not in out:
+                    out.append(e)
+    return [dict(a=a, b=b, Gamma=G) for a, b, G in out]
+
+
+def material_class(layer):
+    """material_class (mod_KSpaceFeatures.jl:54-62): "gyrotropic" for a
+    non-symmetric eps or mu, else from the signs of the eigenvalues of Re eps
+    and Re mu: "dielectric", "metallic" (all of eps negative) or "hyperbolic"."""
+    eps, mu, _, _, _ = stack_arrays([layer])
+    e, m = eps[0], mu[0]
+    if not (np.array_equal(e, e.T) and np.array_equal(m, m.T)):
+        return "gyrotropic"
+    ee = np.linalg.eigvalsh(e.real)
+    em = np.linalg.eigvalsh(m.real)
+    if np.all(ee > 0) and np.all(em > 0):
+        return "dielectric"
+    if np.all(ee < 0) and np.all(em > 0):
+        return "metallic"
+    return "hyperbolic"
+
+
+def choose_engine(stack, engine="auto"):
+    """choose_engine (mod_KSpaceFeatures.jl:71-75): "adaptive" when every layer
+    is dielectric, "fixed" as soon as one is metallic, hyperbolic or gyrotropic."""
+    if engine not in ("auto", "fixed", "adaptive"):
+        raise ValueError(f"engine must be :auto, :fixed or :adaptive (got :{engine})")
+    if engine != "auto":
+        return engine
+    return "adaptive" if all(material_class(l) == "dielectric" for l in stack) else "fixed"

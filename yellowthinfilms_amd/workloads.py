@@ -255,6 +255,34 @@ def operator_volume_nodes(wl=OPERATOR_VOLUME_64):
     return np.concatenate([k1, k2]), np.concatenate([w1, w2]), phi, np.full(wl["n_phi"], 2 * math.pi / wl["n_phi"])
 
 
+POLE_SCAN_HYPERBOLIC = dict(
+    name="pole_scan_hyperbolic",
+    description=("real-axis pole search (real_axis_poles, mod_KSpaceFeatures.jl:151-214) of the field of a source "
+                 "in the hyperbolic biaxial layer of the reference's operator benchmark (vac / biax "
+                 "eps=(2.25, 4.6, -3.2) + 1e-3i / vac, every layer 0.5 thick, source layer 2): the cavity response "
+                 "on a log-spaced k_rho grid (relative step 1e-3) from 0.5 k0 to the benchmark's k_rho cap 3000 "
+                 "along 9 directions in [0, pi/2] (a diagonal stack: kspace_features' N_phi_scan = 32), then "
+                 "golden-section location and half-width of every prominent maximum; complex FP64"),
+    delta=1e-3, thick=0.5, source_layer=2, n_phi=9, k_lo_k0=0.5, k_hi=3000.0, scan_step=1e-3,
+)
+
+
+def pole_scan_stack(wl=POLE_SCAN_HYPERBOLIC):
+    """m_vac / m_hyp / m_vac of Bench_GreenOperator.jl:42-55 at loss delta."""
+    from .materials import IsotropicMaterial, BiaxialMaterial, FlatOpticalLayer
+    d = wl["delta"]
+    vac = IsotropicMaterial(complex(1.0, d))
+    return [FlatOpticalLayer(vac, wl["thick"]),
+            FlatOpticalLayer(BiaxialMaterial(complex(2.25, d), complex(4.6, d), complex(-3.2, d)), wl["thick"]),
+            FlatOpticalLayer(vac, wl["thick"])]
+
+
+def pole_scan_args(wl=POLE_SCAN_HYPERBOLIC):
+    """(phis, k_lo, k_hi, scan_step, exclude) of pole_scan_hyperbolic."""
+    phis = list(np.linspace(0.0, math.pi / 2, wl["n_phi"]))
+    return phis, wl["k_lo_k0"] * OMEGA, wl["k_hi"], wl["scan_step"], [OMEGA]
+
+
 BATCHED_GYRO_512 = dict(
     name="batched_gyro_512",
     description=("5-layer stack with a 40 nm gyrotropic film (g=0.3, Im eps=0.02), uniform 512x512 kx,ky grid "
