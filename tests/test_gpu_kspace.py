@@ -96,3 +96,44 @@ def test_real_axis_poles_against_oracle(label, stack, s, phis, k_lo, k_hi, exclu
     if label == "base":
         strongest = sorted(g, key=lambda p: -p["prom"])[:2]
         assert round(strongest[0]["q"] / W0, 4) == 1.0624 and round(strongest[1]["q"] / W0, 2) == 1.66
+
+
+def test_fixed_rule_resolves_the_plasmon_pole_end_to_end():
+    """kspace_features (GPU pole scan) + fixed_krho_rule on a dipole above a
+    metal: the 1D closed_form_G of the rule agrees with a brute-force rule
+    (uniform Gauss-Legendre panels far finer than the plasmon's width, the
+    u^2 substitution at the vacuum branch point) and with the rule's own
+    halved rule (its convergence check); the same rule without the pole
+    features misses the plasmon."""
+    from yellowthinfilms_amd import CavityTransfer1D, closed_form_G
+    from yellowthinfilms_amd.kspace import kspace_features, fixed_krho_rule
+    from yellowthinfilms_amd.quadrature import gl_rule, halve_panels
+    print("case: fixed rule end to end (surface plasmon)", flush=True)
+    stack = spp_stack(-4 + 0.1j)
+    zs, ho = 0.05, 0.08
+    h_min = ho - zs
+    feat = kspace_features(stack, W0, 1, h_min=h_min)
+    k = W0 * np.sqrt((-4 + 0.1j) / (-3 + 0.1j))
+    assert len(feat["poles"]) == 1 and abs(feat["poles"][0]["q"] - k.real) < 2e-4 * k.real
+    xs = np.array([0.1, 0.5, 1.0])
+
+    def G_of(nodes, wts):
+        cache = CavityTransfer1D(stack, W0, nodes, wts, 1)
+        return closed_form_G(cache, 1, zs, [1.0 - ho], xs, [0.0]).cpu().numpy()
+
+    nodes, wts, panels = fixed_krho_rule(feat)
+    G = G_of(nodes, wts)
+    n2, w2 = gl_rule(halve_panels(panels), order=8, singular=feat["singular_lines"])
+    Gh = G_of(n2, w2)
+    K = feat["Kcut"]
+    edges = np.concatenate([np.linspace(0, W0, 200), np.linspace(W0, K, int((K - W0) / 0.01) + 1)[1:]])
+    nb, wb = gl_rule(list(zip(edges[:-1], edges[1:])), order=16, singular=[W0])
+    Gb = G_of(nb, wb)
+    f0 = dict(feat, clusters=[])
+    n0, w0_, _ = fixed_krho_rule(f0)
+    G0 = G_of(n0, w0_)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))  # noqa: E731
+    print(f"  rule {len(nodes)} nodes: vs brute force ({len(nb)} nodes) {rel(G, Gb):.2e}, vs halved "
+          f"{rel(G, Gh):.2e}; without the pole ({len(n0)} nodes) {rel(G0, Gb):.2e}", flush=True)
+    assert rel(G, Gb) < 1e-5 and rel(G, Gh) < 1e-5
+    assert rel(G0, Gb) > 10 * rel(G, Gb)

@@ -12,6 +12,10 @@ plasmon or polariton poles, found before any quadrature is built.
                       all candidates at once (the reference refines one
                       candidate after another);
   clusters         -- _clusters (:228-241), the intervals a rule resolves;
+  kspace_features  -- (:316-379) light lines, cut-off, poles, clusters, N_phi;
+  fixed_krho_rule  -- fixed_kρ_rule (:407-433), the graded Gauss-Legendre rule
+                      built from the features;
+  adaptive_seeds   -- (:448-460) the seeds of an adaptive grid;
   material_class, choose_engine -- (:54-82).
 """
 import ctypes as ct
@@ -213,3 +217,156 @@ def choose_engine(stack, engine="auto"):
     if engine != "auto":
         return engine
     return "adaptive" if all(material_class(l) == "dielectric" for l in stack) else "fixed"
+
+
+def _is_azimuthally_symmetric(stack):
+    """_is_azimuthally_symmetric_stack (mod_ModeSolver.jl:427-434): every layer
+    takes the closed-form branch (isotropic / uniaxial eps about z, isotropic mu)."""
+    from .materials import EPS_ISOTROPIC, EPS_UNIAXIAL, IsotropicMaterial
+    return all(l.eps.kind in (EPS_ISOTROPIC, EPS_UNIAXIAL) and isinstance(l.mu, IsotropicMaterial) for l in stack)
+
+
+def _pole_track_eta(poles, phi_scan):
+    """_pole_track_η (mod_KSpaceFeatures.jl:257-275): the smallest phi-width
+    Gamma dphi / |dq| of the phi-peaks that poles moving with phi put in the 2D
+    integrand (inf with fewer than two directions or no track)."""
+    ph = sorted(set(float(p) for p in phi_scan))
+    if len(ph) < 2:
+        return math.inf
+    pairs = [(ph[i], ph[i + 1], ph[i + 1] - ph[i]) for i in range(len(ph) - 1)]
+    step = ph[1] - ph[0]
+    if abs(ph[-1] - ph[0] + step - 2 * math.pi) < 1e-9:
+        pairs.append((ph[-1], ph[0], step))
+    eta = math.inf
+    for (f1, f2, dphi) in pairs:
+        for p in poles:
+            if p["phi"] != f1:
+                continue
+            for r in poles:
+                if r["phi"] != f2:
+                    continue
+                dq = abs(p["q"] - r["q"])
+                if 0 < dq <= dphi * max(p["q"], r["q"]):
+                    eta = min(eta, min(p["Gamma"], r["Gamma"]) * dphi / dq)
+    return eta
+
+
+def kspace_features(stack, omega, source_layer, h_min, rho_max=0.0, dims=1, n_phi_scan=32, target_decay=25.0,
+                    krho_cap=None, krho_cap_rel=1e5, scan=True, scan_step=1e-3, prominence=1.05, c_phi=2.5,
+                    n_phi_min=64, Kcut=None, phi_scan=None, rtol=1e-6, c_eta=1.5, device=0):
+    """kspace_features (mod_KSpaceFeatures.jl:316-379): light lines (all layers,
+    every scanned direction; the semi-infinite layers' ones as singular lines),
+    kmax, the cut-off Kcut (target_decay / h_min within [2 kmax, cap]), the
+    real-axis poles of the source layer (real_axis_poles on the GPU) and their
+    clusters, and in 2D the phi count N_phi = max(n_phi_min, c_phi Kcut rho_max,
+    c_eta log(10 / rtol) / eta_min) rounded up to even."""
+    from . import quadrature as Q
+    if dims not in (1, 2):
+        raise ValueError("kspace_features: dims must be 1 or 2")
+    N = len(stack)
+    w = float(omega)
+    classes = [material_class(l) for l in stack]
+    sym = _is_azimuthally_symmetric(stack)
+    eps_mu = [(np.asarray(l.eps.eps), np.asarray(l.mu.eps)) for l in stack]
+    diag = all(np.count_nonzero(e - np.diag(np.diag(e))) == 0 and np.count_nonzero(m - np.diag(np.diag(m))) == 0
+               for e, m in eps_mu)
+    if dims == 1 or sym:
+        phis, merge_rel = [0.0], 0.0
+    elif phi_scan is not None:
+        phis = sorted(float(p) for p in phi_scan)
+        merge_rel = min(np.diff(phis)) if len(phis) > 1 else 0.0
+    elif diag:
+        phis = list(np.linspace(0.0, math.pi / 2, max(2, n_phi_scan // 4 + 1)))
+        merge_rel = (math.pi / 2) / max(1, n_phi_scan // 4)
+    else:
+        phis = [2 * math.pi * p / n_phi_scan for p in range(n_phi_scan)]
+        merge_rel = 2 * math.pi / n_phi_scan
+    phi_ll = [0.0] if (dims == 1 or sym) else sorted(set(float(np.mod(p, math.pi)) for p in phis))
+    ll, sing = [], []
+    for j, lay in enumerate(stack):
+        for p in phi_ll:
+            L = Q.layer_light_lines(lay, w, p)
+            ll += L
+            if j == 0 or j == N - 1:
+                sing += L
+    ll = Q.dedup(sorted(ll))
+    sing = Q.dedup(sorted(sing))
+    kmax = ll[-1] if ll else w
+    K_geom = Q.decay_cutoff(h_min, target_decay)
+    K_cap = krho_cap_rel * kmax if krho_cap is None else float(krho_cap)
+    capped = False
+    if Kcut is None:
+        Kcut = Q.tail_bound(K_geom, kmax, krho_cap=K_cap, kx_ratio_min=2.0)
+        capped = Kcut < K_geom
+    else:
+        Kcut = float(Kcut)
+    poles = []
+    if scan:
+        k_lo = 0.5 * (ll[0] if ll else w)
+        poles = real_axis_poles(stack, w, source_layer, phis=phis, k_lo=k_lo, k_hi=Kcut, scan_step=scan_step,
+                                prominence=prominence, exclude=ll, device=device)
+    cl = clusters(poles, merge_rel)
+    n_lat = int(math.ceil(c_phi * Kcut * rho_max))
+    eta = _pole_track_eta(poles, phis)
+    n_pole = int(math.ceil(c_eta * math.log(10 / rtol) / eta)) if math.isfinite(eta) else 0
+    n_phi = 2 * -(-max(n_phi_min, n_lat, n_pole) // 2) if dims == 2 else 0
+    return dict(classes=classes, light_lines=ll, singular_lines=sing, kmax=kmax, Kcut=Kcut, h_min=float(h_min),
+                rho_max=float(rho_max), poles=poles, clusters=cl, N_phi=n_phi, N_phi_lateral=n_lat,
+                N_phi_poles=n_pole, eta_min=eta, phi_scan=phis, dims=dims, K_geom=K_geom, K_cap=K_cap, capped=capped)
+
+
+def fixed_krho_rule(feat, order=8, rtol=1e-6, alpha=0.5, grade=12, r_rel=0.05, c_h=4.0, c_rho=4.0,
+                    pole_safety=0.5):
+    """fixed_kρ_rule (mod_KSpaceFeatures.jl:407-433): composite Gauss-Legendre
+    (`order` nodes per panel) on [0, Kcut], breaks at every light line, panels
+    graded to the smallest of the background width min(c_h / h_min,
+    c_rho / rho_max, max(r_rel k, kmax / 40)), max(L 2^-grade, alpha |k - L|)
+    at every singular light line L (whose touching panels are integrated in
+    u, k = L -+ u^2), and max(w_Gamma, alpha dist(k, [a, b])) at every pole
+    cluster, w_Gamma = pole_safety 2 Gamma / eps_B (the Bernstein-ellipse half
+    width that keeps the panel's error below rtol).  Returns (nodes, weights, panels)."""
+    from . import quadrature as Q
+    if r_rel > alpha:
+        raise ValueError("fixed_kρ_rule: r_rel must be ≤ α")
+    rhoB = (1 / float(rtol)) ** (1 / (2 * order))
+    epsB = (rhoB - 1 / rhoB) / 2
+    w_flat = min(c_h / feat["h_min"], c_rho / feat["rho_max"] if feat["rho_max"] > 0 else math.inf)
+    w_floor = feat["kmax"] / 40
+    sing = np.asarray(feat["singular_lines"], np.float64)
+    ca = np.asarray([c["a"] for c in feat["clusters"]], np.float64)
+    cb = np.asarray([c["b"] for c in feat["clusters"]], np.float64)
+    wG = pole_safety * 2 * np.asarray([c["Gamma"] for c in feat["clusters"]], np.float64) / epsB
+
+    def width(k):
+        wk = min(w_flat, max(r_rel * k, w_floor))
+        if sing.size:
+            wk = min(wk, float(np.min(np.maximum(sing * 2.0 ** -grade, alpha * np.abs(k - sing)))))
+        if ca.size:
+            dist = np.where(k < ca, ca - k, np.where(k > cb, k - cb, 0.0))
+            wk = min(wk, float(np.min(np.maximum(wG, alpha * dist))))
+        return wk
+
+    breaks = [0.0] + [L for L in feat["light_lines"] if L < feat["Kcut"]] + [feat["Kcut"]]
+    panels = Q.graded_panels(breaks, width, alpha=alpha)
+    nodes, wts = Q.gl_rule(panels, order=order, singular=feat["singular_lines"])
+    return nodes, wts, panels
+
+
+def adaptive_seeds(feat, rtol=1e-6, n_gamma=4.0, grade=None):
+    """adaptive_seeds (mod_KSpaceFeatures.jl:448-460): starting breakpoints for an
+    adaptive k_rho grid: light lines, L(1 -+ 2^-j) grading onto the singular
+    ones, each cluster's [a - n Gamma, a, b, b + n Gamma], and x2 seeds from
+    2 kmax to Kcut."""
+    if grade is None:
+        grade = min(50, int(math.ceil(2 * math.log2(10 / float(rtol)))))
+    s = list(feat["light_lines"])
+    for L in feat["singular_lines"]:
+        for j in range(1, grade + 1):
+            s += [L * (1 - 2.0 ** -j), L * (1 + 2.0 ** -j)]
+    for c in feat["clusters"]:
+        s += [c["a"] - n_gamma * c["Gamma"], c["a"], c["b"], c["b"] + n_gamma * c["Gamma"]]
+    k = 2 * feat["kmax"]
+    while k < feat["Kcut"]:
+        s.append(k)
+        k *= 2
+    return sorted(set(x for x in s if 0 < x < feat["Kcut"]))
