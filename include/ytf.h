@@ -285,8 +285,10 @@ int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev
  * branch point), k < origin_tol omega, or |y|_2 > t_big (near a pole).  With
  * fallback, the flagged nodes are solved exactly (ytf_green_nodes with source
  * and observation layer 1, depths zsrc / zobs, YTF_TERMS_SCATTERED) and their
- * g~ replaced.  *n_flagged = number of flagged nodes.  Synchronizes the
- * context's stream once (to size the fallback). */
+ * g~ replaced (the nodes flagged from their inputs alone before the network,
+ * the others after it; every count stays on the device).  n_flagged may be
+ * NULL: the call then only queues work on the context's stream; otherwise it
+ * waits for that stream and sets *n_flagged = number of flagged nodes. */
 int ytf_mlp_surrogate_ghat(ytf_ctx *ctx, ytf_mlp *mlp, const ytf_layer *layers, int n_layers, double omega,
                            const double *kx_dev, const double *ky_dev, int64_t n, const float *xconst, double zsum,
                            double zsrc, double zobs, double branch_tol, double origin_tol, double t_big, int fallback,

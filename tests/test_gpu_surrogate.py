@@ -174,8 +174,21 @@ def test_fused_surrogate_matches_the_composed_path():
     err = float(torch.linalg.norm((g - g_ref)[ok]) / torch.linalg.norm(g_ref[ok]))
     print(f"  flagged {int(mask.sum())} of {n}; rel err vs composed {err:.2e}", flush=True)
     assert err < 1e-12, err
-    gf, maskf = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_factor=2.0)
-    assert sg.last_flagged == int(maskf.sum())
-    exact = sg.exact_scattered(kx[maskf], ky[maskf], hs, ho, omega, loss)
-    assert torch.equal(gf[maskf], exact)
-    assert torch.equal(gf[~maskf], g[~maskf])
+    # the input-flagged nodes (light line, origin) are solved beside the
+    # network, the response-flagged ones after it: both sets non-empty here
+    kr = torch.sqrt(kx * kx + ky * ky)
+    near = ((kr - omega).abs() < sg.BRANCH_TOL * omega) | (kr < sg.ORIGIN_TOL * omega)
+    for pf in (2.0, 0.7):
+        gf, maskf = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_factor=pf)
+        n_near, n_big = int(near.sum()), int((maskf & ~near).sum())
+        print(f"  pole_factor {pf}: {n_near} input-flagged, {n_big} response-flagged", flush=True)
+        assert n_near > 0 and torch.equal(maskf & near, near)
+        assert torch.equal(maskf, sg.flags(kx, ky, t, omega, pole_factor=pf))
+        assert sg.last_flagged == int(maskf.sum())
+        exact = sg.exact_scattered(kx[maskf], ky[maskf], hs, ho, omega, loss)
+        assert torch.equal(gf[maskf], exact)
+        assert torch.equal(gf[~maskf], g[~maskf])
+        # without the count the call only queues the same work
+        gq, mq = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_factor=pf, count=False)
+        assert torch.equal(gq, gf) and torch.equal(mq, maskf)
+    assert n_big > 0

@@ -54,6 +54,7 @@ int ytf_ctx_destroy(ytf_ctx *c) {
   if (c->small) hipFree(c->small);
   if (c->work) hipFree(c->work);
   if (c->aux) hipFree(c->aux);
+  if (c->pinned) hipHostFree(c->pinned);
   if (c->own) hipStreamDestroy(c->own);
   delete c;
   return YTF_OK;
@@ -113,10 +114,14 @@ static bool stack_needs_general(const ytf_layer *layers, int n_layers) {
   return false;
 }
 
-int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
-                       const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
-                       const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
-                       int terms, double dz_box, int ez_local) {
+// Validate the arguments of a node solve, send the stack and depth pairs to
+// the device (when they differ from what its copy holds) and fill *P.  With
+// n_nodes == 0 nothing is sent and P->nn = 0.
+static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                              const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                              const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                              int terms, double dz_box, int ez_local, ytf::NodeParams *P_out) {
+  P_out->nn = 0;
   if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_green_nodes: need at least 2 layers (got %d)", n_layers);
   if (source_layer < 1 || source_layer > n_layers - 1)
     return fail(c, YTF_EINVAL,
@@ -174,7 +179,7 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
     c->small_host = hb;
   }
   char *sm = (char *)c->small;
-  ytf::NodeParams P;
+  ytf::NodeParams &P = *P_out;
   P.L = (const LayerDev *)sm;
   P.nl = n_layers;
   P.omega = omega;
@@ -198,12 +203,25 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
   P.flags = cache ? (unsigned char *)cache->flags_dev : nullptr;
   if ((P.sf == nullptr) != (P.sb == nullptr)) return fail(c, YTF_EINVAL, "ytf_green_nodes: sf and sb go together");
   P.needs_general = stack_needs_general(layers, n_layers);
-  int64_t blocks = (n_nodes + 63) / 64;
+  return YTF_OK;
+}
+
+// One node solve of P.nn nodes on stream s (one wave of 64 nodes per block).
+static hipError_t launch_node_solve(const ytf::NodeParams &P, hipStream_t s) {
+  int64_t blocks = (P.nn + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
-  hipError_t e = timed(c, "node_solve", [&] {
-    return (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, c->stream)
-                                      : ytf::launch_node_general(P, (int)blocks, c->stream);
-  });
+  return (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, s) : ytf::launch_node_general(P, (int)blocks, s);
+}
+
+int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
+                       const double *ky_dev, int64_t n_nodes, int source_layer, int obs_layer, int n_pairs,
+                       const double *zsrc, const double *zobs, double *ghat_dev, const ytf_node_cache *cache,
+                       int terms, double dz_box, int ez_local) {
+  ytf::NodeParams P;
+  int rc = node_params_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n_nodes, source_layer, obs_layer, n_pairs,
+                              zsrc, zobs, ghat_dev, cache, terms, dz_box, ez_local, &P);
+  if (rc || P.nn == 0) return rc;
+  hipError_t e = timed(c, "node_solve", [&] { return launch_node_solve(P, c->stream); });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_green_nodes: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }
@@ -1071,57 +1089,90 @@ int ytf_mlp_surrogate_ghat(ytf_ctx *c, ytf_mlp *m, const ytf_layer *layers, int 
   if (!c || !m) return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: NULL argument");
   std::lock_guard<std::mutex> lk(c->mu);
   if (n < 0 || n > INT32_MAX || (n > 0 && (!kx_dev || !ky_dev || !ghat_dev || !flags_dev)) ||
-      (m->nin > 2 && !xconst) || !n_flagged)
+      (m->nin > 2 && !xconst))
     return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: bad arguments");
   if (m->nout != 18 || m->nin < 2 || !mlp_fused_ok(m))
     return fail(c, YTF_EINVAL,
                 "ytf_mlp_surrogate_ghat: needs 18 outputs, >= 2 inputs and a hidden width of 128 or 256 (got %d, %d, %d)",
                 m->nout, m->nin, m->width);
   if (fallback && (!layers || n_layers < 2)) return fail(c, YTF_EINVAL, "ytf_mlp_surrogate_ghat: no stack for the fallback");
-  *n_flagged = 0;
+  if (n_flagged) *n_flagged = 0;
   if (n == 0) return YTF_OK;
   HIPCHK(c, hipSetDevice(c->device));
-  // flagged-row counter and list
-  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, align256(sizeof(int)) + align256(sizeof(int) * (size_t)n));
+  // The rows flagged from their inputs alone (light line, origin: k_sur_preflag)
+  // are known before the network runs: their exact solve (row-list node solve,
+  // writing g~ in place) needs no host round trip, and the network leaves their
+  // g~ alone.  The rows the network flags (large response) are solved after it,
+  // their count also read on the device: nothing waits on the host unless the
+  // flagged count is asked for.
+  ytf::NodeParams P;
+  if (fallback) {  // validates and sends the stack before this call's first launch
+    int rc = node_params_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n, 1, 1, 1, &zsrc, &zobs, ghat_dev,
+                                nullptr, YTF_TERMS_SCATTERED, 0.0, 1, &P);
+    if (rc) return rc;
+  }
+  // counters (input-flagged, response-flagged) and the two row lists
+  const size_t cb = align256(2 * sizeof(int)), lb = align256(sizeof(int) * (size_t)n);
+  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, cb + 2 * lb);
   if (rc) return rc;
   int *count = (int *)c->aux;
-  int *list = (int *)((char *)c->aux + align256(sizeof(int)));
+  int *list_in = (int *)((char *)c->aux + cb), *list_big = (int *)((char *)c->aux + cb + lb);
   ytf::MlpFused p = mlp_fused_params(m, n);
   p.kx = kx_dev;
   p.ky = ky_dev;
   for (int i = 0; i + 2 < m->nin; ++i) p.xconst[i] = xconst[i];
   p.omega = omega;
   p.zsum = zsum;
-  p.branch_tol = branch_tol;
-  p.origin_tol = origin_tol;
   p.tbig = t_big;
+  p.skip_flagged = fallback ? 1 : 0;
   p.ghat = ghat_dev;
   p.flags = flags_dev;
-  p.n_flagged = count;
-  p.flagged = list;
-  hipError_t e = ytf::launch_mlp_zero(1, (float *)count, c->stream);  // zero bits = integer 0
-  if (e == hipSuccess) e = mlp_pack(c, m);
+  p.n_flagged = count + 1;
+  p.flagged = list_big;
+  auto launch_err = [&](hipError_t e) {
+    return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
+  };
+  hipError_t e = ytf::launch_mlp_zero(2, (float *)count, c->stream);  // zero bits = integer 0
+  if (e == hipSuccess)
+    e = ytf::launch_sur_preflag(n, kx_dev, ky_dev, omega, branch_tol, origin_tol, flags_dev, count, list_in, c->stream);
+  if (e != hipSuccess) return launch_err(e);
+  // exact solve of the rows list[0 .. *cnt), g~ written in place (a grid of at
+  // most 1024 one-wave blocks; the kernel strides over the count)
+  auto solve_rows = [&](const int *list, const int *cnt) -> hipError_t {
+    ytf::NodeParams Q = P;
+    Q.kx = kx_dev;
+    Q.ky = ky_dev;
+    Q.nn = n;
+    Q.ghat = (cd *)ghat_dev;
+    Q.rows = list;
+    Q.n_rows = cnt;
+    const int blocks = (int)std::min<int64_t>((n + 63) / 64, 1024);
+    return timed(c, "node_solve", [&] {
+      return (Q.src == 0 && Q.obs == 0) ? ytf::launch_node_top(Q, blocks, c->stream)
+                                        : ytf::launch_node_general(Q, blocks, c->stream);
+    });
+  };
+  // (On a second stream beside the network this solve is starved: the
+  // network's blocks take every CU slot it frees, and it ends after the network.)
+  if (fallback) {
+    e = solve_rows(list_in, count);
+    if (e != hipSuccess) return launch_err(e);
+  }
+  e = mlp_pack(c, m);
   if (e == hipSuccess)
     e = timed(c, "mlp_fused", [&] { return ytf::launch_mlp_fused(p, m->width, true, c->stream); });
-  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
-  int nf = 0;
-  HIPCHK(c, hipMemcpyAsync(&nf, count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  *n_flagged = nf;
-  if (!fallback || nf == 0) return YTF_OK;
-  // the exact FP64 solve (scattered terms, source and observation in layer 1) on the flagged rows
-  const size_t kb = align256(sizeof(double) * nf), gb = align256(sizeof(cd) * 9 * (size_t)nf);
-  rc = ensure_buf(c, &c->work, &c->work_bytes, 2 * kb + gb);
-  if (rc) return rc;
-  double *kxc = (double *)c->work, *kyc = (double *)((char *)c->work + kb);
-  double *gc = (double *)((char *)c->work + 2 * kb);
-  e = ytf::launch_mlp_gather_nodes(nf, list, kx_dev, ky_dev, kxc, kyc, c->stream);
-  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
-  rc = green_nodes_locked(c, layers, n_layers, omega, kxc, kyc, nf, 1, 1, 1, &zsrc, &zobs, gc, nullptr,
-                          YTF_TERMS_SCATTERED, 0.0, 1);
-  if (rc) return rc;
-  e = ytf::launch_mlp_scatter_rows(nf, list, gc, ghat_dev, n, c->stream);
-  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_mlp_surrogate_ghat: launch failed: %s", hipGetErrorString(e));
+  if (e != hipSuccess) return launch_err(e);
+  if (fallback) {  // the rows the network flagged, counted on the device
+    e = solve_rows(list_big, count + 1);
+    if (e != hipSuccess) return launch_err(e);
+  }
+  if (n_flagged) {
+    rc = ensure_pinned(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pinned, count, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n_flagged = (int64_t)c->pinned[0] + c->pinned[1];
+  }
   return YTF_OK;
 }
 

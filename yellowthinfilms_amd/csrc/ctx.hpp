@@ -40,6 +40,8 @@ struct ytf_ctx {
   size_t work_bytes = 0;
   void *aux = nullptr;
   size_t aux_bytes = 0;
+  // pinned host words for counts read back
+  int *pinned = nullptr;
   // kernel timing: events bracketing each launch while enabled
   bool timing = false;
   struct Mark {
@@ -99,6 +101,12 @@ inline int ensure_buf(ytf_ctx *c, void **buf, size_t *cur, size_t bytes) {
   return YTF_OK;
 }
 
+// Allocate the pinned words on first use.
+inline int ensure_pinned(ytf_ctx *c) {
+  if (!c->pinned) HIPCHK(c, hipHostMalloc((void **)&c->pinned, 256, hipHostMallocDefault));
+  return YTF_OK;
+}
+
 inline hipEvent_t take_event(ytf_ctx *c) {
   if (!c->pool.empty()) {
     hipEvent_t e = c->pool.back();
@@ -110,18 +118,22 @@ inline hipEvent_t take_event(ytf_ctx *c) {
   return e;
 }
 
-// Launch through `f`; while timing is on, bracket it with events on the
-// context's stream under `name`.
+// Launch through `f`; while timing is on, bracket it with events on stream
+// s (timed: the context's stream) under `name`.
 template <class F>
-inline hipError_t timed(ytf_ctx *c, const char *name, F &&f) {
+inline hipError_t timed_on(ytf_ctx *c, hipStream_t s, const char *name, F &&f) {
   if (!c->timing) return f();
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return f();
-  hipEventRecord(a, c->stream);
+  hipEventRecord(a, s);
   hipError_t e = f();
-  hipEventRecord(b, c->stream);
+  hipEventRecord(b, s);
   c->marks.push_back({name, a, b});
   return e;
+}
+template <class F>
+inline hipError_t timed(ytf_ctx *c, const char *name, F &&f) {
+  return timed_on(c, c->stream, name, static_cast<F &&>(f));
 }
 
 // Fold finished marks into the per-name totals (synchronizes the stream).

@@ -297,19 +297,27 @@ hipError_t launch_mlp_zero(int64_t n, float *x, hipStream_t s) {
 // 32-feature tile and 16-wide k step, two steps ahead; L2-resident, since
 // every block reads the same weights) and the activations (B operand) from LDS: the accumulator of a lane then holds 4
 // consecutive features of one row, stored back as one 8-byte LDS write.
-// Hidden layer: wave w owns features [w W/4, (w + 1) W/4) (W/128 tiles of 32)
-// for all 4 row tiles; output layer: ceil(nout / 32) feature tiles spread
-// over the waves.  The sums run over k in 16-wide steps in increasing order,
+// NW waves per block (8 for width 256: 4 waves per SIMD at two blocks per CU,
+// which hides the LDS and weight-load latency better than 4 waves per block;
+// 4 for width 128).  Hidden layer: wave w owns features [w W/NW, (w + 1) W/NW)
+// (W/(32 NW) tiles of 32) for all 4 row tiles; output layer: the 4 ceil(nout / 32)
+// (feature tile, row tile) pairs spread over the waves.  The sums run over k in 16-wide steps in increasing order,
 // as in k_gemm, and the epilogues (bias, SiLU, output scaling) are the same.
 // Two variants:
 //   generic:   x[n][nin] (normalised in-kernel, light-line features) -> y[n][nout];
 //   surrogate: inputs (kx/omega, ky/omega, constants); the 18 outputs t go
 //              through LDS to one thread per row, which forms
 //              g~_s = t omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0 in FP64
-//              ([9][n] complex, entry c + 3q), and flags the row (|k - omega| <
-//              branch_tol omega, k < origin_tol omega, or |t|_2 > tbig) into
-//              the flag array and the flagged-row list.
+//              ([9][n] complex, entry c + 3q).  The rows flagged from their
+//              inputs alone (|k - omega| < branch_tol omega or k < origin_tol
+//              omega) are already in the flag array (k_sur_preflag, launched
+//              before; with skip_flagged their g~ is left to the exact solve);
+//              a row not among them with |t|_2 > tbig is flagged here and
+//              appended to the flagged-row list.
 constexpr int FB = 128;
+#ifndef MLP_NW256
+#define MLP_NW256 8  // waves per block of the width-256 network
+#endif
 
 typedef __bf16 b4 __attribute__((ext_vector_type(4)));
 
@@ -335,7 +343,8 @@ __device__ __forceinline__ void half_phase(const b8 *Wp, const __bf16 *actr, f16
                                            f16v (&accE)[FT][2], int he, const float *bE, bool do_epi, __bf16 *act,
                                            int f0, int lr, int lk, int hi) {
   constexpr int NKS = WIDTH / 16;
-  static_assert(NKS == FT * 2 * 4, "one epilogue group per k step");
+  constexpr int NG = FT * 2 * 4;  // epilogue groups of the other half
+  static_assert(NKS % NG == 0, "epilogue groups spread evenly over the k steps");
 #pragma unroll
   for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -360,8 +369,9 @@ __device__ __forceinline__ void half_phase(const b8 *Wp, const __bf16 *actr, f16
 #pragma unroll
       for (int j = 0; j < 2; ++j)
         accM[ft][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[ft], bb[j], accM[ft][j], 0, 0, 0);
-    if (do_epi) {  // group k of the other half: tile (k / 8, (k / 4) % 2), rows 8 g + 4 hi .. of g = k % 4
-      const int ft = k / 8, j = (k / 4) % 2, g = k % 4;
+    if (do_epi && k % (NKS / NG) == 0) {  // group q of the other half: tile (q / 8, (q / 4) % 2), g = q % 4
+      const int q = k / (NKS / NG);
+      const int ft = q / 8, j = (q / 4) % 2, g = q % 4;
       epi4<LA>(act, (2 * he + j) * 32 + lr, f0 + ft * 32 + 8 * g + 4 * hi, accE[ft][j], g, bE);
     }
 #pragma unroll
@@ -384,10 +394,12 @@ __device__ __forceinline__ void half_epilogue(__bf16 *act, f16v (&acc)[FT][2], i
       for (int g = 0; g < 4; ++g) epi4<LA>(act, (2 * h + j) * 32 + lr, f0 + ft * 32 + 8 * g + 4 * hi, acc[ft][j], g, b);
 }
 
-template <int WIDTH, bool SUR>
-__global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
+template <int WIDTH, bool SUR, int NW>
+__global__ __launch_bounds__(64 * NW, 2) void k_mlp_fused(MlpFused p) {
+  constexpr int NT = 64 * NW;         // threads per block
   constexpr int LA = WIDTH + 8;
-  constexpr int FT = WIDTH / 128;
+  constexpr int FT = WIDTH / (32 * NW);  // 32-feature tiles per wave in a hidden layer
+  static_assert(FT >= 1, "at least one feature tile per wave");
   constexpr int TS = 20;  // floats per row of the surrogate's output staging
   static_assert(FB * TS * 4 <= FB * LA * 2, "output staging fits the activations");
   __shared__ __bf16 act[FB * LA];
@@ -400,9 +412,9 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
   const int kin = p.nin + 2 * p.kzf;
   const int k0e = (kin + 15) / 16 * 16;
 
-  for (int e = tid; e < (p.nl - 1) * WIDTH; e += 256) bsh[e] = p.P32[p.offb[e / WIDTH] + e % WIDTH];
+  for (int e = tid; e < (p.nl - 1) * WIDTH; e += NT) bsh[e] = p.P32[p.offb[e / WIDTH] + e % WIDTH];
   // layer-0 inputs, zero padded to k0e columns
-  for (int e = tid; e < FB * k0e; e += 256) {
+  for (int e = tid; e < FB * k0e; e += NT) {
     const int r = e / k0e, c = e % k0e;
     const int64_t row = r0 + r;
     float v = 0.0f;
@@ -434,7 +446,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
   //   then the epilogue of B of the last hidden layer.
   // Every bracket ends with a barrier: a half's rows are rewritten only after
   // every wave has read them, and read only after every wave has written them.
-  const int f0 = wave * (WIDTH / 4);
+  const int f0 = wave * (WIDTH / NW);
   f16v accA[FT][2], accB[FT][2];
   {
     const int ksl = p.kp0 / 16, nks = k0e / 16;
@@ -478,16 +490,18 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
     const int l = p.nl - 1;
     const __bf16 *W = p.W16 + p.offW[l];
     const float *b = p.P32 + p.offb[l];
+    // tiles t = 4 ft + st (nt feature tiles x 4 row tiles), ceil(4 nt / NW) consecutive ones per wave
     const int nt = (p.nout + 31) / 32;
-    const int ft = nt == 1 ? 0 : (wave & 1);
-    const int st0 = nt == 1 ? wave : 2 * (wave >> 1);
-    const int nst = nt == 1 ? 1 : 2;
+    const int per = (4 * nt + NW - 1) / NW;
+    const int t0 = wave * per;
+    const int nst = t0 >= 4 * nt ? 0 : min(per, 4 * nt - t0);
+    const int ft = t0 / 4, st0 = t0 % 4;
     f16v acc[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[j][e] = 0.0f;
-    const b8 *Wp = (const b8 *)W + (int64_t)ft * (WIDTH / 16) * 64 + lane;
+    const b8 *Wp = (const b8 *)W + (int64_t)(nst ? ft : 0) * (WIDTH / 16) * 64 + lane;
     b8 a0 = Wp[0], a1 = Wp[64];
     for (int k = 0; k < WIDTH / 16; ++k) {
       const b8 a2 = k + 2 < WIDTH / 16 ? Wp[(k + 2) * 64] : a1;
@@ -529,19 +543,24 @@ __global__ __launch_bounds__(256, 2) void k_mlp_fused(MlpFused p) {
 #pragma unroll
         for (int c = 0; c < 18; ++c) tn2 += (double)t[c] * (double)t[c];
         const double kx = p.kx[row], ky = p.ky[row], w = p.omega;
-        const double k2 = kx * kx + ky * ky, kr = sqrt(k2);
-        const bool flag = fabs(kr - w) < p.branch_tol * w || kr < p.origin_tol * w || sqrt(tn2) > p.tbig;
+        const double k2 = kx * kx + ky * ky;
+        const bool pre = p.flags[row] != 0;
+        const bool big = !pre && sqrt(tn2) > p.tbig;
         const cd kz = csqrt(cd{w * w - k2, 0.0});
         const cd P = cexp(cd{-kz.i * p.zsum, kz.r * p.zsum});
         const cd fac = (cd{w * P.r, w * P.i} * cd{1.0 + k2 / (w * w), 0.0}) / kz;
+        if (!(pre && p.skip_flagged)) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-          const cd v = cd{(double)t[e], (double)t[9 + e]} * fac;
-          p.ghat[2 * (e * p.n + row)] = v.r;
-          p.ghat[2 * (e * p.n + row) + 1] = v.i;
+          for (int e = 0; e < 9; ++e) {
+            const cd v = cd{(double)t[e], (double)t[9 + e]} * fac;
+            p.ghat[2 * (e * p.n + row)] = v.r;
+            p.ghat[2 * (e * p.n + row) + 1] = v.i;
+          }
         }
-        p.flags[row] = flag ? 1 : 0;
-        if (flag) p.flagged[atomicAdd(p.n_flagged, 1)] = (int)row;
+        if (big) {
+          p.flags[row] = 1;
+          p.flagged[atomicAdd(p.n_flagged, 1)] = (int)row;
+        }
       }
     }
   }
@@ -576,43 +595,44 @@ hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStr
   if (p.n <= 0) return hipSuccess;
   const dim3 grid((unsigned)((p.n + FB - 1) / FB));
   if (width == 256) {
-    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<256, true>), grid, dim3(256), 0, s, p);
-    else hipLaunchKernelGGL((k_mlp_fused<256, false>), grid, dim3(256), 0, s, p);
+    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<256, true, MLP_NW256>), grid, dim3(64 * MLP_NW256), 0, s, p);
+    else hipLaunchKernelGGL((k_mlp_fused<256, false, MLP_NW256>), grid, dim3(64 * MLP_NW256), 0, s, p);
   } else {
-    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<128, true>), grid, dim3(256), 0, s, p);
-    else hipLaunchKernelGGL((k_mlp_fused<128, false>), grid, dim3(256), 0, s, p);
+    if (surrogate) hipLaunchKernelGGL((k_mlp_fused<128, true, 4>), grid, dim3(256), 0, s, p);
+    else hipLaunchKernelGGL((k_mlp_fused<128, false, 4>), grid, dim3(256), 0, s, p);
   }
   return hipGetLastError();
 }
 
-__global__ void k_mlp_scatter_rows(int n, const int *__restrict__ idx, const double *__restrict__ src,
-                                   double *__restrict__ ghat, int64_t ld) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= 9 * (int64_t)n) return;
-  const int e = (int)(t / n), i = (int)(t % n);
-  const int64_t o = 2 * (e * ld + idx[i]);
-  ghat[o] = src[2 * t];
-  ghat[o + 1] = src[2 * t + 1];
+// Input flags of the surrogate rows: flags[i] = 1 where |k - omega| <
+// branch_tol omega or k < origin_tol omega (near the light line or the
+// origin), else 0; the flagged rows are appended to list (one counter
+// increment per wave).
+__global__ void k_sur_preflag(int64_t n, const double *__restrict__ kx, const double *__restrict__ ky, double w,
+                              double branch_tol, double origin_tol, unsigned char *__restrict__ flags,
+                              int *__restrict__ count, int *__restrict__ list) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool f = false;
+  if (i < n) {
+    const double a = kx[i], b = ky[i];
+    const double kr = sqrt(a * a + b * b);
+    f = fabs(kr - w) < branch_tol * w || kr < origin_tol * w;
+    flags[i] = f ? 1 : 0;
+  }
+  const unsigned long long bal = __ballot(f);
+  if (bal == 0) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(count, __popcll(bal));
+  base = __shfl(base, 0);
+  if (f) list[base + __popcll(bal & ((1ull << lane) - 1))] = (int)i;
 }
 
-__global__ void k_mlp_gather_nodes(int n, const int *__restrict__ idx, const double *__restrict__ kx,
-                                   const double *__restrict__ ky, double *__restrict__ kxo, double *__restrict__ kyo) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  kxo[i] = kx[idx[i]];
-  kyo[i] = ky[idx[i]];
-}
-
-hipError_t launch_mlp_scatter_rows(int n, const int *idx, const double *src, double *ghat, int64_t ld, hipStream_t s) {
+hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, double w, double branch_tol,
+                              double origin_tol, unsigned char *flags, int *count, int *list, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_scatter_rows, dim3((unsigned)((9 * (int64_t)n + 255) / 256)), dim3(256), 0, s, n, idx, src,
-                     ghat, ld);
-  return hipGetLastError();
-}
-hipError_t launch_mlp_gather_nodes(int n, const int *idx, const double *kx, const double *ky, double *kxo, double *kyo,
-                                   hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_gather_nodes, dim3((n + 255) / 256), dim3(256), 0, s, n, idx, kx, ky, kxo, kyo);
+  hipLaunchKernelGGL(k_sur_preflag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, kx, ky, w, branch_tol,
+                     origin_tol, flags, count, list);
   return hipGetLastError();
 }
 

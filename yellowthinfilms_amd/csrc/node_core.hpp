@@ -1065,6 +1065,7 @@ __device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, c
 
 template <int CASE, bool GENERAL>
 __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
+  if (P.rows) k = P.rows[k];
   const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
   bool ok = true, sing = false;
@@ -1450,7 +1451,8 @@ __global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
   __shared__ cd lds[LdsSlots<CASE>::n * 64];
   const LaneMat st{lds, (int)threadIdx.x};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < P.nn; k += stride)
+  const int64_t n = P.n_rows ? min((int64_t)*P.n_rows, P.nn) : P.nn;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
     solve_node<CASE, GENERAL>(P, k, st);
 }
 

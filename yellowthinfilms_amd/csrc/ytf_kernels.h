@@ -38,6 +38,10 @@ struct NodeParams {
   int terms;                  // TERMS_*: which terms of g~
   double dz;                  // box thickness (0: point source)
   int ez_local;               // add the local Ez inside the box's slab
+  // Optional row list: node k < *n_rows (at most nn) solves row rows[k] of
+  // the inputs and outputs (nn is then their row count); null: node k, k < nn.
+  const int *rows = nullptr;
+  const int *n_rows = nullptr;
 };
 
 // term classes of the spectral dyadic (mod_ClosedFormField.jl:143-160)
@@ -149,10 +153,13 @@ struct MlpFused {
   float *y;
   // surrogate evaluation: inputs (kx/omega, ky/omega, xconst[0 .. nin-3]);
   // output t (18 reals) -> g~_s = t omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0
-  // as [9][n] complex, flags and the list of flagged rows
+  // as [9][n] complex (not written for rows with flags[row] != 0 when
+  // skip_flagged); rows with flags[row] == 0 and |t|_2 > tbig are flagged and
+  // appended to the list of flagged rows
   const double *kx, *ky;
   float xconst[62];
-  double omega, zsum, branch_tol, origin_tol, tbig;
+  double omega, zsum, tbig;
+  int skip_flagged;
   double *ghat;
   unsigned char *flags;
   int *n_flagged, *flagged;
@@ -162,10 +169,10 @@ bool mlp_fused_supported(int width, int nin_total, int nout, int nl);
 // of (tile, step) holds row 32 tile + L % 32, columns 16 step + 8 (L / 32) .. + 8.
 hipError_t launch_mlp_pack(int N, int K, const __bf16 *W, __bf16 *Wpk, hipStream_t s);
 hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStream_t s);
-// g~[e][idx[i]] = src[e][i] for the n flagged rows (e < 9), complex.
-hipError_t launch_mlp_scatter_rows(int n, const int *idx, const double *src, double *ghat, int64_t ld, hipStream_t s);
-hipError_t launch_mlp_gather_nodes(int n, const int *idx, const double *kx, const double *ky, double *kxo, double *kyo,
-                                   hipStream_t s);
+// Input flags of the surrogate rows (near the light line or the origin) into
+// flags[n] (0/1), the flagged rows appended to list through *count.
+hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, double w, double branch_tol,
+                              double origin_tol, unsigned char *flags, int *count, int *list, hipStream_t s);
 
 // 2D type-1 nonuniform FFT (nufft.hip).
 int nu_width(double eps);

@@ -315,11 +315,14 @@ class SurrogateGreen:
         return near_branch | near_origin | big
 
     def scattered_ghat(self, kx, ky, hs_nm, ho_nm, omega, loss, fallback=True, branch_tol=None, origin_tol=None,
-                       pole_factor=3.0):
+                       pole_factor=3.0, count=True):
         """g~_s (n, 9) complex on the device; returns (g, flagged mask).
-        One call of ytf_mlp_surrogate_ghat: the network, the exact factors
-        (ghat_from_target) and the flags (`flags`) in one kernel, then the
-        exact FP64 solve of the flagged nodes."""
+        One call of ytf_mlp_surrogate_ghat: the flags of the nodes near the
+        light line or the origin, the network with the exact factors
+        (ghat_from_target) and the response flags (`flags`) in one kernel, the
+        exact FP64 solve of the input-flagged nodes before it and that of the
+        response-flagged nodes after it.  count=True waits for the stream and
+        sets `last_flagged`; count=False only queues the work."""
         torch = _torch()
         from .green import layer_structs
         dev = torch.device("cuda", self.device)
@@ -340,6 +343,7 @@ class SurrogateGreen:
             ctx.handle, self.mlp.handle, layers, len(st), float(omega), _p(kx), _p(ky), n, xc,
             float((hs_nm + ho_nm) * NM), float(hs_nm * NM), float(st[0].d - ho_nm * NM), float(branch_tol),
             float(origin_tol), float(pole_factor * self.t_ref), 1 if fallback else 0, _p(g), _p(flags),
-            ct.byref(nflag)), ctx.handle)
-        self.last_flagged = int(nflag.value)
-        return g.T, flags.bool()
+            ct.byref(nflag) if count else None), ctx.handle)
+        if count:
+            self.last_flagged = int(nflag.value)
+        return g.T, flags.view(torch.bool)
