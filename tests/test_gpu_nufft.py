@@ -64,3 +64,25 @@ def test_million_points_on_a_1024_grid_sampled_modes():
     assert relerr(got, ref) < 1e-8, relerr(got, ref)
     f2 = nufft2d1(x, y, (1 - 2j) * c, 1024, 1024, eps=1e-9)[0]
     assert float(torch.linalg.norm(f2 - (1 - 2j) * f) / torch.linalg.norm(f)) < 1e-12
+
+
+def test_clustered_points_many_binning_blocks():
+    """300,000 points, 90% of them in a disc of radius 0.05 around the origin
+    (a few tiles hold most points: many subproblems per tile, and every
+    binning block of the sort contributes to the same tiles), checked on 200
+    sampled modes against the direct sum."""
+    print("case: nufft clustered 3e5 points, 256^2", flush=True)
+    from yellowthinfilms_amd import nufft2d1
+    rng = np.random.default_rng(6)
+    M, Mc = 300_000, 270_000
+    r, th = 0.05 * np.sqrt(rng.uniform(0, 1, Mc)), rng.uniform(0, 2 * np.pi, Mc)
+    x = np.concatenate([r * np.cos(th), rng.uniform(-np.pi, np.pi, M - Mc)])
+    y = np.concatenate([r * np.sin(th), rng.uniform(-np.pi, np.pi, M - Mc)])
+    perm = rng.permutation(M)
+    x, y = x[perm], y[perm]
+    c = rng.normal(size=M) + 1j * rng.normal(size=M)
+    f = nufft2d1(x, y, c, 256, 256, eps=1e-9)[0]
+    modes = np.stack([rng.integers(-128, 128, 200), rng.integers(-128, 128, 200)], axis=1)
+    ref = po.nudft2d1(x, y, c, modes=modes)
+    got = f.cpu().numpy()[modes[:, 1] + 128, modes[:, 0] + 128]
+    assert relerr(got, ref) < 1e-8, relerr(got, ref)

@@ -1251,16 +1251,18 @@ int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)ntrans, (int64_t)((512u << 20) / per_t), 65535}));
   const size_t uv_b = align256(sizeof(double) * 2 * Mc), i_b = align256(sizeof(int) * Mc),
                t_b = align256(sizeof(int) * (ntiles + 1)), s_b = align256(sizeof(int) * nsub),
+               h_b = align256(sizeof(int) * (size_t)std::max(ytf::nu_bin_blocks(Mc), 1) * ntiles),
                g_b = align256(sizeof(cd) * (size_t)nf * nf * nb), T_b = align256(sizeof(cd) * (size_t)N1 * nf * nb);
-  int rc = ensure_buf(c, &c->work, &c->work_bytes, uv_b + 2 * i_b + 4 * t_b + 2 * s_b + g_b + T_b);
+  int rc = ensure_buf(c, &c->work, &c->work_bytes, uv_b + 3 * i_b + h_b + 3 * t_b + 2 * s_b + g_b + T_b);
   if (rc) return rc;
   char *p = (char *)c->work;
   double *uv = (double *)p; p += uv_b;
   int *tile = (int *)p; p += i_b;
+  int *rank = (int *)p; p += i_b;
   int *order = (int *)p; p += i_b;
+  int *H = (int *)p; p += h_b;
   int *count = (int *)p; p += t_b;
   int *start = (int *)p; p += t_b;
-  int *cursor = (int *)p; p += t_b;
   int *substart = (int *)p; p += t_b;
   int *sub_tile = (int *)p; p += s_b;
   int *sub_p0 = (int *)p; p += s_b;
@@ -1284,7 +1286,7 @@ int nufft2d1_locked(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_
   rc = twiddles(c, nf, &tw);
   if (rc) return rc;
   hipError_t e = timed(c, "nufft_sort", [&] {
-    return ytf::launch_nufft_sort(M, x_dev, y_dev, nf, uv, tile, count, start, cursor, substart, sub_tile, sub_p0,
+    return ytf::launch_nufft_sort(M, x_dev, y_dev, nf, uv, tile, rank, H, count, start, substart, sub_tile, sub_p0,
                                   order, c->stream);
   });
   for (int t0 = 0; t0 < ntrans && e == hipSuccess; t0 += nb) {

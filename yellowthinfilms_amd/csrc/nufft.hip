@@ -46,23 +46,81 @@ __device__ __forceinline__ double es_kernel(double z, double beta) {
   return s > 0.0 ? exp(beta * (sqrt(s) - 1.0)) : 0.0;
 }
 
-// fine-grid coordinates (u, v) in [0, nf) and the tile of each point
-__global__ void k_nu_bin(int64_t M, const double *__restrict__ x, const double *__restrict__ y, int nf, int nt,
-                         double *__restrict__ uv, int *__restrict__ tile, int *__restrict__ count) {
-  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j >= M) return;
+// The sort is a counting sort by tile without global atomics.  Binning block
+// b takes points [b P, (b + 1) P): it computes their fine-grid coordinates
+// (u, v) in [0, nf) and tiles, ranks each point among the block's points of
+// its tile (LDS histogram), and writes the histogram as row b of H[nbb][ntiles].
+// k_nu_colscan sums H's columns into the tile counts and turns each column
+// into the offset of block b's run inside its tile, k_nu_scan turns the
+// counts into tile starts, and k_nu_scatter places point j at
+// start[tile] + H[b][tile] + rank.
+constexpr int NU_BIN_T = 1024;      // threads per binning block
+constexpr int NU_BIN_BLOCKS = 256;  // at most this many binning blocks
+
+static int64_t nu_bin_chunk(int64_t M) {  // P: a multiple of NU_BIN_T, >= 8192
+  int64_t P = (M + NU_BIN_BLOCKS - 1) / NU_BIN_BLOCKS;
+  P = (P + NU_BIN_T - 1) / NU_BIN_T * NU_BIN_T;
+  return P < 8192 ? 8192 : P;
+}
+int nu_bin_blocks(int64_t M) {
+  const int64_t P = nu_bin_chunk(M);
+  return (int)((M + P - 1) / P);
+}
+
+__global__ __launch_bounds__(NU_BIN_T) void k_nu_bin(int64_t M, int64_t P, const double *__restrict__ x,
+                                                     const double *__restrict__ y, int nf, int nt,
+                                                     double *__restrict__ uv, int *__restrict__ tile,
+                                                     int *__restrict__ rank, int *__restrict__ H) {
+  extern __shared__ int hist[];
+  const int ntiles = nt * nt;
+  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
   const double sc = nf / 6.283185307179586476925287;
-  double u = fold2pi(x[j]) * sc, v = fold2pi(y[j]) * sc;
-  if (u >= nf) u -= nf;
-  if (v >= nf) v -= nf;
-  int tx = (int)(u / NU_TS), ty = (int)(v / NU_TS);
-  tx = tx < nt ? tx : nt - 1;
-  ty = ty < nt ? ty : nt - 1;
-  uv[2 * j] = u;
-  uv[2 * j + 1] = v;
-  const int t = tx + nt * ty;
-  tile[j] = t;
-  atomicAdd(count + t, 1);
+  const int64_t j1 = min((int64_t)(blockIdx.x + 1) * P, M);
+  for (int64_t j = blockIdx.x * P + threadIdx.x; j < j1; j += blockDim.x) {
+    double u = fold2pi(x[j]) * sc, v = fold2pi(y[j]) * sc;
+    if (u >= nf) u -= nf;
+    if (v >= nf) v -= nf;
+    int tx = (int)(u / NU_TS), ty = (int)(v / NU_TS);
+    tx = tx < nt ? tx : nt - 1;
+    ty = ty < nt ? ty : nt - 1;
+    uv[2 * j] = u;
+    uv[2 * j + 1] = v;
+    const int t = tx + nt * ty;
+    tile[j] = t;
+    rank[j] = atomicAdd(hist + t, 1);
+  }
+  __syncthreads();
+  int *h = H + (int64_t)blockIdx.x * ntiles;
+  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) h[t] = hist[t];
+}
+
+// Per tile t (a column of H): count[t] = sum over the nbb binning blocks of
+// H[b][t], and H[b][t] <- sum over b' < b of H[b'][t] (where block b's run
+// starts inside the tile).  A block takes 64 tiles; its 16 waves split the
+// binning blocks into 16 contiguous ranges.
+__global__ __launch_bounds__(1024) void k_nu_colscan(int ntiles, int nbb, int *__restrict__ H, int *__restrict__ count) {
+  __shared__ int part[16][64];
+  const int tl = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int t = blockIdx.x * 64 + tl;
+  const int per = (nbb + 15) / 16;
+  const int b0 = g * per, b1 = min(b0 + per, nbb);
+  int s = 0;
+  if (t < ntiles)
+    for (int b = b0; b < b1; ++b) s += H[(int64_t)b * ntiles + t];
+  part[g][tl] = s;
+  __syncthreads();
+  int off = 0;
+  for (int q = 0; q < g; ++q) off += part[q][tl];
+  if (t < ntiles) {
+    for (int b = b0; b < b1; ++b) {
+      int *h = H + (int64_t)b * ntiles + t;
+      const int o = *h;
+      *h = off;
+      off += o;
+    }
+    if (g == 15) count[t] = off;
+  }
 }
 
 // Exclusive scan over ntiles values of one block of 1024 threads: out[t] =
@@ -96,13 +154,12 @@ __device__ void block_scan(int ntiles, F val, int *__restrict__ out, int *part) 
   __syncthreads();
 }
 
-// Tile starts (start, and the scatter cursor) and subproblem starts
-// (substart: tile t owns subproblems substart[t] .. substart[t + 1] - 1).
-__global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__restrict__ start, int *__restrict__ cursor,
+// Tile starts and subproblem starts (substart: tile t owns subproblems
+// substart[t] .. substart[t + 1] - 1).
+__global__ void k_nu_scan(int ntiles, const int *__restrict__ count, int *__restrict__ start,
                           int *__restrict__ substart) {
   __shared__ int part[1024];
   block_scan(ntiles, [&](int t) { return count[t]; }, start, part);
-  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) cursor[t] = start[t];
   block_scan(ntiles, [&](int t) { return (count[t] + NU_SUB - 1) / NU_SUB; }, substart, part);
 }
 
@@ -117,16 +174,13 @@ __global__ void k_nu_subs(int ntiles, const int *__restrict__ start, const int *
   }
 }
 
-__global__ void k_nu_scatter(int64_t M, const int *__restrict__ tile, int *__restrict__ cursor, int *__restrict__ order) {
+__global__ void k_nu_scatter(int64_t M, int64_t P, int ntiles, const int *__restrict__ tile,
+                             const int *__restrict__ rank, const int *__restrict__ H, const int *__restrict__ start,
+                             int *__restrict__ order) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (j >= M) return;
-  const int pos = atomicAdd(cursor + tile[j], 1);
-  order[pos] = (int)j;
-}
-
-__global__ void k_nu_zero_int(int n, int *__restrict__ a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0;
+  const int t = tile[j];
+  order[start[t] + H[(j / P) * ntiles + t] + rank[j]] = (int)j;
 }
 
 __global__ void k_nu_zero(int64_t n, cd *__restrict__ g) {
@@ -275,26 +329,29 @@ int nu_max_subproblems(int64_t M, int nf) {
   return nt * nt + (int)((M + NU_SUB - 1) / NU_SUB);
 }
 
-hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
-                             int *start, int *cursor, int *substart, int *sub_tile, int *sub_p0, int *order,
+hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *rank,
+                             int *H, int *count, int *start, int *substart, int *sub_tile, int *sub_p0, int *order,
                              hipStream_t s) {
   const int nt = nf / NU_TS;
   const int ntiles = nt * nt;
-  hipLaunchKernelGGL(k_nu_zero_int, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, count);
+  const int nbb = M > 0 ? nu_bin_blocks(M) : 0;
+  const int64_t P = nu_bin_chunk(M);
+  if (nbb > 0) {
+    hipLaunchKernelGGL(k_nu_bin, dim3(nbb), dim3(NU_BIN_T), sizeof(int) * ntiles, s, M, P, x, y, nf, nt, uv, tile,
+                       rank, H);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_nu_colscan, dim3((ntiles + 63) / 64), dim3(1024), 0, s, ntiles, nbb, H, count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (M > 0) {
-    hipLaunchKernelGGL(k_nu_bin, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, x, y, nf, nt, uv, tile,
-                       count);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, cursor, substart);
+  hipLaunchKernelGGL(k_nu_scan, dim3(1), dim3(1024), 0, s, ntiles, (const int *)count, start, substart);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_nu_subs, dim3((ntiles + 255) / 256), dim3(256), 0, s, ntiles, (const int *)start,
                      (const int *)substart, sub_tile, sub_p0);
   if ((e = hipGetLastError()) != hipSuccess || M == 0) return e;
-  hipLaunchKernelGGL(k_nu_scatter, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, (const int *)tile, cursor,
-                     order);
+  hipLaunchKernelGGL(k_nu_scatter, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, s, M, P, ntiles,
+                     (const int *)tile, (const int *)rank, (const int *)H, (const int *)start, order);
   return hipGetLastError();
 }
 

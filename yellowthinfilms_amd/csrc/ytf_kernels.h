@@ -178,11 +178,14 @@ hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, dou
 int nu_width(double eps);
 // Upper bound on the spreading subproblems of M points on an nf x nf fine grid.
 int nu_max_subproblems(int64_t M, int nf);
+// Binning blocks of the sort for M points (rows of its histogram matrix).
+int nu_bin_blocks(int64_t M);
 // Sort (once per point set): fine-grid coordinates, tile bins, the sorted
-// order and the subproblem table (substart: ntiles + 1, sub_tile / sub_p0:
-// nu_max_subproblems entries).
-hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *count,
-                             int *start, int *cursor, int *substart, int *sub_tile, int *sub_p0, int *order,
+// order and the subproblem table (count, start: ntiles; substart: ntiles + 1;
+// sub_tile / sub_p0: nu_max_subproblems entries).  Scratch: rank (M) and H
+// (nu_bin_blocks(M) x ntiles).
+hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *rank,
+                             int *H, int *count, int *start, int *substart, int *sub_tile, int *sub_p0, int *order,
                              hipStream_t s);
 // nb transforms at once: strengths c + t * c_stride, fine grids grid + t * nf^2.
 hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, const int *start, const int *substart,
