@@ -220,7 +220,7 @@ def bench_point(a):
     out.update({
         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
         "rel_err_vs_fp64_oracle": rel_err,
-        "roofline": {"kernel": "synth_fp64 (radial Bessel contraction, FP64)", "bound": "mfma",
+        "roofline": {"kernel": "synth_fp64 (radial Bessel contraction, FP64: Bessel table, node-chunk contraction, chunk sum)", "bound": "mfma",
                      "achieved": flops / (syn_ms * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": flops / (syn_ms * 1e-3) / PEAK_FP64, "traffic": None,
                      "flops_per_launch": flops, "bessel_evals_per_launch": 2.0 * n * nr},

@@ -1336,18 +1336,25 @@ static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const
       return fail(c, YTF_EHIP, "%s: matrix-core launch failed: %s", who, hipGetErrorString(e));
     return YTF_OK;
   }
-  const int nks = n > 0 ? ytf::synth_fp64_splits(n, np, nr) : 1;
-  const size_t part_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)nr * (size_t)nks);
-  int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + part_b);
+  const ytf::SynthPlan SP = ytf::synth_fp64_plan(n, np, nr);
+  const size_t jtab_b = align256(sizeof(double) * 3 * (size_t)std::max<int64_t>(SP.batch_nodes, 1) * (size_t)nr),
+               part_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)nr * (size_t)std::max(SP.nchunks, 1)),
+               sum_b = align256(sizeof(cd) * 9 * (size_t)np * (size_t)nr);
+  int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + jtab_b + part_b + sum_b);
   if (rc) return rc;
-  cd *coef = (cd *)c->work;
-  cd *part = (cd *)((char *)c->work + coef_b);
+  char *w = (char *)c->work;
+  cd *coef = (cd *)w;
+  double *jtab = (double *)(w + coef_b);
+  cd *part = (cd *)(w + coef_b + jtab_b);
+  cd *S = (cd *)(w + coef_b + jtab_b + part_b);
   if (n > 0 &&
       (e = timed(c, "synth_coef", [&] { return ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream); })))
     return fail(c, YTF_EHIP, "%s: coefficient launch failed: %s", who, hipGetErrorString(e));
-  // with no nodes the contraction kernel writes zeros
-  if ((e = timed(c, "synth_fp64", [&] { return ytf::launch_synth_fp64(n, np, nr, nks, kx, rho, coef, part, c->stream); })) ||
-      (e = timed(c, "synth_reduce", [&] { return ytf::launch_synth_reduce(np, nr, nks, part, R, c->stream); })))
+  // with no nodes the chunk sum writes zeros
+  if ((e = timed(c, "synth_fp64", [&] {
+         return ytf::launch_synth_fp64(n, np, nr, SP, kx, rho, coef, jtab, part, S, c->stream);
+       })) ||
+      (e = timed(c, "synth_reduce", [&] { return ytf::launch_synth_reduce(np, nr, 1, S, R, c->stream); })))
     return fail(c, YTF_EHIP, "%s: synthesis launch failed: %s", who, hipGetErrorString(e));
   return YTF_OK;
 }

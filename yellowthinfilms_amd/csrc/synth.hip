@@ -23,6 +23,8 @@
 // R_xz = gxz-column, R_yz, R_zx, R_zy likewise (g entry c + 3q).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ytf_kernels.h"
 
 namespace ytf {
@@ -78,48 +80,132 @@ __global__ void k_synth_coef(int64_t n, int np, const double *__restrict__ kx, c
   o[SC_D * n + k] = rmul(-hw, gxy + gyx);
 }
 
-// Partial contraction over one node chunk.  Thread = one radius; the block's
-// PC depth pairs share the Bessel values.  part[ks][p][SC_N][nr].
-template <int PC>
-__global__ __launch_bounds__(256) void k_synth_fp64(int64_t n, int np, int nr, int64_t chunk,
-                                                    const double *__restrict__ kx, const double *__restrict__ rho,
-                                                    const cd *__restrict__ coef, cd *__restrict__ part) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p0 = blockIdx.y * PC;
-  const int ks = blockIdx.z;
-  const int64_t k0 = ks * chunk;
-  int64_t k1 = k0 + chunk;
-  if (k1 > n) k1 = n;
-  const double rh = r < nr ? rho[r] : 0.0;
-  cd acc[PC][SC_N];
+// FP64 path: a Bessel table of one batch of nodes, then a contraction per
+// node chunk into partials, a fixed-order sum over the chunks, and the nine
+// entries.
+//
+// J[ord][kk][r] = J_ord(kx[k0 + kk] rho[r]), ord = 0, 1, 2 (r fastest).
+__global__ void k_synth_bessel(int64_t k0, int64_t nk, int nr, const double *__restrict__ kx,
+                               const double *__restrict__ rho, double *__restrict__ J) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= nk * nr) return;
+  const int64_t kk = e / nr;
+  const int r = (int)(e - kk * nr);
+  double b0, b1, b2;
+  bessel012(kx[k0 + kk] * rho[r], b0, b1, b2);
+  const int64_t ne = nk * nr;
+  J[e] = b0;
+  J[ne + e] = b1;
+  J[2 * ne + e] = b2;
+}
+
+// Contraction of node chunk c (nodes [c chunk, min((c + 1) chunk, n)), inside
+// the table's batch starting at node k0): part[c][p][SC_N][nr].  Lane = radius
+// (64 per block); wave w of the block takes the depth pairs 8 blockIdx.y + 2w
+// and + 1.
+__global__ __launch_bounds__(256) void k_synth_contract(int64_t n, int np, int nr, int64_t chunk, int c0, int64_t k0,
+                                                        int64_t nk, const double *__restrict__ J,
+                                                        const cd *__restrict__ coef, cd *__restrict__ part) {
+  const int r = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int p0 = blockIdx.y * 8 + 2 * wv;
+  const int c = c0 + blockIdx.z;
+  const int64_t ka = c * chunk, kb = min(ka + chunk, n);
+  const bool live = r < nr;
+  const int64_t ne = nk * nr;
+  cd acc[2][SC_N];
 #pragma unroll
-  for (int q = 0; q < PC; ++q)
+  for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int i = 0; i < SC_N; ++i) acc[q][i] = cd{0.0, 0.0};
-  for (int64_t k = k0; k < k1; ++k) {
-    double b0, b1, b2;
-    bessel012(kx[k] * rh, b0, b1, b2);
+  const int nq = min(2, np - p0);
+  // Per step of CP nodes the block stages in LDS the table values (wave u
+  // loads node kg + u for the block's 64 radii) and the coefficients of its
+  // 8 depth pairs; the next step's values are in flight while the current CP
+  // nodes are summed.  Coefficient reads are wave-uniform (LDS broadcast).
+  constexpr int CP = 4, NROW = 8 * SC_N;  // coefficient rows (pair, column) per block
+  __shared__ double jl[2][CP][3][64];
+  __shared__ cd cl[2][NROW][CP];
+  const int lane = threadIdx.x & 63, pb = blockIdx.y * 8;
+  auto gload = [&](int64_t kg, double *b, cd *cv) {
+    const int64_t k = kg + wv;
+    const bool ok = live && k < kb;
+    const int64_t e = (k - k0) * nr + r;
 #pragma unroll
-    for (int q = 0; q < PC; ++q) {
-      if (p0 + q >= np) break;
-      const cd *c = coef + (size_t)(p0 + q) * SC_N * n + k;
+    for (int o = 0; o < 3; ++o) b[o] = ok ? J[o * ne + e] : 0.0;
 #pragma unroll
-      for (int i = 0; i < SC_N; ++i) {
-        const double b = i < SC_XZ ? b0 : (i < SC_C ? b1 : b2);
-        const cd v = c[(size_t)i * n];
-        acc[q][i].r = fma(b, v.r, acc[q][i].r);
-        acc[q][i].i = fma(b, v.i, acc[q][i].i);
+    for (int h = 0; h < 2; ++h) {
+      const int x = threadIdx.x + 256 * h;  // coefficient element (row, node)
+      const int row = x / CP, u = x % CP;
+      const int p = pb + row / SC_N;
+      const bool okc = x < NROW * CP && p < np && kg + u < kb;
+      cv[h] = okc ? coef[((size_t)p * SC_N + row % SC_N) * n + kg + u] : cd{0.0, 0.0};
+    }
+  };
+  auto stash = [&](int b2, const double *b, const cd *cv) {
+#pragma unroll
+    for (int o = 0; o < 3; ++o) jl[b2][wv][o][lane] = b[o];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int x = threadIdx.x + 256 * h;
+      if (x < NROW * CP) cl[b2][x / CP][x % CP] = cv[h];
+    }
+  };
+  double g[3];
+  cd gc[2];
+  gload(ka, g, gc);
+  stash(0, g, gc);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t kg = ka; kg < kb; kg += CP, buf ^= 1) {
+    gload(kg + CP, g, gc);
+#pragma unroll
+    for (int u = 0; u < CP; ++u) {
+      if (kg + u < kb) {
+        const double b0 = jl[buf][u][0][lane], b1 = jl[buf][u][1][lane], b2 = jl[buf][u][2][lane];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          if (q < nq) {
+#pragma unroll
+            for (int i = 0; i < SC_N; ++i) {
+              const double b = i < SC_XZ ? b0 : (i < SC_C ? b1 : b2);
+              const cd v = cl[buf][(2 * wv + q) * SC_N + i][u];
+              acc[q][i].r = fma(b, v.r, acc[q][i].r);
+              acc[q][i].i = fma(b, v.i, acc[q][i].i);
+            }
+          }
+        }
       }
     }
+    stash(buf ^ 1, g, gc);
+    __syncthreads();
   }
-  if (r >= nr) return;
+  if (!live) return;
 #pragma unroll
-  for (int q = 0; q < PC; ++q) {
-    if (p0 + q >= np) break;
-    cd *o = part + ((size_t)ks * np + p0 + q) * SC_N * nr + r;
+  for (int q = 0; q < 2; ++q) {
+    if (q < nq) {
+      cd *o = part + ((size_t)c * np + p0 + q) * SC_N * nr + r;
 #pragma unroll
-    for (int i = 0; i < SC_N; ++i) o[(size_t)i * nr] = acc[q][i];
+      for (int i = 0; i < SC_N; ++i) o[(size_t)i * nr] = acc[q][i];
+    }
   }
+}
+
+// S[p][i][r] = sum over the nc chunks, in chunk order, of part[c][p][i][r].
+__global__ void k_synth_chunksum(int64_t m, int nc, const cd *__restrict__ part, cd *__restrict__ S) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  cd s{0.0, 0.0};
+  int c = 0;
+  for (; c + 8 <= nc; c += 8) {
+    cd v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = part[(size_t)(c + u) * m + e];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = s + v[u];
+  }
+  for (; c < nc; ++c) s = s + part[(size_t)c * m + e];
+  S[e] = s;
 }
 
 // Sum the node-chunk partials in a fixed order and form the nine entries:
@@ -188,22 +274,40 @@ hipError_t launch_synth_coef(int64_t n, int np, const double *kx, const double *
   return hipGetLastError();
 }
 
-int synth_fp64_splits(int64_t n, int np, int nr) {
-  const int PC = 4;
-  const int64_t waves = (int64_t)((nr + 63) / 64) * ((np + PC - 1) / PC);
-  int64_t ks = (4096 + waves - 1) / waves;
-  int64_t maxks = (n + 31) / 32;
-  if (ks > maxks) ks = maxks;
-  if (ks < 1) ks = 1;
-  return (int)ks;
+SynthPlan synth_fp64_plan(int64_t n, int np, int nr) {
+  SynthPlan P;
+  const int64_t groups = (int64_t)((nr + 63) / 64) * ((np + 7) / 8) * 4;  // waves per chunk
+  int64_t C = (2048 + groups - 1) / groups;
+  const int64_t cmax = (n + 15) / 16;  // at least 16 nodes per chunk
+  if (C > cmax) C = cmax;
+  if (C < 1) C = 1;
+  int64_t chunk = (n + C - 1) / C;
+  const int64_t tab = (int64_t)1 << 23;  // table entries per order and batch
+  if (chunk * nr > tab) chunk = std::max<int64_t>(1, tab / std::max(nr, 1));
+  if (chunk < 1) chunk = 1;
+  P.chunk = chunk;
+  P.nchunks = (int)((n + chunk - 1) / chunk);
+  P.chunks_per_batch = (int)std::max<int64_t>(1, std::min<int64_t>(P.nchunks, tab / std::max<int64_t>(chunk * nr, 1)));
+  P.batch_nodes = std::min<int64_t>(n, chunk * P.chunks_per_batch);
+  return P;
 }
 
-hipError_t launch_synth_fp64(int64_t n, int np, int nr, int nks, const double *kx, const double *rho,
-                             const cd *coef, cd *part, hipStream_t s) {
-  const int PC = 4;
-  const int64_t chunk = (n + nks - 1) / nks;
-  dim3 grid((unsigned)((nr + 255) / 256), (unsigned)((np + PC - 1) / PC), (unsigned)nks);
-  hipLaunchKernelGGL(k_synth_fp64<PC>, grid, dim3(256), 0, s, n, np, nr, chunk, kx, rho, coef, part);
+hipError_t launch_synth_fp64(int64_t n, int np, int nr, const SynthPlan &P, const double *kx, const double *rho,
+                             const cd *coef, double *J, cd *part, cd *S, hipStream_t s) {
+  for (int c0 = 0; c0 < P.nchunks; c0 += P.chunks_per_batch) {
+    const int nc = std::min(P.chunks_per_batch, P.nchunks - c0);
+    const int64_t k0 = c0 * P.chunk, nk = std::min<int64_t>((int64_t)nc * P.chunk, n - k0);
+    const int64_t ne = nk * nr;
+    hipLaunchKernelGGL(k_synth_bessel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, k0, nk, nr, kx, rho, J);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 grid((unsigned)((nr + 63) / 64), (unsigned)((np + 7) / 8), (unsigned)nc);
+    hipLaunchKernelGGL(k_synth_contract, grid, dim3(256), 0, s, n, np, nr, P.chunk, c0, k0, nk, J, coef, part);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  const int64_t m = (int64_t)np * SC_N * nr;
+  hipLaunchKernelGGL(k_synth_chunksum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, P.nchunks,
+                     (const cd *)part, S);
   return hipGetLastError();
 }
 

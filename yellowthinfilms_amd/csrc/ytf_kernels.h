@@ -56,12 +56,22 @@ hipError_t launch_spectral_apply(int nx, int ny, const cd *ghat_xy, const cd *J,
                                  const cd *tw_y, double scale, hipStream_t stream);
 
 // Radial Bessel synthesis (synth.hip): node coefficients from g~, the FP64
-// contraction over node chunks + fixed-order reduction, the pixel rotation.
+// contraction (Bessel table, node chunks, fixed-order sum), the nine entries,
+// the pixel rotation.
 hipError_t launch_synth_coef(int64_t n, int np, const double *kx, const double *wt, const cd *g, cd *coef,
                              hipStream_t s);
-int synth_fp64_splits(int64_t n, int np, int nr);
-hipError_t launch_synth_fp64(int64_t n, int np, int nr, int nks, const double *kx, const double *rho,
-                             const cd *coef, cd *part, hipStream_t s);
+// FP64 contraction plan: nchunks node chunks of `chunk` nodes (about 2048
+// waves in all), Bessel tables of chunks_per_batch chunks (batch_nodes nodes,
+// at most 2^23 entries per order).
+struct SynthPlan {
+  int64_t chunk, batch_nodes;
+  int nchunks, chunks_per_batch;
+};
+SynthPlan synth_fp64_plan(int64_t n, int np, int nr);
+// Bessel table J (3 x batch_nodes x nr doubles), chunk partials part
+// (nchunks x np x 9 x nr complex), their fixed-order sum S (np x 9 x nr).
+hipError_t launch_synth_fp64(int64_t n, int np, int nr, const SynthPlan &P, const double *kx, const double *rho,
+                             const cd *coef, double *J, cd *part, cd *S, hipStream_t s);
 hipError_t launch_synth_reduce(int np, int nr, int nks, const cd *part, cd *R, hipStream_t s);
 hipError_t launch_synth_rotate(int nx, int ny, int np, int nr, const double *xs, const double *ys, const int *rmap,
                                const cd *R, cd *G, hipStream_t s);
