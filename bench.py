@@ -850,7 +850,8 @@ def cpu_operator_volume(stack, cache, op, P, grid, wl, zs, zobs, kw):
 def bench_pole_scan(a):
     import torch
     from yellowthinfilms_amd import workloads as W
-    from yellowthinfilms_amd.kspace import real_axis_poles, _Evaluator
+    from yellowthinfilms_amd import kspace
+    from yellowthinfilms_amd.kspace import real_axis_poles
     from yellowthinfilms_amd._lib import Context
 
     wl = W.POLE_SCAN_HYPERBOLIC
@@ -858,11 +859,11 @@ def bench_pole_scan(a):
     phis, k_lo, k_hi, step, exclude = W.pole_scan_args(wl)
     res = {}
     calls = {"n": 0}
-    orig = _Evaluator.__call__
+    orig = kspace.cavity_response
 
-    def counting(self, k, phi):  # nodes evaluated per scan (scan + refinement)
-        calls["n"] += int(np.asarray(k).size)
-        return orig(self, k, phi)
+    def counting(stack_, omega_, s_, kx, ky, device=0):  # nodes evaluated per scan (scan + refinement)
+        calls["n"] += int(np.asarray(kx).size)
+        return orig(stack_, omega_, s_, kx, ky, device=device)
 
     def step_fn():
         res["poles"] = real_axis_poles(stack, W.OMEGA, wl["source_layer"], phis=phis, k_lo=k_lo, k_hi=k_hi,
@@ -871,15 +872,18 @@ def bench_pole_scan(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step_fn()
-    _Evaluator.__call__ = counting
+    kspace.cavity_response = counting
     ctx.set_timing(True)
     try:
         wall = _timed_steps(dict_ns(a, warmup=0), step_fn, torch)
     finally:
-        _Evaluator.__call__ = orig
-    kt, kn = ctx.kernel_time("cavity_response")
+        kspace.cavity_response = orig
+    kms = {k: ctx.kernel_time(k) for k in ("cavity_response", "pole_candidates", "pole_refine")}
+    kt = kms["cavity_response"][0] + kms["pole_refine"][0]  # every response evaluation
+    kn = kms["cavity_response"][1] + kms["pole_refine"][1]
     ctx.set_timing(False)
-    nodes_per_scan = calls["n"] / a.steps
+    # scan nodes counted at the launch, refinement evaluations counted in the kernel
+    nodes_per_scan = calls["n"] / a.steps + kspace.search_stats["refine_evaluations"]
     n_scan = len(phis) * (int(math.floor(math.log(k_hi / k_lo) / step + 1e-10)) + 1)
     # the scan launch alone (every direction's log grid in one launch), timed on its own
     from yellowthinfilms_amd.kspace import cavity_response
@@ -908,6 +912,7 @@ def bench_pole_scan(a):
     out.update({
         "scans_per_s": a.steps / wall, "nodes_per_scan": nodes_per_scan, "poles_found": len(res["poles"]),
         "response_launches_per_scan": kn / a.steps, "response_kernel_ms_per_scan": kt / a.steps,
+        "kernel_ms_per_scan": {k: v[0] / a.steps for k, v in kms.items()},
         "poles_vs_oracle": cmp,
         "all_launches_tflops": flops * nodes_per_scan / (kt / a.steps * 1e-3) / 1e12,
         "roofline": {"kernel": "cavity_response scan launch (every direction's log grid: Berreman modes + S cascade "

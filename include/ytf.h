@@ -161,6 +161,38 @@ int ytf_green_nodes_box(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, dou
 int ytf_cavity_response(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
                         const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev);
 
+/* Prominent maxima of a real-axis scan (real_axis_poles, mod_KSpaceFeatures.jl:
+ * 151-214, the candidate step).  R_dev[m][i] (n_dirs x n doubles) is the
+ * response along direction m at k = ks_dev[i]; y = log(R + 1e-300).  A
+ * candidate is a local maximum 1 <= i <= n-2 (y[i] > y[i-1], y[i] >= y[i+1],
+ * y[i] >= max(max_i y[m][i] + log_floor_rel, log_floor_abs)) whose prominence
+ * y[i] - base is >= log_prom, base the higher of its two valleys: left, the
+ * lowest sample after the last one above y[i] within W steps (the last lowest),
+ * right, the lowest before the first one above y[i] within W steps (the first
+ * lowest).  Writes, in no particular order, key_dev[c] = m n + i and
+ * val_dev[5c .. 5c+4] = (base, prominence, k of the left valley, k of the right
+ * valley, y[i]) for c < min(*n_out, max_out); *n_out = number of candidates
+ * (may exceed max_out: call again with room for them).  Synchronizes. */
+int ytf_pole_candidates(ytf_ctx *ctx, int n_dirs, int64_t n, const double *R_dev, const double *ks_dev, int W,
+                        double log_prom, double log_floor_rel, double log_floor_abs, int max_out, int64_t *key_dev,
+                        double *val_dev, int64_t *n_out);
+
+/* Location and half-width of pole candidates (real_axis_poles, mod_KSpaceFeatures.jl:
+ * 151-214, step 3), every response evaluated on the device.  Candidate c is
+ * cand_dev[7c .. 7c+6] = (phi, a, b, k_left_valley, k_right_valley, base, y):
+ *   q: golden-section search for the maximum of f(k) = cavity response at
+ *      (k cos phi, k sin phi) on [a, b] (at most 60 steps, until b - a <= 1e-12 b),
+ *      q = (a + b)/2;
+ *   level = e^base + (max(f(q), e^y) - e^base)/sqrt(2);
+ *   the crossings of the level by bisection between q and each valley (at most
+ *      50 steps, until |hi - lo| <= 1e-12 q).
+ * out_dev[3c .. 3c+2] = (q, left crossing, right crossing).  The stack arguments
+ * are those of ytf_cavity_response.  n_evals may be NULL; otherwise the call
+ * waits for the stream and sets *n_evals = the number of responses the
+ * searches took. */
+int ytf_pole_refine(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                    int64_t n_cand, const double *cand_dev, double *out_dev, int64_t *n_evals);
+
 /* Spectral convolution on the uniform DFT grid (the "G(kx,ky) F{J}" route of
  * mod_UniformSource.jl:229-248):
  *     E = scale * bfft2( g~ .* fft2(J) )        per field component c:

@@ -98,6 +98,19 @@ def test_real_axis_poles_against_oracle(label, stack, s, phis, k_lo, k_hi, exclu
         assert round(strongest[0]["q"] / W0, 4) == 1.0624 and round(strongest[1]["q"] / W0, 2) == 1.66
 
 
+def test_real_axis_poles_without_candidates_and_with_a_flat_scan():
+    """A homogeneous stack has a constant response: no maxima, an empty list
+    (the oracle agrees); a scan too short for a maximum also gives none."""
+    from yellowthinfilms_amd.kspace import real_axis_poles
+    print("case: real_axis_poles without candidates", flush=True)
+    st = [lay(IsotropicMaterial(2.0 + 0.01j), 0.5), lay(IsotropicMaterial(2.0 + 0.01j), 0.5),
+          lay(IsotropicMaterial(2.0 + 0.01j), 0.5)]
+    g = real_axis_poles(st, W0, 2, phis=[0.0, 1.0], k_lo=0.5 * W0, k_hi=4.0 * W0)
+    o = po.real_axis_poles(ostack(st), W0, 2, phis=[0.0, 1.0], k_lo=0.5 * W0, k_hi=4.0 * W0)
+    assert g == [] and len(o) == 0
+    assert real_axis_poles(base_stack(), W0, 1, k_lo=1.0 * W0, k_hi=1.0015 * W0) == []
+
+
 def test_fixed_rule_resolves_the_plasmon_pole_end_to_end():
     """kspace_features (GPU pole scan) + fixed_krho_rule on a dipole above a
     metal: the 1D closed_form_G of the rule agrees with a brute-force rule

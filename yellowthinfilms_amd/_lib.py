@@ -73,6 +73,11 @@ _SIGS = {
                                        ct.POINTER(YtfNodeCache), ct.c_int, ct.c_double, ct.c_int]),
     "ytf_cavity_response": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int,
                                        ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p]),
+    "ytf_pole_refine": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int, ct.c_int64,
+                                   ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
+    "ytf_pole_candidates": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                       ct.c_double, ct.c_double, ct.c_double, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                       ct.POINTER(ct.c_int64)]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,

@@ -228,20 +228,51 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
 
 extern "C" {
 
-int ytf_cavity_response(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int source_layer,
-                        const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev) {
-  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_cavity_response: ctx is NULL");
+int ytf_pole_candidates(ytf_ctx *c, int n_dirs, int64_t n, const double *R_dev, const double *ks_dev, int W,
+                        double log_prom, double log_floor_rel, double log_floor_abs, int max_out, int64_t *key_dev,
+                        double *val_dev, int64_t *n_out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_pole_candidates: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
-  if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "ytf_cavity_response: need at least 2 layers");
+  if (n_dirs < 0 || n < 0 || W < 0 || max_out < 0 || !n_out || (n_dirs > 0 && n > 0 && (!R_dev || !ks_dev)) ||
+      (max_out > 0 && (!key_dev || !val_dev)) || (int64_t)n_dirs * n > INT64_MAX / 2)
+    return fail(c, YTF_EINVAL, "ytf_pole_candidates: bad arguments");
+  *n_out = 0;
+  if (n_dirs == 0 || n < 3) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_pinned(c);
+  if (rc) return rc;
+  const size_t yb = align256(sizeof(double) * (size_t)n_dirs * (size_t)n);
+  rc = ensure_buf(c, &c->aux, &c->aux_bytes, align256(sizeof(int)) + align256(sizeof(double) * (size_t)n_dirs) + yb);
+  if (rc) return rc;
+  int *count = (int *)c->aux;
+  double *ymax = (double *)((char *)c->aux + align256(sizeof(int)));
+  double *Y = (double *)((char *)ymax + align256(sizeof(double) * (size_t)n_dirs));
+  hipError_t e = ytf::launch_mlp_zero(1, (float *)count, c->stream);  // zero bits = integer 0
+  if (e == hipSuccess)
+    e = timed(c, "pole_candidates", [&] {
+      return ytf::launch_pole_candidates(n_dirs, n, W, log_prom, log_floor_rel, log_floor_abs, R_dev, ks_dev, Y, ymax,
+                                         max_out, count, key_dev, val_dev, c->stream);
+    });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_pole_candidates: launch failed: %s", hipGetErrorString(e));
+  HIPCHK(c, hipMemcpyAsync(c->pinned, count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *n_out = c->pinned[0];
+  return YTF_OK;
+}
+
+}  // extern "C"
+
+// Validate the stack of a cavity-response call and send it to the device
+// (when it differs from what the device copy holds).
+static int response_stack_locked(ytf_ctx *c, const char *who, const ytf_layer *layers, int n_layers, double omega,
+                                 int source_layer) {
+  if (!layers || n_layers < 2) return fail(c, YTF_EINVAL, "%s: need at least 2 layers", who);
   if (source_layer < 1 || source_layer > n_layers - 1)
-    return fail(c, YTF_EINVAL, "ytf_cavity_response: source_layer=%d must be in 1:%d", source_layer, n_layers - 1);
-  if (n_nodes < 0 || (n_nodes > 0 && (!kx_dev || !ky_dev || !out_dev)))
-    return fail(c, YTF_EINVAL, "ytf_cavity_response: bad nodes");
-  if (!(omega > 0) || !std::isfinite(omega)) return fail(c, YTF_EINVAL, "ytf_cavity_response: omega must be > 0");
+    return fail(c, YTF_EINVAL, "%s: source_layer=%d must be in 1:%d", who, source_layer, n_layers - 1);
+  if (!(omega > 0) || !std::isfinite(omega)) return fail(c, YTF_EINVAL, "%s: omega must be > 0", who);
   for (int l = 0; l < n_layers; ++l)
     if (layers[l].eps_kind < 0 || layers[l].eps_kind > 3 || layers[l].mu_kind < 0 || layers[l].mu_kind > 1)
-      return fail(c, YTF_EINVAL, "ytf_cavity_response: layer %d has an unknown material kind", l + 1);
-  if (n_nodes == 0) return YTF_OK;
+      return fail(c, YTF_EINVAL, "%s: layer %d has an unknown material kind", who, l + 1);
   HIPCHK(c, hipSetDevice(c->device));
   const size_t lb = sizeof(ytf_layer) * n_layers;
   std::vector<char> hb(lb);
@@ -254,12 +285,55 @@ int ytf_cavity_response(ytf_ctx *c, const ytf_layer *layers, int n_layers, doubl
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->small_host = hb;
   }
+  return YTF_OK;
+}
+
+extern "C" {
+
+int ytf_cavity_response(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                        const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_cavity_response: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_nodes < 0 || (n_nodes > 0 && (!kx_dev || !ky_dev || !out_dev)))
+    return fail(c, YTF_EINVAL, "ytf_cavity_response: bad nodes");
+  int rc = response_stack_locked(c, "ytf_cavity_response", layers, n_layers, omega, source_layer);
+  if (rc || n_nodes == 0) return rc;
   const bool general = stack_needs_general(layers, n_layers);
   hipError_t e = timed(c, "cavity_response", [&] {
     return ytf::launch_cavity_response((const LayerDev *)c->small, n_layers, source_layer, omega, kx_dev, ky_dev,
                                        n_nodes, out_dev, general, c->stream);
   });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_cavity_response: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_pole_refine(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                    int64_t n_cand, const double *cand_dev, double *out_dev, int64_t *n_evals) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_pole_refine: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_cand < 0 || n_cand > INT32_MAX || (n_cand > 0 && (!cand_dev || !out_dev)))
+    return fail(c, YTF_EINVAL, "ytf_pole_refine: bad candidates");
+  if (n_evals) *n_evals = 0;
+  int rc = response_stack_locked(c, "ytf_pole_refine", layers, n_layers, omega, source_layer);
+  if (rc || n_cand == 0) return rc;
+  const bool general = stack_needs_general(layers, n_layers);
+  unsigned long long *evals = nullptr;
+  if (n_evals) {
+    if ((rc = ensure_pinned(c)) || (rc = ensure_buf(c, &c->aux, &c->aux_bytes, 256))) return rc;
+    evals = (unsigned long long *)c->aux;
+    hipError_t z = ytf::launch_mlp_zero(2, (float *)evals, c->stream);  // zero bits = integer 0
+    if (z != hipSuccess) return fail(c, YTF_EHIP, "ytf_pole_refine: launch failed: %s", hipGetErrorString(z));
+  }
+  hipError_t e = timed(c, "pole_refine", [&] {
+    return ytf::launch_pole_refine((const LayerDev *)c->small, n_layers, source_layer, omega, n_cand, cand_dev,
+                                   out_dev, evals, general, c->stream);
+  });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_pole_refine: launch failed: %s", hipGetErrorString(e));
+  if (n_evals) {
+    HIPCHK(c, hipMemcpyAsync(c->pinned, evals, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n_evals = (int64_t) * (unsigned long long *)c->pinned;
+  }
   return YTF_OK;
 }
 

@@ -139,4 +139,208 @@ hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Prominent maxima of a real-axis scan (real_axis_poles, mod_KSpaceFeatures.jl:
+// 151-214, step 2).  R[m][i] is the response along direction m at k = ks[i];
+// Y[m][i] = y = log(R + 1e-300) (k_pole_log).
+
+__device__ __forceinline__ double pole_lg(double r) { return log(r + 1e-300); }
+
+// Y = log(R + 1e-300), elementwise.
+__global__ void k_pole_log(int64_t m, const double *__restrict__ R, double *__restrict__ Y) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < m) Y[e] = pole_lg(R[e]);
+}
+
+// ymax[m] = max_i Y[m][i]: one block of 256 threads per direction.
+__global__ void k_pole_rowmax(int64_t n, const double *__restrict__ Y, double *__restrict__ rmax) {
+  __shared__ double part[256];
+  const double *r = Y + blockIdx.x * n;
+  double v = -HUGE_VAL;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) v = fmax(v, r[i]);
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = fmax(part[threadIdx.x], part[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rmax[blockIdx.x] = part[0];
+}
+
+// One thread per (m, i), 1 <= i <= n - 2: y[i] a local maximum (> its left,
+// >= its right neighbour, >= max(log max_m R + log_floor_rel, log_floor_abs));
+// the valleys: left, the samples after the last one above y[i] within W
+// steps (from i - W if none), and its lowest (the last lowest); right, the
+// samples before the first one above y[i] within W steps, and its lowest
+// (the first lowest).  base = the higher valley, prom = y[i] - base; kept
+// when prom >= log_prom: key m n + i and (base, prom, k of the left valley,
+// k of the right valley, y[i]) are appended through *count (at most max_out).
+__global__ void k_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
+                                  double log_floor_abs, const double *__restrict__ Y, const double *__restrict__ ymax,
+                                  const double *__restrict__ ks, int max_out, int *__restrict__ count,
+                                  int64_t *__restrict__ key, double *__restrict__ val) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nphi * n) return;
+  const int m = (int)(e / n);
+  const int64_t i = e - (int64_t)m * n;
+  if (i < 1 || i > n - 2) return;
+  const double *r = Y + (int64_t)m * n;
+  const double y = r[i];
+  const double ymin = fmax(ymax[m] + log_floor_rel, log_floor_abs);
+  if (!(y > r[i - 1] && y >= r[i + 1] && y >= ymin)) return;
+  // Each side in one pass, 16 samples loaded at a time: walking out from i,
+  // the running lowest (strictly lower only: the left one ends as the last
+  // lowest, the right one as the first) until the first sample above y[i].
+  constexpr int CH = 16;
+  const int64_t lo = i - W > 0 ? i - W : 0;
+  double lmin = y;
+  int64_t lpos = i;
+  bool stop = false;
+  for (int64_t j0 = i - 1; j0 >= lo && !stop; j0 -= CH) {
+    double v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) v[u] = j0 - u >= lo ? r[j0 - u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (!stop && j0 - u >= lo) {
+        const double lv = v[u];
+        if (lv > y) stop = true;
+        else if (lv < lmin) {
+          lmin = lv;
+          lpos = j0 - u;
+        }
+      }
+    }
+  }
+  const int64_t hi = i + W < n - 1 ? i + W : n - 1;
+  double rmin = y;
+  int64_t rpos = i;
+  stop = false;
+  for (int64_t j0 = i + 1; j0 <= hi && !stop; j0 += CH) {
+    double v[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) v[u] = j0 + u <= hi ? r[j0 + u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (!stop && j0 + u <= hi) {
+        const double lv = v[u];
+        if (lv > y) stop = true;
+        else if (lv < rmin) {
+          rmin = lv;
+          rpos = j0 + u;
+        }
+      }
+    }
+  }
+  const double base = fmax(lmin, rmin), prom = y - base;
+  if (prom < log_prom) return;
+  const int o = atomicAdd(count, 1);
+  if (o >= max_out) return;
+  key[o] = e;
+  val[5 * o] = base;
+  val[5 * o + 1] = prom;
+  val[5 * o + 2] = ks[lpos];
+  val[5 * o + 3] = ks[rpos];
+  val[5 * o + 4] = y;
+}
+
+hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
+                                  double log_floor_abs, const double *R, const double *ks, double *Y, double *ymax,
+                                  int max_out, int *count, int64_t *key, double *val, hipStream_t s) {
+  if (nphi <= 0 || n <= 0) return hipSuccess;
+  const int64_t tot = (int64_t)nphi * n;
+  const unsigned blocks = (unsigned)((tot + 255) / 256);
+  hipLaunchKernelGGL(k_pole_log, dim3(blocks), dim3(256), 0, s, tot, R, Y);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pole_rowmax, dim3(nphi), dim3(256), 0, s, n, (const double *)Y, ymax);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pole_candidates, dim3(blocks), dim3(256), 0, s, nphi, n, W, log_prom, log_floor_rel,
+                     log_floor_abs, (const double *)Y, (const double *)ymax, ks, max_out, count, key, val);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Location and half-width of the candidates (real_axis_poles step 3), every
+// evaluation inside the kernel.  Thread 2c + side of candidate c = (phi, a, b,
+// kl, kr, base, y) (cand[7c ..]):
+//   golden-section search for the maximum of f(k) = response(k cos phi,
+//   k sin phi) on [a, b] (at most 60 steps, until b - a <= 1e-12 b; both
+//   threads of a candidate take the same steps), q = (a + b) / 2;
+//   level = e^base + (max(f(q), e^y) - e^base) / sqrt 2;
+//   bisection (at most 50 steps, until |hi - lo| <= 1e-12 q) between q and
+//   the valley kl (side 0) or kr (side 1) for the crossing of the level.
+// out[3c] = q, out[3c + 1 + side] = the crossing.  With `evals`, the number
+// of responses the search takes (the golden-section ones once per candidate)
+// is added to *evals.
+struct RefineArgs {
+  const LayerDev *L;
+  int nl, s;
+  double omega;
+  int64_t nc;
+  const double *cand;
+  double *out;
+  unsigned long long *evals;
+};
+
+template <bool GENERAL>
+__global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
+  __shared__ cd vprev[16 * 64];
+  const LaneMat Vp{vprev, (int)threadIdx.x};
+  const int64_t t = blockIdx.x * (int64_t)64 + threadIdx.x;
+  if (t >= 2 * A.nc) return;
+  const int64_t c = t >> 1;
+  const int side = (int)(t & 1);
+  const double *cd7 = A.cand + 7 * c;
+  const double phi = cd7[0], cphi = cos(phi), sphi = sin(phi);
+  unsigned long long ne = 0, ng = 0;
+  auto f = [&](double k) {
+    ++ne;
+    return response_node<GENERAL>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp);
+  };
+  const double g = 0.61803398874989484820;  // (sqrt 5 - 1) / 2
+  double a = cd7[1], b = cd7[2];
+  double cc = b - g * (b - a), dd = a + g * (b - a);
+  double fc = f(cc), fd = f(dd);
+  for (int it = 0; it < 60 && !((b - a) <= 1e-12 * b); ++it) {
+    if (fc >= fd) {  // the maximum is left of d: b <- d
+      b = dd;
+      dd = cc;
+      fd = fc;
+      cc = b - g * (b - a);
+      fc = f(cc);
+    } else {
+      a = cc;
+      cc = dd;
+      fc = fd;
+      dd = a + g * (b - a);
+      fd = f(dd);
+    }
+  }
+  const double q = (a + b) / 2;
+  const double eb = exp(cd7[5]);
+  const double pk = fmax(f(q), exp(cd7[6]));
+  ng = ne;
+  const double lvl = eb + (pk - eb) / 1.41421356237309504880;
+  double lo = q, hi = side ? cd7[4] : cd7[3];
+  for (int it = 0; it < 50 && !(fabs(hi - lo) <= 1e-12 * q); ++it) {
+    const double mid = (lo + hi) / 2;
+    if (f(mid) > lvl) lo = mid;
+    else hi = mid;
+  }
+  if (side == 0) A.out[3 * c] = q;
+  A.out[3 * c + 1 + side] = (lo + hi) / 2;
+  if (A.evals) atomicAdd(A.evals, side == 0 ? ne : ne - ng);
+}
+
+hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,
+                              double *out, unsigned long long *evals, bool general, hipStream_t st) {
+  if (nc <= 0) return hipSuccess;
+  RefineArgs A{L, nl, s, omega, nc, cand, out, evals};
+  const unsigned blocks = (unsigned)((2 * nc + 63) / 64);
+  if (general) hipLaunchKernelGGL(k_pole_refine<true>, dim3(blocks), dim3(64), 0, st, A);
+  else hipLaunchKernelGGL(k_pole_refine<false>, dim3(blocks), dim3(64), 0, st, A);
+  return hipGetLastError();
+}
+
 }  // namespace ytf

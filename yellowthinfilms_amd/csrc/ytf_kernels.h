@@ -115,6 +115,15 @@ hipError_t launch_batch(const BatchLaunch &B, hipStream_t s);
 // real-axis pole search; s 1-based.
 hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega, const double *kx, const double *ky,
                                   int64_t n, double *out, bool general, hipStream_t st);
+// Golden-section location and half-level crossings of nc candidates, every
+// response evaluated in the kernel (response.hip).
+hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,
+                              double *out, unsigned long long *evals, bool general, hipStream_t st);
+// Prominent maxima of a real-axis scan R[nphi][n] (response.hip); scratch Y
+// (nphi x n doubles) and ymax (nphi doubles), *count zeroed by the caller.
+hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
+                                  double log_floor_abs, const double *R, const double *ks, double *Y, double *ymax,
+                                  int max_out, int *count, int64_t *key, double *val, hipStream_t s);
 
 // Surrogate network (mlp.hip): one BF16 matrix-core GEMM with fused epilogues,
 // plus the small elementwise kernels of training and inference.

@@ -52,22 +52,37 @@ def cavity_response(stack, omega, source_layer, kx, ky, device=0):
     return out
 
 
-class _Evaluator:
-    """f(k, phi) for arrays of (k, phi), one launch per call (host arrays in and out)."""
+# Response evaluations of the last real_axis_poles call (scan, refinement).
+search_stats = {"scan_evaluations": 0, "refine_evaluations": 0}
 
-    def __init__(self, stack, omega, source_layer, device):
-        self.args = (stack, omega, source_layer)
-        self.device = device
-        self.calls = 0
 
-    def __call__(self, k, phi):
-        k = np.asarray(k, np.float64)
-        phi = np.asarray(phi, np.float64)
-        if k.size == 0:
-            return np.zeros(0)
-        self.calls += 1
-        r = cavity_response(*self.args, k * np.cos(phi), k * np.sin(phi), device=self.device)
-        return r.cpu().numpy()
+def _pole_candidates(R, ks, nphi, W, log_prom, log_floor_rel, log_floor_abs, device):
+    """ytf_pole_candidates on the scan R (device, nphi x n), sorted by (phi
+    index, i): arrays m, i and val (n_cand, 5) = (base, prom, k of the left
+    valley, k of the right valley, y[i])."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    n = len(ks)
+    ks_d = torch.as_tensor(ks, dtype=torch.float64, device=dev)
+    ctx = _lib.Context.get(device)
+    ctx.bind_stream()
+    cap = nphi * (n // 2 + 1)  # local maxima are at least two samples apart
+    while True:
+        key = torch.empty(cap, dtype=torch.int64, device=dev)
+        val = torch.empty((cap, 5), dtype=torch.float64, device=dev)
+        cnt = ct.c_int64()
+        _lib.check(_lib.lib().ytf_pole_candidates(
+            ctx.handle, int(nphi), int(n), ct.c_void_p(R.data_ptr()), ct.c_void_p(ks_d.data_ptr()), int(W),
+            float(log_prom), float(log_floor_rel), float(log_floor_abs), int(cap), ct.c_void_p(key.data_ptr()),
+            ct.c_void_p(val.data_ptr()), ct.byref(cnt)), ctx.handle)
+        if cnt.value <= cap:
+            break
+        cap = int(cnt.value)
+    nc = int(cnt.value)
+    key = key[:nc].cpu().numpy()
+    val = val[:nc].cpu().numpy()
+    o = np.argsort(key, kind="stable")
+    return key[o] // n, key[o] % n, val[o]
 
 
 def real_axis_poles(stack, omega, source_layer, phis=(0.0,), k_lo=None, k_hi=None, scan_step=1e-3,
@@ -82,95 +97,56 @@ def real_axis_poles(stack, omega, source_layer, phis=(0.0,), k_lo=None, k_hi=Non
          steps at most, to 1e-12 relative), Gamma as the nearer of the two
          crossings of b + (peak - b)/sqrt(2) (b the valley level), found by
          bisection towards each valley.
-    Peaks within 1e-6 q of a value in `exclude` (the light lines) are dropped."""
+    Peaks within 1e-6 q of a value in `exclude` (the light lines) are dropped.
+    Step 1 is one response launch; steps 2 and 3 run on the device
+    (ytf_pole_candidates, ytf_pole_refine: every refinement evaluation inside
+    the kernel)."""
     if k_lo is None or k_hi is None or not (0 < k_lo < k_hi):
         raise ValueError("real_axis_poles: need 0 < k_lo < k_hi")
-    f = _Evaluator(stack, omega, source_layer, device)
     L = math.log(k_hi / k_lo)
     nstep = int(math.floor(L / scan_step + 1e-10))
     ks = k_lo * np.exp(np.arange(nstep + 1) * scan_step)   # Julia's 0.0:step:L
     n = len(ks)
     W = int(math.ceil(math.log(4.0) / scan_step))
     phis = [float(p) for p in phis]
-    K = np.repeat(ks[None, :], len(phis), axis=0)
-    P = np.repeat(np.asarray(phis)[:, None], n, axis=1)
-    Y = np.log(f(K.ravel(), P.ravel()).reshape(len(phis), n) + 1e-300)
-    # candidates: (phi index, i, base, prom, left valley k, right valley k)
-    cand = []
-    for m, phi in enumerate(phis):
-        y = Y[m]
-        ymin = max(y.max() + math.log(floor_rel), math.log(floor_abs))
-        inner = np.arange(1, n - 1)
-        is_max = (y[1:-1] > y[:-2]) & (y[1:-1] >= y[2:]) & (y[1:-1] >= ymin)
-        for i in inner[is_max]:
-            # valleys: walk out from i while the samples stay <= y[i], at most W steps
-            lo = max(0, i - W)
-            above = np.nonzero(y[lo:i] > y[i])[0]
-            jl = lo + above[-1] + 1 if above.size else lo
-            lmin = y[jl:i + 1].min()
-            hi = min(n - 1, i + W)
-            above = np.nonzero(y[i + 1:hi + 1] > y[i])[0]
-            jr = i + above[0] if above.size else hi
-            rmin = y[i:jr + 1].min()
-            base = max(lmin, rmin)
-            prom = y[i] - base
-            if prom < math.log(prominence):
-                continue
-            kl = ks[jl + int(np.nonzero(y[jl:i + 1] == lmin)[0][-1])]
-            kr = ks[i + int(np.nonzero(y[i:jr + 1] == rmin)[0][0])]
-            cand.append((m, i, base, prom, kl, kr))
-    if not cand:
+    ph_all = np.asarray(phis, np.float64)[:, None]
+    R = cavity_response(stack, omega, source_layer, (ks[None, :] * np.cos(ph_all)).ravel(),
+                        (ks[None, :] * np.sin(ph_all)).ravel(), device=device)
+    search_stats.update(scan_evaluations=len(phis) * n, refine_evaluations=0)
+    # candidates (phi index, i; base, prom, left valley k, right valley k, y[i]),
+    # found on the device (ytf_pole_candidates), in (phi, i) order
+    cm, ci, cval = _pole_candidates(R, ks, len(phis), W, math.log(prominence), math.log(floor_rel),
+                                    math.log(floor_abs), device)
+    if cm.size == 0:
         return []
-    nc = len(cand)
-    ph = np.array([phis[c[0]] for c in cand])
-    # golden-section search of every candidate at once
-    a = np.array([ks[c[1] - 1] for c in cand])
-    b = np.array([ks[c[1] + 1] for c in cand])
-    g = (math.sqrt(5.0) - 1) / 2
-    c_ = b - g * (b - a)
-    d_ = a + g * (b - a)
-    fcd = f(np.concatenate([c_, d_]), np.concatenate([ph, ph]))
-    fc, fd = fcd[:nc], fcd[nc:]
-    for _ in range(60):
-        act = np.nonzero(~((b - a) <= 1e-12 * b))[0]
-        if act.size == 0:
-            break
-        left = act[fc[act] >= fd[act]]    # the maximum is left of d: b <- d
-        right = act[fc[act] < fd[act]]
-        b[left], d_[left], fd[left] = d_[left], c_[left], fc[left]
-        c_[left] = b[left] - g * (b[left] - a[left])
-        a[right], c_[right], fc[right] = c_[right], d_[right], fd[right]
-        d_[right] = a[right] + g * (b[right] - a[right])
-        new = f(np.concatenate([c_[left], d_[right]]), np.concatenate([ph[left], ph[right]]))
-        fc[left] = new[:left.size]
-        fd[right] = new[left.size:]
-    q = (a + b) / 2
-    Pk = np.maximum(f(q, ph), np.exp([Y[c[0], c[1]] for c in cand]))
-    keep = np.array([not any(abs(q[j] - Lx) <= 1e-6 * q[j] for Lx in exclude) for j in range(nc)], bool)
-    base = np.array([c[2] for c in cand])
-    lvl = np.exp(base) + (Pk - np.exp(base)) / math.sqrt(2.0)
-    # half-level crossings, towards each valley (2 per candidate), bisected together
-    lo = np.concatenate([q, q])
-    hi = np.concatenate([[c[4] for c in cand], [c[5] for c in cand]])
-    lv = np.concatenate([lvl, lvl])
-    qq = np.concatenate([q, q])
-    pp = np.concatenate([ph, ph])
-    for _ in range(50):
-        act = np.nonzero(~(np.abs(hi - lo) <= 1e-12 * qq))[0]
-        if act.size == 0:
-            break
-        mid = (lo[act] + hi[act]) / 2
-        above = f(mid, pp[act]) > lv[act]
-        lo[act[above]] = mid[above]
-        hi[act[~above]] = mid[~above]
-    xr = (lo + hi) / 2
-    G = np.minimum(q - xr[:nc], xr[nc:] - q)
-    out = []
-    for j, cnd in enumerate(cand):
-        if keep[j]:
-            out.append(dict(q=float(q[j]), Gamma=float(max(G[j], 1e-12 * q[j])), phi=float(ph[j]),
-                            prom=float(cnd[3])))
-    return out
+    ph = np.asarray(phis, np.float64)[cm]
+    # golden-section location and half-level crossings, on the device (ytf_pole_refine)
+    cv = np.stack([ph, ks[ci - 1], ks[ci + 1], cval[:, 2], cval[:, 3], cval[:, 0], cval[:, 4]], axis=1)
+    out = _pole_refine(stack, omega, source_layer, cv, device)
+    q, xl, xr = out[:, 0], out[:, 1], out[:, 2]
+    keep = np.ones(q.size, bool)
+    for Lx in exclude:
+        keep &= ~(np.abs(q - Lx) <= 1e-6 * q)
+    G = np.maximum(np.minimum(q - xl, xr - q), 1e-12 * q)
+    ql, Gl, pl, rl = q.tolist(), G.tolist(), ph.tolist(), cval[:, 1].tolist()
+    return [{"q": ql[j], "Gamma": Gl[j], "phi": pl[j], "prom": rl[j]} for j in np.nonzero(keep)[0].tolist()]
+
+
+def _pole_refine(stack, omega, source_layer, cv, device):
+    """ytf_pole_refine on the candidates cv (n, 7): (n, 3) = (q, left crossing,
+    right crossing); the number of responses taken goes to `search_stats`."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    c = torch.as_tensor(np.ascontiguousarray(cv), device=dev)
+    out = torch.empty((len(cv), 3), dtype=torch.float64, device=dev)
+    ctx = _lib.Context.get(device)
+    ctx.bind_stream()
+    ne = ct.c_int64()
+    _lib.check(_lib.lib().ytf_pole_refine(ctx.handle, layer_structs(stack), len(stack), float(omega),
+                                          int(source_layer), len(cv), ct.c_void_p(c.data_ptr()),
+                                          ct.c_void_p(out.data_ptr()), ct.byref(ne)), ctx.handle)
+    search_stats["refine_evaluations"] = int(ne.value)
+    return out.cpu().numpy()
 
 
 def clusters(poles, merge_rel):
