@@ -186,6 +186,9 @@ int ytf_pole_candidates(ytf_ctx *ctx, int n_dirs, int64_t n, const double *R_dev
  *   level = e^base + (max(f(q), e^y) - e^base)/sqrt(2);
  *   the crossings of the level by bisection between q and each valley (at most
  *      50 steps, until |hi - lo| <= 1e-12 q).
+ * Each search runs on 8 lanes, 3 steps per round (the decision tree of the
+ * next 3 points evaluated at once), with the points and results of the
+ * one-step-at-a-time search.
  * out_dev[3c .. 3c+2] = (q, left crossing, right crossing).  The stack arguments
  * are those of ytf_cavity_response.  n_evals may be NULL; otherwise the call
  * waits for the stream and sets *n_evals = the number of responses the

@@ -262,17 +262,25 @@ hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, d
 
 // ---------------------------------------------------------------------------
 // Location and half-width of the candidates (real_axis_poles step 3), every
-// evaluation inside the kernel.  Thread 2c + side of candidate c = (phi, a, b,
+// evaluation inside the kernel.  Chain 2c + side of candidate c = (phi, a, b,
 // kl, kr, base, y) (cand[7c ..]):
 //   golden-section search for the maximum of f(k) = response(k cos phi,
 //   k sin phi) on [a, b] (at most 60 steps, until b - a <= 1e-12 b; both
-//   threads of a candidate take the same steps), q = (a + b) / 2;
+//   chains of a candidate take the same steps), q = (a + b) / 2;
 //   level = e^base + (max(f(q), e^y) - e^base) / sqrt 2;
 //   bisection (at most 50 steps, until |hi - lo| <= 1e-12 q) between q and
 //   the valley kl (side 0) or kr (side 1) for the crossing of the level.
-// out[3c] = q, out[3c + 1 + side] = the crossing.  With `evals`, the number
-// of responses the search takes (the golden-section ones once per candidate)
-// is added to *evals.
+// out[3c] = q, out[3c + 1 + side] = the crossing.
+//
+// Each chain takes 8 lanes and advances 3 steps per evaluation round: the
+// points a step visits depend only on the earlier steps' decisions, so the
+// lanes evaluate the 7 points of the decision tree of the next 3 steps
+// (lane 0 the next point, lanes 1-2 the two possible points after it, lanes
+// 3-6 the four after those; lane 7 takes f(q) in the first bisection round),
+// then every lane replays the 3 steps with the actual values.  The steps and
+// the points they keep are those of one-step-at-a-time search.
+// With `evals`, the number of responses the one-step search takes (the
+// golden-section ones once per candidate) is added to *evals.
 struct RefineArgs {
   const LayerDev *L;
   int nl, s;
@@ -283,61 +291,140 @@ struct RefineArgs {
   unsigned long long *evals;
 };
 
+constexpr double GOLD = 0.61803398874989484820;  // (sqrt 5 - 1) / 2
+
+// One golden-section step with decision `left` (the maximum is left of d).
+__device__ __forceinline__ double gs_step(bool left, double &a, double &b, double &c, double &d) {
+  if (left) {
+    b = d;
+    d = c;
+    c = b - GOLD * (b - a);
+    return c;
+  }
+  a = c;
+  c = d;
+  d = a + GOLD * (b - a);
+  return d;
+}
+
 template <bool GENERAL>
 __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
   __shared__ cd vprev[16 * 64];
   const LaneMat Vp{vprev, (int)threadIdx.x};
-  const int64_t t = blockIdx.x * (int64_t)64 + threadIdx.x;
-  if (t >= 2 * A.nc) return;
-  const int64_t c = t >> 1;
-  const int side = (int)(t & 1);
+  const int64_t chain = (blockIdx.x * (int64_t)64 + threadIdx.x) >> 3;
+  const int l = threadIdx.x & 7, gb = threadIdx.x & ~7;  // lane in the chain, the chain's first lane
+  if (chain >= 2 * A.nc) return;  // whole chains: 2 nc chains of 8 lanes
+  const int64_t c = chain >> 1;
+  const int side = (int)(chain & 1);
   const double *cd7 = A.cand + 7 * c;
   const double phi = cd7[0], cphi = cos(phi), sphi = sin(phi);
+  auto f = [&](double k) { return response_node<GENERAL>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp); };
+  // lane l < 7: node l of the tree; its level and the decisions on its path
+  const int lev = l == 0 ? 0 : (l < 3 ? 1 : 2);
+  const bool dec1 = l == 1 || l == 3 || l == 4;  // level-1 decision (true: child 2p + 1)
+  const bool dec2 = l == 3 || l == 5;            // level-2 decision
   unsigned long long ne = 0, ng = 0;
-  auto f = [&](double k) {
-    ++ne;
-    return response_node<GENERAL>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp);
-  };
-  const double g = 0.61803398874989484820;  // (sqrt 5 - 1) / 2
+  // One evaluation per lane and round, at one call site.  Phases: 0 f(c) and
+  // f(d) (lanes 0, 1); 1 golden-section rounds; 2 bisection rounds (lane 7
+  // takes f(q) in the first); 3 done.
   double a = cd7[1], b = cd7[2];
-  double cc = b - g * (b - a), dd = a + g * (b - a);
-  double fc = f(cc), fd = f(dd);
-  for (int it = 0; it < 60 && !((b - a) <= 1e-12 * b); ++it) {
-    if (fc >= fd) {  // the maximum is left of d: b <- d
-      b = dd;
-      dd = cc;
-      fd = fc;
-      cc = b - g * (b - a);
-      fc = f(cc);
+  double cc = b - GOLD * (b - a), dd = a + GOLD * (b - a);
+  double fc = 0.0, fd = 0.0, q = 0.0, lvl = 0.0, lo = 0.0, hi = 0.0;
+  const double eb = exp(cd7[5]);
+  int phase = 0, it = 0;
+  bool first = true;
+  while (phase < 3) {
+    double x = 0.0;
+    bool ev = false;
+    if (phase == 0) {
+      x = l == 0 ? cc : dd;
+      ev = l < 2;
+    } else if (phase == 1) {  // this lane's point: step 0 (known decision), then its path
+      double ta = a, tb = b, tc = cc, td = dd;
+      x = gs_step(fc >= fd, ta, tb, tc, td);
+      if (lev >= 1) x = gs_step(dec1, ta, tb, tc, td);
+      if (lev >= 2) x = gs_step(dec2, ta, tb, tc, td);
+      ev = l < 7;
     } else {
-      a = cc;
-      cc = dd;
-      fc = fd;
-      dd = a + g * (b - a);
-      fd = f(dd);
+      double tlo = lo, thi = hi;
+      x = (tlo + thi) / 2;
+      if (lev >= 1) {
+        (dec1 ? tlo : thi) = x;
+        x = (tlo + thi) / 2;
+      }
+      if (lev >= 2) {
+        (dec2 ? tlo : thi) = x;
+        x = (tlo + thi) / 2;
+      }
+      const bool bis = it < 50 && !(fabs(hi - lo) <= 1e-12 * q);
+      if (l == 7) x = q;
+      ev = l < 7 ? bis : first;
+    }
+    const double v = ev ? f(x) : 0.0;
+    if (phase == 0) {
+      fc = __shfl(v, gb);
+      fd = __shfl(v, gb + 1);
+      ne += 2;
+      phase = 1;
+    } else if (phase == 1) {
+      int node = 0;
+      for (int j = 0; j < 3; ++j) {
+        if (!(it < 60 && !((b - a) <= 1e-12 * b))) break;
+        const bool left = fc >= fd;
+        if (j > 0) node = 2 * node + (left ? 1 : 2);
+        const double fx = __shfl(v, gb + node);
+        gs_step(left, a, b, cc, dd);
+        if (left) {
+          fd = fc;
+          fc = fx;
+        } else {
+          fc = fd;
+          fd = fx;
+        }
+        ++it;
+        ++ne;
+      }
+    } else {
+      if (first) {
+        const double pk = fmax(__shfl(v, gb + 7), exp(cd7[6]));
+        lvl = eb + (pk - eb) / 1.41421356237309504880;
+        first = false;
+      }
+      int node = 0;
+      for (int j = 0; j < 3; ++j) {
+        if (!(it < 50 && !(fabs(hi - lo) <= 1e-12 * q))) break;
+        const double mid = (lo + hi) / 2;
+        const bool up = __shfl(v, gb + node) > lvl;
+        if (up) lo = mid;
+        else hi = mid;
+        node = 2 * node + (up ? 1 : 2);
+        ++it;
+        ++ne;
+      }
+      if (!(it < 50 && !(fabs(hi - lo) <= 1e-12 * q))) phase = 3;
+    }
+    if (phase == 1 && !(it < 60 && !((b - a) <= 1e-12 * b))) {  // golden section done: start the bisection
+      q = (a + b) / 2;
+      ng = ne + 1;  // and f(q)
+      ++ne;
+      lo = q;
+      hi = side ? cd7[4] : cd7[3];
+      it = 0;
+      phase = 2;
     }
   }
-  const double q = (a + b) / 2;
-  const double eb = exp(cd7[5]);
-  const double pk = fmax(f(q), exp(cd7[6]));
-  ng = ne;
-  const double lvl = eb + (pk - eb) / 1.41421356237309504880;
-  double lo = q, hi = side ? cd7[4] : cd7[3];
-  for (int it = 0; it < 50 && !(fabs(hi - lo) <= 1e-12 * q); ++it) {
-    const double mid = (lo + hi) / 2;
-    if (f(mid) > lvl) lo = mid;
-    else hi = mid;
+  if (l == 0) {
+    if (side == 0) A.out[3 * c] = q;
+    A.out[3 * c + 1 + side] = (lo + hi) / 2;
+    if (A.evals) atomicAdd(A.evals, side == 0 ? ne : ne - ng);
   }
-  if (side == 0) A.out[3 * c] = q;
-  A.out[3 * c + 1 + side] = (lo + hi) / 2;
-  if (A.evals) atomicAdd(A.evals, side == 0 ? ne : ne - ng);
 }
 
 hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,
                               double *out, unsigned long long *evals, bool general, hipStream_t st) {
   if (nc <= 0) return hipSuccess;
   RefineArgs A{L, nl, s, omega, nc, cand, out, evals};
-  const unsigned blocks = (unsigned)((2 * nc + 63) / 64);
+  const unsigned blocks = (unsigned)((16 * nc + 63) / 64);  // 8 lanes per chain, 2 chains per candidate
   if (general) hipLaunchKernelGGL(k_pole_refine<true>, dim3(blocks), dim3(64), 0, st, A);
   else hipLaunchKernelGGL(k_pole_refine<false>, dim3(blocks), dim3(64), 0, st, A);
   return hipGetLastError();
