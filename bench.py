@@ -39,6 +39,10 @@ Workloads (yellowthinfilms_amd.workloads):
     directions, k_rho from 0.5 k0 to 3000 at a relative step of 1e-3, then
     the refinement of every maximum.  value = cavity-response evaluations/s
     of the whole search.
+  probe_batch_box: the probe integrands of the adaptive box k_rho grid on
+    the reference's grid validation setup (8 box probes, 96 phi nodes) for
+    one batch of 4096 k_rho.  value = k_rho probe sets/s (the reference
+    builds one one-node cache per visited k_rho, ~1.3 ms each).
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -993,10 +997,91 @@ def dict_ns(a, **kw):
     return b
 
 
+def bench_probes(a):
+    import torch
+    from yellowthinfilms_amd import workloads as W
+    from yellowthinfilms_amd.probes import probe_values
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.PROBE_BATCH_BOX
+    st = W.probe_batch_stack(wl)
+    kr, phis, wph, probes = W.probe_batch_args(wl)
+    res = {}
+
+    def step():
+        res["v"] = probe_values(st, W.OMEGA, wl["source_layer"], kr, phis, wph, probes, wl["box"])
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    node_ms, node_n = ctx.kernel_time("node_solve")
+    lat_ms, _ = ctx.kernel_time("lateral_segments")
+    ctx.set_timing(False)
+    nodes = len(kr) * len(phis)
+    flops = W.PROBE_BATCH_BOX_FLOPS_PER_NODE * nodes  # the 3 node-solve launches of one step
+    node_step = node_ms / a.steps
+    cpu = cmp = None
+    if not a.no_cpu_baseline:
+        cpu, cmp = cpu_probes(st, wl, kr, phis, wph, probes, res["v"])
+    out = _base(a, wl, len(kr) * a.steps / wall, "k_rho probe sets/s (8 box probes, 96 phi nodes each)", wall, "f64")
+    out["config"].update({"krho": len(kr), "phi": len(phis), "probes": len(probes), "nodes": nodes})
+    out.update({
+        "kernel_ms": {"node_solve": node_step, "lateral_segments": lat_ms / a.steps},
+        "node_launches_per_step": node_n / a.steps,
+        "ms_per_krho": wall * 1e3 / (a.steps * len(kr)),
+        "rel_err_vs_fp64_oracle": cmp,
+        "roofline": {"kernel": "node_solve (one launch per observation layer, box source, FP64)", "bound": "mfma",
+                     "achieved": flops / (node_step * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / (node_step * 1e-3) / PEAK_FP64, "traffic": None,
+                     "flops_per_node": W.PROBE_BATCH_BOX_FLOPS_PER_NODE, "nodes_per_step": nodes},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
+def cpu_probes(st, wl, kr, phis, wph, probes, v_gpu):
+    """The oracle's probe values (per-node spectral dyadic with the box
+    factors on all host cores, then the exact lateral sum per k_rho) over the
+    whole batch, timed; and the GPU's largest error relative to each probe's
+    largest entry."""
+    try:
+        from oracle import pyoracle as po
+    except Exception as exc:
+        return ({"value": None, "unit": "k_rho probe sets/s", "cores": 0, "kind": "port",
+                 "sample": f"oracle unavailable: {exc}"}, None)
+    from yellowthinfilms_amd.green_operator import box_lateral_factor
+    from yellowthinfilms_amd.materials import stack_arrays
+    ost = po.Stack(*stack_arrays(st))
+    cores = _cores()
+    box = wl["box"]
+    kx = (kr[:, None] * np.cos(phis)[None, :]).ravel()
+    ky = (kr[:, None] * np.sin(phis)[None, :]).ravel()
+    W = (kr[:, None] * wph[None, :]).ravel() / (4 * math.pi ** 2) * box_lateral_factor(kx, ky, box[0], box[1])
+    L = len(phis)
+    t0 = time.perf_counter()
+    ref = np.zeros((len(kr), len(probes), 3, 3), np.complex128)
+    for obs in sorted({p[0] for p in probes}):
+        idx = [m for m, p in enumerate(probes) if p[0] == obs]
+        g = po.ghat_batch(ost, 2 * math.pi, kx, ky, wl["source_layer"], obs, [probes[m][5][0] for m in idx],
+                          [probes[m][1] for m in idx], nthreads=cores, dz_box=box[2], ez_local=True)
+        S = g * W[:, None, None, None]
+        S = np.where(np.isfinite(S), S, 0.0)
+        for j, m in enumerate(idx):
+            ph = np.exp(1j * (kx * probes[m][2] + ky * probes[m][3]))
+            ref[:, m] = (S[:, j] * ph[:, None, None]).reshape(len(kr), L, 3, 3).sum(axis=1)
+    dt = time.perf_counter() - t0
+    scale = np.maximum(np.abs(ref).reshape(len(kr), len(probes), 9).max(axis=2), 1e-300)
+    err = float((np.abs(v_gpu - ref).reshape(len(kr), len(probes), 9).max(axis=2) / scale).max())
+    return ({"value": len(kr) / dt, "unit": "k_rho probe sets/s", "cores": cores, "kind": "port",
+             "sample": f"the whole batch ({len(kr)} k_rho x {L} phi, {len(probes)} probes)"}, err)
+
+
 WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
              "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft,
              "operator_512": bench_operator, "operator_volume_64": bench_operator_volume,
-             "pole_scan_hyperbolic": bench_pole_scan}
+             "pole_scan_hyperbolic": bench_pole_scan, "probe_batch_box": bench_probes}
 
 
 def main():

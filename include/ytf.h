@@ -359,6 +359,31 @@ int ytf_closed_form_G_2d(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, co
                          const double *w_dev, int n_pairs, const double *ghat_dev, const double *xs, int nx,
                          const double *ys, int ny, int lateral, double nufft_eps, double *G_dev);
 
+/* Nodes of one-k_rho-node caches (build_cavity_transfer_2d with one k_rho of
+ * weight 1, mod_ClosedFormGrid.jl:209) for a batch of k_rho, on the device:
+ * node k = i n_phi + j (k_rho i, phi j) at kx = krho_i cos phi_j,
+ * ky = krho_i sin phi_j, with the complex weight
+ *   w_k = krho_i wphi_j / (4 pi^2) x sinc(kx box_dx / 2) sinc(ky box_dy / 2)
+ * (the node weights and box footprint of mod_ClosedFormField.jl:517-524,
+ * 723-727; box_dx = box_dy = 0 for a point source).  All arrays on the device. */
+int ytf_polar_nodes(ytf_ctx *ctx, int64_t n_krho, const double *krho_dev, int n_phi, const double *phi_dev,
+                    const double *wphi_dev, double box_dx, double box_dy, double *kx_dev, double *ky_dev,
+                    double *w_dev);
+
+/* The probe values of the adaptive k_rho grids (mod_ClosedFormGrid.jl:204-218:
+ * closed_form_G of a one-k_rho-node CavityTransferCache2D at one point per
+ * probe, for every visited k_rho), many k_rho per call.  The nodes come in
+ * n_seg segments of seg_len consecutive nodes (one k_rho, every phi); for
+ * segment s and depth pair p:
+ *   out[(s n_pairs + p) 9 + c + 3q] = sum_{j < seg_len} W_k g~_p(k)[c + 3q]
+ *                                     exp(i (kx_k x_p + ky_k y_p)),  k = s seg_len + j,
+ * with the weights, zeroing and layouts of ytf_closed_form_G_2d (ghat_dev:
+ * [n_pairs][9][n_seg seg_len]); x_p, y_p (host arrays, n_pairs) is the probe
+ * point of pair p.  out_dev: n_seg x n_pairs x 9 complex. */
+int ytf_segment_lateral_sum(ytf_ctx *ctx, int64_t n_seg, int seg_len, const double *kx_dev, const double *ky_dev,
+                            const double *w_dev, int n_pairs, const double *ghat_dev, const double *xp,
+                            const double *yp, double *out_dev);
+
 /* The pixel grid of uniform_kxky_grid (mod_UniformSource.jl:89-116) as the
  * operator uses it: Nx x Ny pixels of pitch dx, dy.  The operator samples G
  * on the 2Nx x 2Ny padded offsets (padded_offsets,

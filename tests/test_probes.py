@@ -1,0 +1,49 @@
+"""The probe list of the adaptive box grid (box_kρ_grid, mod_ClosedFormGrid.jl:
+163-197) on the reference's own validation setup (Gz_ClosedForm_Grid_Validation.jl:
+25-28, 67-103: vac 1.0 / glass 0.6 / vac 1.0, box 0.1^3 at zbar = 0.3 in layer 2,
+rho_far = hypot(0.5, 0.5)), written out from the reference's rules."""
+import math
+
+import pytest
+
+from helpers import lay
+
+
+def _glass():
+    from yellowthinfilms_amd.materials import IsotropicMaterial
+    return [lay(IsotropicMaterial(1.0 + 0.001j), 1.0), lay(IsotropicMaterial(2.25 + 0.05j), 0.6),
+            lay(IsotropicMaterial(1.0 + 0.001j), 1.0)]
+
+
+def test_box_probes_of_the_reference_validation():
+    from yellowthinfilms_amd.probes import box_probes
+    rf = math.hypot(0.5, 0.5)
+    got = box_probes(_glass(), 2, (0.1, 0.1, 0.1), zbar=[0.3], rho_far=rf)
+    want = [  # (obs layer, z_obs, x, y, h, tag): delta = 0.5 min(dz, thinnest layer) = 0.05
+        (2, 0.3, 0.0, 0.0, 0.0, "self voxel"),
+        (2, 0.3, 0.1, 0.0, 0.0, "lateral neighbour"),
+        (2, 0.4, 0.0, 0.0, 0.05, "vertical neighbour +"),
+        (2, 0.2, 0.0, 0.0, 0.05, "vertical neighbour −"),
+        (1, 0.95, 0.0, 0.0, 0.3, "layer above, near interface"),       # h = zp - dz/2 + delta
+        (3, 0.05, 0.0, 0.0, 0.3, "layer below, near interface"),       # h = d - zp - dz/2 + delta
+        (2, 0.4, rf, 0.0, 0.05, "far lateral"),                         # just outside the slab, below
+        (2, 0.3, rf, 0.0, 0.0, "far lateral, inside the slab"),
+    ]
+    assert len(got) == len(want)
+    for g, w in zip(got, want):
+        assert g[0] == w[0] and g[5] == (0.3, w[5])
+        assert g[1:5] == pytest.approx(w[1:5], abs=1e-15)
+
+
+def test_box_probes_default_depths_layer_one_and_exterior_only():
+    from yellowthinfilms_amd.probes import box_probes
+    st = _glass()
+    # default depths: the centre and both near-face positions (d/2, dz/2 + 1e-3 d, d - dz/2 - 1e-3 d)
+    got = box_probes(st, 2, (0.1, 0.1, 0.1))
+    assert sorted({p[5][0] for p in got}) == pytest.approx([0.0506, 0.3, 0.5494])
+    # source layer 1: the cavity frame zpos = d - zbar, no layer above
+    got = box_probes(st, 1, (0.1, 0.1, 0.2), zbar=[0.5], include_interior=False)
+    assert [p[5][1] for p in got] == ["vertical neighbour +", "vertical neighbour −", "layer below, near interface"]
+    # delta = 0.5 min(dz = 0.2, thinnest layer 0.6) = 0.1
+    assert [p[1] for p in got] == pytest.approx([0.7, 0.3, 0.1])
+    assert got[2][4] == pytest.approx(1.0 - 0.5 - 0.1 + 0.1)

@@ -55,3 +55,22 @@ def test_stack_op_count_scales_with_layers():
     f = lambda st: sum(po.count_stack_ops(ostack(st), W.OMEGA, 3.0, 1.0)[k] for k in ("add", "mul", "div"))
     assert f(bi) > 3 * f(iso)
     assert abs((f(bi) - f(bi[:3])) / (f(bi[:5]) - f(bi[:2])) - 1.0) < 0.15
+
+
+def test_probe_batch_box_flops_per_node():
+    import math
+    wl = W.PROBE_BATCH_BOX
+    kr, _, _, probes = W.probe_batch_args(wl)
+    st = ostack(W.probe_batch_stack(wl))
+    rng = np.random.default_rng(0)
+    tot = []
+    for _ in range(64):
+        k, phi = rng.choice(kr), rng.uniform(0, 2 * math.pi)
+        f = 0
+        for obs in sorted({p[0] for p in probes}):
+            pr = [p for p in probes if p[0] == obs]
+            c = po.count_node_ops(st, W.OMEGA, k * math.cos(phi), k * math.sin(phi), wl["source_layer"], obs,
+                                  pr[0][5][0], pr[0][1])
+            f += sum(c["solve"][x] for x in ("add", "mul", "div")) + len(pr) * sum(c["ghat"][x] for x in ("add", "mul", "div"))
+        tot.append(f)
+    assert abs(np.mean(tot) - W.PROBE_BATCH_BOX_FLOPS_PER_NODE) < 1.0

@@ -114,6 +114,10 @@ _SIGS = {
                                           ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
     "ytf_nufft2d1": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
                                 ct.c_int, ct.c_double, ct.c_void_p]),
+    "ytf_polar_nodes": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                   ct.c_double, ct.c_double, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "ytf_segment_lateral_sum": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                           ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]),
     "ytf_closed_form_G_2d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
                                         ct.c_void_p, ct.POINTER(ct.c_double), ct.c_int, ct.POINTER(ct.c_double),
                                         ct.c_int, ct.c_int, ct.c_double, ct.c_void_p]),

@@ -573,6 +573,47 @@ bool single_plane(const OpBlock &b, int Ns) {
 
 extern "C" {
 
+int ytf_polar_nodes(ytf_ctx *c, int64_t n_krho, const double *krho, int n_phi, const double *phi,
+                    const double *wphi, double box_dx, double box_dy, double *kx, double *ky, double *w) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_polar_nodes: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_krho < 0 || n_phi < 1 || n_krho > INT64_MAX / n_phi || !(box_dx >= 0.0) || !(box_dy >= 0.0))
+    return fail(c, YTF_EINVAL, "ytf_polar_nodes: bad sizes or box");
+  if (n_krho == 0) return YTF_OK;
+  if (!krho || !phi || !wphi || !kx || !ky || !w) return fail(c, YTF_EINVAL, "ytf_polar_nodes: NULL buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipError_t e = ytf::launch_polar_nodes(n_krho, n_phi, krho, phi, wphi, box_dx, box_dy, kx, ky, (cd *)w, c->stream);
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_polar_nodes: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+int ytf_segment_lateral_sum(ytf_ctx *c, int64_t n_seg, int seg_len, const double *kx, const double *ky,
+                            const double *w, int np, const double *ghat, const double *xp, const double *yp,
+                            double *out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_segment_lateral_sum: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_seg < 0 || n_seg > INT32_MAX || seg_len < 1 || np < 1 || np > 65535)
+    return fail(c, YTF_EINVAL, "ytf_segment_lateral_sum: bad sizes (n_seg=%lld, seg_len=%d, n_pairs=%d)",
+                (long long)n_seg, seg_len, np);
+  if (n_seg == 0) return YTF_OK;
+  if (!kx || !ky || !w || !ghat || !xp || !yp || !out)
+    return fail(c, YTF_EINVAL, "ytf_segment_lateral_sum: NULL buffer");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, sizeof(double) * 2 * (size_t)np);
+  if (rc) return rc;
+  std::vector<double> h(xp, xp + np);
+  h.insert(h.end(), yp, yp + np);
+  rc = upload(c, c->aux, h.data(), sizeof(double) * h.size());
+  if (rc) return rc;
+  const double *xd = (const double *)c->aux;
+  hipError_t e = timed(c, "lateral_segments", [&] {
+    return ytf::launch_lat_segments(n_seg, seg_len, kx, ky, (const cd *)w, np, (const cd *)ghat, xd, xd + np,
+                                    (cd *)out, c->stream);
+  });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_segment_lateral_sum: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
 int ytf_closed_form_G_2d(ytf_ctx *c, int64_t n, const double *kx, const double *ky, const double *w, int np,
                          const double *ghat, const double *xs, int nx, const double *ys, int ny, int lateral,
                          double eps, double *G) {

@@ -128,6 +128,85 @@ __global__ __launch_bounds__(256) void k_lat_direct(int64_t n, const double *__r
   }
 }
 
+// Per-segment lateral sums at one point per depth pair: the probe values of
+// the adaptive k_rho grids (mod_ClosedFormGrid.jl:204-218), i.e. closed_form_G
+// of a one-k_rho-node 2D cache, for many k_rho at once.  Nodes come in
+// segments of L consecutive nodes (one k_rho, every phi);
+//   out[(s np + p) 9 + e] = sum_{j < L} f(W[k] g~[p][e][k]) e^{i kx[k] x[p]} e^{i ky[k] y[p]},
+// k = s L + j, f = finite_or_zero: the terms of k_lat_direct for that point.
+// One wave per (segment, pair); lanes stride over the segment, then a
+// fixed-order shuffle tree adds the lanes.
+__global__ __launch_bounds__(64) void k_lat_segments(int64_t n, int L, const double *__restrict__ kx,
+                                                     const double *__restrict__ ky, const cd *__restrict__ W, int np,
+                                                     const cd *__restrict__ g, const double *__restrict__ xp,
+                                                     const double *__restrict__ yp, cd *__restrict__ out) {
+  const int64_t s = blockIdx.x;
+  const int p = blockIdx.y, lane = threadIdx.x;
+  const double x = xp[p], y = yp[p];
+  const cd *gp = g + (int64_t)p * 9 * n;
+  double acc[18];
+#pragma unroll
+  for (int e = 0; e < 18; ++e) acc[e] = 0.0;
+  for (int j = lane; j < L; j += 64) {
+    const int64_t k = s * L + j;
+    cd ax, ay;
+    sincos(kx[k] * x, &ax.i, &ax.r);
+    sincos(ky[k] * y, &ay.i, &ay.r);
+    const cd ph = ax * ay;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const cd v = fmac(cd{acc[2 * e], acc[2 * e + 1]}, finite_or_zero(W[k] * gp[e * n + k]), ph);
+      acc[2 * e] = v.r;
+      acc[2 * e + 1] = v.i;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int e = 0; e < 18; ++e) acc[e] += __shfl_xor(acc[e], off);
+  if (lane == 0) {
+    cd *o = out + (s * np + p) * 9;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) o[e] = cd{acc[2 * e], acc[2 * e + 1]};
+  }
+}
+
+// Nodes of one-k_rho-node caches for a batch of k_rho: node k = i L + j
+// (k_rho i, phi j) at (kr_i cos phi_j, kr_i sin phi_j) with weight
+// W = kr_i w_j / 4 pi^2 x sinc(kx dx/2) sinc(ky dy/2) (the box footprint,
+// mod_ClosedFormField.jl:517-524, 723-727; sinc(u) = sin(u)/u, 1 at 0).
+__device__ __forceinline__ double sinc_phys(double u) { return u == 0.0 ? 1.0 : sin(u) / u; }
+
+__global__ void k_polar_nodes(int64_t nk, int L, const double *__restrict__ kr, const double *__restrict__ phi,
+                              const double *__restrict__ wphi, double dx, double dy, double *__restrict__ kx,
+                              double *__restrict__ ky, cd *__restrict__ W) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= nk * L) return;
+  const int64_t i = k / L;
+  const int j = (int)(k - i * L);
+  const double r = kr[i], a = r * cos(phi[j]), b = r * sin(phi[j]);
+  kx[k] = a;
+  ky[k] = b;
+  W[k] = cd{r * wphi[j] / (4.0 * 9.8696044010893586188) * (sinc_phys(a * dx / 2) * sinc_phys(b * dy / 2)), 0.0};
+}
+
+hipError_t launch_polar_nodes(int64_t nk, int L, const double *kr, const double *phi, const double *wphi, double dx,
+                              double dy, double *kx, double *ky, cd *W, hipStream_t s) {
+  if (nk <= 0 || L <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_polar_nodes, dim3((unsigned)((nk * L + 255) / 256)), dim3(256), 0, s, nk, L, kr, phi, wphi,
+                     dx, dy, kx, ky, W);
+  return hipGetLastError();
+}
+
+hipError_t launch_lat_segments(int64_t n_seg, int L, const double *kx, const double *ky, const cd *W, int np,
+                               const cd *g, const double *xp, const double *yp, cd *out, hipStream_t s) {
+  if (n_seg <= 0 || np <= 0) return hipSuccess;
+  if (n_seg > 0x7fffffff || np > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_lat_segments, dim3((unsigned)n_seg, (unsigned)np), dim3(64), 0, s, n_seg * L, L, kx, ky, W,
+                     np, g, xp, yp, out);
+  return hipGetLastError();
+}
+
 // --------------------------------------------------------------------------
 // Padded 2D FFT passes
 // --------------------------------------------------------------------------

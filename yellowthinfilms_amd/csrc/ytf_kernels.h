@@ -219,6 +219,12 @@ hipError_t launch_lat_points(int64_t n, const double *kx, const double *ky, cons
                              double x0, double y0, double *xf, double *yf, cd *wr, hipStream_t s);
 hipError_t launch_lat_strengths(int64_t n, int np, const cd *g, const cd *w, cd *out, int64_t tp, int64_t te,
                                 hipStream_t s);
+// Nodes and box-footprint weights of one-k_rho-node caches (operator.hip).
+hipError_t launch_polar_nodes(int64_t nk, int L, const double *kr, const double *phi, const double *wphi, double dx,
+                              double dy, double *kx, double *ky, cd *W, hipStream_t s);
+// Per-segment lateral sums at one point per depth pair (operator.hip).
+hipError_t launch_lat_segments(int64_t n_seg, int L, const double *kx, const double *ky, const cd *W, int np,
+                               const cd *g, const double *xp, const double *yp, cd *out, hipStream_t s);
 hipError_t launch_lat_direct(int64_t n, const double *kx, const double *ky, const cd *W, int np, const cd *g,
                              const double *xs, int nx, const double *ys, int ny, cd *G, int64_t sp, int64_t se,
                              hipStream_t s);

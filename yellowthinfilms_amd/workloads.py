@@ -283,6 +283,42 @@ def pole_scan_args(wl=POLE_SCAN_HYPERBOLIC):
     return phis, wl["k_lo_k0"] * OMEGA, wl["k_hi"], wl["scan_step"], [OMEGA]
 
 
+PROBE_BATCH_BOX = dict(
+    name="probe_batch_box",
+    description=("probe integrands of the adaptive box k_rho grid (box_kρ_grid, mod_ClosedFormGrid.jl:155-255) on "
+                 "the reference's grid validation setup (vac 1.0 / glass 2.25+0.05i 0.6 / vac 1.0, omega = 2 pi, box "
+                 "0.1^3 centred at 0.3 in layer 2, rho_far = hypot(0.5, 0.5): 8 probes in 3 observation layers, "
+                 "96-node phi grid): one batch of 4096 log-spaced k_rho in [1e-3, 6000] -> the 3x3 box Green's "
+                 "tensor of every probe at every k_rho (one-k_rho-node caches: node solves of every (k_rho, phi) "
+                 "with the box factors, interior slab and local Ez, then the phi sums); complex FP64"),
+    n_krho=4096, k_lo=1e-3, k_hi=6000.0, n_phi=96, box=(0.1, 0.1, 0.1), zbar=0.3, source_layer=2,
+)
+# Algorithmic FP64 operations per (k_rho, phi) node of probe_batch_box: the
+# oracle's reference-order op count (real adds + multiplies + divides) of the
+# node solve for each observation layer plus the spectral dyadic of each of
+# that layer's probes (point-source count: the box factors come on top), mean
+# over 64 seeded nodes.  Pinned by tests/test_workloads.py.
+PROBE_BATCH_BOX_FLOPS_PER_NODE = 52578.0
+
+
+def probe_batch_stack(wl=PROBE_BATCH_BOX):
+    """vac / glass / vac of Gz_ClosedForm_Grid_Validation.jl:25-28."""
+    from .materials import IsotropicMaterial, FlatOpticalLayer
+    return [FlatOpticalLayer(IsotropicMaterial(1.0 + 0.001j), 1.0), FlatOpticalLayer(IsotropicMaterial(2.25 + 0.05j), 0.6),
+            FlatOpticalLayer(IsotropicMaterial(1.0 + 0.001j), 1.0)]
+
+
+def probe_batch_args(wl=PROBE_BATCH_BOX):
+    """(k_rho nodes, phi nodes, phi weights, probes) of probe_batch_box."""
+    from .probes import box_probes
+    from .quadrature import build_kphi_grid
+    kr = np.geomspace(wl["k_lo"], wl["k_hi"], wl["n_krho"])
+    phis, wph = build_kphi_grid(wl["n_phi"])
+    probes = box_probes(probe_batch_stack(wl), wl["source_layer"], wl["box"], zbar=[wl["zbar"]],
+                        rho_far=math.hypot(0.5, 0.5))
+    return kr, phis, wph, probes
+
+
 BATCHED_GYRO_512 = dict(
     name="batched_gyro_512",
     description=("5-layer stack with a 40 nm gyrotropic film (g=0.3, Im eps=0.02), uniform 512x512 kx,ky grid "
