@@ -43,6 +43,10 @@ Workloads (yellowthinfilms_amd.workloads):
     the reference's grid validation setup (8 box probes, 96 phi nodes) for
     one batch of 4096 k_rho.  value = k_rho probe sets/s (the reference
     builds one one-node cache per visited k_rho, ~1.3 ms each).
+  box_krho_grid: the whole adaptive box k_rho grid of a 1/8-wavelength
+    voxel in the glass layer of the same stack (22 probes, 64 phi, rtol
+    1e-6, k_rho cap 2000): pole scan, seeds and the lock-step refinement.
+    value = distinct k_rho solved per second of grid building.
 
 Inputs are resident in HBM before the timed region.  Each line carries a
 roofline record of the dominant kernel, timed live with HIP events on the
@@ -1078,10 +1082,69 @@ def cpu_probes(st, wl, kr, phis, wph, probes, v_gpu):
              "sample": f"the whole batch ({len(kr)} k_rho x {L} phi, {len(probes)} probes)"}, err)
 
 
+def bench_box_grid(a):
+    import torch
+    from yellowthinfilms_amd import workloads as W
+    from yellowthinfilms_amd.adaptive import box_krho_grid
+    from yellowthinfilms_amd.probes import probe_values
+    from yellowthinfilms_amd._lib import Context
+
+    wl = W.BOX_KRHO_GRID
+    st = W.probe_batch_stack()
+    res = {}
+
+    def step():
+        res["g"] = box_krho_grid(st, W.OMEGA, wl["source_layer"], wl["box"], n_phi=wl["n_phi"], rtol=wl["rtol"],
+                                 krho_cap=wl["krho_cap"], rho_far=wl["rho_far"])
+
+    ctx = Context.get(0)
+    for _ in range(a.warmup):
+        step()
+    ctx.set_timing(True)
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    node_ms, node_n = ctx.kernel_time("node_solve")
+    lat_ms, _ = ctx.kernel_time("lateral_segments")
+    scan_ms, _ = ctx.kernel_time("cavity_response")
+    ctx.set_timing(False)
+    kr, wr, phis, wph, info = res["g"]
+    nk = info["n_krho_solves"]
+    nodes = nk * len(phis)
+    flops = W.BOX_KRHO_GRID_FLOPS_PER_NODE * nodes
+    node_step = node_ms / a.steps
+    cpu = cmp = None
+    if not a.no_cpu_baseline:
+        probes = W.box_grid_probes(wl)
+        ks = info["krho_visited"][:: max(1, nk // 192)]
+        v = probe_values(st, W.OMEGA, wl["source_layer"], ks, phis, wph, probes, wl["box"])
+        cpu, cmp = cpu_probes(st, dict(wl, box=wl["box"]), ks, phis, wph, probes, v)
+        cpu["sample"] = f"{len(ks)} of the {nk} k_rho the grid visits (every {max(1, nk // 192)}th), " \
+                        f"{len(phis)} phi x {len(probes)} probes each; the reference builds one one-node cache per k_rho"
+    out = _base(a, wl, nk * a.steps / wall, "k_rho probe sets/s (whole adaptive grid build, 22 box probes, 64 phi)",
+                wall, "f64")
+    out["config"].update({"phi": len(phis), "probes": len(info["probes"]), "rtol": wl["rtol"],
+                          "krho_cap": wl["krho_cap"]})
+    out.update({
+        "grid": {"n_panels": info["n_panels"], "n_nodes": info["n_nodes"], "n_krho_solves": nk,
+                 "n_batches": info["n_batches"], "stalled": info["any_stalled"], "tail_bound": info["tail_bound"]},
+        "ms_per_grid": wall * 1e3 / a.steps,
+        "kernel_ms": {"node_solve": node_step, "lateral_segments": lat_ms / a.steps,
+                      "cavity_response (pole scan)": scan_ms / a.steps},
+        "node_launches_per_step": node_n / a.steps,
+        "rel_err_vs_fp64_oracle": cmp,
+        "roofline": {"kernel": "node_solve (one launch per observation layer and refinement batch, box source, FP64)",
+                     "bound": "mfma", "achieved": flops / (node_step * 1e-3) / 1e12, "peak": PEAK_FP64 / 1e12,
+                     "unit": "TFLOP/s", "frac": flops / (node_step * 1e-3) / PEAK_FP64, "traffic": None,
+                     "flops_per_node": W.BOX_KRHO_GRID_FLOPS_PER_NODE, "nodes_per_step": nodes},
+        "cpu_baseline": cpu,
+    })
+    return out
+
+
 WORKLOADS = {"batched_gyro_512": bench_gyro, "point_uniaxial_1d": bench_point, "bessel_gemm": bench_gemm,
              "uneven_256": bench_uneven, "surrogate_1024": bench_surrogate, "nufft_1e6": bench_nufft,
              "operator_512": bench_operator, "operator_volume_64": bench_operator_volume,
-             "pole_scan_hyperbolic": bench_pole_scan, "probe_batch_box": bench_probes}
+             "pole_scan_hyperbolic": bench_pole_scan, "probe_batch_box": bench_probes,
+             "box_krho_grid": bench_box_grid}
 
 
 def main():

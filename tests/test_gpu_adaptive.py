@@ -1,0 +1,87 @@
+"""GPU test of the adaptive box k_rho grid (adaptive.box_krho_grid): the
+probes' integrals on the returned rule against a dense rule (every accepted
+panel split in four, G7-K15 on each), within 10 x rtol per probe -- the
+acceptance test of the reference's own validation
+(Gz_ClosedForm_BoxQuadrature_Validation.jl:18-27, box_kρ_grid at rtol vs a
+dense reference, no stall).  The probe values themselves are checked against
+the oracle in the batched-probe tests."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from helpers import lay
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+W0 = 2 * math.pi
+
+
+def glass_stack():
+    from yellowthinfilms_amd.materials import IsotropicMaterial
+    return [lay(IsotropicMaterial(1.0 + 0.001j), 1.0), lay(IsotropicMaterial(2.25 + 0.05j), 0.6),
+            lay(IsotropicMaterial(1.0 + 0.001j), 1.0)]
+
+
+def _check_against_dense_rule(st, src, box, kr, wr, phis, wph, info, rtol):
+    from yellowthinfilms_amd.adaptive import kronrod_reference_rule
+    from yellowthinfilms_amd.probes import probe_values
+    probes = [(p[0], p[1], p[2], p[3], p[4], (p[6], p[5])) for p in info["probes"]]
+    xi, wt = kronrod_reference_rule(7)
+    fine = [(a + (b - a) * j / 4, a + (b - a) * (j + 1) / 4) for a, b in info["panels"] for j in range(4)]
+    fa = np.array([p[0] for p in fine])
+    fb = np.array([p[1] for p in fine])
+    kd = (0.5 * (fa + fb)[:, None] + 0.5 * (fb - fa)[:, None] * xi[None, :]).ravel()
+    wd = (0.5 * (fb - fa)[:, None] * wt[None, :]).ravel()
+    Vd = probe_values(st, W0, src, kd, phis, wph, probes, box).reshape(kd.size, len(probes), 9)
+    V = probe_values(st, W0, src, kr, phis, wph, probes, box).reshape(kr.size, len(probes), 9)
+    for m in range(len(probes)):
+        ref = (wd[:, None] * Vd[:, m]).sum(axis=0)
+        got = (wr[:, None] * V[:, m]).sum(axis=0)
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert err < 10 * rtol, (probes[m][5], err)
+
+
+@pytest.mark.parametrize("rtol,cap", [(1e-3, 50.0), (1e-6, 200.0)])
+def test_box_krho_grid_against_dense_rule(rtol, cap):
+    print(f"case: box_krho_grid, glass stack, box 1/8, 8 phi, kρ_cap {cap}, rtol {rtol}", flush=True)
+    from yellowthinfilms_amd.adaptive import box_krho_grid
+    box = (1 / 8, 1 / 8, 1 / 8)
+    st = glass_stack()
+    t0 = time.perf_counter()
+    kr, wr, phis, wph, info = box_krho_grid(st, W0, 2, box, n_phi=8, rtol=rtol, krho_cap=cap, rho_far=0.25)
+    dt = time.perf_counter() - t0
+    print(f"  {info['n_panels']} panels, {info['n_krho_solves']} k_rho in {info['n_batches']} batches, "
+          f"{dt:.2f} s", flush=True)
+    assert not info["any_stalled"] and info["tail_bound"] == cap
+    assert kr.size == 15 * info["n_panels"] and np.all(np.diff(kr[::15]) > 0)
+    _check_against_dense_rule(st, 2, box, kr, wr, phis, wph, info, rtol)
+    if rtol < 1e-5:
+        assert info["n_batches"] > 1        # the refinement ran
+
+
+def test_box_krho_grid_grouped():
+    """A symmetric stack gives one group, the same grid as box_krho_grid; a
+    biaxial film gives its mirror classes, which partition the phi grid, each
+    with its own converged k_rho grid (checked like the shared grid)."""
+    print("case: box_krho_grid_grouped, glass (one group) and biaxial 4-layer (mirror classes), 8 phi", flush=True)
+    from helpers import FOUNDATION_STACKS
+    from yellowthinfilms_amd.adaptive import box_krho_grid, box_krho_grid_grouped
+    box = (1 / 8, 1 / 8, 1 / 8)
+    kw = dict(rtol=1e-3, krho_cap=50.0, rho_far=0.25)
+    st = glass_stack()
+    groups, info = box_krho_grid_grouped(st, W0, 2, box, n_phi=8, **kw)
+    kr, wr, _, _, _ = box_krho_grid(st, W0, 2, box, n_phi=8, **kw)
+    assert info["n_groups"] == 1 and np.array_equal(groups[0]["krho"], kr) and np.array_equal(groups[0]["wrho"], wr)
+    bi = FOUNDATION_STACKS["biaxial 4-layer"]
+    groups, info = box_krho_grid_grouped(bi, W0, 2, box, n_phi=8, **kw)
+    print(f"  biaxial: {info['n_groups']} groups, k_rho per group {info['nkrho']}", flush=True)
+    assert info["n_groups"] == 3 and not info["any_stalled"]
+    allphi = np.sort(np.concatenate([g["phi"] for g in groups]))
+    assert np.allclose(allphi, 2 * np.pi * np.arange(8) / 8)
+    assert abs(sum(g["wphi"].sum() for g in groups) - 2 * np.pi) < 1e-12
+    for g, gi in zip(groups, info["per_group"]):
+        _check_against_dense_rule(bi, 2, box, g["krho"], g["wrho"], g["phi"], g["wphi"], gi, 1e-3)

@@ -1,6 +1,7 @@
 """The algorithmic operation count the roofline uses is the oracle's
 reference-order count (op-counting real type) over sampled workload nodes."""
 import numpy as np
+import pytest
 
 from helpers import po, ostack
 from yellowthinfilms_amd import uniform_kxky_grid, workloads as W
@@ -57,11 +58,17 @@ def test_stack_op_count_scales_with_layers():
     assert abs((f(bi) - f(bi[:3])) / (f(bi[:5]) - f(bi[:2])) - 1.0) < 0.15
 
 
-def test_probe_batch_box_flops_per_node():
+@pytest.mark.parametrize("which", ["probe_batch_box", "box_krho_grid"])
+def test_probe_flops_per_node(which):
     import math
-    wl = W.PROBE_BATCH_BOX
-    kr, _, _, probes = W.probe_batch_args(wl)
-    st = ostack(W.probe_batch_stack(wl))
+    if which == "probe_batch_box":
+        wl, const = W.PROBE_BATCH_BOX, W.PROBE_BATCH_BOX_FLOPS_PER_NODE
+        kr, _, _, probes = W.probe_batch_args(wl)
+    else:
+        wl, const = W.BOX_KRHO_GRID, W.BOX_KRHO_GRID_FLOPS_PER_NODE
+        kr, probes = np.geomspace(1e-3, wl["krho_cap"], 4096), W.box_grid_probes(wl)
+        assert len(probes) == 22
+    st = ostack(W.probe_batch_stack())
     rng = np.random.default_rng(0)
     tot = []
     for _ in range(64):
@@ -73,4 +80,4 @@ def test_probe_batch_box_flops_per_node():
                                   pr[0][5][0], pr[0][1])
             f += sum(c["solve"][x] for x in ("add", "mul", "div")) + len(pr) * sum(c["ghat"][x] for x in ("add", "mul", "div"))
         tot.append(f)
-    assert abs(np.mean(tot) - W.PROBE_BATCH_BOX_FLOPS_PER_NODE) < 1.0
+    assert abs(np.mean(tot) - const) < 1.0

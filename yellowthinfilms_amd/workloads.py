@@ -319,6 +319,27 @@ def probe_batch_args(wl=PROBE_BATCH_BOX):
     return kr, phis, wph, probes
 
 
+BOX_KRHO_GRID = dict(
+    name="box_krho_grid",
+    description=("the adaptive box k_rho grid (box_kρ_grid, mod_ClosedFormGrid.jl:155-255) of a 1/8-wavelength voxel in "
+                 "the glass layer of vac 1.0 / glass 2.25+0.05i 0.6 / vac 1.0 (omega = 2 pi): the default box-centre "
+                 "depths (22 probes in 3 observation layers), 64-node phi grid, rtol 1e-6, k_rho cap 2000, rho_far "
+                 "0.25: light lines, the real-axis pole scan, the seeds, then the lock-step G7-K15 refinement of every "
+                 "probe, each step's new k_rho solved as one batch; complex FP64"),
+    box=(0.125, 0.125, 0.125), source_layer=2, n_phi=64, rtol=1e-6, krho_cap=2000.0, rho_far=0.25,
+)
+# Algorithmic FP64 operations per (k_rho, phi) node of box_krho_grid, counted
+# as for probe_batch_box over its 22 probes (mean over 64 seeded nodes,
+# k_rho log-spaced in [1e-3, 2000]).  Pinned by tests/test_workloads.py.
+BOX_KRHO_GRID_FLOPS_PER_NODE = 64763.125
+
+
+def box_grid_probes(wl=BOX_KRHO_GRID):
+    """The probes box_krho_grid builds for box_krho_grid's setup."""
+    from .probes import box_probes
+    return box_probes(probe_batch_stack(), wl["source_layer"], wl["box"], rho_far=wl["rho_far"])
+
+
 BATCHED_GYRO_512 = dict(
     name="batched_gyro_512",
     description=("5-layer stack with a 40 nm gyrotropic film (g=0.3, Im eps=0.02), uniform 512x512 kx,ky grid "
