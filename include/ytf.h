@@ -233,6 +233,17 @@ int ytf_spectral_apply(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, con
 int ytf_radial_synthesis(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
                          const double *ghat_dev, int n_rho, const double *rho_dev, double *R_dev, int mode);
 
+/* Per-node terms of ytf_radial_synthesis: T_dev[k][p][c + 3q][r] = node k's
+ * term of R_p(rho_r), which is R_p(rho_r) of a one-node cache (kx_k, wt_k).
+ * Replaces the per-k_x one-node caches of point_kx_grid's probes
+ * (cache_at / closed_form_pixel, mod_ClosedFormGrid.jl:407-411): with
+ * wt_k = 1 a pixel at (rho_r, 0) sees T[k] · moment.
+ *   kx_dev, wt_dev: n_nodes doubles; ghat_dev: [n_pairs][9][n_nodes] complex
+ *   (ytf_green_nodes output); rho_dev: n_rho radii;
+ *   T_dev: [n_nodes][n_pairs][9][n_rho] complex. */
+int ytf_radial_terms(ytf_ctx *ctx, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                     const double *ghat_dev, int n_rho, const double *rho_dev, double *T_dev);
+
 /* closed_form_G for a CavityTransferCache1D (mod_ClosedFormField.jl:435-487):
  * the radial synthesis at the distinct radii of the pixel grid, rotated to
  * every pixel's azimuth.  Depth pair p is the z row p.

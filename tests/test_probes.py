@@ -47,3 +47,24 @@ def test_box_probes_default_depths_layer_one_and_exterior_only():
     # delta = 0.5 min(dz = 0.2, thinnest layer 0.6) = 0.1
     assert [p[1] for p in got] == pytest.approx([0.7, 0.3, 0.1])
     assert got[2][4] == pytest.approx(1.0 - 0.5 - 0.1 + 0.1)
+
+
+def test_point_probes_and_check_depths():
+    """_point_probes / _check_depths (mod_ClosedFormGrid.jl:453-481) on an inner
+    and a top source layer: per depth, the in-layer points that fit (+-dist,
+    both edges), dist into each neighbour; moments x and z; rho 0 and rho_max."""
+    from helpers import FOUNDATION_STACKS
+    from yellowthinfilms_amd.probes import point_probes, check_depths
+    st = FOUNDATION_STACKS["uniaxial 4-layer"]          # d = 1.0, 0.8, 0.5, 1.0
+    pr = point_probes(st, 2, [0.5], 0.05, 0.3)
+    pts = sorted({(p[0], round(p[1], 12)) for p in pr})
+    assert pts == [(1, 0.95), (2, 1e-09), (2, 0.35), (2, 0.45), (2, round(0.8 - 1e-9, 12)), (3, 0.05)]   # eps = 1e-9 max(d, 1)
+    assert len(pr) == 6 * 2 * 2 and {p[3] for p in pr} == {0.0, 0.3} and all(p[2] == 0.4 for p in pr)
+    assert {p[4] for p in pr} == {(1.0, 0.0, 0.0), (0.0, 0.0, 1.0)}
+    # top layer: local z counts up from its bottom interface, no layer above; a depth near the edge drops +-dist
+    pr1 = point_probes(st, 1, [0.02], 0.05, 0.0)
+    assert sorted({(p[0], round(p[1], 12)) for p in pr1}) == [(1, 1e-09), (1, 0.93), (1, round(1 - 1e-9, 12)), (2, 0.05)]
+    assert len(pr1) == 4 * 2
+    assert check_depths(st, 2, [0.01, 0.5, 0.99], 0.1) == [0.4]
+    assert check_depths(st, 2, [0.01, 0.99], 0.1) == [0.1]
+    assert check_depths(st, 1, [0.05, 0.5], 0.1) == [0.5]

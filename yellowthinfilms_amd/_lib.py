@@ -82,6 +82,8 @@ _SIGS = {
                                       ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
                                         ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
+    "ytf_radial_terms": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
+                                    ct.c_int, ct.c_void_p, ct.c_void_p]),
     "ytf_closed_form_G_1d": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
                                         ct.POINTER(ct.c_double), ct.c_int, ct.POINTER(ct.c_double), ct.c_int,
                                         ct.c_void_p, ct.c_int]),

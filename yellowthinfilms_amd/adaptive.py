@@ -364,3 +364,73 @@ def box_krho_grid_grouped(stack, omega, source_layer, box, n_phi=64, classes_per
             "nkrho": [g["krho"].size for g in groups], "any_stalled": any(i["any_stalled"] for i in infos),
             "per_group": infos}
     return groups, info
+
+
+def kronrod_rule(panels, kr_order=7):
+    """kronrod_rule (mod_KxGrid.jl:313-318): the G7-K15 Kronrod rule on each
+    panel (the rule adaptive_kx_grid returns, rebuilt from its panels)."""
+    xi, wt = kronrod_reference_rule(kr_order)
+    a = np.array([float(p[0]) for p in panels])
+    b = np.array([float(p[1]) for p in panels])
+    h, c = 0.5 * (b - a), 0.5 * (a + b)
+    return (c[:, None] + h[:, None] * xi[None, :]).ravel(), (h[:, None] * wt[None, :]).ravel()
+
+
+def _rel_change(v, v2):
+    """_rel_change (mod_ClosedFormGrid.jl:489-494): per probe |v - v2| /
+    max(|v2|, 1e-6 x the largest |v2|), and its maximum."""
+    n2 = np.linalg.norm(v2, axis=1)
+    scale = float(n2.max()) if n2.size else 0.0
+    rel = np.linalg.norm(v - v2, axis=1) / np.maximum(n2, 1e-6 * scale)
+    return {"rel_change": float(rel.max()) if rel.size else 0.0, "per_probe": rel}
+
+
+def point_kx_grid(stack, omega, source_layer, rho_max=0.0, pz=None, z_fracs=(0.1, 0.3, 0.5, 0.7, 0.9), engine="auto",
+                  rtol=1e-6, atol=0.0, kr_order=7, maxevals=10_000, krho_cap=None, krho_cap_rel=1e5,
+                  target_decay=25.0, check=False, scan_poles=True, rule=None, h_min=None, device=0):
+    """point_kx_grid (mod_ClosedFormGrid.jl:381-424): the layer-level radial
+    rule of a point source in source_layer (the 1D path), reused for every
+    depth, moment and lateral offset.  The band is cut at target_decay /
+    (pz/2) (pz default thinnest_finite_layer / 20; h_min overrides pz/2).
+    Engine (choose_engine): "fixed" -> fixed_krho_rule on the features, no
+    probes; "adaptive" -> adaptive_kx_grid on the point probes
+    (probes.point_probes), every refinement step's new k_x one device batch
+    (probes.point_probe_values).  check: the probes on the rule and on the
+    rule with every panel halved, info["check"] = _rel_change of the two.
+    Returns (kx, wt, info)."""
+    from .kspace import kspace_features, adaptive_seeds, fixed_krho_rule, choose_engine
+    from .probes import thinnest_finite_layer, check_depths, point_probes, point_probe_values
+    from .quadrature import gl_rule, halve_panels
+    N = len(stack)
+    if not 1 <= source_layer < N:
+        raise ValueError(f"point_kx_grid: source_layer={source_layer} must be in 1:{N - 1}")
+    om = float(omega)
+    pzf = thinnest_finite_layer(stack) / 20 if pz is None else float(pz)
+    dist = pzf / 2 if h_min is None else float(h_min)
+    d = float(stack[source_layer - 1].d)
+    zf_probe = list(z_fracs) if h_min is None else [z / d for z in check_depths(stack, source_layer, z_fracs, dist)]
+    probes = point_probes(stack, source_layer, zf_probe, dist, float(rho_max))
+    feat = kspace_features(stack, om, source_layer, dist, rho_max=rho_max, dims=1, target_decay=target_decay,
+                           krho_cap=krho_cap, krho_cap_rel=krho_cap_rel, scan=scan_poles, device=device)
+    eng = choose_engine(stack, engine)
+    rule = dict(rule or {})
+    if eng == "fixed":
+        kx, wt, panels = fixed_krho_rule(feat, rtol=rtol, **rule)
+        info = {"engine": "fixed", "n_nodes": kx.size, "n_panels": len(panels), "panels": panels, "any_stalled": False}
+        refined = lambda: gl_rule(halve_panels(panels), order=rule.get("order", 8), singular=feat["singular_lines"])
+    else:
+        def F(ks):
+            return point_probe_values(stack, om, source_layer, ks, probes, device=device)
+        kx, wt, info = adaptive_kx_grid(F, len(probes), adaptive_seeds(feat, rtol=rtol), feat["Kcut"], rtol=float(rtol),
+                                        atol=float(atol), kr_order=kr_order, maxevals=maxevals)
+        info.update({"engine": "adaptive", "n_kx_solves": info["n_krho_evaluated"]})
+        refined = lambda: kronrod_rule(halve_panels(info["panels"]), kr_order=kr_order)
+    chk = None
+    if check:
+        v = point_probe_values(stack, om, source_layer, kx, probes, wt=wt, device=device).sum(axis=0)
+        k2, w2 = refined()
+        v2 = point_probe_values(stack, om, source_layer, k2, probes, wt=w2, device=device).sum(axis=0)
+        chk = _rel_change(v, v2)
+    info.update({"n_probes": len(probes), "tail_bound": feat["Kcut"], "z_fracs": list(z_fracs),
+                 "rho_max": rho_max, "features": feat, "check": chk})
+    return kx, wt, info

@@ -1443,6 +1443,27 @@ int ytf_radial_synthesis(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, cons
                                  rho_dev, (cd *)R_dev, mode);
 }
 
+int ytf_radial_terms(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
+                     const double *ghat_dev, int n_rho, const double *rho_dev, double *T_dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_radial_terms: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n_nodes < 0 || n_pairs < 0 || n_rho < 0)
+    return fail(c, YTF_EINVAL, "ytf_radial_terms: negative size (n_nodes=%lld, n_pairs=%d, n_rho=%d)",
+                (long long)n_nodes, n_pairs, n_rho);
+  if (n_nodes == 0 || n_pairs == 0 || n_rho == 0) return YTF_OK;
+  if (!kx_dev || !wt_dev || !ghat_dev || !rho_dev || !T_dev)
+    return fail(c, YTF_EINVAL, "ytf_radial_terms: NULL pointer");
+  if (n_nodes * n_rho > ((int64_t)1 << 31) * 255)
+    return fail(c, YTF_EINVAL, "ytf_radial_terms: n_nodes x n_rho too large");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipError_t e = timed(c, "radial_terms", [&] {
+    return ytf::launch_synth_terms(n_nodes, n_pairs, n_rho, kx_dev, wt_dev, (const cd *)ghat_dev, rho_dev, (cd *)T_dev,
+                                   c->stream);
+  });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_radial_terms: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
 int ytf_closed_form_G_1d(ytf_ctx *c, int64_t n_nodes, const double *kx_dev, const double *wt_dev, int n_pairs,
                          const double *ghat_dev, const double *xs, int nx, const double *ys, int ny, double *G_dev,
                          int mode) {

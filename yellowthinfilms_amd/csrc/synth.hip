@@ -80,6 +80,38 @@ __global__ void k_synth_coef(int64_t n, int np, const double *__restrict__ kx, c
   o[SC_D * n + k] = rmul(-hw, gxy + gyx);
 }
 
+// Per-node terms of the radial synthesis: T[k][p][c + 3q][r] = node k's
+// term of R_p(rho_r), i.e. R_p(rho_r) of a one-node cache (kx_k, wt_k).  One
+// thread per (node, radius), depth pair p = blockIdx.y.
+__global__ void k_synth_terms(int64_t n, int np, int nr, const double *__restrict__ kx, const double *__restrict__ wt,
+                              const cd *__restrict__ g, const double *__restrict__ rho, cd *__restrict__ T) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (e >= n * nr || p >= np) return;
+  const int64_t k = e / nr;
+  const int r = (int)(e - k * nr);
+  const cd *gp = g + (size_t)p * 9 * n;
+  const cd gxx = gp[0 * n + k], gyx = gp[1 * n + k], gzx = gp[2 * n + k];
+  const cd gxy = gp[3 * n + k], gyy = gp[4 * n + k], gzy = gp[5 * n + k];
+  const cd gxz = gp[6 * n + k], gyz = gp[7 * n + k], gzz = gp[8 * n + k];
+  const double wi = kx[k] * wt[k] * 0.15915494309189533577;
+  const double hw = 0.5 * wi;
+  double b0, b1, b2;
+  bessel012(kx[k] * rho[r], b0, b1, b2);
+  const cd a = rmul(b0 * hw, gxx + gyy), b = rmul(b0 * hw, gxy - gyx);
+  const cd c = rmul(-b2 * hw, gxx - gyy), d = rmul(-b2 * hw, gxy + gyx);
+  cd *o = T + ((size_t)k * np + p) * 9 * nr + r;
+  o[0 * nr] = a + c;                              // xx
+  o[1 * nr] = d - b;                              // yx
+  o[2 * nr] = times_i(rmul(b1 * wi, gzx));        // zx
+  o[3 * nr] = b + d;                              // xy
+  o[4 * nr] = a - c;                              // yy
+  o[5 * nr] = times_i(rmul(b1 * wi, gzy));        // zy
+  o[6 * nr] = times_i(rmul(b1 * wi, gxz));        // xz
+  o[7 * nr] = times_i(rmul(b1 * wi, gyz));        // yz
+  o[8 * nr] = rmul(b0 * wi, gzz);                 // zz
+}
+
 // FP64 path: a Bessel table of one batch of nodes, then a contraction per
 // node chunk into partials, a fixed-order sum over the chunks, and the nine
 // entries.
@@ -308,6 +340,15 @@ hipError_t launch_synth_fp64(int64_t n, int np, int nr, const SynthPlan &P, cons
   const int64_t m = (int64_t)np * SC_N * nr;
   hipLaunchKernelGGL(k_synth_chunksum, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, P.nchunks,
                      (const cd *)part, S);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_terms(int64_t n, int np, int nr, const double *kx, const double *wt, const cd *g,
+                              const double *rho, cd *T, hipStream_t s) {
+  const int64_t m = n * nr;
+  if (m == 0 || np == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_synth_terms, dim3((unsigned)((m + 255) / 256), (unsigned)np), dim3(256), 0, s, n, np, nr, kx, wt,
+                     g, rho, T);
   return hipGetLastError();
 }
 

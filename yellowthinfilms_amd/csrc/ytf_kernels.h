@@ -72,6 +72,8 @@ SynthPlan synth_fp64_plan(int64_t n, int np, int nr);
 // (nchunks x np x 9 x nr complex), their fixed-order sum S (np x 9 x nr).
 hipError_t launch_synth_fp64(int64_t n, int np, int nr, const SynthPlan &P, const double *kx, const double *rho,
                              const cd *coef, double *J, cd *part, cd *S, hipStream_t s);
+hipError_t launch_synth_terms(int64_t n, int np, int nr, const double *kx, const double *wt, const cd *g,
+                              const double *rho, cd *T, hipStream_t s);
 hipError_t launch_synth_reduce(int np, int nr, int nks, const cd *part, cd *R, hipStream_t s);
 hipError_t launch_synth_rotate(int nx, int ny, int np, int nr, const double *xs, const double *ys, const int *rmap,
                                const cd *R, cd *G, hipStream_t s);

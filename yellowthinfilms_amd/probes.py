@@ -134,3 +134,85 @@ def probe_values(stack, omega, source_layer, krho, phis, wphis, probes, box, dev
         v = vals.cpu().numpy().reshape(nk, len(idx), 3, 3)  # entry c + 3q -> [q][c]
         out[:, idx] = np.swapaxes(v, 2, 3)
     return out
+
+
+def check_depths(stack, source_layer, z_fracs, hmin):
+    """_check_depths (mod_ClosedFormGrid.jl:474-481): the source depths
+    z_frac d (clamped inside the layer) at least hmin from the layer's real
+    interfaces (layer 1: only its bottom one); if none is, hmin itself."""
+    d = float(stack[source_layer - 1].d)
+    dist = (lambda z: z) if source_layer == 1 else (lambda z: min(z, d - z))
+    zs = [min(max(zf * d, 1e-9 * d), d - 1e-9 * d) for zf in z_fracs]
+    ok = [z for z in zs if dist(z) >= hmin]
+    return ok if ok else [min(hmin, d)]
+
+
+def point_probes(stack, source_layer, z_fracs, dist, rho_max):
+    """_point_probes (mod_ClosedFormGrid.jl:453-472): (obs_layer, z_obs,
+    z_src, rho, moment) for each depth z_frac d: in the source layer dist
+    above and below the source and both layer edges, dist into each
+    neighbouring layer; moments x and z; rho = 0 and rho_max."""
+    d = float(stack[source_layer - 1].d)
+    eps = 1e-9 * max(d, 1.0)
+    probes = []
+    for zf in z_fracs:
+        z = min(max(zf * d, eps), d - eps)
+        zp = d - z if source_layer == 1 else z
+        pts = []
+        if zp + dist <= d:
+            pts.append((source_layer, zp + dist))
+        if zp - dist >= 0:
+            pts.append((source_layer, zp - dist))
+        pts += [(source_layer, eps), (source_layer, d - eps)]
+        if source_layer > 1:
+            pts.append((source_layer - 1, float(stack[source_layer - 2].d) - dist))
+        pts.append((source_layer + 1, min(dist, float(stack[source_layer].d))))
+        rhos = [0.0] if rho_max == 0.0 else [0.0, float(rho_max)]
+        for obs, zo in pts:
+            for mom in ((1.0, 0.0, 0.0), (0.0, 0.0, 1.0)):
+                for rho in rhos:
+                    probes.append((obs, zo, z, rho, mom))
+    return probes
+
+
+def point_probe_values(stack, omega, source_layer, kx, probes, wt=None, device=0):
+    """The point field of every probe, node by node: (n_kx, n_probes, 3)
+    complex, entry [k, m] = closed_form_pixel at probe m of the 1D cache of
+    node k alone (weight wt[k], default 1) -- the probe integrands of
+    point_kx_grid (cache_at, mod_ClosedFormGrid.jl:407-411) for a whole batch
+    of k_x.  Summed over k with the rule's weights it is the probes' field on
+    that rule (_probe_values_1d, :483-487).  One ytf_green_nodes launch per
+    observation layer (its distinct (z_src, z_obs) pairs), then
+    ytf_radial_terms at the probes' radii."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    kxh = np.ascontiguousarray(np.atleast_1d(np.asarray(kx, np.float64)))
+    n = kxh.size
+    out = np.zeros((n, len(probes), 3), np.complex128)
+    if n == 0 or not probes:
+        return out
+    wth = np.ones(n) if wt is None else np.ascontiguousarray(np.asarray(wt, np.float64))
+    if wth.shape != (n,):
+        raise ValueError("point_probe_values: wt must have one weight per k_x")
+    kx_d = torch.as_tensor(kxh, device=dev)
+    ky_d = torch.zeros(n, dtype=torch.float64, device=dev)
+    wt_d = torch.as_tensor(wth, device=dev)
+    ctx = _lib.Context.get(device)
+    for obs in sorted({p[0] for p in probes}):
+        idx = [m for m, p in enumerate(probes) if p[0] == obs]
+        pairs = sorted({(probes[m][2], probes[m][1]) for m in idx})
+        rhos = sorted({probes[m][3] for m in idx})
+        _, g = green_nodes(stack, omega, kx_d, ky_d, source_layer, obs, [p[0] for p in pairs], [p[1] for p in pairs],
+                           cache=False, ghat=True, device=device)
+        rho_d = torch.as_tensor(np.asarray(rhos, np.float64), device=dev)
+        T = torch.empty((n, len(pairs), 9, len(rhos)), dtype=torch.complex128, device=dev)
+        ctx.bind_stream()
+        _lib.check(_lib.lib().ytf_radial_terms(ctx.handle, n, ct.c_void_p(kx_d.data_ptr()), ct.c_void_p(wt_d.data_ptr()),
+                                               len(pairs), ct.c_void_p(g.data_ptr()), len(rhos),
+                                               ct.c_void_p(rho_d.data_ptr()), ct.c_void_p(T.data_ptr())), ctx.handle)
+        Th = T.cpu().numpy().reshape(n, len(pairs), 3, 3, len(rhos))       # [k, pair, q, c, r] (entry c + 3q)
+        for m in idx:
+            j = pairs.index((probes[m][2], probes[m][1]))
+            r = rhos.index(probes[m][3])
+            out[:, m] = np.einsum("kqc,q->kc", Th[:, j, :, :, r], np.asarray(probes[m][4], np.complex128))
+    return out
