@@ -85,3 +85,33 @@ def test_box_krho_grid_grouped():
     assert abs(sum(g["wphi"].sum() for g in groups) - 2 * np.pi) < 1e-12
     for g, gi in zip(groups, info["per_group"]):
         _check_against_dense_rule(bi, 2, box, g["krho"], g["wrho"], g["phi"], g["wphi"], gi, 1e-3)
+
+
+def test_point_krho_grid_2d():
+    """point_kρ_grid_2d: the feature rule and phi count on a symmetric and a
+    gyrotropic stack, with the check (every panel halved; 2 N_phi) evaluated
+    over chunks of the 2D cache -- chunking must not change the sum.  The
+    symmetric stack's phi check is exact (its G does not depend on phi beyond
+    the rotation the trapezoid rule integrates); 32 phi under-resolve the
+    gyrotropic film at rho 0.1, which the phi check shows (it is only required
+    to be finite)."""
+    print("case: point_krho_grid_2d, iso 5-layer and gyrotropic 4-layer, rho_max 0.1", flush=True)
+    from helpers import FOUNDATION_STACKS
+    from yellowthinfilms_amd.adaptive import point_krho_grid_2d, chunked_2d
+    from yellowthinfilms_amd.green_operator import closed_form_G_2d
+    for label, n_phi in (("iso 5-layer lossy", None), ("gyrotropic 4-layer", 32)):
+        st = FOUNDATION_STACKS[label]
+        t0 = time.perf_counter()
+        kr, wr, ph, wph, info = point_krho_grid_2d(st, W0, 2, 0.1, pz=0.1, n_phi=n_phi, z_fracs=(0.5,), check=True)
+        print(f"  {label}: {info['n_krho']} k_rho x {info['N_phi']} phi, check k_rho "
+              f"{info['check']['krho']['rel_change']:.2e}, phi {info['check']['phi']['rel_change']:.2e}, "
+              f"{time.perf_counter() - t0:.2f} s", flush=True)
+        assert info["n_nodes"] == kr.size * ph.size and ph.size == info["N_phi"] and info["N_phi"] % 2 == 0
+        assert info["check"]["krho"]["rel_change"] < 1e-4 and np.isfinite(info["check"]["phi"]["rel_change"])
+        if n_phi is None:
+            assert info["N_phi"] >= 64
+    st = FOUNDATION_STACKS["gyrotropic 4-layer"]
+    f = lambda c: closed_form_G_2d(c, 2, 0.3, [0.35], [0.05], [0.02])[:, :, 0, 0, 0].cpu().numpy()
+    whole = chunked_2d(f, st, W0, 2, kr, wr, ph, wph, obs_layers=[2])
+    parts = chunked_2d(f, st, W0, 2, kr, wr, ph, wph, obs_layers=[2], max_nodes=7 * ph.size)
+    assert np.abs(whole - parts).max() <= 1e-12 * np.abs(whole).max()

@@ -434,3 +434,90 @@ def point_kx_grid(stack, omega, source_layer, rho_max=0.0, pz=None, z_fracs=(0.1
     info.update({"n_probes": len(probes), "tail_bound": feat["Kcut"], "z_fracs": list(z_fracs),
                  "rho_max": rho_max, "features": feat, "check": chk})
     return kx, wt, info
+
+
+def chunked_2d(f, stack, omega, source_layer, krho, wrho, phi, wphi, obs_layers=None, max_nodes=1_500_000, device=0):
+    """chunked_2d (mod_ClosedFormGrid.jl:457-467): the sum over consecutive
+    chunks of the k_rho rule (at most max_nodes k_rho x phi nodes each) of
+    f(cache) on each chunk's 2D cache -- exact for anything linear in the
+    quadrature sum, with one chunk's nodes resident at a time."""
+    from .green_operator import CavityTransfer2D
+    kr = np.asarray(krho, np.float64)
+    wr = np.asarray(wrho, np.float64)
+    step = max(1, max_nodes // max(1, len(phi)))
+    acc = None
+    for i in range(0, kr.size, step):
+        c = CavityTransfer2D(stack, omega, source_layer, kr[i:i + step], wr[i:i + step], phi, wphi,
+                             obs_layers=obs_layers, device=device)
+        v = f(c)
+        acc = v if acc is None else acc + v
+        del c
+    return acc
+
+
+def point_krho_grid_2d(stack, omega, source_layer, rho_max, pz=None, z_fracs=(0.1, 0.5, 0.9), n_phi=None, rtol=1e-6,
+                       krho_cap=None, krho_cap_rel=1e5, target_decay=25.0, c_phi=2.5, n_phi_scan=32, check=False,
+                       scan_poles=True, rule=None, h_min=None, terms="all", max_nodes=1_500_000, device=0):
+    """point_kρ_grid_2d (mod_ClosedFormGrid.jl:536-585): the (k_rho, phi)
+    tensor grid of a point source in any stack: fixed_krho_rule on the 2D
+    features (light lines at every scanned phi, the poles along their tracks,
+    cut-off target_decay / h_min) and a uniform phi rule of n_phi nodes
+    (default the features' N_phi).  check: closed_form_G at probe points
+    (per depth: +-dist in the layer, dist into each neighbour; the source
+    plane too for terms="scattered"; lateral offsets 0 and, with rho_max > 0,
+    (+-rho_max, 0), (0, rho_max)) on the rule, on the rule with every panel
+    halved (info["check"]["krho"]) and with 2 n_phi nodes (the mean of the
+    rule and the rule shifted by pi / n_phi: info["check"]["phi"]), each
+    summed over chunks of at most max_nodes nodes.  Returns (krho, wrho, phi,
+    wphi, info)."""
+    from .green_operator import closed_form_G_2d
+    from .kspace import kspace_features, fixed_krho_rule
+    from .probes import thinnest_finite_layer, check_depths
+    from .quadrature import gl_rule, halve_panels
+    N = len(stack)
+    if not 1 <= source_layer < N:
+        raise ValueError(f"point_kρ_grid_2d: source_layer={source_layer} must be in 1:{N - 1}")
+    om = float(omega)
+    pzf = thinnest_finite_layer(stack) / 20 if pz is None else float(pz)
+    dist = pzf / 2 if h_min is None else float(h_min)
+    feat = kspace_features(stack, om, source_layer, dist, rho_max=rho_max, dims=2, n_phi_scan=n_phi_scan,
+                           target_decay=target_decay, krho_cap=krho_cap, krho_cap_rel=krho_cap_rel, scan=scan_poles,
+                           c_phi=c_phi, rtol=rtol, device=device)
+    rule = dict(rule or {})
+    kr, wr, panels = fixed_krho_rule(feat, rtol=rtol, **rule)
+    nphi = feat["N_phi"] if n_phi is None else int(n_phi)
+    phis, wph = build_kphi_grid(nphi)
+    chk = None
+    if check:
+        d = float(stack[source_layer - 1].d)
+        zs = check_depths(stack, source_layer, z_fracs, dist / 2)
+        pts = []                                           # (obs, z_obs in the obs layer's frame, z_src local)
+        for z in zs:
+            zp = d - z if source_layer == 1 else z
+            if zp + dist <= d:
+                pts.append((source_layer, zp + dist, z))
+            if zp - dist >= 0:
+                pts.append((source_layer, zp - dist, z))
+            if source_layer > 1:
+                pts.append((source_layer - 1, float(stack[source_layer - 2].d) - dist, z))
+            pts.append((source_layer + 1, min(dist, float(stack[source_layer].d)), z))
+        if terms == "scattered":
+            for z in zs:
+                pts.append((source_layer, d - z if source_layer == 1 else z, z))
+        lat = [(0.0, 0.0), (rho_max, 0.0), (-rho_max, 0.0), (0.0, rho_max)] if rho_max > 0 else [(0.0, 0.0)]
+        obs_layers = sorted({p[0] for p in pts})
+
+        def f(c):
+            return np.stack([closed_form_G_2d(c, o, z, [zo], [x], [y], terms=(terms if o == source_layer else "all"))
+                             [:, :, 0, 0, 0].cpu().numpy().T.ravel() for (o, zo, z) in pts for (x, y) in lat])
+
+        def run(kn, kw, phv):
+            return chunked_2d(f, stack, om, source_layer, kn, kw, phv, wph, obs_layers=obs_layers,
+                              max_nodes=max_nodes, device=device)
+        v = run(kr, wr, phis)
+        k2, w2 = gl_rule(halve_panels(panels), order=rule.get("order", 8), singular=feat["singular_lines"])
+        v2phi = 0.5 * (v + run(kr, wr, phis + math.pi / nphi))
+        chk = {"krho": _rel_change(v, run(k2, w2, phis)), "phi": _rel_change(v, v2phi)}
+    info = {"engine": "fixed", "n_krho": kr.size, "N_phi": nphi, "n_nodes": kr.size * nphi, "n_panels": len(panels),
+            "panels": panels, "tail_bound": feat["Kcut"], "features": feat, "check": chk}
+    return kr, wr, phis, wph, info
