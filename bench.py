@@ -1108,7 +1108,7 @@ def bench_box_grid(a):
     ctx.set_timing(False)
     kr, wr, phis, wph, info = res["g"]
     nk = info["n_krho_solves"]
-    nodes = nk * len(phis)
+    nodes = info["n_krho_evaluated"] * len(phis)          # every solved node, the speculative ones included
     flops = W.BOX_KRHO_GRID_FLOPS_PER_NODE * nodes
     node_step = node_ms / a.steps
     cpu = cmp = None
@@ -1125,7 +1125,8 @@ def bench_box_grid(a):
                           "krho_cap": wl["krho_cap"]})
     out.update({
         "grid": {"n_panels": info["n_panels"], "n_nodes": info["n_nodes"], "n_krho_solves": nk,
-                 "n_batches": info["n_batches"], "stalled": info["any_stalled"], "tail_bound": info["tail_bound"]},
+                 "n_krho_evaluated": info["n_krho_evaluated"], "n_batches": info["n_batches"],
+                 "n_probe_calls": info["n_calls"], "stalled": info["any_stalled"], "tail_bound": info["tail_bound"]},
         "ms_per_grid": wall * 1e3 / a.steps,
         "kernel_ms": {"node_solve": node_step, "lateral_segments": lat_ms / a.steps,
                       "cavity_response (pole scan)": scan_ms / a.steps},

@@ -84,8 +84,14 @@ def test_lockstep_refinement_closed_forms(rtol):
         own_bps = {a for a, _ in own["panels"]} | {own["panels"][-1][1]}
         assert own_bps <= bps
         assert own["per_probe"][0]["E"] == info["per_probe"][p]["E"]
-    # the memo evaluates each k_rho once: distinct points = 15 per segment ever evaluated (no repeats)
-    assert info["n_krho_evaluated"] <= 15 * (4 + 2 * (info["n_batches"] - 1) * 3)
+    # the rule evaluations read at most 15 points per segment ever evaluated
+    assert info["n_krho_used"] <= 15 * (4 + 2 * (info["n_batches"] - 1) * 3)
+    # speculation changes only which calls evaluate the points: the same rule, fewer calls
+    n2, w2, plain = adaptive_kx_grid(F, 3, [1.0, 2.0, 30.0, 55.0], K, rtol=rtol, speculate=False)
+    assert np.array_equal(n2, nodes) and np.array_equal(w2, w)
+    assert [q["E"] for q in plain["per_probe"]] == [q["E"] for q in info["per_probe"]]
+    assert plain["n_calls"] <= plain["n_batches"]
+    assert info["n_calls"] <= (plain["n_calls"] + 1) // 2 + 1 and info["n_krho_used"] == plain["n_krho_used"]
 
 
 def test_stall_and_errors():
@@ -93,7 +99,7 @@ def test_stall_and_errors():
     _, _, info = adaptive_kx_grid(F, 1, [], 10.0, rtol=1e-10, maxevals=300)
     assert info["any_stalled"] and info["per_probe"][0]["stalled"]
     # at most maxevals plus one split's points
-    assert info["n_krho_evaluated"] <= 300 + 30
+    assert info["n_krho_used"] <= 300 + 30
     with pytest.raises(ValueError):
         adaptive_kx_grid(F, 0, [], 10.0)
     with pytest.raises(ValueError):

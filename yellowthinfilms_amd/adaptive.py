@@ -130,18 +130,25 @@ class _Memo:
             self.n += len(new)
         return len(new)
 
+    def has(self, ks):
+        return all(k in self.row for k in np.asarray(ks, np.float64).ravel().tolist())
+
     def rows(self, ks):
         ks = np.asarray(ks, np.float64)
         return np.fromiter((self.row[k] for k in ks.ravel().tolist()), np.int64, count=ks.size).reshape(ks.shape)
 
 
-def adaptive_kx_grid(F, n_probes, kbreaks, kcut, rtol=1e-6, atol=0.0, kr_order=7, maxevals=10_000):
+def adaptive_kx_grid(F, n_probes, kbreaks, kcut, rtol=1e-6, atol=0.0, kr_order=7, maxevals=10_000, speculate=True):
     """adaptive_kx_grid (mod_KxGrid.jl:272-305) with batched probes.
     F(k) -> (len(k), n_probes, ...) values of every probe integrand at the
     k_rho values k.  Returns (nodes, weights, info) with info: n_panels,
     n_nodes, per_probe ({E, stalled} per probe), any_stalled,
     min_panel_width, kcut, panels, n_krho_evaluated, krho_visited (the
-    distinct k_rho evaluated, sorted), n_batches."""
+    distinct k_rho evaluated, sorted), n_krho_used (those the rule
+    evaluations read), n_batches (refinement steps + 1), n_calls (calls of
+    F).  speculate: each call of F also takes the points of every split a
+    probe can make next (see the loop), which halves the calls and leaves
+    the refinement unchanged."""
     if n_probes < 1:
         raise ValueError("adaptive_kx_grid: fs cannot be empty")
     kc = float(kcut)
@@ -155,7 +162,8 @@ def adaptive_kx_grid(F, n_probes, kbreaks, kcut, rtol=1e-6, atol=0.0, kr_order=7
     X = _points_many(A, B)
     memo = _Memo(F, P)
     memo.fill(X)
-    n_batches = 1
+    n_batches = n_calls = 1
+    used = set(X.ravel().tolist())
     V = memo.store[memo.rows(X)]                                  # (S, 15, P, D)
     D = V.shape[-1]
     I0, E0 = _evalrule_many(np.tile(A, P), np.tile(B, P), V.transpose(2, 0, 1, 3).reshape(P * S, 15, D))
@@ -184,7 +192,24 @@ def adaptive_kx_grid(F, n_probes, kbreaks, kcut, rtol=1e-6, atol=0.0, kr_order=7
         a2 = np.array([(t[2][0], t[3]) for t in todo]).ravel()        # left and right halves, interleaved
         b2 = np.array([(t[3], t[2][1]) for t in todo]).ravel()
         X = _points_many(a2, b2)
-        memo.fill(X)
+        if not memo.has(X):
+            # Speculation: a probe's next split is one of the two new halves or its heap top, so
+            # their halves' points go into the same batch (memo entries only: the splits made
+            # are unchanged, and every other step then needs no device call).
+            sa, sb = [], []
+            for _, r, sg, mid in todo:
+                for a, b in ((sg[0], mid), (mid, sg[1])):
+                    m = (a + b) / 2
+                    sa += [a, m]
+                    sb += [m, b]
+                if r["heap"]:
+                    t = r["heap"][0][2]
+                    m = (t[0] + t[1]) / 2
+                    sa += [t[0], m]
+                    sb += [m, t[1]]
+            memo.fill(np.concatenate([X.ravel(), _points_many(sa, sb).ravel()]) if speculate else X)
+            n_calls += 1
+        used.update(X.ravel().tolist())
         n_batches += 1
         pidx = np.repeat(np.array([t[0] for t in todo]), 2)
         I2, E2 = _evalrule_many(a2, b2, memo.store[memo.rows(X), pidx[:, None]])
@@ -217,7 +242,8 @@ def adaptive_kx_grid(F, n_probes, kbreaks, kcut, rtol=1e-6, atol=0.0, kr_order=7
     info = {"n_panels": len(panels), "n_nodes": nodes.size, "per_probe": per_probe,
             "any_stalled": any(q["stalled"] for q in per_probe),
             "min_panel_width": float((b - a).min()) if panels else math.inf, "kcut": kc, "panels": panels,
-            "n_krho_evaluated": len(memo.row), "krho_visited": np.array(sorted(memo.row)), "n_batches": n_batches}
+            "n_krho_evaluated": len(memo.row), "krho_visited": np.array(sorted(memo.row)), "n_batches": n_batches,
+            "n_krho_used": len(used), "n_calls": n_calls}
     return nodes, weights, info
 
 
@@ -292,7 +318,7 @@ def box_krho_grid(stack, omega, source_layer, box, zbar=None, n_phi=64, phi=None
     kr, wr, info = adaptive_kx_grid(F, len(probes), seeds, Kcut, rtol=float(rtol), atol=float(atol),
                                     kr_order=kr_order, maxevals=maxevals)
     info.update({"probes": [(p[0], p[1], p[2], p[3], p[4], p[5][1], p[5][0]) for p in probes], "h_min": h_min,
-                 "tail_bound": Kcut, "n_krho_solves": info["n_krho_evaluated"], "features": feat})
+                 "tail_bound": Kcut, "n_krho_solves": info["n_krho_used"], "features": feat})
     stalled = [i for i, q in enumerate(info["per_probe"]) if q["stalled"]]
     if on_stall == "error" and stalled:
         lst = "\n".join(f"  layer {probes[i][0]}, z̄ = {probes[i][5][0]}, {probes[i][5][1]} (error estimate "
@@ -423,7 +449,7 @@ def point_kx_grid(stack, omega, source_layer, rho_max=0.0, pz=None, z_fracs=(0.1
             return point_probe_values(stack, om, source_layer, ks, probes, device=device)
         kx, wt, info = adaptive_kx_grid(F, len(probes), adaptive_seeds(feat, rtol=rtol), feat["Kcut"], rtol=float(rtol),
                                         atol=float(atol), kr_order=kr_order, maxevals=maxevals)
-        info.update({"engine": "adaptive", "n_kx_solves": info["n_krho_evaluated"]})
+        info.update({"engine": "adaptive", "n_kx_solves": info["n_krho_used"]})
         refined = lambda: kronrod_rule(halve_panels(info["panels"]), kr_order=kr_order)
     chk = None
     if check:
