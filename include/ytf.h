@@ -288,7 +288,7 @@ int ytf_batch_tune(ytf_ctx *ctx, ytf_batch *batch, const double *kx_dev, const d
 
 /* Learned surrogate of the spectral dyadic (no counterpart in the reference,
  * which evaluates every node with the exact solver; that solve remains the
- * fallback and the judge).  A fully connected network n_in -> width x depth
+ * fallback and the arbiter of accuracy).  A fully connected network n_in -> width x depth
  * (SiLU) -> n_out, trained and evaluated on the matrix cores in BF16 with FP32
  * accumulation (FP32 master weights, Adam).  Inputs and targets are raw FP32
  * rows (x[n][n_in], y[n][n_out], device); the network sees

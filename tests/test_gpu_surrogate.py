@@ -1,7 +1,7 @@
 """The surrogate network on the GPU (ytf_mlp_*) through the C-ABI: its loss
 and gradients against the CPU reference (bf16 rounding points reproduced),
 training convergence, and the surrogate of the scattered spectral dyadic with
-the exact FP64 solve as fallback and judge.
+the exact FP64 solve as fallback and arbiter of accuracy.
 
 Tolerances: loss <= 1e-4 relative, per-layer gradients <= 2e-3 relative
 (FP32 accumulation order and the odd bf16 rounding-boundary flip against the

@@ -2,7 +2,7 @@
 // network (SiLU hidden layers) trained and evaluated on the gfx950 matrix
 // cores in BF16 with FP32 accumulation.  No reference counterpart (the
 // reference computes every node with the exact solver, which here stays the
-// fallback and the judge of the surrogate).
+// fallback and the check on the surrogate).
 //
 // Every layer product is one kernel, k_gemm: C(m, n) = sum_k A(m, k) B(n, k)
 // on v_mfma_f32_32x32x16_bf16, with either operand stored k-fastest or

@@ -1,7 +1,7 @@
 """Learned surrogate of the spectral dyadic, with the exact solve as fallback
 (host side).  No reference counterpart: the reference evaluates every node
 with the exact Berreman + S-matrix solve (mod_CavityTransfer.jl,
-mod_ClosedFormField.jl), which here generates the training data, judges the
+mod_ClosedFormField.jl), which here generates the training data, checks the
 surrogate, and replaces it on flagged nodes.
 
   MLP             -- the network (ytf_mlp_*): BF16 matrix-core training and
