@@ -8,6 +8,7 @@ column-major interleaved layout the C side reads), d (n,), ek (n,) eps kind
 (0 isotropic, 1 uniaxial, 2 biaxial, 3 general 3x3), mk (n,) mu kind (0 iso).
 Layer indices are 1-based, as in the reference.
 """
+import contextlib
 import ctypes as ct
 import os
 
@@ -28,6 +29,19 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
         _LIB = ct.CDLL(path)
     return _LIB
+
+
+@contextlib.contextmanager
+def using(path):
+    """Run the block against another build of this same source (a variant with
+    one planted mistake, oracle/_mutants/); the default build is restored after."""
+    global _LIB
+    saved = _LIB
+    _LIB = ct.CDLL(path)  # RTLD_LOCAL: its symbols do not interpose on the default build's
+    try:
+        yield
+    finally:
+        _LIB = saved
 
 
 def _p(a, typ=_d):

@@ -18,7 +18,23 @@
 //     (Gz_ClosedForm_Foundation_Validation.jl:38-76); Modal_amp restated here;
 //   * spectral dyadic + 1D Bessel synthesis: free-space dipole closed form
 //     (Gz_ClosedForm_Point_Validation.jl:164-167), reciprocity, rotation
-//     covariance, 1D = 2D-direct-sum on an azimuthally symmetric stack.
+//     covariance, 1D = 2D-direct-sum on an azimuthally symmetric stack;
+//   * anisotropic layers: bulk-Maxwell null vectors of every Berreman
+//     eigenpair, the gyrotropic normal-incidence modes (handedness fixed), the
+//     seed's particular solution against a bulk 3x3 Maxwell solve, the layer-1
+//     dyadic against the angular-spectrum closed form with TE/TM Airy
+//     reflections, the 3-layer semi-analytical model
+//     (mod_Dipz_SemiAnalytical.jl) and the reference's logged s/p reflection
+//     norms of its hyperbolic stack (tests/closed_forms.py).
+//
+// ORACLE_MUTANT (default 0) plants one known mistake; tests/test_oracle_mutants.py
+// builds each variant on the CPU and shows that the checks above reject it:
+//   1  eps_12 and eps_21 exchanged in the Berreman matrix (eps -> eps^T, g -> -g)
+//   2  the eps_tz term of the point-source seed dropped
+//   3  eps_t and eps_z exchanged in the uniaxial extraordinary k_z
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -340,7 +356,11 @@ template <class R> struct Modes {
 // _berreman_core(eps, mu, omega, k) -- mod_ModeSolver.jl:122-193
 template <class R> Mat<R, 4, 4> berreman_core(const Layer<R> &L, R omega, R kx, R ky) {
   typedef cx<R> T;
+#if ORACLE_MUTANT == 1
+  T e11 = L.e(0, 0), e12 = L.e(1, 0), e21 = L.e(0, 1), e22 = L.e(1, 1);
+#else
   T e11 = L.e(0, 0), e12 = L.e(0, 1), e21 = L.e(1, 0), e22 = L.e(1, 1);
+#endif
   T e13 = L.e(0, 2), e23 = L.e(1, 2), e31 = L.e(2, 0), e32 = L.e(2, 1), e33 = L.e(2, 2);
   T m11 = L.m(0, 0), m12 = L.m(0, 1), m21 = L.m(1, 0), m22 = L.m(1, 1);
   T m13 = L.m(0, 2), m23 = L.m(1, 2), m31 = L.m(2, 0), m32 = L.m(2, 1), m33 = L.m(2, 2);
@@ -593,7 +613,11 @@ template <class R> void eigen_closed(const Layer<R> &L, R omega, R kx, R ky, cx<
   R cp = r_cos(phi), sp = r_sin(phi);
   T kr2 = Cr<R>(kr * kr);
   T kzo = csqrt(wc * wc * et * mr - kr2);
+#if ORACLE_MUTANT == 3
+  T kze = csqrt(ez * (wc * wc * mr - kr2 / et));
+#else
   T kze = csqrt(et * (wc * wc * mr - kr2 / ez));
+#endif
   T A = C<R>(1.0) / (wc * mr);
   T Bn = C<R>(1.0) / (wc * et);
   T c = Cr(cp), s = Cr(sp);
@@ -789,7 +813,12 @@ template <class R> bool source_projection(R kx, R ky, R omega, const Layer<R> &L
   for (int q = 0; q < 3; ++q) { J(0, q) = k1 * jz[q] / den; J(1, q) = k2 * jz[q] / den; }
   // J_Ht = R2s * Jt_cols - (R2s * etz) * transpose(Jz_row) / ezz ; R2s = [0 1; -1 0]
   T jt[2][3] = {{mio, rmul(R(0.0), mio), rmul(R(0.0), mio)}, {rmul(R(0.0), mio), mio, rmul(R(0.0), mio)}};
+#if ORACLE_MUTANT == 2
+  T r1 = zero, r2 = zero;
+  (void)etz1; (void)etz2;
+#else
   T r1 = etz2, r2 = -etz1;  // R2s * etz
+#endif
   for (int q = 0; q < 3; ++q) {
     T a = jt[1][q], b = -jt[0][q];  // R2s * Jt_cols
     J(2, q) = a - r1 * jz[q] / ezz;

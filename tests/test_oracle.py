@@ -214,29 +214,8 @@ def test_foundation_transfer_identity(label):
     assert worst_same < 1e-11, worst_same
 
 
-def test_projection_gives_the_interior_particular_solution():
-    # (sf, sb) encode the jump of the Berreman state vector across a point
-    # source: V_f sf - V_b sib_raw... checked as W'V s = W'J (the defining
-    # 4x4 system of Source_projection_Berreman_xyz) solved independently.
-    stack = c2_stack()
-    st = ostack(stack)
-    for kx, ky in [(0.7, 0.1), (9.0, -4.0)]:
-        for src in (1, 3):
-            c = po.nodes_batch(st, W0, [kx], [ky], src, [src])
-            M = po.stack_smatrix(st, W0, kx, ky)[0][src - 1]
-            V = np.hstack([M["Vf"], M["Vb"]])
-            W = np.hstack([M["Wf"], M["Wb"]])
-            eps = stack[src - 1].eps.eps
-            mio = -1j * W0
-            Jt = mio * np.array([[1, 0, 0], [0, 1, 0]])
-            Jz = mio * np.array([0, 0, 1])
-            JEt = np.outer([kx, ky], Jz) / (eps[2, 2] * W0)
-            R2 = np.array([[0, 1], [-1, 0]])
-            JHt = R2 @ Jt - np.outer(R2 @ eps[:2, 2], Jz) / eps[2, 2]
-            J = np.vstack([JEt, JHt])
-            s = np.linalg.solve(W.conj().T @ V, W.conj().T @ J)
-            assert relerr(c.sf[0], s[:2]) < 1e-12
-            assert relerr(c.sb[0], -s[2:]) < 1e-12
+# The seed's particular solution against a bulk Maxwell solve (which pins the
+# eps_tz term of the source vector) is in tests/test_oracle_closed_forms.py.
 
 
 # ---------------------------------------------------------------------------
