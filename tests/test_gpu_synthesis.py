@@ -231,6 +231,65 @@ def test_f16x3_gemm_workload_scaled_down():
     assert err <= bound
 
 
+def _oracle_R(stack, kx, wt, zs, zo, rho):
+    """The oracle's closed_form_G (mod_ClosedFormField.jl:435-487) at pixels
+    (rho_r, 0) for every depth pair, in R's layout (np, 9 = c + 3q, nr)."""
+    c = po.nodes_batch(ostack(stack), W0, kx, np.zeros_like(kx), 1, [1])
+    out = np.zeros((len(zs), 9, len(rho)), complex)
+    for s in np.unique(zs):
+        idx = np.nonzero(zs == s)[0]
+        G = po.closed_form_G_1d(c, 1, wt, float(s), zo[idx], rho, [0.0])  # (nr, 1, nz, 3, 3)
+        out[idx] = G[:, 0].transpose(1, 3, 2, 0).reshape(len(idx), 9, len(rho))
+    return out
+
+
+def test_f16x3_point_uniaxial_workload_against_the_oracle():
+    """The matrix-core path on the whole point_uniaxial_1d configuration (4608
+    nodes, 256 radii, 8 heights) against the oracle's FP64 closed_form_G: the
+    difference stays within the split-fp16 rounding bound plus the FP64 path's
+    own 1e-10 bar."""
+    print("case: f16x3 point_uniaxial_1d vs oracle", flush=True)
+    from yellowthinfilms_amd import CavityTransfer1D, radial_synthesis, workloads as W
+    wl = W.POINT_UNIAXIAL_1D
+    kx, wt = W.point_uniaxial_rule()
+    zo, rho = W.point_uniaxial_geometry()
+    zs = np.full(len(zo), wl["zsrc"])
+    stack = W.point_uniaxial_stack()
+    cache = CavityTransfer1D(stack, W0, kx, wt, 1)
+    g = cache.ghat(1, wl["zsrc"], zo)
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    Ro = _oracle_R(stack, kx, wt, zs, zo, rho)
+    bound = _f16x3_error_bound(kx, wt, g.cpu().numpy(), rho)
+    err = np.linalg.norm(R16 - Ro)
+    print("  rel err vs oracle", relerr(R16, Ro), "abs err", err, "bound", bound, flush=True)
+    assert err <= bound + 1e-10 * np.linalg.norm(Ro)
+
+
+def test_f16x3_bessel_gemm_full_size_against_the_oracle():
+    """The mixed-precision configuration at its full size: 16384 nodes x 8192
+    radii x 64 depth pairs through the matrix-core GEMM, every 128th radius (and
+    the first and last) against the oracle's FP64 closed_form_G for all 64 pairs,
+    within the rounding bound (the fp32 accumulation runs over all 16384 nodes)."""
+    print("case: f16x3 bessel_gemm full size vs oracle", flush=True)
+    from yellowthinfilms_amd import workloads as W, green_nodes
+    from yellowthinfilms_amd.synthesis import CavityTransfer1D, radial_synthesis
+    wl = W.BESSEL_GEMM
+    kx, wt = W.bessel_gemm_rule(wl)
+    zs, zo, rho = W.bessel_gemm_geometry(wl)
+    assert len(kx) == 16384 and len(rho) == 8192 and len(zs) == 64
+    stack = W.point_uniaxial_stack()
+    cache = CavityTransfer1D(stack, W0, kx, wt, 1)
+    _, g = green_nodes(cache.stack, W0, cache.kx, cache.ky, 1, 1, zs, zo, cache=False)
+    R16 = radial_synthesis(cache, g, rho, mode="f16x3").cpu().numpy()
+    pick = np.unique(np.concatenate([np.arange(0, len(rho), 128), [len(rho) - 1]]))
+    Ro = _oracle_R(stack, kx, wt, zs, zo, rho[pick])
+    bound = _f16x3_error_bound(kx, wt, g.cpu().numpy(), rho[pick])
+    err = np.linalg.norm(R16[:, :, pick] - Ro)
+    print("  rel err vs oracle", relerr(R16[:, :, pick], Ro), "abs err", err, "bound", bound, flush=True)
+    assert np.isfinite(R16).all()
+    assert err <= bound + 1e-10 * np.linalg.norm(Ro)
+
+
 def test_kernel_timing_counts_launches():
     print("case: kernel timing", flush=True)
     from yellowthinfilms_amd import CavityTransfer1D, closed_form_G, workloads as W
