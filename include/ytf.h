@@ -22,6 +22,7 @@
 #ifndef YTF_H
 #define YTF_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -423,6 +424,10 @@ typedef struct ytf_operator_options {
   int32_t uniform_z;
   double dz_box;       /* box sources of this thickness (Δz_box; 0: points)  */
   int32_t ez_local;    /* local Ez inside a box's own slab          (true)   */
+  /* skeletons=: precomputed z skeletons (ytf_z_skeletons) shared by the
+   * blocks without z structure; NULL builds them from this build's nodes.
+   * They must cover the source planes and every observation layer's planes. */
+  const struct ytf_skeletons *skeletons;
 } ytf_operator_options;
 
 /* Block kinds (ZBlock.kind, mod_GreenOperator.jl:85-96). */
@@ -493,6 +498,60 @@ int ytf_operator_skeleton(ytf_ctx *ctx, const ytf_operator *op, int block, int s
  * part 1 Hankel plane i + j, part 2 dense pair i + n_src_skel * j (source
  * planes in increasing cavity-frame depth). */
 int ytf_operator_ghat(ytf_ctx *ctx, const ytf_operator *op, int block, int part, int slot, double *ghat_dev);
+
+/* z_skeletons (mod_GreenOperator.jl:335-359) and z_skeletons_grouped
+ * (:372-382): the interpolative decomposition of each side's z-exponential
+ * family (ZSkeleton, :186-208) from the mode wavenumbers of the source layer and
+ * of every observation layer at the given nodes, to `tol` per node term (sides
+ * of more than 64 planes are kept whole).  For the grouped form pass the nodes
+ * of every group (phi = 0 only on an azimuthally symmetric stack).  Planes as
+ * for ytf_operator_build: z_src in the caller's order, n_obs[l] depths of layer
+ * obs_layers[l] concatenated in z_obs.  The handle goes to
+ * ytf_operator_options.skeletons. */
+typedef struct ytf_skeletons ytf_skeletons;
+int ytf_z_skeletons(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                    int64_t n_nodes, const double *kx_dev, const double *ky_dev, const double *z_src, int n_src,
+                    int n_obs_layers, const int32_t *obs_layers, const int32_t *n_obs, const double *z_obs,
+                    double tol, ytf_skeletons **out);
+/* z_skeletons in the reference's exported form (mod_GreenOperator.jl:335-345):
+ * the mode wavenumbers given on every quadrature node, [4][n_nodes] complex
+ * (Qf1 Qf2 Qb1 Qb2 per node, device) for the source layer (Qsrc_dev) and for
+ * each observation layer (Qobs_dev[l], the (KZf, KZb) of the cache's basis);
+ * zs_int: the source planes in increasing cavity-frame depth (as the operator
+ * stores them).  Only the layers' thicknesses are read from `layers`. */
+int ytf_z_skeletons_q(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, int source_layer, int64_t n_nodes,
+                      const double *Qsrc_dev, const double *zs_int, int n_src, int n_obs_layers,
+                      const int32_t *obs_layers, const double *const *Qobs_dev, const int32_t *n_obs,
+                      const double *z_obs, double tol, ytf_skeletons **out);
+/* One side of a skeleton set: layer 0 the source side (planes in increasing
+ * cavity-frame depth), otherwise observation layer `layer` (1-based).
+ * n_planes receives N, n_skel k (N when kept whole); idx (k, 0-based,
+ * increasing), P (N x k complex column-major, host) and err may be NULL. */
+int ytf_skeletons_get(ytf_ctx *ctx, const ytf_skeletons *sk, int layer, int32_t *n_planes, int32_t *n_skel,
+                      int32_t *idx, double *P, double *err);
+int ytf_skeletons_destroy(ytf_ctx *ctx, ytf_skeletons *sk);
+
+/* The block-by-block sum of build_layer_operator_grouped (mod_GreenOperator.jl:
+ * 512-527): adds every stored plane of `src` into `dst`.  Both must come from
+ * builds with the same grid, planes, block kinds and z skeletons (the grouped
+ * build passes one skeleton set to every chunk); otherwise the reference's
+ * errors ("groups chose different block kinds", "groups have different z
+ * skeletons").  Build times add up. */
+int ytf_operator_accumulate(ytf_ctx *ctx, ytf_operator *dst, const ytf_operator *src);
+
+/* apply_operator (mod_GreenOperator.jl:674-712) with the reference's host
+ * arrays: P_host (Nx, Ny, n_src, 3) complex column-major, E_host[b] (Nx, Ny,
+ * n_obs(b), 3) per block.  The operator stages them in device buffers it keeps;
+ * returns after E has been copied back. */
+int ytf_operator_apply_host(ytf_ctx *ctx, ytf_operator *op, const double *P_host, double *const *E_host);
+
+/* Device buffers for callers without a GPU array type of their own (the
+ * Julia binding's node arrays).  Copies run on the context's stream and
+ * return when done. */
+int ytf_buffer_alloc(ytf_ctx *ctx, size_t bytes, void **dev);
+int ytf_buffer_free(ytf_ctx *ctx, void *dev);
+int ytf_buffer_upload(ytf_ctx *ctx, void *dev, const void *host, size_t bytes);
+int ytf_buffer_download(ytf_ctx *ctx, void *host, const void *dev, size_t bytes);
 
 /* Library version string. */
 const char *ytf_version(void);

@@ -412,3 +412,97 @@ def test_box_sources_closed_form_and_operator():
     c1 = CavityTransfer1D(stack, W0, np.linspace(0.1, 10, 50), np.full(50, 0.2), 2)
     with pytest.raises(ValueError, match="inside the box's own z-slab"):
         closed_form_G(c1, 2, 0.35, [0.3], [0.1], [0.0], dz_box=0.2)
+
+
+def test_precomputed_skeletons_and_host_apply():
+    """z_skeletons gives exactly the sides the build computes for itself; a build
+    given them (skeletons=) applies identically; a set built on other planes is
+    refused with the reason; apply_operator on host arrays (the reference's
+    Dict of Arrays) equals the device apply."""
+    from yellowthinfilms_amd.green_operator import build_layer_operator, cache_skeletons, z_skeletons, apply_operator
+    from yellowthinfilms_amd._lib import YtfError
+    stack = FOUNDATION_STACKS["iso 3-layer vac/glass/vac"]
+    cache = _cache(stack, 1)
+    grid = _grid(8, 8, 0.06, 0.05)
+    zs = list(np.linspace(0.1, 1.0, 24))
+    zobs = {2: list(np.linspace(0.05, 1.95, 40)), 3: list(np.linspace(0.1, 1.5, 24))}
+    print("case: precomputed skeletons", flush=True)
+    op = build_layer_operator(cache, grid, zs, zobs, skeleton_tol=1e-6)
+    sk = cache_skeletons(cache, zs, zobs, tol=1e-6)
+    assert not sk.src.whole
+    for bi, b in enumerate(op.blocks):
+        for side, s in (("source", sk.src), ("observation", sk.obs[b["obs_layer"]])):
+            idx, P, err = op.skeleton(bi, side)
+            assert np.array_equal(idx, s.idx) and np.allclose(P, s.P, rtol=0, atol=1e-13), (b["obs_layer"], side)
+            assert s.err <= 1e-6 and s.pnorm >= 1.0
+    op2 = build_layer_operator(cache, grid, zs, zobs, skeletons=sk)
+    assert [b["kind"] for b in op2.blocks] == [b["kind"] for b in op.blocks]
+    P = _moments(8, 8, len(zs), seed=3, n=8)
+    E1, E2 = _run(op, P), _run(op2, P)
+    for o in zobs:
+        assert relerr(E2[o], E1[o]) < 1e-14
+    print("case: host-array apply", flush=True)
+    Eh = apply_operator(op, P.transpose(3, 2, 1, 0).copy())
+    for o in zobs:
+        assert isinstance(Eh[o], np.ndarray)
+        assert np.array_equal(Eh[o].transpose(3, 2, 1, 0), E1[o])
+    with pytest.raises(ValueError, match="P must be"):
+        apply_operator(op, np.zeros((3, 2, 8, 8), np.complex128))
+    print("case: skeletons of other planes", flush=True)
+    with pytest.raises(YtfError, match="other source planes"):
+        build_layer_operator(cache, grid, zs, zobs, skeletons=cache_skeletons(cache, zs[1:], zobs))
+    with pytest.raises(YtfError, match="no side for observation layer 3"):
+        build_layer_operator(cache, grid, zs, zobs, skeletons=cache_skeletons(cache, zs, {2: zobs[2]}))
+    with pytest.raises(YtfError, match="other planes of layer 2"):
+        build_layer_operator(cache, grid, zs, zobs,
+                             skeletons=cache_skeletons(cache, zs, {2: zobs[2][1:], 3: zobs[3]}))
+    print("case: z_skeletons from the oracle's mode wavenumbers", flush=True)
+    # the reference's exported form, fed the oracle cache's Q_src and (KZf, KZb):
+    # the same skeleton planes as the GPU's own node solve gives
+    oc = po.nodes_batch(ostack(stack), W0, cache.kx_host, cache.ky_host, 1, [2, 3])
+    zs_int = zs[::-1]
+    Qobs = {o: (oc.basis[o][:, 6, :], oc.basis[o][:, 7, :]) for o in (2, 3)}
+    skq = z_skeletons((oc.Qf, oc.Qb), Qobs, stack, 1, zs_int, zobs, [2, 3], 1e-6)
+    # (P = R11^-1 R12 of a numerically rank-deficient family amplifies the last-bit
+    # differences of the two mode solves, so P agrees to its conditioning only)
+    for a, b in [(skq.src, sk.src)] + [(skq.obs[o], sk.obs[o]) for o in (2, 3)]:
+        assert np.array_equal(a.idx, b.idx) and abs(a.err - b.err) <= 1e-6 * max(b.err, 1e-300)
+        assert np.allclose(a.P, b.P, rtol=0, atol=1e-6)
+
+
+def test_grouped_build_in_phi_chunks_equals_one_build():
+    """build_layer_operator_grouped (mod_GreenOperator.jl:497-531) on two groups
+    of a biaxial stack: split into phi chunks by a small node_budget, the chunk
+    operators summed on the device, it equals the build with one chunk per
+    group and the single build on the concatenated nodes with the same shared
+    skeletons (z_skeletons_grouped)."""
+    from yellowthinfilms_amd.green_operator import (build_layer_operator, build_layer_operator_grouped,
+                                                    CavityTransfer2D)
+    stack = FOUNDATION_STACKS["biaxial 4-layer"]
+    kr, wr = kr_grid(stack, W0, kmax=12.0, width=0.5, order=6)
+    h = len(kr) // 2
+    pa, wa = uniform_phi(12)
+    pb, wb = uniform_phi(8)
+    groups = [(kr[:h], wr[:h], pa, wa), (kr[h:], wr[h:], pb + 0.1, wb)]
+    grid = _grid(8, 8, 0.06, 0.05)
+    zs = list(np.linspace(0.04, 0.76, 20))
+    zobs = {2: list(np.linspace(0.04, 0.76, 20)), 4: list(np.linspace(0.1, 1.0, 20))}
+    print("case: grouped build, one chunk per group", flush=True)
+    op1, _, _ = build_layer_operator_grouped(stack, W0, 2, groups, grid, zs, zobs, skeleton_tol=1e-7)
+    print("case: grouped build, phi chunks of 3", flush=True)
+    op2, tc, tb = build_layer_operator_grouped(stack, W0, 2, groups, grid, zs, zobs, skeleton_tol=1e-7,
+                                               node_budget=3 * (len(kr) - h))
+    kinds = [b["kind"] for b in op1.blocks]
+    assert kinds == ["toeplitz_skeleton", "skeleton"] and [b["kind"] for b in op2.blocks] == kinds
+    assert tb > 0
+    nodes = [CavityTransfer2D(stack, W0, 2, *g) for g in groups]
+    cat = CavityTransfer2D.from_nodes(stack, W0, 2, np.concatenate([c.kx_host for c in nodes]),
+                                      np.concatenate([c.ky_host for c in nodes]),
+                                      np.concatenate([c.W_host for c in nodes]))
+    op3 = build_layer_operator(cat, grid, zs, zobs, skeletons=op1.skeletons)
+    P = _moments(8, 8, len(zs), seed=9, n=6)
+    E1, E2, E3 = _run(op1, P), _run(op2, P), _run(op3, P)
+    for o in zobs:
+        print(f"  obs {o}: chunks vs one {relerr(E2[o], E1[o]):.1e}, groups vs concatenated "
+              f"{relerr(E1[o], E3[o]):.1e}", flush=True)
+        assert relerr(E2[o], E1[o]) < 1e-12 and relerr(E1[o], E3[o]) < 1e-12

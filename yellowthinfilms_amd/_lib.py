@@ -35,7 +35,7 @@ class YtfOperatorOptions(ct.Structure):
     _fields_ = [("structure", ct.c_int32), ("check", ct.c_int32), ("check_tol", ct.c_double),
                 ("skeleton", ct.c_int32), ("skeleton_tol", ct.c_double), ("lateral", ct.c_int32),
                 ("nufft_eps", ct.c_double), ("uniform_z", ct.c_int32), ("dz_box", ct.c_double),
-                ("ez_local", ct.c_int32)]
+                ("ez_local", ct.c_int32), ("skeletons", ct.c_void_p)]
 
 
 class YtfBlockInfo(ct.Structure):
@@ -136,6 +136,24 @@ _SIGS = {
     "ytf_operator_ghat": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]),
     "ytf_operator_skeleton": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(ct.c_int32),
                                          ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]),
+    "ytf_operator_accumulate": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p]),
+    "ytf_operator_apply_host": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_void_p)]),
+    "ytf_z_skeletons": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int, ct.c_int64,
+                                   ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.c_int, ct.c_int,
+                                   ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double),
+                                   ct.c_double, ct.POINTER(ct.c_void_p)]),
+    "ytf_z_skeletons_q": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_int, ct.c_int64, ct.c_void_p,
+                                     ct.POINTER(ct.c_double), ct.c_int, ct.c_int, ct.POINTER(ct.c_int32),
+                                     ct.POINTER(ct.c_void_p), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double),
+                                     ct.c_double, ct.POINTER(ct.c_void_p)]),
+    "ytf_skeletons_get": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.POINTER(ct.c_int32),
+                                     ct.POINTER(ct.c_int32), ct.POINTER(ct.c_int32), ct.POINTER(ct.c_double),
+                                     ct.POINTER(ct.c_double)]),
+    "ytf_skeletons_destroy": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_buffer_alloc": (ct.c_int, [ct.c_void_p, ct.c_size_t, ct.POINTER(ct.c_void_p)]),
+    "ytf_buffer_free": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
+    "ytf_buffer_upload": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_size_t]),
+    "ytf_buffer_download": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_size_t]),
 }
 
 TERMS = {"all": 0, "direct": 1, "scattered": 2, "toeplitz": 3, "hankel": 4}

@@ -76,6 +76,9 @@ struct ytf_operator {
   // apply scratch: source spectra, skeleton-folded source, block output, skeleton output
   void *work = nullptr;
   size_t work_bytes = 0;
+  // device copies of P and E for ytf_operator_apply_host
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
 };
 
 namespace {
@@ -257,6 +260,43 @@ struct Skel {
   double err = 0.0;
 };
 
+}  // namespace
+
+// A set of z skeletons (z_skeletons, mod_GreenOperator.jl:335-359): the source
+// side and one per observation layer, with the planes they were built on.
+struct ytf_skeletons {
+  int src = 1;
+  std::vector<double> zs_int;  // source planes, increasing cavity-frame depth
+  Skel src_sk;
+  std::vector<int> obs;        // observation layers, ascending
+  std::vector<std::vector<double>> zo;
+  std::vector<Skel> obs_sk;
+};
+
+namespace {
+
+// y += x over n doubles (the block-by-block sum of a grouped build).
+__global__ void k_add_f64(double *__restrict__ y, const double *__restrict__ x, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] += x[i];
+}
+
+int add_planes(ytf_ctx *c, cd *dst, const cd *src, int64_t n_complex) {
+  if (!dst || !src || n_complex <= 0) return YTF_OK;
+  const int64_t n = 2 * n_complex;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_add_f64, dim3(grid), dim3(256), 0, c->stream, (double *)dst, (const double *)src, n);
+  HIPCHK(c, hipGetLastError());
+  return YTF_OK;
+}
+
+bool same_planes(const std::vector<double> &a, const double *b, int n) {
+  if ((int)a.size() != n) return false;
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(a[i] - b[i]) > 1e-12 * std::max(1.0, std::fabs(a[i]))) return false;
+  return true;
+}
+
 // Column-pivoted Householder QR of the N x N matrix R in place (pivot: the
 // largest remaining column norm, the first on ties, as LAPACK's geqp3 /
 // qr(R, ColumnNorm())); p receives the column order.
@@ -429,6 +469,31 @@ int fill_dense(ytf_ctx *c, const Build &B, OpBlock &b, const std::vector<double>
   return fill_planes(c, B, b.obs, reqs, terms);
 }
 
+// Source side (_src_family, mod_GreenOperator.jl:329-333): planes at
+// cavity-frame depth zpos, Psi_f = e^{iQf (d - zpos)}, Psi_b = e^{-iQb zpos}; a
+// source in layer 1 feeds through Psi_f only.  Q: [4][n] Qf1 Qf2 Qb1 Qb2.
+int src_skeleton(ytf_ctx *c, const ytf_layer *layers, int src, int64_t n, const cd *Q,
+                 const std::vector<double> &zs_int, double tol, Skel *out) {
+  const double d = layers[src - 1].d;
+  std::vector<double> zpos(zs_int);
+  if (src == 1)
+    for (auto &v : zpos) v = d - v;
+  const int fam[4] = {0, 0, src != 1 ? 1 : -1, src != 1 ? 1 : -1};
+  const double sig[2] = {-1.0, -1.0}, cc[2] = {d, 0.0};
+  return compute_skeleton(c, n, Q, fam, sig, cc, zpos, tol, out);
+}
+
+// Observation side (_obs_family, :318-323): e^{iKZf z'}, e^{iKZb (z' - d)};
+// layer 1 carries the downward wave only, the last layer the upward one only.
+// Q: [4][n] KZf1 KZf2 KZb1 KZb2.
+int obs_skeleton(ytf_ctx *c, const ytf_layer *layers, int nl, int obs, int64_t n, const cd *Q,
+                 const std::vector<double> &zo, double tol, Skel *out) {
+  const double d = layers[obs - 1].d;
+  const int fam[4] = {obs != 1 ? 0 : -1, obs != 1 ? 0 : -1, obs != nl ? 1 : -1, obs != nl ? 1 : -1};
+  const double sig[2] = {1.0, 1.0}, cc[2] = {0.0, -d};
+  return compute_skeleton(c, n, Q, fam, sig, cc, zo, tol, out);
+}
+
 // Skeletons of the source side and of observation layer `obs` from one node
 // solve's mode wavenumbers (_cache_skeletons, _src_family, _obs_family,
 // mod_GreenOperator.jl:315-359).
@@ -446,23 +511,27 @@ int side_skeletons(ytf_ctx *c, const Build &B, int obs, const std::vector<double
                           YTF_TERMS_ALL, 0.0, 1);
   if (rc) return rc;
   if (need_src) {
-    // source planes at cavity-frame depth zpos: Psi_f = e^{iQf (d - zpos)},
-    // Psi_b = e^{-iQb zpos}; a source in layer 1 feeds through Psi_f only
-    const double d = B.layers[B.src - 1].d;
-    std::vector<double> zpos(zs_int);
-    if (B.src == 1)
-      for (auto &v : zpos) v = d - v;
-    const int fam[4] = {0, 0, B.src != 1 ? 1 : -1, B.src != 1 ? 1 : -1};
-    const double sig[2] = {-1.0, -1.0}, cc[2] = {d, 0.0};
-    rc = compute_skeleton(c, n, (const cd *)qb.p, fam, sig, cc, zpos, tol, src_sk);
+    rc = src_skeleton(c, B.layers, B.src, n, (const cd *)qb.p, zs_int, tol, src_sk);
     if (rc) return rc;
   }
-  // observation planes: e^{iKZf z'}, e^{iKZb (z' - d)}; layer 1 carries the
-  // downward wave only, the last layer the upward one only
-  const double d = B.layers[obs - 1].d;
-  const int fam[4] = {obs != 1 ? 0 : -1, obs != 1 ? 0 : -1, obs != B.nl ? 1 : -1, obs != B.nl ? 1 : -1};
-  const double sig[2] = {1.0, 1.0}, cc[2] = {0.0, -d};
-  return compute_skeleton(c, n, (const cd *)qb.p + 16 * n, fam, sig, cc, zo, tol, obs_sk);
+  return obs_skeleton(c, B.layers, B.nl, obs, n, (const cd *)qb.p + 16 * n, zo, tol, obs_sk);
+}
+
+// The skeletons= of build_layer_operator: the given set's source side and its
+// side for observation layer `obs`, which must have been built on these planes.
+int given_skeletons(ytf_ctx *c, const ytf_skeletons *sk, int src, const std::vector<double> &zs_int, int obs,
+                    const std::vector<double> &zo, Skel *src_sk, Skel *obs_sk) {
+  if (sk->src != src || !same_planes(sk->zs_int, zs_int.data(), (int)zs_int.size()))
+    return fail(c, YTF_EINVAL, "build_layer_operator: skeletons were built for other source planes");
+  for (size_t l = 0; l < sk->obs.size(); ++l)
+    if (sk->obs[l] == obs) {
+      if (!same_planes(sk->zo[l], zo.data(), (int)zo.size()))
+        return fail(c, YTF_EINVAL, "build_layer_operator: skeletons were built for other planes of layer %d", obs);
+      *src_sk = sk->src_sk;
+      *obs_sk = sk->obs_sk[l];
+      return YTF_OK;
+    }
+  return fail(c, YTF_EINVAL, "build_layer_operator: skeletons have no side for observation layer %d", obs);
 }
 
 int upload_matrix(ytf_ctx *c, cd **dst, const std::vector<zc> &h, int64_t *bytes) {
@@ -802,6 +871,7 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
     const auto t0 = std::chrono::steady_clock::now();
     Skel obs_sk;
     auto skel = [&](OpBlock &blk) -> int {
+      if (opt->skeletons) return given_skeletons(c, opt->skeletons, source_layer, zs_int, o, blk.zo, &src_sk, &obs_sk);
       const int r = side_skeletons(c, B, o, zs_int, blk.zo, opt->skeleton_tol, &src_sk, !have_src_sk, &obs_sk);
       have_src_sk = true;
       return r;
@@ -869,6 +939,7 @@ int ytf_operator_destroy(ytf_ctx *c, ytf_operator *op) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (auto &b : op->blocks) free_block(b);
   if (op->work) hipFree(op->work);
+  if (op->stage) hipFree(op->stage);
   delete op;
   return YTF_OK;
 }
@@ -934,9 +1005,11 @@ int ytf_operator_ghat(ytf_ctx *c, const ytf_operator *op, int block, int part, i
   return YTF_OK;
 }
 
-int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double *const *E_dev) {
-  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_apply: ctx is NULL");
-  std::lock_guard<std::mutex> lk(c->mu);
+}  // extern "C"
+
+namespace {
+
+int apply_locked(ytf_ctx *c, ytf_operator *op, const double *P_dev, double *const *E_dev) {
   if (!op || !P_dev || (!op->blocks.empty() && !E_dev)) return fail(c, YTF_EINVAL, "ytf_operator_apply: NULL argument");
   for (size_t b = 0; b < op->blocks.size(); ++b)
     if (!E_dev[b]) return fail(c, YTF_EINVAL, "ytf_operator_apply: E for block %zu is NULL", b);
@@ -1039,6 +1112,282 @@ int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double
     });
   }
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_operator_apply: launch failed: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
+
+const char *block_kind_name(int k) {
+  static const char *names[] = {":structured", ":toeplitz_dense", ":dense", ":skeleton", ":toeplitz_skeleton"};
+  return (k >= 0 && k < 5) ? names[k] : ":unknown";
+}
+
+}  // namespace
+
+extern "C" {
+
+int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double *const *E_dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_apply: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return apply_locked(c, op, P_dev, E_dev);
+}
+
+int ytf_operator_apply_host(ytf_ctx *c, ytf_operator *op, const double *P_host, double *const *E_host) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_apply_host: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!op || !P_host || (!op->blocks.empty() && !E_host))
+    return fail(c, YTF_EINVAL, "ytf_operator_apply_host: NULL argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nxy3 = 3 * (size_t)op->Nx * op->Ny;
+  const size_t pb = align256(sizeof(cd) * nxy3 * op->zs.size());
+  std::vector<size_t> eoff;
+  size_t need = pb;
+  for (auto &b : op->blocks) {
+    eoff.push_back(need);
+    need += align256(sizeof(cd) * nxy3 * b.zo.size());
+  }
+  if (need > op->stage_bytes) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (op->stage) hipFree(op->stage);
+    op->stage = nullptr;
+    op->stage_bytes = 0;
+    HIPCHK(c, hipMalloc(&op->stage, need));
+    op->stage_bytes = need;
+  }
+  char *base = (char *)op->stage;
+  HIPCHK(c, hipMemcpyAsync(base, P_host, sizeof(cd) * nxy3 * op->zs.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<double *> E(op->blocks.size());
+  for (size_t b = 0; b < E.size(); ++b) E[b] = (double *)(base + eoff[b]);
+  int rc = apply_locked(c, op, (const double *)base, E.data());
+  if (rc) return rc;
+  for (size_t b = 0; b < E.size(); ++b) {
+    if (!E_host[b]) return fail(c, YTF_EINVAL, "ytf_operator_apply_host: E for block %zu is NULL", b);
+    HIPCHK(c, hipMemcpyAsync(E_host[b], E[b], sizeof(cd) * nxy3 * op->blocks[b].zo.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return YTF_OK;
+}
+
+int ytf_operator_accumulate(ytf_ctx *c, ytf_operator *dst, const ytf_operator *src) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_accumulate: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!dst || !src) return fail(c, YTF_EINVAL, "ytf_operator_accumulate: NULL argument");
+  if (dst == src) return fail(c, YTF_EINVAL, "ytf_operator_accumulate: dst and src are the same operator");
+  if (dst->src != src->src || dst->Nx != src->Nx || dst->Ny != src->Ny || dst->dx != src->dx ||
+      dst->dy != src->dy || !same_planes(dst->zs, src->zs.data(), (int)src->zs.size()) ||
+      dst->blocks.size() != src->blocks.size())
+    return fail(c, YTF_EINVAL, "build_layer_operator_grouped: groups have different grids, source planes or blocks");
+  for (size_t i = 0; i < dst->blocks.size(); ++i) {
+    const OpBlock &a = dst->blocks[i], &b = src->blocks[i];
+    if (a.obs != b.obs || !same_planes(a.zo, b.zo.data(), (int)b.zo.size()))
+      return fail(c, YTF_EINVAL, "build_layer_operator_grouped: groups have different observation planes");
+    if (a.kind != b.kind)
+      return fail(c, YTF_EINVAL, "build_layer_operator_grouped: groups chose different block kinds (%s vs %s)",
+                  block_kind_name(a.kind), block_kind_name(b.kind));
+    if (a.sidx != b.sidx || a.oidx != b.oidx || a.Ps != b.Ps || a.Po != b.Po)
+      return fail(c, YTF_EINVAL, "build_layer_operator_grouped: groups have different z skeletons");
+    if (a.nT != b.nT || a.nH != b.nH || a.nDs != b.nDs || a.nDo != b.nDo || !a.T != !b.T || !a.H != !b.H ||
+        !a.D != !b.D)
+      return fail(c, YTF_EINVAL, "build_layer_operator_grouped: groups stored different planes");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t nn9 = 9 * (int64_t)dst->Px * dst->Py;
+  for (size_t i = 0; i < dst->blocks.size(); ++i) {
+    OpBlock &a = dst->blocks[i];
+    const OpBlock &b = src->blocks[i];
+    int rc = add_planes(c, a.T, b.T, a.nT * nn9);
+    if (!rc) rc = add_planes(c, a.H, b.H, a.nH * nn9);
+    if (!rc) rc = add_planes(c, a.D, b.D, (int64_t)a.nDs * a.nDo * nn9);
+    if (rc) return rc;
+    a.build_ms += b.build_ms;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return YTF_OK;
+}
+
+int ytf_z_skeletons(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int source_layer,
+                    int64_t n_nodes, const double *kx_dev, const double *ky_dev, const double *z_src, int n_src,
+                    int n_obs_layers, const int32_t *obs_layers, const int32_t *n_obs, const double *z_obs,
+                    double tol, ytf_skeletons **out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_z_skeletons: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!out || !layers) return fail(c, YTF_EINVAL, "ytf_z_skeletons: NULL argument");
+  *out = nullptr;
+  if (source_layer < 1 || source_layer > n_layers - 1)
+    return fail(c, YTF_EINVAL, "ytf_z_skeletons: source_layer=%d must be in 1:%d", source_layer, n_layers - 1);
+  if (n_src < 1 || !z_src) return fail(c, YTF_EINVAL, "z_skeletons: need at least one source plane");
+  if (n_nodes < 0 || (n_nodes > 0 && (!kx_dev || !ky_dev))) return fail(c, YTF_EINVAL, "ytf_z_skeletons: bad nodes");
+  if (!(tol > 0)) return fail(c, YTF_EINVAL, "z_skeletons: tol must be > 0");
+  if (n_obs_layers < 0 || (n_obs_layers > 0 && (!obs_layers || !n_obs)))
+    return fail(c, YTF_EINVAL, "ytf_z_skeletons: bad observation layers");
+  std::vector<std::pair<int, int>> order;
+  std::vector<int64_t> zoff(n_obs_layers);
+  int64_t off = 0;
+  for (int l = 0; l < n_obs_layers; ++l) {
+    if (obs_layers[l] < 1 || obs_layers[l] > n_layers || n_obs[l] < 0)
+      return fail(c, YTF_EINVAL, "ytf_z_skeletons: bad observation layer %d", obs_layers[l]);
+    for (int k = 0; k < l; ++k)
+      if (obs_layers[k] == obs_layers[l])
+        return fail(c, YTF_EINVAL, "ytf_z_skeletons: observation layer %d given twice", obs_layers[l]);
+    zoff[l] = off;
+    off += n_obs[l];
+    order.push_back({obs_layers[l], l});
+  }
+  if (off > 0 && !z_obs) return fail(c, YTF_EINVAL, "ytf_z_skeletons: z_obs is NULL");
+  std::sort(order.begin(), order.end());
+  HIPCHK(c, hipSetDevice(c->device));
+  Build B;
+  B.layers = layers;
+  B.nl = n_layers;
+  B.omega = omega;
+  B.src = source_layer;
+  B.n = n_nodes;
+  B.kx = kx_dev;
+  B.ky = ky_dev;
+  ytf_skeletons *sk = new ytf_skeletons();
+  sk->src = source_layer;
+  sk->zs_int.assign(z_src, z_src + n_src);
+  if (source_layer == 1) std::reverse(sk->zs_int.begin(), sk->zs_int.end());
+  bool have_src = false;
+  for (auto &ol : order) {
+    std::vector<double> zo(z_obs + zoff[ol.second], z_obs + zoff[ol.second] + n_obs[ol.second]);
+    Skel os;
+    int rc = side_skeletons(c, B, ol.first, sk->zs_int, zo, tol, &sk->src_sk, !have_src, &os);
+    if (rc) {
+      delete sk;
+      return rc;
+    }
+    have_src = true;
+    sk->obs.push_back(ol.first);
+    sk->zo.push_back(zo);
+    sk->obs_sk.push_back(os);
+  }
+  if (!have_src) {  // no observation layer: the source side alone
+    Skel os;
+    int rc = side_skeletons(c, B, source_layer, sk->zs_int, std::vector<double>(), tol, &sk->src_sk, true, &os);
+    if (rc) {
+      delete sk;
+      return rc;
+    }
+  }
+  *out = sk;
+  return YTF_OK;
+}
+
+int ytf_z_skeletons_q(ytf_ctx *c, const ytf_layer *layers, int n_layers, int source_layer, int64_t n_nodes,
+                      const double *Qsrc_dev, const double *zs_int, int n_src, int n_obs_layers,
+                      const int32_t *obs_layers, const double *const *Qobs_dev, const int32_t *n_obs,
+                      const double *z_obs, double tol, ytf_skeletons **out) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_z_skeletons_q: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!out || !layers) return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: NULL argument");
+  *out = nullptr;
+  if (source_layer < 1 || source_layer > n_layers - 1)
+    return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: source_layer=%d must be in 1:%d", source_layer, n_layers - 1);
+  if (n_src < 1 || !zs_int) return fail(c, YTF_EINVAL, "z_skeletons: need at least one source plane");
+  if (n_nodes < 0 || (n_nodes > 0 && !Qsrc_dev)) return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: bad Qsrc");
+  if (!(tol > 0)) return fail(c, YTF_EINVAL, "z_skeletons: tol must be > 0");
+  if (n_obs_layers < 0 || (n_obs_layers > 0 && (!obs_layers || !n_obs || !Qobs_dev)))
+    return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: bad observation layers");
+  std::vector<std::pair<int, int>> order;
+  std::vector<int64_t> zoff(n_obs_layers);
+  int64_t off = 0;
+  for (int l = 0; l < n_obs_layers; ++l) {
+    if (obs_layers[l] < 1 || obs_layers[l] > n_layers || n_obs[l] < 0 || (n_nodes > 0 && !Qobs_dev[l]))
+      return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: bad observation layer %d", obs_layers[l]);
+    for (int k = 0; k < l; ++k)
+      if (obs_layers[k] == obs_layers[l])
+        return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: observation layer %d given twice", obs_layers[l]);
+    zoff[l] = off;
+    off += n_obs[l];
+    order.push_back({obs_layers[l], l});
+  }
+  if (off > 0 && !z_obs) return fail(c, YTF_EINVAL, "ytf_z_skeletons_q: z_obs is NULL");
+  std::sort(order.begin(), order.end());
+  HIPCHK(c, hipSetDevice(c->device));
+  ytf_skeletons *sk = new ytf_skeletons();
+  sk->src = source_layer;
+  sk->zs_int.assign(zs_int, zs_int + n_src);
+  int rc = src_skeleton(c, layers, source_layer, n_nodes, (const cd *)Qsrc_dev, sk->zs_int, tol, &sk->src_sk);
+  for (size_t i = 0; i < order.size() && !rc; ++i) {
+    const int l = order[i].second;
+    std::vector<double> zo(z_obs + zoff[l], z_obs + zoff[l] + n_obs[l]);
+    Skel os;
+    rc = obs_skeleton(c, layers, n_layers, order[i].first, n_nodes, (const cd *)Qobs_dev[l], zo, tol, &os);
+    sk->obs.push_back(order[i].first);
+    sk->zo.push_back(zo);
+    sk->obs_sk.push_back(os);
+  }
+  if (rc) {
+    delete sk;
+    return rc;
+  }
+  *out = sk;
+  return YTF_OK;
+}
+
+int ytf_skeletons_get(ytf_ctx *c, const ytf_skeletons *sk, int layer, int32_t *n_planes, int32_t *n_skel,
+                      int32_t *idx, double *P, double *err) {
+  if (!sk) return fail(c, YTF_EINVAL, "ytf_skeletons_get: NULL skeleton set");
+  const Skel *s = nullptr;
+  if (layer == 0) {
+    s = &sk->src_sk;
+  } else {
+    for (size_t l = 0; l < sk->obs.size(); ++l)
+      if (sk->obs[l] == layer) s = &sk->obs_sk[l];
+  }
+  if (!s) return fail(c, YTF_EINVAL, "ytf_skeletons_get: no side for layer %d", layer);
+  const int N = s->N, k = (int)s->idx.size();
+  if (n_planes) *n_planes = N;
+  if (n_skel) *n_skel = k;
+  if (idx)
+    for (int q = 0; q < k; ++q) idx[q] = s->idx[q];
+  if (P) std::memcpy(P, s->P.data(), sizeof(zc) * (size_t)N * k);
+  if (err) *err = s->err;
+  return YTF_OK;
+}
+
+int ytf_skeletons_destroy(ytf_ctx *c, ytf_skeletons *sk) {
+  (void)c;
+  delete sk;
+  return YTF_OK;
+}
+
+int ytf_buffer_alloc(ytf_ctx *c, size_t bytes, void **dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_buffer_alloc: ctx is NULL");
+  if (!dev) return fail(c, YTF_EINVAL, "ytf_buffer_alloc: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMalloc(dev, bytes < 256 ? 256 : bytes));
+  return YTF_OK;
+}
+
+int ytf_buffer_free(ytf_ctx *c, void *dev) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_buffer_free: ctx is NULL");
+  if (!dev) return YTF_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipFree(dev));
+  return YTF_OK;
+}
+
+int ytf_buffer_upload(ytf_ctx *c, void *dev, const void *host, size_t bytes) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_buffer_upload: ctx is NULL");
+  if (bytes && (!dev || !host)) return fail(c, YTF_EINVAL, "ytf_buffer_upload: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (bytes) HIPCHK(c, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return YTF_OK;
+}
+
+int ytf_buffer_download(ytf_ctx *c, void *host, const void *dev, size_t bytes) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_buffer_download: ctx is NULL");
+  if (bytes && (!dev || !host)) return fail(c, YTF_EINVAL, "ytf_buffer_download: NULL argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  if (bytes) HIPCHK(c, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return YTF_OK;
 }
 

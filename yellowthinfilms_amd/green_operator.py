@@ -12,8 +12,9 @@ Host-side mirror of the reference's 2D route:
     precompute_ghats, closed_form_field -- the multi-source lateral assembly
     of mod_ClosedFormAssembly.jl (G on the 2Nx x 2Ny padded offsets, FFT'd
     once, exact linear convolution);
-  * Dipole, build_layer_operator, build_layer_operator_grouped,
-    apply_operator, dipole_moments, operator_summary -- mod_GreenOperator.jl.
+  * Dipole, z_skeletons, cache_skeletons, z_skeletons_grouped, build_layer_operator,
+    build_layer_operator_grouped, apply_operator, dipole_moments,
+    operator_summary -- mod_GreenOperator.jl.
 
 Array conventions: the reference's column-major Julia arrays are contiguous
 torch tensors with the axes reversed: P (Nx, Ny, Ns, 3) is a (3, Ns, Ny, Nx)
@@ -242,8 +243,15 @@ class LayerGreenOperator:
         self.flip = cache.source_layer == 1
         self.kwargs = kwargs
         self.device = device
+        self.skeletons = None
+        self.refresh()
+
+    def refresh(self):
+        """Re-read the block descriptions (after ytf_operator_accumulate the
+        build times are the sums over the groups)."""
         L = _lib.lib()
-        ctx = _lib.Context.get(device)
+        ctx = _lib.Context.get(self.device)
+        handle = self.handle
         self.blocks = []
         for b in range(L.ytf_operator_num_blocks(handle)):
             info = _lib.YtfBlockInfo()
@@ -300,21 +308,162 @@ def _grid_struct(grid):
     return _lib.YtfGrid(int(grid["Nx"]), int(grid["Ny"]), float(grid["dx"]), float(grid["dy"]))
 
 
-def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skeleton, skeleton_tol, lateral,
-           nufft_eps, uniform_z, box_lateral, dz_box, ez_local=True):
-    if structure not in _lib.STRUCTURE:
-        raise ValueError("build_layer_operator: structure must be :auto, :structured or :dense")
-    for o in z_obs_by_layer:
-        cache.check_obs(o)
-    zs = np.ascontiguousarray(np.atleast_1d(np.asarray(z_src, np.float64)))
+def _obs_arrays(z_obs_by_layer):
     obs = list(z_obs_by_layer)
     n_obs = (ct.c_int32 * max(len(obs), 1))(*[len(np.atleast_1d(z_obs_by_layer[o])) for o in obs])
     layers_c = (ct.c_int32 * max(len(obs), 1))(*obs)
     zo = np.ascontiguousarray(np.concatenate([np.atleast_1d(np.asarray(z_obs_by_layer[o], np.float64))
                                               for o in obs]) if obs else np.zeros(1))
+    return obs, n_obs, layers_c, zo
+
+
+class ZSkeletons:
+    """The result of z_skeletons (mod_GreenOperator.jl:335-359): `src` and
+    `obs[o]` are ZSkeleton records (idx 0-based and increasing, P the (N, k)
+    interpolation matrix, err, pnorm, whole); the device handle goes to
+    build_layer_operator(..., skeletons=).  Source planes are in increasing
+    cavity-frame depth, as the operator stores them."""
+
+    def __init__(self, handle, source_layer, obs_layers, device):
+        self.handle, self.source_layer, self.device = handle, source_layer, device
+        self.src = self._side(0)
+        self.obs = {o: self._side(o) for o in obs_layers}
+
+    def _side(self, layer):
+        L = _lib.lib()
+        ctx = _lib.Context.get(self.device)
+        N, k = ct.c_int32(), ct.c_int32()
+        _lib.check(L.ytf_skeletons_get(ctx.handle, self.handle, layer, ct.byref(N), ct.byref(k), None, None, None),
+                   ctx.handle)
+        idx = (ct.c_int32 * max(k.value, 1))()
+        P = np.zeros(2 * N.value * k.value, np.float64)
+        err = ct.c_double()
+        _lib.check(L.ytf_skeletons_get(ctx.handle, self.handle, layer, ct.byref(N), ct.byref(k), idx,
+                                       P.ctypes.data_as(_DP), ct.byref(err)), ctx.handle)
+        P = P.view(np.complex128).reshape((N.value, k.value), order="F")
+        whole = k.value == N.value
+        return ZSkeleton(np.array(idx[:k.value]), P, err.value,
+                         float(np.linalg.norm(P, 2)) if P.size else 1.0, whole)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                _lib.lib().ytf_skeletons_destroy(_lib.Context.get(self.device).handle, h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+@dataclass
+class ZSkeleton:
+    """ZSkeleton (mod_GreenOperator.jl:202-208)."""
+    idx: np.ndarray
+    P: np.ndarray
+    err: float
+    pnorm: float
+    whole: bool
+
+
+def _z_skeletons(stack, omega, source_layer, kx, ky, z_src, z_obs_by_layer, tol, device):
+    torch = _torch()
+    if not tol > 0:
+        raise ValueError("z_skeletons: tol must be > 0")
+    dev = torch.device("cuda", device)
+    kx = torch.as_tensor(np.asarray(kx, np.float64), device=dev).contiguous()
+    ky = torch.as_tensor(np.asarray(ky, np.float64), device=dev).contiguous()
+    zs = np.ascontiguousarray(np.atleast_1d(np.asarray(z_src, np.float64)))
+    obs, n_obs, layers_c, zo = _obs_arrays(z_obs_by_layer)
+    ctx = _lib.Context.get(device)
+    ctx.bind_stream()
+    h = ct.c_void_p()
+    _lib.check(_lib.lib().ytf_z_skeletons(ctx.handle, layer_structs(stack), len(stack), float(omega), source_layer,
+                                          kx.shape[0], _dptr(kx), _dptr(ky), zs.ctypes.data_as(_DP), len(zs),
+                                          len(obs), layers_c, n_obs, zo.ctypes.data_as(_DP), float(tol),
+                                          ct.byref(h)), ctx.handle)
+    return ZSkeletons(h, source_layer, [o for o in sorted(obs) if len(np.atleast_1d(z_obs_by_layer[o]))], device)
+
+
+def cache_skeletons(cache, z_src, z_obs_by_layer, tol=1e-6):
+    """The z skeletons of a cache's source side and of every observation
+    layer of z_obs_by_layer, from the mode wavenumbers at the cache's nodes
+    (_cache_skeletons, mod_GreenOperator.jl:361-363): what build_layer_operator
+    builds for itself.  z_src in the caller's order (Dipole.z)."""
+    return _z_skeletons(cache.stack, cache.omega, cache.source_layer, cache.kx_host, cache.ky_host, z_src,
+                        z_obs_by_layer, tol, cache.device)
+
+
+def _q4(Q):
+    """(Qf, Qb) as 2 x n arrays (or n-vectors of 2-vectors) -> (4, n) complex."""
+    Qf, Qb = (np.asarray(q, np.complex128) for q in Q)
+    Qf = Qf if Qf.ndim == 2 and Qf.shape[0] == 2 else Qf.reshape(-1, 2).T
+    Qb = Qb if Qb.ndim == 2 and Qb.shape[0] == 2 else Qb.reshape(-1, 2).T
+    return np.ascontiguousarray(np.concatenate([Qf, Qb]))
+
+
+def z_skeletons(Qsrc, Qobs, stack, s, zs_int, zobs, layers, tol, device=0):
+    """z_skeletons(Qsrc, Qobs, stack, s, zs_int, zobs, layers; tol)
+    (mod_GreenOperator.jl:335-345): Qsrc = (Qf, Qb) of source layer s and
+    Qobs[o] = (KZf, KZb) for o in layers, on every quadrature node; zs_int the
+    source planes in increasing cavity-frame depth.  Returns a ZSkeletons."""
+    torch = _torch()
+    dev = torch.device("cuda", device)
+    q_src = torch.as_tensor(_q4(Qsrc), device=dev).contiguous()
+    n = q_src.shape[1]
+    layers = [int(o) for o in layers]
+    q_obs = [torch.as_tensor(_q4(Qobs[o]), device=dev).contiguous() for o in layers]
+    if any(q.shape[1] != n for q in q_obs):
+        raise ValueError("z_skeletons: every Q list must hold the same nodes")
+    zs = np.ascontiguousarray(np.atleast_1d(np.asarray(zs_int, np.float64)))
+    _, n_obs, layers_c, zo = _obs_arrays({o: zobs[o] for o in layers})
+    qp = (ct.c_void_p * max(len(q_obs), 1))(*[q.data_ptr() for q in q_obs])
+    ctx = _lib.Context.get(device)
+    ctx.bind_stream()
+    h = ct.c_void_p()
+    _lib.check(_lib.lib().ytf_z_skeletons_q(ctx.handle, layer_structs(stack), len(stack), int(s), n, _dptr(q_src),
+                                            zs.ctypes.data_as(_DP), len(zs), len(layers), layers_c, qp, n_obs,
+                                            zo.ctypes.data_as(_DP), float(tol), ct.byref(h)), ctx.handle)
+    return ZSkeletons(h, int(s), sorted(layers), device)
+
+
+def _group_arrays(g):
+    if isinstance(g, dict):
+        return g["krho"], g["wrho"], g["phi"], g["wphi"]
+    return g
+
+
+def z_skeletons_grouped(stack, omega, source_layer, groups, z_src, z_obs_by_layer, tol=1e-6, device=0):
+    """z_skeletons_grouped (mod_GreenOperator.jl:372-382): one skeleton per
+    side for all groups, from each layer's modes at every (k_rho, phi) node of
+    every group (phi = 0 only, each k_rho once, on an azimuthally symmetric stack)."""
+    from .kspace import _is_azimuthally_symmetric
+    sym = _is_azimuthally_symmetric(stack)
+    kx, ky = [], []
+    for g in groups:
+        kr, _, ph, _ = (np.asarray(a, np.float64) for a in _group_arrays(g))
+        ph = np.zeros(1) if sym else ph
+        K, P = np.meshgrid(kr, ph, indexing="ij")
+        kx.append((K * np.cos(P)).ravel(order="F"))
+        ky.append((K * np.sin(P)).ravel(order="F"))
+    kx, ky = np.concatenate(kx), np.concatenate(ky)
+    if sym:
+        kx, i = np.unique(kx, return_index=True)
+        ky = ky[i]
+    return _z_skeletons(stack, omega, source_layer, kx, ky, z_src, z_obs_by_layer, tol, device)
+
+
+def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skeleton, skeleton_tol, lateral,
+           nufft_eps, uniform_z, box_lateral, dz_box, ez_local=True, skeletons=None):
+    if structure not in _lib.STRUCTURE:
+        raise ValueError("build_layer_operator: structure must be :auto, :structured or :dense")
+    for o in z_obs_by_layer:
+        cache.check_obs(o)
+    zs = np.ascontiguousarray(np.atleast_1d(np.asarray(z_src, np.float64)))
+    obs, n_obs, layers_c, zo = _obs_arrays(z_obs_by_layer)
     opt = _lib.YtfOperatorOptions(_lib.STRUCTURE[structure], int(bool(check)), float(check_tol),
                                   int(bool(skeleton)), float(skeleton_tol), _lateral(lateral), float(nufft_eps),
-                                  int(bool(uniform_z)), float(dz_box), int(bool(ez_local)))
+                                  int(bool(uniform_z)), float(dz_box), int(bool(ez_local)),
+                                  skeletons.handle if skeletons is not None else None)
     W = cache.weights(box_lateral)
     ctx = _lib.Context.get(cache.device)
     ctx.bind_stream()
@@ -330,6 +479,7 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
                   skeleton_tol=skeleton_tol, lateral=lateral, nufft_eps=nufft_eps, box_lateral=box_lateral,
                   dz_box=dz_box, ez_local=ez_local)
     op = LayerGreenOperator(h, cache, grid, zs, kwargs, cache.device)
+    op.skeletons = skeletons
     if warn.value & _lib.WARN_STRUCTURE_CHECK:
         for b in op.blocks:
             if b["obs_layer"] == cache.source_layer and b["kind"] != "structured":
@@ -339,46 +489,85 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
 
 
 def build_layer_operator(cache, grid, z_src, z_obs_by_layer, structure="auto", check=True, check_tol=1e-8,
-                         skeleton=True, skeleton_tol=1e-6, lateral="direct", nufft_eps=1e-9, box_lateral=None,
-                         dz_box=0.0, ez_local=True):
+                         skeleton=True, skeleton_tol=1e-6, skeletons=None, lateral="direct", nufft_eps=1e-9,
+                         box_lateral=None, dz_box=0.0, ez_local=True):
     """build_layer_operator (mod_GreenOperator.jl:430-480): the Ĝ blocks of
     `cache`'s source layer onto every layer of z_obs_by_layer (uniform z grids),
-    Toeplitz+Hankel where the media allow, dense (or skeleton) elsewhere."""
+    Toeplitz+Hankel where the media allow, dense (or skeleton) elsewhere.
+    `skeletons`: precomputed z_skeletons / z_skeletons_grouped shared by the
+    blocks without z structure (by default built from this cache's nodes)."""
     return _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skeleton, skeleton_tol, lateral,
-                  nufft_eps, True, box_lateral, dz_box, ez_local)
+                  nufft_eps, True, box_lateral, dz_box, ez_local, skeletons)
 
 
 def build_layer_operator_grouped(stack, omega, source_layer, groups, grid, z_src, z_obs_by_layer, device=0,
-                                 **kwargs):
+                                 node_budget=900_000, **kwargs):
     """build_layer_operator_grouped (mod_GreenOperator.jl:497-531): the
-    operator of several (k_rho, w_rho, phi, w_phi) groups.  Every block is
-    linear in the quadrature sum, so the groups' nodes are concatenated and
-    built at once.  Returns (op, t_cache, t_build) like the reference (the
-    node solve runs inside the build, so t_cache is 0)."""
-    kx, ky, W = [], [], []
+    operator of several (k_rho, w_rho, phi, w_phi) groups, one cache and build
+    per group, a group of more than `node_budget` k_rho x phi nodes split into
+    phi chunks, the chunk operators summed block by block on the device
+    (ytf_operator_accumulate).  The z skeletons are built once on the nodes of
+    all groups (z_skeletons_grouped) and shared; only the first chunk runs the
+    structure check.  Returns (op, t_cache, t_build)."""
+    obs = sorted(z_obs_by_layer)
+    op = None
+    t_cache = t_build = 0.0
+    sk = kwargs.pop("skeletons", None)
+    if sk is None and kwargs.get("skeleton", True) and kwargs.get("structure", "auto") != "dense":
+        t0 = time.perf_counter()
+        sk = z_skeletons_grouped(stack, omega, source_layer, groups, z_src, z_obs_by_layer,
+                                 tol=kwargs.get("skeleton_tol", 1e-6), device=device)
+        t_build += time.perf_counter() - t0
+    check = kwargs.pop("check", True)
+    L = _lib.lib()
     for g in groups:
-        c = CavityTransfer2D(stack, omega, source_layer, g["krho"], g["wrho"], g["phi"], g["wphi"], device=device) \
-            if isinstance(g, dict) else CavityTransfer2D(stack, omega, source_layer, *g, device=device)
-        kx.append(c.kx_host)
-        ky.append(c.ky_host)
-        W.append(c.W_host)
-    cache = CavityTransfer2D.from_nodes(stack, omega, source_layer, np.concatenate(kx), np.concatenate(ky),
-                                        np.concatenate(W), obs_layers=list(z_obs_by_layer), device=device)
-    t0 = time.perf_counter()
-    op = build_layer_operator(cache, grid, z_src, z_obs_by_layer, **kwargs)
-    return op, 0.0, time.perf_counter() - t0
+        kr, wr, ph, wp = (np.asarray(a, np.float64) for a in _group_arrays(g))
+        chunk = int(np.clip(node_budget // max(len(kr), 1), 1, max(len(ph), 1)))
+        for r0 in range(0, len(ph), chunk):
+            r = slice(r0, r0 + chunk)
+            t0 = time.perf_counter()
+            c = CavityTransfer2D(stack, omega, source_layer, kr, wr, ph[r], wp[r], obs_layers=obs, device=device)
+            t_cache += time.perf_counter() - t0
+            t0 = time.perf_counter()
+            opc = build_layer_operator(c, grid, z_src, z_obs_by_layer, skeletons=sk,
+                                       check=(op is None and check), **kwargs)
+            if op is None:
+                op = opc
+            else:
+                ctx = _lib.Context.get(device)
+                ctx.bind_stream()
+                _lib.check(L.ytf_operator_accumulate(ctx.handle, op.handle, opc.handle), ctx.handle)
+                del opc
+            t_build += time.perf_counter() - t0
+    if op is not None:
+        op.refresh()
+    return op, t_cache, t_build
 
 
 def apply_operator(op, P):
     """apply_operator (mod_GreenOperator.jl:674-712).  P: (3, Ns, Ny, Nx)
-    dipole moments (column-major (Nx, Ny, Ns, 3)).  Returns
-    {obs_layer: (3, No, Ny, Nx) complex128 tensor}."""
-    torch = _torch()
+    dipole moments (column-major (Nx, Ny, Ns, 3)).  A torch tensor stays on the
+    device and gives {obs_layer: (3, No, Ny, Nx) complex128 tensor}; a numpy
+    array is the reference's host array and gives host arrays, staged through
+    the operator's own device buffers (ytf_operator_apply_host)."""
     g = op.grid
     Ns = len(op.z_src)
+    shape = (3, Ns, g["Ny"], g["Nx"])
+    if isinstance(P, np.ndarray):
+        if P.shape != shape:
+            raise ValueError(f"apply_operator: P must be (Nx, Ny, Ns, 3) = {(g['Nx'], g['Ny'], Ns, 3)}, got "
+                             f"{tuple(reversed(P.shape))}")
+        Ph = np.ascontiguousarray(P, dtype=np.complex128)
+        outs = [np.empty((3, b["n_obs"], g["Ny"], g["Nx"]), np.complex128) for b in op.blocks]
+        ptrs = (ct.c_void_p * max(len(outs), 1))(*[E.ctypes.data for E in outs])
+        ctx = _lib.Context.get(op.device)
+        ctx.bind_stream()
+        _lib.check(_lib.lib().ytf_operator_apply_host(ctx.handle, op.handle, Ph.ctypes.data, ptrs), ctx.handle)
+        return {b["obs_layer"]: E for b, E in zip(op.blocks, outs)}
+    torch = _torch()
     dev = torch.device("cuda", op.device)
     P = torch.as_tensor(P, dtype=torch.complex128, device=dev)
-    if tuple(P.shape) != (3, Ns, g["Ny"], g["Nx"]):
+    if tuple(P.shape) != shape:
         raise ValueError(f"apply_operator: P must be (Nx, Ny, Ns, 3) = {(g['Nx'], g['Ny'], Ns, 3)}, got "
                          f"{tuple(reversed(P.shape))}")
     P = P.contiguous()
