@@ -719,7 +719,7 @@ def closed_form_field(caches, grid, z_obs_by_layer, dips, ghat_cache=None, **kw)
                 ghat_cache[key] = op
         # one sheet per depth: moments snapped to pixels (point_current_sheet's
         # density times dx dy)
-        E = apply_operator(op, dipole_moments(op, ds))
+        E = apply_operator(op, torch.as_tensor(dipole_moments(op, ds), device=torch.device("cuda", op.device)))
         for o, e in E.items():
             out[o] = out[o] + e if o in out else e
     dev = torch.device("cuda", next(iter(caches.values())).device)
