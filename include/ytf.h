@@ -210,6 +210,18 @@ int ytf_pole_refine(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double 
  *   work_dev: scratch of 3*nx*ny complex. */
 int ytf_spectral_apply(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, const double *J_dev, double *E_dev,
                        double *work_dev, double scale);
+/* No counterpart in the reference: ytf_spectral_apply for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing and takes no work
+ * buffer: the work array and the root tables live in one file-scope device
+ * array of 12.06 MiB that each GPU the library runs on holds until the process
+ * ends, set up by ytf_ctx_create.  nx, ny: powers of two in 8..2048 with
+ * nx*ny <= 512*512 (larger grids return YTF_EINVAL; use ytf_spectral_apply).
+ * Same arguments and result, to the bit, as ytf_spectral_apply.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other contexts,
+ * other streams) race. */
+int ytf_spectral_apply_noalloc(ytf_ctx *ctx, int nx, int ny, const double *ghat_dev, const double *J_dev,
+                               double *E_dev, double scale);
 
 /* Precision of the radial synthesis. */
 #define YTF_SYNTH_FP64 0  /* complex FP64 throughout (the reference's arithmetic) */

@@ -80,6 +80,8 @@ _SIGS = {
                                        ct.POINTER(ct.c_int64)]),
     "ytf_spectral_apply": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
                                       ct.c_void_p, ct.c_double]),
+    "ytf_spectral_apply_noalloc": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p,
+                                              ct.c_void_p, ct.c_double]),
     "ytf_radial_synthesis": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,
                                         ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]),
     "ytf_radial_terms": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p,

@@ -36,6 +36,12 @@ int ytf_ctx_create(int device, ytf_ctx **out) {
     return fail(nullptr, YTF_EHIP, "ytf_ctx_create: stream creation failed");
   }
   c->stream = c->own;
+  std::string err;
+  if (int rc = noalloc_init(device, &err)) {
+    hipStreamDestroy(c->own);
+    delete c;
+    return fail(nullptr, rc, "ytf_ctx_create: %s", err.c_str());
+  }
   *out = c;
   return YTF_OK;
 }

@@ -153,16 +153,20 @@ inline int drain_marks(ytf_ctx *c) {
   return YTF_OK;
 }
 
-inline int twiddles(ytf_ctx *c, int n, cd **out) {
-  auto it = c->tw.find(n);
-  if (it != c->tw.end()) { *out = it->second; return YTF_OK; }
-  std::vector<cd> h(n);
+// The length-n root table exp(-2 pi i j / n), j < n, rounded from long double.
+inline void twiddle_table(int n, cd *h) {
   for (int j = 0; j < n; ++j) {
-    // exp(-2 pi i j / n) with the argument reduced exactly to the first octant
     long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
     h[j].r = (double)cosl(a);
     h[j].i = (double)sinl(a);
   }
+}
+
+inline int twiddles(ytf_ctx *c, int n, cd **out) {
+  auto it = c->tw.find(n);
+  if (it != c->tw.end()) { *out = it->second; return YTF_OK; }
+  std::vector<cd> h(n);
+  twiddle_table(n, h.data());
   cd *d = nullptr;
   HIPCHK(c, hipMalloc(&d, sizeof(cd) * n));
   HIPCHK(c, hipMemcpy(d, h.data(), sizeof(cd) * n, hipMemcpyHostToDevice));
@@ -172,6 +176,10 @@ inline int twiddles(ytf_ctx *c, int n, cd **out) {
 }
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Per-device set-up of the allocation-free entry points (noalloc.hip); run by
+// ytf_ctx_create with the device current.
+int noalloc_init(int device, std::string *err);
 
 // Entry points for callers that already hold the context's lock (capi.hip).
 int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
