@@ -445,9 +445,8 @@ __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double
   const cd et = L.eps[0], ez = L.eps[8], mr = L.mu[0];
   const cd wc = mk(w);
   const double kr = hypot(kx, ky);
-  const double phi = kr == 0.0 ? 0.0 : atan2(ky, kx);
-  double sp, cp;
-  sincos(phi, &sp, &cp);
+  // cos and sin of phi = atan(ky, kx), taken as kx / kr and ky / kr
+  const double cp = kr == 0.0 ? 1.0 : kx / kr, sp = kr == 0.0 ? 0.0 : ky / kr;
   const cd kr2 = mk(kr * kr);
   const cd kzo = csqrt(wc * wc * et * mr - kr2);
   const cd kze = csqrt(et * (wc * wc * mr - kr2 / ez));
@@ -464,12 +463,16 @@ __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double
 // _modes_layer -- mod_ModeSolver.jl:274-326.  Output columns/entries are in
 // slot order (f1, f2, b1, b2).  Returns false if the eigen solve failed or
 // more than two modes fall on one side.
+// `canon` (may be NULL) receives whether the layer took the closed form and its
+// slots hold (TEf, TMf, TEb, TMb) unpermuted.
 template <bool GENERAL = true>
-__device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double kx, double ky, cd *Vm, cd *Qm) {
+__device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double kx, double ky, cd *Vm, cd *Qm,
+                                            bool *canon = nullptr) {
   const double tol = 1.4901161193847656e-08;
   cd Qs[4], Vs[16];
   bool ok = true;
-  if ((L.ek == 0 || L.ek == 1) && L.mk == 0) {
+  const bool closed = (L.ek == 0 || L.ek == 1) && L.mk == 0;
+  if (closed) {
     eigen_closed(L, w, kx, ky, Qs, Vs);
   } else {
     cd B[16];
@@ -509,6 +512,34 @@ __device__ __forceinline__ bool modes_layer(const LayerDev &L, double w, double 
       dj = (nf + i < 2) ? nf + i : 2 + nb + (i - (2 - nf));
     }
     dst[j] = dj;
+  }
+  // Move raw mode j to slot dst[j].  When every lane of the wave has the same
+  // order and it is one of the usual two (closed form: TEf TMf | TEb TMb; the
+  // (Re, Im)-sorted eigen solves: the two forward modes last), the moves are
+  // fixed; otherwise each slot selects among the four modes.
+  const int code = dst[0] | dst[1] << 2 | dst[2] << 4 | dst[3] << 6;
+  const int wcode = __builtin_amdgcn_readfirstlane(code);
+  constexpr int kClosed = 0 | 2 << 2 | 1 << 4 | 3 << 6, kSorted = 2 | 3 << 2 | 0 << 4 | 1 << 6;
+  if (canon) *canon = closed && code == kClosed;
+  if (__all(code == wcode) && (wcode == kClosed || wcode == kSorted)) {
+    if (wcode == kClosed) {
+#pragma unroll
+      for (int slot = 0; slot < 4; ++slot) {
+        const int j = slot == 0 ? 0 : slot == 1 ? 2 : slot == 2 ? 1 : 3;
+        Qm[slot] = Qs[j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Vm[4 * slot + r] = Vs[4 * j + r];
+      }
+    } else {
+#pragma unroll
+      for (int slot = 0; slot < 4; ++slot) {
+        const int j = slot == 0 ? 2 : slot == 1 ? 3 : slot == 2 ? 0 : 1;
+        Qm[slot] = Qs[j];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Vm[4 * slot + r] = Vs[4 * j + r];
+      }
+    }
+    return ok;
   }
 #pragma unroll
   for (int slot = 0; slot < 4; ++slot) {
@@ -572,20 +603,43 @@ __device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
       if (ai > amax) { amax = ai; p = i; }
     }
     ok = ok && amax != 0.0;
+    // Row exchange: none when no lane pivots, fixed moves when every lane of
+    // the wave pivots on the same row, per-lane selects otherwise.
+    const int wp = __builtin_amdgcn_readfirstlane(p);
+    const bool uniform = __all(p == wp);
+    if (!(uniform && wp == k)) {
 #pragma unroll
-    for (int i = k + 1; i < 4; ++i) {
-      const bool t = p == i;
+      for (int i = k + 1; i < 4; ++i) {
+        if (uniform) {
+          if (wp == i) {
 #pragma unroll
-      for (int j = k; j < 4; ++j) {
-        const cd a = A[k + 4 * j], b = A[i + 4 * j];
-        A[k + 4 * j] = sel(t, b, a);
-        A[i + 4 * j] = sel(t, a, b);
-      }
+            for (int j = k; j < 4; ++j) {
+              const cd a = A[k + 4 * j];
+              A[k + 4 * j] = A[i + 4 * j];
+              A[i + 4 * j] = a;
+            }
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        const cd a = B[k + 4 * j], b = B[i + 4 * j];
-        B[k + 4 * j] = sel(t, b, a);
-        B[i + 4 * j] = sel(t, a, b);
+            for (int j = 0; j < N; ++j) {
+              const cd a = B[k + 4 * j];
+              B[k + 4 * j] = B[i + 4 * j];
+              B[i + 4 * j] = a;
+            }
+          }
+        } else {
+          const bool t = p == i;
+#pragma unroll
+          for (int j = k; j < 4; ++j) {
+            const cd a = A[k + 4 * j], b = A[i + 4 * j];
+            A[k + 4 * j] = sel(t, b, a);
+            A[i + 4 * j] = sel(t, a, b);
+          }
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            const cd a = B[k + 4 * j], b = B[i + 4 * j];
+            B[k + 4 * j] = sel(t, b, a);
+            B[i + 4 * j] = sel(t, a, b);
+          }
+        }
       }
     }
     const cd rp = cinv(A[k + 4 * k]);
@@ -609,6 +663,30 @@ __device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
       B[i + 4 * j] = s * A[i + 4 * i];
     }
   return ok;
+}
+
+// The closed form's mode coefficients: with u = (c, s), v = (-s, c) the TE
+// columns are E = v, H = -+a u and the TM columns E = +-b u, H = v, where
+// a = kzo / (w mu) and b = kze / (w eps_t) (mod_ModeSolver.jl:243-265).
+__device__ __forceinline__ void closed_ab(const LayerDev &L, double w, const cd *Q, cd &a, cd &b) {
+  a = Q[0] / (mk(w) * L.mu[0]);
+  b = Q[1] / (mk(w) * L.eps[0]);
+}
+
+// _S_Interface_Matrix between two closed-form layers whose slots hold
+// (TEf, TMf, TEb, TMb): TE and TM decouple, and with x = a (TE) or b (TM) the
+// solve [-Vf_B Vb_A] [fB; bA] = [-Vf_A Vb_B] [fA; bB] gives, per polarisation,
+// fB = (2 xA fA + (xB - xA) bB) / (xA + xB), bA = ((xA - xB) fA + 2 xB bB) / (xA + xB),
+// so every 2x2 block of S is diagonal.
+__device__ __forceinline__ void s_interface_closed(cd aA, cd bA, cd aB, cd bB, cd *S) {
+  const cd ra = cinv(aA + aB), rb = cinv(bA + bB);
+  const cd z = mk(0.0);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) S[i] = z;
+  S[0] = rmul(2.0, aA) * ra;       S[5] = rmul(2.0, bA) * rb;        // S11: fB <- fA
+  S[8] = (aB - aA) * ra;           S[13] = (bB - bA) * rb;           // S12: fB <- bB
+  S[2] = (aA - aB) * ra;           S[7] = (bA - bB) * rb;            // S21: bA <- fA
+  S[10] = rmul(2.0, aB) * ra;      S[15] = rmul(2.0, bB) * rb;       // S22: bA <- bB
 }
 
 // _S_Interface_Matrix -- mod_ScatteringMatrix.jl:27-47:
@@ -701,6 +779,43 @@ __device__ __forceinline__ bool ill_conditioned(const cd *A) {
   const double l1 = (tr + disc) / 2.0, l2 = (tr - disc) / 2.0;
   if (l2 <= 0.0) return true;
   return sqrt(l1 / l2) > 1e8;
+}
+
+// Source_projection_Berreman_xyz for a closed-form source layer whose slots
+// hold (TEf, TMf, TEb, TMb): (W'V) \ (W'J) = V^-1 J, and V^-1 splits by
+// polarisation.  With J's components E_v, E_u, H_u, H_v along v = (-s, c),
+// u = (c, s): f_TE = (E_v - H_u / a) / 2, b_TE = (E_v + H_u / a) / 2,
+// f_TM = (H_v + E_u / b) / 2, b_TM = (H_v - E_u / b) / 2 (the closed form has
+// eps_tz = 0, so J_Ht = R2s Jt).
+__device__ __forceinline__ bool source_projection_closed(const LayerDev &L, double w, double kx, double ky,
+                                                         const cd *Q, cd *sif, cd *sib) {
+  const double kr = hypot(kx, ky);
+  const double c = kr == 0.0 ? 1.0 : kx / kr, sn = kr == 0.0 ? 0.0 : ky / kr;
+  cd a, b;
+  closed_ab(L, w, Q, a, b);
+  const cd ia = cinv(a), ib = cinv(b);
+  const cd mio = mk(0.0, -w);
+  const cd jz = mio / (L.eps[8] * mk(w));  // J_Et of the z dipole is (kx, ky) jz
+  // per moment q: (E_x, E_y, H_x, H_y) of J_Berreman = (0, 0, 0, -mio), (0, 0, mio, 0), (kx jz, ky jz, 0, 0)
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    cd Ev, Eu, Hu, Hv;
+    if (q == 2) {
+      Ev = mk(0.0); Eu = rmul(kr, jz); Hu = mk(0.0); Hv = mk(0.0);
+    } else {
+      const double hx = q == 0 ? 0.0 : 1.0, hy = q == 0 ? -1.0 : 0.0;
+      Ev = mk(0.0); Eu = mk(0.0);
+      Hu = rmul(c * hx + sn * hy, mio);
+      Hv = rmul(-sn * hx + c * hy, mio);
+    }
+    const cd hu = Hu * ia, eu = Eu * ib;
+    sif[2 * q] = rmul(0.5, Ev - hu);
+    sib[2 * q] = rmul(0.5, Ev + hu);
+    sif[2 * q + 1] = rmul(0.5, Hv + eu);
+    sib[2 * q + 1] = rmul(0.5, Hv - eu);
+  }
+  // a = 0 or b = 0 (k_z = 0 on a light line) is the singular W'V of the reference
+  return !(a.r == 0.0 && a.i == 0.0) && !(b.r == 0.0 && b.i == 0.0);
 }
 
 // Source_projection_Berreman_xyz -- mod_SourceProj.jl:34-60.  sif, sib:
@@ -1076,17 +1191,25 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) { RL[i] = mk(0.0); SLs22[i] = mk(i == 0 || i == 3 ? 1.0 : 0.0); Ts[i] = mk(0.0); }
     cd R[4] = {mk(0.0), mk(0.0), mk(0.0), mk(0.0)};
-    cd Vn[16];
-    {
-      cd Qn[4];
-      ok = modes_layer<GENERAL>(P.L[N - 1], w, kx, ky, Vn, Qn) && ok;
-    }
+    cd Vn[16], Qn[4];
+    bool canon_n = false;
+    ok = modes_layer<GENERAL>(P.L[N - 1], w, kx, ky, Vn, Qn, &canon_n) && ok;
     for (int j = N - 2; j >= 0; --j) {
       const LayerDev &L = P.L[j];
       cd V[16], Q[4];
-      ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q) && ok;
+      bool canon = false;
+      ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q, &canon) && ok;
       cd Sint[16];
-      sing = !s_interface(V, Vn, Sint) || sing;
+      if (__all(canon && canon_n)) {
+        // both layers in the closed form's slot order: TE and TM decouple
+        cd aA, bA, aB, bB;
+        closed_ab(L, w, Q, aA, bA);
+        closed_ab(P.L[j + 1], w, Qn, aB, bB);
+        s_interface_closed(aA, bA, aB, bB, Sint);
+        sing = !(isfin(Sint[0]) && isfin(Sint[5]) && isfin(Sint[8]) && isfin(Sint[13])) || sing;
+      } else {
+        sing = !s_interface(V, Vn, Sint) || sing;
+      }
       const cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
       cd e11[4], e12[4], e21[4], e22[4];
 #pragma unroll
@@ -1108,6 +1231,9 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       for (int i = 0; i < 4; ++i) R[i] = e21[i] + Z[i];
 #pragma unroll
       for (int i = 0; i < 16; ++i) Vn[i] = V[i];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) Qn[i] = Q[i];
+      canon_n = canon;
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) Rs[i] = R[i];
@@ -1133,10 +1259,12 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   // The forward pass applied, at layer s, only the swap relative to layer s-1's
   // matched order; replaying the raw solve with the recorded decisions
   // reproduces the matched order of layer s exactly.
-  modes_layer<GENERAL>(Ls, w, kx, ky, Vs, Qs);
+  bool canon_s = false;
+  modes_layer<GENERAL>(Ls, w, kx, ky, Vs, Qs, &canon_s);
   apply_swaps(swS_f, swS_b, Vs, Qs);
   cd sif[6], sib[6];
-  sing = !source_projection(Ls, w, kx, ky, Vs, sif, sib) || sing;
+  if (__all(canon_s && !swS_f && !swS_b)) sing = !source_projection_closed(Ls, w, kx, ky, Qs, sif, sib) || sing;
+  else sing = !source_projection(Ls, w, kx, ky, Vs, sif, sib) || sing;
   bool seeds_fin = true;
 #pragma unroll
   for (int i = 0; i < 6; ++i) seeds_fin = seeds_fin && isfin(sif[i]) && isfin(sib[i]);
