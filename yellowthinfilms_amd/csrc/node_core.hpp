@@ -449,7 +449,8 @@ __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double
   const double cp = kr == 0.0 ? 1.0 : kx / kr, sp = kr == 0.0 ? 0.0 : ky / kr;
   const cd kr2 = mk(kr * kr);
   const cd kzo = csqrt(wc * wc * et * mr - kr2);
-  const cd kze = csqrt(et * (wc * wc * mr - kr2 / ez));
+  // isotropic eps: kze equals kzo, taken as the same number
+  const cd kze = L.ek == 0 ? kzo : csqrt(et * (wc * wc * mr - kr2 / ez));
   const cd A = mk(1.0) / (wc * mr);
   const cd Bn = mk(1.0) / (wc * et);
   const cd c = mk(cp), s = mk(sp);
@@ -1210,7 +1211,20 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       } else {
         sing = !s_interface(V, Vn, Sint) || sing;
       }
-      const cd p[4] = {expi(Q[0], L.d), expi(Q[1], L.d), expi(-Q[2], L.d), expi(-Q[3], L.d)};
+      // the closed form's backward slots hold -Q of the forward ones, so their
+      // phases e^{-i Qb d} are the forward phases (an isotropic layer has one)
+      cd p[4];
+      p[0] = expi(Q[0], L.d);
+      if (__all(canon)) {
+        if (L.ek == 0) p[1] = p[0];
+        else p[1] = expi(Q[1], L.d);
+        p[2] = p[0];
+        p[3] = p[1];
+      } else {
+        p[1] = expi(Q[1], L.d);
+        p[2] = expi(-Q[2], L.d);
+        p[3] = expi(-Q[3], L.d);
+      }
       cd e11[4], e12[4], e21[4], e22[4];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
@@ -1273,8 +1287,21 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   const double ds = Ls.d;
   // RRr = diag(ph) * SRight[s] (_R_cavity): its blocks 21 (RR) and 11
   cd RR[4], RRr11[4];
+  // closed-form source layer in slot order: Qb = -Qf, so e^{i Qb d} = e^{-i Qf d}
+  const bool phase_pairs = __all(canon_s && !swS_f && !swS_b);
   {
-    const cd ph[4] = {expi(Qs[0], -ds), expi(Qs[1], -ds), expi(Qs[2], ds), expi(Qs[3], ds)};
+    cd ph[4];
+    ph[0] = expi(Qs[0], -ds);
+    if (phase_pairs) {
+      if (Ls.ek == 0) ph[1] = ph[0];
+      else ph[1] = expi(Qs[1], -ds);
+      ph[2] = ph[0];
+      ph[3] = ph[1];
+    } else {
+      ph[1] = expi(Qs[1], -ds);
+      ph[2] = expi(Qs[2], ds);
+      ph[3] = expi(Qs[3], ds);
+    }
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -1286,8 +1313,18 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   // F+ = (I - PF RL PB RR) \ (E+ + PF RL E-),  F- = E- + PB RR F+   (2x4 each)
   cd Fp[8], Fm[8];
   {
-    const cd PF[2] = {expi(Qs[0], ds), expi(Qs[1], ds)};
-    const cd PB[2] = {expi(Qs[2], -ds), expi(Qs[3], -ds)};
+    cd PF[2], PB[2];
+    PF[0] = expi(Qs[0], ds);
+    if (phase_pairs) {
+      if (Ls.ek == 0) PF[1] = PF[0];
+      else PF[1] = expi(Qs[1], ds);
+      PB[0] = PF[0];
+      PB[1] = PF[1];
+    } else {
+      PF[1] = expi(Qs[1], ds);
+      PB[0] = expi(Qs[2], -ds);
+      PB[1] = expi(Qs[3], -ds);
+    }
     cd t[4], A[4];
     // (PF*RL)*PB
 #pragma unroll
