@@ -1200,49 +1200,73 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       cd V[16], Q[4];
       bool canon = false;
       ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q, &canon) && ok;
-      cd Sint[16];
       if (__all(canon && canon_n)) {
-        // both layers in the closed form's slot order: TE and TM decouple
+        // both layers in the closed form's slot order: TE and TM decouple and
+        // every block of the interface S and of the layer's (S_int, S_prop)
+        // product is diagonal (per polarisation c: TE 0, TM 1)
         cd aA, bA, aB, bB;
         closed_ab(L, w, Q, aA, bA);
         closed_ab(P.L[j + 1], w, Qn, aB, bB);
+        cd Sint[16];
         s_interface_closed(aA, bA, aB, bB, Sint);
         sing = !(isfin(Sint[0]) && isfin(Sint[5]) && isfin(Sint[8]) && isfin(Sint[13])) || sing;
-      } else {
-        sing = !s_interface(V, Vn, Sint) || sing;
-      }
-      // the closed form's backward slots hold -Q of the forward ones, so their
-      // phases e^{-i Qb d} are the forward phases (an isotropic layer has one)
-      cd p[4];
-      p[0] = expi(Q[0], L.d);
-      if (__all(canon)) {
+        cd p[2];
+        p[0] = expi(Q[0], L.d);
         if (L.ek == 0) p[1] = p[0];
         else p[1] = expi(Q[1], L.d);
-        p[2] = p[0];
-        p[3] = p[1];
+        const cd e11[2] = {Sint[0] * p[0], Sint[5] * p[1]};
+        const cd e12[2] = {Sint[8], Sint[13]};
+        const cd e21[2] = {p[0] * (Sint[2] * p[0]), p[1] * (Sint[7] * p[1])};
+        const cd e22[2] = {p[0] * Sint[10], p[1] * Sint[15]};
+        // R <- e21 + e22 ((I - R e12) \ R) e11
+        cd M[4], X[4];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int r = 0; r < 2; ++r) M[r + 2 * c] = mk(r == c ? 1.0 : 0.0) - R[r + 2 * c] * e12[c];
+        solve2<2>(M, R, X);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int r = 0; r < 2; ++r) R[r + 2 * c] = e22[r] * (X[r + 2 * c] * e11[c]);
+        R[0] = e21[0] + R[0];
+        R[3] = e21[1] + R[3];
       } else {
-        p[1] = expi(Q[1], L.d);
-        p[2] = expi(-Q[2], L.d);
-        p[3] = expi(-Q[3], L.d);
-      }
-      cd e11[4], e12[4], e21[4], e22[4];
-#pragma unroll
-      for (int c = 0; c < 2; ++c)
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          e11[r + 2 * c] = Sint[r + 4 * c] * p[c];
-          e12[r + 2 * c] = Sint[r + 4 * (c + 2)];
-          e21[r + 2 * c] = p[2 + r] * (Sint[r + 2 + 4 * c] * p[c]);
-          e22[r + 2 * c] = p[2 + r] * Sint[r + 2 + 4 * (c + 2)];
+        cd Sint[16];
+        sing = !s_interface(V, Vn, Sint) || sing;
+        // the closed form's backward slots hold -Q of the forward ones, so their
+        // phases e^{-i Qb d} are the forward phases (an isotropic layer has one)
+        cd p[4];
+        p[0] = expi(Q[0], L.d);
+        if (__all(canon)) {
+          if (L.ek == 0) p[1] = p[0];
+          else p[1] = expi(Q[1], L.d);
+          p[2] = p[0];
+          p[3] = p[1];
+        } else {
+          p[1] = expi(Q[1], L.d);
+          p[2] = expi(-Q[2], L.d);
+          p[3] = expi(-Q[3], L.d);
         }
-      cd t[4], M[4], X[4], Y[4], Z[4];
-      mm<2, 2, 2>(R, e12, t);
-      M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
-      solve2<2>(M, R, X);
-      mm<2, 2, 2>(X, e11, Y);
-      mm<2, 2, 2>(e22, Y, Z);
+        cd e11[4], e12[4], e21[4], e22[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) R[i] = e21[i] + Z[i];
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            e11[r + 2 * c] = Sint[r + 4 * c] * p[c];
+            e12[r + 2 * c] = Sint[r + 4 * (c + 2)];
+            e21[r + 2 * c] = p[2 + r] * (Sint[r + 2 + 4 * c] * p[c]);
+            e22[r + 2 * c] = p[2 + r] * Sint[r + 2 + 4 * (c + 2)];
+          }
+        cd t[4], M[4], X[4], Y[4], Z[4];
+        mm<2, 2, 2>(R, e12, t);
+        M[0] = mk(1.0) - t[0]; M[1] = -t[1]; M[2] = -t[2]; M[3] = mk(1.0) - t[3];
+        solve2<2>(M, R, X);
+        mm<2, 2, 2>(X, e11, Y);
+        mm<2, 2, 2>(e22, Y, Z);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) R[i] = e21[i] + Z[i];
+      }
 #pragma unroll
       for (int i = 0; i < 16; ++i) Vn[i] = V[i];
 #pragma unroll
@@ -1508,6 +1532,9 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   }
 
   // ---- spectral dyadic g~ for each (zsrc, zobs) pair -----------------------
+  // an isotropic closed-form source layer in slot order: kze = kzo, so the two
+  // modes of a side share their phases
+  const bool iso_pair = phase_pairs && Ls.ek == 0;
   if (P.ghat) {
     const double d_src = Ls.d, d_obs = P.L[o].d;
     for (int pr = 0; pr < P.npairs; ++pr) {
@@ -1521,8 +1548,16 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       const int terms = P.terms;
       const double dz = P.dz;
       if (Tvalid && terms != TERMS_DIRECT) {
-        cd Pf[2] = {expi(Qs[0], d_src - zpos), expi(Qs[1], d_src - zpos)};
-        cd Pb[2] = {expi(-Qs[2], zpos), expi(-Qs[3], zpos)};
+        cd Pf[2], Pb[2];
+        Pf[0] = expi(Qs[0], d_src - zpos);
+        Pb[0] = expi(-Qs[2], zpos);
+        if (iso_pair) {  // an isotropic closed-form layer: the TM phases are the TE ones
+          Pf[1] = Pf[0];
+          Pb[1] = Pb[0];
+        } else {
+          Pf[1] = expi(Qs[1], d_src - zpos);
+          Pb[1] = expi(-Qs[3], zpos);
+        }
         if (dz > 0.0) {  // box seeds D.Psi (_seed_phases)
 #pragma unroll
           for (int m = 0; m < 2; ++m) {
@@ -1573,26 +1608,40 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
           // point: strict sign of zeta; box: |zeta| >= dz/2 (_direct_branch)
           const bool fwd = dz > 0.0 ? zeta >= 0.5 * dz : zeta > 0;
           const bool bwd = dz > 0.0 ? zeta <= -0.5 * dz : zeta < 0;
+          if (fwd || bwd) {
+            cd e[2];
 #pragma unroll
-          for (int m = 0; m < 2; ++m) {
-            if (fwd) {
-              const cd e = dz > 0.0 ? boxphase(Qs[m], zeta, dz) : phase(Qs[m], zeta);
-#pragma unroll
-              for (int q = 0; q < 3; ++q) Afd[m + 2 * q] = sif[m + 2 * q] * e;
-            } else if (bwd) {
-              const cd e = dz > 0.0 ? boxphase(Qs[2 + m], zeta, dz) : phase(Qs[2 + m], zeta);
-#pragma unroll
-              for (int q = 0; q < 3; ++q) Abd[m + 2 * q] = -(sib[m + 2 * q] * e);
+            for (int m = 0; m < 2; ++m) {
+              const cd Qd = fwd ? Qs[m] : Qs[2 + m];
+              if (m == 1 && iso_pair && dz == 0.0) e[1] = e[0];
+              else e[m] = dz > 0.0 ? boxphase(Qd, zeta, dz) : phase(Qd, zeta);
             }
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) {
+                if (fwd) Afd[m + 2 * q] = sif[m + 2 * q] * e[m];
+                else Abd[m + 2 * q] = -(sib[m + 2 * q] * e[m]);
+              }
           }
         }
       }
       cd g[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) g[i] = z0;
+      cd pfv[2], pbv[2];
+      pfv[0] = phase(basis[12], zob);
+      pbv[0] = phase(basis[14], zob - d_obs);
+      if ((CASE == CASE_SAME || CASE == CASE_TOP) && iso_pair) {
+        pfv[1] = pfv[0];
+        pbv[1] = pbv[0];
+      } else {
+        pfv[1] = phase(basis[13], zob);
+        pbv[1] = phase(basis[15], zob - d_obs);
+      }
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
-        const cd pf = phase(basis[12 + m], zob), pb = phase(basis[14 + m], zob - d_obs);
+        const cd pf = pfv[m], pb = pbv[m];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const cd af = fin0(Af[m + 2 * q] * pf + Afd[m + 2 * q]);
