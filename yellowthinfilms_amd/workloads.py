@@ -363,6 +363,12 @@ BATCHED_GYRO_512_SPECIAL_PER_NODE = 278.0
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scales from a 256 MiB device copy):
 # 6.03 MB read (kx, ky, stack) + 37.75 MB written (g~).
 BATCHED_GYRO_512_NODE_HBM_BYTES = 43.78e6
+# FP64 flops the node kernel executes per node on batched_gyro_512 (rocprofv3
+# counters of one launch: 64 x (SQ_INSTS_VALU_ADD_F64 + SQ_INSTS_VALU_MUL_F64 +
+# 2 SQ_INSTS_VALU_FMA_F64) / 262,144 nodes = 64 x (3.588e6 + 8.268e6 + 2 x 11.932e6)
+# / 262,144).  The kernel solves each node with less arithmetic than the reference
+# formulation above (block-form eigen, 2x2 recursions, closed-form interfaces).
+BATCHED_GYRO_512_EXECUTED_FLOPS_PER_NODE = 8721.0
 
 
 def uniform_c128_source(nx, ny, seed):
