@@ -161,6 +161,21 @@ def bench_gyro(a):
                      "achieved_executed": executed / 1e12, "frac_executed": executed / PEAK_FP64},
         "cpu_baseline": cpu,
     })
+    if not a.no_surrogate:
+        # the metric's other half: the configured surrogate (surrogate_1024) against the plain GPU solve
+        sw = dict_ns(a, steps=min(a.steps, 10), warmup=min(a.warmup, 2))
+        sr, sf = surrogate_runs(sw, [(W.SURROGATE_1024["width"], W.SURROGATE_1024["depth"])])
+        r = sr[0]
+        out.update({
+            "surrogate_speedup_vs_plain": r["speedup_vs_plain_gpu"],
+            "surrogate_rel_err_vs_fp64": r["rel_err_ghat_vs_fp64"],
+            "surrogate_field_rel_err_vs_fp64": r["rel_err_field_vs_fp64"],
+            "flagged_fraction": r["flagged_fraction"],
+            "surrogate_knodes_per_s": r["knodes_per_s"],
+            "surrogate_config": (f"surrogate_1024: MLP 6->{r['width']}x{r['depth']}->18 BF16, "
+                                 f"{sf['train_samples']} samples x {sf['epochs']} epochs, {sf['grid'][0]}^2 grid, "
+                                 "exact fallback on light line, origin, cavity-response pole windows, large |t|"),
+        })
     return out
 
 
@@ -424,7 +439,12 @@ def cpu_uneven(stacks, counts, kx, ky, out):
 
 
 # ---------------------------------------------------------------------------
-def bench_surrogate(a):
+def surrogate_runs(a, configs, windows=None):
+    """The configured surrogate workload: 2^22 exact-solver samples (generated
+    once), then per (width, depth) in `configs` a network trained on them and
+    the surrogate path (network + flags + exact fallback) timed against the
+    plain GPU solve of the same 1024^2 grid, with g~ and field errors against
+    the exact FP64 path.  Returns (per-config results, shared facts)."""
     import torch
     from yellowthinfilms_amd import uniform_kxky_grid, green_nodes, workloads as W
     from yellowthinfilms_amd.surrogate import GyroFamily, SurrogateGreen
@@ -432,17 +452,11 @@ def bench_surrogate(a):
 
     wl = W.SURROGATE_1024
     fam = GyroFamily(seed=wl["family_seed"])
-    sg = SurrogateGreen(fam, width=wl["width"], depth=wl["depth"], seed=wl["seed"])
+    gen = SurrogateGreen(fam, width=wl["width"], depth=wl["depth"], seed=wl["seed"])
     t0 = time.perf_counter()
-    x, t = sg.generate(wl["n_train"], seed=wl["data_seed"])
+    x, t = gen.generate(wl["n_train"], seed=wl["data_seed"])
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    hist = sg.fit(x, t, epochs=wl["epochs"], batch=wl["batch"], lr=wl["lr"])
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    del x, t
-    # evaluation grid: uniform 1024^2, |kx|, |ky| <= 3 omega
     omega = 2 * math.pi * wl["f"]
     grid = uniform_kxky_grid(wl["L"] / wl["f"], wl["dx"] / wl["f"])
     nx, ny = grid["Nx"], grid["Ny"]
@@ -454,27 +468,16 @@ def bench_surrogate(a):
     nn = nx * ny
     g_exact = torch.empty((1, 9, nn), dtype=torch.complex128, device=dev)
     ctx = Context.get(0)
-    res = {}
 
     def plain():
         green_nodes(st, omega, KX, KY, 1, 1, [hs * W.NM], [1.0 - ho * W.NM], cache=False, ghat_out=g_exact,
                     terms="scattered")
 
-    def surrogate():
-        res["g"], res["mask"] = sg.scattered_ghat(KX, KY, hs, ho, omega, loss, fallback=True, count=False)
-
     for _ in range(a.warmup):
         plain()
-        surrogate()
     wall_plain = _timed_steps(dict_ns(a, warmup=0), plain, torch)
-    ctx.set_timing(True)
-    wall_sur = _timed_steps(dict_ns(a, warmup=0), surrogate, torch)
-    gemm_ms, gemm_n = ctx.kernel_time("mlp_fused")
-    ctx.set_timing(False)
-    # errors of the spectral dyadic and of the field of a seeded source
+    plain_rate = nn * a.steps / wall_plain
     ge = g_exact[0].T.contiguous()            # (nn, 9)
-    gs = res["g"]
-    rel_g = float(torch.linalg.norm(gs - ge) / torch.linalg.norm(ge))
     J = torch.as_tensor(W.uniform_c128_source(nx, ny, wl["seed"]), device=dev)
     work = torch.empty((3, nx, ny), dtype=torch.complex128, device=dev)
     scale = grid["dx"] * grid["dy"] * grid["dkx"] * grid["dky"] / (2 * math.pi) ** 2
@@ -488,29 +491,92 @@ def bench_surrogate(a):
                                        ct.c_void_p(E.data_ptr()), ct.c_void_p(work.data_ptr()), scale), ctx.handle)
         return E
 
-    E_ex, E_su = field(ge), field(gs)
-    rel_E = float(torch.linalg.norm(E_su - E_ex) / torch.linalg.norm(E_ex))
-    flagged = float(res["mask"].float().mean())
-    mlp_flops = sg.mlp.flops_per_sample * nn  # one forward per step
-    gemm_per_step = gemm_ms / a.steps
-    sur_rate, plain_rate = nn * a.steps / wall_sur, nn * a.steps / wall_plain
-    out = _base(a, wl, sur_rate, "Green k-nodes/s (surrogate + exact fallback)", wall_sur,
+    E_ex = field(ge)
+    kr = torch.sqrt(KX * KX + KY * KY)
+    windows = [wl["pole_window"]] if windows is None else windows
+    poles, win_conf = gen.pole_windows(omega, loss, wl["pole_window"])
+    in_conf = torch.zeros(nn, dtype=torch.bool, device=dev)
+    for lo, hi in win_conf:
+        in_conf |= (kr >= lo) & (kr <= hi)
+    results = []
+    for width, depth in configs:
+        sg = gen if (width, depth) == (wl["width"], wl["depth"]) else SurrogateGreen(fam, width=width, depth=depth,
+                                                                                      seed=wl["seed"])
+        t0 = time.perf_counter()
+        hist = sg.fit(x, t, epochs=wl["epochs"], batch=wl["batch"], lr=wl["lr"])
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        for pw in windows:
+            res = {}
+            _, win = gen.pole_windows(omega, loss, pw) if pw > 0 else (None, np.zeros((0, 2)))
+
+            def surrogate():
+                res["g"], res["mask"] = sg.scattered_ghat(KX, KY, hs, ho, omega, loss, fallback=True, count=False,
+                                                          pole_window=pw)
+
+            for _ in range(a.warmup):
+                surrogate()
+            ctx.set_timing(True)
+            wall_sur = _timed_steps(dict_ns(a, warmup=0), surrogate, torch)
+            gemm_ms, _ = ctx.kernel_time("mlp_fused")
+            ctx.set_timing(False)
+            gs, mask = res["g"], res["mask"]
+            in_win = torch.zeros_like(mask)
+            for lo, hi in win:
+                in_win |= (kr >= lo) & (kr <= hi)
+            resp = mask & ~in_win & ~(((kr - omega).abs() < sg.BRANCH_TOL * omega) | (kr < sg.ORIGIN_TOL * omega))
+            results.append({
+                "width": width, "depth": depth, "pole_window": pw, "flops_per_sample": sg.mlp.flops_per_sample,
+                "knodes_per_s": nn * a.steps / wall_sur,
+                "speedup_vs_plain_gpu": (nn * a.steps / wall_sur) / plain_rate,
+                "rel_err_ghat_vs_fp64": float(torch.linalg.norm(gs - ge) / torch.linalg.norm(ge)),
+                "rel_err_field_vs_fp64": float(torch.linalg.norm(field(gs) - E_ex) / torch.linalg.norm(E_ex)),
+                "flagged_fraction": float(mask.float().mean()),
+                "flagged_pole_windows": float(in_win.float().mean()),
+                "flagged_response": float(resp.float().mean()),
+                # of the nodes within the configured pole windows, the share this run flags
+                "flagged_of_configured_window_nodes": float((mask & in_conf).float().sum() /
+                                                            in_conf.float().sum().clamp_min(1)),
+                "mlp_ms_per_forward": gemm_ms / a.steps, "train_s": t_train,
+                "train_loss_per_epoch": [round(h, 6) for h in hist],
+            })
+        del sg
+    facts = {"nodes": nn, "grid": [nx, ny], "plain_gpu_knodes_per_s": plain_rate, "data_s": t_data,
+             "train_samples": wl["n_train"], "epochs": wl["epochs"],
+             "poles": [(round(p["q"] / omega, 6), round(p["Gamma"] / omega, 6)) for p in poles]}
+    return results, facts
+
+
+def bench_surrogate(a):
+    from yellowthinfilms_amd import workloads as W
+    wl = W.SURROGATE_1024
+    configs = [(wl["width"], wl["depth"])]
+    for c in (a.sur_sweep.split(",") if a.sur_sweep else []):
+        w, d = (int(v) for v in c.lower().split("x"))
+        if (w, d) not in configs:
+            configs.append((w, d))
+    wins = [wl["pole_window"]] + [float(v) for v in a.sur_windows.split(",") if v.strip() and
+                                  float(v) != wl["pole_window"]] if a.sur_windows else None
+    results, facts = surrogate_runs(a, configs, wins)
+    main = results[0]
+    wall = a.steps * facts["nodes"] / main["knodes_per_s"]
+    out = _base(a, wl, main["knodes_per_s"], "Green k-nodes/s (surrogate + exact fallback)", wall,
                 "bf16 (MLP, fp32 accumulate) + f64 (fallback)")
-    out["config"].update({"nodes": nn, "grid": [nx, ny], "mlp": f"6->{wl['width']}x{wl['depth']}->18 SiLU",
-                          "train_samples": wl["n_train"], "epochs": wl["epochs"]})
-    out.update({
-        "plain_gpu_knodes_per_s": plain_rate, "speedup_vs_plain_gpu": sur_rate / plain_rate,
-        "rel_err_ghat_vs_fp64": rel_g, "rel_err_field_vs_fp64": rel_E, "flagged_fraction": flagged,
-        "train_loss_per_epoch": [round(h, 6) for h in hist], "data_s": t_data, "train_s": t_train,
-        "roofline": {"kernel": "mlp_fused (the whole network in one BF16 matrix-core kernel, with the exact "
-                               "factors and flags of the surrogate)",
-                     "bound": "mfma", "achieved": mlp_flops / (gemm_per_step * 1e-3) / 1e12,
-                     "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
-                     "frac": mlp_flops / (gemm_per_step * 1e-3) / PEAK_FP16, "traffic": None,
-                     "flops_per_sample": sg.mlp.flops_per_sample, "samples_per_launch_set": nn,
-                     "mlp_ms_per_forward": gemm_per_step},
-        "cpu_baseline": None,
-    })
+    out["config"].update({"nodes": facts["nodes"], "grid": facts["grid"],
+                          "mlp": f"6->{wl['width']}x{wl['depth']}->18 SiLU", "train_samples": wl["n_train"],
+                          "epochs": wl["epochs"]})
+    out.update({k: v for k, v in main.items() if k not in ("width", "depth")})
+    out["config"]["pole_window_halfwidths"] = main["pole_window"]
+    out.update({"plain_gpu_knodes_per_s": facts["plain_gpu_knodes_per_s"], "data_s": facts["data_s"],
+                "poles": facts["poles"], "tradeoff": results[1:]})
+    out["roofline"] = {"kernel": "mlp_fused (the whole network in one BF16 matrix-core kernel, with the exact "
+                                 "factors and flags of the surrogate)",
+                       "bound": "mfma", "achieved": main["flops_per_sample"] * facts["nodes"] /
+                       (main["mlp_ms_per_forward"] * 1e-3) / 1e12, "peak": PEAK_FP16 / 1e12, "unit": "TFLOP/s",
+                       "frac": main["flops_per_sample"] * facts["nodes"] / (main["mlp_ms_per_forward"] * 1e-3) /
+                       PEAK_FP16, "traffic": None, "flops_per_sample": main["flops_per_sample"],
+                       "samples_per_launch_set": facts["nodes"], "mlp_ms_per_forward": main["mlp_ms_per_forward"]}
+    out["cpu_baseline"] = None
     return out
 
 
@@ -1161,6 +1227,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--tune-db", default=None, help="autotuner database file (uneven_256)")
+    ap.add_argument("--sur-sweep", default="",
+                    help="surrogate_1024: further network sizes to train and time on the same data, e.g. 128x2,256x2")
+    ap.add_argument("--sur-windows", default="",
+                    help="surrogate_1024: further pole-window widths (half-widths) to evaluate, e.g. 0,2,4")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="batched_gyro_512: leave out the surrogate-vs-plain comparison of the metric")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="after the timed region, one device-to-device copy of this many MiB (a kernel of known "
                          "traffic, to scale the FETCH_SIZE/WRITE_SIZE counters of a profiling run)")

@@ -331,6 +331,14 @@ int ytf_mlp_loss_grad(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev,
  * (before the step) is copied to loss_dev when not NULL. */
 int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev, const float *y_dev, float lr,
                        float *loss_dev);
+/* The pole flag of ytf_mlp_surrogate_ghat from the cavity response: nodes
+ * with k_rho inside one of n_windows windows [lo, hi] (windows: lo0 hi0 lo1
+ * hi1 ..., host) are flagged from their inputs and solved exactly.  The caller
+ * places the windows around the poles of _cavity_response found by
+ * real_axis_poles (mod_KSpaceFeatures.jl:109-127, 151-214) -- the fallback
+ * indicator of sigma_max(RL), sigma_max(RR), sigma_max((I - PF RL PB RR)^-1);
+ * n_windows = 0 clears them.  At most 4096 windows. */
+int ytf_mlp_set_pole_windows(ytf_ctx *ctx, ytf_mlp *mlp, int n_windows, const double *windows);
 /* Surrogate evaluation of the scattered spectral dyadic above the stack, with
  * the exact solve as fallback (no reference counterpart: the reference runs
  * the exact solve, build_cavity_transfer_2d + closed_form_G's
@@ -341,7 +349,8 @@ int ytf_mlp_train_step(ytf_ctx *ctx, ytf_mlp *mlp, int64_t n, const float *x_dev
  *     t_e = y_e + i y_{9+e},  kz0 = sqrt(omega^2 - k^2) (Im >= 0),
  * into ghat_dev [9][n] complex (entry c + 3q, as ytf_green_nodes).  A node is
  * flagged (flags_dev[i] = 1) when |k - omega| < branch_tol omega (the air
- * branch point), k < origin_tol omega, or |y|_2 > t_big (near a pole).  With
+ * branch point), k < origin_tol omega, k lies in a pole window
+ * (ytf_mlp_set_pole_windows), or |y|_2 > t_big (a large predicted response).  With
  * fallback, the flagged nodes are solved exactly (ytf_green_nodes with source
  * and observation layer 1, depths zsrc / zobs, YTF_TERMS_SCATTERED) and their
  * g~ replaced (the nodes flagged from their inputs alone before the network,

@@ -6,7 +6,9 @@ the exact FP64 solve as fallback and arbiter of accuracy.
 Tolerances: loss <= 1e-4 relative, per-layer gradients <= 2e-3 relative
 (FP32 accumulation order and the odd bf16 rounding-boundary flip against the
 float64 reference); the flagged nodes of the surrogate path equal the exact
-solve bit for bit."""
+solve bit for bit; on the 1024^2 grid the surrogate's g~ is within 1e-2
+(256 x 4 with pole windows) or 5e-2 (128 x 2, at >= 1.3x the exact path's
+speed) of the exact solve."""
 import math
 
 import numpy as np
@@ -99,42 +101,64 @@ def test_training_fits_a_smooth_function():
     assert l1 < l0 / 30
 
 
-def test_surrogate_of_the_scattered_dyadic_with_exact_fallback():
-    """Train on 2^16 solver samples of the gyrotropic family, then on a grid:
-    the trained network beats the untrained one by >= 10x, the fallback
-    nodes are exact, and the error with fallback is reported."""
-    print("case: surrogate end to end (small)", flush=True)
+def _grid_1024():
+    from yellowthinfilms_amd import uniform_kxky_grid, workloads as W
+    wl = W.SURROGATE_1024
+    grid = uniform_kxky_grid(wl["L"] / wl["f"], wl["dx"] / wl["f"])
+    nx, ny = grid["Nx"], grid["Ny"]
+    kx = torch.as_tensor(np.repeat(grid["kx"], ny), device="cuda")
+    ky = torch.as_tensor(np.tile(grid["ky"], nx), device="cuda")
+    return wl, kx, ky
+
+
+def _ms(fn, reps=5):
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def test_surrogate_against_the_exact_path():
+    """The method with the network against the method without it (the exact
+    FP64 solve of every node), on the 1024 x 1024 grid of the surrogate
+    workload (omega = 2 pi, loss 0.1, heights 40 / 50 nm), after training on
+    2^18 exact-solver samples of the gyrotropic family:
+      - the configured network (256 x 4) with the cavity-response pole windows
+        (+-8 half-widths): relative error of g~ below 1e-2, the flagged nodes
+        bit-identical to the exact solve;
+      - a small network (128 x 2) with +-2 half-width windows: relative error
+        below 5e-2 at >= 1.3x the exact path's nodes per second.
+    (The configured workload trains on 2^22 samples; its accuracy/speed curve
+    over network sizes and window widths is a benchmark.)"""
+    print("case: surrogate vs exact path, 1024^2", flush=True)
     from yellowthinfilms_amd.surrogate import GyroFamily, SurrogateGreen
-    fam = GyroFamily(seed=1)
-    sg = SurrogateGreen(fam, seed=7)
-    x, t = sg.generate(1 << 18, seed=2)
-    xv, tv = sg.generate(4096, seed=9)  # held out
+    wl, kx, ky = _grid_1024()
+    omega, loss, hs, ho = 2 * math.pi * wl["f"], wl["loss"], wl["h_src_nm"], wl["h_obs_nm"]
+    fam = GyroFamily(seed=wl["family_seed"])
+    big = SurrogateGreen(fam, width=256, depth=4, seed=wl["seed"])
+    x, t = big.generate(1 << 18, seed=wl["data_seed"])
     assert torch.isfinite(t).all()
-    sg0 = SurrogateGreen(fam, seed=7)  # the same network, untrained
-    sg0.fit(x, t, epochs=0)  # normalisation only
-    sg.fit(x, t, epochs=0)
-    l0, _ = sg.mlp.loss_grad(xv, tv)
-    hist = sg.fit(x, t, epochs=10, batch=4096, lr=2e-3)
-    l1, _ = sg.mlp.loss_grad(xv, tv)
-    print("held-out loss", l0, "->", l1, "epochs", [round(h, 4) for h in hist], flush=True)
-    assert l1 < l0 / 10
-    omega, loss, hs, ho = 2 * math.pi, 0.1, 40.0, 50.0
-    k = np.linspace(-2.9, 2.9, 96) * omega
-    KX, KY = np.meshgrid(k, k, indexing="ij")
-    kx, ky = KX.ravel(), KY.ravel()
-    exact = sg.exact_scattered(kx, ky, hs, ho, omega, loss)
-    g1, mask = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=False)
-    g0, _ = sg0.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=False)
-    keep = ~mask
-    e1 = float(torch.linalg.norm((g1 - exact)[keep]) / torch.linalg.norm(exact[keep]))
-    e0 = float(torch.linalg.norm((g0 - exact)[keep]) / torch.linalg.norm(exact[keep]))
-    gf, maskf = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True)
-    ef = float(torch.linalg.norm(gf - exact) / torch.linalg.norm(exact))
-    print(f"surrogate rel err on unflagged nodes: trained {e1:.3e}, untrained {e0:.3e}; with fallback {ef:.3e}; "
-          f"flagged {float(maskf.float().mean()):.3f}", flush=True)
-    assert e1 < e0 / 10
-    assert torch.equal(gf[maskf], exact[maskf])
-    assert ef < 0.5
+    exact = big.exact_scattered(kx, ky, hs, ho, omega, loss)
+    t_exact = _ms(lambda: big.exact_scattered(kx, ky, hs, ho, omega, loss))
+    small = SurrogateGreen(fam, width=128, depth=2, seed=wl["seed"])
+    for sg, pw, bar, epochs in ((big, wl["pole_window"], 1e-2, 10), (small, 2.0, 5e-2, 40)):
+        hist = sg.fit(x, t, epochs=epochs, batch=4096, lr=2e-3)
+        g, mask = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_window=pw)
+        err = float(torch.linalg.norm(g - exact) / torch.linalg.norm(exact))
+        ms = _ms(lambda: sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, count=False, pole_window=pw))
+        speed = t_exact / ms
+        print(f"  {sg.mlp.width}x{sg.mlp.depth} window {pw}: loss {hist[0]:.3e} -> {hist[-1]:.3e}; rel err {err:.3e} "
+              f"(bar {bar:g}); flagged {float(mask.float().mean()):.3f}; {ms:.3f} ms against exact {t_exact:.3f} ms "
+              f"= {speed:.2f}x", flush=True)
+        assert torch.equal(g[mask], exact[mask])
+        assert err < bar, err
+        if sg is small:
+            assert speed >= 1.3, speed
 
 
 def test_fused_surrogate_matches_the_composed_path():
@@ -166,10 +190,14 @@ def test_fused_surrogate_matches_the_composed_path():
     x[:, 2], x[:, 3], x[:, 4], x[:, 5] = hs, ho, omega / (2 * math.pi), loss
     t = sg.mlp.forward(x)
     g_ref = ghat_from_target(t, kx, ky, hs * NM, ho * NM, omega)
-    mask_ref = sg.flags(kx, ky, t, omega, pole_factor=2.0)
-    g, mask = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=False, pole_factor=2.0)
+    poles, win = sg.pole_windows(omega, loss, 8.0)
+    assert len(win) >= 1
+    print("  poles (q/omega, Gamma/omega):", [(round(p["q"] / omega, 4), round(p["Gamma"] / omega, 4)) for p in poles],
+          flush=True)
+    mask_ref = sg.flags(kx, ky, t, omega, pole_factor=2.0, windows=win)
+    g, mask = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=False, pole_factor=2.0, pole_window=8.0)
     assert torch.equal(mask, mask_ref)
-    assert 0 < int(mask.sum()) < n // 4
+    assert 0 < int(mask.sum()) < n // 2
     ok = torch.isfinite(g_ref).all(dim=1)
     err = float(torch.linalg.norm((g - g_ref)[ok]) / torch.linalg.norm(g_ref[ok]))
     print(f"  flagged {int(mask.sum())} of {n}; rel err vs composed {err:.2e}", flush=True)
@@ -178,12 +206,14 @@ def test_fused_surrogate_matches_the_composed_path():
     # network, the response-flagged ones after it: both sets non-empty here
     kr = torch.sqrt(kx * kx + ky * ky)
     near = ((kr - omega).abs() < sg.BRANCH_TOL * omega) | (kr < sg.ORIGIN_TOL * omega)
+    for lo, hi in win:
+        near |= (kr >= lo) & (kr <= hi)
     for pf in (2.0, 0.7):
         gf, maskf = sg.scattered_ghat(kx, ky, hs, ho, omega, loss, fallback=True, pole_factor=pf)
         n_near, n_big = int(near.sum()), int((maskf & ~near).sum())
         print(f"  pole_factor {pf}: {n_near} input-flagged, {n_big} response-flagged", flush=True)
         assert n_near > 0 and torch.equal(maskf & near, near)
-        assert torch.equal(maskf, sg.flags(kx, ky, t, omega, pole_factor=pf))
+        assert torch.equal(maskf, sg.flags(kx, ky, t, omega, pole_factor=pf, windows=win))
         assert sg.last_flagged == int(maskf.sum())
         exact = sg.exact_scattered(kx[maskf], ky[maskf], hs, ho, omega, loss)
         assert torch.equal(gf[maskf], exact)

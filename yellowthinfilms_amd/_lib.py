@@ -112,6 +112,7 @@ _SIGS = {
                                      ct.POINTER(ct.c_float), ct.POINTER(ct.c_float)]),
     "ytf_mlp_train_step": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_float,
                                       ct.c_void_p]),
+    "ytf_mlp_set_pole_windows": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.c_int, ct.POINTER(ct.c_double)]),
     "ytf_mlp_surrogate_ghat": (ct.c_int, [ct.c_void_p, ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double,
                                           ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_float), ct.c_double,
                                           ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_int,

@@ -762,12 +762,15 @@ struct ytf_mlp {
   // activation workspace
   void *act = nullptr;
   size_t act_bytes = 0;
+  // k_rho windows of the surrogate's pole flag (lo, hi pairs), device
+  double *pole_win = nullptr;
+  int n_pole_win = 0;
 };
 
 static size_t ru64(size_t x) { return (x + 63) / 64 * 64; }
 
 static int mlp_free(ytf_mlp *m) {
-  void *ps[] = {m->P, m->Mo, m->Vo, m->G, m->P16, m->P16pk, m->norm, m->loss, m->act};
+  void *ps[] = {m->P, m->Mo, m->Vo, m->G, m->P16, m->P16pk, m->norm, m->loss, m->act, m->pole_win};
   for (void *q : ps)
     if (q) hipFree(q);
   delete m;
@@ -1162,6 +1165,26 @@ int ytf_mlp_forward(ytf_ctx *c, ytf_mlp *m, int64_t n, const float *x_dev, float
   return YTF_OK;
 }
 
+int ytf_mlp_set_pole_windows(ytf_ctx *c, ytf_mlp *m, int n_windows, const double *windows) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_mlp_set_pole_windows: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!m || n_windows < 0 || n_windows > 4096 || (n_windows > 0 && !windows))
+    return fail(c, YTF_EINVAL, "ytf_mlp_set_pole_windows: bad arguments (at most 4096 windows)");
+  for (int t = 0; t < n_windows; ++t)
+    if (!(windows[2 * t] <= windows[2 * t + 1]))
+      return fail(c, YTF_EINVAL, "ytf_mlp_set_pole_windows: window %d has lo > hi", t);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (m->pole_win) HIPCHK(c, hipFree(m->pole_win));
+  m->pole_win = nullptr;
+  m->n_pole_win = 0;
+  if (n_windows == 0) return YTF_OK;
+  HIPCHK(c, hipMalloc((void **)&m->pole_win, sizeof(double) * 2 * (size_t)n_windows));
+  HIPCHK(c, hipMemcpy(m->pole_win, windows, sizeof(double) * 2 * (size_t)n_windows, hipMemcpyHostToDevice));
+  m->n_pole_win = n_windows;
+  return YTF_OK;
+}
+
 int ytf_mlp_surrogate_ghat(ytf_ctx *c, ytf_mlp *m, const ytf_layer *layers, int n_layers, double omega,
                            const double *kx_dev, const double *ky_dev, int64_t n, const float *xconst, double zsum,
                            double zsrc, double zobs, double branch_tol, double origin_tol, double t_big, int fallback,
@@ -1214,7 +1237,8 @@ int ytf_mlp_surrogate_ghat(ytf_ctx *c, ytf_mlp *m, const ytf_layer *layers, int 
   };
   hipError_t e = ytf::launch_mlp_zero(2, (float *)count, c->stream);  // zero bits = integer 0
   if (e == hipSuccess)
-    e = ytf::launch_sur_preflag(n, kx_dev, ky_dev, omega, branch_tol, origin_tol, flags_dev, count, list_in, c->stream);
+    e = ytf::launch_sur_preflag(n, kx_dev, ky_dev, omega, branch_tol, origin_tol, m->pole_win, m->n_pole_win,
+                                flags_dev, count, list_in, c->stream);
   if (e != hipSuccess) return launch_err(e);
   // exact solve of the rows list[0 .. *cnt), g~ written in place (a grid of at
   // most 1024 one-wave blocks; the kernel strides over the count)

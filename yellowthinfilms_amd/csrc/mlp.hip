@@ -606,17 +606,19 @@ hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStr
 
 // Input flags of the surrogate rows: flags[i] = 1 where |k - omega| <
 // branch_tol omega or k < origin_tol omega (near the light line or the
-// origin), else 0; the flagged rows are appended to list (one counter
+// origin) or k lies in one of the nwin pole windows [lo, hi] (win: lo0 hi0
+// lo1 hi1 ...), else 0; the flagged rows are appended to list (one counter
 // increment per wave).
 __global__ void k_sur_preflag(int64_t n, const double *__restrict__ kx, const double *__restrict__ ky, double w,
-                              double branch_tol, double origin_tol, unsigned char *__restrict__ flags,
-                              int *__restrict__ count, int *__restrict__ list) {
+                              double branch_tol, double origin_tol, const double *__restrict__ win, int nwin,
+                              unsigned char *__restrict__ flags, int *__restrict__ count, int *__restrict__ list) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   bool f = false;
   if (i < n) {
     const double a = kx[i], b = ky[i];
     const double kr = sqrt(a * a + b * b);
     f = fabs(kr - w) < branch_tol * w || kr < origin_tol * w;
+    for (int t = 0; t < nwin; ++t) f = f || (kr >= win[2 * t] && kr <= win[2 * t + 1]);
     flags[i] = f ? 1 : 0;
   }
   const unsigned long long bal = __ballot(f);
@@ -629,10 +631,11 @@ __global__ void k_sur_preflag(int64_t n, const double *__restrict__ kx, const do
 }
 
 hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, double w, double branch_tol,
-                              double origin_tol, unsigned char *flags, int *count, int *list, hipStream_t s) {
+                              double origin_tol, const double *win, int nwin, unsigned char *flags, int *count,
+                              int *list, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_sur_preflag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, kx, ky, w, branch_tol,
-                     origin_tol, flags, count, list);
+                     origin_tol, win, nwin, flags, count, list);
   return hipGetLastError();
 }
 

@@ -193,7 +193,8 @@ hipError_t launch_mlp_fused(const MlpFused &p, int width, bool surrogate, hipStr
 // Input flags of the surrogate rows (near the light line or the origin) into
 // flags[n] (0/1), the flagged rows appended to list through *count.
 hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, double w, double branch_tol,
-                              double origin_tol, unsigned char *flags, int *count, int *list, hipStream_t s);
+                              double origin_tol, const double *win, int nwin, unsigned char *flags, int *count,
+                              int *list, hipStream_t s);
 
 // 2D type-1 nonuniform FFT (nufft.hip).
 int nu_width(double eps);

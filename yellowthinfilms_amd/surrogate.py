@@ -20,8 +20,11 @@ order one over the whole band; P, 1/kz0 and the quasi-static growth
 (1 + k^2/omega^2) are applied exactly.  Nodes are flagged for the exact
 solve near the air light line k = omega (the branch point, where 1/kz0
 diverges), near k = 0 (where the dyadic's angular factors are
-discontinuous), and where the predicted |t| is large (near guided-mode
-poles); training samples exclude the first two neighbourhoods.
+discontinuous), within a few half-widths of a pole of the cavity response
+(_cavity_response's sigma_max indicator, mod_KSpaceFeatures.jl:109-127: the
+poles come from real_axis_poles on the stack, once per (omega, loss)), and
+where the predicted |t| is large; training samples exclude the first two
+neighbourhoods.
 """
 import ctypes as ct
 import math
@@ -217,7 +220,8 @@ class SurrogateGreen:
         self.mlp = MLP(6, width, depth, 18, seed=seed, device=device, kz_features=True)
         self.out_shift = np.zeros(18, np.float32)
         self.out_scale = np.ones(18, np.float32)
-        self.t_ref = 1.0  # typical |t|, for the pole flag
+        self.t_ref = 1.0  # typical |t|, for the response flag
+        self._windows = {}  # (omega, loss, factor) -> (poles, k_rho windows)
 
     # -- data from the exact solver ------------------------------------------
     def exact_scattered(self, kx, ky, hs_nm, ho_nm, omega, loss):
@@ -300,31 +304,72 @@ class SurrogateGreen:
                 print(f"epoch {ep + 1}: loss {hist[-1]:.3e}", flush=True)
         return hist
 
+    # -- the pole indicator -------------------------------------------------------
+    def pole_windows(self, omega, loss, factor=8.0):
+        """k_rho windows [q - factor Gamma, q + factor Gamma] around the poles of
+        the cavity response of the family's stack at (omega, loss):
+        real_axis_poles (mod_KSpaceFeatures.jl:151-214) on _cavity_response
+        (sigma_max of RL, RR and (I - PF RL PB RR)^-1, :109-127) with the source in
+        the air, from half the air light line to the evaluation range's corner,
+        light lines excluded.  The film is gyrotropic about z, so the stack is
+        symmetric about z and the poles are rings: one direction suffices.
+        Returns (poles, windows (m, 2))."""
+        key = (float(omega), float(loss), float(factor))
+        if key not in self._windows:
+            from . import quadrature as Q
+            from .kspace import real_axis_poles
+            st = self.family.stack(loss)
+            ll = Q.dedup(sorted(x for lay in st for x in Q.layer_light_lines(lay, omega, 0.0)))
+            k_hi = math.sqrt(2.0) * self.family.K_RANGE * omega
+            poles = real_axis_poles(st, omega, 1, phis=(0.0,), k_lo=0.5 * omega, k_hi=k_hi, exclude=ll,
+                                    device=self.device)
+            win = np.array([[p["q"] - factor * p["Gamma"], p["q"] + factor * p["Gamma"]] for p in poles],
+                           np.float64).reshape(-1, 2)
+            self._windows[key] = (poles, win)
+        return self._windows[key]
+
+    def set_pole_windows(self, windows):
+        """Load k_rho windows into the fused path's input flags (ytf_mlp_set_pole_windows)."""
+        w = np.ascontiguousarray(np.asarray(windows, np.float64).reshape(-1, 2))
+        _lib.check(_lib.lib().ytf_mlp_set_pole_windows(self.mlp.ctx.handle, self.mlp.handle, w.shape[0],
+                                                       w.ctypes.data_as(ct.POINTER(ct.c_double))),
+                   self.mlp.ctx.handle)
+
     # -- prediction -------------------------------------------------------------
-    def flags(self, kx, ky, t_pred, omega, branch_tol=None, origin_tol=None, pole_factor=3.0):
-        """Nodes to solve exactly: near the air light line, near k = 0, or
-        with a predicted |t| above pole_factor x the 99th percentile of the
-        training targets."""
+    def flags(self, kx, ky, t_pred, omega, branch_tol=None, origin_tol=None, pole_factor=3.0, windows=()):
+        """Nodes to solve exactly: near the air light line, near k = 0, inside a
+        pole window (k_rho in [lo, hi]), or with a predicted |t| above
+        pole_factor x the 99th percentile of the training targets."""
         torch = _torch()
         branch_tol = self.BRANCH_TOL if branch_tol is None else branch_tol
         origin_tol = self.ORIGIN_TOL if origin_tol is None else origin_tol
         kr = torch.sqrt(kx * kx + ky * ky)
         near_branch = (kr - omega).abs() < branch_tol * omega
         near_origin = kr < origin_tol * omega
+        near_pole = torch.zeros_like(near_origin)
+        for lo, hi in np.asarray(windows, np.float64).reshape(-1, 2):
+            near_pole |= (kr >= lo) & (kr <= hi)
         big = torch.linalg.norm(t_pred.double(), dim=1) > pole_factor * self.t_ref
-        return near_branch | near_origin | big
+        return near_branch | near_origin | near_pole | big
 
     def scattered_ghat(self, kx, ky, hs_nm, ho_nm, omega, loss, fallback=True, branch_tol=None, origin_tol=None,
-                       pole_factor=3.0, count=True):
+                       pole_factor=3.0, count=True, pole_window=8.0):
         """g~_s (n, 9) complex on the device; returns (g, flagged mask).
         One call of ytf_mlp_surrogate_ghat: the flags of the nodes near the
         light line or the origin, the network with the exact factors
         (ghat_from_target) and the response flags (`flags`) in one kernel, the
         exact FP64 solve of the input-flagged nodes before it and that of the
         response-flagged nodes after it.  count=True waits for the stream and
-        sets `last_flagged`; count=False only queues the work."""
+        sets `last_flagged`; count=False only queues the work.  pole_window > 0:
+        nodes within that many half-widths of a cavity-response pole
+        (pole_windows) are flagged from their inputs; 0 leaves the poles to the
+        response flag alone."""
         torch = _torch()
         from .green import layer_structs
+        wkey = (float(omega), float(loss), float(pole_window)) if pole_window > 0 else None
+        if wkey != getattr(self, "_loaded_windows", "none"):
+            self.set_pole_windows(self.pole_windows(omega, loss, pole_window)[1] if wkey else np.zeros((0, 2)))
+            self._loaded_windows = wkey
         dev = torch.device("cuda", self.device)
         kx = torch.as_tensor(kx, dtype=torch.float64, device=dev).contiguous()
         ky = torch.as_tensor(ky, dtype=torch.float64, device=dev).contiguous()

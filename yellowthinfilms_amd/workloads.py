@@ -151,9 +151,11 @@ SURROGATE_1024 = dict(
                  "(g ~ U[0,0.5] drawn once): MLP 6->256x4->18 SiLU in BF16 with FP32 accumulation, trained on 2^22 "
                  "exact FP64 solver samples (kx, ky, z, z', omega, loss); evaluated on a 1024x1024 uniform grid "
                  "(|k| <= 3 omega) at z = 40 nm, z' = 50 nm, omega = 2pi, loss = 0.1, with the exact solve on "
-                 "flagged nodes (light line, origin, large response), against the plain GPU solve"),
+                 "flagged nodes (light line, origin, cavity-response pole windows, large response), against the plain GPU "
+                 "solve"),
     family_seed=1, seed=7, data_seed=2, width=256, depth=4, n_train=1 << 22, epochs=8, batch=8192, lr=2e-3,
     f=1.0, L=1024 / 12.0, dx=1.0 / 6.0, h_src_nm=40.0, h_obs_nm=50.0, loss=0.1,
+    pole_window=8.0,  # nodes within this many half-widths of a cavity-response pole take the exact solve
 )
 
 
