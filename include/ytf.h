@@ -449,6 +449,19 @@ typedef struct ytf_operator_options {
    * blocks without z structure; NULL builds them from this build's nodes.
    * They must cover the source planes and every observation layer's planes. */
   const struct ytf_skeletons *skeletons;
+  /* phi-rotation fill (build_cavity_transfer_2d(...; rotation), mod_CavityTransfer.jl:361-378,
+   * with _rotate_modes, mod_ModeSolver.jl:471-482).  rot_n_phi > 0 declares the nodes the
+   * product of rot_n_krho values k_rho (host array rot_krho) and rot_n_phi angles (host array
+   * rot_phi), k_rho fastest: n_nodes = rot_n_krho rot_n_phi and node k + rot_n_krho q sits at
+   * (k_rho_k cos phi_q, k_rho_k sin phi_q) -- the caller keeps kx, ky, w consistent with it.
+   * Every layer must take the closed-form branch (isotropic or uniaxial eps, isotropic mu;
+   * YTF_EINVAL otherwise).  Each k_rho is then solved once, at phi = 0, and every angle is its
+   * rotation, g~(k_rho, phi) = R(phi) g~(k_rho, 0) R(phi)^T with R the rotation about z.
+   * rot_n_phi = 0: every node is solved. */
+  int32_t rot_n_phi;
+  int64_t rot_n_krho;
+  const double *rot_krho;
+  const double *rot_phi;
 } ytf_operator_options;
 
 /* Block kinds (ZBlock.kind, mod_GreenOperator.jl:85-96). */

@@ -506,3 +506,67 @@ def test_grouped_build_in_phi_chunks_equals_one_build():
         print(f"  obs {o}: chunks vs one {relerr(E2[o], E1[o]):.1e}, groups vs concatenated "
               f"{relerr(E1[o], E3[o]):.1e}", flush=True)
         assert relerr(E2[o], E1[o]) < 1e-12 and relerr(E1[o], E3[o]) < 1e-12
+
+
+@pytest.mark.parametrize("label,src,zs,zobs,box", [
+    ("uniaxial 4-layer", 2, [0.2, 0.4, 0.6], {1: [0.3, 0.5], 2: [0.1, 0.3, 0.5, 0.7], 4: [0.25]}, None),
+    ("iso 5-layer lossy", 2, [0.1, 0.3, 0.5], {2: [0.2, 0.4, 0.6], 3: [0.1, 0.2]}, ((0.06, 0.05), 0.1)),
+])
+def test_rotation_fill_against_every_node_solved(label, src, zs, zobs, box):
+    """build_cavity_transfer_2d(...; rotation) on an azimuthally symmetric stack
+    (mod_CavityTransfer.jl:361-378): each k_rho solved once at phi = 0 and turned
+    to every phi gives the operator of the build that solves every node, on
+    angles that do not include 0, with skeleton blocks and box sources; and the
+    oracle (which solves every node) agrees with it at 1e-10."""
+    from yellowthinfilms_amd.green_operator import CavityTransfer2D, build_layer_operator
+    from yellowthinfilms_amd._lib import Context
+    stack = FOUNDATION_STACKS[label]
+    kr, wr = kr_grid(stack, W0, kmax=12.0, width=0.5, order=6)
+    phis, wph = uniform_phi(12)
+    phis = phis + 0.37
+    rot = CavityTransfer2D(stack, W0, src, kr, wr, phis, wph, rotation=True)
+    full = CavityTransfer2D(stack, W0, src, kr, wr, phis, wph, rotation=False)
+    assert rot.rotation and not full.rotation
+    assert CavityTransfer2D(stack, W0, src, kr, wr, phis, wph).rotation  # :auto on a symmetric stack
+    grid = _grid(8, 8, 0.06, 0.05)
+    kw = dict(skeleton=True, skeleton_tol=1e-10)
+    if box is not None:
+        kw.update(box_lateral=box[0], dz_box=box[1], structure="dense")
+    print(f"case: rotation fill {label} src={src} box={box} ({len(kr)} k_rho x {len(phis)} phi)", flush=True)
+    ctx = Context.get(0)
+    ctx.set_timing(True)
+    op_r = build_layer_operator(rot, grid, zs, zobs, **kw)
+    ms_rot, n_rot = ctx.kernel_time("rot_fill")
+    ms_solve_r, _ = ctx.kernel_time("node_solve")
+    ctx.set_timing(True)
+    op_f = build_layer_operator(full, grid, zs, zobs, **kw)
+    ms_solve_f, _ = ctx.kernel_time("node_solve")
+    assert n_rot > 0 and ctx.kernel_time("rot_fill")[1] == 0
+    ctx.set_timing(False)
+    print(f"  node solves {ms_solve_r:.3f} ms + rotation {ms_rot:.3f} ms against {ms_solve_f:.3f} ms", flush=True)
+    assert [(b["obs_layer"], b["kind"]) for b in op_r.blocks] == [(b["obs_layer"], b["kind"]) for b in op_f.blocks]
+    P = _moments(8, 8, len(zs), seed=11, n=6)
+    Er, Ef = _run(op_r, P), _run(op_f, P)
+    for o in zobs:
+        e = relerr(Er[o], Ef[o])
+        print(f"  obs {o}: rotation vs every node {e:.2e}", flush=True)
+        assert e < 1e-13, (o, e)
+    if box is None:
+        ref = _oracle_apply(stack, full, zs, zobs, P, grid)
+        for o in zobs:
+            assert relerr(Er[o], ref[o]) < 1e-10, (o, relerr(Er[o], ref[o]))
+
+
+def test_rotation_fill_errors():
+    from yellowthinfilms_amd.green_operator import CavityTransfer2D, build_layer_operator
+    from yellowthinfilms_amd._lib import YtfError
+    print("case: rotation fill errors", flush=True)
+    stack = FOUNDATION_STACKS["uniaxial 4-layer"]
+    kr, wr = kr_grid(stack, W0, kmax=6.0, width=0.5, order=4)
+    phis, wph = uniform_phi(6)
+    c = CavityTransfer2D(stack, W0, 2, kr, wr, phis, wph, rotation=True)
+    c.krho_nodes = c.krho_nodes[:-1]  # no longer the product of the node set
+    with pytest.raises(YtfError, match="rotation fill needs n_nodes"):
+        build_layer_operator(c, _grid(8, 8, 0.06, 0.05), [0.2, 0.4], {2: [0.3]})
+    with pytest.raises(ValueError, match="requires an azimuthally symmetric stack"):
+        CavityTransfer2D(FOUNDATION_STACKS["biaxial 4-layer"], W0, 2, kr, wr, phis, wph, rotation=True)

@@ -114,3 +114,14 @@ def test_spectral_apply_is_circular_convolution():
             sh = np.roll(np.roll(J, x, axis=0), y, axis=1)  # J[(i - x), (j - y)]
             Eb += np.einsum("cq,ijq->ijc", G[:, :, x, y], sh)
     assert relerr(E, Eb) < 1e-12
+
+
+def test_rotation_fill_is_refused_on_a_stack_without_symmetry():
+    """build_cavity_transfer_2d(...; rotation=true) on a stack with a biaxial
+    layer errors before any device work (mod_CavityTransfer.jl:322-324)."""
+    from helpers import FOUNDATION_STACKS
+    from yellowthinfilms_amd.green_operator import CavityTransfer2D
+    with pytest.raises(ValueError, match="requires an azimuthally symmetric stack"):
+        CavityTransfer2D(FOUNDATION_STACKS["biaxial 4-layer"], 6.28, 2, [1.0], [1.0], [0.0], [1.0], rotation=True)
+    with pytest.raises(ValueError, match="φ nodes/weights length mismatch"):
+        CavityTransfer2D(FOUNDATION_STACKS["iso 3-layer vac/glass/vac"], 6.28, 2, [1.0], [1.0], [0.0], [1.0, 2.0])

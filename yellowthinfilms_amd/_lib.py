@@ -35,7 +35,9 @@ class YtfOperatorOptions(ct.Structure):
     _fields_ = [("structure", ct.c_int32), ("check", ct.c_int32), ("check_tol", ct.c_double),
                 ("skeleton", ct.c_int32), ("skeleton_tol", ct.c_double), ("lateral", ct.c_int32),
                 ("nufft_eps", ct.c_double), ("uniform_z", ct.c_int32), ("dz_box", ct.c_double),
-                ("ez_local", ct.c_int32), ("skeletons", ct.c_void_p)]
+                ("ez_local", ct.c_int32), ("skeletons", ct.c_void_p), ("rot_n_phi", ct.c_int32),
+                ("rot_n_krho", ct.c_int64), ("rot_krho", ct.POINTER(ct.c_double)),
+                ("rot_phi", ct.POINTER(ct.c_double))]
 
 
 class YtfBlockInfo(ct.Structure):

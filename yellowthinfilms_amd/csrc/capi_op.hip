@@ -109,7 +109,44 @@ struct Build {
   cd *wr = nullptr;
   double *xs = nullptr, *ys = nullptr;
   cd *twx = nullptr, *twy = nullptr;
+  // phi-rotation fill: the n nodes are rot_nk k_rho x rot_nphi angles, k_rho
+  // fastest; rot_kr: k_rho then rot_nk zeros (the phi = 0 nodes), rot_cs:
+  // cos then sin of the angles (device)
+  int64_t rot_nk = 0;
+  int rot_nphi = 0;
+  const double *rot_kr = nullptr, *rot_cs = nullptr;
 };
+
+// Every layer takes the closed-form branch, so the whole problem is invariant
+// under rotations about z (_is_azimuthally_symmetric_stack,
+// mod_ModeSolver.jl:426-434).
+bool azimuthally_symmetric(const ytf_layer *L, int nl) {
+  for (int l = 0; l < nl; ++l)
+    if (!((L[l].eps_kind == YTF_EPS_ISOTROPIC || L[l].eps_kind == YTF_EPS_UNIAXIAL) && L[l].mu_kind == YTF_MU_ISOTROPIC))
+      return false;
+  return true;
+}
+
+// g~ [np][9][n] of every node of the build at np depth pairs: one node solve,
+// or with the rotation fill one solve per k_rho at phi = 0 and every angle its
+// rotation (build_cavity_transfer_2d's rotation branch, mod_CavityTransfer.jl:361-378).
+int solve_nodes(ytf_ctx *c, const Build &B, int obs_layer, int np, const double *zs, const double *zo, cd *g,
+                int terms) {
+  if (B.rot_nphi == 0)
+    return green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, B.n, B.src, obs_layer, np, zs, zo, (double *)g,
+                              nullptr, terms, B.dz, B.ez_local);
+  DevBuf g0;
+  int rc = dev_alloc(c, g0, sizeof(cd) * 9 * (size_t)np * (size_t)std::max<int64_t>(B.rot_nk, 1));
+  if (rc) return rc;
+  rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.rot_kr, B.rot_kr + B.rot_nk, B.rot_nk, B.src, obs_layer, np,
+                          zs, zo, (double *)g0.p, nullptr, terms, B.dz, B.ez_local);
+  if (rc) return rc;
+  hipError_t e = timed(c, "rot_fill", [&] {
+    return ytf::launch_rot_fill(B.rot_nk, B.rot_nphi, np, (const cd *)g0.p, B.rot_cs, g, c->stream);
+  });
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "operator build: rotation fill: %s", hipGetErrorString(e));
+  return YTF_OK;
+}
 
 struct Req {
   double zs, zo;  // local source and observation depths
@@ -139,8 +176,7 @@ int fill_planes(ytf_ctx *c, const Build &B, int obs_layer, const std::vector<Req
       zs[p] = reqs[r0 + p].zs;
       zo[p] = reqs[r0 + p].zo;
     }
-    rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, B.n, B.src, obs_layer, np, zs.data(), zo.data(),
-                            (double *)g, nullptr, terms, B.dz, B.ez_local);
+    rc = solve_nodes(c, B, obs_layer, np, zs.data(), zo.data(), g, terms);
     if (rc) return rc;
     hipError_t e = hipSuccess;
     if (B.lateral == YTF_LATERAL_NUFFT) {
@@ -207,8 +243,7 @@ int check_structure(ytf_ctx *c, const Build &B, const std::vector<double> &zs, c
   double xy[5] = {xs[0], xs[1], xs[2], ys[0], ys[1]};
   rc = upload(c, xb.p, xy, sizeof xy);
   if (rc) return rc;
-  rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, B.n, B.src, B.src, 4, zsv, zov, (double *)gb.p,
-                          nullptr, YTF_TERMS_SCATTERED, B.dz, B.ez_local);
+  rc = solve_nodes(c, B, B.src, 4, zsv, zov, (cd *)gb.p, YTF_TERMS_SCATTERED);
   if (rc) return rc;
   const double *xd = (const double *)xb.p;
   hipError_t e = ytf::launch_lat_direct(B.n, B.kx, B.ky, B.W, 4, (const cd *)gb.p, xd, 3, xd + 3, 2, (cd *)ob.p, 9,
@@ -499,7 +534,11 @@ int obs_skeleton(ytf_ctx *c, const ytf_layer *layers, int nl, int obs, int64_t n
 // mod_GreenOperator.jl:315-359).
 int side_skeletons(ytf_ctx *c, const Build &B, int obs, const std::vector<double> &zs_int,
                    const std::vector<double> &zo, double tol, Skel *src_sk, bool need_src, Skel *obs_sk) {
-  const int64_t n = B.n;
+  // the mode wavenumbers do not depend on the angle: with the rotation fill,
+  // each k_rho once
+  const bool rot = B.rot_nphi > 0;
+  const int64_t n = rot ? B.rot_nk : B.n;
+  const double *kx = rot ? B.rot_kr : B.kx, *ky = rot ? B.rot_kr + B.rot_nk : B.ky;
   DevBuf qb;
   int rc = dev_alloc(c, qb, sizeof(cd) * 20 * (size_t)std::max<int64_t>(n, 1));
   if (rc) return rc;
@@ -507,7 +546,7 @@ int side_skeletons(ytf_ctx *c, const Build &B, int obs, const std::vector<double
   std::memset(&nc, 0, sizeof nc);
   nc.Qsrc_dev = (double *)qb.p;
   nc.basis_dev = (double *)((cd *)qb.p + 4 * n);
-  rc = green_nodes_locked(c, B.layers, B.nl, B.omega, B.kx, B.ky, n, B.src, obs, 0, nullptr, nullptr, nullptr, &nc,
+  rc = green_nodes_locked(c, B.layers, B.nl, B.omega, kx, ky, n, B.src, obs, 0, nullptr, nullptr, nullptr, &nc,
                           YTF_TERMS_ALL, 0.0, 1);
   if (rc) return rc;
   if (need_src) {
@@ -759,6 +798,14 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
     return fail(c, YTF_EINVAL, "ytf_operator_build: bad nodes");
   if (source_layer < 1 || source_layer > n_layers - 1)
     return fail(c, YTF_EINVAL, "ytf_operator_build: source_layer=%d must be in 1:%d", source_layer, n_layers - 1);
+  if (opt->rot_n_phi < 0 ||
+      (opt->rot_n_phi > 0 && (opt->rot_n_krho < 0 || !opt->rot_krho || !opt->rot_phi ||
+                              opt->rot_n_krho > INT64_MAX / opt->rot_n_phi ||
+                              opt->rot_n_krho * (int64_t)opt->rot_n_phi != n_nodes)))
+    return fail(c, YTF_EINVAL, "ytf_operator_build: the rotation fill needs n_nodes = rot_n_krho x rot_n_phi and both "
+                               "axes");
+  if (opt->rot_n_phi > 0 && !azimuthally_symmetric(layers, n_layers))
+    return fail(c, YTF_EINVAL, "build_cavity_transfer_2d: rotation=true requires an azimuthally symmetric stack");
   const bool skeleton = opt->skeleton && opt->structure != YTF_STRUCTURE_DENSE;
   // the reference requires uniform z grids (errors otherwise)
   double ds = 0.0;
@@ -861,6 +908,25 @@ int ytf_operator_build(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
                                             B.wr, c->stream);
       if (e != hipSuccess) return bail(fail(c, YTF_EHIP, "operator build: %s", hipGetErrorString(e)));
     }
+  }
+
+  DevBuf rotb;
+  if (opt->rot_n_phi > 0) {
+    const int64_t nk = opt->rot_n_krho;
+    const int nphi = opt->rot_n_phi;
+    std::vector<double> h(2 * (size_t)nk + 2 * (size_t)nphi, 0.0);
+    std::copy(opt->rot_krho, opt->rot_krho + nk, h.begin());
+    for (int q = 0; q < nphi; ++q) {
+      h[2 * nk + q] = std::cos(opt->rot_phi[q]);
+      h[2 * nk + nphi + q] = std::sin(opt->rot_phi[q]);
+    }
+    rc = dev_alloc(c, rotb, sizeof(double) * h.size());
+    if (!rc) rc = upload(c, rotb.p, h.data(), sizeof(double) * h.size());
+    if (rc) return bail(rc);
+    B.rot_nk = nk;
+    B.rot_nphi = nphi;
+    B.rot_kr = (const double *)rotb.p;
+    B.rot_cs = B.rot_kr + 2 * nk;
   }
 
   Skel src_sk;

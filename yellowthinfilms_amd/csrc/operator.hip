@@ -443,6 +443,43 @@ __global__ void k_op_add(int64_t n, const cd *__restrict__ src, cd *__restrict__
   if (k < n) dst[k] = dst[k] + src[k];
 }
 
+// phi-rotation fill of an azimuthally symmetric stack: g0 [np][9][nk] solved
+// at phi = 0, g [np][9][nk*nphi] with node k + nk*q at angle phi_q,
+// g(phi) = R g0 R^T, R = [[c, -s, 0], [s, c, 0], [0, 0, 1]]; entry a + 3b of a
+// node is row a, column b (both indices are vectors and turn alike).  cs:
+// cos phi_q at [q], sin phi_q at [nphi + q].
+__global__ void k_rot_fill(int64_t nk, int nphi, int np, const cd *__restrict__ g0, const double *__restrict__ cs,
+                           cd *__restrict__ g) {
+  const int64_t n = nk * nphi;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * np) return;
+  const int64_t p = t / n, node = t - p * n;
+  const int64_t q = node / nk, k = node - q * nk;
+  const double c = cs[q], s = cs[nphi + q];
+  const cd *a = g0 + p * 9 * nk + k;
+  cd G[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) G[e] = a[e * nk];
+  // columns first: H = G R^T (column b of H = sum_j G[., j] R[b][j])
+  cd H[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const cd gx = G[r], gy = G[r + 3];
+    H[r] = rmul(c, gx) - rmul(s, gy);
+    H[r + 3] = rmul(s, gx) + rmul(c, gy);
+    H[r + 6] = G[r + 6];
+  }
+  // then rows: out = R H
+  cd *o = g + p * 9 * n + node;
+#pragma unroll
+  for (int b = 0; b < 3; ++b) {
+    const cd hx = H[3 * b], hy = H[3 * b + 1];
+    o[(3 * b) * n] = rmul(c, hx) - rmul(s, hy);
+    o[(3 * b + 1) * n] = rmul(s, hx) + rmul(c, hy);
+    o[(3 * b + 2) * n] = H[3 * b + 2];
+  }
+}
+
 // A stored plane [9][x][y] (node y + py*x) to the reference layout
 // (Px, Py, 3, 3) = out[x + px*(y + py*e)].
 __global__ void k_op_export(int px, int py, const cd *__restrict__ slot, cd *__restrict__ out) {
@@ -605,6 +642,12 @@ hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *o
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_op_add, dim3(nblk(n, 256)), dim3(256), 0, s, n, src, dst);
+  return hipGetLastError();
+}
+hipError_t launch_rot_fill(int64_t nk, int nphi, int np, const cd *g0, const double *cs, cd *g, hipStream_t s) {
+  const int64_t n = nk * nphi * (int64_t)np;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rot_fill, dim3(nblk(n, 256)), dim3(256), 0, s, nk, nphi, np, g0, cs, g);
   return hipGetLastError();
 }
 hipError_t launch_op_export(int px, int py, const cd *slot, cd *out, hipStream_t s) {

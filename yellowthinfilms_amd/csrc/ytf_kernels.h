@@ -242,6 +242,9 @@ hipError_t launch_op_mix(int64_t nn, const cd *J, int ns, cd *E, int no, const c
 hipError_t launch_op_zcomb(int64_t nn, const cd *in, int nin, const cd *M, cd *out, int nout, int accumulate,
                            hipStream_t s);
 hipError_t launch_op_add(int64_t n, const cd *src, cd *dst, hipStream_t s);
+// g [np][9][nk*nphi] = R(phi_q) g0[p][.][k] R(phi_q)^T at node k + nk*q
+// (g0 [np][9][nk]; cs: cos phi at [q], sin phi at [nphi + q]).
+hipError_t launch_rot_fill(int64_t nk, int nphi, int np, const cd *g0, const double *cs, cd *g, hipStream_t s);
 hipError_t launch_opy_mix1(int PY, const cd *in, cd *out, int px, const cd *K, int ny_in, int ny_out, const cd *tw,
                            hipStream_t s);
 // The fused single-plane y pass exists for power-of-two PY 8..2048 with px a

@@ -68,22 +68,35 @@ class CavityTransfer2D:
     """The (k_rho, phi) nodes of a CavityTransferCache2D for one stack and
     source layer: kx = k_rho cos(phi), ky = k_rho sin(phi), flattened with
     k_rho fastest (_node_kxky, mod_ClosedFormField.jl:526-528), and the node
-    weights W = k_rho w_rho w_phi / 4 pi^2 (:723-724)."""
+    weights W = k_rho w_rho w_phi / 4 pi^2 (:723-724).
+
+    rotation ("auto", True, False; build_cavity_transfer_2d, mod_CavityTransfer.jl:296-306):
+    on an azimuthally symmetric stack every k_rho is solved once, at phi = 0,
+    and the builds turn it to every phi (mod_CavityTransfer.jl:361-378); "auto"
+    takes it iff the stack is symmetric, True on another stack is an error,
+    False solves every node."""
 
     def __init__(self, stack, omega, source_layer, krho_nodes, krho_weights, phi_nodes, phi_weights,
-                 obs_layers=None, device=0):
+                 obs_layers=None, device=0, rotation="auto"):
+        from .kspace import _is_azimuthally_symmetric
         kr = np.asarray(krho_nodes, np.float64)
         wr = np.asarray(krho_weights, np.float64)
         ph = np.asarray(phi_nodes, np.float64)
         wp = np.asarray(phi_weights, np.float64)
         if kr.shape != wr.shape or ph.shape != wp.shape:
-            raise ValueError("build_cavity_transfer_2d: nodes and weights must have equal lengths")
+            raise ValueError("build_cavity_transfer_2d: kρ nodes/weights length mismatch" if kr.shape != wr.shape
+                             else "build_cavity_transfer_2d: φ nodes/weights length mismatch")
+        sym = _is_azimuthally_symmetric(stack)
+        use_rot = sym if (isinstance(rotation, str) and rotation == "auto") else bool(rotation)
+        if use_rot and not sym:
+            raise ValueError("build_cavity_transfer_2d: rotation=true requires an azimuthally symmetric stack")
         K, Ph = np.meshgrid(kr, ph, indexing="ij")
         kx = (K * np.cos(Ph)).ravel(order="F")
         ky = (K * np.sin(Ph)).ravel(order="F")
         W = (np.outer(kr * wr, wp) / (4 * np.pi ** 2)).ravel(order="F")
         self._init(stack, omega, source_layer, kx, ky, W, obs_layers, device)
         self.krho_nodes, self.krho_weights, self.phi_nodes, self.phi_weights = kr, wr, ph, wp
+        self.rotation = bool(use_rot and len(kr) > 0 and len(ph) > 0)
 
     @classmethod
     def from_nodes(cls, stack, omega, source_layer, kx, ky, W, obs_layers=None, device=0):
@@ -92,7 +105,16 @@ class CavityTransfer2D:
         self = cls.__new__(cls)
         self._init(stack, omega, source_layer, np.asarray(kx, np.float64), np.asarray(ky, np.float64),
                    np.asarray(W), obs_layers, device)
+        self.rotation = False
         return self
+
+    def rotation_fields(self):
+        """(rot_n_phi, rot_n_krho, rot_krho, rot_phi) of ytf_operator_options."""
+        if not getattr(self, "rotation", False):
+            return 0, 0, None, None
+        kr = np.ascontiguousarray(self.krho_nodes)
+        ph = np.ascontiguousarray(self.phi_nodes)
+        return len(ph), len(kr), kr.ctypes.data_as(_DP), ph.ctypes.data_as(_DP)
 
     def _init(self, stack, omega, source_layer, kx, ky, W, obs_layers, device):
         torch = _torch()
@@ -388,9 +410,15 @@ def cache_skeletons(cache, z_src, z_obs_by_layer, tol=1e-6):
     """The z skeletons of a cache's source side and of every observation
     layer of z_obs_by_layer, from the mode wavenumbers at the cache's nodes
     (_cache_skeletons, mod_GreenOperator.jl:361-363): what build_layer_operator
-    builds for itself.  z_src in the caller's order (Dipole.z)."""
-    return _z_skeletons(cache.stack, cache.omega, cache.source_layer, cache.kx_host, cache.ky_host, z_src,
-                        z_obs_by_layer, tol, cache.device)
+    builds for itself.  z_src in the caller's order (Dipole.z).  Under the
+    rotation fill the wavenumbers do not depend on phi, and each k_rho counts
+    once (at phi = 0), as in the build."""
+    if getattr(cache, "rotation", False):
+        kx, ky = cache.krho_nodes, np.zeros_like(cache.krho_nodes)
+    else:
+        kx, ky = cache.kx_host, cache.ky_host
+    return _z_skeletons(cache.stack, cache.omega, cache.source_layer, kx, ky, z_src, z_obs_by_layer, tol,
+                        cache.device)
 
 
 def _q4(Q):
@@ -463,7 +491,7 @@ def _build(cache, grid, z_src, z_obs_by_layer, structure, check, check_tol, skel
     opt = _lib.YtfOperatorOptions(_lib.STRUCTURE[structure], int(bool(check)), float(check_tol),
                                   int(bool(skeleton)), float(skeleton_tol), _lateral(lateral), float(nufft_eps),
                                   int(bool(uniform_z)), float(dz_box), int(bool(ez_local)),
-                                  skeletons.handle if skeletons is not None else None)
+                                  skeletons.handle if skeletons is not None else None, *cache.rotation_fields())
     W = cache.weights(box_lateral)
     ctx = _lib.Context.get(cache.device)
     ctx.bind_stream()
