@@ -31,7 +31,9 @@ Workloads (yellowthinfilms_amd.workloads):
   operator_volume_64: the reference's own operator benchmark at size L
     (Bench_GreenOperator.jl, iso vac/glass/vac): 16 box-source planes of
     layer 2 onto the 16 planes of each of layers 1-3 on a 64 x 64 grid
-    (padded 128^2), z skeletons on; one step = one apply_operator of a full
+    (padded 128^2), z skeletons on, built on the benchmark's own grid
+    (box_krho_grid_grouped, N_phi 256, rtol 1e-3, k_rho cap 3000) with the
+    phi-rotation fill; one step = one apply_operator of a full
     64 x 64 x 16 x 3 moment volume.  value = operator applies/s;
     vs_baseline against the reference's 90 ms per apply.
   pole_scan_hyperbolic: the real-axis pole search of a source in the
@@ -742,10 +744,25 @@ def cpu_operator(stack, cache, op, P, grid, wl):
              "sample": f"{reps} applies of the oracle's FFT route (numpy FFT) on the GPU-built Ĝ"}, rel, rel_apply)
 
 
+def _group_nodes(groups, box):
+    """Host kx, ky and box-weighted node weights of a list of (k_rho, phi)
+    groups, in the layout CavityTransfer2D gives each group (k_rho fastest)."""
+    from yellowthinfilms_amd.green_operator import box_lateral_factor
+    kx, ky, w = [], [], []
+    for g in groups:
+        K, Ph = np.meshgrid(g["krho"], g["phi"], indexing="ij")
+        kx.append((K * np.cos(Ph)).ravel(order="F"))
+        ky.append((K * np.sin(Ph)).ravel(order="F"))
+        w.append((np.outer(g["krho"] * g["wrho"], g["wphi"]) / (4 * np.pi ** 2)).ravel(order="F"))
+    kx, ky, w = (np.concatenate(v) for v in (kx, ky, w))
+    return kx, ky, w * box_lateral_factor(kx, ky, *box)
+
+
 def bench_operator_volume(a):
     import torch
     from yellowthinfilms_amd import workloads as W, uniform_kxky_grid
-    from yellowthinfilms_amd.green_operator import CavityTransfer2D, build_layer_operator, apply_operator
+    from yellowthinfilms_amd.green_operator import build_layer_operator_grouped, apply_operator
+    from yellowthinfilms_amd.kspace import _is_azimuthally_symmetric
     from yellowthinfilms_amd._lib import Context
 
     wl = W.OPERATOR_VOLUME_64
@@ -755,13 +772,21 @@ def bench_operator_volume(a):
     zs, zobs = W.operator_volume_planes(wl)
     Ns = len(zs)
     dev = torch.device("cuda", 0)
-    cache = CavityTransfer2D(stack, W.OMEGA, wl["source_layer"], *W.operator_volume_nodes(wl))
+    t0 = time.perf_counter()
+    groups, ginfo = W.operator_volume_groups(wl)
+    t_grid = time.perf_counter() - t0
     kw = dict(lateral="nufft", nufft_eps=wl["eps"], skeleton=True, skeleton_tol=wl["skeleton_tol"],
               box_lateral=(wl["dx"], wl["dx"]), dz_box=wl["dx"])
+    ctx = Context.get(0)
+    ctx.set_timing(True)
     t0 = time.perf_counter()
-    op = build_layer_operator(cache, grid, zs, zobs, **kw)
+    op, t_cache, t_build = build_layer_operator_grouped(stack, W.OMEGA, wl["source_layer"], groups, grid, zs, zobs,
+                                                        node_budget=wl["node_budget"], **kw)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
+    build_kernels = {k: ctx.kernel_time(k) for k in ("node_solve", "rot_fill", "nufft_sort", "nufft_spread",
+                                                      "nufft_fft")}
+    ctx.set_timing(False)
     rng = np.random.default_rng(wl["seed"])
     P = torch.as_tensor((rng.standard_normal((3, Ns, Ny, Nx)) + 1j * rng.standard_normal((3, Ns, Ny, Nx)))
                         / math.sqrt(2.0), device=dev)  # full volume, column-major (Nx, Ny, Ns, 3)
@@ -769,7 +794,6 @@ def bench_operator_volume(a):
     def step():
         apply_operator(op, P)
 
-    ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
     ctx.set_timing(True)
@@ -784,13 +808,16 @@ def bench_operator_volume(a):
     mix_ms = t["op_mix"][0] / a.steps
     cpu, rel_err, rel_apply = None, None, None
     if not a.no_cpu_baseline:
-        cpu, rel_err, rel_apply = cpu_operator_volume(stack, cache, op, P, grid, wl, zs, zobs, kw)
+        cpu, rel_err, rel_apply = cpu_operator_volume(stack, groups, op, P, grid, wl, zs, zobs, kw)
     value = a.steps / wall
     out = _base(a, wl, value, "operator applies/s (64x64x16x3 volume onto 3 x 16 planes, padded 128^2)", wall,
                 "f64")
     out["vs_baseline"] = value / wl["reference_applies_per_s"]
     out["config"].update({"grid": [Nx, Ny], "padded": [2 * Nx, 2 * Ny], "source_planes": Ns,
-                          "obs_planes": {b["obs_layer"]: b["n_obs"] for b in op.blocks}, "nodes": cache.n_nodes,
+                          "obs_planes": {b["obs_layer"]: b["n_obs"] for b in op.blocks},
+                          "krho_x_phi": [ginfo["nkrho"], wl["n_phi"]], "nodes": ginfo["n_nodes"],
+                          "phi_groups": ginfo["n_groups"], "any_stalled": ginfo["any_stalled"],
+                          "rotation_fill": bool(_is_azimuthally_symmetric(stack)),
                           "blocks": {b["obs_layer"]: [b["kind"], b["n_src_skel"], b["n_obs_skel"]]
                                      for b in op.blocks},
                           "lateral": "nufft", "nufft_eps": wl["eps"], "skeleton_tol": wl["skeleton_tol"]})
@@ -799,7 +826,9 @@ def bench_operator_volume(a):
         "reference_note": "apply_operator at size L, iso vac/glass/vac: 90 ms on an Apple M5 with 4 Julia threads "
                           "(STATUS_closed_form_rewrite.md:75); vs_baseline = value / 11.1",
         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in t.items()},
+        "grid_s": t_grid,
         "build_s": build_s,
+        "build_kernel_ms": {k: v[0] for k, v in build_kernels.items()},
         "ghat_bytes": ghat_bytes,
         "rel_err_vs_fp64_oracle": rel_err,
         "rel_err_apply_vs_oracle_fft_route": rel_apply,
@@ -830,99 +859,80 @@ def _export_blocks(op):
     return out
 
 
-def _volume_apply_port(blocks, Jh):
-    """The apply of mod_GreenOperator.jl:674-712 restated in numpy on the
-    GPU-built blocks: per spectral node E_j = sum_i G_ij J_i with G_ij from
-    the block's Toeplitz (j - i), Hankel (i + j), dense (i, j) or skeleton
-    (Ps D Po^T) planes.  Jh: (Ns, 3, Px, Py) spectra of the padded moments.
-    Returns {obs_layer: (No, 3, Px, Py) spectra}."""
-    Ns = Jh.shape[0]
-    out = {}
-    for b in blocks:
-        No, T, H, D = b["No"], b["T"], b["H"], b["D"]
-        E = np.zeros((No, 3) + Jh.shape[2:], np.complex128)
-        if T or H:
-            for j in range(No):
-                for i in range(Ns):
-                    G = (T[j - i + Ns - 1] if T else 0) + (H[i + j] if H else 0)
-                    E[j] += np.einsum("cqxy,qxy->cxy", G, Jh[i])
-        if D:
-            Js = np.einsum("ia,icxy->acxy", b["Ps"], Jh) if b["skeleton"] else Jh
-            rs, ro = (b["Ps"].shape[1], b["Po"].shape[1]) if b["skeleton"] else (Ns, No)
-            Es = np.zeros((ro, 3) + Jh.shape[2:], np.complex128)
-            for bo in range(ro):
-                for ai in range(rs):
-                    Es[bo] += np.einsum("cqxy,qxy->cxy", D[ai + rs * bo], Js[ai])
-            E += np.einsum("jb,bcxy->jcxy", b["Po"], Es) if b["skeleton"] else Es
-        out[b["obs_layer"]] = E
-    return out
-
-
-def cpu_operator_volume(stack, cache, op, P, grid, wl, zs, zobs, kw):
-    """(1) Four seeded voxel dipoles: the GPU field at 48 seeded (pixel,
-    plane) points of every observation layer against the oracle's exact
-    lateral sum of its own box spectral dyadics (mod_ClosedFormField.jl:
-    743-753, 815-862) over the same nodes.  (2) The apply restated in numpy
-    on the GPU-built blocks: timed on the benchmark volume, and compared with
-    the GPU apply."""
+def cpu_operator_volume(stack, groups, op, P, grid, wl, zs, zobs, kw):
+    """(1) Two seeded voxel dipoles: the GPU field at 6 seeded (pixel, plane)
+    points of every observation layer against the oracle's exact lateral sum
+    of its own box spectral dyadics (mod_ClosedFormField.jl:743-753, 815-862)
+    over every node of the grid (no rotation: the oracle solves each node).
+    (2) The apply on all cores: the GPU-built planes read back, the lateral
+    FFTs by scipy.fft on every core and the z coupling by the oracle's OpenMP
+    ora_op_zmix; timed on the benchmark volume and compared with the GPU apply."""
     import torch
+    from yellowthinfilms_amd import workloads as W
     from yellowthinfilms_amd.green_operator import apply_operator
     try:
+        import scipy.fft as sfft
         from oracle import pyoracle as po
         from yellowthinfilms_amd.materials import stack_arrays
     except Exception as exc:
         return ({"value": None, "unit": "operator applies/s", "cores": 0, "kind": "port",
                  "sample": f"oracle unavailable: {exc}"}, None, None)
+    cores = _cores()
     Nx, Ny = grid["Nx"], grid["Ny"]
     Ns = len(zs)
+    kx, ky, Wn = _group_nodes(groups, kw["box_lateral"])
     rng = np.random.default_rng(3)
     dip = [(int(rng.integers(Nx)), int(rng.integers(Ny)), int(rng.integers(Ns)),
-            rng.normal(size=3) + 1j * rng.normal(size=3)) for _ in range(4)]
+            rng.normal(size=3) + 1j * rng.normal(size=3)) for _ in range(2)]
     Pv = np.zeros((3, Ns, Ny, Nx), np.complex128)
     for i, j, k, p in dip:
         Pv[:, k, j, i] += p
     Ev = {o: e.cpu().numpy() for o, e in apply_operator(op, torch.as_tensor(Pv, device=P.device)).items()}
     st = po.Stack(*stack_arrays(stack))
-    W = cache.weights(kw["box_lateral"])
-    W = W.cpu().numpy() if hasattr(W, "cpu") else np.asarray(W)
     got, ref = [], []
     for o, zo in zobs.items():
-        pts = [(int(rng.integers(Nx)), int(rng.integers(Ny)), int(rng.integers(len(zo)))) for _ in range(16)]
+        pts = [(int(rng.integers(Nx)), int(rng.integers(Ny)), int(rng.integers(len(zo)))) for _ in range(6)]
         pairs = sorted({(k, m) for (_, _, k, _) in dip for (_, _, m) in pts})
-        g = po.ghat_batch(st, cache.omega, cache.kx_host, cache.ky_host, wl["source_layer"], o,
-                          [zs[k] for k, _ in pairs], [zo[m] for _, m in pairs], nthreads=_cores(),
-                          dz_box=kw["dz_box"])
+        g = po.ghat_batch(st, W.OMEGA, kx, ky, wl["source_layer"], o, [zs[k] for k, _ in pairs],
+                          [zo[m] for _, m in pairs], nthreads=cores, dz_box=kw["dz_box"])
         for (x, y, m) in pts:
             e = np.zeros(3, np.complex128)
             for (i, j, k, p) in dip:
-                G = po.lateral_sum_points(g[:, pairs.index((k, m))], W, cache.kx_host, cache.ky_host,
+                G = po.lateral_sum_points(g[:, pairs.index((k, m))], Wn, kx, ky,
                                           [((x - i) * grid["dx"], (y - j) * grid["dy"])])[0]
                 e += G @ p
             ref.append(e)
             got.append(Ev[o][:, m, y, x])
     rel = float(np.linalg.norm(np.array(got) - np.array(ref)) / np.linalg.norm(np.array(ref)))
-    # the apply, restated in numpy, on the benchmark volume
+    # the apply on every core, on the benchmark volume
     Px, Py = 2 * Nx, 2 * Ny
+    nxy = Px * Py
     Pn = P.cpu().numpy()  # (3, Ns, Ny, Nx)
     Eg = {o: e.cpu().numpy() for o, e in apply_operator(op, P).items()}
-    blocks = _export_blocks(op)
+    blocks = [{k: (np.ascontiguousarray(np.array(v).reshape(len(v), 3, 3, nxy)) if k in ("T", "H", "D") and v else v)
+               for k, v in b.items()} for b in _export_blocks(op)]
     reps, t = 0, time.perf_counter()
     while True:
         Jp = np.zeros((Ns, 3, Px, Py), np.complex128)
         Jp[:, :, :Nx, :Ny] = Pn.transpose(1, 0, 3, 2)
-        Jh = np.fft.fft2(Jp, axes=(2, 3))
-        Eh = _volume_apply_port(blocks, Jh)
-        Ec = {o: (np.fft.ifft2(e, axes=(2, 3)) * (Px * Py))[:, :, :Nx, :Ny] for o, e in Eh.items()}
+        Jh = sfft.fft2(Jp, axes=(2, 3), workers=cores).reshape(Ns, 3, nxy)
+        Ec = {}
+        for b in blocks:
+            E = po.op_zmix(nxy, Jh, b["No"], T=b["T"] if len(b["T"]) else None, H=b["H"] if len(b["H"]) else None,
+                           D=b["D"] if len(b["D"]) else None, Ps=b.get("Ps"), Po=b.get("Po"), nthreads=cores)
+            Ec[b["obs_layer"]] = (sfft.ifft2(E.reshape(b["No"], 3, Px, Py), axes=(2, 3), workers=cores)
+                                  * nxy)[:, :, :Nx, :Ny]
         reps += 1
-        if time.perf_counter() - t > 20.0 or reps >= 3:
+        if time.perf_counter() - t > 20.0 or reps >= 5:
             break
     dt = (time.perf_counter() - t) / reps
     num = sum(np.linalg.norm(Eg[o].transpose(1, 0, 3, 2) - Ec[o]) ** 2 for o in Ec)
     den = sum(np.linalg.norm(Ec[o]) ** 2 for o in Ec)
     rel_apply = float(math.sqrt(num / den))
-    return ({"value": 1.0 / dt, "unit": "operator applies/s", "cores": 1, "kind": "port",
-             "sample": f"{reps} applies of the numpy restatement of apply_operator on the GPU-built planes "
-                       "(read back beforehand) and the benchmark volume"}, rel, rel_apply)
+    return ({"value": 1.0 / dt, "unit": "operator applies/s", "cores": cores, "kind": "port",
+             "sample": f"{reps} applies on the benchmark volume: the GPU-built planes read back, lateral FFTs by "
+                       f"scipy.fft on {cores} workers, z coupling by the oracle's OpenMP ora_op_zmix"},
+            rel, rel_apply)
 
 
 def bench_pole_scan(a):

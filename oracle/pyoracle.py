@@ -582,3 +582,26 @@ def real_axis_poles(st, omega, s, phis=(0.0,), k_lo=None, k_hi=None, scan_step=1
             G = min(q - cross(q, kl), cross(q, kr) - q)
             out.append((q, max(G, 1e-12 * q), float(phi), prom))
     return out
+
+
+def op_zmix(nxy, J, no, T=None, H=None, D=None, Ps=None, Po=None, E=None, nthreads=0):
+    """The z coupling of one operator block at every spectral node
+    (ora_op_zmix): J (ns, 3, nxy) complex; T, H, D: plane stacks (slots, 3, 3,
+    nxy) or None; Ps (ns, rs), Po (no, ro) for a skeleton block.  Returns E
+    (no, 3, nxy), accumulated into E when given."""
+    J = np.ascontiguousarray(J, np.complex128)
+    ns = J.shape[0]
+    E = np.zeros((no, 3, nxy), np.complex128) if E is None else E
+    assert E.flags.c_contiguous and E.dtype == np.complex128
+
+    def arr(a):
+        return None if a is None else np.ascontiguousarray(a, np.complex128)
+    T, H, D, Ps, Po = arr(T), arr(H), arr(D), arr(Ps), arr(Po)
+    ptr = lambda a: None if a is None else _p(a.view(np.float64))
+    rs = Ps.shape[1] if Ps is not None else 0
+    ro = Po.shape[1] if Po is not None else 0
+    lib().ora_op_zmix(ct.c_int64(nxy), ct.c_int(ns), ptr(J), ct.c_int(no), ptr(E), ct.c_int(0 if T is None else len(T)),
+                      ptr(T), ct.c_int(0 if H is None else len(H)), ptr(H), ct.c_int(rs), ct.c_int(ro), ptr(D), ptr(Ps),
+                      ptr(Po), ct.c_int(nthreads))
+    return E
+

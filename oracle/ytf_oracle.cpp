@@ -36,6 +36,7 @@
 #define ORACLE_MUTANT 0
 #endif
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -1590,6 +1591,93 @@ int ora_nudft2d1_modes(int64_t M, const double *x, const double *y, const double
     }
     f[2 * m] = sr;
     f[2 * m + 1] = si;
+  }
+  return 0;
+}
+
+// The z coupling of apply_operator at every padded spectral node
+// (_zmul_add!, _dense_add!, _zfold, _zexpand_add!, mod_GreenOperator.jl:604-662,
+// and their use at :690-707), restated as direct sums over plane pairs (the
+// reference forms the Toeplitz / Hankel sums as z convolutions by FFT; the sums
+// are the same):
+//   structured part: E[j] += sum_i (T[j - i + ns - 1] + H[i + j]) J[i]  (nT, nH > 0);
+//   dense part:      E[j] += sum_i D[i + ns j] J[i]                     (Ps == NULL);
+//   skeleton part:   Js[a] = sum_i Ps[i, a] J[i], Es[b] = sum_a D[a + rs b] Js[a],
+//                    E[j] += sum_b Po[j, b] Es[b].
+// Each product is the 3x3 plane [c][q] at the node times the 3-vector J[.][q].
+// Layouts (complex, interleaved): planes [slot][c][q][nxy], J [ns][3][nxy],
+// E [no][3][nxy] (accumulated), Ps [ns][rs], Po [no][ro] row-major.  OpenMP over
+// chunks of nodes.
+int ora_op_zmix(int64_t nxy, int ns, const double *J, int no, double *E, int nT, const double *T, int nH,
+                const double *H, int rs, int ro, const double *D, const double *Ps, const double *Po, int nthreads) {
+  using C = std::complex<double>;
+  const C *Jc = (const C *)J, *Tc = (const C *)T, *Hc = (const C *)H, *Dc = (const C *)D, *Psc = (const C *)Ps,
+          *Poc = (const C *)Po;
+  C *Ec = (C *)E;
+  const int64_t CH = 128;
+  const int64_t nch = (nxy + CH - 1) / CH;
+  const bool skel = Dc && Psc;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t ch = 0; ch < nch; ++ch) {
+    const int64_t x0 = ch * CH, nx = std::min<int64_t>(CH, nxy - x0);
+    auto plane = [&](const C *P, int slot, int c, int q) { return P + ((int64_t)(slot * 9 + c * 3 + q)) * nxy + x0; };
+    auto add = [&](C *out, const C *K, const C *v) {
+      for (int c = 0; c < 3; ++c)
+        for (int q = 0; q < 3; ++q) {
+          const C *k = K + (int64_t)(c * 3 + q) * nxy;
+          const C *x = v + (int64_t)q * nxy;
+          C *o = out + (int64_t)c * nxy;
+          for (int64_t t = 0; t < nx; ++t) o[t] += k[t] * x[t];
+        }
+    };
+    if (nT > 0 || nH > 0)
+      for (int j = 0; j < no; ++j)
+        for (int i = 0; i < ns; ++i) {
+          C *out = Ec + (int64_t)j * 3 * nxy + x0;
+          const C *v = Jc + (int64_t)i * 3 * nxy + x0;
+          if (nT > 0) add(out, plane(Tc, j - i + ns - 1, 0, 0), v);
+          if (nH > 0) add(out, plane(Hc, i + j, 0, 0), v);
+        }
+    if (Dc && !skel)
+      for (int j = 0; j < no; ++j)
+        for (int i = 0; i < ns; ++i)
+          add(Ec + (int64_t)j * 3 * nxy + x0, plane(Dc, i + ns * j, 0, 0), Jc + (int64_t)i * 3 * nxy + x0);
+    if (skel) {
+      // the folded sources and skeleton outputs of this chunk (stride CH)
+      std::vector<C> Js((size_t)std::max(rs, 1) * 3 * CH), Es((size_t)std::max(ro, 1) * 3 * CH);
+      for (int a = 0; a < rs; ++a)
+        for (int q = 0; q < 3; ++q) {
+          C *dst = Js.data() + ((size_t)a * 3 + q) * CH;
+          for (int64_t t = 0; t < nx; ++t) dst[t] = 0.0;
+          for (int i = 0; i < ns; ++i) {
+            const C w = Psc[(int64_t)i * rs + a];
+            const C *src = Jc + ((int64_t)i * 3 + q) * nxy + x0;
+            for (int64_t t = 0; t < nx; ++t) dst[t] += w * src[t];
+          }
+        }
+      std::fill(Es.begin(), Es.end(), C(0.0));
+      for (int b = 0; b < ro; ++b)
+        for (int a = 0; a < rs; ++a)
+          for (int c = 0; c < 3; ++c)
+            for (int q = 0; q < 3; ++q) {
+              const C *k = plane(Dc, a + rs * b, c, q);
+              const C *x = Js.data() + ((size_t)a * 3 + q) * CH;
+              C *o = Es.data() + ((size_t)b * 3 + c) * CH;
+              for (int64_t t = 0; t < nx; ++t) o[t] += k[t] * x[t];
+            }
+      for (int j = 0; j < no; ++j)
+        for (int b = 0; b < ro; ++b) {
+          const C w = Poc[(int64_t)j * ro + b];
+          for (int c = 0; c < 3; ++c) {
+            const C *src = Es.data() + ((size_t)b * 3 + c) * CH;
+            C *o = Ec + ((int64_t)j * 3 + c) * nxy + x0;
+            for (int64_t t = 0; t < nx; ++t) o[t] += w * src[t];
+          }
+        }
+    }
   }
   return 0;
 }

@@ -570,3 +570,39 @@ def test_rotation_fill_errors():
         build_layer_operator(c, _grid(8, 8, 0.06, 0.05), [0.2, 0.4], {2: [0.3]})
     with pytest.raises(ValueError, match="requires an azimuthally symmetric stack"):
         CavityTransfer2D(FOUNDATION_STACKS["biaxial 4-layer"], W0, 2, kr, wr, phis, wph, rotation=True)
+
+
+def test_skeletons_with_deep_evanescent_nodes_match_the_oracle():
+    """z skeletons over nodes out to k_rho = 3000 (the operator benchmark's cap):
+    rows exp(-kappa |z - z_face|) whose columns fall below 1e-145 must not
+    poison the device QR (they once made 2/vv overflow and kept every side
+    whole).  Both sides of the vac/glass/vac cross blocks against the
+    oracle's _zskeleton on the same wavenumbers, in the Q form and the node
+    form."""
+    from yellowthinfilms_amd.green_operator import _z_skeletons, z_skeletons
+    from yellowthinfilms_amd import workloads as Wl
+    print("case: skeletons with deep evanescent nodes", flush=True)
+    stack = Wl.operator_volume_stack()
+    zs, zobs = Wl.operator_volume_planes()
+    for kmax, n in ((1000.0, 1000), (3000.0, 4000)):
+        kr = np.concatenate([np.linspace(0.01, 6 * W0, n // 2), np.geomspace(6 * W0, kmax, n - n // 2)])
+        kz = {}
+        for eps, name in ((1 + 1e-3j, "vac"), (2.25 + 1e-3j, "glass")):
+            q = np.sqrt(W0 ** 2 * eps - kr ** 2 + 0j)
+            kz[name] = np.where(q.imag < 0, -q, q)
+        Qs = (np.stack([kz["glass"]] * 2), np.stack([-kz["glass"]] * 2))
+        Qo = {o: (np.stack([kz["vac"]] * 2), np.stack([-kz["vac"]] * 2)) for o in (1, 3)}
+        skq = z_skeletons(Qs, Qo, stack, 2, zs, zobs, [1, 3], 1e-6)
+        skn = _z_skeletons(stack, W0, 2, kr, np.zeros_like(kr), zs, {1: zobs[1], 3: zobs[3]}, 1e-6, 0)
+        ref = {1: po.zskeleton(zobs[1], [(po._qlist(-kz["vac"]), 1.0, -0.5)], 1e-6),
+               3: po.zskeleton(zobs[3], [(po._qlist(kz["vac"]), 1.0, 0.0)], 1e-6)}
+        ref_s = po.zskeleton(zs, [(po._qlist(kz["glass"]), -1.0, 0.5), (po._qlist(-kz["glass"]), -1.0, 0.0)], 1e-6)
+        for o in (1, 3):
+            for sk in (skq, skn):
+                s = sk.obs[o]
+                print(f"  k_rho <= {kmax:g}, obs {o}: {len(s.idx)} planes (oracle "
+                      f"{'whole' if ref[o]['whole'] else len(ref[o]['idx'])}), err {s.err:.2e}", flush=True)
+                assert not ref[o]["whole"]
+                assert np.array_equal(s.idx, ref[o]["idx"]) and s.err <= 1e-6
+        for sk in (skq, skn):
+            assert sk.src.whole == ref_s["whole"]

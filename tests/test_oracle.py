@@ -627,3 +627,31 @@ def test_box_must_fit_its_layer():
     st = ostack(FOUNDATION_STACKS["iso 5-layer lossy"])
     with pytest.raises(ValueError, match="fit"):
         po.ghat_batch(st, W0, [1.0], [0.0], 2, 2, [0.05], [0.3], dz_box=0.2)
+
+
+def test_operator_z_coupling_against_the_definition():
+    """ora_op_zmix (the z coupling of apply_operator, every block form) against
+    its definition summed plane pair by plane pair in numpy, on ragged sizes
+    (node count not a multiple of the chunk, No != Ns)."""
+    rng = np.random.default_rng(5)
+
+    def c(*s):
+        return rng.normal(size=s) + 1j * rng.normal(size=s)
+    ns, no, nxy, rs, ro = 5, 4, 301, 3, 2
+    J = c(ns, 3, nxy)
+    T, H = c(ns + no - 1, 3, 3, nxy), c(ns + no - 1, 3, 3, nxy)
+    D, Dk, Ps, Po = c(ns * no, 3, 3, nxy), c(rs * ro, 3, 3, nxy), c(ns, rs), c(no, ro)
+    mv = lambda K, v: np.einsum("cqx,qx->cx", K, v)
+    ref_s = np.array([sum(mv(T[j - i + ns - 1] + H[i + j], J[i]) for i in range(ns)) for j in range(no)])
+    ref_d = np.array([sum(mv(D[i + ns * j], J[i]) for i in range(ns)) for j in range(no)])
+    Js = np.einsum("ia,iqx->aqx", Ps, J)
+    Es = np.array([sum(mv(Dk[a + rs * b], Js[a]) for a in range(rs)) for b in range(ro)])
+    ref_k = np.einsum("jb,bcx->jcx", Po, Es)
+    for got, ref in ((po.op_zmix(nxy, J, no, T=T, H=H), ref_s), (po.op_zmix(nxy, J, no, D=D), ref_d),
+                     (po.op_zmix(nxy, J, no, D=Dk, Ps=Ps, Po=Po), ref_k)):
+        assert relerr(got, ref) < 1e-14
+    # accumulation into a given E, and the Toeplitz part alone
+    E0 = c(no, 3, nxy)
+    got = po.op_zmix(nxy, J, no, T=T, E=E0.copy())
+    ref = E0 + np.array([sum(mv(T[j - i + ns - 1], J[i]) for i in range(ns)) for j in range(no)])
+    assert relerr(got, ref) < 1e-14

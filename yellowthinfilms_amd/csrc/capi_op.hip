@@ -352,8 +352,12 @@ void pivoted_qr(std::vector<zc> &R, int N, std::vector<int> &p) {
     }
     double nx = 0.0;
     for (int i = k; i < N; ++i) nx += std::norm(R[i + (size_t)N * k]);
+    if (nx < 1e-290) {
+      // entries below 1e-145: dropped (the reflection's 2/vv would overflow)
+      for (int i = k + 1; i < N; ++i) R[i + (size_t)N * k] = 0.0;
+      continue;
+    }
     nx = std::sqrt(nx);
-    if (nx == 0.0) continue;
     const zc alpha = R[k + (size_t)N * k];
     const double aa = std::abs(alpha);
     const zc ph = aa > 0.0 ? alpha / aa : zc(1.0, 0.0);

@@ -61,6 +61,7 @@ __device__ void sk_householder(cd *A, int N, cd *red, double *vvs, cd *v0s) {
     }
     const double nrm2 = red[0].r;
     const cd alpha = A[k + LD * k];
+    __syncthreads();  // every thread has read red[0] before thread 0 reuses it
     if (tid == 0) {
       const double nx = sqrt(nrm2), aa = cabs(alpha);
       // beta = -exp(i arg alpha) |x|, v = x - beta e1, v^H v = 2 |x| (|x| + |alpha|)
@@ -74,7 +75,12 @@ __device__ void sk_householder(cd *A, int N, cd *red, double *vvs, cd *v0s) {
     const double vv = *vvs;
     const cd v0 = *v0s, beta = red[0];
     __syncthreads();
-    if (vv > 0.0) {
+    if (nrm2 < 1e-290) {
+      // every entry of the column below 1e-145: its squares underflow and
+      // 2/vv overflows, so the reflection would be garbage.  The entries are
+      // dropped (zero), far below any interpolation tolerance.
+      for (int i = k + 1 + tid; i < M; i += 256) A[i + LD * k] = cd{0.0, 0.0};
+    } else if (vv > 0.0) {
       // columns j > k: w = v^H A[k:, j]; A[k:, j] -= (2/vv) v w, one wave per column
       for (int j = k + 1 + wave; j < N; j += 4) {
         cd w = cd{0.0, 0.0};

@@ -214,11 +214,14 @@ OPERATOR_VOLUME_64 = dict(
                  "(Bench_GreenOperator.jl): iso vac/glass/vac, every layer 0.5 thick, Im eps = 1e-3 on every layer, "
                  "box source voxels of side 1/32 (box_lateral = (1/32, 1/32), dz_box = 1/32) on the 16 planes of "
                  "layer 2, observed on the 16 planes of each of layers 1, 2 and 3; 64x64 pixels (L=1, dx=1/32, "
-                 "padded 128x128); NUFFT lateral sum (eps 1e-12), z skeletons at tol 1e-6; built from 81920 polar "
-                 "nodes (1280 k_rho to 3000 x 64 phi); one apply = apply_operator of a full 64x64x16x3 complex "
-                 "N(0,1) moment volume; complex FP64"),
+                 "padded 128x128); the benchmark's own k_rho x phi grid (box_krho_grid_grouped: N_phi 256, rtol "
+                 "1e-3, k_rho cap 3000, rho_far = L, maxevals 200000, 4 mirror classes per group), built by "
+                 "build_layer_operator_grouped (node budget 900000, phi-rotation fill on this symmetric stack); "
+                 "NUFFT lateral sum (eps 1e-12), z skeletons at tol 1e-6; one apply = apply_operator of a full "
+                 "64x64x16x3 complex N(0,1) moment volume; complex FP64"),
     L=1.0, dx=1.0 / 32.0, delta=1e-3, thick=0.5, source_layer=2, obs_layers=(1, 2, 3),
-    n_band=768, kband_k0=6.0, n_tail=512, kcap=3000.0, n_phi=64, eps=1e-12, skeleton_tol=1e-6, seed=1,
+    n_phi=256, rtol=1e-3, kcap=3000.0, maxevals=200_000, classes_per_group=4, node_budget=900_000,
+    eps=1e-12, skeleton_tol=1e-6, seed=1,
     # the reference's own apply time for this case: 90 ms on an Apple M5, 4 threads
     # (STATUS_closed_form_rewrite.md:75), i.e. 11.1 applies/s
     reference_applies_per_s=1.0 / 0.090,
@@ -242,19 +245,16 @@ def operator_volume_planes(wl=OPERATOR_VOLUME_64):
     return z, {o: z.copy() for o in wl["obs_layers"]}
 
 
-def operator_volume_nodes(wl=OPERATOR_VOLUME_64):
-    """(k_rho, w_rho, phi, w_phi): Gauss-Legendre panels on [0, 6 k0] broken at the light
-    lines, geometrically graded panels from 6 k0 to the reference's k_rho cap 3000
-    (BOX_KCAP, Bench_GreenOperator.jl:71), uniform phi."""
-    from . import quadrature as Q
+def operator_volume_groups(wl=OPERATOR_VOLUME_64, device=0):
+    """The benchmark's k_rho x phi grid: box_krho_grid_grouped(st, omega, 2; box=(dx, dx, dx),
+    N_phi, rtol, k_rho_cap, rho_far = L, maxevals, classes_per_group) (Bench_GreenOperator.jl:
+    163-168).  Returns (groups, info)."""
+    from .adaptive import box_krho_grid_grouped
     stack = operator_volume_stack(wl)
-    kb = wl["kband_k0"] * OMEGA
-    breaks = [0.0, kb] + [k for k in Q.light_lines(stack, OMEGA) if 0 < k < kb]
-    k1, w1 = Q.panel_rule(breaks, wl["n_band"], order=16)
-    edges = np.geomspace(kb, wl["kcap"], wl["n_tail"] // 16 + 1)
-    k2, w2 = Q.gl_rule(list(zip(edges[:-1], edges[1:])), order=16)
-    phi = 2 * math.pi * np.arange(wl["n_phi"]) / wl["n_phi"]
-    return np.concatenate([k1, k2]), np.concatenate([w1, w2]), phi, np.full(wl["n_phi"], 2 * math.pi / wl["n_phi"])
+    d = wl["dx"]
+    return box_krho_grid_grouped(stack, OMEGA, wl["source_layer"], (d, d, d), n_phi=wl["n_phi"],
+                                 classes_per_group=wl["classes_per_group"], rtol=wl["rtol"], krho_cap=wl["kcap"],
+                                 rho_far=wl["L"], maxevals=wl["maxevals"], device=device)
 
 
 POLE_SCAN_HYPERBOLIC = dict(
