@@ -362,6 +362,35 @@ int ytf_mlp_surrogate_ghat(ytf_ctx *ctx, ytf_mlp *mlp, const ytf_layer *layers, 
                            double zsrc, double zobs, double branch_tol, double origin_tol, double t_big, int fallback,
                            double *ghat_dev, unsigned char *flags_dev, int64_t *n_flagged);
 
+/* The tabulated k_rho surrogate of the same scattered spectral dyadic, for a
+ * stack whose every layer is invariant under rotations about z (isotropic,
+ * uniaxial about z, or gyrotropic about z: eps and mu of the form
+ * [[a, b, 0], [-b, a, 0], [0, 0, c]]; the caller checks).  No counterpart in
+ * the reference, which solves every node; the table is built from exact
+ * solves at phi = 0 (the reference's 1D k_rho treatment of such stacks,
+ * CavityTransferCache1D, mod_CavityTransfer.jl), and the exact solve stays the
+ * fallback.  Per node, with k_rho = |(kx, ky)|, phi its angle and R(phi) the
+ * rotation about z:
+ *     g~_s = R(phi) [t(k_rho) omega (1 + k^2/omega^2) exp(i kz0 zsum) / kz0] R(phi)^T,
+ * t from panel p (kb_dev[p] <= k_rho < kb_dev[p + 1], n_panels contiguous
+ * panels, at most 2048): four-point (cubic) Lagrange interpolation of the
+ * m_dev[p] >= 4 uniform samples of t over the panel variable s from
+ * sab_dev[3p] to sab_dev[3p + 1]; kind_dev[p] & 3 selects s around the
+ * panel's real branch point k_b = sab_dev[3p + 2] (0: k_rho, 1: sqrt(k_b^2 -
+ * k_rho^2), 2: sqrt(k_rho^2 - k_b^2)).  Sample q of panel p is row
+ * row0_dev[p] + q of val_dev (18 doubles: Re t_e for e < 9, Im t_{e-9} for
+ * e >= 9, entry c + 3q).  A node is flagged (flags_dev[i] = 1) outside
+ * [kb_dev[0], kb_dev[n_panels]], in a panel with kind bit 4 (not converged),
+ * or at kz0 = 0; with fallback the flagged nodes are solved exactly as in
+ * ytf_mlp_surrogate_ghat (source and observation layer 1, depths zsrc / zobs,
+ * YTF_TERMS_SCATTERED).  All table arrays are device arrays; n_flagged as in
+ * ytf_mlp_surrogate_ghat. */
+int ytf_ktable_ghat(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int n_panels,
+                    const double *kb_dev, const int32_t *kind_dev, const double *sab_dev, const int64_t *row0_dev,
+                    const int32_t *m_dev, const double *val_dev, const double *kx_dev, const double *ky_dev,
+                    int64_t n, double zsum, double zsrc, double zobs, int fallback, double *ghat_dev,
+                    unsigned char *flags_dev, int64_t *n_flagged);
+
 /* 2D type-1 nonuniform FFT (FINUFFT's type 1 with iflag +1, the transform of
  * closed_form_G's 2D lateral sum with lateral=:nufft,
  * mod_ClosedFormField.jl:758-780):

@@ -119,6 +119,10 @@ _SIGS = {
                                           ct.c_void_p, ct.c_void_p, ct.c_int64, ct.POINTER(ct.c_float), ct.c_double,
                                           ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_int,
                                           ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
+    "ytf_ktable_ghat": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_int,
+                                   ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                   ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_double, ct.c_double, ct.c_double,
+                                   ct.c_int, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_int64)]),
     "ytf_nufft2d1": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
                                 ct.c_int, ct.c_double, ct.c_void_p]),
     "ytf_polar_nodes": (ct.c_int, [ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p,

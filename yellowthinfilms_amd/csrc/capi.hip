@@ -1280,6 +1280,62 @@ int ytf_mlp_surrogate_ghat(ytf_ctx *c, ytf_mlp *m, const ytf_layer *layers, int 
   return YTF_OK;
 }
 
+int ytf_ktable_ghat(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, int n_panels,
+                    const double *kb_dev, const int32_t *kind_dev, const double *sab_dev, const int64_t *row0_dev,
+                    const int32_t *m_dev, const double *val_dev, const double *kx_dev, const double *ky_dev,
+                    int64_t n, double zsum, double zsrc, double zobs, int fallback, double *ghat_dev,
+                    unsigned char *flags_dev, int64_t *n_flagged) {
+  if (!c) return fail(nullptr, YTF_EINVAL, "ytf_ktable_ghat: ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (n < 0 || n > INT32_MAX || (n > 0 && (!kx_dev || !ky_dev || !ghat_dev || !flags_dev)))
+    return fail(c, YTF_EINVAL, "ytf_ktable_ghat: bad arguments");
+  if (n_panels < 1 || n_panels > 2048 || !kb_dev || !kind_dev || !sab_dev || !row0_dev || !m_dev || !val_dev)
+    return fail(c, YTF_EINVAL, "ytf_ktable_ghat: bad table (n_panels=%d must be in 1..2048, no NULL arrays)",
+                n_panels);
+  if (!(omega > 0)) return fail(c, YTF_EINVAL, "ytf_ktable_ghat: omega must be > 0");
+  if (fallback && (!layers || n_layers < 2)) return fail(c, YTF_EINVAL, "ytf_ktable_ghat: no stack for the fallback");
+  if (n_flagged) *n_flagged = 0;
+  if (n == 0) return YTF_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  ytf::NodeParams P;
+  if (fallback) {
+    int rc = node_params_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n, 1, 1, 1, &zsrc, &zobs, ghat_dev,
+                                nullptr, YTF_TERMS_SCATTERED, 0.0, 1, &P);
+    if (rc) return rc;
+  }
+  const size_t cb = align256(sizeof(int)), lb = align256(sizeof(int) * (size_t)n);
+  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, cb + lb);
+  if (rc) return rc;
+  int *count = (int *)c->aux;
+  int *list = (int *)((char *)c->aux + cb);
+  ytf::KTable T{n_panels, kb_dev, (const int *)kind_dev, sab_dev, row0_dev, (const int *)m_dev, val_dev, 0.0};
+  hipError_t e = ytf::launch_mlp_zero(1, (float *)count, c->stream);  // zero bits = integer 0
+  if (e == hipSuccess)
+    e = timed(c, "ktable", [&] {
+      return ytf::launch_ktable_ghat(n, kx_dev, ky_dev, omega, zsum, T, (cd *)ghat_dev, flags_dev, count, list,
+                                     c->stream);
+    });
+  if (e == hipSuccess && fallback) {
+    ytf::NodeParams Q = P;
+    Q.rows = list;
+    Q.n_rows = count;
+    const int blocks = (int)std::min<int64_t>((n + 63) / 64, 1024);
+    e = timed(c, "node_solve", [&] {
+      return (Q.src == 0 && Q.obs == 0) ? ytf::launch_node_top(Q, blocks, c->stream)
+                                        : ytf::launch_node_general(Q, blocks, c->stream);
+    });
+  }
+  if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_ktable_ghat: launch failed: %s", hipGetErrorString(e));
+  if (n_flagged) {
+    rc = ensure_pinned(c);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pinned, count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *n_flagged = (int64_t)c->pinned[0];
+  }
+  return YTF_OK;
+}
+
 // ---------------------------------------------------------------------------
 // 2D type-1 nonuniform FFT (nufft.hip)
 // ---------------------------------------------------------------------------

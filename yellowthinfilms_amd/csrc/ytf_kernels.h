@@ -196,6 +196,27 @@ hipError_t launch_sur_preflag(int64_t n, const double *kx, const double *ky, dou
                               double origin_tol, const double *win, int nwin, unsigned char *flags, int *count,
                               int *list, hipStream_t s);
 
+// The tabulated k_rho surrogate (ktable.hip): np contiguous panels with
+// boundaries kb[np + 1] (increasing k_rho), kind[np] (0: s = k_rho,
+// 1: s = sqrt(k_b^2 - k_rho^2), 2: s = sqrt(k_rho^2 - k_b^2); bit 4: exact
+// solve), sab[3 np] (s range, k_b), and per panel m[p] >= 4 uniform samples
+// of Re, Im of t's 9 entries in s, rows row0[p] .. row0[p] + m[p] - 1 of
+// val[rows][18].
+struct KTable {
+  int np;
+  const double *kb;
+  const int *kind;
+  const double *sab;
+  const int64_t *row0;
+  const int *m;
+  const double *val;
+  double inv_w;  // omega^-1 (set by the launch)
+};
+// g~_s [9][n] of the nodes from the table; flagged nodes (outside the table,
+// unconverged panels, kz0 = 0) get flags 1, no g~, and go to list via *count.
+hipError_t launch_ktable_ghat(int64_t n, const double *kx, const double *ky, double w, double zsum, const KTable &T,
+                              cd *ghat, unsigned char *flags, int *count, int *list, hipStream_t s);
+
 // 2D type-1 nonuniform FFT (nufft.hip).
 int nu_width(double eps);
 // Upper bound on the spreading subproblems of M points on an nf x nf fine grid.

@@ -156,6 +156,7 @@ SURROGATE_1024 = dict(
     family_seed=1, seed=7, data_seed=2, width=256, depth=4, n_train=1 << 22, epochs=8, batch=8192, lr=2e-3,
     f=1.0, L=1024 / 12.0, dx=1.0 / 6.0, h_src_nm=40.0, h_obs_nm=50.0, loss=0.1,
     pole_window=8.0,  # nodes within this many half-widths of a cavity-response pole take the exact solve
+    ktable_tol=1e-10, ktable_deg=16,  # the tabulated k_rho surrogate of the same stack
 )
 
 
