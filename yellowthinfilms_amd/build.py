@@ -28,10 +28,8 @@ FLAGS = [
 ]
 
 
-# Optimisation level per source (default -O3).  node_general.hip is built at
-# -O1, the level its kernels have been validated at across the whole test suite
-# (the header of that file gives the history).
-OPT = {"node_general.hip": "-O1"}
+# Optimisation level per source (default -O3 for every file).
+OPT = {}
 
 
 def sources():

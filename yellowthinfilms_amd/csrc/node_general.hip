@@ -2,12 +2,15 @@
 // (source layer, observation layer) pair: same layer inside the stack
 // (CASE_SAME), below (CASE_RIGHT) or above (CASE_LEFT) the source.
 //
-// Built at -O1.  An earlier formulation of these cases, which kept full 4x4
-// S-matrix accumulators live through the layer loop and spilled heavily, gave
-// wrong results on gfx950 at -O3 and later at -O1 (a host build of the same
-// code was exact).  The current two-pass cascade (cascade_general in
-// node_core.hpp) keeps its carried blocks in LDS and matches the oracle at -O1
-// and at -O3; -O1 is the level the whole test suite has been run with.
+// Built at -O3 like every other file.  An earlier formulation of these cases,
+// which kept full 4x4 S-matrix accumulators live through the layer loop and
+// spilled heavily, gave wrong results on gfx950 (a host build of the same code
+// was exact).  The current two-pass cascade (cascade_general in node_core.hpp)
+// keeps its carried blocks in LDS; the host build of this source under
+// AddressSanitizer and UndefinedBehaviorSanitizer, with every LDS slot
+// poisoned before each node, finds no out-of-range slot, no slot read before
+// it is written and no undefined behaviour in any case
+// (tests/test_host_node_sanitized.py).
 #include "node_core.hpp"
 
 namespace ytf {
