@@ -38,11 +38,12 @@ typedef float f16x __attribute__((ext_vector_type(16)));
 //            J_nu(x) = sqrt(2/(pi x)) (P_nu cos chi - Q_nu sin chi),
 //            chi = x - (nu/2 + 1/4) pi,
 //            P_nu = sum_m (-1)^m a_2m / x^2m,  Q_nu = sum_m (-1)^m a_2m+1 / x^2m+1,
-//            a_k = prod_{j <= k} (4 nu^2 - (2j - 1)^2) / (k! 8^k), k < 22
-//            (truncation ~1e-12 at x = 12); cos chi and sin chi come from
-//            sincos(x), so no rounding of x - chi enters.
-// Against scipy's j0/j1 on [0, 5e4] the largest absolute error is 1.1e-12.
-constexpr int BS_SER = 31, BS_ASY = 11;
+//            a_k = prod_{j <= k} (4 nu^2 - (2j - 1)^2) / (k! 8^k), k < 12
+//            (truncation 7e-11 at x = 12, far below the table's 2^-22
+//            relative split); cos chi and sin chi come from sin x and cos x
+//            (sincos_reduced), so no rounding of x - chi enters.
+// Against scipy's j0/j1 on [12, 8000] the asymptotic branch is within 7.4e-11.
+constexpr int BS_SER = 31, BS_ASY = 6;
 struct BesselCoef {
   double s0[BS_SER], s1[BS_SER];                   // 1/(m!)^2, 1/(m! (m+1)!)
   double p0[BS_ASY], q0[BS_ASY], p1[BS_ASY], q1[BS_ASY];  // (-1)^m a_2m, (-1)^m a_2m+1
@@ -71,6 +72,35 @@ constexpr BesselCoef make_bessel_coef() {
 }
 static constexpr BesselCoef BSC = make_bessel_coef();
 
+// sin x and cos x for 0 <= x < 2^30: x = n pi/2 + r, |r| <= pi/4, by a
+// two-term Cody-Waite reduction (the fma makes x - n P1 exact, and P1 + P2
+// is pi/2 to 1e-33), then Taylor polynomials in r to r^15 and r^14 (the
+// next terms are below 1e-15 on |r| <= pi/4), and the quadrant n mod 4.  A
+// fraction of the device library's sincos, whose reduction serves any double.
+__device__ inline void sincos_reduced(double x, double &sn, double &cs) {
+  const double n = rint(x * 0.63661977236758134308);
+  const double r = fma(-n, 6.123233995736766036e-17, fma(-n, 1.5707963267948965580, x));
+  const double r2 = r * r;
+  double s = -1.0 / 1307674368000.0, c = -1.0 / 87178291200.0;  // r^15/15!, r^14/14!
+  s = fma(s, r2, 1.0 / 6227020800.0);
+  c = fma(c, r2, 1.0 / 479001600.0);
+  s = fma(s, r2, -1.0 / 39916800.0);
+  c = fma(c, r2, -1.0 / 3628800.0);
+  s = fma(s, r2, 1.0 / 362880.0);
+  c = fma(c, r2, 1.0 / 40320.0);
+  s = fma(s, r2, -1.0 / 5040.0);
+  c = fma(c, r2, -1.0 / 720.0);
+  s = fma(s, r2, 1.0 / 120.0);
+  c = fma(c, r2, 1.0 / 24.0);
+  s = fma(s, r2, -1.0 / 6.0);
+  c = fma(c, r2, -0.5);
+  s = fma(s * r2, r, r);
+  c = fma(c, r2, 1.0);
+  const int q = (int)((long long)n & 3);
+  sn = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+  cs = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
 // rx = 1/ax (used for ax >= 12).
 __device__ inline void bessel01_table(double ax, double rx, double &j0, double &j1) {
   if (ax < 12.0) {
@@ -96,7 +126,8 @@ __device__ inline void bessel01_table(double ax, double rx, double &j0, double &
     q0 *= rx;
     q1 *= rx;
     double sn, cs;
-    sincos(ax, &sn, &cs);
+    if (ax < 1073741824.0) sincos_reduced(ax, sn, cs);
+    else sincos(ax, &sn, &cs);
     // sqrt(2/(pi x)) / sqrt(2): cos(x - pi/4) = (cs + sn)/sqrt2, sin(x - pi/4) = (sn - cs)/sqrt2,
     // cos(x - 3pi/4) = (sn - cs)/sqrt2, sin(x - 3pi/4) = -(sn + cs)/sqrt2
     const double amp = sqrt(0.31830988618379067154 * rx);
@@ -130,9 +161,17 @@ __device__ inline void bessel012m(double x, double &b0, double &b1, double &b2) 
 // can leave h up to (1/2 + 2^-14) fp16 ulp from v instead of 1/2, so
 // |v - h - l| stays within (1 + 2^-12) of the bound of correctly rounded
 // halves (2^-22 |v|, or half the fp16 subnormal spacing).
+// The empty asm keeps the f32 step: without it the compiler folds the two
+// conversions into one f64 -> fp16 rounding, done in software (~40
+// instructions each), which made the table kernel compute-bound.
+__device__ inline _Float16 to_h(double v) {
+  float f = (float)v;
+  __asm__ volatile("" : "+v"(f));
+  return (_Float16)f;
+}
 __device__ inline void split_h(double v, _Float16 &h, _Float16 &l) {
-  h = (_Float16)(float)v;
-  l = (_Float16)(float)(v - (double)h);
+  h = to_h(v);
+  l = to_h(v - (double)h);
 }
 
 // Bessel table: one thread = one radius x 8 consecutive nodes.
