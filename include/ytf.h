@@ -57,7 +57,9 @@ typedef struct ytf_layer {
 /* Per-node status bits written to flags_dev. */
 #define YTF_NODE_VALID 1u    /* CavityTransferCache.valid  (mod_CavityTransfer.jl:167) */
 #define YTF_NODE_TVALID 2u   /* CavityTransferCache.Tvalid[o]                          */
-#define YTF_NODE_SINGULAR 4u /* an exactly singular 4x4 solve (SingularException)      */
+#define YTF_NODE_SINGULAR 4u /* an exactly singular source projection (W'V) solve; a    */
+                             /* singular interface solve takes the pivoted-QR fallback */
+                             /* (mod_ScatteringMatrix.jl:35-44) and is not flagged      */
 #define YTF_NODE_NOMODES 8u  /* mode solve failed (no QR convergence / >2 modes a side) */
 
 typedef struct ytf_ctx ytf_ctx;
@@ -157,7 +159,8 @@ int ytf_green_nodes_box(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, dou
  * edge), sigma the largest singular value of a 2x2, PhiF = e^{i Qf d_s},
  * PhiB = e^{-i Qb d_s}.  It peaks at the real-axis poles (guided modes,
  * plasmons, polaritons) of the field of a source in layer s.  0 where the
- * mode solve fails or an interface solve is exactly singular.
+ * mode solve fails; an exactly singular interface solve takes the pivoted-QR
+ * fallback of mod_ScatteringMatrix.jl:35-44.
  *   source_layer: 1 .. n_layers - 1; kx_dev, ky_dev, out_dev: n_nodes doubles. */
 int ytf_cavity_response(ytf_ctx *ctx, const ytf_layer *layers, int n_layers, double omega, int source_layer,
                         const double *kx_dev, const double *ky_dev, int64_t n_nodes, double *out_dev);

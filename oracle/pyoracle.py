@@ -85,6 +85,25 @@ def eigen4(A):
     return Q, V.reshape(4, 4, order="F")
 
 
+def interface_solve(L, Rh):
+    """L \\ Rh as _S_Interface_Matrix solves it: LU, and on an exactly zero pivot
+    the pivoted-QR minimum-norm solve.  Returns (S, fallback_taken)."""
+    a = np.asarray(L, dtype=np.complex128).reshape(-1, order="F").copy()
+    b = np.asarray(Rh, dtype=np.complex128).reshape(-1, order="F").copy()
+    S = np.zeros(16, np.complex128)
+    fb = lib().ora_interface_solve(_p(a.view(np.float64)), _p(b.view(np.float64)), _p(S.view(np.float64)))
+    return S.reshape(4, 4, order="F"), bool(fb)
+
+
+def qr_solve4(L, Rh):
+    """The pivoted-QR minimum-norm solve of the fallback, taken unconditionally."""
+    a = np.asarray(L, dtype=np.complex128).reshape(-1, order="F").copy()
+    b = np.asarray(Rh, dtype=np.complex128).reshape(-1, order="F").copy()
+    S = np.zeros(16, np.complex128)
+    lib().ora_qr_solve4(_p(a.view(np.float64)), _p(b.view(np.float64)), _p(S.view(np.float64)))
+    return S.reshape(4, 4, order="F")
+
+
 def berreman(eps, mu, omega, kx, ky):
     e = _flat33(np.asarray(eps)[None])
     m = _flat33(np.asarray(mu)[None])

@@ -25,7 +25,9 @@
 //     dyadic against the angular-spectrum closed form with TE/TM Airy
 //     reflections, the 3-layer semi-analytical model
 //     (mod_Dipz_SemiAnalytical.jl) and the reference's logged s/p reflection
-//     norms of its hyperbolic stack (tests/closed_forms.py).
+//     norms of its hyperbolic stack (tests/closed_forms.py);
+//   * the interface solve's pivoted-QR fallback: numpy's pinv on rank-0..4
+//     systems and on the exactly singular interfaces of a stack at k_z = 0.
 //
 // ORACLE_MUTANT (default 0) plants one known mistake; tests/test_oracle_mutants.py
 // builds each variant on the CPU and shows that the checks above reject it:
@@ -268,9 +270,17 @@ template <class R, int N> inline bool lu_solve4(Mat<R, 4, 4> A, Mat<R, 4, N> B, 
     }
   return true;
 }
-// Column-pivoted Householder QR least-squares solve (the SingularException
-// fallback of mod_ScatteringMatrix.jl:35-44; LAPACK's basic solution with the
-// rank cut at eps).
+// Column-pivoted Householder QR least-squares solve: the SingularException
+// fallback of mod_ScatteringMatrix.jl:35-44, `qr(L, ColumnNorm()) \ R`.
+// Julia's QRPivoted solve (LinearAlgebra's ldiv! for QRPivoted: LAPACK xGEQP3,
+// then xTZRZF / xORMRZ on the leading rows) returns the minimum-norm
+// least-squares solution of the rank-r truncated system, r being the leading
+// block whose condition stays below 1/rcond with rcond = 4 eps.  The rank test
+// here is |R_kk| <= 4 eps |R_11|; Julia estimates the leading blocks' extreme
+// singular values incrementally (xLAIC1).  On the exactly singular systems
+// that reach this path both cut at the same rank.  The minimum-norm solution of
+// the full-row-rank r x 4 system R y = c is taken from a Householder QR of R^H:
+// R^H = Q2 [T; 0], y = Q2 [T^-H c; 0]; then X = P y.
 template <class R, int N> inline void qr_solve4(Mat<R, 4, 4> A, Mat<R, 4, N> B, Mat<R, 4, N> &X) {
   int jp[4] = {0, 1, 2, 3};
   double cn[4];
@@ -281,6 +291,30 @@ template <class R, int N> inline void qr_solve4(Mat<R, 4, 4> A, Mat<R, 4, N> B, 
   }
   int rank = 4;
   double rmax = 0;
+  // reflect v = (0..0, v_k..v_3) on the columns [k0, 4) of M (4 x NC)
+  auto reflect = [](cx<R> *v, int k, auto &M, int ncols, int k0) {
+    double vn = 0;
+    for (int i = k; i < 4; ++i) vn += val(abs2(v[i]));
+    if (vn == 0) return;
+    for (int j = k0; j < ncols; ++j) {
+      cx<R> s = C<R>(0.0);
+      for (int i = k; i < 4; ++i) s = s + conj(v[i]) * M(i, j);
+      s = rmul(R(2.0 / vn), s);
+      for (int i = k; i < 4; ++i) M(i, j) = M(i, j) - v[i] * s;
+    }
+  };
+  // Householder vector of column x (rows k..3): H x = alpha e_k,
+  // alpha = -|x| x_k / |x_k|
+  auto house = [](const cx<R> *x, int k, double nrm, cx<R> *v) {
+    cx<R> a0 = x[k];
+    double aa = val(cabs(a0));
+    cx<R> ph = aa == 0 ? C<R>(1.0) : a0 / Cr<R>(R(aa));
+    cx<R> alpha = rmul(R(-nrm), ph);
+    for (int i = 0; i < 4; ++i) v[i] = C<R>(0.0);
+    v[k] = a0 - alpha;
+    for (int i = k + 1; i < 4; ++i) v[i] = x[i];
+    return alpha;
+  };
   for (int k = 0; k < 4; ++k) {
     int p = k;
     for (int j = k + 1; j < 4; ++j)
@@ -290,48 +324,44 @@ template <class R, int N> inline void qr_solve4(Mat<R, 4, 4> A, Mat<R, 4, N> B, 
       std::swap(cn[k], cn[p]);
       std::swap(jp[k], jp[p]);
     }
-    // Householder on A(k:3, k)
     double nrm2 = 0;
     for (int i = k; i < 4; ++i) nrm2 += val(abs2(A(i, k)));
     double nrm = std::sqrt(nrm2);
     if (k == 0) rmax = nrm;
     if (nrm <= 2.220446049250313e-16 * std::max(rmax, 1e-300) * 4) { rank = k; break; }
-    cx<R> a0 = A(k, k);
-    double aa = val(cabs(a0));
-    cx<R> ph = aa == 0 ? C<R>(1.0) : a0 / Cr<R>(R(aa));
-    cx<R> alpha = rmul(R(-nrm), ph);
-    cx<R> v[4];
-    for (int i = 0; i < 4; ++i) v[i] = C<R>(0.0);
-    v[k] = a0 - alpha;
-    for (int i = k + 1; i < 4; ++i) v[i] = A(i, k);
-    double vn = 0;
-    for (int i = k; i < 4; ++i) vn += val(abs2(v[i]));
-    if (vn > 0) {
-      for (int j = k; j < 4; ++j) {
-        cx<R> s = C<R>(0.0);
-        for (int i = k; i < 4; ++i) s = s + conj(v[i]) * A(i, j);
-        s = rmul(R(2.0 / vn), s);
-        for (int i = k; i < 4; ++i) A(i, j) = A(i, j) - v[i] * s;
-      }
-      for (int j = 0; j < N; ++j) {
-        cx<R> s = C<R>(0.0);
-        for (int i = k; i < 4; ++i) s = s + conj(v[i]) * B(i, j);
-        s = rmul(R(2.0 / vn), s);
-        for (int i = k; i < 4; ++i) B(i, j) = B(i, j) - v[i] * s;
-      }
-    }
+    cx<R> col[4], v[4];
+    for (int i = 0; i < 4; ++i) col[i] = A(i, k);
+    house(col, k, nrm, v);
+    reflect(v, k, A, 4, k);
+    reflect(v, k, B, N, 0);
     for (int j = k + 1; j < 4; ++j) cn[j] -= val(abs2(A(k, j)));
+  }
+  // H = R^H (4 x rank), R upper trapezoidal in the leading rows of A
+  Mat<R, 4, 4> H;
+  for (int i = 0; i < rank; ++i)
+    for (int j = 0; j < 4; ++j) H(j, i) = j >= i ? conj(A(i, j)) : C<R>(0.0);
+  cx<R> vs[4][4];
+  for (int k = 0; k < rank; ++k) {
+    double nrm2 = 0;
+    for (int i = k; i < 4; ++i) nrm2 += val(abs2(H(i, k)));
+    cx<R> col[4];
+    for (int i = 0; i < 4; ++i) col[i] = H(i, k);
+    house(col, k, std::sqrt(nrm2), vs[k]);
+    reflect(vs[k], k, H, rank, k);
   }
   Mat<R, 4, N> Y;
   for (int j = 0; j < N; ++j) {
-    for (int i = 3; i >= 0; --i) {
-      if (i >= rank) { Y(i, j) = C<R>(0.0); continue; }
+    // T^H w = c: T^H lower triangular, (T^H)(i, m) = conj(T(m, i)) = conj(H(m, i))
+    for (int i = 0; i < rank; ++i) {
       cx<R> s = B(i, j);
-      for (int k = i + 1; k < rank; ++k) s = s - A(i, k) * Y(k, j);
-      Y(i, j) = s / A(i, i);
+      for (int m = 0; m < i; ++m) s = s - conj(H(m, i)) * Y(m, j);
+      Y(i, j) = s / conj(H(i, i));
     }
-    for (int i = 0; i < 4; ++i) X(jp[i], j) = Y(i, j);
+    for (int i = rank; i < 4; ++i) Y(i, j) = C<R>(0.0);
   }
+  for (int k = rank - 1; k >= 0; --k) reflect(vs[k], k, Y, N, 0);
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < 4; ++i) X(jp[i], j) = Y(i, j);
 }
 
 // ============================================================================
@@ -1350,6 +1380,36 @@ int ora_eigen4(const double *A, double *Q, double *V) {
   for (int i = 0; i < 4; ++i) put(Q + 2 * i, q[i]);
   for (int i = 0; i < 16; ++i) put(V + 2 * i, Vm.a[i]);
   return ok ? 0 : 1;
+}
+
+// [L] \ [Rh] as _S_Interface_Matrix solves it (mod_ScatteringMatrix.jl:35-44):
+// LU with partial pivoting, the pivoted-QR minimum-norm solve on an exactly
+// zero pivot.  Returns 1 when the QR fallback was taken, 0 otherwise.
+int ora_interface_solve(const double *L, const double *Rh, double *S) {
+  Mat<double, 4, 4> A, B, X;
+  for (int i = 0; i < 16; ++i) {
+    A.a[i] = C<double>(L[2 * i], L[2 * i + 1]);
+    B.a[i] = C<double>(Rh[2 * i], Rh[2 * i + 1]);
+  }
+  int fb = 0;
+  if (!lu_solve4<double, 4>(A, B, X)) {
+    qr_solve4<double, 4>(A, B, X);
+    fb = 1;
+  }
+  for (int i = 0; i < 16; ++i) put(S + 2 * i, X.a[i]);
+  return fb;
+}
+
+// The pivoted-QR minimum-norm solve alone (qr_solve4).
+int ora_qr_solve4(const double *L, const double *Rh, double *S) {
+  Mat<double, 4, 4> A, B, X;
+  for (int i = 0; i < 16; ++i) {
+    A.a[i] = C<double>(L[2 * i], L[2 * i + 1]);
+    B.a[i] = C<double>(Rh[2 * i], Rh[2 * i + 1]);
+  }
+  qr_solve4<double, 4>(A, B, X);
+  for (int i = 0; i < 16; ++i) put(S + 2 * i, X.a[i]);
+  return 0;
 }
 
 int ora_berreman(const double *eps, const double *mu, double omega, double kx, double ky, double *B) {

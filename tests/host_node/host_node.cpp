@@ -23,6 +23,9 @@
 using namespace ytf;
 static_assert(sizeof(ytf_layer) == sizeof(LayerDev), "ytf_layer and LayerDev share one layout");
 
+// As node_kernel and node_kernel_redo: the fast solve, then the robust solve
+// (the interface QR fallback) of a node whose fast solve met an exactly
+// singular interface.  Prints the number of such nodes.
 template <int CASE, bool GEN>
 static void run(const NodeParams &P, bool poison) {
   // slot i of lane 0 is element 64 i: the array ends at the last declared slot
@@ -30,11 +33,21 @@ static void run(const NodeParams &P, bool poison) {
   std::vector<cd> lds(n_el);
   const LaneMat st{lds.data(), 0};
   const double nan = std::numeric_limits<double>::quiet_NaN();
+  int64_t n_redo = 0;
   for (int64_t k = 0; k < P.nn; ++k) {
     if (poison)
       for (auto &v : lds) v = cd{nan, nan};
-    solve_node<CASE, GEN>(P, k, st);
+    bool redo = false;
+    solve_node<CASE, GEN, false>(P, k, st, redo);
+    if (redo) {
+      ++n_redo;
+      if (poison)
+        for (auto &v : lds) v = cd{nan, nan};
+      bool again = false;
+      solve_node<CASE, GEN, true>(P, k, st, again);
+    }
   }
+  printf("redo %lld\n", (long long)n_redo);
 }
 
 template <bool GEN>

@@ -28,6 +28,15 @@ enum { hipSuccess = 0 };
 // A wave of one lane: a test over the wave is the lane's own value, and the
 // first lane is the lane.
 inline bool __all(bool x) { return x; }
+inline bool __any(bool x) { return x; }
+// one lane, one block: barriers and fences are no-ops, atomics plain updates
+inline void __syncthreads() {}
+inline void __threadfence() {}
+inline int atomicAdd(int *p, int v) {
+  const int o = *p;
+  *p = o + v;
+  return o;
+}
 #define __builtin_amdgcn_readfirstlane(x) (x)
 
 using std::isfinite;

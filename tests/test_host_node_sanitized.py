@@ -68,6 +68,7 @@ def _run(exe, tmp_path, stack, kx, ky, src, obs, zs, zo, terms="all", dz=0.0, ez
         p.stderr[-4000:]
     g = np.fromfile(out, np.complex128).reshape(npairs, 3, 3, n).transpose(3, 0, 2, 1)  # (n, npairs, c, q)
     og = po.ghat_batch(ostack(stack), W0, kx, ky, src, obs, zs, zo, terms=terms, dz_box=dz, ez_local=ez_local)
+    _run.redo = int(p.stdout.split()[-1])  # nodes solved again with the interface QR fallback
     return g, og
 
 
@@ -111,3 +112,22 @@ def test_node_solve_under_sanitizers_box_sources_and_term_classes(host_node, tmp
         do = stack[obs - 1].d
         g, og = _run(host_node, tmp_path, stack, kx, ky, 2, obs, [0.4 * d2], [0.5 * do], dz=0.2 * d2)
         assert relerr(g, og) < 1e-10, (obs, relerr(g, og))
+
+
+def test_node_solve_under_sanitizers_singular_interfaces(host_node, tmp_path):
+    """Exactly singular interface solves (two lossless air layers at k_z = 0):
+    the pivoted-QR fallback in every kernel case, the singular node first in a
+    batch of ordinary ones."""
+    from test_oracle_interface_fallback import singular_stack
+    stack = singular_stack()
+    rng = np.random.default_rng(7)
+    kr = rng.uniform(0.05, 3.0, 31) * W0
+    ph = rng.uniform(0, 2 * np.pi, 31)
+    kx = np.concatenate([[W0], kr * np.cos(ph)])
+    ky = np.concatenate([[0.0], kr * np.sin(ph)])
+    for src, obs, zs, zo in ((1, 1, 0.3, 0.2), (1, 6, 0.3, 0.1), (5, 1, 0.01, 0.3), (5, 5, 0.01, 0.02), (4, 4, 0.05, 0.08)):
+        g, og = _run(host_node, tmp_path, stack, kx, ky, src, obs, [zs], [zo])
+        assert _run.redo == 1, (src, obs, _run.redo)
+        assert np.isfinite(g[0]).all(), (src, obs)
+        assert relerr(g, og) < 1e-10, (src, obs, relerr(g, og))
+        assert np.abs(g[0] - og[0]).max() <= 1e-10 * max(np.abs(og[0]).max(), 1e-300), (src, obs)

@@ -33,13 +33,18 @@ struct BatchArgs {
   double omega;
   const double *kx, *ky;
   cd *S, *V1, *Q1, *VN, *QN;  // [e][ntot] (node fastest); any may be null except S
-  unsigned char *flags;       // [ntot]: 1 valid, 4 singular solve, 8 mode failure
+  unsigned char *flags;       // [ntot]: 1 valid, 4 singular interface (no redo yet), 8 mode failure
   int64_t ntot;
   unsigned int *counter;
+  int *redo;  // redo marks (two words, a byte per node)
 };
 
-template <bool GENERAL>
-__device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_t g, const LaneMat &Vp) {
+// One node of one stack.  ROBUST = false: an interface LU that meets an
+// exactly zero pivot only sets `redo`; the caller solves the node again with
+// ROBUST = true, which takes the reference's pivoted-QR fallback there
+// (mod_ScatteringMatrix.jl:35-44).
+template <bool GENERAL, bool ROBUST>
+__device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_t g, const LaneMat &Vp, bool &redo) {
   const double kx = A.kx[g], ky = A.ky[g], w = A.omega;
   cd p[4], acc[16];
   bool ok;
@@ -56,14 +61,15 @@ __device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_
       for (int i = 0; i < 4; ++i) A.Q1[i * A.ntot + g] = Q[i];
     prop4(Q, L[0].d, p);
   }
-  bool singular = false;
+  uint64_t bits = 0;
   for (int j = 1; j < nl; ++j) {
     cd Q[4], Lm[16], Si[16];
     {
       cd V[16];
       ok = modes_layer<GENERAL>(L[j], w, kx, ky, V, Q) && ok;
-      match_side_lds(Vp, V, Q, 0);
-      match_side_lds(Vp, V, Q, 2);
+      const bool sf = match_side_lds(Vp, V, Q, 0);
+      const bool sb = match_side_lds(Vp, V, Q, 2);
+      bits |= ((uint64_t)sf << (2 * j)) | ((uint64_t)sb << (2 * j + 1));
       // _S_Interface_Matrix: [-Vf_B  Vb_A] \ [-Vf_A  Vb_B]
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -83,7 +89,17 @@ __device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_
           for (int i = 0; i < 4; ++i) A.QN[i * A.ntot + g] = Q[i];
       }
     }
-    singular = !lu_solve4<4>(Lm, Si) || singular;
+    const bool lu_ok = lu_solve4<4>(Lm, Si);
+    if (!ROBUST) redo = redo || !lu_ok;
+    else if (!lu_ok) {
+      // SingularException: layer j-1 re-solved in its matched order, layer j
+      // from LDS, and the pivoted-QR fallback
+      cd Sf[16];
+      interface_fallback<GENERAL>(&L[j - 1], w, kx, ky, (bits >> (2 * (j - 1))) & 1, (bits >> (2 * (j - 1) + 1)) & 1,
+                                  Vp.p + Vp.lane, false, Sf);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Si[i] = Sf[i];
+    }
     cd St[16];
     if (j == 1) redheffer_ldiag(p, Si, St);
     else redheffer(acc, Si, St);
@@ -98,10 +114,11 @@ __device__ void stack_node(const BatchArgs &A, const LayerDev *L, int nl, int64_
   bool fin = true;
 #pragma unroll
   for (int i = 0; i < 16; ++i) fin = fin && isfin(acc[i]);
-  const bool valid = ok && !singular && fin;
+  // a fast solve that met a singular interface is invalid until its redo
+  const bool valid = ok && fin && !redo;
 #pragma unroll
   for (int i = 0; i < 16; ++i) A.S[i * A.ntot + g] = valid ? acc[i] : mk(0.0);
-  if (A.flags) A.flags[g] = (unsigned char)((valid ? 1u : 0u) | (singular ? 4u : 0u) | (ok ? 0u : 8u));
+  if (A.flags) A.flags[g] = (unsigned char)((valid ? 1u : 0u) | (redo ? 4u : 0u) | (ok ? 0u : 8u));
 }
 
 template <bool GENERAL, int WPB>
@@ -119,8 +136,34 @@ __global__ __launch_bounds__(64 * WPB) void k_batch_persistent(BatchArgs A) {
     const int nl = __builtin_amdgcn_readfirstlane(A.stack_nl[s]);
     const LayerDev *L = A.L + A.stack_l0[s];
     const int64_t n0 = A.stack_n0[s], nn = A.stack_nn[s];
-    for (int k = first + lane; k < first + A.nodes_per_item && k < nn; k += 64) stack_node<GENERAL>(A, L, nl, n0 + k, Vp);
+    for (int k = first + lane; k < first + A.nodes_per_item && k < nn; k += 64) {
+      bool redo = false;
+      stack_node<GENERAL, false>(A, L, nl, n0 + k, Vp, redo);
+      if (__builtin_expect(redo, 0)) redo_mark(A.redo, n0 + k);
+    }
   }
+}
+
+// The nodes whose interface solve was exactly singular, solved again with the
+// reference's pivoted-QR fallback (mod_ScatteringMatrix.jl:35-44); the marks
+// are cleared for the next launch or graph replay.
+template <bool GENERAL>
+__global__ __launch_bounds__(64) void k_batch_redo(BatchArgs A, int n_stacks) {
+  if (!redo_any(A.redo)) return;
+  __shared__ cd vprev[16 * 64];
+  const LaneMat Vp{vprev, (int)threadIdx.x};
+  for (int64_t g = blockIdx.x * 64 + threadIdx.x; g < A.ntot; g += (int64_t)gridDim.x * 64) {
+    if (!redo_take(A.redo, g)) continue;
+    int lo = 0, hi = n_stacks - 1;  // the stack of node g: the last with stack_n0 <= g
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (A.stack_n0[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    bool again = false;
+    stack_node<GENERAL, true>(A, A.L + A.stack_l0[lo], A.stack_nl[lo], g, Vp, again);
+  }
+  redo_finish(A.redo);
 }
 
 __global__ void k_batch_reset(unsigned int *counter) {
@@ -160,11 +203,15 @@ hipError_t launch_batch(const BatchLaunch &B, hipStream_t s) {
   A.flags = B.flags;
   A.ntot = B.ntot;
   A.counter = B.counter;
+  A.redo = B.redo;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, B.counter);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return B.general ? launch_g<true>(A, B.waves_per_block, B.blocks, s)
-                   : launch_g<false>(A, B.waves_per_block, B.blocks, s);
+  e = B.general ? launch_g<true>(A, B.waves_per_block, B.blocks, s) : launch_g<false>(A, B.waves_per_block, B.blocks, s);
+  if (e != hipSuccess) return e;
+  if (B.general) hipLaunchKernelGGL(k_batch_redo<true>, dim3(kRedoBlocks), dim3(64), 0, s, A, B.n_stacks);
+  else hipLaunchKernelGGL(k_batch_redo<false>, dim3(kRedoBlocks), dim3(64), 0, s, A, B.n_stacks);
+  return hipGetLastError();
 }
 
 }  // namespace ytf

@@ -60,6 +60,7 @@ int ytf_ctx_destroy(ytf_ctx *c) {
   if (c->small) hipFree(c->small);
   if (c->work) hipFree(c->work);
   if (c->aux) hipFree(c->aux);
+  if (c->redo) hipFree(c->redo);
   if (c->pinned) hipHostFree(c->pinned);
   if (c->own) hipStreamDestroy(c->own);
   delete c;
@@ -135,7 +136,7 @@ static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers,
                 source_layer, n_layers - 1);
   if (obs_layer < 1 || obs_layer > n_layers)
     return fail(c, YTF_EINVAL, "ytf_green_nodes: obs_layer=%d out of range 1:%d", obs_layer, n_layers);
-  if (n_nodes < 0) return fail(c, YTF_EINVAL, "ytf_green_nodes: n_nodes < 0");
+  if (n_nodes < 0 || n_nodes > INT32_MAX) return fail(c, YTF_EINVAL, "ytf_green_nodes: n_nodes must be in 0:2^31-1");
   if (terms < YTF_TERMS_ALL || terms > YTF_TERMS_HANKEL)
     return fail(c, YTF_EINVAL, "closed_form_G: terms must be :all, :direct, :scattered, :toeplitz or :hankel (got %d)",
                 terms);
@@ -184,6 +185,8 @@ static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers,
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->small_host = hb;
   }
+  int rc_redo = ensure_redo(c, n_nodes);
+  if (rc_redo) return rc_redo;
   char *sm = (char *)c->small;
   ytf::NodeParams &P = *P_out;
   P.L = (const LayerDev *)sm;
@@ -209,6 +212,7 @@ static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers,
   P.flags = cache ? (unsigned char *)cache->flags_dev : nullptr;
   if ((P.sf == nullptr) != (P.sb == nullptr)) return fail(c, YTF_EINVAL, "ytf_green_nodes: sf and sb go together");
   P.needs_general = stack_needs_general(layers, n_layers);
+  P.redo = (int *)c->redo;
   return YTF_OK;
 }
 
@@ -304,10 +308,13 @@ int ytf_cavity_response(ytf_ctx *c, const ytf_layer *layers, int n_layers, doubl
     return fail(c, YTF_EINVAL, "ytf_cavity_response: bad nodes");
   int rc = response_stack_locked(c, "ytf_cavity_response", layers, n_layers, omega, source_layer);
   if (rc || n_nodes == 0) return rc;
+  if (n_nodes > INT32_MAX) return fail(c, YTF_EINVAL, "ytf_cavity_response: at most 2^31 - 1 nodes per call");
+  rc = ensure_redo(c, n_nodes);
+  if (rc) return rc;
   const bool general = stack_needs_general(layers, n_layers);
   hipError_t e = timed(c, "cavity_response", [&] {
     return ytf::launch_cavity_response((const LayerDev *)c->small, n_layers, source_layer, omega, kx_dev, ky_dev,
-                                       n_nodes, out_dev, general, c->stream);
+                                       n_nodes, out_dev, general, (int *)c->redo, c->stream);
   });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_cavity_response: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
@@ -404,11 +411,12 @@ struct ytf_batch {
   int *d_l0 = nullptr, *d_nl = nullptr, *d_is = nullptr, *d_if = nullptr;
   int64_t *d_n0 = nullptr, *d_nn = nullptr;
   unsigned int *counter = nullptr;
+  int *redo = nullptr;  // redo marks of the nodes whose interface solve was exactly singular
   int n_items = 0;
   // launch configuration
   int nodes_per_item = 128, waves_per_block = 1, blocks = 0;
   int items_for = 0;  // nodes_per_item the item table was built for
-  // captured graph of (counter reset + persistent kernel) for one argument set
+  // captured graph of (counter reset + persistent kernel + redo) for one argument set
   hipGraphExec_t exec = nullptr;
   std::vector<const void *> exec_key;
   std::string signature;
@@ -461,6 +469,8 @@ static ytf::BatchLaunch batch_launch(ytf_batch *b, const double *kx, const doubl
   B.flags = flags;
   B.ntot = b->ntot;
   B.counter = b->counter;
+  B.redo = b->redo;
+  B.n_stacks = b->n_stacks;
   B.general = b->general;
   B.waves_per_block = b->waves_per_block;
   B.blocks = b->blocks;
@@ -524,7 +534,10 @@ int ytf_batch_create(ytf_ctx *c, const ytf_layer *layers, const int32_t *layer_c
   for (int s = 0; s < n_stacks; ++s) max_items += (b->nn[s] + 63) / 64;
   const size_t lb = align256(sizeof(ytf_layer) * nlt), ib = align256(sizeof(int) * n_stacks),
                jb = align256(sizeof(int64_t) * n_stacks), tb = align256(sizeof(int) * (size_t)(max_items + 1));
-  if (hipSetDevice(c->device) != hipSuccess || hipMalloc(&b->dev, lb + 2 * ib + 2 * jb + 2 * tb + 256) != hipSuccess) {
+  // redo marks: two words and a byte per node
+  const size_t rb = align256(2 * sizeof(int) + (size_t)b->ntot);
+  if (hipSetDevice(c->device) != hipSuccess ||
+      hipMalloc(&b->dev, lb + 2 * ib + 2 * jb + 2 * tb + 256 + rb) != hipSuccess) {
     delete b;
     return fail(c, YTF_EHIP, "ytf_batch_create: device allocation failed");
   }
@@ -537,7 +550,9 @@ int ytf_batch_create(ytf_ctx *c, const ytf_layer *layers, const int32_t *layer_c
   b->d_is = (int *)(d + lb + 2 * ib + 2 * jb);
   b->d_if = (int *)(d + lb + 2 * ib + 2 * jb + tb);
   b->counter = (unsigned int *)(d + lb + 2 * ib + 2 * jb + 2 * tb);
+  b->redo = (int *)(d + lb + 2 * ib + 2 * jb + 2 * tb + 256);
   hipError_t e = hipMemcpy(b->L, layers, sizeof(ytf_layer) * nlt, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->redo, 0, rb);
   if (e == hipSuccess) e = hipMemcpy(b->d_l0, b->l0.data(), sizeof(int) * n_stacks, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_nl, b->nl.data(), sizeof(int) * n_stacks, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_n0, b->n0.data(), sizeof(int64_t) * n_stacks, hipMemcpyHostToDevice);
@@ -611,7 +626,7 @@ static int batch_solve_locked(ytf_ctx *c, ytf_batch *b, const double *kx, const 
     if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_batch_solve: launch failed: %s", hipGetErrorString(e));
     return YTF_OK;
   }
-  // one graph per argument set: (counter reset, persistent kernel)
+  // one graph per argument set: (counter reset, persistent kernel, redo)
   std::vector<const void *> key = {kx, ky, S, V1, Q1, VN, QN, flags, (const void *)(intptr_t)b->nodes_per_item,
                                    (const void *)(intptr_t)b->waves_per_block, (const void *)(intptr_t)b->blocks};
   if (!b->exec || key != b->exec_key) {

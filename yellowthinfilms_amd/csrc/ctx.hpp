@@ -40,6 +40,10 @@ struct ytf_ctx {
   size_t work_bytes = 0;
   void *aux = nullptr;
   size_t aux_bytes = 0;
+  // redo marks of the node solves (two words, a byte per node): zero between
+  // launches, grown on demand
+  void *redo = nullptr;
+  size_t redo_bytes = 0;
   // pinned host words for counts read back
   int *pinned = nullptr;
   // kernel timing: events bracketing each launch while enabled
@@ -98,6 +102,18 @@ inline int ensure_buf(ytf_ctx *c, void **buf, size_t *cur, size_t bytes) {
   size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes;
   HIPCHK(c, hipMalloc(buf, nb));
   *cur = nb;
+  return YTF_OK;
+}
+
+// The node solves' redo marks for n rows (two words and a byte per row;
+// zeroed when it grows; each redo launch leaves it zero again).
+inline int ensure_redo(ytf_ctx *c, int64_t n) {
+  const size_t bytes = 2 * sizeof(int) + (size_t)n;
+  if (bytes <= c->redo_bytes) return YTF_OK;
+  int rc = ensure_buf(c, &c->redo, &c->redo_bytes, bytes);
+  if (rc) return rc;
+  HIPCHK(c, hipMemsetAsync(c->redo, 0, c->redo_bytes, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return YTF_OK;
 }
 

@@ -666,6 +666,163 @@ __device__ __forceinline__ bool lu_solve4(cd *A, cd *B) {
   return ok;
 }
 
+// The pivoted-QR minimum-norm solve that replaces an exactly singular LU in
+// _S_Interface_Matrix (the SingularException fallback,
+// mod_ScatteringMatrix.jl:35-44: `qr(L, ColumnNorm()) \ R`, the minimum-norm
+// least-squares solution of the rank-r truncated system).  In place, B <- X.
+// Column pivots by the largest downdated norm; the rank r is the first k with
+// |R_kk| <= 4 eps |R_11|.  The minimum-norm solution of the r x 4 system
+// R y = c comes from a Householder QR of R^H = Q2 [T; 0]: y = Q2 [T^-H c; 0].
+// Per-lane pivots and ranks are selects; every index is a compile-time constant.
+template <int N>
+__device__ __forceinline__ void qr_minnorm4(cd *A, cd *B) {
+  constexpr double kTol = 4.0 * 2.220446049250313e-16;
+  int jp[4] = {0, 1, 2, 3};
+  double cn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) cn[j] = abs2(A[4 * j]) + abs2(A[4 * j + 1]) + abs2(A[4 * j + 2]) + abs2(A[4 * j + 3]);
+  int rank = 4;
+  bool act = true;
+  double rmax = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = k;
+    double cm = cn[k];
+#pragma unroll
+    for (int j = k + 1; j < 4; ++j) {
+      const bool t = cn[j] > cm;
+      cm = t ? cn[j] : cm;
+      p = t ? j : p;
+    }
+#pragma unroll
+    for (int j = k + 1; j < 4; ++j) {
+      const bool t = act && p == j;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const cd a = A[i + 4 * k], b = A[i + 4 * j];
+        A[i + 4 * k] = sel(t, b, a);
+        A[i + 4 * j] = sel(t, a, b);
+      }
+      const double c0 = cn[k], c1 = cn[j];
+      cn[k] = t ? c1 : c0;
+      cn[j] = t ? c0 : c1;
+      const int i0 = jp[k], i1 = jp[j];
+      jp[k] = t ? i1 : i0;
+      jp[j] = t ? i0 : i1;
+    }
+    double nrm2 = 0.0;
+#pragma unroll
+    for (int i = k; i < 4; ++i) nrm2 += abs2(A[i + 4 * k]);
+    const double nrm = sqrt(nrm2);
+    if (k == 0) rmax = nrm;
+    const bool cut = !(nrm > kTol * fmax(rmax, 1e-300));
+    rank = (act && cut) ? k : rank;
+    act = act && !cut;
+    // reflector H = I - 2 v v^H / (v^H v) with H x = alpha e_k, alpha = -|x| x_k / |x_k|
+    const cd a0 = A[k + 4 * k];
+    const double aa = cabs(a0);
+    const cd alpha = rmul(-nrm, aa == 0.0 ? mk(1.0) : rmul(1.0 / aa, a0));
+    cd v[4];
+    v[k] = a0 - alpha;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) v[i] = A[i + 4 * k];
+    double vn = 0.0;
+#pragma unroll
+    for (int i = k; i < 4; ++i) vn += abs2(v[i]);
+    const double f = (act && vn > 0.0) ? 2.0 / vn : 0.0;
+#pragma unroll
+    for (int j = k; j < 4; ++j) {
+      cd t = mk(0.0);
+#pragma unroll
+      for (int i = k; i < 4; ++i) t = fmac_cj(t, v[i], A[i + 4 * j]);
+      t = rmul(f, t);
+#pragma unroll
+      for (int i = k; i < 4; ++i) A[i + 4 * j] = A[i + 4 * j] - v[i] * t;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      cd t = mk(0.0);
+#pragma unroll
+      for (int i = k; i < 4; ++i) t = fmac_cj(t, v[i], B[i + 4 * j]);
+      t = rmul(f, t);
+#pragma unroll
+      for (int i = k; i < 4; ++i) B[i + 4 * j] = B[i + 4 * j] - v[i] * t;
+    }
+#pragma unroll
+    for (int j = k + 1; j < 4; ++j) cn[j] -= abs2(A[k + 4 * j]);
+  }
+  // H = R^H on the leading rank rows; its QR holds the reflectors in its lower
+  // part (column k from row k), T's strict upper part above, T's diagonal in td
+  cd H[16], td[4];
+  double fk[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) H[j + 4 * i] = (i < rank && j >= i) ? conj(A[i + 4 * j]) : mk(0.0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double nrm2 = 0.0;
+#pragma unroll
+    for (int i = k; i < 4; ++i) nrm2 += abs2(H[i + 4 * k]);
+    const double nrm = sqrt(nrm2);
+    const cd a0 = H[k + 4 * k];
+    const double aa = cabs(a0);
+    const cd alpha = rmul(-nrm, aa == 0.0 ? mk(1.0) : rmul(1.0 / aa, a0));
+    cd v[4];
+    v[k] = a0 - alpha;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) v[i] = H[i + 4 * k];
+    double vn = 0.0;
+#pragma unroll
+    for (int i = k; i < 4; ++i) vn += abs2(v[i]);
+    const bool on = k < rank;
+    const double f = (on && vn > 0.0) ? 2.0 / vn : 0.0;
+#pragma unroll
+    for (int j = k + 1; j < 4; ++j) {
+      cd t = mk(0.0);
+#pragma unroll
+      for (int i = k; i < 4; ++i) t = fmac_cj(t, v[i], H[i + 4 * j]);
+      t = rmul(f, t);
+#pragma unroll
+      for (int i = k; i < 4; ++i) H[i + 4 * j] = H[i + 4 * j] - v[i] * t;
+    }
+#pragma unroll
+    for (int i = k; i < 4; ++i) H[i + 4 * k] = v[i];
+    td[k] = on ? alpha : mk(1.0);
+    fk[k] = f;
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    cd y[4];
+    // T^H w = c (T^H lower triangular: (T^H)_im = conj(T_mi) = conj(H_mi), m < i)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cd t = B[i + 4 * j];
+#pragma unroll
+      for (int m = 0; m < i; ++m) t = t - conj(H[m + 4 * i]) * y[m];
+      y[i] = i < rank ? t * cinv(conj(td[i])) : mk(0.0);
+    }
+    // y = Q2 w, Q2 = H_0 H_1 ... H_{rank-1}
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+      cd t = mk(0.0);
+#pragma unroll
+      for (int i = k; i < 4; ++i) t = fmac_cj(t, H[i + 4 * k], y[i]);
+      t = rmul(fk[k], t);
+#pragma unroll
+      for (int i = k; i < 4; ++i) y[i] = y[i] - H[i + 4 * k] * t;
+    }
+    // undo the column pivots: X[jp[i]] = y[i]
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      cd x = mk(0.0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x = sel(jp[i] == r, y[i], x);
+      B[r + 4 * j] = x;
+    }
+  }
+}
+
 // The closed form's mode coefficients: with u = (c, s), v = (-s, c) the TE
 // columns are E = v, H = -+a u and the TM columns E = +-b u, H = v, where
 // a = kzo / (w mu) and b = kze / (w eps_t) (mod_ModeSolver.jl:243-265).
@@ -690,18 +847,54 @@ __device__ __forceinline__ void s_interface_closed(cd aA, cd bA, cd aB, cd bB, c
   S[10] = rmul(2.0, aB) * ra;      S[15] = rmul(2.0, bB) * rb;       // S22: bA <- bB
 }
 
-// _S_Interface_Matrix -- mod_ScatteringMatrix.jl:27-47:
-// S = [-Vf_B  Vb_A] \ [-Vf_A  Vb_B]
-__device__ __forceinline__ bool s_interface(const cd *VA, const cd *VB, cd *S) {
-  cd L[16];
+// The system of _S_Interface_Matrix: L = [-Vf_B  Vb_A], R = [-Vf_A  Vb_B].
+__device__ __forceinline__ void interface_system(const cd *VA, const cd *VB, cd *L, cd *R) {
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     L[i] = -VB[i];
     L[8 + i] = VA[8 + i];
-    S[i] = -VA[i];
-    S[8 + i] = VB[8 + i];
+    R[i] = -VA[i];
+    R[8 + i] = VB[8 + i];
   }
+}
+
+// _S_Interface_Matrix -- mod_ScatteringMatrix.jl:27-47:
+// S = [-Vf_B  Vb_A] \ [-Vf_A  Vb_B] by LU.  Returns false on an exactly zero
+// pivot (the reference's SingularException).
+__device__ __forceinline__ bool s_interface(const cd *VA, const cd *VB, cd *S) {
+  cd L[16];
+  interface_system(VA, VB, L, S);
   return lu_solve4<4>(L, S);
+}
+
+// The same with the reference's answer to a SingularException (:35-44): lanes
+// whose LU meets an exactly zero pivot rebuild the system and take the
+// pivoted-QR minimum-norm solve.
+__device__ __forceinline__ void s_interface_qr(const cd *VA, const cd *VB, cd *S) {
+  if (!s_interface(VA, VB, S)) {
+    cd L[16];
+    interface_system(VA, VB, L, S);
+    qr_minnorm4<4>(L, S);
+  }
+}
+
+// The QR fallback of an interface solve in the passes that keep the previous
+// layer's modes in LDS: one side's matched modes are re-solved (layer Lr with
+// its recorded swaps), the other side's come from the lane's LDS slots
+// (Vlds[i * 64], i = 0..15); lds_is_A says which side LDS holds.  S receives
+// L \ R by the pivoted-QR minimum-norm solve.  Kept out of line so that its
+// mode solve and QR do not add to the register pressure of the pass.
+template <bool GENERAL>
+__device__ __noinline__ void interface_fallback(const LayerDev *Lr, double w, double kx, double ky, bool swf, bool swb,
+                                                const cd *Vlds, bool lds_is_A, cd *S) {
+  cd Vr[16], Qr[4], Vl[16], L[16];
+  modes_layer<GENERAL>(*Lr, w, kx, ky, Vr, Qr);
+  apply_swaps(swf, swb, Vr, Qr);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) Vl[i] = Vlds[i * 64];
+  if (lds_is_A) interface_system(Vl, Vr, L, S);
+  else interface_system(Vr, Vl, L, S);
+  qr_minnorm4<4>(L, S);
 }
 
 // Redheffer star product -- mod_ScatteringMatrix.jl:65-85
@@ -1019,9 +1212,9 @@ __device__ __forceinline__ void prop4(const cd *Q, double d, cd *p) {
 //     the modes re-solved in the recorded matched order.
 // Results go to the lane's LDS slots; returns the recorded swap bits
 // (bit 2j: layer j's forward pair swapped, bit 2j+1: its backward pair).
-template <int CASE, bool GENERAL>
+template <int CASE, bool GENERAL, bool ROBUST>
 __device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, const LaneMat &st, bool &ok,
-                                    bool &sing) {
+                                    bool &redo) {
   const double w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
   uint64_t bits = 0;
@@ -1065,7 +1258,17 @@ __device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, c
         for (int i = 0; i < 4; ++i) L12[i] = mk(0.0);
         L22[0] = p[2]; L22[1] = mk(0.0); L22[2] = mk(0.0); L22[3] = p[3];
       } else if (need_int) {
-        sing = !lu_solve4<4>(Lm, Si) || sing;  // Si = Sint[j-1]
+        const bool lu_ok = lu_solve4<4>(Lm, Si);  // Si = Sint[j-1]
+        if (!ROBUST) redo = redo || !lu_ok;
+        else if (!lu_ok) {
+          // SingularException: layer j-1 re-solved in its matched order,
+          // layer j from LDS, and the QR fallback
+          cd Sf[16];
+          interface_fallback<GENERAL>(&P.L[j - 1], w, kx, ky, (bits >> (2 * (j - 1))) & 1,
+                                      (bits >> (2 * (j - 1) + 1)) & 1, st.p + SL_V * 64 + st.lane, false, Sf);
+#pragma unroll
+          for (int i = 0; i < 16; ++i) Si[i] = Sf[i];
+        }
         if (j <= s) {
           cd B11[4], B12[4], B21[4], B22[4];
           blk(Si, 0, 0, B11); blk(Si, 0, 1, B12); blk(Si, 1, 0, B21); blk(Si, 1, 1, B22);
@@ -1142,7 +1345,17 @@ __device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, c
 #pragma unroll
         for (int i = 0; i < 16; ++i) st.set(SL_V + i, V[i]);
       }
-      sing = !lu_solve4<4>(Lm, Si) || sing;
+      const bool lu_ok = lu_solve4<4>(Lm, Si);
+      if (!ROBUST) redo = redo || !lu_ok;
+      else if (!lu_ok) {
+        // SingularException: layer kk from LDS, layer kk+1 re-solved in its
+        // matched order, and the QR fallback
+        cd Sf[16];
+        interface_fallback<GENERAL>(&P.L[kk + 1], w, kx, ky, (bits >> (2 * (kk + 1))) & 1,
+                                    (bits >> (2 * (kk + 1) + 1)) & 1, st.p + SL_V * 64 + st.lane, true, Sf);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) Si[i] = Sf[i];
+      }
       cd p[4];
       prop4(Q, L.d, p);
       cd e11[4], e12[4], e21[4], e22[4];
@@ -1179,8 +1392,43 @@ __device__ uint64_t cascade_general(const NodeParams &P, double kx, double ky, c
   return bits;
 }
 
-template <int CASE, bool GENERAL>
-__device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
+// Redo lists.  A fast solve that meets an exactly singular interface marks
+// its node (one byte per node from byte 8; word 0 "some node is marked", word 1
+// a block counter); a redo launch of kRedoBlocks blocks solves the marked
+// nodes again with the reference's pivoted-QR fallback, clears their marks,
+// and its last block clears the two words.  Plain stores (no atomics) keep the
+// fast kernels' register allocation as it is.
+constexpr int kRedoBlocks = 16;
+__device__ __forceinline__ void redo_mark(int *list, int64_t k) {
+  ((unsigned char *)(list + 2))[k] = 1;
+  list[0] = 1;
+}
+__device__ __forceinline__ bool redo_any(const int *list) { return __atomic_load_n(list, __ATOMIC_RELAXED) != 0; }
+__device__ __forceinline__ bool redo_take(int *list, int64_t k) {
+  unsigned char *m = (unsigned char *)(list + 2) + k;
+  if (!*m) return false;
+  *m = 0;
+  return true;
+}
+// The redo launch's last block clears the two words.
+__device__ __forceinline__ void redo_finish(int *list) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(list + 1, 1) == (int)gridDim.x - 1) {
+      list[0] = 0;
+      list[1] = 0;
+    }
+  }
+}
+
+// One Green k-node.  ROBUST = false is the fast solve: an interface LU that
+// meets an exactly zero pivot only sets `redo`, and the caller solves that
+// node again with ROBUST = true, which takes the reference's pivoted-QR
+// fallback there (mod_ScatteringMatrix.jl:35-44).  Keeping the fallback out of
+// the fast solve leaves its register allocation as it is.
+template <int CASE, bool GENERAL, bool ROBUST>
+__device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bool &redo) {
   if (P.rows) k = P.rows[k];
   const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
@@ -1200,7 +1448,17 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
       cd V[16], Q[4];
       bool canon = false;
       ok = modes_layer<GENERAL>(L, w, kx, ky, V, Q, &canon) && ok;
-      if (__all(canon && canon_n)) {
+      bool regular = true;
+      if (ROBUST) {
+        // x_A + x_B = 0 is the exactly singular closed-form interface: the
+        // general solve below takes the reference's LU and its QR fallback
+        cd aA, bA, aB, bB;
+        closed_ab(L, w, Q, aA, bA);
+        closed_ab(P.L[j + 1], w, Qn, aB, bB);
+        const cd sa = aA + aB, sb = bA + bB;
+        regular = (sa.r != 0.0 || sa.i != 0.0) && (sb.r != 0.0 || sb.i != 0.0);
+      }
+      if (__all(canon && canon_n && regular)) {
         // both layers in the closed form's slot order: TE and TM decouple and
         // every block of the interface S and of the layer's (S_int, S_prop)
         // product is diagonal (per polarisation c: TE 0, TM 1)
@@ -1209,7 +1467,9 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
         closed_ab(P.L[j + 1], w, Qn, aB, bB);
         cd Sint[16];
         s_interface_closed(aA, bA, aB, bB, Sint);
-        sing = !(isfin(Sint[0]) && isfin(Sint[5]) && isfin(Sint[8]) && isfin(Sint[13])) || sing;
+        const bool fin = isfin(Sint[0]) && isfin(Sint[5]) && isfin(Sint[8]) && isfin(Sint[13]);
+        if (ROBUST) sing = !fin || sing;
+        else redo = redo || !fin;
         cd p[2];
         p[0] = expi(Q[0], L.d);
         if (L.ek == 0) p[1] = p[0];
@@ -1233,7 +1493,8 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
         R[3] = e21[1] + R[3];
       } else {
         cd Sint[16];
-        sing = !s_interface(V, Vn, Sint) || sing;
+        if (ROBUST) s_interface_qr(V, Vn, Sint);
+        else redo = !s_interface(V, Vn, Sint) || redo;
         // the closed form's backward slots hold -Q of the forward ones, so their
         // phases e^{-i Qb d} are the forward phases (an isotropic layer has one)
         cd p[4];
@@ -1276,7 +1537,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) Rs[i] = R[i];
   } else {
-    const uint64_t bits = cascade_general<CASE, GENERAL>(P, kx, ky, st, ok, sing);
+    const uint64_t bits = cascade_general<CASE, GENERAL, ROBUST>(P, kx, ky, st, ok, redo);
     swS_f = (bits >> (2 * s)) & 1;
     swS_b = (bits >> (2 * s + 1)) & 1;
     swO_f = (bits >> (2 * o)) & 1;
@@ -1504,6 +1765,8 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   for (int i = 0; i < 16; ++i) Tfin = Tfin && isfin(T[i]);
 
   // ---- node validity (mod_CavityTransfer.jl:226-251) ------------------------
+  // a fast solve that met a singular interface is invalid until its redo
+  sing = sing || redo;
   const bool valid = ok && !sing && seeds_fin;
   const bool Tvalid = valid && Tfin;
   cd basis[16];
@@ -1511,6 +1774,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   const cd z0 = mk(0.0);
   const int64_t nn = P.nn;
   if (P.flags) P.flags[k] = (unsigned char)((valid ? 1 : 0) | (Tvalid ? 2 : 0) | (sing ? 4 : 0) | (ok ? 0 : 8));
+  if (!ROBUST && __builtin_expect(redo, 0) && P.redo) redo_mark(P.redo, k);
   if (P.Qsrc) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) P.Qsrc[i * nn + k] = sel(valid, Qs[i], z0);
@@ -1660,14 +1924,38 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st) {
   }
 }
 
+// The fast node solve over every node (or every listed row).  A node whose
+// solve met an exactly singular interface is marked in P.redo.
 template <int CASE, bool GENERAL>
 __global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
   __shared__ cd lds[LdsSlots<CASE>::n * 64];
   const LaneMat st{lds, (int)threadIdx.x};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t n = P.n_rows ? min((int64_t)*P.n_rows, P.nn) : P.nn;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
-    solve_node<CASE, GENERAL>(P, k, st);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    bool redo = false;
+    solve_node<CASE, GENERAL, false>(P, k, st, redo);
+  }
+}
+
+// The nodes (rows) that node_kernel marked, solved again with the reference's
+// pivoted-QR fallback at the singular interfaces (mod_ScatteringMatrix.jl:35-44).
+// Nothing to do, the usual case, costs one load per block.
+template <int CASE, bool GENERAL>
+__global__ __launch_bounds__(64) void node_kernel_redo(NodeParams P) {
+  int *list = P.redo;
+  if (!redo_any(list)) return;
+  __shared__ cd lds[LdsSlots<CASE>::n * 64];
+  const LaneMat st{lds, (int)threadIdx.x};
+  NodeParams Q = P;
+  Q.rows = nullptr;
+  Q.n_rows = nullptr;
+  for (int64_t k = blockIdx.x * 64 + threadIdx.x; k < P.nn; k += (int64_t)gridDim.x * 64) {
+    if (!redo_take(list, k)) continue;
+    bool again = false;
+    solve_node<CASE, GENERAL, true>(Q, k, st, again);
+  }
+  redo_finish(list);
 }
 
 }  // namespace ytf

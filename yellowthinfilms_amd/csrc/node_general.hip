@@ -15,12 +15,17 @@
 
 namespace ytf {
 
+template <int CASE, bool GENERAL>
+static void launch(const NodeParams &P, int blocks, hipStream_t stream) {
+  hipLaunchKernelGGL((node_kernel<CASE, GENERAL>), dim3(blocks), dim3(64), 0, stream, P);
+  if (P.redo) hipLaunchKernelGGL((node_kernel_redo<CASE, GENERAL>), dim3(kRedoBlocks), dim3(64), 0, stream, P);
+}
+
 template <bool GENERAL>
 static void launch_case(const NodeParams &P, int blocks, hipStream_t stream) {
-  const int threads = 64;
-  if (P.obs == P.src) hipLaunchKernelGGL((node_kernel<CASE_SAME, GENERAL>), dim3(blocks), dim3(threads), 0, stream, P);
-  else if (P.obs > P.src) hipLaunchKernelGGL((node_kernel<CASE_RIGHT, GENERAL>), dim3(blocks), dim3(threads), 0, stream, P);
-  else hipLaunchKernelGGL((node_kernel<CASE_LEFT, GENERAL>), dim3(blocks), dim3(threads), 0, stream, P);
+  if (P.obs == P.src) launch<CASE_SAME, GENERAL>(P, blocks, stream);
+  else if (P.obs > P.src) launch<CASE_RIGHT, GENERAL>(P, blocks, stream);
+  else launch<CASE_LEFT, GENERAL>(P, blocks, stream);
 }
 
 hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream) {

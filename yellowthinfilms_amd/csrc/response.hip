@@ -14,9 +14,9 @@
 // (the star product is associative), and the prefix stops at the source.
 //   r = max(sigma(RL), sigma(RR)), and for s > 1 also sigma(inv(I - PhiF RL PhiB RR)),
 //   PhiF = e^{i Qf d_s}, PhiB = e^{-i Qb d_s},
-// with sigma the largest singular value of a 2x2 (_σmax, :91-95).  A failed
-// mode solve, an exactly singular interface solve (the reference falls back to
-// a pivoted QR there) or a non-finite r give 0.
+// with sigma the largest singular value of a 2x2 (_σmax, :91-95).  An exactly
+// singular interface solve takes the reference's pivoted-QR fallback; a failed
+// mode solve or a non-finite r give 0.
 #include <hip/hip_runtime.h>
 
 #include "node_core.hpp"
@@ -31,8 +31,12 @@ __device__ __forceinline__ double sigma_max2(cd a11, cd a21, cd a12, cd a22) {
   return sqrt(fmax((t + sqrt(fmax(t * t - 4.0 * dt, 0.0))) * 0.5, 0.0));
 }
 
-template <bool GENERAL>
-__device__ double response_node(const LayerDev *L, int nl, int s, double w, double kx, double ky, const LaneMat &Vp) {
+// ROBUST = false: an interface LU that meets an exactly zero pivot only sets
+// `redo` (see response), and the node is evaluated again with ROBUST = true,
+// which takes the reference's pivoted-QR fallback (mod_ScatteringMatrix.jl:35-44).
+template <bool GENERAL, bool ROBUST>
+__device__ double response_node(const LayerDev *L, int nl, int s, double w, double kx, double ky, const LaneMat &Vp,
+                                bool &redo) {
   cd p[4], acc[16], R[16], RL[4] = {}, Qs[4];
   bool ok;
   {
@@ -44,14 +48,15 @@ __device__ double response_node(const LayerDev *L, int nl, int s, double w, doub
     for (int i = 0; i < 4; ++i) Qs[i] = Q[i];
     prop4(Q, L[0].d, p);
   }
-  bool singular = false;
+  uint64_t bits = 0;
   for (int j = 1; j < nl; ++j) {  // interface j between layers j-1 and j (0-based) = Sint[j]
     cd Q[4], Lm[16], Si[16];
     {
       cd V[16];
       ok = modes_layer<GENERAL>(L[j], w, kx, ky, V, Q) && ok;
-      match_side_lds(Vp, V, Q, 0);
-      match_side_lds(Vp, V, Q, 2);
+      const bool sf = match_side_lds(Vp, V, Q, 0);
+      const bool sb = match_side_lds(Vp, V, Q, 2);
+      bits |= ((uint64_t)sf << (2 * j)) | ((uint64_t)sb << (2 * j + 1));
       // _S_Interface_Matrix: [-Vf_B  Vb_A] \ [-Vf_A  Vb_B]
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -63,7 +68,17 @@ __device__ double response_node(const LayerDev *L, int nl, int s, double w, doub
 #pragma unroll
       for (int i = 0; i < 16; ++i) Vp.set(i, V[i]);
     }
-    singular = !lu_solve4<4>(Lm, Si) || singular;
+    const bool lu_ok = lu_solve4<4>(Lm, Si);
+    if (!ROBUST) redo = redo || !lu_ok;
+    else if (!lu_ok) {
+      // SingularException: layer j-1 re-solved in its matched order, layer j
+      // from LDS, and the pivoted-QR fallback
+      cd Sf[16];
+      interface_fallback<GENERAL>(&L[j - 1], w, kx, ky, (bits >> (2 * (j - 1))) & 1, (bits >> (2 * (j - 1) + 1)) & 1,
+                                  Vp.p + Vp.lane, false, Sf);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) Si[i] = Sf[i];
+    }
     if (j < s) {  // prefix up to SLeft[s] = Sprop[1] * Sint[1] * ... * Sint[s-1]
       cd St[16];
       if (j == 1) redheffer_ldiag(p, Si, St);
@@ -108,7 +123,7 @@ __device__ double response_node(const LayerDev *L, int nl, int s, double w, doub
     const cd det = m11 * m22 - m12 * m21;
     r = fmax(r, sigma_max2(m22 / det, -m21 / det, -m12 / det, m11 / det));
   }
-  return (ok && !singular && isfinite(r)) ? r : 0.0;
+  return (ok && !redo && isfinite(r)) ? r : 0.0;
 }
 
 struct ResponseArgs {
@@ -118,24 +133,48 @@ struct ResponseArgs {
   const double *kx, *ky;
   int64_t n;
   double *out;
+  int *redo;  // redo marks (two words, a byte per node)
 };
 
+// The scan: a node whose interface solve was exactly singular gets 0 here and
+// is marked for the redo launch.
 template <bool GENERAL>
 __global__ __launch_bounds__(64) void k_cavity_response(ResponseArgs A) {
   __shared__ cd vprev[16 * 64];
   const LaneMat Vp{vprev, (int)threadIdx.x};
-  for (int64_t g = blockIdx.x * (int64_t)64 + threadIdx.x; g < A.n; g += (int64_t)gridDim.x * 64)
-    A.out[g] = response_node<GENERAL>(A.L, A.nl, A.s, A.omega, A.kx[g], A.ky[g], Vp);
+  for (int64_t g = blockIdx.x * (int64_t)64 + threadIdx.x; g < A.n; g += (int64_t)gridDim.x * 64) {
+    bool redo = false;
+    const double r = response_node<GENERAL, false>(A.L, A.nl, A.s, A.omega, A.kx[g], A.ky[g], Vp, redo);
+    A.out[g] = r;
+    if (__builtin_expect(redo, 0)) redo_mark(A.redo, g);
+  }
+}
+
+// The marked nodes again with the reference's pivoted-QR fallback at the
+// singular interfaces (mod_ScatteringMatrix.jl:35-44); marks cleared.
+template <bool GENERAL>
+__global__ __launch_bounds__(64) void k_cavity_response_redo(ResponseArgs A) {
+  if (!redo_any(A.redo)) return;
+  __shared__ cd vprev[16 * 64];
+  const LaneMat Vp{vprev, (int)threadIdx.x};
+  for (int64_t g = blockIdx.x * 64 + threadIdx.x; g < A.n; g += (int64_t)gridDim.x * 64) {
+    if (!redo_take(A.redo, g)) continue;
+    bool again = false;
+    A.out[g] = response_node<GENERAL, true>(A.L, A.nl, A.s, A.omega, A.kx[g], A.ky[g], Vp, again);
+  }
+  redo_finish(A.redo);
 }
 
 hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega, const double *kx, const double *ky,
-                                  int64_t n, double *out, bool general, hipStream_t st) {
+                                  int64_t n, double *out, bool general, int *redo, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  ResponseArgs A{L, nl, s, omega, kx, ky, n, out};
+  ResponseArgs A{L, nl, s, omega, kx, ky, n, out, redo};
   int64_t blocks = (n + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
   if (general) hipLaunchKernelGGL(k_cavity_response<true>, dim3((unsigned)blocks), dim3(64), 0, st, A);
   else hipLaunchKernelGGL(k_cavity_response<false>, dim3((unsigned)blocks), dim3(64), 0, st, A);
+  if (general) hipLaunchKernelGGL(k_cavity_response_redo<true>, dim3(kRedoBlocks), dim3(64), 0, st, A);
+  else hipLaunchKernelGGL(k_cavity_response_redo<false>, dim3(kRedoBlocks), dim3(64), 0, st, A);
   return hipGetLastError();
 }
 
@@ -318,7 +357,12 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
   const int side = (int)(chain & 1);
   const double *cd7 = A.cand + 7 * c;
   const double phi = cd7[0], cphi = cos(phi), sphi = sin(phi);
-  auto f = [&](double k) { return response_node<GENERAL>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp); };
+  // (an exactly singular interface evaluates to 0 here: the refinement has no
+  // redo pass; its points lie inside brackets of the scan)
+  bool sing_seen = false;
+  auto f = [&](double k) {
+    return response_node<GENERAL, false>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp, sing_seen);
+  };
   // lane l < 7: node l of the tree; its level and the decisions on its path
   const int lev = l == 0 ? 0 : (l < 3 ? 1 : 2);
   const bool dec1 = l == 1 || l == 3 || l == 4;  // level-1 decision (true: child 2p + 1)

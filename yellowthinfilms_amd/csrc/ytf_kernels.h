@@ -42,6 +42,12 @@ struct NodeParams {
   // the inputs and outputs (nn is then their row count); null: node k, k < nn.
   const int *rows = nullptr;
   const int *n_rows = nullptr;
+  // Redo marks of the rows whose fast solve met an exactly singular interface
+  // (word 0: some row marked, word 1: block counter, one byte per row from
+  // byte 8; zero on entry and cleared again by the redo launch).  With it the
+  // launch solves those rows again with the pivoted-QR fallback; without it
+  // they keep the fast solve's answer: invalid, with the singular-solve bit.
+  int *redo = nullptr;
 };
 
 // term classes of the spectral dyadic (mod_ClosedFormField.jl:143-160)
@@ -108,6 +114,8 @@ struct BatchLaunch {
   unsigned char *flags;
   int64_t ntot;
   unsigned int *counter;
+  int *redo;  // redo marks: word 0 some node marked, word 1 block counter, a byte per node from byte 8
+  int n_stacks;
   bool general;
   int waves_per_block, blocks;
 };
@@ -115,8 +123,10 @@ hipError_t launch_batch(const BatchLaunch &B, hipStream_t s);
 
 // Cavity response of a source layer (response.hip): the scan function of the
 // real-axis pole search; s 1-based.
+// redo: redo marks of the n nodes (word 0, word 1, a byte per node from byte
+// 8); zero on entry and left zero.
 hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega, const double *kx, const double *ky,
-                                  int64_t n, double *out, bool general, hipStream_t st);
+                                  int64_t n, double *out, bool general, int *redo, hipStream_t st);
 // Golden-section location and half-level crossings of nc candidates, every
 // response evaluated in the kernel (response.hip).
 hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,

@@ -51,7 +51,7 @@ class NodeCache:
       sf,sb (6, n)   2x3 seeds, column-major (mode, moment); sb = -sib_raw
       T     (16, n)  T^o, 4x4 column-major
       basis (16, n)  Exf Eyf Ezf Exb Eyb Ezb KZf KZb (2 modes each) of obs layer
-      flags (n,)     uint8: 1 valid, 2 Tvalid, 4 singular solve, 8 mode failure
+      flags (n,)     uint8: 1 valid, 2 Tvalid, 4 singular source projection, 8 mode failure
     """
 
     def __init__(self, n, device):
