@@ -34,6 +34,8 @@ struct ytf_ctx {
   std::map<int, cd *> tw;
   // NUFFT deconvolution factors per (modes, fine grid, kernel width)
   std::map<std::tuple<int, int, int>, double *> nu_deconv;
+  // NUFFT spreading kernel's piecewise-polynomial coefficients per width
+  std::map<int, double *> nu_horner;
   // scratch of the synthesis kernels and its host-side inputs (radii, pixel
   // coordinates, radius map), grown on demand, freed with the context
   void *work = nullptr;

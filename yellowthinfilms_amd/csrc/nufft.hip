@@ -41,11 +41,6 @@ __device__ __forceinline__ double fold2pi(double x) {
   return u;
 }
 
-__device__ __forceinline__ double es_kernel(double z, double beta) {
-  const double s = 1.0 - z * z;
-  return s > 0.0 ? exp(beta * (sqrt(s) - 1.0)) : 0.0;
-}
-
 // The sort is a counting sort by tile without global atomics.  Binning block
 // b takes points [b P, (b + 1) P): it computes their fine-grid coordinates
 // (u, v) in [0, nf) and tiles, ranks each point among the block's points of
@@ -192,12 +187,18 @@ __global__ void k_nu_zero(int64_t n, cd *__restrict__ g) {
 // into LDS (its tile + halo), then add the LDS tile to the transform's
 // periodic fine grid.  blockIdx.x = subproblem (blocks past the total exit),
 // blockIdx.y = transform (strengths c + y * c_stride, grid + y * g_stride).
+// The kernel values come from its piecewise-polynomial form (hc: tap i of a
+// point at offset t = mu - u + W/2 in [0, 1) is sum_d hc[d W + i] s^d with
+// s = 2 t - 1, degree W + 2), W (W + 2) FMAs per dimension instead of W
+// square roots and exponentials.
 template <int W>
-__global__ __launch_bounds__(256) void k_nu_spread(int nf, int nt, double beta, const int *__restrict__ start,
-                                                   const int *__restrict__ substart, const int *__restrict__ sub_tile,
-                                                   const int *__restrict__ sub_p0, const int *__restrict__ order,
-                                                   const double *__restrict__ uv, const cd *__restrict__ c,
-                                                   int64_t c_stride, cd *__restrict__ grid, int64_t g_stride) {
+__global__ __launch_bounds__(256) void k_nu_spread(int nf, int nt, const double *__restrict__ hc,
+                                                   const int *__restrict__ start, const int *__restrict__ substart,
+                                                   const int *__restrict__ sub_tile, const int *__restrict__ sub_p0,
+                                                   const int *__restrict__ order, const double *__restrict__ uv,
+                                                   const cd *__restrict__ c, int64_t c_stride, cd *__restrict__ grid,
+                                                   int64_t g_stride) {
+  constexpr int NP = W + 3;  // coefficients per tap
   constexpr int H = (W + 1) / 2;
   constexpr int S = NU_TS + 2 * H;
   __shared__ double tre[S * S], tim[S * S];
@@ -210,18 +211,22 @@ __global__ __launch_bounds__(256) void k_nu_spread(int nf, int nt, double beta, 
   for (int i = threadIdx.x; i < S * S; i += blockDim.x) { tre[i] = 0.0; tim[i] = 0.0; }
   __syncthreads();
   const int p0 = sub_p0[sub], p1 = min(p0 + NU_SUB, start[t + 1]);
-  const double inv_half = 2.0 / W;
   for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
     const int j = order[p];
     const double u = uv[2 * j], v = uv[2 * j + 1];
     const cd cj = c[j];
     const double mu = ceil(u - 0.5 * W), mv = ceil(v - 0.5 * W);
+    const double su = 2.0 * (mu - u) + (W - 1.0), sv = 2.0 * (mv - v) + (W - 1.0);
     double ku[W], kv[W];
 #pragma unroll
-    for (int i = 0; i < W; ++i) {
-      ku[i] = es_kernel((mu + i - u) * inv_half, beta);
-      kv[i] = es_kernel((mv + i - v) * inv_half, beta);
-    }
+    for (int i = 0; i < W; ++i) ku[i] = kv[i] = hc[(NP - 1) * W + i];
+#pragma unroll
+    for (int d = NP - 2; d >= 0; --d)
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        ku[i] = __builtin_fma(ku[i], su, hc[d * W + i]);
+        kv[i] = __builtin_fma(kv[i], sv, hc[d * W + i]);
+      }
     const int ou = (int)mu - tx * NU_TS + H, ov = (int)mv - ty * NU_TS + H;
 #pragma unroll
     for (int b = 0; b < W; ++b) {
@@ -355,9 +360,9 @@ hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf
   return hipGetLastError();
 }
 
-hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, const int *start, const int *substart,
-                               const int *sub_tile, const int *sub_p0, const int *order, const double *uv,
-                               const cd *c, int64_t c_stride, cd *grid, hipStream_t s) {
+hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, const double *hc, const int *start,
+                               const int *substart, const int *sub_tile, const int *sub_p0, const int *order,
+                               const double *uv, const cd *c, int64_t c_stride, cd *grid, hipStream_t s) {
   const int nt = nf / NU_TS;
   const int64_t ng = (int64_t)nf * nf;
   hipLaunchKernelGGL(k_nu_zero, dim3((unsigned)((nb * ng + 255) / 256)), dim3(256), 0, s, nb * ng, grid);
@@ -366,7 +371,7 @@ hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, co
   const dim3 g(nu_max_subproblems(M, nf), nb);
 #define YTF_SPREAD(WW)                                                                                         \
   case WW:                                                                                                     \
-    hipLaunchKernelGGL(k_nu_spread<WW>, g, dim3(256), 0, s, nf, nt, beta, start, substart, sub_tile, sub_p0, \
+    hipLaunchKernelGGL(k_nu_spread<WW>, g, dim3(256), 0, s, nf, nt, hc, start, substart, sub_tile, sub_p0,   \
                        order, uv, c, c_stride, grid, ng);                                                      \
     break;
   switch (w) {

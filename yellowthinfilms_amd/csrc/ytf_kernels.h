@@ -240,10 +240,11 @@ int nu_bin_blocks(int64_t M);
 hipError_t launch_nufft_sort(int64_t M, const double *x, const double *y, int nf, double *uv, int *tile, int *rank,
                              int *H, int *count, int *start, int *substart, int *sub_tile, int *sub_p0, int *order,
                              hipStream_t s);
-// nb transforms at once: strengths c + t * c_stride, fine grids grid + t * nf^2.
-hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, double beta, const int *start, const int *substart,
-                               const int *sub_tile, const int *sub_p0, const int *order, const double *uv,
-                               const cd *c, int64_t c_stride, cd *grid, hipStream_t s);
+// nb transforms at once: strengths c + t * c_stride, fine grids grid + t * nf^2;
+// hc: the kernel's piecewise-polynomial coefficients, (w + 3) x w, d-major.
+hipError_t launch_nufft_spread(int64_t M, int nb, int nf, int w, const double *hc, const int *start,
+                               const int *substart, const int *sub_tile, const int *sub_p0, const int *order,
+                               const double *uv, const cd *c, int64_t c_stride, cd *grid, hipStream_t s);
 // nb transforms at once: grids grid + t * nf^2, T1 + t * N1 nf, f + t * N1 N2.
 hipError_t launch_nufft_fft(int nb, int nf, const cd *grid, cd *T1, cd *f, int N1, int N2, const double *d1,
                             const double *d2, const cd *tw, hipStream_t s);
