@@ -89,6 +89,26 @@ def _timed_steps(a, step, torch):
     return time.perf_counter() - t0
 
 
+def _timed_kernels(a, step, torch, ctx, roof, others=()):
+    """The K timed steps with HIP events on the launches named in `roof` (a
+    name or a tuple of names: the roofline's kernels) and on no other launch,
+    since every event pair adds a few microseconds to the wall clock; then,
+    if `others` names more kernels, a second pass of K steps outside the wall
+    clock with every launch bracketed, for their per-kernel times.  Returns
+    (wall seconds of the timed steps, {name: (total ms, launches)})."""
+    roof = (roof,) if isinstance(roof, str) else tuple(roof)
+    ctx.set_timing_only(",".join(roof))
+    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+    t = {k: ctx.kernel_time(k) for k in roof}
+    if others:
+        ctx.set_timing(True)
+        for _ in range(a.steps):
+            step()
+        t.update({k: ctx.kernel_time(k) for k in others if k not in roof})
+    ctx.set_timing(False)
+    return wall, t
+
+
 def _calib_copy(a, torch, dev):
     # one device-to-device copy of known size: the scale of the FETCH_SIZE /
     # WRITE_SIZE counters in a profiling run
@@ -131,11 +151,9 @@ def bench_gyro(a):
 
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    node_ms, nlaunch = ctx.kernel_time("node_solve")
-    apply_ms, _ = ctx.kernel_time("spectral_apply")
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "node_solve", ("spectral_apply",))
+    node_ms, nlaunch = t["node_solve"]
+    apply_ms, _ = t["spectral_apply"]
     node_ms /= nlaunch
     apply_ms /= a.steps
     _calib_copy(a, torch, dev)
@@ -244,10 +262,8 @@ def bench_point(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    t = {k: ctx.kernel_time(k) for k in ("node_solve", "synth_coef", "synth_fp64", "synth_reduce", "synth_rotate")}
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "synth_fp64", ("node_solve", "synth_coef", "synth_reduce",
+                                                                  "synth_rotate"))
     n, nr, nz = len(kx), len(rho), len(zo)
     syn_ms = t["synth_fp64"][0] / t["synth_fp64"][1]
     flops = 36.0 * n * nr * nz  # 9 complex coefficients x real J per (node, radius, depth pair)
@@ -316,11 +332,8 @@ def bench_gemm(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
     names = ("node_solve", "synth_coef", "bessel_table", "split_columns", "bessel_gemm", "mfma_combine")
-    t = {k: ctx.kernel_time(k) for k in names}
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "bessel_gemm", names)
     gemm_ms = t["bessel_gemm"][0] / t["bessel_gemm"][1]
     alg = 4.0 * nr * n * 9 * npairs          # real J x complex coefficient, 9 complex columns per pair
     issued = 3 * 2.0 * nr * n * 18 * npairs  # three fp16 products per term on the real GEMM
@@ -397,10 +410,8 @@ def bench_uneven(a):
 
     for _ in range(a.warmup):
         step()
-    b.ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    ms, nl = b.ctx.kernel_time("batch_solve")
-    b.ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, b.ctx, "batch_solve")
+    ms, nl = t["batch_solve"]
     ms /= nl
     n = len(kx)
     flops = W.UNEVEN_256_FLOPS_PER_LAUNCH
@@ -517,10 +528,8 @@ def surrogate_runs(a, configs, windows=None):
 
     for _ in range(a.warmup):
         table()
-    ctx.set_timing(True)
-    wall_tab = _timed_steps(dict_ns(a, warmup=0), table, torch)
-    ktab_ms, _ = ctx.kernel_time("ktable")
-    ctx.set_timing(False)
+    wall_tab, tk = _timed_kernels(a, table, torch, ctx, "ktable")
+    ktab_ms, _ = tk["ktable"]
     gt, mt = gt_out[0].T, gt_out[1].bool()
     ktable = {
         "knodes_per_s": nn * a.steps / wall_tab, "speedup_vs_plain_gpu": (nn * a.steps / wall_tab) / plain_rate,
@@ -556,10 +565,8 @@ def surrogate_runs(a, configs, windows=None):
 
             for _ in range(a.warmup):
                 surrogate()
-            ctx.set_timing(True)
-            wall_sur = _timed_steps(dict_ns(a, warmup=0), surrogate, torch)
-            gemm_ms, _ = ctx.kernel_time("mlp_fused")
-            ctx.set_timing(False)
+            wall_sur, tk = _timed_kernels(a, surrogate, torch, ctx, "mlp_fused")
+            gemm_ms, _ = tk["mlp_fused"]
             gs, mask = res["g"], res["mask"]
             in_win = torch.zeros_like(mask)
             for lo, hi in win:
@@ -639,13 +646,10 @@ def bench_nufft(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    t = {k: ctx.kernel_time(k) for k in ("nufft_sort", "nufft_spread", "nufft_fft")}
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "nufft_apply", ("nufft_sort", "nufft_spread", "nufft_fft"))
     M = len(x)
     alg_bytes = M * (16 + 16) + N * N * 16  # points + strengths in, modes out
-    ms = sum(v[0] for v in t.values()) / a.steps
+    ms = t.pop("nufft_apply")[0] / a.steps  # the whole transform, one event pair per apply
     cpu, rel_err = None, None
     if not a.no_cpu_baseline:
         cpu, rel_err = cpu_nufft(x, y, c, f[0].cpu().numpy(), N)
@@ -707,13 +711,10 @@ def bench_operator(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    t = {k: ctx.kernel_time(k) for k in ("op_fft_fwd", "op_mix", "op_fft_inv")}
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "op_apply", ("op_fft_fwd", "op_mix", "op_fft_inv"))
     Px, Py = 2 * Nx, 2 * Ny
     alg_bytes = 16 * (3 * Nx * Ny + 9 * Px * Py + 3 * Nx * Ny)  # read J and Ĝ, write E
-    ms = sum(v[0] for v in t.values()) / a.steps
+    ms = t.pop("op_apply")[0] / a.steps  # the whole apply, one event pair
     cpu, rel_err, rel_apply = None, None, None
     if not a.no_cpu_baseline:
         cpu, rel_err, rel_apply = cpu_operator(stack, cache, op, P, grid, wl)
@@ -834,15 +835,12 @@ def bench_operator_volume(a):
 
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    t = {k: ctx.kernel_time(k) for k in ("op_fft_fwd", "op_mix", "op_fft_inv")}
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "op_apply", ("op_fft_fwd", "op_mix", "op_fft_inv"))
     n_obs = sum(b["n_obs"] for b in op.blocks)
     ghat_bytes = sum(b["bytes"] for b in op.blocks)
     # read J, read every stored plane and skeleton matrix once, write E
     alg_bytes = 16 * 3 * Nx * Ny * (Ns + n_obs) + ghat_bytes
-    ms = sum(v[0] for v in t.values()) / a.steps
+    ms = t.pop("op_apply")[0] / a.steps  # the whole apply, one event pair
     mix_ms = t["op_mix"][0] / a.steps
     cpu, rel_err, rel_apply = None, None, None
     if not a.no_cpu_baseline:
@@ -999,15 +997,17 @@ def bench_pole_scan(a):
     for _ in range(a.warmup):
         step_fn()
     kspace.cavity_response = counting
-    ctx.set_timing(True)
     try:
-        wall = _timed_steps(dict_ns(a, warmup=0), step_fn, torch)
+        wall, kms = _timed_kernels(a, step_fn, torch, ctx, ("cavity_response", "pole_refine"))
     finally:
         kspace.cavity_response = orig
-    kms = {k: ctx.kernel_time(k) for k in ("cavity_response", "pole_candidates", "pole_refine")}
+    ctx.set_timing_only("pole_candidates")  # the candidate kernel, in a pass outside the wall clock
+    for _ in range(a.steps):
+        step_fn()
+    kms["pole_candidates"] = ctx.kernel_time("pole_candidates")
+    ctx.set_timing(False)
     kt = kms["cavity_response"][0] + kms["pole_refine"][0]  # every response evaluation
     kn = kms["cavity_response"][1] + kms["pole_refine"][1]
-    ctx.set_timing(False)
     # scan nodes counted at the launch, refinement evaluations counted in the kernel
     nodes_per_scan = calls["n"] / a.steps + kspace.search_stats["refine_evaluations"]
     n_scan = len(phis) * (int(math.floor(math.log(k_hi / k_lo) / step + 1e-10)) + 1)
@@ -1136,11 +1136,9 @@ def bench_probes(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    node_ms, node_n = ctx.kernel_time("node_solve")
-    lat_ms, _ = ctx.kernel_time("lateral_segments")
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "node_solve", ("lateral_segments",))
+    node_ms, node_n = t["node_solve"]
+    lat_ms, _ = t["lateral_segments"]
     nodes = len(kr) * len(phis)
     flops = W.PROBE_BATCH_BOX_FLOPS_PER_NODE * nodes  # the 3 node-solve launches of one step
     node_step = node_ms / a.steps
@@ -1218,12 +1216,10 @@ def bench_box_grid(a):
     ctx = Context.get(0)
     for _ in range(a.warmup):
         step()
-    ctx.set_timing(True)
-    wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
-    node_ms, node_n = ctx.kernel_time("node_solve")
-    lat_ms, _ = ctx.kernel_time("lateral_segments")
-    scan_ms, _ = ctx.kernel_time("cavity_response")
-    ctx.set_timing(False)
+    wall, t = _timed_kernels(a, step, torch, ctx, "node_solve", ("lateral_segments", "cavity_response"))
+    node_ms, node_n = t["node_solve"]
+    lat_ms, _ = t["lateral_segments"]
+    scan_ms, _ = t["cavity_response"]
     kr, wr, phis, wph, info = res["g"]
     nk = info["n_krho_solves"]
     nodes = info["n_krho_evaluated"] * len(phis)          # every solved node, the speculative ones included

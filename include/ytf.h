@@ -85,6 +85,13 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * "nufft_spread", "nufft_fft", "lateral_direct", "op_fft_fwd", "op_mix",
  * "op_fft_inv". */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
+/* As ytf_ctx_set_timing(ctx, 1), but only the launches whose names appear in
+ * the comma-separated list `names` are bracketed, so the events (a few
+ * microseconds each) stay out of the other launches of a timed loop;
+ * ytf_ctx_set_timing ends it.  Besides the per-kernel names above,
+ * "nufft_apply" brackets a whole ytf_nufft2d1 call and "op_apply" a whole
+ * ytf_operator_apply call. */
+int ytf_ctx_set_timing_only(ytf_ctx *ctx, const char *names);
 int ytf_ctx_kernel_time(ytf_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 
 /* Device cache outputs of the node solve, structure-of-arrays with the node
@@ -381,8 +388,10 @@ int ytf_mlp_surrogate_ghat(ytf_ctx *ctx, ytf_mlp *mlp, const ytf_layer *layers, 
  * sab_dev[3p] to sab_dev[3p + 1]; kind_dev[p] & 3 selects s around the
  * panel's real branch point k_b = sab_dev[3p + 2] (0: k_rho, 1: sqrt(k_b^2 -
  * k_rho^2), 2: sqrt(k_rho^2 - k_b^2)).  Sample q of panel p is row
- * row0_dev[p] + q of val_dev (18 doubles: Re t_e for e < 9, Im t_{e-9} for
- * e >= 9, entry c + 3q).  A node is flagged (flags_dev[i] = 1) outside
+ * r = row0_dev[p] + q of the table, 18 doubles (Re t_e for e < 9, Im t_{e-9}
+ * for e >= 9, entry c + 3q) stored entry-major: entry e of row r at
+ * val_dev[e * n_rows + r], n_rows = row0_dev[n_panels - 1] + m_dev[n_panels - 1]
+ * (the four rows a node reads are then adjacent per entry).  A node is flagged (flags_dev[i] = 1) outside
  * [kb_dev[0], kb_dev[n_panels]], in a panel with kind bit 4 (not converged),
  * or at kz0 = 0; with fallback the flagged nodes are solved exactly as in
  * ytf_mlp_surrogate_ghat (source and observation layer 1, depths zsrc / zobs,

@@ -64,6 +64,7 @@ _SIGS = {
     "ytf_ctx_set_stream": (ct.c_int, [ct.c_void_p, ct.c_void_p]),
     "ytf_ctx_synchronize": (ct.c_int, [ct.c_void_p]),
     "ytf_ctx_set_timing": (ct.c_int, [ct.c_void_p, ct.c_int]),
+    "ytf_ctx_set_timing_only": (ct.c_int, [ct.c_void_p, ct.c_char_p]),
     "ytf_ctx_kernel_time": (ct.c_int, [ct.c_void_p, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
     "ytf_green_nodes": (ct.c_int, [ct.c_void_p, ct.POINTER(YtfLayer), ct.c_int, ct.c_double, ct.c_void_p,
                                    ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int,
@@ -227,6 +228,11 @@ class Context:
     def set_timing(self, on=True):
         """Bracket every kernel launch of this context with HIP events (clears totals)."""
         check(lib().ytf_ctx_set_timing(self.handle, 1 if on else 0), self.handle)
+
+    def set_timing_only(self, name):
+        """Bracket only the launches of kernel `name` with HIP events (clears totals);
+        set_timing ends it."""
+        check(lib().ytf_ctx_set_timing_only(self.handle, name.encode()), self.handle)
 
     def kernel_time(self, name):
         """(total ms, launches) of one kernel name since set_timing; synchronizes."""

@@ -381,6 +381,26 @@ int ytf_ctx_set_timing(ytf_ctx *c, int on) {
   if (rc) return rc;
   c->totals.clear();
   c->timing = on != 0;
+  c->timing_only.clear();
+  return YTF_OK;
+}
+
+int ytf_ctx_set_timing_only(ytf_ctx *c, const char *name) {
+  if (!c || !name || !*name) return fail(c, YTF_EINVAL, "ytf_ctx_set_timing_only: NULL or empty argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  int rc = drain_marks(c);
+  if (rc) return rc;
+  c->totals.clear();
+  c->timing = true;
+  c->timing_only.clear();
+  for (const char *p = name;;) {
+    const char *q = std::strchr(p, ',');
+    const std::string t = q ? std::string(p, q - p) : std::string(p);
+    if (!t.empty()) c->timing_only.push_back(t);
+    if (!q) break;
+    p = q + 1;
+  }
+  if (c->timing_only.empty()) c->timing = false;
   return YTF_OK;
 }
 
@@ -1442,7 +1462,12 @@ int ytf_nufft2d1(ytf_ctx *c, int64_t M, const double *x_dev, const double *y_dev
                  int N1, int N2, double eps, double *f_dev) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_nufft2d1: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
-  return nufft2d1_locked(c, M, x_dev, y_dev, ntrans, c_dev, N1, N2, eps, f_dev);
+  int rc = YTF_OK;
+  timed(c, "nufft_apply", [&] {
+    rc = nufft2d1_locked(c, M, x_dev, y_dev, ntrans, c_dev, N1, N2, eps, f_dev);
+    return hipSuccess;
+  });
+  return rc;
 }
 
 }  // extern "C"

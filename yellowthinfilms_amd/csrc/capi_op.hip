@@ -1197,7 +1197,12 @@ extern "C" {
 int ytf_operator_apply(ytf_ctx *c, ytf_operator *op, const double *P_dev, double *const *E_dev) {
   if (!c) return fail(nullptr, YTF_EINVAL, "ytf_operator_apply: ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
-  return apply_locked(c, op, P_dev, E_dev);
+  int rc = YTF_OK;
+  timed(c, "op_apply", [&] {
+    rc = apply_locked(c, op, P_dev, E_dev);
+    return hipSuccess;
+  });
+  return rc;
 }
 
 int ytf_operator_apply_host(ytf_ctx *c, ytf_operator *op, const double *P_host, double *const *E_host) {

@@ -50,6 +50,7 @@ struct ytf_ctx {
   int *pinned = nullptr;
   // kernel timing: events bracketing each launch while enabled
   bool timing = false;
+  std::vector<std::string> timing_only;  // non-empty: only launches of these names are bracketed
   struct Mark {
     std::string name;
     hipEvent_t a, b;
@@ -131,8 +132,12 @@ inline hipEvent_t take_event(ytf_ctx *c) {
     c->pool.pop_back();
     return e;
   }
+  // timing-only events: no system-scope fence (cache write-back and
+  // invalidate) when they are recorded, so bracketing a launch costs the
+  // launches around it next to nothing; drain_marks synchronizes the stream
+  // before it reads them
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
@@ -141,6 +146,11 @@ inline hipEvent_t take_event(ytf_ctx *c) {
 template <class F>
 inline hipError_t timed_on(ytf_ctx *c, hipStream_t s, const char *name, F &&f) {
   if (!c->timing) return f();
+  if (!c->timing_only.empty()) {
+    bool hit = false;
+    for (const auto &t : c->timing_only) hit = hit || t == name;
+    if (!hit) return f();
+  }
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return f();
   hipEventRecord(a, s);

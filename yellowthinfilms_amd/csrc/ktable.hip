@@ -18,13 +18,15 @@
 //
 // One thread per node: binary search of the panel boundaries (staged in LDS),
 // four-point (cubic) Lagrange interpolation of the 18 reals on the panel's
-// grid (four consecutive 144-byte rows, read through L1/L2 since neighbouring
-// nodes share them), the exact factors, the rotation, and nine coalesced
-// complex stores ([9][n], entry-major); flagged rows are appended to a row
-// list with one atomic per wave.  A Chebyshev series per panel, evaluated by
-// Clenshaw, was measured first: every lane then reads its panel's 17 x 18
-// coefficients, and that per-lane delivery bounds the kernel at ~0.08 ms per
-// 10^6 nodes; the four rows here are a quarter of it.
+// grid, the exact factors, the rotation, and nine coalesced complex stores
+// ([9][n], entry-major); flagged rows are appended to a row list with one
+// atomic per wave.  The table is entry-major (entry e of row r at
+// val[e * nrow + r]): a wave's lanes sit on nearby rows, so each of the 72
+// loads of a node touches a few cache lines for the whole wave, where the
+// row-major table (a 144-byte row per lane) touched one line per lane and the
+// L1's line rate bounded the kernel (61 -> 45 us per 2^20 nodes).  A
+// Chebyshev series per panel, evaluated by Clenshaw, was measured first:
+// every lane then reads its panel's 17 x 18 coefficients (~0.08 ms).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +45,7 @@ __global__ __launch_bounds__(256) void k_ktable_ghat(int64_t n, const double *__
   for (int q = threadIdx.x; q < T.np; q += blockDim.x)
     invh[q] = (double)(T.m[q] - 1) / (T.sab[3 * q + 1] - T.sab[3 * q]);
   __syncthreads();
+  const int64_t nrow = T.row0[T.np - 1] + T.m[T.np - 1];  // entry e of row r at val[e * nrow + r]
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double a = 0.0, b = 0.0, kr = 0.0;
   int p = -1;
@@ -77,10 +80,12 @@ __global__ __launch_bounds__(256) void k_ktable_ghat(int64_t n, const double *__
     const double rm = r - 1.0, rp = r + 1.0, r2 = r - 2.0;
     const double w0 = -r * rm * r2 * (1.0 / 6.0), w1 = 0.5 * rp * rm * r2, w2 = -0.5 * rp * r * r2,
                  w3 = rp * r * rm * (1.0 / 6.0);
-    const double *v = T.val + (T.row0[p] + j - 1) * 18;
+    const double *v = T.val + (T.row0[p] + j - 1);
 #pragma unroll
-    for (int e = 0; e < 18; ++e)
-      val[e] = __builtin_fma(w0, v[e], __builtin_fma(w1, v[18 + e], __builtin_fma(w2, v[36 + e], w3 * v[54 + e])));
+    for (int e = 0; e < 18; ++e) {
+      const double *ve = v + e * nrow;
+      val[e] = __builtin_fma(w0, ve[0], __builtin_fma(w1, ve[1], __builtin_fma(w2, ve[2], w3 * ve[3])));
+    }
   }
   if (i < n) {
     if (work) {

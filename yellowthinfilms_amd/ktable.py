@@ -31,9 +31,9 @@ its samples within tol (relative to the largest |t| sampled); a panel that
 cannot be split further is marked for the exact solve.  For evaluation
 each panel's series is sampled on a uniform grid in s, fine enough that
 four-point (cubic) Lagrange interpolation stays within tol of the series;
-the kernel (ytf_ktable_ghat) then reads four 144-byte rows per node: panel
-search, interpolation, exact factors, rotation, and the exact solve of the
-flagged nodes.
+the kernel (ytf_ktable_ghat) then reads four rows per node from the
+entry-major device copy: panel search, interpolation, exact factors,
+rotation, and the exact solve of the flagged nodes.
 """
 import ctypes as ct
 import math
@@ -216,8 +216,9 @@ class KrhoTable:
         self._fine_grids()
         torch = _torch()
         dev = torch.device("cuda", self.device)
+        # the device copy of the rows is entry-major: entry e of row r at e * n_rows + r
         self._dev = [torch.as_tensor(a, device=dev).contiguous()
-                     for a in (self.kb, self.kind, self.sab, self.row0, self.m, self.val)]
+                     for a in (self.kb, self.kind, self.sab, self.row0, self.m, np.ascontiguousarray(self.val.T))]
 
     @staticmethod
     def _cubic(f, u):
