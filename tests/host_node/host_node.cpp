@@ -13,6 +13,7 @@
 // Output file: g~ [npairs][9][n] complex (interleaved doubles).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <limits>
 #include <vector>
@@ -25,7 +26,10 @@ static_assert(sizeof(ytf_layer) == sizeof(LayerDev), "ytf_layer and LayerDev sha
 
 // As node_kernel and node_kernel_redo: the fast solve, then the robust solve
 // (the interface QR fallback) of a node whose fast solve met an exactly
-// singular interface.  Prints the number of such nodes.
+// singular interface.  Prints the number of such nodes.  The fast solve takes
+// the depth pairs one at a time, as node_kernel does with grid.y
+// groups (each group solves the node again; the first writes the flags and
+// the cache outputs), so a pair-range mistake changes g~.
 template <int CASE, bool GEN>
 static void run(const NodeParams &P, bool poison) {
   // slot i of lane 0 is element 64 i: the array ends at the last declared slot
@@ -38,13 +42,19 @@ static void run(const NodeParams &P, bool poison) {
     if (poison)
       for (auto &v : lds) v = cd{nan, nan};
     bool redo = false;
-    solve_node<CASE, GEN, false>(P, k, st, redo);
+    for (int p0 = 0; p0 < std::max(P.npairs, 1); ++p0) {
+      if (poison && p0 > 0)
+        for (auto &v : lds) v = cd{nan, nan};
+      bool r = false;
+      solve_node<CASE, GEN, false>(P, k, st, r, p0, std::min(P.npairs, p0 + 1));
+      if (p0 == 0) redo = r;
+    }
     if (redo) {
       ++n_redo;
       if (poison)
         for (auto &v : lds) v = cd{nan, nan};
       bool again = false;
-      solve_node<CASE, GEN, true>(P, k, st, again);
+      solve_node<CASE, GEN, true>(P, k, st, again, 0, P.npairs);
     }
   }
   printf("redo %lld\n", (long long)n_redo);

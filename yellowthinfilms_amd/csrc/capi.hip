@@ -36,6 +36,7 @@ int ytf_ctx_create(int device, ytf_ctx **out) {
     return fail(nullptr, YTF_EHIP, "ytf_ctx_create: stream creation failed");
   }
   c->stream = c->own;
+  if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) c->n_cu = 0;
   std::string err;
   if (int rc = noalloc_init(device, &err)) {
     hipStreamDestroy(c->own);
@@ -218,10 +219,22 @@ static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers,
 }
 
 // One node solve of P.nn nodes on stream s (one wave of 64 nodes per block).
-static hipError_t launch_node_solve(const ytf::NodeParams &P, hipStream_t s) {
+static hipError_t launch_node_solve(ytf_ctx *c, const ytf::NodeParams &P) {
   int64_t blocks = (P.nn + 63) / 64;
   if (blocks > 1 << 20) blocks = 1 << 20;
-  return (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, s) : ytf::launch_node_general(P, (int)blocks, s);
+  // The node kernels run one wave per SIMD.  When the nodes fill less than
+  // one round of the GPU's SIMDs and there are several depth pairs, the pairs
+  // are split over the idle SIMDs (each group repeats the node solve, and the
+  // per-pair g~ work shrinks by the group count); the groups fill one round.
+  int groups = 1;
+  if (P.ghat && P.npairs > 1 && !P.rows) {
+    const int64_t slots = 4 * (int64_t)std::max(c->n_cu, 1);
+    groups = (int)std::max<int64_t>(1, std::min<int64_t>(slots / blocks, P.npairs));
+    const int ppg = (P.npairs + groups - 1) / groups;
+    groups = (P.npairs + ppg - 1) / ppg;  // no empty group
+  }
+  return (P.src == 0 && P.obs == 0) ? ytf::launch_node_top(P, (int)blocks, c->stream, groups)
+                                    : ytf::launch_node_general(P, (int)blocks, c->stream, groups);
 }
 
 int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double omega, const double *kx_dev,
@@ -232,7 +245,7 @@ int green_nodes_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers, double
   int rc = node_params_locked(c, layers, n_layers, omega, kx_dev, ky_dev, n_nodes, source_layer, obs_layer, n_pairs,
                               zsrc, zobs, ghat_dev, cache, terms, dz_box, ez_local, &P);
   if (rc || P.nn == 0) return rc;
-  hipError_t e = timed(c, "node_solve", [&] { return launch_node_solve(P, c->stream); });
+  hipError_t e = timed(c, "node_solve", [&] { return launch_node_solve(c, P); });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_green_nodes: launch failed: %s", hipGetErrorString(e));
   return YTF_OK;
 }

@@ -49,6 +49,7 @@ struct ytf_ctx {
   // pinned host words for counts read back
   int *pinned = nullptr;
   // kernel timing: events bracketing each launch while enabled
+  int n_cu = 0;  // compute units of the device
   bool timing = false;
   std::vector<std::string> timing_only;  // non-empty: only launches of these names are bracketed
   struct Mark {

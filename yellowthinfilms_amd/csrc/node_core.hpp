@@ -1428,7 +1428,7 @@ __device__ __forceinline__ void redo_finish(int *list) {
 // fallback there (mod_ScatteringMatrix.jl:35-44).  Keeping the fallback out of
 // the fast solve leaves its register allocation as it is.
 template <int CASE, bool GENERAL, bool ROBUST>
-__device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bool &redo) {
+__device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bool &redo, int pair0, int pair1) {
   if (P.rows) k = P.rows[k];
   const double kx = P.kx[k], ky = P.ky[k], w = P.omega;
   const int N = P.nl, s = P.src, o = P.obs;
@@ -1773,24 +1773,27 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bo
   obs_basis(P.L[o], w, kx, ky, Vo, Qo, basis);
   const cd z0 = mk(0.0);
   const int64_t nn = P.nn;
-  if (P.flags) P.flags[k] = (unsigned char)((valid ? 1 : 0) | (Tvalid ? 2 : 0) | (sing ? 4 : 0) | (ok ? 0 : 8));
-  if (!ROBUST && __builtin_expect(redo, 0) && P.redo) redo_mark(P.redo, k);
-  if (P.Qsrc) {
+  // the flags, the mark and the cache outputs come from the group of depth
+  // pairs that starts at pair 0 (every group solves the node identically)
+  const bool lead = pair0 == 0;
+  if (lead && P.flags) P.flags[k] = (unsigned char)((valid ? 1 : 0) | (Tvalid ? 2 : 0) | (sing ? 4 : 0) | (ok ? 0 : 8));
+  if (!ROBUST && __builtin_expect(redo, 0) && lead && P.redo) redo_mark(P.redo, k);
+  if (lead && P.Qsrc) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) P.Qsrc[i * nn + k] = sel(valid, Qs[i], z0);
   }
-  if (P.sf) {
+  if (lead && P.sf) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       P.sf[i * nn + k] = sel(valid, sif[i], z0);
       P.sb[i * nn + k] = sel(valid, -sib[i], z0);
     }
   }
-  if (P.T) {
+  if (lead && P.T) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) P.T[i * nn + k] = sel(Tvalid, T[i], z0);
   }
-  if (P.basis) {
+  if (lead && P.basis) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) P.basis[i * nn + k] = sel(valid, basis[i], z0);
   }
@@ -1801,7 +1804,7 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bo
   const bool iso_pair = phase_pairs && Ls.ek == 0;
   if (P.ghat) {
     const double d_src = Ls.d, d_obs = P.L[o].d;
-    for (int pr = 0; pr < P.npairs; ++pr) {
+    for (int pr = pair0; pr < pair1; ++pr) {
       const double zsl = P.zsrc[pr], zob = P.zobs[pr];
       const double zpos = s == 0 ? d_src - zsl : zsl;
       cd Af[6], Ab[6], Afd[6], Abd[6];  // [m + 2q]
@@ -1925,16 +1928,21 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bo
 }
 
 // The fast node solve over every node (or every listed row).  A node whose
-// solve met an exactly singular interface is marked in P.redo.
+// solve met an exactly singular interface is marked in P.redo.  With
+// gridDim.y = G > 1 the depth pairs are split into G contiguous groups, one
+// per blockIdx.y: each group solves the node again and writes its pairs' g~,
+// which fills the GPU when the nodes alone would not.
 template <int CASE, bool GENERAL>
 __global__ __launch_bounds__(64) void node_kernel(NodeParams P) {
   __shared__ cd lds[LdsSlots<CASE>::n * 64];
   const LaneMat st{lds, (int)threadIdx.x};
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t n = P.n_rows ? min((int64_t)*P.n_rows, P.nn) : P.nn;
+  const int ppg = (P.npairs + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int pair0 = (int)blockIdx.y * ppg, pair1 = min(P.npairs, pair0 + ppg);
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     bool redo = false;
-    solve_node<CASE, GENERAL, false>(P, k, st, redo);
+    solve_node<CASE, GENERAL, false>(P, k, st, redo, pair0, pair1);
   }
 }
 
@@ -1953,7 +1961,7 @@ __global__ __launch_bounds__(64) void node_kernel_redo(NodeParams P) {
   for (int64_t k = blockIdx.x * 64 + threadIdx.x; k < P.nn; k += (int64_t)gridDim.x * 64) {
     if (!redo_take(list, k)) continue;
     bool again = false;
-    solve_node<CASE, GENERAL, true>(Q, k, st, again);
+    solve_node<CASE, GENERAL, true>(Q, k, st, again, 0, Q.npairs);
   }
   redo_finish(list);
 }

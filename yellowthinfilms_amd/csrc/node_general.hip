@@ -16,24 +16,24 @@
 namespace ytf {
 
 template <int CASE, bool GENERAL>
-static void launch(const NodeParams &P, int blocks, hipStream_t stream) {
-  hipLaunchKernelGGL((node_kernel<CASE, GENERAL>), dim3(blocks), dim3(64), 0, stream, P);
+static void launch(const NodeParams &P, int blocks, int groups, hipStream_t stream) {
+  hipLaunchKernelGGL((node_kernel<CASE, GENERAL>), dim3(blocks, groups), dim3(64), 0, stream, P);
   if (P.redo) hipLaunchKernelGGL((node_kernel_redo<CASE, GENERAL>), dim3(kRedoBlocks), dim3(64), 0, stream, P);
 }
 
 template <bool GENERAL>
-static void launch_case(const NodeParams &P, int blocks, hipStream_t stream) {
-  if (P.obs == P.src) launch<CASE_SAME, GENERAL>(P, blocks, stream);
-  else if (P.obs > P.src) launch<CASE_RIGHT, GENERAL>(P, blocks, stream);
-  else launch<CASE_LEFT, GENERAL>(P, blocks, stream);
+static void launch_case(const NodeParams &P, int blocks, int groups, hipStream_t stream) {
+  if (P.obs == P.src) launch<CASE_SAME, GENERAL>(P, blocks, groups, stream);
+  else if (P.obs > P.src) launch<CASE_RIGHT, GENERAL>(P, blocks, groups, stream);
+  else launch<CASE_LEFT, GENERAL>(P, blocks, groups, stream);
 }
 
-hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream) {
+hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream, int pair_groups) {
   // the general 4x4 eigen-solver is compiled only into the variant for stacks
   // that need it (a layer with xz/yz coupling in eps or mu); its registers
   // would otherwise be reserved by every launch
-  if (P.needs_general) launch_case<true>(P, blocks, stream);
-  else launch_case<false>(P, blocks, stream);
+  if (P.needs_general) launch_case<true>(P, blocks, pair_groups, stream);
+  else launch_case<false>(P, blocks, pair_groups, stream);
   return hipGetLastError();
 }
 

@@ -5,9 +5,10 @@
 
 namespace ytf {
 
-hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream) {
-  if (P.needs_general) hipLaunchKernelGGL((node_kernel<CASE_TOP, true>), dim3(blocks), dim3(64), 0, stream, P);
-  else hipLaunchKernelGGL((node_kernel<CASE_TOP, false>), dim3(blocks), dim3(64), 0, stream, P);
+hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream, int pair_groups) {
+  const dim3 grid(blocks, pair_groups);
+  if (P.needs_general) hipLaunchKernelGGL((node_kernel<CASE_TOP, true>), grid, dim3(64), 0, stream, P);
+  else hipLaunchKernelGGL((node_kernel<CASE_TOP, false>), grid, dim3(64), 0, stream, P);
   if (P.redo) {
     if (P.needs_general) hipLaunchKernelGGL((node_kernel_redo<CASE_TOP, true>), dim3(kRedoBlocks), dim3(64), 0, stream, P);
     else hipLaunchKernelGGL((node_kernel_redo<CASE_TOP, false>), dim3(kRedoBlocks), dim3(64), 0, stream, P);

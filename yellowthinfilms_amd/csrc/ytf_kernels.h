@@ -53,8 +53,10 @@ struct NodeParams {
 // term classes of the spectral dyadic (mod_ClosedFormField.jl:143-160)
 enum { TERMS_ALL = 0, TERMS_DIRECT = 1, TERMS_SCATTERED = 2, TERMS_TOEPLITZ = 3, TERMS_HANKEL = 4 };
 
-hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream);
-hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream);
+// pair_groups > 1 splits the depth pairs into that many groups (grid.y), each
+// solving the node again for its share of g~ (at most npairs groups)
+hipError_t launch_node_top(const NodeParams &P, int blocks, hipStream_t stream, int pair_groups = 1);
+hipError_t launch_node_general(const NodeParams &P, int blocks, hipStream_t stream, int pair_groups = 1);
 
 // 2D spectral convolution E = scale * bfft2(g~ . fft2(J)) on a power-of-two
 // nx x ny grid; see fft_kernels.hip for layouts.
