@@ -38,6 +38,7 @@ inline int atomicAdd(int *p, int v) {
   return o;
 }
 #define __builtin_amdgcn_readfirstlane(x) (x)
+#define __builtin_amdgcn_rcp(x) (1.0 / (x))
 
 using std::isfinite;
 using std::isnan;

@@ -85,7 +85,7 @@ template <int N> struct YTile { static constexpr int XT = (N <= 1024) ? 2 : 1; }
 // Pass 2: for XT columns x, all c: fft_y, 3x3 multiply by g~ (times scale),
 // bfft_y; in place on work[c][x][y].  g~ is [9][nn] with node y + ny*x.
 template <int NY>
-__global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const cd *__restrict__ g, int nx,
+__global__ __launch_bounds__(NY && NY <= 512 ? 3 * YTile<NY>::XT * (NY / 8) : 1024) void k_fft_y_mul(cd *__restrict__ work, const cd *__restrict__ g, int nx,
                                                     const cd *__restrict__ tw, double scale, RowCfg rc) {
   const int ny = NY ? NY : rc.n;
   const int XT = NY ? YTile<NY>::XT : 1;
@@ -96,6 +96,25 @@ __global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const
   const int tid = threadIdx.x;
   const int nthr = 3 * XT * TPR;
   const int64_t nn = (int64_t)nx * ny;
+  // With a compile-time length, this thread's nodes' g~ (the larger part of
+  // the pass's reads) is loaded into registers first, so that its latency
+  // overlaps the loads of the work rows and the forward transform.
+  // (lengths up to 512: longer rows leave too few registers for it)
+  constexpr bool PRE = NY != 0 && NY <= 512;
+  constexpr int NPT = PRE ? (YTile<NY>::XT * NY + 3 * YTile<NY>::XT * (NY / 8) - 1) / (3 * YTile<NY>::XT * (NY / 8)) : 1;
+  cd gpre[NPT][9];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int u = 0; u < NPT; ++u) {
+      const int e = tid + u * nthr;
+      const int xi = e / ny, y = e % ny;
+      if (e < XT * ny && x0 + xi < nx) {
+        const int64_t node = y + (int64_t)ny * (x0 + xi);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) gpre[u][i] = g[i * nn + node];
+      }
+    }
+  }
   for (int e = tid; e < 3 * XT * ny; e += nthr) {
     const int col = e / ny, y = e % ny;
     const int c = col / XT, xi = col % XT;
@@ -103,22 +122,43 @@ __global__ __launch_bounds__(1024) void k_fft_y_mul(cd *__restrict__ work, const
   }
   __syncthreads();
   row_fft<NY, false>(lds + (tid / TPR) * S, lds + (3 * XT + tid / TPR) * S, tid % TPR, TPR, ny, tw);
-  for (int e = tid; e < XT * ny; e += nthr) {
-    const int xi = e / ny, y = e % ny;
-    if (x0 + xi >= nx) continue;
-    const int64_t node = y + (int64_t)ny * (x0 + xi);
-    const int yp = lpad(y);
-    const cd j0 = lds[(0 * XT + xi) * S + yp], j1 = lds[(1 * XT + xi) * S + yp], j2 = lds[(2 * XT + xi) * S + yp];
-    cd out[3];
+  if constexpr (PRE) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cd acc = g[(c + 0) * nn + node] * j0;
-      acc = fmac(acc, g[(c + 3) * nn + node], j1);
-      acc = fmac(acc, g[(c + 6) * nn + node], j2);
-      out[c] = rmul(scale, acc);
+    for (int u = 0; u < NPT; ++u) {
+      const int e = tid + u * nthr;
+      const int xi = e / ny, y = e % ny;
+      if (e >= XT * ny || x0 + xi >= nx) continue;
+      const int yp = lpad(y);
+      const cd j0 = lds[(0 * XT + xi) * S + yp], j1 = lds[(1 * XT + xi) * S + yp], j2 = lds[(2 * XT + xi) * S + yp];
+      cd out[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        cd acc = gpre[u][c + 0] * j0;
+        acc = fmac(acc, gpre[u][c + 3], j1);
+        acc = fmac(acc, gpre[u][c + 6], j2);
+        out[c] = rmul(scale, acc);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * S + yp] = out[c];
     }
+  } else {
+    for (int e = tid; e < XT * ny; e += nthr) {
+      const int xi = e / ny, y = e % ny;
+      if (x0 + xi >= nx) continue;
+      const int64_t node = y + (int64_t)ny * (x0 + xi);
+      const int yp = lpad(y);
+      const cd j0 = lds[(0 * XT + xi) * S + yp], j1 = lds[(1 * XT + xi) * S + yp], j2 = lds[(2 * XT + xi) * S + yp];
+      cd out[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * S + yp] = out[c];
+      for (int c = 0; c < 3; ++c) {
+        cd acc = g[(c + 0) * nn + node] * j0;
+        acc = fmac(acc, g[(c + 3) * nn + node], j1);
+        acc = fmac(acc, g[(c + 6) * nn + node], j2);
+        out[c] = rmul(scale, acc);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) lds[(c * XT + xi) * S + yp] = out[c];
+    }
   }
   __syncthreads();
   row_fft<NY, true>(lds + (tid / TPR) * S, lds + (3 * XT + tid / TPR) * S, tid % TPR, TPR, ny, tw);

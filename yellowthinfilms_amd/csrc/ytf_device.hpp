@@ -35,13 +35,19 @@ __device__ __forceinline__ cd fmac_cj(cd acc, cd b, cd c) {
   return cd{__builtin_fma(b.r, c.r, __builtin_fma(b.i, c.i, acc.r)), __builtin_fma(b.r, c.i, __builtin_fma(-b.i, c.r, acc.i))};
 }
 // 1/b with one real reciprocal: b is first scaled by an exact power of two
-// so |b'|^2 neither overflows nor underflows (b = 0 gives Inf/NaN components
-// like IEEE division does).
+// so that |b'|^2 lies in [1, 8) (b = 0, Inf or NaN give NaN components, as
+// the IEEE-division form of this scaling does).  On that range the hardware
+// reciprocal and two Newton steps need no special cases, where a division
+// would run the general scaled sequence; the result differs from the
+// correctly rounded reciprocal by at most an ulp or so.
 __device__ __forceinline__ cd cinv(cd b) {
   const double m = fmax(fabs(b.r), fabs(b.i));
   const int e = (m > 0.0 && isfinite(m)) ? ilogb(m) : 0;
   const double br = ldexp(b.r, -e), bi = ldexp(b.i, -e);
-  const double rden = 1.0 / __builtin_fma(br, br, bi * bi);
+  const double den = __builtin_fma(br, br, bi * bi);
+  double rden = __builtin_amdgcn_rcp(den);
+  rden = __builtin_fma(rden, __builtin_fma(-den, rden, 1.0), rden);
+  rden = __builtin_fma(rden, __builtin_fma(-den, rden, 1.0), rden);
   const double sc = ldexp(rden, -e);
   return cd{br * sc, -bi * sc};
 }
