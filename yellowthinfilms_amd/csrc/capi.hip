@@ -1583,19 +1583,22 @@ static int radial_synthesis_locked(ytf_ctx *c, const char *who, int64_t n, const
   if (mode == YTF_SYNTH_F16X3 && n > 0) {
     const ytf::MfmaPlan P = ytf::mfma_plan(n, np, nr);
     const size_t jb = align256(P.jtab_bytes), cb = align256(P.ctab_bytes), eb = align256(P.cexp_bytes),
-                 ob = align256(P.out_bytes);
-    int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + jb + cb + eb + ob);
+                 ob = align256(P.out_bytes), sb = align256(P.split_bytes);
+    int rc = ensure_buf(c, &c->work, &c->work_bytes, coef_b + jb + cb + eb + ob + sb);
     if (rc) return rc;
     char *w = (char *)c->work;
     cd *coef = (cd *)w;
     void *jtab = w + coef_b, *ctab = w + coef_b + jb;
     int *cexp = (int *)(w + coef_b + jb + cb);
     float *out = (float *)(w + coef_b + jb + cb + eb);
+    int *split = (int *)(w + coef_b + jb + cb + eb + ob);
     if ((e = timed(c, "synth_coef", [&] { return ytf::launch_synth_coef(n, np, kx, wt, (const cd *)ghat, coef, c->stream); })) ||
         (e = timed(c, "bessel_table", [&] { return ytf::launch_bessel_table(n, np, nr, kx, rho, jtab, c->stream); })) ||
         (e = timed(c, "split_columns", [&] { return ytf::launch_split_columns(n, np, nr, coef, ctab, cexp, c->stream); })) ||
-        (e = timed(c, "bessel_gemm", [&] { return ytf::launch_bessel_gemm(n, np, nr, jtab, ctab, out, c->stream); })) ||
-        (e = timed(c, "mfma_combine", [&] { return ytf::launch_mfma_combine(n, np, nr, out, cexp, R, c->stream); })))
+        (e = timed(c, "bessel_gemm", [&] {
+           return ytf::launch_bessel_gemm(n, np, nr, jtab, ctab, out, split, c->n_cu, c->stream);
+         })) ||
+        (e = timed(c, "mfma_combine", [&] { return ytf::launch_mfma_combine(n, np, nr, out, split, cexp, R, c->stream); })))
       return fail(c, YTF_EHIP, "%s: matrix-core launch failed: %s", who, hipGetErrorString(e));
     return YTF_OK;
   }

@@ -22,6 +22,8 @@
 //   Out   [order][Mpad][Npad_n] fp32
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ytf_kernels.h"
 
 namespace ytf {
@@ -251,144 +253,185 @@ __global__ void k_split_columns(int64_t n, int np, int64_t Kpad, MfmaCols cols, 
 // GEMM: block tile 128 (radii) x 128 (columns), K step 64; 4 waves in 2 x 2,
 // each 64 x 64 = 2 x 2 MFMA tiles of 32 x 32.  Global -> registers prefetch of
 // the next K step while the current one is multiplied out of LDS.
+//
+// Work split (stream-K): the (tile, K step) iterations of all tiles, in tile
+// order, are cut into gridDim.x equal contiguous ranges, one per block, with
+// two blocks per CU.  The 576 tiles of the 8192 x 1152 product would
+// otherwise fill 1.125 rounds of the 512 block slots, the last round on an
+// eighth of the GPU.  A range covers at least one tile's K steps, so a tile
+// is cut at most once: the piece that starts at K step 0 writes output slot 0,
+// the rest slot 1, and split[t] tells the combine whether slot 1 holds a part.
 // ---------------------------------------------------------------------------
 constexpr int BM = 128, BN = 128, BK = 64, LDK = BK + 8;  // LDS row stride (halfs), padded
 
 struct GemmArgs {
   const _Float16 *J;  // Jtab
   const _Float16 *C;  // Ctab
-  float *out;         // Out
+  float *out;         // Out: slot 0, then slot 1 `slot` floats further
+  int *split;         // per tile: 1 if slot 1 holds the tile's later K steps
+  int64_t slot;
   int64_t Kpad;
   int Mpad;
   int mtiles;
   int ntiles[3];
   int nt_total;  // ntiles[0] + ntiles[1] + ntiles[2]
-  int nblocks;
   MfmaCols cols;
 };
 
 __global__ __launch_bounds__(256, 2) void k_bessel_gemm(GemmArgs A) {
-  // XCD-grouped tile order: hardware places block b on XCD b % 8; give each
-  // XCD a contiguous run of tiles (radius tile major), so the blocks that share
-  // a radius tile's J slab run on the same L2.
-  const int b = blockIdx.x;
-  const int per = A.nblocks / 8, rem = A.nblocks % 8, x = b % 8;
-  const int t = x * per + (x < rem ? x : rem) + b / 8;
-  const int mt = t / A.nt_total;
-  int u = t % A.nt_total, o = 0;
-  while (u >= A.ntiles[o]) { u -= A.ntiles[o]; ++o; }
-  const int nt = u;
-  const int m0 = mt * BM, n0 = nt * BN;
+  // XCD-grouped ranges: hardware places block b on XCD b % 8; the ranges of
+  // one XCD's blocks are contiguous in tile order (radius tile major), so the
+  // blocks that share a radius tile's J slab run on the same L2.
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = nb / 8, rem = nb % 8, x = b % 8;
+  const int v = x * per + (x < rem ? x : rem) + b / 8;
   const int64_t K = A.Kpad;
+  const int ksteps = (int)(K / BK);
+  const int64_t iters = (int64_t)A.mtiles * A.nt_total * ksteps;
+  int64_t it = v * iters / nb;
+  const int64_t it_end = (v + 1) * iters / nb;
   const size_t jplane = (size_t)A.Mpad * K;
-  const _Float16 *Jh = A.J + (size_t)(2 * o) * jplane + (size_t)m0 * K;
-  const _Float16 *Jl = Jh + jplane;
-  const int npad = A.cols.npad[o];
-  const _Float16 *Ch = A.C + A.cols.ctab[o] + (size_t)n0 * K;
-  const _Float16 *Cl = Ch + (size_t)npad * K;
 
   __shared__ _Float16 sAh[BM * LDK], sAl[BM * LDK], sBh[BN * LDK], sBl[BN * LDK];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // global load slots: 1024 chunks of 8 halfs per array and K step, 4 per thread
-  h8 ra[2][4], rb[2][4];
-  auto gload = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int c = tid + 256 * i;
-      const int row = c >> 3, kc = (c & 7) * 8;
-      ra[0][i] = *(const h8 *)(Jh + (size_t)row * K + k0 + kc);
-      ra[1][i] = *(const h8 *)(Jl + (size_t)row * K + k0 + kc);
-      rb[0][i] = *(const h8 *)(Ch + (size_t)row * K + k0 + kc);
-      rb[1][i] = *(const h8 *)(Cl + (size_t)row * K + k0 + kc);
-    }
-  };
-  auto sstore = [&]() {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int c = tid + 256 * i;
-      const int row = c >> 3, kc = (c & 7) * 8;
-      *(h8 *)(sAh + row * LDK + kc) = ra[0][i];
-      *(h8 *)(sAl + row * LDK + kc) = ra[1][i];
-      *(h8 *)(sBh + row * LDK + kc) = rb[0][i];
-      *(h8 *)(sBl + row * LDK + kc) = rb[1][i];
-    }
-  };
-
-  f16x acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-
-  const int ksteps = (int)(K / BK);
-  gload(0);
   const int lr = lane & 31, lk = (lane >> 5) * 8;
-  for (int ks = 0; ks < ksteps; ++ks) {
-    __syncthreads();
-    sstore();
-    __syncthreads();
-    if (ks + 1 < ksteps) gload((int64_t)(ks + 1) * BK);
-#pragma unroll
-    for (int s = 0; s < BK / 16; ++s) {
-      h8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int ar = wm * 64 + i * 32 + lr;
-        ah[i] = *(const h8 *)(sAh + ar * LDK + s * 16 + lk);
-        al[i] = *(const h8 *)(sAl + ar * LDK + s * 16 + lk);
-        const int br = wn * 64 + i * 32 + lr;
-        bh[i] = *(const h8 *)(sBh + br * LDK + s * 16 + lk);
-        bl[i] = *(const h8 *)(sBl + br * LDK + s * 16 + lk);
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        }
-    }
-  }
-  // D layout of the 32 x 32 tile: lane holds column lane % 32, rows
-  // 8 g + 4 (lane / 32) + e for element 4 g + e.
-  float *out = A.out + A.cols.out[o];
   const int hi = lane >> 5;
+
+  while (it < it_end) {
+    const int t = (int)(it / ksteps);
+    const int k0 = (int)(it % ksteps);
+    const int k1 = (int)std::min<int64_t>(ksteps, k0 + (it_end - it));
+    const int mt = t / A.nt_total;
+    int u = t % A.nt_total, o = 0;
+    while (u >= A.ntiles[o]) { u -= A.ntiles[o]; ++o; }
+    const int nt = u;
+    const int m0 = mt * BM, n0 = nt * BN;
+    const _Float16 *Jh = A.J + (size_t)(2 * o) * jplane + (size_t)m0 * K;
+    const _Float16 *Jl = Jh + jplane;
+    const int npad = A.cols.npad[o];
+    const _Float16 *Ch = A.C + A.cols.ctab[o] + (size_t)n0 * K;
+    const _Float16 *Cl = Ch + (size_t)npad * K;
+
+    // global load slots: 1024 chunks of 8 halfs per array and K step, 4 per thread
+    h8 ra[2][4], rb[2][4];
+    auto gload = [&](int64_t kk) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i) {
+        const int c = tid + 256 * i;
+        const int row = c >> 3, kc = (c & 7) * 8;
+        ra[0][i] = *(const h8 *)(Jh + (size_t)row * K + kk + kc);
+        ra[1][i] = *(const h8 *)(Jl + (size_t)row * K + kk + kc);
+        rb[0][i] = *(const h8 *)(Ch + (size_t)row * K + kk + kc);
+        rb[1][i] = *(const h8 *)(Cl + (size_t)row * K + kk + kc);
+      }
+    };
+    auto sstore = [&]() {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = n0 + wn * 64 + j * 32 + lr;
+      for (int i = 0; i < 4; ++i) {
+        const int c = tid + 256 * i;
+        const int row = c >> 3, kc = (c & 7) * 8;
+        *(h8 *)(sAh + row * LDK + kc) = ra[0][i];
+        *(h8 *)(sAl + row * LDK + kc) = ra[1][i];
+        *(h8 *)(sBh + row * LDK + kc) = rb[0][i];
+        *(h8 *)(sBl + row * LDK + kc) = rb[1][i];
+      }
+    };
+
+    f16x acc[2][2];
 #pragma unroll
-      for (int g = 0; g < 4; ++g)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int row = m0 + wm * 64 + i * 32 + 8 * g + 4 * hi + e;
-          out[(size_t)row * npad + col] = acc[i][j][4 * g + e];
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+    gload((int64_t)k0 * BK);
+    for (int ks = k0; ks < k1; ++ks) {
+      __syncthreads();
+      sstore();
+      __syncthreads();
+      if (ks + 1 < k1) gload((int64_t)(ks + 1) * BK);
+#pragma unroll
+      for (int s = 0; s < BK / 16; ++s) {
+        h8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const int ar = wm * 64 + i * 32 + lr;
+          ah[i] = *(const h8 *)(sAh + ar * LDK + s * 16 + lk);
+          al[i] = *(const h8 *)(sAl + ar * LDK + s * 16 + lk);
+          const int br = wn * 64 + i * 32 + lr;
+          bh[i] = *(const h8 *)(sBh + br * LDK + s * 16 + lk);
+          bl[i] = *(const h8 *)(sBl + br * LDK + s * 16 + lk);
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          }
+      }
     }
+    // D layout of the 32 x 32 tile: lane holds column lane % 32, rows
+    // 8 g + 4 (lane / 32) + e for element 4 g + e.
+    float *out = A.out + (k0 == 0 ? 0 : A.slot) + A.cols.out[o];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 64 + j * 32 + lr;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = m0 + wm * 64 + i * 32 + 8 * g + 4 * hi + e;
+            out[(size_t)row * npad + col] = acc[i][j][4 * g + e];
+          }
+      }
+    // a whole tile clears its flag; the later piece of a cut tile sets it
+    if (tid == 0) {
+      if (k0 != 0) A.split[t] = 1;
+      else if (k1 == ksteps) A.split[t] = 0;
+    }
+    it += k1 - k0;
+  }
 }
 
 // Combine the three orders into R[p][c + 3q][r] (FP64), undoing the column scales.
-__global__ void k_mfma_combine(int np, int nr, MfmaCols cols, const float *__restrict__ out,
-                               const int *__restrict__ Cexp, cd *__restrict__ R) {
+__global__ void k_mfma_combine(int np, int nr, MfmaCols cols, const float *__restrict__ out, int64_t slot,
+                               const int *__restrict__ split, const int *__restrict__ Cexp, cd *__restrict__ R) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
   if (r >= nr) return;
+  int nt_total = 0;
+#pragma unroll
+  for (int o = 0; o < 3; ++o) nt_total += cols.npad[o] / BN;
+  // column counts per order (J0 3, J1 4, J2 2) as compile-time constants, so
+  // that s[] stays in registers
+  constexpr int NC[3] = {3, 4, 2}, FIRST[3] = {0, 3, 7};
   cd s[9];
+  int toff = 0;
 #pragma unroll
   for (int o = 0; o < 3; ++o) {
     const float *row = out + cols.out[o] + (size_t)r * cols.npad[o];
     const int *ex = Cexp + cols.cexp[o];
-    for (int jj = 0; jj < cols.nc[o]; ++jj) {
-      const int c = (p * cols.nc[o] + jj) * 2;
-      s[cols.first[o] + jj] = cd{ldexp((double)row[c], ex[c]), ldexp((double)row[c + 1], ex[c + 1])};
+    const int *sp = split + (r / BM) * nt_total + toff;
+#pragma unroll
+    for (int jj = 0; jj < NC[o]; ++jj) {
+      const int c = (p * NC[o] + jj) * 2;
+      // the two K parts of a cut tile are added in FP64 (the pair shares a tile)
+      double vr = (double)row[c], vi = (double)row[c + 1];
+      if (sp[c / BN]) {
+        vr += (double)row[slot + c];
+        vi += (double)row[slot + c + 1];
+      }
+      s[FIRST[o] + jj] = cd{ldexp(vr, ex[c]), ldexp(vi, ex[c + 1])};
     }
+    toff += cols.npad[o] / BN;
   }
   // index names as in synth.hip: A B E | XZ YZ ZX ZY | C D
   cd *o9 = R + (size_t)p * 9 * nr + r;
@@ -432,7 +475,9 @@ MfmaPlan mfma_plan(int64_t n, int np, int nr) {
   P.jtab_bytes = sizeof(_Float16) * 6 * (size_t)P.Mpad * P.Kpad;
   P.ctab_bytes = sizeof(_Float16) * 2 * ncol * P.Kpad;
   P.cexp_bytes = sizeof(int) * ncol;
-  P.out_bytes = sizeof(float) * (size_t)P.Mpad * ncol;
+  P.out_slot = (int64_t)P.Mpad * ncol;
+  P.out_bytes = sizeof(float) * 2 * (size_t)P.out_slot;  // two slots (a cut tile's two K parts)
+  P.split_bytes = sizeof(int) * (size_t)(P.Mpad / BM) * (ncol / BN);
   return P;
 }
 
@@ -456,14 +501,16 @@ hipError_t launch_split_columns(int64_t n, int np, int nr, const cd *coef, void 
   return hipGetLastError();
 }
 
-hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out,
-                              hipStream_t s) {
+hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out, int *split,
+                              int n_cu, hipStream_t s) {
   const MfmaPlan P = mfma_plan(n, np, nr);
   const MfmaCols cols = make_cols(np, P.Kpad, P.Mpad);
   GemmArgs A;
   A.J = (const _Float16 *)jtab;
   A.C = (const _Float16 *)ctab;
   A.out = out;
+  A.split = split;
+  A.slot = P.out_slot;
   A.Kpad = P.Kpad;
   A.Mpad = P.Mpad;
   A.mtiles = P.Mpad / BM;
@@ -472,16 +519,21 @@ hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const
     A.ntiles[o] = cols.npad[o] / BN;
     A.nt_total += A.ntiles[o];
   }
-  A.nblocks = A.mtiles * A.nt_total;
   A.cols = cols;
-  hipLaunchKernelGGL(k_bessel_gemm, dim3(A.nblocks), dim3(256), 0, s, A);
+  // two blocks per CU, and never more blocks than tiles (a range then covers
+  // at least one tile's K steps, so no tile is cut twice)
+  const int tiles = A.mtiles * A.nt_total;
+  const int nblocks = std::max(1, std::min(tiles, 2 * std::max(n_cu, 1)));
+  hipLaunchKernelGGL(k_bessel_gemm, dim3(nblocks), dim3(256), 0, s, A);
   return hipGetLastError();
 }
 
-hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *cexp, cd *R, hipStream_t s) {
+hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *split, const int *cexp, cd *R,
+                               hipStream_t s) {
   const MfmaPlan P = mfma_plan(n, np, nr);
   const MfmaCols cols = make_cols(np, P.Kpad, P.Mpad);
-  hipLaunchKernelGGL(k_mfma_combine, dim3((nr + 255) / 256, np), dim3(256), 0, s, np, nr, cols, out, cexp, R);
+  hipLaunchKernelGGL(k_mfma_combine, dim3((nr + 255) / 256, np), dim3(256), 0, s, np, nr, cols, out, P.out_slot, split,
+                     cexp, R);
   return hipGetLastError();
 }
 

@@ -92,15 +92,19 @@ hipError_t launch_synth_rotate(int nx, int ny, int np, int nr, const double *xs,
 struct MfmaPlan {
   int64_t Kpad;
   int Mpad;
-  size_t jtab_bytes, ctab_bytes, cexp_bytes, out_bytes;
+  int64_t out_slot;  // floats per output slot (the GEMM writes two)
+  size_t jtab_bytes, ctab_bytes, cexp_bytes, out_bytes, split_bytes;
 };
 MfmaPlan mfma_plan(int64_t n, int np, int nr);
 hipError_t launch_bessel_table(int64_t n, int np, int nr, const double *kx, const double *rho, void *jtab,
                                hipStream_t s);
 hipError_t launch_split_columns(int64_t n, int np, int nr, const cd *coef, void *ctab, int *cexp, hipStream_t s);
-hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out,
-                              hipStream_t s);
-hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *cexp, cd *R, hipStream_t s);
+// n_cu: compute units of the device (the GEMM runs two blocks per CU over
+// equal contiguous ranges of all tiles' K steps); split: MfmaPlan::split_bytes
+hipError_t launch_bessel_gemm(int64_t n, int np, int nr, const void *jtab, const void *ctab, float *out, int *split,
+                              int n_cu, hipStream_t s);
+hipError_t launch_mfma_combine(int64_t n, int np, int nr, const float *out, const int *split, const int *cexp, cd *R,
+                               hipStream_t s);
 
 // Uneven batch of stacks (batch.hip): total S-matrix + end-layer modes per
 // node, one persistent launch (counter reset + kernel).
