@@ -81,6 +81,23 @@ def test_foundation_stacks_every_layer_pair(label):
             _compare(stack, kx, ky, src, obs, zs, zo)
 
 
+@pytest.mark.parametrize("src,obs", [(1, 1), (2, 4), (3, 3), (4, 2)])
+def test_depth_pairs_split_over_groups(src, obs):
+    """9,600 nodes (150 one-wave blocks) and 7 depth pairs: on a 256-CU GPU
+    the launch splits the pairs into 4 groups on grid.y (2, 2, 2, 1 pairs),
+    each group solving the node again; every pair's g~ and the cache written
+    by the first group must match the oracle."""
+    print(f"case: depth-pair groups, C2 stack src={src} obs={obs}", flush=True)
+    stack = c2_stack()
+    rng = np.random.default_rng(19)
+    kr = rng.uniform(0.0, 4.0, 9600) * W0
+    ph = rng.uniform(0, 2 * np.pi, 9600)
+    ds, do = stack[src - 1].d, stack[obs - 1].d
+    zs = list(np.linspace(0.1, 0.9, 7) * min(ds, 0.5))
+    zo = list(np.linspace(0.85, 0.15, 7) * min(do, 0.5))
+    _compare(stack, kr * np.cos(ph), kr * np.sin(ph), src, obs, zs, zo)
+
+
 def test_c1_stack_radial_nodes_all_obs_layers():
     stack = c1_stack()
     # near (not on) the light lines of air (k0) and glass (1.5 k0): exactly on a
