@@ -71,6 +71,9 @@ sys.path.insert(0, HERE)
 PEAK_FP64 = 78.6e12    # MI355X FP64 (vector and matrix), FLOP/s
 PEAK_FP16 = 2.5e15     # MI355X dense fp16/bf16 matrix, FLOP/s
 PEAK_HBM = 8.0e12      # bytes/s
+# FP64 LDS atomic updates per second: 7 ds_add_f64 lane-updates per clock per CU
+# (an LDS probe, 64 lanes on 64 distinct doubles), 256 CUs, 2.4 GHz
+LDS_F64_ATOMIC_PEAK = 7 * 256 * 2.4e9
 METRIC = "Green k-nodes/s + operator applies/s, surrogate vs plain GPU, <=1e-6 rel err"
 
 
@@ -650,6 +653,10 @@ def bench_nufft(a):
     M = len(x)
     alg_bytes = M * (16 + 16) + N * N * 16  # points + strengths in, modes out
     ms = t.pop("nufft_apply")[0] / a.steps  # the whole transform, one event pair per apply
+    w_nu = min(16, max(2, math.ceil(math.log10(1.0 / wl["eps"])) + 1))
+    w_nu += w_nu & 1                             # the library's spreading width (even)
+    spread_updates = 2.0 * w_nu * w_nu * M       # ds_add_f64 lane-updates (re, im) per transform
+    spread_ms = t["nufft_spread"][0] / max(t["nufft_spread"][1], 1)
     cpu, rel_err = None, None
     if not a.no_cpu_baseline:
         cpu, rel_err = cpu_nufft(x, y, c, f[0].cpu().numpy(), N)
@@ -661,6 +668,12 @@ def bench_nufft(a):
         "roofline": {"kernel": "nufft (sort + LDS-tiled spreading + FFT passes, one apply)", "bound": "hbm",
                      "achieved": alg_bytes / (ms * 1e-3) / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                      "frac": alg_bytes / (ms * 1e-3) / PEAK_HBM, "traffic": None, "algorithmic_bytes": alg_bytes},
+        # what binds the spreading kernel: its FP64 LDS atomics (2 w^2 per point), against the
+        # rate an LDS probe reaches (7 ds_add_f64 lane-updates per clock per CU, 256 CUs, 2.4 GHz)
+        "spread_lds_atomics": {"updates": spread_updates, "achieved": spread_updates / (spread_ms * 1e-3),
+                               "peak": LDS_F64_ATOMIC_PEAK, "unit": "lane-updates/s",
+                               "frac": spread_updates / (spread_ms * 1e-3) / LDS_F64_ATOMIC_PEAK,
+                               "width": w_nu, "spread_ms": spread_ms},
         "cpu_baseline": cpu,
     })
     return out
