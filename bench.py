@@ -92,14 +92,26 @@ def _timed_steps(a, step, torch):
     return time.perf_counter() - t0
 
 
-def _timed_kernels(a, step, torch, ctx, roof, others=()):
+def _timed_kernels(a, step, torch, ctx, roof, others=(), events_in_wall=True):
     """The K timed steps with HIP events on the launches named in `roof` (a
     name or a tuple of names: the roofline's kernels) and on no other launch,
     since every event pair adds a few microseconds to the wall clock; then,
     if `others` names more kernels, a second pass of K steps outside the wall
-    clock with every launch bracketed, for their per-kernel times.  Returns
-    (wall seconds of the timed steps, {name: (total ms, launches)})."""
+    clock with every launch bracketed, for their per-kernel times.  With
+    events_in_wall=False the timed steps carry no events at all and the
+    `roof` kernels are timed in the second pass too (for a rate that is
+    compared with one measured without events).  Returns (wall seconds of the
+    timed steps, {name: (total ms, launches)})."""
     roof = (roof,) if isinstance(roof, str) else tuple(roof)
+    if not events_in_wall:
+        ctx.set_timing(False)
+        wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
+        ctx.set_timing(True)
+        for _ in range(a.steps):
+            step()
+        t = {k: ctx.kernel_time(k) for k in roof + tuple(others)}
+        ctx.set_timing(False)
+        return wall, t
     ctx.set_timing_only(",".join(roof))
     wall = _timed_steps(dict_ns(a, warmup=0), step, torch)
     t = {k: ctx.kernel_time(k) for k in roof}
@@ -531,7 +543,8 @@ def surrogate_runs(a, configs, windows=None):
 
     for _ in range(a.warmup):
         table()
-    wall_tab, tk = _timed_kernels(a, table, torch, ctx, "ktable")
+    # the plain solve above is timed without events, and so is this
+    wall_tab, tk = _timed_kernels(a, table, torch, ctx, "ktable", events_in_wall=False)
     ktab_ms, _ = tk["ktable"]
     gt, mt = gt_out[0].T, gt_out[1].bool()
     ktable = {
@@ -568,7 +581,7 @@ def surrogate_runs(a, configs, windows=None):
 
             for _ in range(a.warmup):
                 surrogate()
-            wall_sur, tk = _timed_kernels(a, surrogate, torch, ctx, "mlp_fused")
+            wall_sur, tk = _timed_kernels(a, surrogate, torch, ctx, "mlp_fused", events_in_wall=False)
             gemm_ms, _ = tk["mlp_fused"]
             gs, mask = res["g"], res["mask"]
             in_win = torch.zeros_like(mask)
