@@ -80,10 +80,14 @@ int ytf_ctx_synchronize(ytf_ctx *ctx);
  * set_timing clears the totals.  kernel_time synchronizes the stream and
  * returns the summed time (ms) and launch count of one kernel name:
  * "node_solve", "spectral_apply", "synth_coef", "synth_fp64", "synth_reduce",
- * "synth_rotate", "bessel_table", "split_columns", "bessel_gemm",
- * "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam", "nufft_sort",
- * "nufft_spread", "nufft_fft", "lateral_direct", "op_fft_fwd", "op_mix",
- * "op_fft_inv". */
+ * "synth_rotate", "radial_terms", "bessel_table", "split_columns",
+ * "bessel_gemm", "mfma_combine", "batch_solve", "mlp_gemm", "mlp_adam",
+ * "mlp_fused", "ktable", "nufft_sort", "nufft_spread", "nufft_fft",
+ * "lateral_direct", "lateral_segments", "rot_fill", "op_fft_fwd", "op_mix",
+ * "op_fft_inv", "cavity_response", "pole_candidates", "pole_refine", and for
+ * whole calls "nufft_apply" (ytf_nufft2d1) and "op_apply"
+ * (ytf_operator_apply).  The events are created for timing only
+ * (hipEventDisableSystemFence). */
 int ytf_ctx_set_timing(ytf_ctx *ctx, int on);
 /* As ytf_ctx_set_timing(ctx, 1), but only the launches whose names appear in
  * the comma-separated list `names` are bracketed, so the events (a few
