@@ -59,7 +59,8 @@ int ytf_ctx_destroy(ytf_ctx *c) {
     hipEventDestroy(m.b);
   }
   for (auto e : c->pool) hipEventDestroy(e);
-  if (c->small) hipFree(c->small);
+  for (auto &s : c->small_slots)
+    if (s.dev) hipFree(s.dev);
   if (c->work) hipFree(c->work);
   if (c->aux) hipFree(c->aux);
   if (c->redo) hipFree(c->redo);
@@ -177,16 +178,7 @@ static int node_params_locked(ytf_ctx *c, const ytf_layer *layers, int n_layers,
     std::memcpy(hb.data() + lb, zsrc, zb);
     std::memcpy(hb.data() + lb + zb, zobs, zb);
   }
-  if (hb != c->small_host) {
-    // the previous kernels may still read the buffer: wait, copy, wait (the
-    // host bytes are pageable and local to this call)
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = ensure_small(c, hb.size() + 64);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->small, hb.data(), hb.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->small_host = hb;
-  }
+  if (int rc = set_small(c, hb)) return rc;
   int rc_redo = ensure_redo(c, n_nodes);
   if (rc_redo) return rc_redo;
   char *sm = (char *)c->small;
@@ -301,15 +293,7 @@ static int response_stack_locked(ytf_ctx *c, const char *who, const ytf_layer *l
   const size_t lb = sizeof(ytf_layer) * n_layers;
   std::vector<char> hb(lb);
   std::memcpy(hb.data(), layers, lb);
-  if (hb != c->small_host) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = ensure_small(c, hb.size() + 64);
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->small, hb.data(), hb.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->small_host = hb;
-  }
-  return YTF_OK;
+  return set_small(c, hb);
 }
 
 extern "C" {

@@ -711,15 +711,25 @@ int ytf_segment_lateral_sum(ytf_ctx *c, int64_t n_seg, int seg_len, const double
   if (!kx || !ky || !w || !ghat || !xp || !yp || !out)
     return fail(c, YTF_EINVAL, "ytf_segment_lateral_sum: NULL buffer");
   HIPCHK(c, hipSetDevice(c->device));
-  int rc = ensure_buf(c, &c->aux, &c->aux_bytes, sizeof(double) * 2 * (size_t)np);
-  if (rc) return rc;
-  std::vector<double> h(xp, xp + np);
-  h.insert(h.end(), yp, yp + np);
-  rc = upload(c, c->aux, h.data(), sizeof(double) * h.size());
-  if (rc) return rc;
-  const double *xd = (const double *)c->aux;
+  // up to LatPts::kMax points travel with the launch (no upload, so no wait
+  // for earlier kernels); more go through the context's buffer
+  ytf::LatPts pts{};
+  const double *xd = nullptr, *yd = nullptr;
+  if (np <= ytf::LatPts::kMax) {
+    std::copy(xp, xp + np, pts.x);
+    std::copy(yp, yp + np, pts.y);
+  } else {
+    int rc = ensure_buf(c, &c->aux, &c->aux_bytes, sizeof(double) * 2 * (size_t)np);
+    if (rc) return rc;
+    std::vector<double> h(xp, xp + np);
+    h.insert(h.end(), yp, yp + np);
+    rc = upload(c, c->aux, h.data(), sizeof(double) * h.size());
+    if (rc) return rc;
+    xd = (const double *)c->aux;
+    yd = xd + np;
+  }
   hipError_t e = timed(c, "lateral_segments", [&] {
-    return ytf::launch_lat_segments(n_seg, seg_len, kx, ky, (const cd *)w, np, (const cd *)ghat, xd, xd + np,
+    return ytf::launch_lat_segments(n_seg, seg_len, kx, ky, (const cd *)w, np, (const cd *)ghat, xd, yd, pts,
                                     (cd *)out, c->stream);
   });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_segment_lateral_sum: launch failed: %s", hipGetErrorString(e));

@@ -25,11 +25,20 @@ struct ytf_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::mutex mu;
-  // small per-call inputs (layers, z pairs), grown on demand; `small_host`
-  // mirrors what the device copy holds so unchanged inputs are not re-sent
-  void *small = nullptr;
-  size_t small_bytes = 0;
-  std::vector<char> small_host;
+  // small per-call inputs (layers, z pairs) in a few device slots keyed by
+  // their bytes: inputs that alternate between calls (one set per
+  // observation layer, say) are sent once, and a call waits for earlier
+  // kernels only when it has to overwrite a slot they may still read
+  struct SmallSlot {
+    std::vector<char> host;  // what the device copy holds (and the copy's source)
+    void *dev = nullptr;
+    size_t cap = 0;
+    uint64_t used = 0;
+  };
+  static constexpr int kSmallSlots = 8;
+  SmallSlot small_slots[kSmallSlots];
+  uint64_t small_tick = 0;
+  void *small = nullptr;  // the slot of the current call
   // FFT twiddle tables, one per length
   std::map<int, cd *> tw;
   // NUFFT deconvolution factors per (modes, fine grid, kernel width)
@@ -80,17 +89,41 @@ inline int fail(ytf_ctx *c, int code, const char *fmt, ...) {
     if (_e != hipSuccess) return fail((c), YTF_EHIP, "%s: %s", #expr, hipGetErrorString(_e));     \
   } while (0)
 
-inline int ensure_small(ytf_ctx *c, size_t bytes) {
-  if (bytes <= c->small_bytes) return YTF_OK;
-  if (c->small) {
+// Point c->small at a device copy of hb: a slot that already holds these
+// bytes, else an empty slot, else the least recently used one (after the
+// stream has drained, since queued kernels may still read it).
+inline int set_small(ytf_ctx *c, const std::vector<char> &hb) {
+  const uint64_t tick = ++c->small_tick;
+  for (auto &s : c->small_slots)
+    if (s.dev && s.host == hb) {
+      s.used = tick;
+      c->small = s.dev;
+      return YTF_OK;
+    }
+  ytf_ctx::SmallSlot *v = nullptr;
+  for (auto &s : c->small_slots)
+    if (!s.dev) {
+      v = &s;
+      break;
+    }
+  if (!v) {
+    v = &c->small_slots[0];
+    for (auto &s : c->small_slots)
+      if (s.used < v->used) v = &s;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipFree(c->small));
-    c->small = nullptr;
-    c->small_bytes = 0;
   }
-  size_t nb = bytes < 65536 ? 65536 : bytes;
-  HIPCHK(c, hipMalloc(&c->small, nb));
-  c->small_bytes = nb;
+  if (v->cap < hb.size()) {
+    if (v->dev) HIPCHK(c, hipFree(v->dev));
+    v->dev = nullptr;
+    v->cap = 0;
+    const size_t nb = hb.size() < 4096 ? 4096 : hb.size();
+    HIPCHK(c, hipMalloc(&v->dev, nb));
+    v->cap = nb;
+  }
+  v->host = hb;  // the copy's source lives until the slot is reused, after a drain
+  v->used = tick;
+  HIPCHK(c, hipMemcpyAsync(v->dev, v->host.data(), hb.size(), hipMemcpyHostToDevice, c->stream));
+  c->small = v->dev;
   return YTF_OK;
 }
 

@@ -136,13 +136,16 @@ __global__ __launch_bounds__(256) void k_lat_direct(int64_t n, const double *__r
 // k = s L + j, f = finite_or_zero: the terms of k_lat_direct for that point.
 // One wave per (segment, pair); lanes stride over the segment, then a
 // fixed-order shuffle tree adds the lanes.
+// The points come in the kernel arguments (up to LatPts::kMax of them), so a
+// call needs no upload, and no wait for the kernels still reading an earlier
+// upload; more points come from device arrays xp, yp.
 __global__ __launch_bounds__(64) void k_lat_segments(int64_t n, int L, const double *__restrict__ kx,
                                                      const double *__restrict__ ky, const cd *__restrict__ W, int np,
                                                      const cd *__restrict__ g, const double *__restrict__ xp,
-                                                     const double *__restrict__ yp, cd *__restrict__ out) {
+                                                     const double *__restrict__ yp, LatPts pts, cd *__restrict__ out) {
   const int64_t s = blockIdx.x;
   const int p = blockIdx.y, lane = threadIdx.x;
-  const double x = xp[p], y = yp[p];
+  const double x = xp ? xp[p] : pts.x[p], y = yp ? yp[p] : pts.y[p];
   const cd *gp = g + (int64_t)p * 9 * n;
   double acc[18];
 #pragma unroll
@@ -199,11 +202,12 @@ hipError_t launch_polar_nodes(int64_t nk, int L, const double *kr, const double 
 }
 
 hipError_t launch_lat_segments(int64_t n_seg, int L, const double *kx, const double *ky, const cd *W, int np,
-                               const cd *g, const double *xp, const double *yp, cd *out, hipStream_t s) {
+                               const cd *g, const double *xp, const double *yp, const LatPts &pts, cd *out,
+                               hipStream_t s) {
   if (n_seg <= 0 || np <= 0) return hipSuccess;
-  if (n_seg > 0x7fffffff || np > 65535) return hipErrorInvalidValue;
+  if (n_seg > 0x7fffffff || np > 65535 || (!xp && np > LatPts::kMax)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_lat_segments, dim3((unsigned)n_seg, (unsigned)np), dim3(64), 0, s, n_seg * L, L, kx, ky, W,
-                     np, g, xp, yp, out);
+                     np, g, xp, yp, pts, out);
   return hipGetLastError();
 }
 

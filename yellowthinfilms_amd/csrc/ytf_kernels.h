@@ -263,9 +263,16 @@ hipError_t launch_lat_strengths(int64_t n, int np, const cd *g, const cd *w, cd 
 // Nodes and box-footprint weights of one-k_rho-node caches (operator.hip).
 hipError_t launch_polar_nodes(int64_t nk, int L, const double *kr, const double *phi, const double *wphi, double dx,
                               double dy, double *kx, double *ky, cd *W, hipStream_t s);
-// Per-segment lateral sums at one point per depth pair (operator.hip).
+// Per-segment lateral sums at one point per depth pair (operator.hip).  The
+// points (x, y) come from device arrays xp, yp, or, when those are null, from
+// pts (at most LatPts::kMax points, passed by value with the launch).
+struct LatPts {
+  static constexpr int kMax = 64;
+  double x[kMax], y[kMax];
+};
 hipError_t launch_lat_segments(int64_t n_seg, int L, const double *kx, const double *ky, const cd *W, int np,
-                               const cd *g, const double *xp, const double *yp, cd *out, hipStream_t s);
+                               const cd *g, const double *xp, const double *yp, const LatPts &pts, cd *out,
+                               hipStream_t s);
 hipError_t launch_lat_direct(int64_t n, const double *kx, const double *ky, const cd *W, int np, const cd *g,
                              const double *xs, int nx, const double *ys, int ny, cd *G, int64_t sp, int64_t se,
                              hipStream_t s);

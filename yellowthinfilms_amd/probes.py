@@ -117,6 +117,9 @@ def probe_values(stack, omega, source_layer, krho, phis, wphis, probes, box, dev
     _lib.check(_lib.lib().ytf_polar_nodes(ctx.handle, nk, ct.c_void_p(kr_d.data_ptr()), L, ct.c_void_p(ph_d.data_ptr()),
                                           ct.c_void_p(wp_d.data_ptr()), dx, dy, ct.c_void_p(kx_d.data_ptr()),
                                           ct.c_void_p(ky_d.data_ptr()), ct.c_void_p(W_d.data_ptr())), ctx.handle)
+    # every observation layer's launches first, the values to the host once at the end
+    # (one wait instead of one per layer)
+    parts = []
     for obs in sorted({p[0] for p in probes}):
         idx = [m for m, p in enumerate(probes) if p[0] == obs]
         zs = [probes[m][5][0] for m in idx]
@@ -131,8 +134,14 @@ def probe_values(stack, omega, source_layer, krho, phis, wphis, probes, box, dev
             ctx.handle, nk, L, ct.c_void_p(kx_d.data_ptr()), ct.c_void_p(ky_d.data_ptr()),
             ct.c_void_p(W_d.data_ptr()), len(idx), ct.c_void_p(g.data_ptr()), xp.ctypes.data_as(ct.c_void_p),
             yp.ctypes.data_as(ct.c_void_p), ct.c_void_p(vals.data_ptr())), ctx.handle)
-        v = vals.cpu().numpy().reshape(nk, len(idx), 3, 3)  # entry c + 3q -> [q][c]
+        del g
+        parts.append((idx, vals))
+    host = torch.cat([v.reshape(nk, -1) for _, v in parts], dim=1).cpu().numpy()
+    col = 0
+    for idx, _ in parts:
+        v = host[:, col:col + 9 * len(idx)].reshape(nk, len(idx), 3, 3)  # entry c + 3q -> [q][c]
         out[:, idx] = np.swapaxes(v, 2, 3)
+        col += 9 * len(idx)
     return out
 
 
