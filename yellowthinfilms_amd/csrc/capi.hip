@@ -337,7 +337,7 @@ int ytf_pole_refine(ytf_ctx *c, const ytf_layer *layers, int n_layers, double om
   }
   hipError_t e = timed(c, "pole_refine", [&] {
     return ytf::launch_pole_refine((const LayerDev *)c->small, n_layers, source_layer, omega, n_cand, cand_dev,
-                                   out_dev, evals, general, c->stream);
+                                   out_dev, evals, general, c->n_cu, c->stream);
   });
   if (e != hipSuccess) return fail(c, YTF_EHIP, "ytf_pole_refine: launch failed: %s", hipGetErrorString(e));
   if (n_evals) {

@@ -19,6 +19,8 @@
 // mode solve or a non-finite r give 0.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "node_core.hpp"
 
 namespace ytf {
@@ -311,13 +313,17 @@ hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, d
 //   the valley kl (side 0) or kr (side 1) for the crossing of the level.
 // out[3c] = q, out[3c + 1 + side] = the crossing.
 //
-// Each chain takes 8 lanes and advances 3 steps per evaluation round: the
+// Each chain takes 2^D lanes and advances D steps per evaluation round: the
 // points a step visits depend only on the earlier steps' decisions, so the
-// lanes evaluate the 7 points of the decision tree of the next 3 steps
+// lanes evaluate the 2^D - 1 points of the decision tree of the next D steps
 // (lane 0 the next point, lanes 1-2 the two possible points after it, lanes
-// 3-6 the four after those; lane 7 takes f(q) in the first bisection round),
-// then every lane replays the 3 steps with the actual values.  The steps and
-// the points they keep are those of one-step-at-a-time search.
+// 3-6 the four after those; the last lane takes f(q) in the first bisection
+// round), then every lane replays the D steps with the actual values.  The
+// steps and the points they keep are those of one-step-at-a-time search.
+// The kernel runs one wave per SIMD and every chain takes about the same
+// time, so D = 3 is used when the chains fill at most one wave per SIMD and
+// D = 2 otherwise (a second round of waves would cost more than the rounds
+// the deeper tree saves).
 // With `evals`, the number of responses the one-step search takes (the
 // golden-section ones once per candidate) is added to *evals.
 struct RefineArgs {
@@ -346,13 +352,14 @@ __device__ __forceinline__ double gs_step(bool left, double &a, double &b, doubl
   return d;
 }
 
-template <bool GENERAL>
+template <bool GENERAL, int D>
 __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
+  constexpr int NL = 1 << D;  // lanes per chain: 2^D - 1 tree points and the f(q) lane
   __shared__ cd vprev[16 * 64];
   const LaneMat Vp{vprev, (int)threadIdx.x};
-  const int64_t chain = (blockIdx.x * (int64_t)64 + threadIdx.x) >> 3;
-  const int l = threadIdx.x & 7, gb = threadIdx.x & ~7;  // lane in the chain, the chain's first lane
-  if (chain >= 2 * A.nc) return;  // whole chains: 2 nc chains of 8 lanes
+  const int64_t chain = (blockIdx.x * (int64_t)64 + threadIdx.x) >> D;
+  const int l = threadIdx.x & (NL - 1), gb = threadIdx.x & ~(NL - 1);  // lane in the chain, its first lane
+  if (chain >= 2 * A.nc) return;  // whole chains: 2 nc chains of NL lanes
   const int64_t c = chain >> 1;
   const int side = (int)(chain & 1);
   const double *cd7 = A.cand + 7 * c;
@@ -363,10 +370,12 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
   auto f = [&](double k) {
     return response_node<GENERAL, false>(A.L, A.nl, A.s, A.omega, k * cphi, k * sphi, Vp, sing_seen);
   };
-  // lane l < 7: node l of the tree; its level and the decisions on its path
+  // lane l < NL - 1: node l of the tree (children of p: 2p + 1 on a true
+  // decision, 2p + 2 on a false one); its level and the decisions on its path
   const int lev = l == 0 ? 0 : (l < 3 ? 1 : 2);
-  const bool dec1 = l == 1 || l == 3 || l == 4;  // level-1 decision (true: child 2p + 1)
-  const bool dec2 = l == 3 || l == 5;            // level-2 decision
+  const int a1 = lev == 2 ? (l - 1) / 2 : l;     // the level-1 ancestor
+  const bool dec1 = lev >= 1 && (a1 & 1);        // level-1 decision
+  const bool dec2 = lev >= 2 && (l & 1);         // level-2 decision
   unsigned long long ne = 0, ng = 0;
   // One evaluation per lane and round, at one call site.  Phases: 0 f(c) and
   // f(d) (lanes 0, 1); 1 golden-section rounds; 2 bisection rounds (lane 7
@@ -388,7 +397,7 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
       x = gs_step(fc >= fd, ta, tb, tc, td);
       if (lev >= 1) x = gs_step(dec1, ta, tb, tc, td);
       if (lev >= 2) x = gs_step(dec2, ta, tb, tc, td);
-      ev = l < 7;
+      ev = l < NL - 1;
     } else {
       double tlo = lo, thi = hi;
       x = (tlo + thi) / 2;
@@ -401,8 +410,8 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
         x = (tlo + thi) / 2;
       }
       const bool bis = it < 50 && !(fabs(hi - lo) <= 1e-12 * q);
-      if (l == 7) x = q;
-      ev = l < 7 ? bis : first;
+      if (l == NL - 1) x = q;
+      ev = l < NL - 1 ? bis : first;
     }
     const double v = ev ? f(x) : 0.0;
     if (phase == 0) {
@@ -412,7 +421,7 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
       phase = 1;
     } else if (phase == 1) {
       int node = 0;
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < D; ++j) {
         if (!(it < 60 && !((b - a) <= 1e-12 * b))) break;
         const bool left = fc >= fd;
         if (j > 0) node = 2 * node + (left ? 1 : 2);
@@ -430,12 +439,12 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
       }
     } else {
       if (first) {
-        const double pk = fmax(__shfl(v, gb + 7), exp(cd7[6]));
+        const double pk = fmax(__shfl(v, gb + NL - 1), exp(cd7[6]));
         lvl = eb + (pk - eb) / 1.41421356237309504880;
         first = false;
       }
       int node = 0;
-      for (int j = 0; j < 3; ++j) {
+      for (int j = 0; j < D; ++j) {
         if (!(it < 50 && !(fabs(hi - lo) <= 1e-12 * q))) break;
         const double mid = (lo + hi) / 2;
         const bool up = __shfl(v, gb + node) > lvl;
@@ -465,12 +474,20 @@ __global__ __launch_bounds__(64) void k_pole_refine(RefineArgs A) {
 }
 
 hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,
-                              double *out, unsigned long long *evals, bool general, hipStream_t st) {
+                              double *out, unsigned long long *evals, bool general, int n_cu, hipStream_t st) {
   if (nc <= 0) return hipSuccess;
   RefineArgs A{L, nl, s, omega, nc, cand, out, evals};
-  const unsigned blocks = (unsigned)((16 * nc + 63) / 64);  // 8 lanes per chain, 2 chains per candidate
-  if (general) hipLaunchKernelGGL(k_pole_refine<true>, dim3(blocks), dim3(64), 0, st, A);
-  else hipLaunchKernelGGL(k_pole_refine<false>, dim3(blocks), dim3(64), 0, st, A);
+  // 2 chains per candidate; 8 lanes each if all waves fit one per SIMD, else 4
+  const int64_t simds = 4 * (int64_t)std::max(n_cu, 1);
+  const bool deep = (16 * nc + 63) / 64 <= simds;
+  const unsigned blocks = (unsigned)(((deep ? 16 : 8) * nc + 63) / 64);
+  if (deep) {
+    if (general) hipLaunchKernelGGL((k_pole_refine<true, 3>), dim3(blocks), dim3(64), 0, st, A);
+    else hipLaunchKernelGGL((k_pole_refine<false, 3>), dim3(blocks), dim3(64), 0, st, A);
+  } else {
+    if (general) hipLaunchKernelGGL((k_pole_refine<true, 2>), dim3(blocks), dim3(64), 0, st, A);
+    else hipLaunchKernelGGL((k_pole_refine<false, 2>), dim3(blocks), dim3(64), 0, st, A);
+  }
   return hipGetLastError();
 }
 

@@ -135,8 +135,10 @@ hipError_t launch_cavity_response(const LayerDev *L, int nl, int s, double omega
                                   int64_t n, double *out, bool general, int *redo, hipStream_t st);
 // Golden-section location and half-level crossings of nc candidates, every
 // response evaluated in the kernel (response.hip).
+// n_cu: compute units of the device (the lanes per search are chosen so
+// that the waves fit one per SIMD when they can)
 hipError_t launch_pole_refine(const LayerDev *L, int nl, int s, double omega, int64_t nc, const double *cand,
-                              double *out, unsigned long long *evals, bool general, hipStream_t st);
+                              double *out, unsigned long long *evals, bool general, int n_cu, hipStream_t st);
 // Prominent maxima of a real-axis scan R[nphi][n] (response.hip); scratch Y
 // (nphi x n doubles) and ymax (nphi doubles), *count zeroed by the caller.
 hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
