@@ -150,3 +150,83 @@ def test_fixed_rule_resolves_the_plasmon_pole_end_to_end():
           f"{rel(G, Gh):.2e}; without the pole ({len(n0)} nodes) {rel(G0, Gb):.2e}", flush=True)
     assert rel(G, Gb) < 1e-5 and rel(G, Gh) < 1e-5
     assert rel(G0, Gb) > 10 * rel(G, Gb)
+
+def _candidates_serial(Y, ks, W, log_prom, log_floor_rel, log_floor_abs):
+    """The candidate rule of real_axis_poles step 2 (local maxima, valleys
+    within W samples, prominence; mod_KSpaceFeatures.jl:151-214), one sample
+    at a time, on the log scan Y (nphi, n)."""
+    out = {}
+    nphi, n = Y.shape
+    for m in range(nphi):
+        r = Y[m]
+        ymin = max(np.nanmax(r) + log_floor_rel, log_floor_abs)
+        for i in range(1, n - 1):
+            y = r[i]
+            if not (y > r[i - 1] and y >= r[i + 1] and y >= ymin):
+                continue
+            mins = []
+            for step, lim in ((-1, max(i - W, 0)), (1, min(i + W, n - 1))):
+                vmin, pmin, j = y, i, i + step
+                while (j >= lim) if step < 0 else (j <= lim):
+                    if r[j] > y:
+                        break
+                    if r[j] < vmin:
+                        vmin, pmin = r[j], j
+                    j += step
+                mins.append((vmin, pmin))
+            base = max(mins[0][0], mins[1][0])
+            if y - base >= log_prom:
+                out[m * n + i] = (base, y - base, ks[mins[0][1]], ks[mins[1][1]], y)
+    return out
+
+
+def test_pole_candidates_against_a_serial_walk():
+    """ytf_pole_candidates (one wave per 64 samples, 64-sample walk steps) on a
+    synthetic scan with sharp and broad peaks, plateaus of equal samples (ties
+    in the valleys), zeros (log -inf) and NaNs, against the rule walked one
+    sample at a time: the same maxima, valleys and prominences, bit for bit."""
+    import ctypes as ct
+    from yellowthinfilms_amd import _lib
+    print("case: pole candidates vs serial walk", flush=True)
+    rng = np.random.default_rng(3)
+    nphi, n, W = 3, 5000, 300
+    k = np.arange(n, dtype=np.float64)
+    R = np.empty((nphi, n))
+    for m in range(nphi):
+        base = 1.0 + 0.3 * np.sin(k / (40.0 + 17 * m)) + 0.05 * rng.random(n)
+        for c in rng.uniform(0, n, 40):
+            w = rng.uniform(0.5, 30.0)
+            base += rng.uniform(0.2, 20.0) / (1.0 + ((k - c) / w) ** 2)
+        R[m] = base
+    R[0, 1000:1040] = 0.9          # a plateau of equal samples
+    R[1, 2000:2010] = 0.0          # log -inf
+    R[2, 3000] = np.nan
+    R[2, 3500:3503] = R[2, 3499]   # ties next to a sample
+    ks = 1.0 + 1e-3 * k
+    dev = torch.device("cuda", 0)
+    Rd = torch.as_tensor(R, device=dev).contiguous()
+    ksd = torch.as_tensor(ks, device=dev)
+    cap = nphi * n
+    key = torch.empty(cap, dtype=torch.int64, device=dev)
+    val = torch.empty((cap, 5), dtype=torch.float64, device=dev)
+    cnt = ct.c_int64()
+    ctx = _lib.Context.get(0)
+    ctx.bind_stream()
+    lp, lfr, lfa = math.log(1.05), math.log(1e-6), math.log(1e-10)
+    _lib.check(_lib.lib().ytf_pole_candidates(ctx.handle, nphi, n, ct.c_void_p(Rd.data_ptr()),
+                                              ct.c_void_p(ksd.data_ptr()), W, lp, lfr, lfa, cap,
+                                              ct.c_void_p(key.data_ptr()), ct.c_void_p(val.data_ptr()),
+                                              ct.byref(cnt)), ctx.handle)
+    nc = int(cnt.value)
+    got = {int(kk): tuple(v) for kk, v in zip(key[:nc].cpu().numpy(), val[:nc].cpu().numpy())}
+    # the kernel's scan is log(R + 1e-300); the host log may differ from the
+    # device's in the last bit, so levels are compared to 1e-14 and the
+    # chosen samples (maxima, valley positions) exactly
+    with np.errstate(invalid="ignore"):
+        ref = _candidates_serial(np.log(R + 1e-300), ks, W, lp, lfr, lfa)
+    print(f"  {nc} candidates (serial {len(ref)})", flush=True)
+    assert nc > 50 and set(got) == set(ref)
+    for kk in ref:
+        g, r = got[kk], ref[kk]
+        assert g[2] == r[2] and g[3] == r[3], (kk, g, r)
+        assert all(abs(a - b) <= 1e-14 * max(1.0, abs(b)) for a, b in zip((g[0], g[1], g[4]), (r[0], r[1], r[4])))

@@ -208,81 +208,95 @@ __global__ void k_pole_rowmax(int64_t n, const double *__restrict__ Y, double *_
   if (threadIdx.x == 0) rmax[blockIdx.x] = part[0];
 }
 
-// One thread per (m, i), 1 <= i <= n - 2: y[i] a local maximum (> its left,
-// >= its right neighbour, >= max(log max_m R + log_floor_rel, log_floor_abs));
-// the valleys: left, the samples after the last one above y[i] within W
-// steps (from i - W if none), and its lowest (the last lowest); right, the
-// samples before the first one above y[i] within W steps, and its lowest
-// (the first lowest).  base = the higher valley, prom = y[i] - base; kept
-// when prom >= log_prom: key m n + i and (base, prom, k of the left valley,
-// k of the right valley, y[i]) are appended through *count (at most max_out).
-__global__ void k_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
-                                  double log_floor_abs, const double *__restrict__ Y, const double *__restrict__ ymax,
-                                  const double *__restrict__ ks, int max_out, int *__restrict__ count,
-                                  int64_t *__restrict__ key, double *__restrict__ val) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= (int64_t)nphi * n) return;
-  const int m = (int)(e / n);
-  const int64_t i = e - (int64_t)m * n;
-  if (i < 1 || i > n - 2) return;
+// y[i] (1 <= i <= n - 2) is a local maximum when it is > its left and >= its
+// right neighbour and >= max(log max_m R + log_floor_rel, log_floor_abs).  Its
+// valleys: left, the samples after the last one above y[i] within W steps
+// (from i - W if none), and its lowest (the last lowest); right, the samples
+// before the first one above y[i] within W steps, and its lowest (the first
+// lowest).  base = the higher valley, prom = y[i] - base; kept when prom >=
+// log_prom: key m n + i and (base, prom, k of the left valley, k of the right
+// valley, y[i]) are appended through *count (at most max_out).
+//
+// One wave per 64 samples of a direction finds the maxima among them, then
+// walks each maximum's two sides together, 64 samples per step: a ballot
+// finds the first sample above y[i] in the step, and a shuffle reduction
+// the lowest sample before it (ties to the one met first walking out, as a
+// one-sample-at-a-time walk keeps them).  Walks span up to W = ln 4 / scan
+// step samples (~1,400), which one lane per maximum took serially.
+__global__ __launch_bounds__(256) void k_pole_candidates(int nphi, int64_t n, int W, double log_prom,
+                                                         double log_floor_rel, double log_floor_abs,
+                                                         const double *__restrict__ Y,
+                                                         const double *__restrict__ ymax,
+                                                         const double *__restrict__ ks, int max_out,
+                                                         int *__restrict__ count, int64_t *__restrict__ key,
+                                                         double *__restrict__ val) {
+  const int64_t gw = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t segs = (n + 63) / 64;
+  const int m = (int)(gw / segs);
+  if (m >= nphi) return;  // whole waves
+  const int64_t i0 = (gw % segs) * 64, i = i0 + lane;
   const double *r = Y + (int64_t)m * n;
-  const double y = r[i];
   const double ymin = fmax(ymax[m] + log_floor_rel, log_floor_abs);
-  if (!(y > r[i - 1] && y >= r[i + 1] && y >= ymin)) return;
-  // Each side in one pass, 16 samples loaded at a time: walking out from i,
-  // the running lowest (strictly lower only: the left one ends as the last
-  // lowest, the right one as the first) until the first sample above y[i].
-  constexpr int CH = 16;
-  const int64_t lo = i - W > 0 ? i - W : 0;
-  double lmin = y;
-  int64_t lpos = i;
-  bool stop = false;
-  for (int64_t j0 = i - 1; j0 >= lo && !stop; j0 -= CH) {
-    double v[CH];
+  double y = 0.0;
+  bool is_max = false;
+  if (i >= 1 && i <= n - 2) {
+    y = r[i];
+    is_max = y > r[i - 1] && y >= r[i + 1] && y >= ymin;
+  }
+  unsigned long long mask = __ballot(is_max);
+  while (mask) {
+    const int bl = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const int64_t im = i0 + bl;
+    const double ym = __shfl(y, bl);
+    double best[2];
+    int64_t bpos[2];
 #pragma unroll
-    for (int u = 0; u < CH; ++u) v[u] = j0 - u >= lo ? r[j0 - u] : 0.0;
+    for (int side = 0; side < 2; ++side) {
+      // side 0 walks left (j = start - lane), side 1 right (j = start + lane)
+      const int64_t lim = side == 0 ? (im - W > 0 ? im - W : 0) : (im + W < n - 1 ? im + W : n - 1);
+      double vmin = ym;
+      int64_t pmin = im;
+      for (int64_t j0 = side == 0 ? im - 1 : im + 1; side == 0 ? j0 >= lim : j0 <= lim; j0 += side == 0 ? -64 : 64) {
+        const int64_t j = side == 0 ? j0 - lane : j0 + lane;
+        const bool in = side == 0 ? j >= lim : j <= lim;
+        const double v = in ? r[j] : 0.0;
+        const unsigned long long above = __ballot(in && v > ym);
+        const int first = above ? __ffsll((long long)above) - 1 : 64;
+        double bv = (in && lane < first && !isnan(v)) ? v : HUGE_VAL;  // a NaN never becomes the lowest
+        int64_t bj = j;
 #pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      if (!stop && j0 - u >= lo) {
-        const double lv = v[u];
-        if (lv > y) stop = true;
-        else if (lv < lmin) {
-          lmin = lv;
-          lpos = j0 - u;
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off);
+          const int64_t oj = __shfl_xor(bj, off);
+          // the lower value; on a tie the sample met first (nearer im)
+          const bool take = ov < bv || (ov == bv && (side == 0 ? oj > bj : oj < bj));
+          bv = take ? ov : bv;
+          bj = take ? oj : bj;
         }
+        if (bv < vmin) {  // strictly lower: earlier steps keep their ties
+          vmin = bv;
+          pmin = bj;
+        }
+        if (above) break;
+      }
+      best[side] = vmin;
+      bpos[side] = pmin;
+    }
+    const double base = fmax(best[0], best[1]), prom = ym - base;
+    if (lane == 0 && prom >= log_prom) {
+      const int o = atomicAdd(count, 1);
+      if (o < max_out) {
+        key[o] = (int64_t)m * n + im;
+        val[5 * o] = base;
+        val[5 * o + 1] = prom;
+        val[5 * o + 2] = ks[bpos[0]];
+        val[5 * o + 3] = ks[bpos[1]];
+        val[5 * o + 4] = ym;
       }
     }
   }
-  const int64_t hi = i + W < n - 1 ? i + W : n - 1;
-  double rmin = y;
-  int64_t rpos = i;
-  stop = false;
-  for (int64_t j0 = i + 1; j0 <= hi && !stop; j0 += CH) {
-    double v[CH];
-#pragma unroll
-    for (int u = 0; u < CH; ++u) v[u] = j0 + u <= hi ? r[j0 + u] : 0.0;
-#pragma unroll
-    for (int u = 0; u < CH; ++u) {
-      if (!stop && j0 + u <= hi) {
-        const double lv = v[u];
-        if (lv > y) stop = true;
-        else if (lv < rmin) {
-          rmin = lv;
-          rpos = j0 + u;
-        }
-      }
-    }
-  }
-  const double base = fmax(lmin, rmin), prom = y - base;
-  if (prom < log_prom) return;
-  const int o = atomicAdd(count, 1);
-  if (o >= max_out) return;
-  key[o] = e;
-  val[5 * o] = base;
-  val[5 * o + 1] = prom;
-  val[5 * o + 2] = ks[lpos];
-  val[5 * o + 3] = ks[rpos];
-  val[5 * o + 4] = y;
 }
 
 hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, double log_floor_rel,
@@ -296,8 +310,10 @@ hipError_t launch_pole_candidates(int nphi, int64_t n, int W, double log_prom, d
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_pole_rowmax, dim3(nphi), dim3(256), 0, s, n, (const double *)Y, ymax);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pole_candidates, dim3(blocks), dim3(256), 0, s, nphi, n, W, log_prom, log_floor_rel,
-                     log_floor_abs, (const double *)Y, (const double *)ymax, ks, max_out, count, key, val);
+  const int64_t waves = (int64_t)nphi * ((n + 63) / 64);  // one wave per 64 samples of a direction
+  hipLaunchKernelGGL(k_pole_candidates, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, nphi, n, W, log_prom,
+                     log_floor_rel, log_floor_abs, (const double *)Y, (const double *)ymax, ks, max_out, count, key,
+                     val);
   return hipGetLastError();
 }
 
