@@ -26,21 +26,36 @@ def _torch():
     return torch
 
 
+_LAYER_DTYPE = None
+
+
+def _layer_dtype():
+    """numpy record type with ytf_layer's field offsets (from the ctypes layout)."""
+    global _LAYER_DTYPE
+    if _LAYER_DTYPE is None:
+        L = _lib.YtfLayer
+        _LAYER_DTYPE = np.dtype({"names": ["eps", "mu", "d", "eps_kind", "mu_kind"],
+                                 "formats": [(np.float64, 18), (np.float64, 18), np.float64, np.int32, np.int32],
+                                 "offsets": [L.eps.offset, L.mu.offset, L.d.offset, L.eps_kind.offset,
+                                             L.mu_kind.offset],
+                                 "itemsize": ct.sizeof(L)})
+    return _LAYER_DTYPE
+
+
 def layer_structs(stack):
-    """ytf_layer array for a stack (list of FlatOpticalLayer / FlatLayer)."""
+    """ytf_layer array for a stack (list of FlatOpticalLayer / FlatLayer):
+    eps and mu column-major as interleaved (re, im) pairs, built as one numpy
+    record array and copied into the ctypes array."""
     eps, mu, d, ek, mk = stack_arrays(stack)
     n = len(d)
-    arr = (_lib.YtfLayer * n)()
-    for l in range(n):
-        e = np.ascontiguousarray(eps[l].T).reshape(-1)  # column-major
-        m = np.ascontiguousarray(mu[l].T).reshape(-1)
-        for i in range(9):
-            arr[l].eps[2 * i], arr[l].eps[2 * i + 1] = e[i].real, e[i].imag
-            arr[l].mu[2 * i], arr[l].mu[2 * i + 1] = m[i].real, m[i].imag
-        arr[l].d = d[l]
-        arr[l].eps_kind = int(ek[l])
-        arr[l].mu_kind = int(mk[l])
-    return arr
+    rec = np.zeros(n, dtype=_layer_dtype())
+    # column-major 3x3 per layer, then (re, im) interleaved: complex128 viewed as float64 pairs
+    rec["eps"] = np.ascontiguousarray(np.transpose(eps, (0, 2, 1)).reshape(n, 9), np.complex128).view(np.float64)
+    rec["mu"] = np.ascontiguousarray(np.transpose(mu, (0, 2, 1)).reshape(n, 9), np.complex128).view(np.float64)
+    rec["d"] = d
+    rec["eps_kind"] = ek
+    rec["mu_kind"] = mk
+    return (_lib.YtfLayer * n).from_buffer_copy(rec.tobytes())
 
 
 class NodeCache:
