@@ -91,7 +91,8 @@ inline int fail(ytf_ctx *c, int code, const char *fmt, ...) {
 
 // Point c->small at a device copy of hb: a slot that already holds these
 // bytes, else an empty slot, else the least recently used one (after the
-// stream has drained, since queued kernels may still read it).
+// device has drained, since queued kernels, on this stream or on one the
+// context was bound to before, may still read it).
 inline int set_small(ytf_ctx *c, const std::vector<char> &hb) {
   const uint64_t tick = ++c->small_tick;
   for (auto &s : c->small_slots)
@@ -110,10 +111,10 @@ inline int set_small(ytf_ctx *c, const std::vector<char> &hb) {
     v = &c->small_slots[0];
     for (auto &s : c->small_slots)
       if (s.used < v->used) v = &s;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipDeviceSynchronize());
   }
   if (v->cap < hb.size()) {
-    if (v->dev) HIPCHK(c, hipFree(v->dev));
+    if (v->dev) HIPCHK(c, hipFree(v->dev));  // hipFree waits for the device
     v->dev = nullptr;
     v->cap = 0;
     const size_t nb = hb.size() < 4096 ? 4096 : hb.size();
