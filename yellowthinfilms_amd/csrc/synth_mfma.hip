@@ -143,7 +143,12 @@ __device__ inline void bessel01_table(double ax, double rx, double &j0, double &
 // difference cancels.
 __device__ inline void bessel012m(double x, double &b0, double &b1, double &b2) {
   const double ax = fabs(x);
-  const double rx = 1.0 / ax;
+  // 1/ax by the hardware reciprocal and two Newton steps (within an ulp or
+  // so; ax = 0 gives NaN, unused: below 1/2 J2 takes its series and J0, J1
+  // theirs)
+  double rx = __builtin_amdgcn_rcp(ax);
+  rx = fma(rx, fma(-ax, rx, 1.0), rx);
+  rx = fma(rx, fma(-ax, rx, 1.0), rx);
   double t;
   bessel01_table(ax, rx, b0, t);
   b1 = x < 0.0 ? -t : t;
