@@ -446,7 +446,8 @@ __device__ __forceinline__ void eigen_closed(const LayerDev &L, double w, double
   const cd wc = mk(w);
   const double kr = hypot(kx, ky);
   // cos and sin of phi = atan(ky, kx), taken as kx / kr and ky / kr
-  const double cp = kr == 0.0 ? 1.0 : kx / kr, sp = kr == 0.0 ? 0.0 : ky / kr;
+  const double ikr = kr > 1e-300 ? rcp_nr(kr) : 1.0 / kr;
+  const double cp = kr == 0.0 ? 1.0 : kx * ikr, sp = kr == 0.0 ? 0.0 : ky * ikr;
   const cd kr2 = mk(kr * kr);
   const cd kzo = csqrt(wc * wc * et * mr - kr2);
   // isotropic eps: kze equals kzo, taken as the same number
@@ -984,7 +985,8 @@ __device__ __forceinline__ bool ill_conditioned(const cd *A) {
 __device__ __forceinline__ bool source_projection_closed(const LayerDev &L, double w, double kx, double ky,
                                                          const cd *Q, cd *sif, cd *sib) {
   const double kr = hypot(kx, ky);
-  const double c = kr == 0.0 ? 1.0 : kx / kr, sn = kr == 0.0 ? 0.0 : ky / kr;
+  const double ikr = kr > 1e-300 ? rcp_nr(kr) : 1.0 / kr;
+  const double c = kr == 0.0 ? 1.0 : kx * ikr, sn = kr == 0.0 ? 0.0 : ky * ikr;
   cd a, b;
   closed_ab(L, w, Q, a, b);
   const cd ia = cinv(a), ib = cinv(b);

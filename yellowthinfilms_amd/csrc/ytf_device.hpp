@@ -52,6 +52,13 @@ __device__ __forceinline__ cd cinv(cd b) {
   return cd{br * sc, -bi * sc};
 }
 __device__ __forceinline__ cd operator/(cd a, cd b) { return a * cinv(b); }
+// 1/x for a finite nonzero x whose reciprocal is a normal double: the
+// hardware reciprocal and two Newton steps (within an ulp or so of 1/x).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
 __device__ __forceinline__ cd cexp(cd z) {
   double e = exp(z.r);
   double s, c;
@@ -64,8 +71,9 @@ __device__ __forceinline__ cd expi(cd q, double d) { return cexp(cd{-q.i * d, q.
 __device__ __forceinline__ cd csqrt(cd z) {
   if (z.r == 0.0 && z.i == 0.0) return cd{0.0, z.i};
   double t = sqrt((fabs(z.r) + hypot(z.r, z.i)) * 0.5);
-  if (z.r >= 0.0) return cd{t, z.i / (2.0 * t)};
-  return cd{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
+  const double h = t < 1e300 ? 0.5 * rcp_nr(t) : 0.5 / t;  // t > 0 here
+  if (z.r >= 0.0) return cd{t, z.i * h};
+  return cd{fabs(z.i) * h, copysign(t, z.i)};
 }
 __device__ __forceinline__ cd sel(bool c, cd a, cd b) { return cd{c ? a.r : b.r, c ? a.i : b.i}; }
 // |x| < 1e-250 -> 0 (denormal floor of the stack walk)

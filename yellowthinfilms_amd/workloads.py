@@ -368,10 +368,10 @@ BATCHED_GYRO_512_SPECIAL_PER_NODE = 278.0
 BATCHED_GYRO_512_NODE_HBM_BYTES = 43.78e6
 # FP64 flops the node kernel executes per node on batched_gyro_512 (rocprofv3
 # counters of one launch: 64 x (SQ_INSTS_VALU_ADD_F64 + SQ_INSTS_VALU_MUL_F64 +
-# 2 SQ_INSTS_VALU_FMA_F64) / 262,144 nodes = 64 x (3.588e6 + 8.075e6 + 2 x 11.547e6)
+# 2 SQ_INSTS_VALU_FMA_F64) / 262,144 nodes = 64 x (3.555e6 + 8.151e6 + 2 x 11.362e6)
 # / 262,144).  The kernel solves each node with less arithmetic than the reference
 # formulation above (block-form eigen, 2x2 recursions, closed-form interfaces).
-BATCHED_GYRO_512_EXECUTED_FLOPS_PER_NODE = 8486.0
+BATCHED_GYRO_512_EXECUTED_FLOPS_PER_NODE = 8406.0
 
 
 def uniform_c128_source(nx, ny, seed):
